@@ -1,0 +1,183 @@
+"""Host mirror of the reference's K-lane batched PHAST API over the C ABI.
+
+`BatchedPhastEngine` keeps the method names, argument meaning and error
+behaviour of route/src/matrix/batched_phast.rs (sources and targets are CCH
+ranks, matrices are row-major u32, unreachable = u32::MAX), with the work done
+by the HIP engine in libbfgpu.so.  `run_phast_bounded_fast_seeded` mirrors
+route/src/server/isochrone_handler.rs:338-491.
+"""
+import ctypes as C
+import dataclasses
+
+import numpy as np
+
+from . import _native
+from .graph import INF, CchGraph
+
+K_LANES = 8  # the reference's lane count (batched_phast.rs:33); the GPU runs wider groups
+
+
+class BfgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bfgpu error {code}: {msg}")
+        self.code = code
+
+
+def _check(rc):
+    if rc != 0:
+        raise BfgError(rc, _native.bfgpu().bfg_last_error().decode())
+
+
+@dataclasses.dataclass
+class BatchedPhastStats:
+    """batched_phast.rs:67-80, with device phase times."""
+    n_sources: int = 0
+    upward_time_ms: float = 0.0
+    downward_time_ms: float = 0.0
+    output_time_ms: float = 0.0
+    total_time_ms: float = 0.0
+    batches: int = 0
+    sweep_launches: int = 0
+
+
+@dataclasses.dataclass
+class BatchedPhastResult:
+    """batched_phast.rs:56-65: dist[lane][node]."""
+    dist: np.ndarray
+    n_lanes: int
+    stats: BatchedPhastStats
+
+
+def _u32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+class BatchedPhastEngine:
+    """BatchedPhastEngine::new(topo, weights) (batched_phast.rs:89) on one GPU."""
+
+    def __init__(self, graph: CchGraph, device: int = 0):
+        self.graph = graph
+        self._lib = _native.bfgpu()
+        h = C.c_void_p()
+        d = graph.desc()
+        _check(self._lib.bfg_ctx_create(C.byref(d), device, C.byref(h)))
+        self._h = h
+        self.n_nodes = graph.n
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.bfg_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- introspection ------------------------------------------------------
+    def info(self):
+        i = _native.CtxInfo()
+        _check(self._lib.bfg_ctx_get_info(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in i._fields_}
+
+    def last_stats(self, n_sources=0):
+        t = _native.Timings()
+        _check(self._lib.bfg_ctx_last_timings(self._h, C.byref(t)))
+        return BatchedPhastStats(n_sources=n_sources, upward_time_ms=t.up_ms, downward_time_ms=t.sweep_ms,
+                                 output_time_ms=t.output_ms, total_time_ms=t.total_ms, batches=t.batches,
+                                 sweep_launches=t.sweep_launches)
+
+    def configure(self, lanes=0, lane_capacity=0):
+        _check(self._lib.bfg_ctx_configure(self._h, lanes, lane_capacity))
+
+    def rank_to_filtered(self):
+        return self.graph.rank_to_filtered
+
+    # -- queries --------------------------------------------------------------
+    def query_batch(self, sources):
+        """batched_phast.rs:131-260 (any number of sources): dist[lane][node]."""
+        return self.query_batch_bounded(sources, INF)
+
+    query_batch_soa = query_batch      # batched_phast.rs:470-609: same result
+    query_batch_blocked = query_batch  # batched_phast.rs:455-459
+
+    def query_batch_bounded(self, sources, threshold):
+        """batched_phast.rs:271-445: exact within threshold, INF beyond."""
+        src = _u32(sources)
+        out = np.empty((src.size, self.n_nodes), dtype=np.uint32)
+        _check(self._lib.bfg_query_batch(self._h, _native.ptr(src), src.size, int(threshold) & INF,
+                                         _native.ptr(out)))
+        return BatchedPhastResult(dist=out, n_lanes=src.size, stats=self.last_stats(src.size))
+
+    query_batch_block_gated = query_batch_bounded  # batched_phast.rs:946-1218
+
+    def compute_matrix_flat(self, sources, targets):
+        """batched_phast.rs:667-706 -> (row-major S*T u32, stats)."""
+        return self.compute_matrix_block_gated(sources, targets, INF)
+
+    compute_matrix_flat_soa = compute_matrix_flat      # :756-797
+    compute_matrix_flat_blocked = compute_matrix_flat  # :710-751
+
+    def compute_matrix_block_gated(self, sources, targets, threshold):
+        """batched_phast.rs:1223-1265: cells > threshold are INF."""
+        src, tgt = _u32(sources), _u32(targets)
+        out = np.empty(src.size * tgt.size, dtype=np.uint32)
+        _check(self._lib.bfg_matrix_bounded(self._h, _native.ptr(src), src.size, _native.ptr(tgt), tgt.size,
+                                            int(threshold) & INF, _native.ptr(out)))
+        return out, self.last_stats(src.size)
+
+    def compute_matrix(self, sources, targets, callback):
+        """batched_phast.rs:619-661: callback(first_source_idx, rows, chunk) per 8-source chunk."""
+        src = _u32(sources)
+        tgt = _u32(targets) if len(targets) else np.arange(self.n_nodes, dtype=np.uint32)
+        mat, stats = self.compute_matrix_flat(src, tgt)
+        mat = mat.reshape(src.size, tgt.size)
+        for b in range(0, src.size, K_LANES):
+            callback(b, list(mat[b:b + K_LANES]), src[b:b + K_LANES])
+        return stats
+
+    def isochrones(self, origins, limits, want_times=True):
+        """Per-lane-limit isochrone sweeps: (reach bits [L, ceil(n/64)] u64, times [L, n] or None)."""
+        org, lim = _u32(origins), _u32(limits)
+        if org.size != lim.size:
+            raise ValueError("origins and limits differ in length")
+        words = (self.n_nodes + 63) // 64
+        bits = np.empty((org.size, words), dtype=np.uint64)
+        times = np.empty((org.size, self.n_nodes), dtype=np.uint32) if want_times else None
+        _check(self._lib.bfg_isochrone(self._h, _native.ptr(org), _native.ptr(lim), org.size,
+                                       _native.ptr(bits, _native.u64p),
+                                       _native.ptr(times) if want_times else None))
+        return bits, times
+
+    def phast_seeded(self, seed_lists, thresholds):
+        """Many run_phast_bounded_fast_seeded calls in one launch: dense rows [L, n]."""
+        L = len(seed_lists)
+        off = np.zeros(L + 1, dtype=np.uint32)
+        ranks, costs = [], []
+        for i, seeds in enumerate(seed_lists):
+            for r, c in seeds:
+                ranks.append(r)
+                costs.append(c)
+            off[i + 1] = len(ranks)
+        ranks, costs, thr = _u32(ranks), _u32(costs), _u32(thresholds)
+        out = np.empty((L, self.n_nodes), dtype=np.uint32)
+        _check(self._lib.bfg_phast_seeded(self._h, _native.ptr(off), _native.ptr(ranks), _native.ptr(costs), L,
+                                          _native.ptr(thr), _native.ptr(out)))
+        return out
+
+    def run_phast_bounded_fast_seeded(self, seeds, threshold):
+        """isochrone_handler.rs:338-344: Vec<(rank, dist)> ascending rank, dist <= threshold."""
+        row = self.phast_seeded([list(seeds)], [threshold])[0]
+        idx = np.nonzero(row != INF)[0]
+        return [(int(r), int(row[r])) for r in idx]
+
+    def run_phast_bounded_fast(self, origin_rank, threshold):
+        """isochrone_handler.rs:315-329."""
+        return self.run_phast_bounded_fast_seeded([(origin_rank, 0)], threshold)

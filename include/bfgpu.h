@@ -1,0 +1,138 @@
+/*
+ * bfgpu.h — C ABI of the MI355X K-lane batched PHAST engine.
+ *
+ * A drop-in for butterfly-route's K-lane batched PHAST over the customized
+ * edge-based CCH (route/src/matrix/batched_phast.rs) and for the seeded,
+ * bounded single-source PHAST behind isochrones and PHAST matrix rows
+ * (route/src/server/isochrone_handler.rs:338-491).  The reference has no FFI
+ * for this path; each entry point below names the Rust signature it replaces.
+ *
+ * Conventions (mirroring dl/include/butterfly.h:44-54,70-76 of the reference):
+ *   - every call returns a bfg_result; 0 is success; bfg_last_error() gives the
+ *     message of the last failure on the calling thread;
+ *   - every buffer is caller-owned; inputs are copied, outputs are written in
+ *     place; nothing returned needs freeing except the context;
+ *   - nodes are CCH ranks (rank-aligned CCH: node id == rank), distances are
+ *     u32 seconds, 0xFFFFFFFF (BFG_INF) means unreachable / beyond the limit;
+ *   - calls on one context are serialised internally; distinct contexts may be
+ *     used concurrently from different threads.
+ * Results are byte-identical to the reference CPU PHAST on the same CCH:
+ * exact shortest up-down distances with saturating u32 adds; for bounded
+ * calls every value <= the limit is exact and every other value is BFG_INF.
+ */
+#ifndef BFGPU_H
+#define BFGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BFG_INF 0xFFFFFFFFu
+
+typedef enum bfg_result {
+    BFG_SUCCESS = 0,
+    BFG_INVALID_PARAMETER = 1, /* bad pointer, size or rank >= n_nodes        */
+    BFG_OUT_OF_MEMORY = 2,     /* device allocation failed                     */
+    BFG_DEVICE_ERROR = 3,      /* HIP runtime or kernel failure                */
+    BFG_NOT_SUPPORTED = 4,     /* valid request outside this build's limits    */
+    BFG_INTERNAL_ERROR = 5
+} bfg_result;
+
+/* The customized CCH as the reference holds it: CchTopo up/down CSR
+ * (route/src/formats/cch_topo.rs:250-286) with CchWeights decoded to u32
+ * (route/src/formats/cch_weights.rs:171-187, sentinel -> BFG_INF).
+ * UP arcs at tail r have targets > r, DOWN arcs at tail r have targets < r.
+ * INF-weight arcs are dropped on upload (bucket_ch.rs:191-424). */
+typedef struct bfg_graph_desc {
+    uint32_t n_nodes;
+    const uint64_t *up_offsets;   /* n_nodes + 1 */
+    const uint32_t *up_targets;
+    const uint32_t *up_weights;
+    const uint64_t *down_offsets; /* n_nodes + 1 */
+    const uint32_t *down_targets;
+    const uint32_t *down_weights;
+} bfg_graph_desc;
+
+typedef struct bfg_ctx bfg_ctx;
+
+typedef struct bfg_ctx_info {
+    uint32_t n_nodes;
+    uint64_t m_up, m_down;          /* finite arcs after INF filtering           */
+    uint32_t etree_depth;           /* deepest node of the elimination tree      */
+    uint32_t n_levels, n_tasks, n_steps;
+    uint32_t lanes;                 /* K: source lanes interleaved per group     */
+    uint32_t lane_capacity;         /* sources per device batch                  */
+    uint64_t device_bytes;          /* resident CCH + arenas                     */
+} bfg_ctx_info;
+
+/* Per-phase device time of the last call (HIP events on the context stream). */
+typedef struct bfg_timings {
+    float up_ms, sweep_ms, output_ms, total_ms;
+    uint32_t batches, sweep_launches;
+    uint64_t lanes_run;
+} bfg_timings;
+
+/* BatchedPhastEngine::new (batched_phast.rs:89): copies the CCH to `device`,
+ * derives the device layout and keeps it resident until destroy. */
+int bfg_ctx_create(const bfg_graph_desc *graph, int device, bfg_ctx **out);
+void bfg_ctx_destroy(bfg_ctx *ctx);
+const char *bfg_last_error(void);
+int bfg_ctx_get_info(const bfg_ctx *ctx, bfg_ctx_info *info);
+int bfg_ctx_last_timings(const bfg_ctx *ctx, bfg_timings *t);
+/* Lane width K in {16, 32, 64} and the batch capacity (0 = keep/auto).
+ * Reallocates the arenas; used by the autotuner. */
+int bfg_ctx_configure(bfg_ctx *ctx, uint32_t lanes, uint32_t lane_capacity);
+
+/* Host-only check of a CCH before upload (no device is touched): derives the
+ * device layout, verifies that every finite arc joins an elimination-tree
+ * ancestor/descendant pair and that the sweep schedule respects every arc,
+ * and reports the layout figures (the role of the reference's validate-cch /
+ * validate-phast CLI checks, route/src/cli.rs:1040-1090). */
+int bfg_validate_graph(const bfg_graph_desc *graph, bfg_ctx_info *info);
+
+/* BatchedPhastEngine::compute_matrix_flat (batched_phast.rs:667-706):
+ * out[s * n_tgt + t] = d(sources[s], targets[t]), row-major, host memory. */
+int bfg_matrix(bfg_ctx *ctx, const uint32_t *sources, uint32_t n_src,
+               const uint32_t *targets, uint32_t n_tgt, uint32_t *out);
+
+/* BatchedPhastEngine::compute_matrix_block_gated (batched_phast.rs:1223-1265):
+ * as bfg_matrix, cells > threshold are BFG_INF. */
+int bfg_matrix_bounded(bfg_ctx *ctx, const uint32_t *sources, uint32_t n_src,
+                       const uint32_t *targets, uint32_t n_tgt, uint32_t threshold,
+                       uint32_t *out);
+
+/* Device-resident variant of the two above (no counterpart in the reference;
+ * it is the same computation with HIP device pointers for callers that keep
+ * queries and results on the GPU).  threshold = BFG_INF for unbounded. */
+int bfg_matrix_device(bfg_ctx *ctx, const uint32_t *d_sources, uint32_t n_src,
+                      const uint32_t *d_targets, uint32_t n_tgt, uint32_t threshold,
+                      uint32_t *d_out);
+
+/* BatchedPhastEngine::query_batch / query_batch_soa (batched_phast.rs:131-260,
+ * :470-609) for any number of sources: dist[i * n_nodes + v] = d(sources[i], v).
+ * With threshold != BFG_INF this is query_batch_bounded / query_batch_block_gated
+ * (:271-445, :946-1218): values > threshold are BFG_INF. */
+int bfg_query_batch(bfg_ctx *ctx, const uint32_t *sources, uint32_t n_src,
+                    uint32_t threshold, uint32_t *dist);
+
+/* Block-gated isochrone sweeps with one limit per lane (the K-lane caller of
+ * range/batched_isochrone.rs:121-226 generalised to uneven limits).
+ * reach_bits[i * ceil(n/64) + v/64] bit v%64 set iff d(origins[i], v) <= limits[i];
+ * times (nullable) [i * n_nodes + v] = that distance, BFG_INF otherwise. */
+int bfg_isochrone(bfg_ctx *ctx, const uint32_t *origins, const uint32_t *limits,
+                  uint32_t n_lanes, uint64_t *reach_bits, uint32_t *times);
+
+/* run_phast_bounded_fast_seeded (isochrone_handler.rs:338-491) for n_lanes
+ * independent queries: lane i has seeds (seed_ranks[k], seed_costs[k]) for
+ * k in [seed_offsets[i], seed_offsets[i+1]) (at most 64 per lane) and limit
+ * thresholds[i].  dist[i * n_nodes + v] = bounded distance, BFG_INF beyond. */
+int bfg_phast_seeded(bfg_ctx *ctx, const uint32_t *seed_offsets, const uint32_t *seed_ranks,
+                     const uint32_t *seed_costs, uint32_t n_lanes, const uint32_t *thresholds,
+                     uint32_t *dist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BFGPU_H */

@@ -1,0 +1,230 @@
+"""ctypes wrapper of liborc.so — the CPU restatement of the reference's PHAST.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, always as the checker or the timed CPU
+reference, never by the product path.  Each function names the reference code
+it restates (see phast_oracle.c for the file:line table).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+INF = 0xFFFFFFFF
+K_LANES = 8
+
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+
+
+class _Cch(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("up_off", u64p), ("up_tgt", u32p), ("up_w", u32p),
+                ("dn_off", u64p), ("dn_tgt", u32p), ("dn_w", u32p)]
+
+
+class _Flat(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("off", u64p), ("tgt", u32p), ("w", u32p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liborc.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run oracle/build.py (or __graft_entry__.build())")
+        _lib = C.CDLL(path)
+        cp = C.POINTER(_Cch)
+        fp = C.POINTER(_Flat)
+        sig = {
+            "orc_query_batch": [cp, u32p, C.c_uint32, u32p],
+            "orc_query_batch_soa": [cp, u32p, C.c_uint32, u32p],
+            "orc_query_batch_bounded": [cp, u32p, C.c_uint32, C.c_uint32, u32p],
+            "orc_query_batch_block_gated": [cp, u32p, C.c_uint32, C.c_uint32, u32p],
+            "orc_compute_matrix_flat": [cp, u32p, C.c_uint32, u32p, C.c_uint32, u32p],
+            "orc_compute_matrix_flat_soa": [cp, u32p, C.c_uint32, u32p, C.c_uint32, u32p],
+            "orc_compute_matrix_block_gated": [cp, u32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32, u32p],
+            "orc_compute_matrix_flat_mt": [cp, u32p, C.c_uint32, u32p, C.c_uint32, u32p, C.c_int],
+            "orc_phast_query": [cp, C.c_uint32, u32p],
+            "orc_phast_query_reverse": [cp, C.c_uint32, u32p],
+            "orc_build_up_flat": [cp, fp],
+            "orc_build_down_flat": [cp, fp],
+            "orc_build_down_reverse_flat": [cp, fp],
+            "orc_dijkstra": [C.c_uint32, u64p, u32p, u32p, C.c_uint32, u32p],
+        }
+        for k, v in sig.items():
+            f = getattr(_lib, k)
+            f.argtypes = v
+            f.restype = C.c_int
+        _lib.orc_flat_free.argtypes = [fp]
+        _lib.orc_flat_free.restype = None
+        _lib.orc_seeded_state_new.argtypes = [C.c_uint32]
+        _lib.orc_seeded_state_new.restype = C.c_void_p
+        _lib.orc_seeded_state_free.argtypes = [C.c_void_p]
+        _lib.orc_seeded_state_free.restype = None
+        _lib.orc_phast_seeded.argtypes = [C.c_void_p, fp, fp, u32p, u32p, C.c_uint32, C.c_uint32, u32p, u32p]
+        _lib.orc_phast_seeded.restype = C.c_uint64
+    return _lib
+
+
+def _p(a, t=u32p):
+    return a.ctypes.data_as(t)
+
+
+def _u32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+class Oracle:
+    """Holds one CCH (anything with n, up_off, up_tgt, up_w, dn_off, dn_tgt, dn_w)."""
+
+    def __init__(self, g):
+        self.g = g
+        self.n = int(g.n)
+        self._arrs = [np.ascontiguousarray(g.up_off, dtype=np.uint64), _u32(g.up_tgt), _u32(g.up_w),
+                      np.ascontiguousarray(g.dn_off, dtype=np.uint64), _u32(g.dn_tgt), _u32(g.dn_w)]
+        a = self._arrs
+        self._c = _Cch(self.n, _p(a[0], u64p), _p(a[1]), _p(a[2]), _p(a[3], u64p), _p(a[4]), _p(a[5]))
+        self.L = lib()
+
+    # batched_phast.rs:131-260 (k <= 8)
+    def query_batch(self, sources):
+        s = _u32(sources)
+        assert s.size <= K_LANES, "Too many sources for batch"
+        out = np.empty((s.size, self.n), dtype=np.uint32)
+        self.L.orc_query_batch(C.byref(self._c), _p(s), s.size, _p(out))
+        return out
+
+    # batched_phast.rs:800-917, returned as AoS
+    def query_batch_soa(self, sources):
+        s = _u32(sources)
+        assert s.size <= K_LANES
+        ds = np.empty(self.n * K_LANES, dtype=np.uint32)
+        self.L.orc_query_batch_soa(C.byref(self._c), _p(s), s.size, _p(ds))
+        return ds.reshape(self.n, K_LANES)[:, :s.size].T.copy()
+
+    def query_batch_bounded(self, sources, threshold):
+        s = _u32(sources)
+        assert s.size <= K_LANES
+        out = np.empty((s.size, self.n), dtype=np.uint32)
+        self.L.orc_query_batch_bounded(C.byref(self._c), _p(s), s.size, threshold, _p(out))
+        return out
+
+    def query_batch_block_gated(self, sources, threshold):
+        s = _u32(sources)
+        assert s.size <= K_LANES
+        out = np.empty((s.size, self.n), dtype=np.uint32)
+        self.L.orc_query_batch_block_gated(C.byref(self._c), _p(s), s.size, threshold, _p(out))
+        return out
+
+    def compute_matrix_flat(self, sources, targets):
+        s, t = _u32(sources), _u32(targets)
+        out = np.empty(s.size * t.size, dtype=np.uint32)
+        rc = self.L.orc_compute_matrix_flat(C.byref(self._c), _p(s), s.size, _p(t), t.size, _p(out))
+        assert rc == 0
+        return out
+
+    def compute_matrix_flat_soa(self, sources, targets):
+        s, t = _u32(sources), _u32(targets)
+        out = np.empty(s.size * t.size, dtype=np.uint32)
+        assert self.L.orc_compute_matrix_flat_soa(C.byref(self._c), _p(s), s.size, _p(t), t.size, _p(out)) == 0
+        return out
+
+    def compute_matrix_block_gated(self, sources, targets, threshold):
+        s, t = _u32(sources), _u32(targets)
+        out = np.empty(s.size * t.size, dtype=np.uint32)
+        assert self.L.orc_compute_matrix_block_gated(C.byref(self._c), _p(s), s.size, _p(t), t.size, threshold,
+                                                     _p(out)) == 0
+        return out
+
+    def compute_matrix_flat_mt(self, sources, targets, n_threads=0):
+        s, t = _u32(sources), _u32(targets)
+        out = np.empty(s.size * t.size, dtype=np.uint32)
+        assert self.L.orc_compute_matrix_flat_mt(C.byref(self._c), _p(s), s.size, _p(t), t.size, _p(out),
+                                                 n_threads) == 0
+        return out
+
+    def phast_query(self, origin):
+        out = np.empty(self.n, dtype=np.uint32)
+        self.L.orc_phast_query(C.byref(self._c), int(origin), _p(out))
+        return out
+
+    def phast_query_reverse(self, target):
+        out = np.empty(self.n, dtype=np.uint32)
+        self.L.orc_phast_query_reverse(C.byref(self._c), int(target), _p(out))
+        return out
+
+    def _flat(self, which):
+        f = _Flat()
+        getattr(self.L, which)(C.byref(self._c), C.byref(f))
+        m = int(f.off[self.n])
+        res = (np.ctypeslib.as_array(f.off, shape=(self.n + 1,)).copy(),
+               np.ctypeslib.as_array(f.tgt, shape=(max(m, 1),))[:m].copy(),
+               np.ctypeslib.as_array(f.w, shape=(max(m, 1),))[:m].copy())
+        self.L.orc_flat_free(C.byref(f))
+        return res
+
+    def up_flat(self):
+        return self._flat("orc_build_up_flat")
+
+    def down_flat(self):
+        return self._flat("orc_build_down_flat")
+
+    def down_reverse_flat(self):
+        return self._flat("orc_build_down_reverse_flat")
+
+    def seeded(self, seeds, threshold):
+        """run_phast_bounded_fast_seeded -> list of (rank, dist)."""
+        if not hasattr(self, "_st"):
+            self._st = self.L.orc_seeded_state_new(self.n)
+            uo, ut, uw = self.up_flat()
+            do, dt, dw = self.down_flat()
+            self._flats = [uo, ut, uw, do, dt, dw]
+            self._fu = _Flat(self.n, _p(uo, u64p), _p(ut), _p(uw))
+            self._fd = _Flat(self.n, _p(do, u64p), _p(dt), _p(dw))
+        r = _u32([s[0] for s in seeds])
+        c = _u32([s[1] for s in seeds])
+        orank = np.empty(self.n, dtype=np.uint32)
+        odist = np.empty(self.n, dtype=np.uint32)
+        cnt = self.L.orc_phast_seeded(self._st, C.byref(self._fu), C.byref(self._fd), _p(r), _p(c), r.size,
+                                      threshold, _p(orank), _p(odist))
+        return list(zip(orank[:cnt].tolist(), odist[:cnt].tolist()))
+
+    def __del__(self):
+        st = getattr(self, "_st", None)
+        if st:
+            self.L.orc_seeded_state_free(st)
+            self._st = None
+
+
+def dijkstra(n, off, head, w, src):
+    out = np.empty(int(n), dtype=np.uint32)
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    head, w = _u32(head), _u32(w)
+    lib().orc_dijkstra(int(n), _p(off, u64p), _p(head), _p(w), int(src), _p(out))
+    return out
+
+
+def union_graph(g):
+    """CSR of the CCH union graph (UP and DOWN arcs by tail), for brute force."""
+    n = int(g.n)
+    up_cnt = np.diff(np.asarray(g.up_off, dtype=np.int64))
+    dn_cnt = np.diff(np.asarray(g.dn_off, dtype=np.int64))
+    off = np.zeros(n + 1, dtype=np.uint64)
+    off[1:] = np.cumsum(up_cnt + dn_cnt)
+    head = np.empty(int(off[-1]), dtype=np.uint32)
+    w = np.empty(int(off[-1]), dtype=np.uint32)
+    uo = np.asarray(g.up_off, dtype=np.int64)
+    do = np.asarray(g.dn_off, dtype=np.int64)
+    for r in range(n):
+        a = int(off[r])
+        k = int(up_cnt[r])
+        head[a:a + k] = g.up_tgt[uo[r]:uo[r + 1]]
+        w[a:a + k] = g.up_w[uo[r]:uo[r + 1]]
+        k2 = int(dn_cnt[r])
+        head[a + k:a + k + k2] = g.dn_tgt[do[r]:do[r + 1]]
+        w[a + k:a + k + k2] = g.dn_w[do[r]:do[r + 1]]
+    return off, head, w

@@ -1,0 +1,175 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bit-exact everywhere (u32 integer work).  Sizes are chosen so the oracle
+finishes in seconds; large shapes use size-independent properties (broom
+closed form)."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import broom_expect, broom_graph, grid_graph, hand_graph
+from oracle.oracle import INF, Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def mask(d, thr):
+    d = np.array(d, dtype=np.uint32, copy=True)
+    d[d > thr] = INF
+    return d
+
+
+@pytest.fixture(scope="module")
+def engine_cls():
+    from bfgpu import BatchedPhastEngine
+    return BatchedPhastEngine
+
+
+@pytest.mark.parametrize("name", ["five_node", "asymmetric_three_node", "inf_arc_four_node"])
+def test_hand_vectors(engine_cls, name):
+    say("hand", name)
+    g, hv = hand_graph(name)
+    o = Oracle(g)
+    with engine_cls(g) as e:
+        allsrc = list(range(g.n))
+        r = e.query_batch(allsrc)
+        for s in allsrc:
+            assert np.array_equal(r.dist[s], o.phast_query(s))
+        for s, exp in hv.get("forward", {}).items():
+            assert r.dist[int(s)].tolist() == exp
+        m, _ = e.compute_matrix_flat(allsrc, allsrc)
+        assert np.array_equal(m, o.compute_matrix_flat(allsrc, allsrc))
+
+
+@pytest.mark.parametrize("n_leaf", [12, 200_000])
+def test_broom_closed_form(engine_cls, n_leaf):
+    say("broom", n_leaf)
+    g, wu, wd = broom_graph(n_leaf)
+    rng = np.random.default_rng(n_leaf)
+    s = rng.choice(n_leaf, min(n_leaf, 300), replace=False)
+    t = rng.choice(n_leaf, min(n_leaf, 500), replace=False)
+    t[0] = s[0]  # a diagonal cell
+    with engine_cls(g) as e:
+        m, _ = e.compute_matrix_flat(s, t)
+        assert np.array_equal(m.reshape(s.size, t.size), broom_expect(wu, wd, s, t))
+        for thr in (0, 75, 130, 100_000):
+            mb, _ = e.compute_matrix_block_gated(s, t, thr)
+            assert np.array_equal(mb.reshape(s.size, t.size), mask(broom_expect(wu, wd, s, t), thr))
+
+
+def test_matrix_config1_100x100(engine_cls, grid_config1):
+    """224x224 grid (the reference's CPU configuration), 100x100 uniform ranks."""
+    g = grid_config1
+    say("config1 100x100 n", g.n)
+    rng = np.random.default_rng(20261203)
+    s = rng.integers(0, g.n, 100).astype(np.uint32)
+    t = rng.integers(0, g.n, 100).astype(np.uint32)
+    o = Oracle(g)
+    exp = o.compute_matrix_flat(s, t)
+    with engine_cls(g) as e:
+        got, st = e.compute_matrix_flat(s, t)
+    assert np.array_equal(got, exp)
+    assert (got != INF).mean() > 0.5
+
+
+@pytest.mark.parametrize("lanes,cap", [(32, 0), (16, 48), (64, 64)])
+def test_matrix_lanes_and_batches(engine_cls, grid_config1, lanes, cap):
+    """K in {16, 32, 64}, and a lane capacity that forces several device batches."""
+    g = grid_config1
+    say("lanes", lanes, "cap", cap)
+    rng = np.random.default_rng(lanes)
+    s = rng.integers(0, g.n, 200).astype(np.uint32)
+    t = rng.integers(0, g.n, 333).astype(np.uint32)
+    o = Oracle(g)
+    exp = o.compute_matrix_flat_mt(s, t)
+    with engine_cls(g) as e:
+        e.configure(lanes=lanes, lane_capacity=cap)
+        got, st = e.compute_matrix_flat(s, t)
+        assert np.array_equal(got, exp)
+        if cap:
+            assert st.batches == (200 + cap - 1) // cap
+
+
+def test_query_batch_full_rows_and_edges(engine_cls, grid_small):
+    g = grid_small
+    o = Oracle(g)
+    say("full rows n", g.n)
+    src = np.array([0, g.n - 1, 5, 5, 17], dtype=np.uint32)  # extremes and a duplicate
+    with engine_cls(g) as e:
+        r = e.query_batch(src)
+        for i, s in enumerate(src):
+            assert np.array_equal(r.dist[i], o.phast_query(int(s)))
+        # empty inputs
+        m, _ = e.compute_matrix_flat([], [1, 2])
+        assert m.size == 0
+        m, _ = e.compute_matrix_flat([1, 2], [])
+        assert m.size == 0
+        # out-of-range rank is an error, not a crash
+        from bfgpu import BfgError
+        with pytest.raises(BfgError):
+            e.compute_matrix_flat([g.n], [0])
+
+
+def test_bounded_and_isochrones_uneven_limits(engine_cls, grid_config1):
+    g = grid_config1
+    o = Oracle(g)
+    rng = np.random.default_rng(99)
+    L = 40
+    org = rng.integers(0, g.n, L).astype(np.uint32)
+    lim = rng.choice([300, 600, 900, 1800, 2700, 3600], L).astype(np.uint32)
+    say("isochrones", L)
+    with engine_cls(g) as e:
+        bits, times = e.isochrones(org, lim)
+        for i in range(L):
+            full = o.phast_query(int(org[i]))
+            exp = mask(full, lim[i])
+            assert np.array_equal(times[i], exp)
+            # reach bits
+            reach = np.unpackbits(bits[i].view(np.uint8), bitorder="little")[:g.n].astype(bool)
+            assert np.array_equal(reach, exp != INF)
+            # same lane, reference bounded engines (values > T masked)
+        blk = e.query_batch_block_gated(org[:8], 900)
+        ref = o.query_batch_block_gated(org[:8], 900)
+        assert np.array_equal(blk.dist, mask(ref, 900))
+
+
+def test_seeded_multi_seed(engine_cls, grid_small):
+    g = grid_small
+    o = Oracle(g)
+    rng = np.random.default_rng(4)
+    lists, thr = [], []
+    for i in range(20):
+        k = int(rng.integers(1, 6))
+        lists.append([(int(r), int(c)) for r, c in zip(rng.integers(0, g.n, k), rng.integers(0, 200, k))])
+        thr.append(int(rng.integers(50, 3000)))
+    lists.append([(3, 10), (3, 4)])  # duplicate seed rank keeps the smaller cost
+    thr.append(5000)
+    say("seeded", len(lists))
+    with engine_cls(g) as e:
+        rows = e.phast_seeded(lists, thr)
+        for i, seeds in enumerate(lists):
+            exp = np.full(g.n, INF, dtype=np.uint32)
+            for r, d in o.seeded(seeds, thr[i]):
+                exp[r] = d
+            assert np.array_equal(rows[i], exp), i
+        got = e.run_phast_bounded_fast_seeded(lists[0], thr[0])
+        assert got == o.seeded(lists[0], thr[0])
+
+
+def test_chords_graph(engine_cls):
+    """A grid with long motorway chords (the country-graph generator at small size)."""
+    g = grid_graph(64, chord_frac=0.02, seed=7)
+    o = Oracle(g)
+    rng = np.random.default_rng(8)
+    s = rng.integers(0, g.n, 64).astype(np.uint32)
+    t = rng.integers(0, g.n, 64).astype(np.uint32)
+    say("chords n", g.n)
+    with engine_cls(g) as e:
+        got, _ = e.compute_matrix_flat(s, t)
+    assert np.array_equal(got, o.compute_matrix_flat(s, t))

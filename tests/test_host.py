@@ -1,0 +1,85 @@
+"""CPU tests of the host logic: C-ABI exports, layout/schedule validation, input
+checks and the weight-width codec.  No device is touched."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import REPO, broom_graph, hand_graph
+
+from bfgpu import CchGraph, _native, weight_decode, weight_encode, weight_width_choose
+from bfgpu.graph import INF
+
+
+def header_symbols():
+    with open(os.path.join(REPO, "include", "bfgpu.h")) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?\s*(bfg_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    syms = header_symbols()
+    assert len(syms) >= 12
+    lib = _native.bfgpu()  # loads without a device
+    for s in syms:
+        assert hasattr(lib, s), s
+        assert s in _native.BFGPU_SYMBOLS, f"{s} has no ctypes binding"
+
+
+def validate(g):
+    info = _native.CtxInfo()
+    d = g.desc()
+    rc = _native.bfgpu().bfg_validate_graph(C.byref(d), C.byref(info))
+    return rc, info
+
+
+@pytest.mark.parametrize("name", ["five_node", "asymmetric_three_node", "inf_arc_four_node"])
+def test_validate_hand_graphs(name):
+    g, _ = hand_graph(name)
+    rc, info = validate(g)
+    assert rc == 0, _native.bfgpu().bfg_last_error()
+    assert info.n_nodes == g.n
+    if name == "inf_arc_four_node":
+        assert (info.m_up, info.m_down) == (3, 3)  # INF arcs dropped
+
+
+def test_validate_broom_and_grids(grid_small, grid_config1):
+    g, _, _ = broom_graph(1000)
+    rc, info = validate(g)
+    assert rc == 0 and info.etree_depth == 1
+    for gg in (grid_small, grid_config1):
+        rc, info = validate(gg)
+        assert rc == 0, _native.bfgpu().bfg_last_error()
+        assert info.m_down == gg.m_down and info.m_up == gg.m_up
+        assert info.n_steps >= info.n_tasks >= info.n_levels >= 1
+
+
+def test_rejects_non_rank_aligned_cch():
+    g = CchGraph.from_arcs(3, [(2, 1, 5)], [])  # UP arc going down
+    rc, _ = validate(g)
+    assert rc == 1
+    assert b"UP arc" in _native.bfgpu().bfg_last_error()
+    g = CchGraph.from_arcs(3, [], [(0, 2, 5)])  # DOWN arc going up
+    assert validate(g)[0] == 1
+
+
+def test_weight_width_codec():
+    # WeightWidth::choose boundaries (cch_weights.rs:106-132)
+    assert weight_width_choose([0, 65534, INF]) == "u16"
+    assert weight_width_choose([65535]) == "u24"
+    assert weight_width_choose([0xFFFFFE]) == "u24"
+    assert weight_width_choose([0xFFFFFF]) == "u32"
+    rng = np.random.default_rng(1)
+    for hi in (65534, 0xFFFFFE, 0xFFFFFFFE):
+        w = rng.integers(0, hi + 1, 1000, dtype=np.uint64).astype(np.uint32)
+        w[::7] = INF
+        width = weight_width_choose(w)
+        enc = weight_encode(w, width)
+        assert enc.dtype.itemsize * (3 if width == "u24" else 1) == {"u16": 2, "u24": 3, "u32": 4}[width] * (
+            3 if width == "u24" else 1) or width == "u24"
+        assert np.array_equal(weight_decode(enc, width), w)
+    # sentinel of each width widens to u32::MAX (WeightArray::get)
+    assert weight_decode(np.array([0xFFFF, 7], dtype=np.uint16), "u16").tolist() == [INF, 7]
+    assert weight_decode(np.array([0xFF, 0xFF, 0xFF, 1, 0, 0], dtype=np.uint8), "u24").tolist() == [INF, 1]
