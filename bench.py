@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark: K-lane batched PHAST S x T duration matrix on one MI355X.
+
+Default workload: 10000 x 10000 matrix (100M u32 cells) on the seeded
+1414 x 1414 jittered road grid with 2 % long motorway chords (~2M NBG nodes,
+edge-based CCH of ~6.6M nodes), sources/targets 80 % uniform + 20 % in five
+Gaussian hot-spots (sigma = 2 % of the extent) snapped to the nearest node.
+A step is one full matrix with queries and results resident in HBM.
+
+Prints one JSON line (the contract of this repository's driver) with
+`roofline` (the sweep kernel against the 8 TB/s HBM peak) and `cpu_baseline`
+(the oracle's restatement of the reference's compute_matrix_flat, timed on a
+bounded sample on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "butterfly-osm_amd")
+for _p in (REPO, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "matrix cells/s 10k×10k u32 bit-exact, % HBM roofline; serving p50/p99 vs plain"
+HBM_PEAK_GBPS = 8000.0
+
+WORKLOADS = {
+    # name: (grid rows, chord fraction, S, T)
+    "matrix10k": (1414, 0.02, 10000, 10000),
+    "matrix1k": (1414, 0.02, 1000, 1000),
+    "config1": (224, 0.0, 100, 100),
+    "mid": (448, 0.02, 2000, 2000),
+}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_queries(g, count, seed, hot_frac=0.2, n_hot=5, sigma_frac=0.02):
+    """80 % uniform ranks + 20 % around 5 Gaussian hot-spots, snapped to nodes."""
+    rng = np.random.default_rng(seed)
+    n_hot_pts = int(round(hot_frac * count))
+    uni = rng.integers(0, g.n, count - n_hot_pts)
+    if n_hot_pts == 0 or g.x is None:
+        return rng.permutation(np.concatenate([uni, rng.integers(0, g.n, n_hot_pts)])).astype(np.uint32)
+    from scipy.spatial import cKDTree
+    xy = np.c_[g.x, g.y].astype(np.float64)
+    lo, hi = xy.min(0), xy.max(0)
+    ext = hi - lo
+    centers = lo + rng.random((n_hot, 2)) * ext
+    which = rng.integers(0, n_hot, n_hot_pts)
+    pts = centers[which] + rng.normal(0.0, 1.0, (n_hot_pts, 2)) * (sigma_frac * ext)
+    _, idx = cKDTree(xy).query(pts)
+    return rng.permutation(np.concatenate([uni, idx])).astype(np.uint32)
+
+
+def cpu_baseline(g, src, tgt, budget_s, gpu_rows):
+    """Single-thread oracle (compute_matrix_flat, as the reference runs it) on a
+    bounded prefix of the sources; the sample rows are checked against the GPU."""
+    from oracle.oracle import Oracle
+    o = Oracle(g)
+    t0 = time.perf_counter()
+    first = o.compute_matrix_flat_soa(src[:8], tgt)
+    dt1 = time.perf_counter() - t0
+    n_batches = max(1, min(int(budget_s / max(dt1, 1e-3)), (len(src) + 7) // 8))
+    rows = [first]
+    t0 = time.perf_counter()
+    if n_batches > 1:
+        rows.append(o.compute_matrix_flat_soa(src[8:8 * n_batches], tgt))
+    dt = time.perf_counter() - t0 + dt1
+    cpu = np.concatenate(rows).reshape(-1, len(tgt))
+    ok = bool(np.array_equal(cpu, gpu_rows[:cpu.shape[0]]))
+    cells = cpu.shape[0] * len(tgt)
+    return {"value": cells / dt, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": f"{cpu.shape[0]} sources x {len(tgt)} targets (first sources of the workload), "
+                      f"{dt:.1f} s single-thread, orc_compute_matrix_flat_soa (K=8 lanes, "
+                      f"batched_phast.rs:756-917 semantics)",
+            "bit_exact_vs_gpu_rows": ok}, ok
+
+
+def cpu_baseline_mt(g, src, tgt, budget_s):
+    from oracle.oracle import Oracle
+    o = Oracle(g)
+    nth = os.cpu_count() or 1
+    nth = min(nth, 16)
+    k = 8 * nth
+    t0 = time.perf_counter()
+    o.compute_matrix_flat_mt(src[:k], tgt, nth)
+    dt = time.perf_counter() - t0
+    return {"value": k * len(tgt) / dt, "unit": "cells/s", "cores": nth, "kind": "port",
+            "sample": f"{k} sources x {len(tgt)} targets, one K=8 batch per thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="matrix10k", choices=sorted(WORKLOADS))
+    ap.add_argument("--lanes", type=int, default=32, help="K lanes per group (16, 32, 64)")
+    ap.add_argument("--lane-capacity", type=int, default=0, help="sources per device batch (0 = auto)")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of single-thread CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph-cache", default=os.environ.get("BFG_GRAPH_CACHE", ""))
+    ap.add_argument("--seed", type=int, default=20261203)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU: --gpus {args.gpus} is not supported by this build "
+              f"(the path shards by sources; see DESIGN.md for the multi-GPU design)", file=sys.stderr)
+        sys.exit(2)
+
+    from bfgpu import BatchedPhastEngine, GridSpec, build_grid, cached_grid
+    from bfgpu.hip import DeviceArray, sync
+
+    rows, chords, S, T = WORKLOADS[args.workload]
+    spec = GridSpec(rows=rows, cols=rows, seed=args.seed, chord_frac=chords)
+    t0 = time.perf_counter()
+    log(f"building graph {rows}x{rows} chords={chords} ...")
+    g = cached_grid(spec, args.graph_cache, verbose=True, keep_ebg=False) if args.graph_cache else \
+        build_grid(spec, verbose=True, keep_ebg=False)
+    log(f"graph n={g.n} m_up={g.m_up} m_down={g.m_down} built in {time.perf_counter() - t0:.1f}s")
+    src = make_queries(g, S, args.seed + 1)
+    tgt = make_queries(g, T, args.seed + 2)
+
+    t0 = time.perf_counter()
+    eng = BatchedPhastEngine(g, device=0)
+    eng.configure(lanes=args.lanes, lane_capacity=args.lane_capacity)
+    info = eng.info()
+    log(f"context ready in {time.perf_counter() - t0:.1f}s: {info}")
+
+    d_src = DeviceArray.from_host(src)
+    d_tgt = DeviceArray.from_host(tgt)
+    d_out = DeviceArray(S * T, np.uint32)
+
+    def step():
+        return eng.compute_matrix_device(d_src.ptr, S, d_tgt.ptr, T, d_out.ptr)
+
+    for i in range(args.warmup):
+        st = step()
+        log(f"warmup {i}: up {st.upward_time_ms:.1f} ms sweep {st.downward_time_ms:.1f} ms "
+            f"out {st.output_time_ms:.1f} ms batches {st.batches}")
+    sync()
+    sweep_ms = up_ms = out_ms = 0.0
+    launches = batches = 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = step()
+        sweep_ms += st.downward_time_ms
+        up_ms += st.upward_time_ms
+        out_ms += st.output_time_ms
+        launches += st.sweep_launches
+        batches += st.batches
+        log(f"step {i}: up {st.upward_time_ms:.1f} ms sweep {st.downward_time_ms:.1f} ms "
+            f"out {st.output_time_ms:.1f} ms")
+    sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1000.0 * wall / args.steps
+    cells = S * T
+    value = cells / (ms_per_step / 1000.0)
+
+    info = eng.info()
+    KL = info["lanes"]
+    cap = info["lane_capacity"]
+    groups_per_step = sum((min(cap, S - b) + KL - 1) // KL for b in range(0, S, cap))
+    n, m_dn = info["n_nodes"], info["m_down"]
+    bytes_per_group = 4 * (n + 1) + 8 * m_dn + 8 * KL * n
+    sweep_bytes = groups_per_step * bytes_per_group * args.steps
+    achieved = sweep_bytes / (sweep_ms / 1000.0) / 1e9 if sweep_ms > 0 else 0.0
+
+    n_check = min(S, 64)
+    gpu_rows = d_out.to_host(0, n_check * T).reshape(n_check, T)
+    cpu = mt = None
+    exact = None
+    if not args.no_cpu:
+        log("cpu baseline (single thread) ...")
+        cpu, exact = cpu_baseline(g, src[:n_check], tgt, args.cpu_budget, gpu_rows)
+        log(f"cpu baseline {cpu['value']:.3g} cells/s, bit-exact rows: {exact}")
+        mt = cpu_baseline_mt(g, src, tgt, args.cpu_budget)
+        log(f"cpu baseline all cores {mt['value']:.3g} cells/s ({mt['cores']} threads)")
+
+    res = {
+        "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"{S}x{T} duration matrix, {rows}x{rows} jittered grid"
+                               f"{' + 2% motorway chords' if chords else ''}, edge-based CCH n={n}",
+                   "sources": S, "targets": T, "graph_seed": args.seed, "lanes": KL,
+                   "lane_capacity": cap, "n_nodes": n, "m_up": info["m_up"], "m_down": m_dn,
+                   "etree_depth": info["etree_depth"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "sweep_kernel", "bytes_per_lane_group": bytes_per_group,
+                     "lane_groups_per_step": groups_per_step, "sweep_launches_per_step": launches // args.steps,
+                     "sweep_ms_per_step": sweep_ms / args.steps},
+        "phases_ms_per_step": {"up": up_ms / args.steps, "sweep": sweep_ms / args.steps,
+                               "gather": out_ms / args.steps},
+        "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": mt,
+        "bit_exact_sample": exact,
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

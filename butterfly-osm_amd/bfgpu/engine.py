@@ -133,6 +133,12 @@ class BatchedPhastEngine:
                                             int(threshold) & INF, _native.ptr(out)))
         return out, self.last_stats(src.size)
 
+    def compute_matrix_device(self, d_sources, n_src, d_targets, n_tgt, d_out, threshold=INF):
+        """Device-resident matrix (bfg_matrix_device): raw HIP pointers (ints or c_void_p)."""
+        _check(self._lib.bfg_matrix_device(self._h, d_sources, n_src, d_targets, n_tgt, int(threshold) & INF,
+                                           d_out))
+        return self.last_stats(n_src)
+
     def compute_matrix(self, sources, targets, callback):
         """batched_phast.rs:619-661: callback(first_source_idx, rows, chunk) per 8-source chunk."""
         src = _u32(sources)

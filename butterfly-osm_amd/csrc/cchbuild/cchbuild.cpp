@@ -6,6 +6,8 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <atomic>
+#include <memory>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -518,27 +520,55 @@ struct Orderer {
 struct Ent { uint32_t v, w; };
 typedef std::vector<Ent> List;
 
-// merge sorted `add` into sorted `dst`, min on equal keys; returns #new keys
-size_t merge_into(List &dst, const Ent *add, size_t na, List &tmp) {
-    size_t i = 0, j = 0, fresh = 0;
-    // fast path: count new keys while updating mins in place
-    while (i < dst.size() && j < na) {
-        if (dst[i].v < add[j].v) ++i;
-        else if (dst[i].v > add[j].v) { ++fresh; ++j; }
-        else { if (add[j].w < dst[i].w) dst[i].w = add[j].w; ++i; ++j; }
+// Merges the sorted entries add[0..na) — key add[j].v, weight base + add[j].w
+// (saturating) — into sorted `dst`, keeping the minimum weight on equal keys.
+// Returns the number of keys that were new.  The common clique case (every key
+// already present) is updated in place without allocation.
+size_t merge_add(List &dst, const Ent *add, size_t na, uint32_t base, List &tmp) {
+    const size_t nd = dst.size();
+    Ent *d = dst.data();
+    size_t fresh = 0;
+    if (na * 16 < nd) {
+        size_t lo = 0;
+        for (size_t j = 0; j < na; ++j) {
+            const uint32_t key = add[j].v;
+            size_t pos = (size_t)(std::lower_bound(d + lo, d + nd, key,
+                                                   [](const Ent &e, uint32_t k) { return e.v < k; }) - d);
+            if (pos < nd && d[pos].v == key) {
+                const uint32_t cand = sat_add(base, add[j].w);
+                if (cand < d[pos].w) d[pos].w = cand;
+                lo = pos + 1;
+            } else {
+                ++fresh;
+                lo = pos;
+            }
+        }
+    } else {
+        size_t i = 0, j = 0;
+        while (i < nd && j < na) {
+            const uint32_t a = d[i].v, b = add[j].v;
+            if (a == b) {
+                const uint32_t cand = sat_add(base, add[j].w);
+                d[i].w = cand < d[i].w ? cand : d[i].w;
+            }
+            fresh += (b < a);
+            i += (a <= b);
+            j += (b <= a);
+        }
+        fresh += na - j;
     }
-    fresh += na - j;
     if (fresh == 0) return 0;
-    tmp.clear();
-    tmp.reserve(dst.size() + fresh);
-    i = 0; j = 0;
-    while (i < dst.size() && j < na) {
-        if (dst[i].v < add[j].v) tmp.push_back(dst[i++]);
-        else if (dst[i].v > add[j].v) tmp.push_back(add[j++]);
-        else { tmp.push_back(dst[i]); ++i; ++j; }
+    tmp.resize(nd + fresh);
+    Ent *o = tmp.data();
+    size_t i = 0, j = 0, k = 0;
+    while (i < nd && j < na) {
+        if (d[i].v < add[j].v) o[k++] = d[i++];
+        else if (d[i].v > add[j].v) { o[k++] = {add[j].v, sat_add(base, add[j].w)}; ++j; }
+        else { o[k++] = d[i]; ++i; ++j; }
     }
-    while (i < dst.size()) tmp.push_back(dst[i++]);
-    while (j < na) tmp.push_back(add[j++]);
+    while (i < nd) o[k++] = d[i++];
+    while (j < na) { o[k++] = {add[j].v, sat_add(base, add[j].w)}; ++j; }
+    tmp.resize(k);
     dst.swap(tmp);
     return fresh;
 }
@@ -613,43 +643,132 @@ bfc_cch *bfc_build_grid(const bfc_params *pp) {
     uint64_t m_orig = 0;
     for (uint32_t r = 0; r < n; ++r) m_orig += out_hi[r].size() + in_hi[r].size();
 
+    // Every fill arc joins two ancestors of the contracted node in the
+    // elimination tree of the symmetrised original pattern (Liu), so a node's
+    // lists are final once its subtree is contracted.  Phase A contracts the
+    // small bottom subtrees in parallel (one thread per subtree, spin locks on
+    // lists owned by nodes outside the subtree); phase B contracts the rest in
+    // rank order with the pair loops spread over threads.
+    std::vector<uint32_t> etp(n, INF);
+    {
+        std::vector<uint32_t> lo_off(n + 1, 0);
+        for (uint32_t i = 0; i < n; ++i) {
+            for (auto &e : out_hi[i]) lo_off[e.v + 1]++;
+            for (auto &e : in_hi[i]) lo_off[e.v + 1]++;
+        }
+        for (uint32_t j = 0; j < n; ++j) lo_off[j + 1] += lo_off[j];
+        std::vector<uint32_t> lo_adj(lo_off[n]);
+        {
+            std::vector<uint32_t> cur(lo_off.begin(), lo_off.end() - 1);
+            for (uint32_t i = 0; i < n; ++i) {
+                for (auto &e : out_hi[i]) lo_adj[cur[e.v]++] = i;
+                for (auto &e : in_hi[i]) lo_adj[cur[e.v]++] = i;
+            }
+        }
+        std::vector<uint32_t> anc(n, INF);
+        for (uint32_t j = 0; j < n; ++j)
+            for (uint32_t k = lo_off[j]; k < lo_off[j + 1]; ++k) {
+                uint32_t r = lo_adj[k];
+                while (anc[r] != INF && anc[r] != j) { uint32_t t = anc[r]; anc[r] = j; r = t; }
+                if (anc[r] == INF) { anc[r] = j; etp[r] = j; }
+            }
+    }
+    std::vector<uint32_t> sub(n, 1);
+    for (uint32_t v = 0; v < n; ++v) if (etp[v] != INF) sub[etp[v]] += sub[v];
+    const uint32_t SA = std::max<uint32_t>(4096, n / (uint32_t)(64 * std::max(1, omp_get_max_threads())));
+    // A-root of every bottom node (INF for phase-B nodes); parents before children
+    std::vector<uint32_t> aroot(n, INF);
+    std::vector<uint32_t> roots;
+    for (int64_t vv = (int64_t)n - 1; vv >= 0; --vv) {
+        uint32_t v = (uint32_t)vv, p = etp[v];
+        if (sub[v] > SA) continue;
+        if (p == INF || sub[p] > SA) { aroot[v] = v; roots.push_back(v); }
+        else aroot[v] = aroot[p];
+    }
+    // nodes of each A-subtree in ascending rank
+    std::vector<uint32_t> rid(n, INF);
+    for (size_t i = 0; i < roots.size(); ++i) rid[roots[i]] = (uint32_t)i;
+    std::vector<uint32_t> ar_off(roots.size() + 1, 0), ar_nodes;
+    for (uint32_t v = 0; v < n; ++v) if (aroot[v] != INF) ar_off[rid[aroot[v]] + 1]++;
+    for (size_t i = 0; i < roots.size(); ++i) ar_off[i + 1] += ar_off[i];
+    ar_nodes.resize(ar_off[roots.size()]);
+    {
+        std::vector<uint32_t> cur(ar_off.begin(), ar_off.end() - 1);
+        for (uint32_t v = 0; v < n; ++v) if (aroot[v] != INF) ar_nodes[cur[rid[aroot[v]]]++] = v;
+    }
+    std::unique_ptr<std::atomic<uint8_t>[]> lk(new std::atomic<uint8_t>[n]);
+    for (uint32_t v = 0; v < n; ++v) lk[v].store(0, std::memory_order_relaxed);
+    auto lock = [&](uint32_t v) {
+        while (lk[v].exchange(1, std::memory_order_acquire)) {
+            while (lk[v].load(std::memory_order_relaxed)) { }
+        }
+    };
+    auto unlock = [&](uint32_t v) { lk[v].store(0, std::memory_order_release); };
+    auto contract_serial = [&](uint32_t x, uint32_t root, List &tmp) -> uint64_t {
+        const List &I = in_hi[x];
+        const List &O = out_hi[x];
+        uint64_t fresh = 0;
+        if (I.empty() || O.empty()) return 0;
+        for (size_t a = 0; a < I.size(); ++a) {
+            const uint32_t u = I[a].v, dux = I[a].w;
+            auto it = std::upper_bound(O.begin(), O.end(), u, [](uint32_t val, const Ent &e) { return val < e.v; });
+            size_t k0 = (size_t)(it - O.begin());
+            if (k0 == O.size()) continue;
+            const bool ext = aroot[u] != root;
+            if (ext) lock(u);
+            fresh += merge_add(out_hi[u], O.data() + k0, O.size() - k0, dux, tmp);
+            if (ext) unlock(u);
+        }
+        for (size_t b = 0; b < O.size(); ++b) {
+            const uint32_t w = O[b].v, dxw = O[b].w;
+            auto it = std::upper_bound(I.begin(), I.end(), w, [](uint32_t val, const Ent &e) { return val < e.v; });
+            size_t k0 = (size_t)(it - I.begin());
+            if (k0 == I.size()) continue;
+            const bool ext = aroot[w] != root;
+            if (ext) lock(w);
+            fresh += merge_add(in_hi[w], I.data() + k0, I.size() - k0, dxw, tmp);
+            if (ext) unlock(w);
+        }
+        return fresh;
+    };
     uint64_t n_short = 0;
     std::vector<List> tmps(omp_get_max_threads());
-    std::vector<Ent> add;
-    size_t report = std::max<uint32_t>(n / 20, 1);
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : n_short)
+    for (int64_t ri = 0; ri < (int64_t)roots.size(); ++ri) {
+        List &tmp = tmps[omp_get_thread_num()];
+        for (uint32_t k = ar_off[ri]; k < ar_off[ri + 1]; ++k) n_short += contract_serial(ar_nodes[k], roots[ri], tmp);
+    }
+    if (vb)
+        fprintf(stderr, "[cchbuild] contract phase A: %zu subtrees (<= %u nodes), %zu nodes (%.1fs)\n", roots.size(),
+                SA, ar_nodes.size(), now_s() - t2);
+    size_t done_b = 0, n_b = n - ar_nodes.size(), report = std::max<size_t>(n_b / 10, 1);
     for (uint32_t x = 0; x < n; ++x) {
-        if (vb && x % report == 0 && x)
-            fprintf(stderr, "[cchbuild] contract %5.1f%% shortcuts %llu (%.1fs)\n", 100.0 * x / n,
-                    (unsigned long long)n_short, now_s() - t2);
+        if (aroot[x] != INF) continue;
+        if (vb && ++done_b % report == 0)
+            fprintf(stderr, "[cchbuild] contract phase B %5.1f%% (%.1fs)\n", 100.0 * done_b / n_b, now_s() - t2);
         const List &I = in_hi[x];
         const List &O = out_hi[x];
         if (I.empty() || O.empty()) continue;
         const size_t work = I.size() * O.size();
-        const bool par = work > 4096;
+        if (work <= 65536) { n_short += contract_serial(x, INF - 1, tmps[0]); continue; }
         uint64_t fresh_total = 0;
         // (1) u in I: arcs u->w with w > u go to out_hi[u]
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : fresh_total) if (par)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : fresh_total)
         for (int64_t a = 0; a < (int64_t)I.size(); ++a) {
             const uint32_t u = I[a].v, dux = I[a].w;
             auto it = std::upper_bound(O.begin(), O.end(), u, [](uint32_t val, const Ent &e) { return val < e.v; });
             size_t k0 = (size_t)(it - O.begin());
             if (k0 == O.size()) continue;
-            List &tmp = tmps[omp_get_thread_num()];
-            std::vector<Ent> loc(O.size() - k0);
-            for (size_t k = k0; k < O.size(); ++k) loc[k - k0] = {O[k].v, sat_add(dux, O[k].w)};
-            fresh_total += merge_into(out_hi[u], loc.data(), loc.size(), tmp);
+            fresh_total += merge_add(out_hi[u], O.data() + k0, O.size() - k0, dux, tmps[omp_get_thread_num()]);
         }
         // (2) w in O: arcs u->w with u > w go to in_hi[w]
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : fresh_total) if (par)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : fresh_total)
         for (int64_t b = 0; b < (int64_t)O.size(); ++b) {
             const uint32_t w = O[b].v, dxw = O[b].w;
             auto it = std::upper_bound(I.begin(), I.end(), w, [](uint32_t val, const Ent &e) { return val < e.v; });
             size_t k0 = (size_t)(it - I.begin());
             if (k0 == I.size()) continue;
-            List &tmp = tmps[omp_get_thread_num()];
-            std::vector<Ent> loc(I.size() - k0);
-            for (size_t k = k0; k < I.size(); ++k) loc[k - k0] = {I[k].v, sat_add(I[k].w, dxw)};
-            fresh_total += merge_into(in_hi[w], loc.data(), loc.size(), tmp);
+            fresh_total += merge_add(in_hi[w], I.data() + k0, I.size() - k0, dxw, tmps[omp_get_thread_num()]);
         }
         n_short += fresh_total;
     }
