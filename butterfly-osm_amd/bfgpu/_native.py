@@ -22,8 +22,9 @@ class GraphDesc(C.Structure):
 
 class CtxInfo(C.Structure):
     _fields_ = [("n_nodes", C.c_uint32), ("m_up", C.c_uint64), ("m_down", C.c_uint64),
-                ("etree_depth", C.c_uint32), ("n_levels", C.c_uint32), ("n_tasks", C.c_uint32),
-                ("n_steps", C.c_uint32), ("lanes", C.c_uint32), ("lane_capacity", C.c_uint32),
+                ("etree_depth", C.c_uint32), ("n_top", C.c_uint32), ("n_segments", C.c_uint32),
+                ("n_segment_levels", C.c_uint32), ("n_bottom_levels", C.c_uint32),
+                ("lanes", C.c_uint32), ("lane_capacity", C.c_uint32),
                 ("device_bytes", C.c_uint64)]
 
 

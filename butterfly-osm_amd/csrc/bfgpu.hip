@@ -1,26 +1,39 @@
 // bfgpu.hip — MI355X (gfx950) K-lane batched PHAST: kernels + C ABI.
 //
-// Phases of one batch of source lanes (sources are CCH ranks):
-//   K1 up_walk   — per lane, the upward search of the reference
-//                  (batched_phast.rs:156-189, isochrone_handler.rs:388-411)
-//                  as an elimination-tree ancestor walk: the union of the
-//                  seeds' root paths is visited in ascending rank (a valid
-//                  order for UP arcs), relaxing UP arcs into per-seed path
-//                  slots indexed by depth.  One wavefront per lane.
-//   K2 sweep     — the rank-descending DOWN scan of the reference
-//                  (batched_phast.rs:205-250, :538-590) as a pull sweep over
-//                  DOWN arcs keyed by head, scheduled by the etree: kernel
-//                  launches walk task levels, a workgroup owns one task for
-//                  KL interleaved lanes, steps inside a task are separated by
-//                  workgroup barriers.  Labels live in an arena
-//                  [group][position][KL] so each arc read serves KL lanes with
-//                  one coalesced KL*4-byte access.
-//   K4 gather    — target extraction (batched_phast.rs:693-701) through an
-//                  LDS transpose into row-major S x T, with the per-lane limit
-//                  mask; also serves full distance rows and isochrone bits.
-// Arithmetic is u32 with saturating adds (v_add_u32 clamp) and min; the
-// result is the unique shortest up-down distance, so it is bit-identical to
-// the reference whatever the traversal order.
+// One batch of source lanes (sources are CCH ranks; lanes are sorted by the
+// pre-order of their source in the elimination tree so a group of KL lanes
+// shares most of its root path).  Labels live in an arena
+// [group][position][KL]: one node's KL labels are one coalesced KL*4-byte
+// access, so every arc read is reused KL times.
+//
+// Upward phase — the reference's per-lane heap Dijkstra over UP arcs
+// (batched_phast.rs:156-189, isochrone_handler.rs:388-411) as an ancestor
+// walk, since the up search space lies on the source's root path:
+//   fill_top     top-region labels of the batch := INF
+//   bottom_walk  one wavefront per lane walks the bottom part of its root path
+//                (depth-indexed slots), relaxing UP arcs into bottom ancestors
+//                (slots) and top ancestors (atomic min into the arena)
+//   up_seg       per (segment, group), deepest segment level first: the
+//                segment's labels in LDS, intra-segment UP arcs relaxed from
+//                the last node to the first (sparse lists), then the UP arcs
+//                leaving the segment, grouped by target, reduced per lane and
+//                pushed with one atomic min per (target, lane)
+// Downward phase — the rank-descending DOWN scan (batched_phast.rs:205-250,
+// :538-590) scheduled by the elimination tree:
+//   down_seg     per (segment, group) of a segment level: labels in LDS seeded
+//                with the upward labels, DOWN arcs from outside the segment
+//                grouped by source (one source label read serves all its
+//                arcs), then intra-segment DOWN arcs first node to last
+//   bottom_chunk per (bottom chunk, group): seeds from the lanes' bottom path
+//                slots, then DOWN arcs from top-region sources, grouped by
+//                source, into the chunk's LDS labels
+//   bottom_pull  per (node, group) of a bottom height level (descending): min
+//                over DOWN arcs from bottom-region sources
+// Output — gather: target extraction (batched_phast.rs:693-701) through an
+// LDS transpose into row-major rows with the per-lane limit mask; it also
+// writes full distance rows and isochrone reach bits.
+// Arithmetic is u32 with saturating adds (v_add_u32 clamp) and min, so the
+// labels equal the unique shortest up-down distances the reference computes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,6 +42,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -55,6 +69,9 @@ int fail(int code, const std::string &msg) {
                         std::string(#expr) + ": " + hipGetErrorString(e_));                  \
     } while (0)
 
+constexpr int NT = 256;   // threads per workgroup of the gather kernel
+constexpr int NTB = 512;  // threads per workgroup of the segment and block kernels
+
 __device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
     return __builtin_elementwise_add_sat(a, b);
 }
@@ -66,39 +83,65 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
 }
 
 // ---------------------------------------------------------------------------
-// K1: upward ancestor walk, one wavefront (64 threads) per lane.
-// Lane l owns seeds j in [seed_off[l], seed_off[l+1]) (<= 64), each with a path
-// slot pathbuf[j*D1 + i] holding the label of the node i levels above seed j.
+__global__ void fill_top_kernel(uint32_t *__restrict__ arena, uint64_t group_stride, uint64_t top_words,
+                                uint32_t groups) {
+    const uint64_t total = top_words * groups;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t g = i / top_words, r = i - g * top_words;
+        arena[g * group_stride + r] = INF;
+    }
+}
+
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void up_walk_kernel(
+// bottom_walk: one wavefront per lane (lanes in sorted order).  Seeds of lane
+// l are j in [seed_off[l], seed_off[l+1]) (<= 64), at positions seed_pos[j].
+// Seeds in the top region go straight into the arena; every other seed walks
+// up its bottom path, the walkers of one lane merging where paths meet (the
+// union is visited in ascending rank).  Path slot pathbuf[j*D1 + i] holds the
+// label of the node i levels above seed j.
+// ---------------------------------------------------------------------------
+template <int KL>
+__global__ __launch_bounds__(64) void bottom_walk_kernel(
     const uint32_t *__restrict__ seed_off, const uint32_t *__restrict__ seed_rank,
-    const uint32_t *__restrict__ seed_cost, const uint32_t *__restrict__ seed_depth,
-    const uint32_t *__restrict__ lane_limit, const uint32_t *__restrict__ up_off,
-    const uint32_t *__restrict__ up_tgt, const uint32_t *__restrict__ up_w,
-    const uint32_t *__restrict__ parent, const uint32_t *__restrict__ depth,
-    uint32_t *__restrict__ pathbuf, uint32_t D1) {
+    const uint32_t *__restrict__ seed_pos, const uint32_t *__restrict__ seed_cost,
+    const uint32_t *__restrict__ seed_depth, const uint32_t *__restrict__ lane_limit,
+    const uint32_t *__restrict__ parent, const uint32_t *__restrict__ pos_parent,
+    const uint32_t *__restrict__ pos_depth, const uint32_t *__restrict__ up_off,
+    const uint32_t *__restrict__ up_tgt, const uint32_t *__restrict__ up_w, uint32_t n_top,
+    uint32_t *__restrict__ pathbuf, uint32_t D1, uint32_t *__restrict__ arena, uint32_t n) {
     const uint32_t l = blockIdx.x;
     const uint32_t t = threadIdx.x;
+    const uint32_t g = l / KL, ln = l % KL;
+    uint32_t *lab = arena + (size_t)g * n * KL + ln;
     const uint32_t j0 = seed_off[l];
     const uint32_t ns = seed_off[l + 1] - j0;
     const uint32_t lim = lane_limit[l];
     for (uint32_t j = 0; j < ns; ++j) {
         uint32_t *pb = pathbuf + (size_t)(j0 + j) * D1;
-        const uint32_t dd = seed_depth[j0 + j];
-        for (uint32_t i = t; i <= dd; i += 64) pb[i] = INF;
+        for (uint32_t i = t; i < D1; i += 64) pb[i] = INF;
     }
     __syncthreads();
-    uint32_t cur = NONE;
+    uint32_t cur = NONE, cpos = NONE;
     if (t < ns) {
-        cur = seed_rank[j0 + t];
-        pathbuf[(size_t)(j0 + t) * D1] = seed_cost[j0 + t];
+        const uint32_t p = seed_pos[j0 + t], c = seed_cost[j0 + t];
+        if (p < n_top) {
+            atomicMin(lab + (size_t)p * KL, c);
+        } else {
+            cur = seed_rank[j0 + t];
+            cpos = p;
+            pathbuf[(size_t)(j0 + t) * D1] = c;
+        }
     }
     __syncthreads();
     for (;;) {
         const uint32_t x = wave_min(cur);
         if (x == NONE) break;
         const bool at = (cur == x);
-        const uint32_t dxn = depth[x];
+        const uint64_t mask = __ballot(at);
+        const uint32_t first = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+        const uint32_t px = (uint32_t)__shfl((int)cpos, (int)first, 64);
+        const uint32_t dxn = pos_depth[px];
         uint32_t mine = INF;
         size_t my_slot = 0;
         if (at) {
@@ -106,114 +149,419 @@ __global__ __launch_bounds__(64) void up_walk_kernel(
             mine = pathbuf[my_slot];
         }
         const uint32_t dx = wave_min(mine);
-        const uint64_t mask = __ballot(at);
-        const uint32_t first = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
         __syncthreads();
         if (at) pathbuf[my_slot] = dx;
         if (dx != INF && dx <= lim) {
             const uint32_t js = j0 + first;
             const uint32_t dsj = seed_depth[js];
             uint32_t *pb = pathbuf + (size_t)js * D1;
-            const uint32_t a1 = up_off[x + 1];
-            for (uint32_t a = up_off[x] + t; a < a1; a += 64) {
+            const uint32_t a1 = up_off[px + 1];
+            for (uint32_t a = up_off[px] + t; a < a1; a += 64) {
                 const uint32_t y = up_tgt[a];
                 const uint32_t nd = sat_add(dx, up_w[a]);
-                uint32_t *slot = pb + (dsj - depth[y]);
-                if (nd < *slot) *slot = nd;
+                if (y < n_top) {
+                    atomicMin(lab + (size_t)y * KL, nd);
+                } else {
+                    uint32_t *slot = pb + (dsj - pos_depth[y]);
+                    if (nd < *slot) *slot = nd;
+                }
             }
         }
         __syncthreads();
-        if (at) cur = parent[x];
+        if (at) {
+            const uint32_t pp = pos_parent[cpos];
+            if (pp == NONE || pp < n_top) { cur = NONE; cpos = NONE; }
+            else { cur = parent[x]; cpos = pp; }
+        }
+    }
+}
+
+// A slot is the KL threads that carry one source group's KL lanes.  With
+// KL = 64 a slot is exactly one wavefront, so everything indexed by the slot
+// (sources, arcs, weight rows) is wave-uniform: readfirstlane moves the slot
+// index to a scalar register and the arc reads become scalar loads that every
+// lane sees for free.
+template <int KL>
+__device__ __forceinline__ uint32_t slot_id() {
+    const uint32_t s = threadIdx.x / KL;
+    if constexpr (KL >= 64) return (uint32_t)__builtin_amdgcn_readfirstlane((int)s);
+    return s;
+}
+
+// Relaxes the arcs [a0, a1) of one source (label d of this lane) into the
+// LDS labels.  KL >= 64: a0, a1 are wave-uniform and the arc data comes in
+// through scalar loads.  KL < 64 (two slots per wavefront): the slot's lanes
+// load up to KL arcs with one coalesced access and broadcast them by shuffle.
+template <int KL>
+__device__ __forceinline__ void relax_arcs(uint32_t *L, uint32_t d, uint32_t a0, uint32_t a1,
+                                           const uint16_t *__restrict__ av, const uint32_t *__restrict__ aw,
+                                           uint32_t lane) {
+    if constexpr (KL >= 64) {
+        for (uint32_t base = a0; base < a1; base += 64) {
+            const uint32_t cnt = min(64u, a1 - base);
+            const uint32_t wl = __lane_id();
+            uint32_t mv = 0, mw = INF;
+            if (wl < cnt) { mv = av[base + wl]; mw = aw[base + wl]; }
+#pragma unroll 4
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)mv, (int)j);
+                const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)mw, (int)j);
+                if (d != INF) atomicMin(&L[v * KL + lane], sat_add(d, w));
+            }
+        }
+    } else {
+        for (uint32_t base = a0; base < a1; base += KL) {
+            const uint32_t cnt = min((uint32_t)KL, a1 - base);
+            uint32_t mv = 0, mw = INF;
+            if (lane < cnt) { mv = av[base + lane]; mw = aw[base + lane]; }
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const uint32_t v = (uint32_t)__shfl((int)mv, (int)j, KL);
+                const uint32_t w = (uint32_t)__shfl((int)mw, (int)j, KL);
+                if (d != INF) atomicMin(&L[v * KL + lane], sat_add(d, w));
+            }
+        }
+    }
+}
+
+// Min over the arcs [a0, a1) of one target of L[tail] + w (UP arcs leaving a
+// segment); arc reads as in relax_arcs.
+template <int KL>
+__device__ __forceinline__ uint32_t reduce_arcs(const uint32_t *L, uint32_t a0, uint32_t a1,
+                                                const uint16_t *__restrict__ ax, const uint32_t *__restrict__ aw,
+                                                uint32_t lane) {
+    uint32_t m = INF;
+    if constexpr (KL >= 64) {
+        for (uint32_t base = a0; base < a1; base += 64) {
+            const uint32_t cnt = min(64u, a1 - base);
+            const uint32_t wl = __lane_id();
+            uint32_t mx = 0, mw = INF;
+            if (wl < cnt) { mx = ax[base + wl]; mw = aw[base + wl]; }
+#pragma unroll 4
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)mx, (int)j);
+                const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)mw, (int)j);
+                m = min(m, sat_add(L[x * KL + lane], w));
+            }
+        }
+    } else {
+        for (uint32_t base = a0; base < a1; base += KL) {
+            const uint32_t cnt = min((uint32_t)KL, a1 - base);
+            uint32_t mx = 0, mw = INF;
+            if (lane < cnt) { mx = ax[base + lane]; mw = aw[base + lane]; }
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const uint32_t x = (uint32_t)__shfl((int)mx, (int)j, KL);
+                const uint32_t w = (uint32_t)__shfl((int)mw, (int)j, KL);
+                m = min(m, sat_add(L[x * KL + lane], w));
+            }
+        }
+    }
+    return m;
+}
+
+// Broadcast of lane j of the slot to every lane of the slot.
+template <int KL>
+__device__ __forceinline__ uint32_t bcast(uint32_t x, uint32_t j) {
+    if constexpr (KL >= 64) return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)j);
+    return (uint32_t)__shfl((int)x, (int)j, KL);
+}
+
+// Intra-segment relaxation, node by node, over the LDS labels L[len][KL]:
+// DOWN runs first to last over arcs to later nodes, UP last to first over
+// arcs to earlier nodes.  soff (LDS) holds the tails' arc offsets.  A step's
+// arcs are split evenly over the NS slots; each slot fetches its share (at
+// most KL arcs, since a node has fewer than SEG <= NS*KL intra arcs) with one
+// coalesced load and applies it arc by arc; the next step's share is fetched
+// before the current one is applied.  One barrier per step.
+template <int KL, int SEG, int NTH, bool DOWN>
+__device__ __forceinline__ void chain_relax(uint32_t *L, const uint32_t *soff, uint32_t len,
+                                            const uint16_t *__restrict__ at, const uint32_t *__restrict__ aw,
+                                            uint32_t slot, uint32_t lane) {
+    constexpr int NS = NTH / KL;
+    static_assert(SEG <= NS * KL, "a step's share of arcs fits one load per slot");
+    auto fetch = [&](uint32_t step, uint32_t &t, uint32_t &w, uint32_t &cnt) {
+        t = 0;
+        w = INF;
+        cnt = 0;
+        if (step >= len) return;
+        const uint32_t i = DOWN ? step : len - 1 - step;
+        const uint32_t a0 = soff[i], a1 = soff[i + 1];
+        const uint32_t per = (a1 - a0 + NS - 1) / NS;
+        const uint32_t c0 = a0 + slot * per;
+        cnt = c0 < a1 ? min(per, a1 - c0) : 0u;
+        if (lane < cnt) {
+            t = at[c0 + lane];
+            w = aw[c0 + lane];
+        }
+    };
+    uint32_t t0, w0, n0, t1, w1, n1;
+    fetch(0, t0, w0, n0);
+    for (uint32_t step = 0; step < len; ++step) {
+        fetch(step + 1, t1, w1, n1);
+        const uint32_t i = DOWN ? step : len - 1 - step;
+        const uint32_t d = L[i * KL + lane];
+        for (uint32_t j = 0; j < n0; ++j) {
+            const uint32_t v = bcast<KL>(t0, j), w = bcast<KL>(w0, j);
+            if (d != INF) L[v * KL + lane] = min(L[v * KL + lane], sat_add(d, w));
+        }
+        __syncthreads();
+        t0 = t1;
+        w0 = w1;
+        n0 = n1;
     }
 }
 
 // ---------------------------------------------------------------------------
-// K2: pull sweep for the tasks of one level.  grid = (tasks, lane groups).
+// up_seg: segments [seg0, seg0 + gridDim.x) of one segment level, group
+// blockIdx.y.  The segment's labels (pushed from below) go to LDS; upward
+// relaxation inside the segment runs from the last node (lowest rank) to the
+// first; then every UP arc leaving the segment is reduced per target in LDS
+// and pushed with one atomic min per (target, lane).
 // ---------------------------------------------------------------------------
-template <int KL>
-__global__ __launch_bounds__(256) void sweep_kernel(
-    uint32_t task_base, const uint32_t *__restrict__ task_step, const uint32_t *__restrict__ step_pos,
-    const uint32_t *__restrict__ in_off, const uint32_t *__restrict__ in_src,
-    const uint32_t *__restrict__ in_w, const uint32_t *__restrict__ pos_pre,
-    const uint32_t *__restrict__ pos_sub, const uint32_t *__restrict__ pos_depth,
-    const uint32_t *__restrict__ seed_off, const uint32_t *__restrict__ seed_pre,
-    const uint32_t *__restrict__ seed_depth, const uint32_t *__restrict__ pathbuf, uint32_t D1,
-    uint32_t n_lanes, uint32_t *__restrict__ arena, uint32_t n) {
-    constexpr int NT = 256, NS = NT / KL, C = 8192 / KL;
-    __shared__ uint32_t acc[C * KL];
-    const uint32_t task = task_base + blockIdx.x;
-    const uint32_t g = blockIdx.y;
-    const uint32_t lane = threadIdx.x % KL, slot = threadIdx.x / KL;
-    const uint32_t gl = g * KL + lane;
+template <int KL, int SEG, int NTH>
+__global__ __launch_bounds__(NTH) void up_seg_kernel(
+    uint32_t seg0, int push_out, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ su_off,
+    const uint16_t *__restrict__ su_tgt, const uint32_t *__restrict__ su_w, const uint32_t *__restrict__ ut_off, const uint32_t *__restrict__ ut_tgt,
+    const uint32_t *__restrict__ ut_aoff, const uint16_t *__restrict__ uta_x, const uint32_t *__restrict__ uta_w,
+    uint32_t *__restrict__ arena, uint32_t n) {
+    constexpr int NS = NTH / KL;
+    __shared__ uint32_t L[SEG * KL];
+    __shared__ uint32_t soff[SEG + 1];
+    const uint32_t s = seg0 + blockIdx.x, g = blockIdx.y;
+    const uint32_t lane = threadIdx.x % KL, slot = slot_id<KL>();
+    const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
     uint32_t *lab = arena + (size_t)g * n * KL;
-    uint32_t sj0 = 0, sj1 = 0;
-    if (gl < n_lanes) { sj0 = seed_off[gl]; sj1 = seed_off[gl + 1]; }
-    const uint32_t s_end = task_step[task + 1];
-    for (uint32_t s = task_step[task]; s < s_end; ++s) {
-        const uint32_t p0 = step_pos[s], p1 = step_pos[s + 1];
-        for (uint32_t c0 = p0; c0 < p1; c0 += C) {
-            const uint32_t c1 = min(p1, c0 + (uint32_t)C);
-            for (uint32_t i = slot; i < c1 - c0; i += NS) {
-                const uint32_t p = c0 + i;
-                uint32_t v = INF;
-                if (sj0 < sj1) {
-                    const uint32_t pr = pos_pre[p], ps = pos_sub[p], pd = pos_depth[p];
-                    for (uint32_t j = sj0; j < sj1; ++j) {
-                        const uint32_t sp = seed_pre[j];
-                        if (sp >= pr && sp - pr < ps) v = min(v, pathbuf[(size_t)j * D1 + (seed_depth[j] - pd)]);
-                    }
-                }
-                acc[i * KL + lane] = v;
+    for (uint32_t i = threadIdx.x; i <= len; i += NTH) soff[i] = su_off[p0 + i];
+    int any = 0;
+    for (uint32_t i = slot; i < len; i += NS) {
+        const uint32_t v = lab[(size_t)(p0 + i) * KL + lane];
+        L[i * KL + lane] = v;
+        any |= (v != INF);
+    }
+    if (!__syncthreads_or(any)) return;  // no lane of this group reaches the segment
+    chain_relax<KL, SEG, NTH, false>(L, soff, len, su_tgt, su_w, slot, lane);
+    for (uint32_t i = slot; i < len; i += NS) lab[(size_t)(p0 + i) * KL + lane] = L[i * KL + lane];
+    if (!push_out) return;
+    const uint32_t k0 = ut_off[s], k1 = ut_off[s + 1];
+    if constexpr (KL >= 64) {
+        const uint32_t wl = __lane_id();
+        const uint32_t mine = k1 > k0 + slot ? (k1 - k0 - slot + NS - 1) / NS : 0u;
+        for (uint32_t mb = 0; mb < mine; mb += 64) {
+            const uint32_t cntk = min(64u, mine - mb);
+            uint32_t mt = 0, ma0 = 0, ma1 = 0;
+            if (wl < cntk) {
+                const uint32_t k = k0 + slot + (mb + wl) * NS;
+                mt = ut_tgt[k];
+                ma0 = ut_aoff[k];
+                ma1 = ut_aoff[k + 1];
             }
-            __syncthreads();
-            const uint32_t A0 = in_off[c0], A1 = in_off[c1];
-            const uint32_t per = (A1 - A0 + NS - 1) / NS;
-            uint32_t a = A0 + slot * per;
-            const uint32_t ae = min(A1, a + per);
-            if (a < ae) {
-                uint32_t lo = c0, hi = c1;  // in_off[lo] <= a < in_off[hi]
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (in_off[mid] <= a) lo = mid; else hi = mid;
-                }
-                uint32_t node = lo, nend = in_off[node + 1];
-                uint32_t m = INF;
-                for (; a < ae; ++a) {
-                    while (a >= nend) {
-                        if (m != INF) atomicMin(&acc[(node - c0) * KL + lane], m);
-                        m = INF;
-                        ++node;
-                        nend = in_off[node + 1];
-                    }
-                    const uint32_t u = in_src[a];
-                    const uint32_t du = lab[(size_t)u * KL + lane];
-                    m = min(m, sat_add(du, in_w[a]));
-                }
-                if (m != INF) atomicMin(&acc[(node - c0) * KL + lane], m);
+            for (uint32_t j = 0; j < cntk; ++j) {
+                const uint32_t a0 = (uint32_t)__builtin_amdgcn_readlane((int)ma0, (int)j);
+                const uint32_t a1 = (uint32_t)__builtin_amdgcn_readlane((int)ma1, (int)j);
+                const uint32_t y = (uint32_t)__builtin_amdgcn_readlane((int)mt, (int)j);
+                const uint32_t m = reduce_arcs<KL>(L, a0, a1, uta_x, uta_w, lane);
+                if (m != INF) atomicMin(lab + (size_t)y * KL + lane, m);
             }
-            __syncthreads();
-            for (uint32_t i = slot; i < c1 - c0; i += NS) lab[(size_t)(c0 + i) * KL + lane] = acc[i * KL + lane];
-            __syncthreads();
+        }
+    } else {
+        for (uint32_t k = k0 + slot; k < k1; k += NS) {
+            const uint32_t m = reduce_arcs<KL>(L, ut_aoff[k], ut_aoff[k + 1], uta_x, uta_w, lane);
+            if (m != INF) atomicMin(lab + (size_t)ut_tgt[k] * KL + lane, m);
+        }
+    }
+}
+
+// Relaxes the grouped arcs of sources [k0, k1) into the LDS labels: sources
+// go in batches of UNR per slot — ids, arc ranges and labels of a batch are
+// loaded before any is relaxed, so UNR label reads are in flight per slot.
+template <int KL, int NS>
+__device__ __forceinline__ void relax_sources(uint32_t *L, const uint32_t *__restrict__ lab, uint32_t k0, uint32_t k1,
+                                              const uint32_t *__restrict__ src, const uint32_t *__restrict__ aoff,
+                                              const uint16_t *__restrict__ av, const uint32_t *__restrict__ aw,
+                                              uint32_t slot, uint32_t lane) {
+    constexpr int UNR = 8;
+    if constexpr (KL >= 64) {
+        // wavefront `slot` takes sources k0 + slot + NS*m (round robin); the
+        // ids and arc ranges of 64 of them arrive in one vector load and are
+        // read lane by lane
+        const uint32_t wl = __lane_id();
+        const uint32_t mine = k1 > k0 + slot ? (k1 - k0 - slot + NS - 1) / NS : 0u;
+        for (uint32_t mb = 0; mb < mine; mb += 64) {
+            const uint32_t cntk = min(64u, mine - mb);
+            uint32_t ms = 0, ma0 = 0, ma1 = 0;
+            if (wl < cntk) {
+                const uint32_t k = k0 + slot + (mb + wl) * NS;
+                ms = src[k];
+                ma0 = aoff[k];
+                ma1 = aoff[k + 1];
+            }
+            for (uint32_t j0 = 0; j0 < cntk; j0 += UNR) {
+                uint32_t d[UNR];
+#pragma unroll
+                for (int r = 0; r < UNR; ++r) {
+                    const uint32_t u = (uint32_t)__builtin_amdgcn_readlane((int)ms, (int)(j0 + r));
+                    d[r] = (j0 + r < cntk) ? lab[(size_t)u * KL + lane] : INF;
+                }
+#pragma unroll
+                for (int r = 0; r < UNR; ++r) {
+                    if (j0 + r >= cntk) break;
+                    const uint32_t a0 = (uint32_t)__builtin_amdgcn_readlane((int)ma0, (int)(j0 + r));
+                    const uint32_t a1 = (uint32_t)__builtin_amdgcn_readlane((int)ma1, (int)(j0 + r));
+                    relax_arcs<KL>(L, d[r], a0, a1, av, aw, lane);
+                }
+            }
+        }
+    } else {
+        // sources in batches of UNR per slot: ids, arc ranges and labels of a
+        // batch are loaded before any is relaxed
+        for (uint32_t kb = k0 + slot; kb < k1; kb += UNR * NS) {
+            uint32_t u[UNR], a0[UNR], a1[UNR], d[UNR];
+#pragma unroll
+            for (int r = 0; r < UNR; ++r) {
+                const uint32_t k = kb + r * NS;
+                const bool ok = k < k1;
+                u[r] = ok ? src[k] : 0;
+                a0[r] = ok ? aoff[k] : 0;
+                a1[r] = ok ? aoff[k + 1] : 0;
+            }
+#pragma unroll
+            for (int r = 0; r < UNR; ++r) d[r] = a1[r] > a0[r] ? lab[(size_t)u[r] * KL + lane] : INF;
+#pragma unroll
+            for (int r = 0; r < UNR; ++r) relax_arcs<KL>(L, d[r], a0[r], a1[r], av, aw, lane);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// K4: gather target columns of one batch into row-major rows, with the lane
-// limit mask.  Tile = 64 targets x KL lanes, transposed through LDS.
-// out[(row0 + lane) * ld + t] for lanes < n_lanes.  Optional reach bits:
-// bits[(row0+lane) * bits_ld + t/64] (t0 multiple of 64).
+// down_seg: segments [seg0, seg0 + gridDim.x) of one segment level, group
+// blockIdx.y.  Labels in LDS seeded with the upward label; DOWN arcs from
+// outside the segment grouped by source (each source label read once per
+// segment); then the intra-segment arcs, node by node.
+// ---------------------------------------------------------------------------
+template <int KL, int SEG, int NTH>
+__global__ __launch_bounds__(NTH) void down_seg_kernel(
+    uint32_t seg0, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ bs_off,
+    const uint32_t *__restrict__ bs_src, const uint32_t *__restrict__ bs_aoff, const uint16_t *__restrict__ ba_v,
+    const uint32_t *__restrict__ ba_w, const uint32_t *__restrict__ sd_off, const uint16_t *__restrict__ sd_tgt,
+    const uint32_t *__restrict__ sd_w, uint32_t *__restrict__ arena, uint32_t n) {
+    constexpr int NS = NTH / KL;
+    __shared__ uint32_t L[SEG * KL];
+    __shared__ uint32_t soff[SEG + 1];
+    const uint32_t s = seg0 + blockIdx.x, g = blockIdx.y;
+    const uint32_t lane = threadIdx.x % KL, slot = slot_id<KL>();
+    const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
+    uint32_t *lab = arena + (size_t)g * n * KL;
+    for (uint32_t i = threadIdx.x; i <= len; i += NTH) soff[i] = sd_off[p0 + i];
+    for (uint32_t i = slot; i < len; i += NS) L[i * KL + lane] = lab[(size_t)(p0 + i) * KL + lane];
+    __syncthreads();
+    relax_sources<KL, NS>(L, lab, bs_off[s], bs_off[s + 1], bs_src, bs_aoff, ba_v, ba_w, slot, lane);
+    __syncthreads();
+    chain_relax<KL, SEG, NTH, true>(L, soff, len, sd_tgt, sd_w, slot, lane);
+    for (uint32_t i = slot; i < len; i += NS) lab[(size_t)(p0 + i) * KL + lane] = L[i * KL + lane];
+}
+
+// ---------------------------------------------------------------------------
+// bottom_chunk: all bottom chunks in one launch, group blockIdx.y.  A chunk
+// holds bottom nodes of neighbouring subtrees (pre-order), which share their
+// top ancestors: labels in LDS seeded from the lane's bottom path slots, then
+// the DOWN arcs from top sources grouped by source.  Results go to the arena;
+// arcs between bottom nodes follow in bottom_pull.
+// ---------------------------------------------------------------------------
+template <int KL, int BB, int NTH>
+__global__ __launch_bounds__(NTH) void bottom_chunk_kernel(
+    const uint32_t *__restrict__ ch_off, const uint32_t *__restrict__ ch_pos, const uint32_t *__restrict__ cs_off,
+    const uint32_t *__restrict__ cs_src, const uint32_t *__restrict__ cs_aoff, const uint16_t *__restrict__ ca_v,
+    const uint32_t *__restrict__ ca_w, const uint32_t *__restrict__ pos_pre, const uint32_t *__restrict__ pos_sub,
+    const uint32_t *__restrict__ pos_depth, const uint32_t *__restrict__ seed_off,
+    const uint32_t *__restrict__ seed_pre, const uint32_t *__restrict__ seed_depth,
+    const uint32_t *__restrict__ pathbuf, uint32_t D1, uint32_t n_lanes, uint32_t *__restrict__ arena, uint32_t n) {
+    constexpr int NS = NTH / KL;
+    __shared__ uint32_t L[BB * KL];
+    const uint32_t c = blockIdx.x, g = blockIdx.y;
+    const uint32_t lane = threadIdx.x % KL, slot = slot_id<KL>();
+    const uint32_t i0 = ch_off[c], len = ch_off[c + 1] - i0;
+    uint32_t *lab = arena + (size_t)g * n * KL;
+    uint32_t sj0 = 0, sj1 = 0;
+    const uint32_t gl = g * KL + lane;
+    if (gl < n_lanes) { sj0 = seed_off[gl]; sj1 = seed_off[gl + 1]; }
+    for (uint32_t i = slot; i < len; i += NS) {
+        uint32_t v = INF;
+        if (sj0 < sj1) {
+            const uint32_t p = ch_pos[i0 + i];
+            const uint32_t pr = pos_pre[p], ps = pos_sub[p], pd = pos_depth[p];
+            for (uint32_t j = sj0; j < sj1; ++j) {
+                const uint32_t sp = seed_pre[j];
+                if (sp >= pr && sp - pr < ps) v = min(v, pathbuf[(size_t)j * D1 + (seed_depth[j] - pd)]);
+            }
+        }
+        L[i * KL + lane] = v;
+    }
+    __syncthreads();
+    relax_sources<KL, NS>(L, lab, cs_off[c], cs_off[c + 1], cs_src, cs_aoff, ca_v, ca_w, slot, lane);
+    __syncthreads();
+    for (uint32_t i = slot; i < len; i += NS) lab[(size_t)ch_pos[i0 + i] * KL + lane] = L[i * KL + lane];
+}
+
+// bottom_pull: bottom nodes [p0, p1) of one height level, group blockIdx.y:
+// the chunk result min the DOWN arcs from bottom sources (higher levels).
+template <int KL>
+__global__ __launch_bounds__(NT) void bottom_pull_kernel(uint32_t p0, uint32_t p1, const uint32_t *__restrict__ in_off,
+                                                         const uint32_t *__restrict__ in_split,
+                                                         const uint32_t *__restrict__ in_src,
+                                                         const uint32_t *__restrict__ in_w,
+                                                         uint32_t *__restrict__ arena, uint32_t n) {
+    constexpr int NS = NT / KL;
+    const uint32_t lane = threadIdx.x % KL, slot = slot_id<KL>();
+    const uint32_t p = p0 + blockIdx.x * NS + slot;
+    if (p >= p1) return;
+    uint32_t *lab = arena + (size_t)blockIdx.y * n * KL + lane;
+    uint32_t a = in_off[p];
+    const uint32_t a1 = in_split[p];
+    if (a == a1) return;
+    if constexpr (KL >= 64) {
+        uint32_t m = lab[(size_t)p * KL];
+        const uint32_t wl = __lane_id();
+        for (uint32_t base = a; base < a1; base += 64) {
+            const uint32_t cnt = min(64u, a1 - base);
+            uint32_t ms = 0, mw = INF;
+            if (wl < cnt) { ms = in_src[base + wl]; mw = in_w[base + wl]; }
+#pragma unroll 8
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const uint32_t u = (uint32_t)__builtin_amdgcn_readlane((int)ms, (int)j);
+                const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)mw, (int)j);
+                m = min(m, sat_add(lab[(size_t)u * KL], w));
+            }
+        }
+        lab[(size_t)p * KL] = m;
+        return;
+    }
+    uint32_t m0 = lab[(size_t)p * KL], m1 = INF, m2 = INF, m3 = INF;
+    for (; a + 4 <= a1; a += 4) {
+        const uint32_t u0 = in_src[a], u1 = in_src[a + 1], u2 = in_src[a + 2], u3 = in_src[a + 3];
+        const uint32_t w0 = in_w[a], w1 = in_w[a + 1], w2 = in_w[a + 2], w3 = in_w[a + 3];
+        m0 = min(m0, sat_add(lab[(size_t)u0 * KL], w0));
+        m1 = min(m1, sat_add(lab[(size_t)u1 * KL], w1));
+        m2 = min(m2, sat_add(lab[(size_t)u2 * KL], w2));
+        m3 = min(m3, sat_add(lab[(size_t)u3 * KL], w3));
+    }
+    for (; a < a1; ++a) m0 = min(m0, sat_add(lab[(size_t)in_src[a] * KL], in_w[a]));
+    lab[(size_t)p * KL] = min(min(m0, m1), min(m2, m3));
+}
+
+// ---------------------------------------------------------------------------
+// gather: target columns of one batch into rows, limit-masked.  Tile = 64
+// targets x KL lanes through LDS.  Row of lane gl: row_map[gl] if given, else
+// gl.  Optional reach bits per row (t0 is a multiple of 64).
 // ---------------------------------------------------------------------------
 template <int KL>
-__global__ __launch_bounds__(256) void gather_kernel(
+__global__ __launch_bounds__(NT) void gather_kernel(
     const uint32_t *__restrict__ arena, uint32_t n, const uint32_t *__restrict__ tgt_pos, uint32_t T,
-    const uint32_t *__restrict__ lane_limit, uint32_t n_lanes, uint32_t *__restrict__ out, uint64_t ld,
-    uint64_t row0, unsigned long long *__restrict__ bits, uint64_t bits_ld) {
+    const uint32_t *__restrict__ lane_limit, uint32_t n_lanes, const uint32_t *__restrict__ row_map,
+    uint32_t *__restrict__ out, uint64_t ld, unsigned long long *__restrict__ bits, uint64_t bits_ld) {
     __shared__ uint32_t tile[64][KL + 1];
     const uint32_t t0 = blockIdx.x * 64;
     const uint32_t g = blockIdx.y;
     const uint32_t *lab = arena + (size_t)g * n * KL;
-    for (uint32_t idx = threadIdx.x; idx < 64 * KL; idx += 256) {
+    for (uint32_t idx = threadIdx.x; idx < 64 * KL; idx += NT) {
         const uint32_t ti = idx / KL, lane = idx % KL;
         const uint32_t t = t0 + ti;
         uint32_t v = INF;
@@ -226,19 +574,25 @@ __global__ __launch_bounds__(256) void gather_kernel(
     }
     __syncthreads();
     if (out) {
-        for (uint32_t idx = threadIdx.x; idx < 64 * KL; idx += 256) {
+        for (uint32_t idx = threadIdx.x; idx < 64 * KL; idx += NT) {
             const uint32_t lane = idx / 64, ti = idx % 64;
             const uint32_t gl = g * KL + lane, t = t0 + ti;
-            if (gl < n_lanes && t < T) out[(row0 + gl) * ld + t] = tile[ti][lane];
+            if (gl < n_lanes && t < T) {
+                const uint64_t row = row_map ? row_map[gl] : gl;
+                out[row * ld + t] = tile[ti][lane];
+            }
         }
     }
     if (bits) {
         const uint32_t w = threadIdx.x / 64, ti = threadIdx.x % 64;
-        for (uint32_t lane = w; lane < KL; lane += 4) {
+        for (uint32_t lane = w; lane < KL; lane += NT / 64) {
             const uint32_t gl = g * KL + lane;
             const bool r = (t0 + ti < T) && tile[ti][lane] != INF;
             const unsigned long long word = __ballot(r);
-            if (ti == 0 && gl < n_lanes) bits[(row0 + gl) * bits_ld + (t0 / 64)] = word;
+            if (ti == 0 && gl < n_lanes) {
+                const uint64_t row = row_map ? row_map[gl] : gl;
+                bits[row * bits_ld + (t0 / 64)] = word;
+            }
         }
     }
 }
@@ -274,6 +628,8 @@ struct DevBuf {
     }
 };
 
+constexpr int SEG = 256;  // segment limit; LDS = SEG * KL * 4 bytes per chain workgroup
+
 }  // namespace
 
 struct bfg_ctx {
@@ -281,16 +637,20 @@ struct bfg_ctx {
     hipStream_t stream = nullptr;
     std::mutex mu;
     bfg::Layout L;
-    uint32_t lanes = 32;        // KL
+    uint32_t lanes = 32;        // KL: sources per lane group (16, 32 or 64)
     uint32_t lane_cap = 0;      // lanes per batch (multiple of KL)
+    bool cap_fixed = false;
     // resident graph
-    DevBuf<uint32_t> up_off, up_tgt, up_w, parent, depth;
-    DevBuf<uint32_t> in_off, in_src, in_w, pos_pre, pos_sub, pos_depth;
-    DevBuf<uint32_t> step_pos, task_step, pos_of_rank;
+    DevBuf<uint32_t> parent, pos_parent, pos_pre, pos_sub, pos_depth, seg_pos;
+    DevBuf<uint32_t> bs_off, bs_src, bs_aoff, ba_w, sd_off, sd_w, su_off, su_w, in_off, in_split, in_src, in_w;
+    DevBuf<uint16_t> sd_tgt, su_tgt;
+    DevBuf<uint32_t> ch_off, ch_pos, cs_off, cs_src, cs_aoff, ca_w;
+    DevBuf<uint16_t> ca_v;
+    DevBuf<uint32_t> up_off, up_tgt, up_w, ut_off, ut_tgt, ut_aoff, uta_w;
+    DevBuf<uint16_t> ba_v, uta_x;
     // arenas (per batch)
     DevBuf<uint32_t> arena, pathbuf;
-    size_t pathbuf_cap = 0;
-    DevBuf<uint32_t> seed_off, seed_rank, seed_cost, seed_depth, seed_pre, lane_limit;
+    DevBuf<uint32_t> seed_off, seed_rank, seed_pos, seed_cost, seed_depth, seed_pre, lane_limit, row_map;
     size_t seeds_cap = 0, lanes_alloc = 0;
     DevBuf<uint32_t> tgt_pos;
     size_t tgt_cap = 0;
@@ -301,23 +661,23 @@ struct bfg_ctx {
 
 namespace {
 
+uint32_t d1_of(const bfg_ctx *c) { return c->L.max_height + 1; }
+
 int ensure_lane_buffers(bfg_ctx *c, size_t n_lanes, size_t n_seeds) {
     if (n_lanes > c->lanes_alloc) {
         HIP_TRY(c->seed_off.alloc(n_lanes + 1));
         HIP_TRY(c->lane_limit.alloc(n_lanes));
+        HIP_TRY(c->row_map.alloc(n_lanes));
         c->lanes_alloc = n_lanes;
     }
     if (n_seeds > c->seeds_cap) {
         HIP_TRY(c->seed_rank.alloc(n_seeds));
+        HIP_TRY(c->seed_pos.alloc(n_seeds));
         HIP_TRY(c->seed_cost.alloc(n_seeds));
         HIP_TRY(c->seed_depth.alloc(n_seeds));
         HIP_TRY(c->seed_pre.alloc(n_seeds));
+        HIP_TRY(c->pathbuf.alloc(n_seeds * (size_t)d1_of(c)));
         c->seeds_cap = n_seeds;
-    }
-    const size_t need_pb = n_seeds * (size_t)(c->L.max_depth + 1);
-    if (need_pb > c->pathbuf_cap) {
-        HIP_TRY(c->pathbuf.alloc(need_pb));
-        c->pathbuf_cap = need_pb;
     }
     return BFG_SUCCESS;
 }
@@ -334,68 +694,65 @@ int ensure_arena(bfg_ctx *c) {
 uint32_t auto_lane_cap(bfg_ctx *c, uint32_t want) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
-    size_t have = free_b + c->arena.n * sizeof(uint32_t);
-    size_t per_lane = (size_t)c->L.n * 4 + (size_t)(c->L.max_depth + 1) * 4;
-    size_t budget = (size_t)(have * 0.6);
+    const size_t have = free_b + c->arena.n * sizeof(uint32_t);
+    const size_t per_lane = (size_t)c->L.n * 4 + (size_t)d1_of(c) * 4 + 64;
+    const size_t budget = (size_t)(have * 0.6);
     size_t cap = std::max<size_t>(budget / std::max<size_t>(per_lane, 1), c->lanes);
-    cap = std::min<size_t>(cap, 8192);
+    cap = std::min<size_t>(cap, 16384);
     cap = std::min<size_t>(cap, ((size_t)want + c->lanes - 1) / c->lanes * c->lanes);
     cap = std::max<size_t>(cap / c->lanes * c->lanes, c->lanes);
     return (uint32_t)cap;
 }
 
-typedef void (*sweep_fn_t)(dim3, dim3, hipStream_t, bfg_ctx *, uint32_t, uint32_t);
-
 template <int KL>
-void launch_sweep(dim3 grid, dim3 block, hipStream_t s, bfg_ctx *c, uint32_t task_base, uint32_t n_lanes) {
-    hipLaunchKernelGGL(sweep_kernel<KL>, grid, block, 0, s, task_base, c->task_step.p, c->step_pos.p,
-                       c->in_off.p, c->in_src.p, c->in_w.p, c->pos_pre.p, c->pos_sub.p, c->pos_depth.p,
-                       c->seed_off.p, c->seed_pre.p, c->seed_depth.p, c->pathbuf.p, c->L.max_depth + 1,
-                       n_lanes, c->arena.p, c->L.n);
-}
-
-template <int KL>
-void launch_gather(dim3 grid, hipStream_t s, bfg_ctx *c, const uint32_t *tpos, uint32_t T, uint32_t n_lanes,
-                   uint32_t *out, uint64_t ld, uint64_t row0, unsigned long long *bits, uint64_t bits_ld) {
-    hipLaunchKernelGGL(gather_kernel<KL>, grid, dim3(256), 0, s, c->arena.p, c->L.n, tpos, T, c->lane_limit.p,
-                       n_lanes, out, ld, row0, bits, bits_ld);
-}
-
-// One batch: lanes [0, nl) with host seed lists and limits already staged in
-// host vectors; runs K1 + K2; output is done by the caller.
-int run_batch(bfg_ctx *c, uint32_t nl, const std::vector<uint32_t> &soff, const std::vector<uint32_t> &srank,
-              const std::vector<uint32_t> &scost, const std::vector<uint32_t> &limits, float *up_ms,
-              float *sweep_ms, uint32_t *launches) {
-    const size_t ns = srank.size();
-    int rc = ensure_lane_buffers(c, nl, ns);
-    if (rc) return rc;
-    std::vector<uint32_t> sdepth(ns), spre(ns);
-    for (size_t j = 0; j < ns; ++j) { sdepth[j] = c->L.depth[srank[j]]; spre[j] = c->L.pre[srank[j]]; }
+int launch_phases(bfg_ctx *c, uint32_t nl, float *up_ms, float *sweep_ms, uint32_t *launches) {
+    const bfg::Layout &L = c->L;
     hipStream_t s = c->stream;
-    HIP_TRY(hipMemcpyAsync(c->seed_off.p, soff.data(), (nl + 1) * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->lane_limit.p, limits.data(), nl * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->seed_rank.p, srank.data(), ns * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->seed_cost.p, scost.data(), ns * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->seed_depth.p, sdepth.data(), ns * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->seed_pre.p, spre.data(), ns * 4, hipMemcpyHostToDevice, s));
+    const uint32_t groups = (nl + KL - 1) / KL;
+    const uint32_t n = L.n, D1 = d1_of(c);
+    const uint32_t nsl = (uint32_t)L.seglvl_seg.size() - 1;
+    uint32_t nlaunch = 0;
     HIP_TRY(hipEventRecord(c->ev[0], s));
-    hipLaunchKernelGGL(up_walk_kernel, dim3(nl), dim3(64), 0, s, c->seed_off.p, c->seed_rank.p, c->seed_cost.p,
-                       c->seed_depth.p, c->lane_limit.p, c->up_off.p, c->up_tgt.p, c->up_w.p, c->parent.p,
-                       c->depth.p, c->pathbuf.p, c->L.max_depth + 1);
+    if (L.n_top) {
+        const uint64_t words = (uint64_t)L.n_top * KL;
+        const uint64_t total = words * groups;
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(fill_top_kernel, dim3(blocks), dim3(256), 0, s, c->arena.p, (uint64_t)n * KL, words,
+                           groups);
+    }
+    hipLaunchKernelGGL(bottom_walk_kernel<KL>, dim3(nl), dim3(64), 0, s, c->seed_off.p, c->seed_rank.p,
+                       c->seed_pos.p, c->seed_cost.p, c->seed_depth.p, c->lane_limit.p, c->parent.p,
+                       c->pos_parent.p, c->pos_depth.p, c->up_off.p, c->up_tgt.p, c->up_w.p, L.n_top, c->pathbuf.p,
+                       D1, c->arena.p, n);
+    for (int l = (int)nsl - 1; l >= 0; --l) {
+        const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
+        hipLaunchKernelGGL((up_seg_kernel<KL, SEG, NTB>), dim3(s1 - s0, groups), dim3(NTB), 0, s, s0,
+                           l > 0 ? 1 : 0, c->seg_pos.p, c->su_off.p, c->su_tgt.p, c->su_w.p, c->ut_off.p, c->ut_tgt.p,
+                           c->ut_aoff.p, c->uta_x.p, c->uta_w.p, c->arena.p, n);
+        ++nlaunch;
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[1], s));
-    const uint32_t groups = (nl + c->lanes - 1) / c->lanes;
-    const size_t nlev = c->L.level_task.size() - 1;
-    uint32_t nlaunch = 0;
-    for (size_t l = 0; l < nlev; ++l) {
-        const uint32_t t0 = c->L.level_task[l], t1 = c->L.level_task[l + 1];
-        if (t1 == t0) continue;
-        dim3 grid(t1 - t0, groups);
-        switch (c->lanes) {
-            case 16: launch_sweep<16>(grid, dim3(256), s, c, t0, nl); break;
-            case 64: launch_sweep<64>(grid, dim3(256), s, c, t0, nl); break;
-            default: launch_sweep<32>(grid, dim3(256), s, c, t0, nl); break;
-        }
+    for (uint32_t l = 0; l < nsl; ++l) {
+        const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
+        hipLaunchKernelGGL((down_seg_kernel<KL, SEG, NTB>), dim3(s1 - s0, groups), dim3(NTB), 0, s, s0, c->seg_pos.p,
+                           c->bs_off.p, c->bs_src.p, c->bs_aoff.p, c->ba_v.p, c->ba_w.p, c->sd_off.p, c->sd_tgt.p,
+                           c->sd_w.p, c->arena.p, n);
+        ++nlaunch;
+    }
+    const uint32_t nch = (uint32_t)L.ch_off.size() - 1;
+    if (nch) {
+        hipLaunchKernelGGL((bottom_chunk_kernel<KL, SEG, NTB>), dim3(nch, groups), dim3(NTB), 0, s, c->ch_off.p,
+                           c->ch_pos.p, c->cs_off.p, c->cs_src.p, c->cs_aoff.p, c->ca_v.p, c->ca_w.p, c->pos_pre.p,
+                           c->pos_sub.p, c->pos_depth.p, c->seed_off.p, c->seed_pre.p, c->seed_depth.p, c->pathbuf.p,
+                           D1, nl, c->arena.p, n);
+        ++nlaunch;
+    }
+    constexpr int NSP = NT / KL;
+    for (size_t b = 0; b + 1 < L.botlvl_pos.size(); ++b) {
+        const uint32_t p0 = L.botlvl_pos[b], p1 = L.botlvl_pos[b + 1];
+        hipLaunchKernelGGL(bottom_pull_kernel<KL>, dim3((p1 - p0 + NSP - 1) / NSP, groups), dim3(NT), 0, s, p0, p1,
+                           c->in_off.p, c->in_split.p, c->in_src.p, c->in_w.p, c->arena.p, n);
         ++nlaunch;
     }
     HIP_TRY(hipGetLastError());
@@ -410,16 +767,58 @@ int run_batch(bfg_ctx *c, uint32_t nl, const std::vector<uint32_t> &soff, const 
     return BFG_SUCCESS;
 }
 
-int gather_out(bfg_ctx *c, uint32_t nl, const uint32_t *d_tpos, uint32_t T, uint32_t *d_out, uint64_t ld, uint64_t row0,
-               unsigned long long *d_bits, uint64_t bits_ld, float *out_ms) {
+// One batch: lanes [0, nl) in sorted order, staged host seed lists; runs the
+// upward and downward phases; the output step is done by the caller.
+int run_batch(bfg_ctx *c, uint32_t nl, const std::vector<uint32_t> &soff, const std::vector<uint32_t> &srank,
+              const std::vector<uint32_t> &scost, const std::vector<uint32_t> &limits,
+              const std::vector<uint32_t> &rows, float *up_ms, float *sweep_ms, uint32_t *launches) {
+    const size_t ns = srank.size();
+    int rc = ensure_lane_buffers(c, nl, ns);
+    if (rc) return rc;
+    std::vector<uint32_t> sdepth(ns), spre(ns), spos(ns);
+    for (size_t j = 0; j < ns; ++j) {
+        sdepth[j] = c->L.depth[srank[j]];
+        spre[j] = c->L.pre[srank[j]];
+        spos[j] = c->L.pos_of_rank[srank[j]];
+    }
+    hipStream_t s = c->stream;
+    HIP_TRY(hipMemcpyAsync(c->seed_off.p, soff.data(), (nl + 1) * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->lane_limit.p, limits.data(), nl * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->row_map.p, rows.data(), nl * 4, hipMemcpyHostToDevice, s));
+    if (ns) {
+        HIP_TRY(hipMemcpyAsync(c->seed_rank.p, srank.data(), ns * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(c->seed_pos.p, spos.data(), ns * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(c->seed_cost.p, scost.data(), ns * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(c->seed_depth.p, sdepth.data(), ns * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(c->seed_pre.p, spre.data(), ns * 4, hipMemcpyHostToDevice, s));
+    }
+    switch (c->lanes) {
+        case 16: rc = launch_phases<16>(c, nl, up_ms, sweep_ms, launches); break;
+        case 64: rc = launch_phases<64>(c, nl, up_ms, sweep_ms, launches); break;
+        default: rc = launch_phases<32>(c, nl, up_ms, sweep_ms, launches); break;
+    }
+    return rc;  // launch_phases synchronised the stream: the staged vectors may go
+}
+
+int gather_out(bfg_ctx *c, uint32_t nl, const uint32_t *d_tpos, uint32_t T, const uint32_t *d_rows, uint32_t *d_out,
+               uint64_t ld, unsigned long long *d_bits, uint64_t bits_ld, float *out_ms) {
     hipStream_t s = c->stream;
     const uint32_t groups = (nl + c->lanes - 1) / c->lanes;
     dim3 grid((T + 63) / 64, groups);
     HIP_TRY(hipEventRecord(c->ev[0], s));
     switch (c->lanes) {
-        case 16: launch_gather<16>(grid, s, c, d_tpos, T, nl, d_out, ld, row0, d_bits, bits_ld); break;
-        case 64: launch_gather<64>(grid, s, c, d_tpos, T, nl, d_out, ld, row0, d_bits, bits_ld); break;
-        default: launch_gather<32>(grid, s, c, d_tpos, T, nl, d_out, ld, row0, d_bits, bits_ld); break;
+        case 16:
+            hipLaunchKernelGGL(gather_kernel<16>, grid, dim3(NT), 0, s, c->arena.p, c->L.n, d_tpos, T,
+                               c->lane_limit.p, nl, d_rows, d_out, ld, d_bits, bits_ld);
+            break;
+        case 64:
+            hipLaunchKernelGGL(gather_kernel<64>, grid, dim3(NT), 0, s, c->arena.p, c->L.n, d_tpos, T,
+                               c->lane_limit.p, nl, d_rows, d_out, ld, d_bits, bits_ld);
+            break;
+        default:
+            hipLaunchKernelGGL(gather_kernel<32>, grid, dim3(NT), 0, s, c->arena.p, c->L.n, d_tpos, T,
+                               c->lane_limit.p, nl, d_rows, d_out, ld, d_bits, bits_ld);
+            break;
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[1], s));
@@ -431,7 +830,7 @@ int gather_out(bfg_ctx *c, uint32_t nl, const uint32_t *d_tpos, uint32_t T, uint
 }
 
 struct Req {
-    // per lane seeds and limit (host)
+    // per lane seeds and limit (host), original lane order
     std::vector<uint32_t> soff, srank, scost, limit;
 };
 
@@ -441,31 +840,41 @@ int check_ranks(const bfg_ctx *c, const uint32_t *r, size_t k, const char *what)
     return BFG_SUCCESS;
 }
 
-// Runs every lane of `req` in batches of lane_cap; after each batch, the
-// callback `out(batch_first_lane, batch_lanes)` performs the output step.
+// Runs every lane of `req`: lanes are ordered by the pre-order of their first
+// seed and cut into batches of lane_cap; after each batch,
+// out(batch_first, batch_lanes, rows, &ms) writes the outputs, where rows[i]
+// is the original index of sorted lane batch_first + i.
 template <class F>
 int run_all(bfg_ctx *c, const Req &req, F &&out) {
     const uint32_t L = (uint32_t)req.limit.size();
-    if (c->lane_cap == 0) c->lane_cap = auto_lane_cap(c, L);
+    if (!c->cap_fixed) c->lane_cap = auto_lane_cap(c, L);
     int rc = ensure_arena(c);
     if (rc) return rc;
+    std::vector<uint32_t> order(L);
+    std::iota(order.begin(), order.end(), 0);
+    std::vector<uint32_t> key(L, 0);
+    for (uint32_t i = 0; i < L; ++i)
+        if (req.soff[i + 1] > req.soff[i]) key[i] = c->L.pre[req.srank[req.soff[i]]];
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
     auto t_start = std::chrono::steady_clock::now();
     bfg_timings tm{};
     for (uint32_t b0 = 0; b0 < L; b0 += c->lane_cap) {
         const uint32_t nl = std::min(c->lane_cap, L - b0);
-        std::vector<uint32_t> soff(nl + 1), srank, scost, lim(nl);
+        std::vector<uint32_t> soff(nl + 1), srank, scost, lim(nl), rows(nl);
         for (uint32_t i = 0; i < nl; ++i) {
+            const uint32_t o = order[b0 + i];
+            rows[i] = o;
             soff[i] = (uint32_t)srank.size();
-            for (uint32_t k = req.soff[b0 + i]; k < req.soff[b0 + i + 1]; ++k) {
+            for (uint32_t k = req.soff[o]; k < req.soff[o + 1]; ++k) {
                 srank.push_back(req.srank[k]);
                 scost.push_back(req.scost[k]);
             }
-            lim[i] = req.limit[b0 + i];
+            lim[i] = req.limit[o];
         }
         soff[nl] = (uint32_t)srank.size();
-        rc = run_batch(c, nl, soff, srank, scost, lim, &tm.up_ms, &tm.sweep_ms, &tm.sweep_launches);
+        rc = run_batch(c, nl, soff, srank, scost, lim, rows, &tm.up_ms, &tm.sweep_ms, &tm.sweep_launches);
         if (rc) return rc;
-        rc = out(b0, nl, &tm.output_ms);
+        rc = out(b0, nl, rows, &tm.output_ms);
         if (rc) return rc;
         tm.batches++;
     }
@@ -497,29 +906,63 @@ int stage_targets(bfg_ctx *c, const uint32_t *targets, uint32_t T) {
     return BFG_SUCCESS;
 }
 
-// Dense rows for all ranks of a batch (dist and/or bits) to host buffers.
+// Full rows for all ranks (dist and/or reach bits) into host buffers indexed
+// by original lane.
 int dense_rows(bfg_ctx *c, const Req &req, uint32_t *dist_host, uint64_t *bits_host) {
     const uint32_t n = c->L.n;
     const uint64_t words = (n + 63) / 64;
-    DevBuf<uint32_t> d_rows;
+    DevBuf<uint32_t> d_rows, d_all_pos;
     DevBuf<unsigned long long> d_bits;
+    std::vector<uint32_t> all_pos(c->L.pos_of_rank);
+    HIP_TRY(d_all_pos.upload(all_pos, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<uint32_t> hrow;
+    std::vector<uint64_t> hbits;
     uint32_t cap = 0;
-    return run_all(c, req, [&](uint32_t b0, uint32_t nl, float *oms) -> int {
+    return run_all(c, req, [&](uint32_t b0, uint32_t nl, const std::vector<uint32_t> &rows, float *oms) -> int {
+        (void)b0;
         if (cap < nl) {
             if (dist_host) HIP_TRY(d_rows.alloc((size_t)nl * n));
             if (bits_host) HIP_TRY(d_bits.alloc((size_t)nl * words));
             cap = nl;
         }
-        int rc = gather_out(c, nl, c->pos_of_rank.p, n, dist_host ? d_rows.p : nullptr, n, 0,
+        int rc = gather_out(c, nl, d_all_pos.p, n, nullptr, dist_host ? d_rows.p : nullptr, n,
                             bits_host ? d_bits.p : nullptr, words, oms);
         if (rc) return rc;
-        if (dist_host)
-            HIP_TRY(hipMemcpy(dist_host + (size_t)b0 * n, d_rows.p, (size_t)nl * n * 4, hipMemcpyDeviceToHost));
-        if (bits_host)
-            HIP_TRY(hipMemcpy(bits_host + (size_t)b0 * words, d_bits.p, (size_t)nl * words * 8,
-                              hipMemcpyDeviceToHost));
+        if (dist_host) {
+            hrow.resize((size_t)nl * n);
+            HIP_TRY(hipMemcpy(hrow.data(), d_rows.p, (size_t)nl * n * 4, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < nl; ++i)
+                memcpy(dist_host + (size_t)rows[i] * n, hrow.data() + (size_t)i * n, (size_t)n * 4);
+        }
+        if (bits_host) {
+            hbits.resize((size_t)nl * words);
+            HIP_TRY(hipMemcpy(hbits.data(), d_bits.p, (size_t)nl * words * 8, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < nl; ++i)
+                memcpy(bits_host + (size_t)rows[i] * words, hbits.data() + (size_t)i * words, (size_t)words * 8);
+        }
         return BFG_SUCCESS;
     });
+}
+
+void fill_info(const bfg::Layout &L, bfg_ctx_info *info) {
+    memset(info, 0, sizeof(*info));
+    info->n_nodes = L.n;
+    info->m_up = L.m_up;
+    info->m_down = L.m_down;
+    info->etree_depth = L.max_depth;
+    info->n_top = L.n_top;
+    info->n_segments = (uint32_t)L.seg_pos.size() - 1;
+    info->n_segment_levels = (uint32_t)L.seglvl_seg.size() - 1;
+    info->n_bottom_levels = (uint32_t)L.botlvl_pos.size() - 1;
+}
+
+bfg::CchInput input_of(const bfg_graph_desc *gd) {
+    bfg::CchInput in;
+    in.n = gd->n_nodes;
+    in.up_off = gd->up_offsets; in.up_tgt = gd->up_targets; in.up_w = gd->up_weights;
+    in.dn_off = gd->down_offsets; in.dn_tgt = gd->down_targets; in.dn_w = gd->down_weights;
+    return in;
 }
 
 }  // namespace
@@ -537,42 +980,66 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     std::unique_ptr<bfg_ctx> c(new bfg_ctx());
     c->device = device;
     HIP_TRY(hipSetDevice(device));
-    bfg::CchInput in;
-    in.n = gd->n_nodes;
-    in.up_off = gd->up_offsets; in.up_tgt = gd->up_targets; in.up_w = gd->up_weights;
-    in.dn_off = gd->down_offsets; in.dn_tgt = gd->down_targets; in.dn_w = gd->down_weights;
+    const bfg::CchInput in = input_of(gd);
     std::string err;
     bfg::LayoutParams lp;
+    lp.segment_limit = SEG;
     if (!bfg::build_layout(in, lp, c->L, err)) return fail(BFG_INVALID_PARAMETER, err);
-    // every path-slot and position index the kernels derive rests on these
+    // every slot, position and segment index the kernels derive rests on these
     // properties; refuse the graph rather than launch on a broken layout
-    if (!bfg::check_ancestry(in, c->L, err)) return fail(BFG_INTERNAL_ERROR, err);
+    if (!bfg::check_layout(in, c->L, err)) return fail(BFG_INTERNAL_ERROR, err);
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &e : c->ev) HIP_TRY(hipEventCreate(&e));
     hipStream_t s = c->stream;
     auto &L = c->L;
-    HIP_TRY(c->up_off.upload(L.up_off, s));
-    HIP_TRY(c->up_tgt.upload(L.up_tgt, s));
-    HIP_TRY(c->up_w.upload(L.up_w, s));
     HIP_TRY(c->parent.upload(L.parent, s));
-    HIP_TRY(c->depth.upload(L.depth, s));
-    HIP_TRY(c->in_off.upload(L.in_off, s));
-    HIP_TRY(c->in_src.upload(L.in_src, s));
-    HIP_TRY(c->in_w.upload(L.in_w, s));
+    HIP_TRY(c->pos_parent.upload(L.pos_parent, s));
     HIP_TRY(c->pos_pre.upload(L.pos_pre, s));
     HIP_TRY(c->pos_sub.upload(L.pos_subtree, s));
     HIP_TRY(c->pos_depth.upload(L.pos_depth, s));
-    HIP_TRY(c->step_pos.upload(L.step_pos, s));
-    HIP_TRY(c->task_step.upload(L.task_step, s));
-    HIP_TRY(c->pos_of_rank.upload(L.pos_of_rank, s));
+    HIP_TRY(c->seg_pos.upload(L.seg_pos, s));
+    HIP_TRY(c->in_off.upload(L.in_off, s));
+    HIP_TRY(c->in_split.upload(L.in_split, s));
+    HIP_TRY(c->in_src.upload(L.in_src, s));
+    HIP_TRY(c->in_w.upload(L.in_w, s));
+    HIP_TRY(c->ch_off.upload(L.ch_off, s));
+    HIP_TRY(c->ch_pos.upload(L.ch_pos, s));
+    HIP_TRY(c->cs_off.upload(L.cs_off, s));
+    HIP_TRY(c->cs_src.upload(L.cs_src, s));
+    HIP_TRY(c->cs_aoff.upload(L.cs_aoff, s));
+    HIP_TRY(c->ca_v.upload(L.ca_v, s));
+    HIP_TRY(c->ca_w.upload(L.ca_w, s));
+    HIP_TRY(c->bs_off.upload(L.bs_off, s));
+    HIP_TRY(c->bs_src.upload(L.bs_src, s));
+    HIP_TRY(c->bs_aoff.upload(L.bs_aoff, s));
+    HIP_TRY(c->ba_v.upload(L.ba_v, s));
+    HIP_TRY(c->ba_w.upload(L.ba_w, s));
+    HIP_TRY(c->sd_off.upload(L.sd_off, s));
+    HIP_TRY(c->sd_tgt.upload(L.sd_tgt, s));
+    HIP_TRY(c->sd_w.upload(L.sd_w, s));
+    HIP_TRY(c->su_off.upload(L.su_off, s));
+    HIP_TRY(c->su_tgt.upload(L.su_tgt, s));
+    HIP_TRY(c->su_w.upload(L.su_w, s));
+    HIP_TRY(c->up_off.upload(L.up_off, s));
+    HIP_TRY(c->up_tgt.upload(L.up_tgt, s));
+    HIP_TRY(c->up_w.upload(L.up_w, s));
+    HIP_TRY(c->ut_off.upload(L.ut_off, s));
+    HIP_TRY(c->ut_tgt.upload(L.ut_tgt, s));
+    HIP_TRY(c->ut_aoff.upload(L.ut_aoff, s));
+    HIP_TRY(c->uta_x.upload(L.uta_x, s));
+    HIP_TRY(c->uta_w.upload(L.uta_w, s));
     HIP_TRY(hipStreamSynchronize(s));
-    c->graph_bytes = 4ull * (L.up_off.size() + 2 * L.up_tgt.size() + 2ull * L.n + L.in_off.size() +
-                             2 * L.in_src.size() + 4ull * L.n + L.step_pos.size() + L.task_step.size());
-    // release host copies the kernels no longer need
-    std::vector<uint32_t>().swap(L.in_src);
-    std::vector<uint32_t>().swap(L.in_w);
-    std::vector<uint32_t>().swap(L.up_tgt);
-    std::vector<uint32_t>().swap(L.up_w);
+    c->graph_bytes = 4ull * (6ull * L.n + L.seg_pos.size() + L.bs_off.size() + 2ull * L.bs_src.size() +
+                             L.ba_w.size() + L.sd_off.size() + L.sd_w.size() + L.su_off.size() + L.su_w.size() +
+                             (L.n + 1) + 2ull * L.up_tgt.size() + L.ut_off.size() + 2ull * L.ut_tgt.size() +
+                             L.uta_w.size() + 2ull * (L.n + 1) + 2ull * L.in_src.size() + L.ch_off.size() +
+                             L.ch_pos.size() + L.cs_off.size() + 2ull * L.cs_src.size() + L.ca_w.size()) +
+                      2ull * (L.ba_v.size() + L.uta_x.size() + L.ca_v.size() + L.sd_tgt.size() + L.su_tgt.size());
+    // host copies the kernels alone need
+    for (auto *v : {&L.in_src, &L.in_w, &L.up_tgt, &L.up_w, &L.bs_src, &L.bs_aoff, &L.ba_w, &L.ut_tgt, &L.ut_aoff,
+                    &L.uta_w, &L.sd_w, &L.su_w, &L.cs_src, &L.cs_aoff, &L.ca_w, &L.ch_pos})
+        std::vector<uint32_t>().swap(*v);
+    for (auto *v : {&L.ba_v, &L.uta_x, &L.ca_v, &L.sd_tgt, &L.su_tgt}) std::vector<uint16_t>().swap(*v);
     *out = c.release();
     return BFG_SUCCESS;
 }
@@ -589,34 +1056,20 @@ void bfg_ctx_destroy(bfg_ctx *c) {
 
 int bfg_validate_graph(const bfg_graph_desc *gd, bfg_ctx_info *info) {
     if (!gd || !info) return fail(BFG_INVALID_PARAMETER, "null argument");
-    bfg::CchInput in;
-    in.n = gd->n_nodes;
-    in.up_off = gd->up_offsets; in.up_tgt = gd->up_targets; in.up_w = gd->up_weights;
-    in.dn_off = gd->down_offsets; in.dn_tgt = gd->down_targets; in.dn_w = gd->down_weights;
+    const bfg::CchInput in = input_of(gd);
     bfg::Layout L;
     std::string err;
-    if (!bfg::build_layout(in, bfg::LayoutParams(), L, err)) return fail(BFG_INVALID_PARAMETER, err);
-    if (!bfg::check_ancestry(in, L, err)) return fail(BFG_INTERNAL_ERROR, err);
-    memset(info, 0, sizeof(*info));
-    info->n_nodes = L.n;
-    info->m_up = L.m_up;
-    info->m_down = L.m_down;
-    info->etree_depth = L.max_depth;
-    info->n_levels = (uint32_t)L.level_task.size() - 1;
-    info->n_tasks = (uint32_t)L.task_step.size() - 1;
-    info->n_steps = (uint32_t)L.step_pos.size() - 1;
+    bfg::LayoutParams lp;
+    lp.segment_limit = SEG;
+    if (!bfg::build_layout(in, lp, L, err)) return fail(BFG_INVALID_PARAMETER, err);
+    if (!bfg::check_layout(in, L, err)) return fail(BFG_INTERNAL_ERROR, err);
+    fill_info(L, info);
     return BFG_SUCCESS;
 }
 
 int bfg_ctx_get_info(const bfg_ctx *c, bfg_ctx_info *info) {
     if (!c || !info) return fail(BFG_INVALID_PARAMETER, "null argument");
-    info->n_nodes = c->L.n;
-    info->m_up = c->L.m_up;
-    info->m_down = c->L.m_down;
-    info->etree_depth = c->L.max_depth;
-    info->n_levels = (uint32_t)c->L.level_task.size() - 1;
-    info->n_tasks = (uint32_t)c->L.task_step.size() - 1;
-    info->n_steps = (uint32_t)c->L.step_pos.size() - 1;
+    fill_info(c->L, info);
     info->lanes = c->lanes;
     info->lane_capacity = c->lane_cap;
     info->device_bytes = c->graph_bytes + c->arena.n * 4 + c->pathbuf.n * 4;
@@ -637,29 +1090,32 @@ int bfg_ctx_configure(bfg_ctx *c, uint32_t lanes, uint32_t lane_capacity) {
         if (lanes != 16 && lanes != 32 && lanes != 64) return fail(BFG_INVALID_PARAMETER, "lanes must be 16, 32 or 64");
         c->lanes = lanes;
     }
-    if (lane_capacity != 0) c->lane_cap = (lane_capacity + c->lanes - 1) / c->lanes * c->lanes;
-    else if (lanes != 0 && c->lane_cap) c->lane_cap = (c->lane_cap + c->lanes - 1) / c->lanes * c->lanes;
-    return ensure_arena(c);
+    if (lane_capacity != 0) {
+        c->lane_cap = (lane_capacity + c->lanes - 1) / c->lanes * c->lanes;
+        c->cap_fixed = true;
+        return ensure_arena(c);
+    }
+    c->cap_fixed = false;
+    if (c->lane_cap) c->lane_cap = (c->lane_cap + c->lanes - 1) / c->lanes * c->lanes;
+    return BFG_SUCCESS;
 }
 
 static int matrix_impl(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *tgt, uint32_t T, uint32_t thr,
-                       uint32_t *d_out, bool out_is_host, uint32_t *host_out) {
+                       uint32_t *d_out, uint32_t *host_out) {
     Req req = single_source_req(src, S, thr);
     int rc = stage_targets(c, tgt, T);
     if (rc) return rc;
     DevBuf<uint32_t> tmp;
-    uint32_t tmp_rows = 0;
-    return run_all(c, req, [&](uint32_t b0, uint32_t nl, float *oms) -> int {
-        if (!out_is_host) return gather_out(c, nl, c->tgt_pos.p, T, d_out, T, b0, nullptr, 0, oms);
-        if (tmp_rows < nl) {
-            HIP_TRY(tmp.alloc((size_t)nl * T));
-            tmp_rows = nl;
-        }
-        int r2 = gather_out(c, nl, c->tgt_pos.p, T, tmp.p, T, 0, nullptr, 0, oms);
-        if (r2) return r2;
-        HIP_TRY(hipMemcpy(host_out + (size_t)b0 * T, tmp.p, (size_t)nl * T * 4, hipMemcpyDeviceToHost));
-        return BFG_SUCCESS;
+    if (!d_out) {
+        HIP_TRY(tmp.alloc((size_t)S * T));
+        d_out = tmp.p;
+    }
+    rc = run_all(c, req, [&](uint32_t, uint32_t nl, const std::vector<uint32_t> &, float *oms) -> int {
+        return gather_out(c, nl, c->tgt_pos.p, T, c->row_map.p, d_out, T, nullptr, 0, oms);
     });
+    if (rc) return rc;
+    if (host_out) HIP_TRY(hipMemcpy(host_out, d_out, (size_t)S * T * 4, hipMemcpyDeviceToHost));
+    return BFG_SUCCESS;
 }
 
 int bfg_matrix_bounded(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *tgt, uint32_t T, uint32_t thr,
@@ -672,7 +1128,7 @@ int bfg_matrix_bounded(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32
     if (!rc) rc = check_ranks(c, tgt, T, "target");
     if (rc) return rc;
     if (S == 0 || T == 0) return BFG_SUCCESS;
-    return matrix_impl(c, src, S, tgt, T, thr, nullptr, true, out);
+    return matrix_impl(c, src, S, tgt, T, thr, nullptr, out);
 }
 
 int bfg_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *tgt, uint32_t T, uint32_t *out) {
@@ -692,7 +1148,7 @@ int bfg_matrix_device(bfg_ctx *c, const uint32_t *d_src, uint32_t S, const uint3
     int rc = check_ranks(c, src.data(), S, "source");
     if (!rc) rc = check_ranks(c, tgt.data(), T, "target");
     if (rc) return rc;
-    return matrix_impl(c, src.data(), S, tgt.data(), T, thr, d_out, false, nullptr);
+    return matrix_impl(c, src.data(), S, tgt.data(), T, thr, d_out, nullptr);
 }
 
 int bfg_query_batch(bfg_ctx *c, const uint32_t *src, uint32_t S, uint32_t thr, uint32_t *dist) {

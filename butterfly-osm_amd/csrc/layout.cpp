@@ -3,12 +3,13 @@
 
 #include <algorithm>
 #include <numeric>
+#include <tuple>
 
 namespace bfg {
 
 namespace {
 template <class T>
-void prefix(std::vector<T> &v) {  // exclusive prefix sum in place, v has n+1 entries
+void prefix(std::vector<T> &v) {  // exclusive prefix sum in place
     T run = 0;
     for (auto &x : v) { T c = x; x = run; run += c; }
 }
@@ -29,7 +30,7 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
             err = "CSR offsets not monotone";
             return false;
         }
-    // ---- INF-filtered flats (bucket_ch.rs:191-424) ----
+    // ---- validation and INF filtering (bucket_ch.rs:191-424) ----
     uint64_t mu = 0, md = 0;
     for (uint32_t r = 0; r < n; ++r) {
         for (uint64_t i = in.up_off[r]; i < in.up_off[r + 1]; ++i) {
@@ -46,41 +47,35 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
     if (mu >= 0xFFFFFFFFull || md >= 0xFFFFFFFFull) { err = "more than 2^32-1 finite arcs"; return false; }
     L.m_up = mu;
     L.m_down = md;
-    L.up_off.assign(n + 1, 0);
-    L.up_tgt.resize(mu);
-    L.up_w.resize(mu);
+    // UP arcs by tail rank, duplicates (tail, target) merged to the minimum weight
+    std::vector<uint32_t> uo(n + 1, 0), ut, uw;
+    ut.reserve(mu);
+    uw.reserve(mu);
     {
-        // duplicate (tail, target) pairs keep the minimum weight, so the upward
-        // walk relaxes each target once per tail
-        uint32_t k = 0;
         std::vector<std::pair<uint32_t, uint32_t>> tmp;
         for (uint32_t r = 0; r < n; ++r) {
-            L.up_off[r] = k;
+            uo[r] = (uint32_t)ut.size();
             tmp.clear();
             for (uint64_t i = in.up_off[r]; i < in.up_off[r + 1]; ++i)
                 if (in.up_w[i] != INF) tmp.emplace_back(in.up_tgt[i], in.up_w[i]);
             std::sort(tmp.begin(), tmp.end());
             for (size_t i = 0; i < tmp.size(); ++i) {
                 if (i && tmp[i].first == tmp[i - 1].first) continue;
-                L.up_tgt[k] = tmp[i].first; L.up_w[k] = tmp[i].second; ++k;
+                ut.push_back(tmp[i].first);
+                uw.push_back(tmp[i].second);
             }
         }
-        L.up_off[n] = k;
-        L.up_tgt.resize(k);
-        L.up_w.resize(k);
+        uo[n] = (uint32_t)ut.size();
     }
-    // ---- elimination tree of the symmetrised pattern (Liu) ----
-    // lower neighbours of j: down targets of j, and tails of up arcs into j
-    std::vector<uint32_t> uin_off(n + 1, 0), uin_src(L.up_off[n]);
-    for (uint32_t k = 0; k < L.up_off[n]; ++k) uin_off[L.up_tgt[k]]++;
-    prefix(uin_off);
-    {
-        std::vector<uint32_t> cur(uin_off.begin(), uin_off.end() - 1);
-        for (uint32_t r = 0; r < n; ++r)
-            for (uint32_t k = L.up_off[r]; k < L.up_off[r + 1]; ++k) uin_src[cur[L.up_tgt[k]]++] = r;
-    }
+    // ---- elimination tree of the symmetrised finite pattern (Liu) ----
     L.parent.assign(n, NONE);
     {
+        std::vector<uint32_t> uin_off(n + 1, 0), uin_src(ut.size());
+        for (uint32_t t : ut) uin_off[t]++;
+        prefix(uin_off);
+        std::vector<uint32_t> cur(uin_off.begin(), uin_off.end() - 1);
+        for (uint32_t r = 0; r < n; ++r)
+            for (uint32_t k = uo[r]; k < uo[r + 1]; ++k) uin_src[cur[ut[k]]++] = r;
         std::vector<uint32_t> anc(n, NONE);
         auto link = [&](uint32_t i, uint32_t j) {
             uint32_t r = i;
@@ -93,9 +88,6 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
             for (uint32_t k = uin_off[j]; k < uin_off[j + 1]; ++k) link(uin_src[k], j);
         }
     }
-    std::vector<uint32_t>().swap(uin_src);
-    std::vector<uint32_t>().swap(uin_off);
-    // depth, subtree size, preorder interval
     L.depth.assign(n, 0);
     L.subtree.assign(n, 1);
     L.pre.assign(n, 0);
@@ -112,174 +104,382 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
             L.max_depth = std::max(L.max_depth, L.depth[v]);
         }
     }
-    // ---- task schedule ----
+    // ---- regions, chains, segments ----
     const uint32_t BL = std::max<uint32_t>(lp.bottom_limit, 1);
+    const uint32_t SEG = std::max<uint32_t>(lp.segment_limit, 1);
     std::vector<uint8_t> top(n, 0);
     std::vector<uint32_t> ntopc(n, 0);
     for (uint32_t v = 0; v < n; ++v) {
         top[v] = L.subtree[v] > BL;
         if (top[v] && L.parent[v] != NONE) ntopc[L.parent[v]]++;
     }
-    // chains over top nodes, parents first
-    std::vector<uint32_t> chain_of(n, NONE), chain_level;
-    std::vector<std::vector<uint32_t>> chain_nodes;
-    // bottom subtrees grouped by the chain of their parent (key = chain id + 1, 0 = no parent)
-    std::vector<uint32_t> broot(n, NONE);
-    std::vector<std::vector<uint32_t>> group_roots(1);  // index 0: parentless subtrees
+    std::vector<uint32_t> chain_of(n, NONE), idx_in_chain(n, 0);
+    std::vector<std::vector<uint32_t>> chains;
     for (int64_t vv = (int64_t)n - 1; vv >= 0; --vv) {
         uint32_t v = (uint32_t)vv, p = L.parent[v];
-        if (top[v]) {
-            L.n_top++;
-            if (p == NONE || ntopc[p] >= 2) {
-                chain_of[v] = (uint32_t)chain_nodes.size();
-                chain_level.push_back(p == NONE ? 0 : chain_level[chain_of[p]] + 1);
-                chain_nodes.emplace_back();
-                group_roots.emplace_back();
-            } else {
-                chain_of[v] = chain_of[p];
-            }
-            chain_nodes[chain_of[v]].push_back(v);
+        if (!top[v]) continue;
+        if (p == NONE || ntopc[p] >= 2) {
+            chain_of[v] = (uint32_t)chains.size();
+            chains.emplace_back();
         } else {
-            if (p == NONE || top[p]) {
-                broot[v] = v;
-                group_roots[p == NONE ? 0 : chain_of[p] + 1].push_back(v);
-            } else {
-                broot[v] = broot[p];
-            }
+            chain_of[v] = chain_of[p];
+        }
+        idx_in_chain[v] = (uint32_t)chains[chain_of[v]].size();
+        chains[chain_of[v]].push_back(v);
+    }
+    L.n_chains = (uint32_t)chains.size();
+    std::vector<uint32_t> chain_seg0(chains.size());
+    std::vector<uint32_t> seg_level;
+    std::vector<std::pair<uint32_t, uint32_t>> seg_range;  // (chain, first index)
+    for (uint32_t c = 0; c < chains.size(); ++c) {
+        uint32_t head = chains[c][0], p = L.parent[head];
+        uint32_t lvl = 0;
+        if (p != NONE) lvl = seg_level[chain_seg0[chain_of[p]] + idx_in_chain[p] / SEG] + 1;
+        chain_seg0[c] = (uint32_t)seg_level.size();
+        for (uint32_t i = 0; i < chains[c].size(); i += SEG) {
+            seg_level.push_back(lvl++);
+            seg_range.emplace_back(c, i);
         }
     }
-    L.n_chains = (uint32_t)chain_nodes.size();
-    // bottom tasks: pack subtrees of one group up to group_limit nodes
-    struct BTask { uint32_t level; std::vector<uint32_t> roots; };
-    std::vector<BTask> btasks;
-    std::vector<uint32_t> btask_of_root(n, NONE);
-    const uint32_t GL = std::max(lp.group_limit, BL);
-    for (size_t gi = 0; gi < group_roots.size(); ++gi) {
-        auto &roots = group_roots[gi];
-        if (roots.empty()) continue;
-        uint32_t lvl = gi == 0 ? 0 : chain_level[gi - 1] + 1;
-        uint32_t acc = 0;
-        bool fresh = true;
-        for (uint32_t r : roots) {
-            if (fresh || acc + L.subtree[r] > GL) { btasks.push_back({lvl, {}}); acc = 0; fresh = false; }
-            btasks.back().roots.push_back(r);
-            btask_of_root[r] = (uint32_t)btasks.size() - 1;
-            acc += L.subtree[r];
-        }
-    }
-    L.n_bottom_tasks = (uint32_t)btasks.size();
-    // nodes of each bottom task bucketed by relative depth
-    std::vector<std::vector<std::vector<uint32_t>>> bsteps(btasks.size());
+    std::vector<uint32_t> seg_of(n, NONE);
+    for (uint32_t v = 0; v < n; ++v)
+        if (top[v]) seg_of[v] = chain_seg0[chain_of[v]] + idx_in_chain[v] / SEG;
+    // bottom heights
+    std::vector<uint32_t> height(n, 0);
     for (uint32_t v = 0; v < n; ++v) {
-        if (top[v]) continue;
-        uint32_t r = broot[v];
-        uint32_t t = btask_of_root[r];
-        uint32_t rel = L.depth[v] - L.depth[r];
-        auto &st = bsteps[t];
-        if (st.size() <= rel) st.resize(rel + 1);
-        st[rel].push_back(v);  // ascending rank inside a step
+        uint32_t p = L.parent[v];
+        if (!top[v] && p != NONE && !top[p]) height[p] = std::max(height[p], height[v] + 1);
     }
-    // ---- positions: level-major ----
-    uint32_t max_level = 0;
-    for (uint32_t l : chain_level) max_level = std::max(max_level, l);
-    for (auto &b : btasks) max_level = std::max(max_level, b.level);
-    std::vector<std::vector<uint32_t>> lvl_chains(max_level + 1), lvl_btasks(max_level + 1);
-    for (uint32_t c = 0; c < chain_nodes.size(); ++c) lvl_chains[chain_level[c]].push_back(c);
-    for (uint32_t t = 0; t < btasks.size(); ++t) lvl_btasks[btasks[t].level].push_back(t);
+    for (uint32_t v = 0; v < n; ++v)
+        if (!top[v]) L.max_height = std::max(L.max_height, height[v]);
+    // ---- positions ----
+    uint32_t n_seglvl = 0;
+    for (uint32_t l : seg_level) n_seglvl = std::max(n_seglvl, l + 1);
+    std::vector<std::vector<uint32_t>> lvl_segs(n_seglvl);
+    for (uint32_t s = 0; s < seg_level.size(); ++s) lvl_segs[seg_level[s]].push_back(s);
     L.pos_of_rank.assign(n, NONE);
     L.rank_of_pos.assign(n, NONE);
     uint32_t pos = 0;
-    L.step_pos.clear(); L.task_step.clear(); L.level_task.clear();
-    for (uint32_t l = 0; l <= max_level; ++l) {
-        L.level_task.push_back((uint32_t)L.task_step.size());
-        for (uint32_t c : lvl_chains[l]) {
-            L.task_step.push_back((uint32_t)L.step_pos.size());
-            for (uint32_t v : chain_nodes[c]) {
-                L.step_pos.push_back(pos);
-                L.pos_of_rank[v] = pos; L.rank_of_pos[pos] = v; ++pos;
-            }
-            L.max_steps_per_task = std::max<uint32_t>(L.max_steps_per_task, (uint32_t)chain_nodes[c].size());
-        }
-        for (uint32_t t : lvl_btasks[l]) {
-            L.task_step.push_back((uint32_t)L.step_pos.size());
-            for (auto &st : bsteps[t]) {
-                L.step_pos.push_back(pos);
-                for (uint32_t v : st) { L.pos_of_rank[v] = pos; L.rank_of_pos[pos] = v; ++pos; }
-            }
-            L.max_steps_per_task = std::max<uint32_t>(L.max_steps_per_task, (uint32_t)bsteps[t].size());
+    std::vector<uint32_t> seg_start(seg_level.size());
+    for (uint32_t l = 0; l < n_seglvl; ++l) {
+        L.seglvl_seg.push_back((uint32_t)L.seg_pos.size());
+        for (uint32_t s : lvl_segs[l]) {
+            L.seg_pos.push_back(pos);
+            seg_start[s] = pos;
+            const auto &ch = chains[seg_range[s].first];
+            uint32_t i0 = seg_range[s].second, i1 = std::min<uint32_t>((uint32_t)ch.size(), i0 + SEG);
+            L.max_segment = std::max(L.max_segment, i1 - i0);
+            for (uint32_t i = i0; i < i1; ++i) { L.pos_of_rank[ch[i]] = pos; L.rank_of_pos[pos] = ch[i]; ++pos; }
         }
     }
-    L.level_task.push_back((uint32_t)L.task_step.size());
-    L.task_step.push_back((uint32_t)L.step_pos.size());
-    L.step_pos.push_back(pos);
-    if (pos != n) { err = "layout lost nodes"; return false; }
-    // ---- pull flat by position (DownReverseAdjFlat, bucket_ch.rs:343-424) ----
-    L.in_off.assign(n + 1, 0);
-    for (uint32_t r = 0; r < n; ++r)
-        for (uint64_t i = in.dn_off[r]; i < in.dn_off[r + 1]; ++i)
-            if (in.dn_w[i] != INF) L.in_off[L.pos_of_rank[in.dn_tgt[i]]]++;
-    prefix(L.in_off);
-    L.in_src.resize(md);
-    L.in_w.resize(md);
+    L.seglvl_seg.push_back((uint32_t)L.seg_pos.size());
+    L.seg_pos.push_back(pos);
+    L.n_top = pos;
     {
+        std::vector<std::vector<uint32_t>> by_h(L.max_height + 1);
+        for (uint32_t v = 0; v < n; ++v)
+            if (!top[v]) by_h[height[v]].push_back(v);
+        for (int64_t h = L.max_height; h >= 0; --h) {
+            auto &vs = by_h[h];
+            if (vs.empty()) continue;
+            std::sort(vs.begin(), vs.end(), [&](uint32_t a, uint32_t b) { return L.pre[a] < L.pre[b]; });
+            L.botlvl_pos.push_back(pos);
+            for (uint32_t v : vs) { L.pos_of_rank[v] = pos; L.rank_of_pos[pos] = v; ++pos; }
+        }
+        L.botlvl_pos.push_back(pos);
+    }
+    if (pos != n) { err = "layout lost nodes"; return false; }
+    // ---- per position ----
+    L.pos_parent.resize(n); L.pos_pre.resize(n); L.pos_subtree.resize(n); L.pos_depth.resize(n);
+    for (uint32_t p = 0; p < n; ++p) {
+        uint32_t v = L.rank_of_pos[p];
+        L.pos_parent[p] = L.parent[v] == NONE ? NONE : L.pos_of_rank[L.parent[v]];
+        L.pos_pre[p] = L.pre[v];
+        L.pos_subtree[p] = L.subtree[v];
+        L.pos_depth[p] = L.depth[v];
+    }
+    auto same_seg = [&](uint32_t a, uint32_t b) { return top[a] && top[b] && seg_of[a] == seg_of[b]; };
+    // ---- DOWN pull by head position, outside arcs first ----
+    {
+        std::vector<uint32_t> cnt(n + 1, 0);
+        for (uint32_t r = 0; r < n; ++r)
+            for (uint64_t i = in.dn_off[r]; i < in.dn_off[r + 1]; ++i)
+                if (in.dn_w[i] != INF) cnt[L.pos_of_rank[in.dn_tgt[i]]]++;
+        L.in_off = cnt;
+        prefix(L.in_off);
+        L.in_src.resize(md);
+        L.in_w.resize(md);
         std::vector<uint32_t> cur(L.in_off.begin(), L.in_off.end() - 1);
         for (uint32_t r = 0; r < n; ++r) {
             uint32_t pr = L.pos_of_rank[r];
             for (uint64_t i = in.dn_off[r]; i < in.dn_off[r + 1]; ++i) {
                 if (in.dn_w[i] == INF) continue;
-                uint32_t p = L.pos_of_rank[in.dn_tgt[i]];
-                uint32_t k = cur[p]++;
+                uint32_t k = cur[L.pos_of_rank[in.dn_tgt[i]]]++;
                 L.in_src[k] = pr;
                 L.in_w[k] = in.dn_w[i];
             }
         }
-        // sort each node's in-arcs by source position
+        L.in_split.resize(n);
         std::vector<std::pair<uint32_t, uint32_t>> tmp;
         for (uint32_t p = 0; p < n; ++p) {
             uint32_t a = L.in_off[p], b = L.in_off[p + 1];
-            if (b - a < 2) continue;
+            uint32_t v = L.rank_of_pos[p];
+            // second class: top head — arcs from its own segment; bottom head —
+            // arcs from top sources
+            auto second = [&](uint32_t src_pos) {
+                return top[v] ? same_seg(L.rank_of_pos[src_pos], v) : src_pos < L.n_top;
+            };
             tmp.resize(b - a);
             for (uint32_t k = a; k < b; ++k) tmp[k - a] = {L.in_src[k], L.in_w[k]};
-            std::sort(tmp.begin(), tmp.end());
-            for (uint32_t k = a; k < b; ++k) { L.in_src[k] = tmp[k - a].first; L.in_w[k] = tmp[k - a].second; }
+            std::sort(tmp.begin(), tmp.end(), [&](const std::pair<uint32_t, uint32_t> &x,
+                                                  const std::pair<uint32_t, uint32_t> &y) {
+                bool ix = second(x.first), iy = second(y.first);
+                if (ix != iy) return !ix;
+                return x < y;
+            });
+            uint32_t split = b;
+            for (uint32_t k = a; k < b; ++k) {
+                L.in_src[k] = tmp[k - a].first;
+                L.in_w[k] = tmp[k - a].second;
+                if (split == b && second(tmp[k - a].first)) split = k;
+            }
+            L.in_split[p] = split;
+            if (top[v]) L.m_down_intra += b - split;
         }
     }
-    L.pos_pre.resize(n); L.pos_subtree.resize(n); L.pos_depth.resize(n);
-    for (uint32_t p = 0; p < n; ++p) {
-        uint32_t v = L.rank_of_pos[p];
-        L.pos_pre[p] = L.pre[v];
-        L.pos_subtree[p] = L.subtree[v];
-        L.pos_depth[p] = L.depth[v];
+    // ---- UP by tail position, outside arcs first ----
+    {
+        L.up_off.assign(n + 1, 0);
+        for (uint32_t r = 0; r < n; ++r) L.up_off[L.pos_of_rank[r]] = uo[r + 1] - uo[r];
+        prefix(L.up_off);
+        L.up_tgt.resize(ut.size());
+        L.up_w.resize(ut.size());
+        L.up_split.resize(n);
+        std::vector<std::pair<uint32_t, uint32_t>> outs, ins;
+        for (uint32_t p = 0; p < n; ++p) {
+            uint32_t x = L.rank_of_pos[p];
+            outs.clear();
+            ins.clear();
+            for (uint32_t k = uo[x]; k < uo[x + 1]; ++k) {
+                uint32_t y = ut[k];
+                if (same_seg(x, y)) ins.emplace_back(L.pos_of_rank[y] - seg_start[seg_of[y]], uw[k]);
+                else outs.emplace_back(L.pos_of_rank[y], uw[k]);
+            }
+            std::sort(outs.begin(), outs.end());
+            std::sort(ins.begin(), ins.end());
+            uint32_t k = L.up_off[p];
+            for (auto &e : outs) { L.up_tgt[k] = e.first; L.up_w[k] = e.second; ++k; }
+            L.up_split[p] = k;
+            for (auto &e : ins) { L.up_tgt[k] = e.first; L.up_w[k] = e.second; ++k; }
+            L.m_up_intra += ins.size();
+        }
+    }
+    // ---- grouped DOWN arcs: segments (outside arcs) and bottom chunks (top-sourced arcs) ----
+    const uint32_t BB = std::max<uint32_t>(lp.block_limit, 1);
+    // groups arcs (source, local head, weight) of one node set by source
+    auto group = [](std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> &arcs, std::vector<uint32_t> &src,
+                    std::vector<uint32_t> &aoff, std::vector<uint16_t> &av, std::vector<uint32_t> &aw) {
+        std::sort(arcs.begin(), arcs.end());
+        for (size_t i = 0; i < arcs.size(); ++i) {
+            if (i == 0 || std::get<0>(arcs[i]) != std::get<0>(arcs[i - 1])) {
+                src.push_back(std::get<0>(arcs[i]));
+                aoff.push_back((uint32_t)av.size());
+            }
+            av.push_back((uint16_t)std::get<1>(arcs[i]));
+            aw.push_back(std::get<2>(arcs[i]));
+        }
+    };
+    std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> arcs;
+    {
+        const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
+        L.bs_off.assign(nseg + 1, 0);
+        for (uint32_t s = 0; s < nseg; ++s) {
+            L.bs_off[s] = (uint32_t)L.bs_src.size();
+            const uint32_t p0 = L.seg_pos[s], p1 = L.seg_pos[s + 1];
+            L.max_block = std::max(L.max_block, p1 - p0);
+            arcs.clear();
+            for (uint32_t p = p0; p < p1; ++p)
+                for (uint32_t k = L.in_off[p]; k < L.in_split[p]; ++k) arcs.emplace_back(L.in_src[k], p - p0, L.in_w[k]);
+            group(arcs, L.bs_src, L.bs_aoff, L.ba_v, L.ba_w);
+        }
+        L.bs_off[nseg] = (uint32_t)L.bs_src.size();
+        L.bs_aoff.push_back((uint32_t)L.ba_v.size());
+    }
+    {
+        std::vector<uint32_t> bot;
+        for (uint32_t p = L.n_top; p < n; ++p) bot.push_back(p);
+        std::sort(bot.begin(), bot.end(), [&](uint32_t a, uint32_t b) { return L.pos_pre[a] < L.pos_pre[b]; });
+        L.ch_off.clear();
+        L.ch_pos.clear();
+        for (size_t i = 0; i < bot.size(); i += BB) {
+            L.ch_off.push_back((uint32_t)L.ch_pos.size());
+            const size_t e = std::min(bot.size(), i + BB);
+            arcs.clear();
+            L.cs_off.push_back((uint32_t)L.cs_src.size());
+            for (size_t k = i; k < e; ++k) {
+                const uint32_t p = bot[k];
+                L.ch_pos.push_back(p);
+                for (uint32_t a = L.in_split[p]; a < L.in_off[p + 1]; ++a)
+                    arcs.emplace_back(L.in_src[a], (uint32_t)(k - i), L.in_w[a]);
+            }
+            group(arcs, L.cs_src, L.cs_aoff, L.ca_v, L.ca_w);
+        }
+        L.ch_off.push_back((uint32_t)L.ch_pos.size());
+        L.cs_off.push_back((uint32_t)L.cs_src.size());
+        L.cs_aoff.push_back((uint32_t)L.ca_v.size());
+    }
+    // UP arcs leaving each segment, grouped by target position
+    {
+        const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
+        L.ut_off.assign(nseg + 1, 0);
+        L.ut_tgt.clear();
+        L.ut_aoff.clear();
+        L.uta_x.clear();
+        L.uta_w.clear();
+        std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> arcs;  // (target, local tail, weight)
+        for (uint32_t s = 0; s < nseg; ++s) {
+            L.ut_off[s] = (uint32_t)L.ut_tgt.size();
+            const uint32_t p0 = L.seg_pos[s], p1 = L.seg_pos[s + 1];
+            arcs.clear();
+            for (uint32_t p = p0; p < p1; ++p)
+                for (uint32_t k = L.up_off[p]; k < L.up_split[p]; ++k) arcs.emplace_back(L.up_tgt[k], p - p0, L.up_w[k]);
+            std::sort(arcs.begin(), arcs.end());
+            for (size_t i = 0; i < arcs.size(); ++i) {
+                if (i == 0 || std::get<0>(arcs[i]) != std::get<0>(arcs[i - 1])) {
+                    L.ut_tgt.push_back(std::get<0>(arcs[i]));
+                    L.ut_aoff.push_back((uint32_t)L.uta_x.size());
+                }
+                L.uta_x.push_back((uint16_t)std::get<1>(arcs[i]));
+                L.uta_w.push_back(std::get<2>(arcs[i]));
+            }
+        }
+        L.ut_off[nseg] = (uint32_t)L.ut_tgt.size();
+        L.ut_aoff.push_back((uint32_t)L.uta_x.size());
+    }
+    // intra-segment arc lists by tail
+    {
+        const uint32_t nt = L.n_top;
+        L.sd_off.assign(nt + 1, 0);
+        L.su_off.assign(nt + 1, 0);
+        for (uint32_t p = 0; p < nt; ++p) {
+            for (uint32_t k = L.in_split[p]; k < L.in_off[p + 1]; ++k) L.sd_off[L.in_src[k]]++;
+            L.su_off[p] = L.up_off[p + 1] - L.up_split[p];
+        }
+        prefix(L.sd_off);
+        prefix(L.su_off);
+        L.sd_tgt.resize(L.sd_off[nt]);
+        L.sd_w.resize(L.sd_off[nt]);
+        L.su_tgt.resize(L.su_off[nt]);
+        L.su_w.resize(L.su_off[nt]);
+        std::vector<uint32_t> cur(L.sd_off.begin(), L.sd_off.end() - 1);
+        for (size_t s = 0; s + 1 < L.seg_pos.size(); ++s)
+            for (uint32_t p = L.seg_pos[s]; p < L.seg_pos[s + 1]; ++p) {  // heads ascending per tail
+                for (uint32_t k = L.in_split[p]; k < L.in_off[p + 1]; ++k) {
+                    const uint32_t d = cur[L.in_src[k]]++;
+                    L.sd_tgt[d] = (uint16_t)(p - L.seg_pos[s]);
+                    L.sd_w[d] = L.in_w[k];
+                }
+                for (uint32_t k = L.up_split[p], d = L.su_off[p]; k < L.up_off[p + 1]; ++k, ++d) {
+                    L.su_tgt[d] = (uint16_t)L.up_tgt[k];
+                    L.su_w[d] = L.up_w[k];
+                }
+            }
     }
     return true;
 }
 
-bool check_ancestry(const CchInput &in, const Layout &L, std::string &err) {
-    auto is_anc = [&](uint32_t a, uint32_t d) {  // a ancestor-or-self of d
+bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
+    const uint32_t n = in.n;
+    auto is_anc = [&](uint32_t a, uint32_t d) {  // rank a ancestor-or-self of rank d
         return L.pre[a] <= L.pre[d] && L.pre[d] < L.pre[a] + L.subtree[a];
     };
-    for (uint32_t r = 0; r < in.n; ++r) {
+    for (uint32_t r = 0; r < n; ++r) {
         for (uint64_t i = in.up_off[r]; i < in.up_off[r + 1]; ++i)
             if (in.up_w[i] != INF && !is_anc(in.up_tgt[i], r)) { err = "up arc between non-ancestors"; return false; }
         for (uint64_t i = in.dn_off[r]; i < in.dn_off[r + 1]; ++i)
             if (in.dn_w[i] != INF && !is_anc(r, in.dn_tgt[i])) { err = "down arc between non-ancestors"; return false; }
     }
-    // schedule: every in-arc source is final before its head's step starts —
-    // an earlier step of the same task, or a task of an earlier level.
-    const uint32_t n = in.n;
-    std::vector<uint32_t> step_of(n), task_of_step(L.step_pos.size() - 1), level_of_task(L.task_step.size() - 1);
-    for (size_t l = 0; l + 1 < L.level_task.size(); ++l)
-        for (uint32_t t = L.level_task[l]; t < L.level_task[l + 1]; ++t) level_of_task[t] = (uint32_t)l;
-    for (size_t t = 0; t + 1 < L.task_step.size(); ++t)
-        for (uint32_t s = L.task_step[t]; s < L.task_step[t + 1]; ++s) task_of_step[s] = (uint32_t)t;
-    for (size_t s = 0; s + 1 < L.step_pos.size(); ++s)
-        for (uint32_t p = L.step_pos[s]; p < L.step_pos[s + 1]; ++p) step_of[p] = (uint32_t)s;
+    // stage of a position: segment level for top nodes, then the bottom levels
+    std::vector<uint32_t> seg_of_pos(L.n_top), lvl_of_seg(L.seg_pos.size() - 1), stage(n);
+    for (size_t l = 0; l + 1 < L.seglvl_seg.size(); ++l)
+        for (uint32_t s = L.seglvl_seg[l]; s < L.seglvl_seg[l + 1]; ++s) lvl_of_seg[s] = (uint32_t)l;
+    for (size_t s = 0; s + 1 < L.seg_pos.size(); ++s)
+        for (uint32_t p = L.seg_pos[s]; p < L.seg_pos[s + 1]; ++p) { seg_of_pos[p] = (uint32_t)s; stage[p] = lvl_of_seg[s]; }
+    const uint32_t nsl = (uint32_t)L.seglvl_seg.size() - 1;
+    for (size_t b = 0; b + 1 < L.botlvl_pos.size(); ++b)
+        for (uint32_t p = L.botlvl_pos[b]; p < L.botlvl_pos[b + 1]; ++p) stage[p] = nsl + (uint32_t)b;
     for (uint32_t p = 0; p < n; ++p) {
-        uint32_t sp = step_of[p], tp = task_of_step[sp];
+        const bool tp = p < L.n_top;
+        if (L.in_split[p] < L.in_off[p] || L.in_split[p] > L.in_off[p + 1]) { err = "bad in_split"; return false; }
         for (uint32_t k = L.in_off[p]; k < L.in_off[p + 1]; ++k) {
-            uint32_t q = L.in_src[k], sq = step_of[q], tq = task_of_step[sq];
-            bool ok = (tq == tp) ? (sq < sp) : (level_of_task[tq] < level_of_task[tp]);
-            if (!ok) { err = "schedule violates an arc dependency"; return false; }
+            uint32_t q = L.in_src[k];
+            bool second = k >= L.in_split[p];
+            bool ok;
+            if (tp) ok = second ? (q < L.n_top && seg_of_pos[q] == seg_of_pos[p] && q < p)
+                                : (q < n && stage[q] < stage[p]);
+            else ok = second ? q < L.n_top : (q >= L.n_top && q < n && stage[q] < stage[p]);
+            if (!ok) { err = "down schedule violates an arc dependency"; return false; }
+        }
+        if (L.up_split[p] < L.up_off[p] || L.up_split[p] > L.up_off[p + 1]) { err = "bad up_split"; return false; }
+        for (uint32_t k = L.up_off[p]; k < L.up_off[p + 1]; ++k) {
+            bool intra = k >= L.up_split[p];
+            if (intra) {
+                if (!tp) { err = "intra up arc at a bottom node"; return false; }
+                uint32_t q = L.seg_pos[seg_of_pos[p]] + L.up_tgt[k];
+                if (!(q < p && seg_of_pos[q] == seg_of_pos[p])) { err = "bad intra up arc"; return false; }
+            } else {
+                uint32_t q = L.up_tgt[k];
+                if (q >= n) { err = "up target out of range"; return false; }
+                if (tp && !(q < L.n_top && stage[q] < stage[p])) { err = "up schedule violates an arc dependency"; return false; }
+                if (!tp && !(stage[q] < stage[p])) { err = "bottom up arc to a non-ancestor level"; return false; }
+            }
+        }
+        if (L.pos_parent[p] != NONE && L.pos_parent[p] >= n) { err = "bad parent position"; return false; }
+    }
+    // grouped segment arcs: sources final before the segment's stage
+    for (size_t s = 0; s + 1 < L.seg_pos.size(); ++s) {
+        const uint32_t p0 = L.seg_pos[s], p1 = L.seg_pos[s + 1];
+        if (p1 <= p0 || p1 - p0 > 0xFFFF) { err = "bad segment size"; return false; }
+        for (uint32_t p = p0; p < p1; ++p) {
+            for (uint32_t k = L.sd_off[p]; k < L.sd_off[p + 1]; ++k)
+                if (!(L.sd_tgt[k] > p - p0 && L.sd_tgt[k] < p1 - p0)) { err = "intra DOWN arc not to a later node"; return false; }
+            for (uint32_t k = L.su_off[p]; k < L.su_off[p + 1]; ++k)
+                if (!(L.su_tgt[k] < p - p0)) { err = "intra UP arc not to an earlier node"; return false; }
+        }
+        for (uint32_t k = L.bs_off[s]; k < L.bs_off[s + 1]; ++k) {
+            if (!(L.bs_src[k] < n && stage[L.bs_src[k]] < stage[p0])) { err = "segment source not final"; return false; }
+            for (uint32_t a = L.bs_aoff[k]; a < L.bs_aoff[k + 1]; ++a)
+                if (L.ba_v[a] >= p1 - p0) { err = "segment arc head out of segment"; return false; }
+        }
+    }
+    // bottom chunks: every bottom position once, sources in the top region
+    {
+        std::vector<uint8_t> seen(n, 0);
+        const size_t nch = L.ch_off.size() - 1;
+        if (L.cs_off.size() != nch + 1) { err = "chunk tables disagree"; return false; }
+        for (size_t c = 0; c < nch; ++c) {
+            const uint32_t len = L.ch_off[c + 1] - L.ch_off[c];
+            if (len == 0 || len > 0xFFFF) { err = "bad chunk size"; return false; }
+            for (uint32_t i = L.ch_off[c]; i < L.ch_off[c + 1]; ++i) {
+                const uint32_t p = L.ch_pos[i];
+                if (p < L.n_top || p >= n || seen[p]) { err = "chunk positions not a bottom partition"; return false; }
+                seen[p] = 1;
+            }
+            for (uint32_t k = L.cs_off[c]; k < L.cs_off[c + 1]; ++k) {
+                if (L.cs_src[k] >= L.n_top) { err = "chunk source outside the top region"; return false; }
+                for (uint32_t a = L.cs_aoff[k]; a < L.cs_aoff[k + 1]; ++a)
+                    if (L.ca_v[a] >= len) { err = "chunk arc head out of chunk"; return false; }
+            }
+        }
+        if (L.ch_pos.size() != n - L.n_top) { err = "chunks miss bottom positions"; return false; }
+    }
+    for (size_t s = 0; s + 1 < L.seg_pos.size(); ++s) {
+        const uint32_t p0 = L.seg_pos[s], p1 = L.seg_pos[s + 1];
+        for (uint32_t k = L.ut_off[s]; k < L.ut_off[s + 1]; ++k) {
+            const uint32_t y = L.ut_tgt[k];
+            if (!(y < L.n_top && stage[y] < stage[p0])) { err = "segment up target not an earlier top node"; return false; }
+            for (uint32_t a = L.ut_aoff[k]; a < L.ut_aoff[k + 1]; ++a)
+                if (L.uta_x[a] >= p1 - p0) { err = "segment up arc tail out of segment"; return false; }
         }
     }
     return true;

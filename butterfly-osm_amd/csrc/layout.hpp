@@ -3,18 +3,28 @@
 // Input: the rank-aligned CCH the reference engine holds (CchTopo up/down CSR
 // + CchWeights decoded to u32, route/src/formats/cch_topo.rs:250-286,
 // route/src/formats/cch_weights.rs:171-187).  Output: everything the kernels
-// read, derived once per context:
-//   * INF-filtered flats (route/src/matrix/bucket_ch.rs:191-424):
-//       up   — push CSR by tail rank (UpAdjFlat), for the upward walk;
-//       pull — DOWN arcs keyed by head (DownReverseAdjFlat), re-indexed by
-//              sweep position, for the downward pull sweep.
-//   * the elimination tree of the symmetrised finite-arc pattern (Liu), so
-//     every arc joins an ancestor/descendant pair: the up search space of s
-//     is a subset of ancestors(s) and nodes that share no ancestry share no arc.
-//   * a task schedule: chains of the top part of the tree and groups of
-//     small bottom subtrees; tasks of one level are independent, and inside a
-//     task the nodes of one step are independent.
-//   * sweep positions: nodes ordered (task level, task, step, rank).
+// read, derived once per context.
+//
+// Elimination tree.  The tree of the symmetrised finite-arc pattern (Liu) has
+// every arc between an ancestor and a descendant, so (a) the up search space
+// of s lies on the root path of s, and (b) nodes that are not ancestor-related
+// share no arc and can be relaxed concurrently.
+//
+// Regions.  Nodes whose subtree holds more than `bottom_limit` nodes form the
+// top region T (the separators of the nested dissection and the upper part of
+// its leaves); the rest is the bottom region B, a forest of small subtrees.
+//   * T is cut into chains (maximal single-child paths) and the chains into
+//     segments of at most `segment_limit` nodes.  A segment depends on the
+//     segment holding its head's parent; segments of one segment level are
+//     independent.  Arcs into a segment node from outside the segment are
+//     relaxed in parallel (pull); arcs inside it are relaxed sequentially by
+//     one workgroup from LDS.
+//   * B is swept level-synchronously by subtree height (descending for the
+//     downward sweep): nodes of equal height are never ancestor-related.
+//
+// Positions.  Nodes are renumbered: T first (segment levels ascending, a
+// segment's nodes in chain order, parent first), then B by descending height
+// (pre-order within a height).  Label arenas are indexed by position.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -34,38 +44,71 @@ struct CchInput {
 };
 
 struct LayoutParams {
-    uint32_t bottom_limit = 1024;   // subtree size that becomes a bottom task
-    uint32_t group_limit = 4096;    // max nodes packed into one bottom task
+    uint32_t bottom_limit = 1024;   // subtree size at or below which a node is bottom
+    uint32_t segment_limit = 256;   // max nodes of one chain segment (LDS resident)
+    uint32_t block_limit = 256;     // max nodes of one bottom block (LDS resident)
 };
 
 struct Layout {
     uint32_t n = 0;
+    uint32_t n_top = 0;             // positions [0, n_top) are the top region
     uint32_t max_depth = 0;         // etree depth of the deepest node (root = 0)
+    uint32_t max_height = 0;        // largest bottom-subtree height
     // per rank
     std::vector<uint32_t> parent, depth, pre, subtree;
     std::vector<uint32_t> pos_of_rank, rank_of_pos;
-    // up flat by rank (push)
-    std::vector<uint32_t> up_off, up_tgt, up_w;
-    // pull flat by position (in-arcs of the node at position p)
-    std::vector<uint32_t> in_off, in_src, in_w;
-    // per position copies the sweep needs
+    // per position
+    std::vector<uint32_t> pos_parent;          // parent position (NONE at roots)
     std::vector<uint32_t> pos_pre, pos_subtree, pos_depth;
+    // DOWN in-arcs by head position.  Top head p: [in_off[p], in_split[p]) come
+    // from outside p's segment, [in_split[p], in_off[p+1]) from earlier nodes of
+    // the same segment.  Bottom head: see the bottom chunks below.
+    std::vector<uint32_t> in_off, in_split, in_src, in_w;
+    // UP flat by tail position: arcs [up_off[p], up_split[p]) leave p's segment
+    // (all arcs for bottom nodes), [up_split[p], up_off[p+1]) stay inside it
+    // (target as offset from the segment start); targets are positions
+    std::vector<uint32_t> up_off, up_split, up_tgt, up_w;
     // schedule
-    std::vector<uint32_t> step_pos;        // n_steps+1: step s covers positions [step_pos[s], step_pos[s+1])
-    std::vector<uint32_t> task_step;       // n_tasks+1: task t covers steps [task_step[t], task_step[t+1])
-    std::vector<uint32_t> level_task;      // n_levels+1: level l covers tasks [level_task[l], level_task[l+1])
+    std::vector<uint32_t> seg_pos;        // n_seg+1: segment s covers [seg_pos[s], seg_pos[s+1])
+    std::vector<uint32_t> seglvl_seg;     // n_seglvl+1: level l covers segments [seglvl_seg[l], seglvl_seg[l+1])
+    std::vector<uint32_t> botlvl_pos;     // n_botlvl+1: bottom level l covers positions [botlvl_pos[l], botlvl_pos[l+1])
+    // DOWN arcs entering segment s from outside it, grouped by source:
+    // sources bs_src[bs_off[s] .. bs_off[s+1]), arcs of source k are
+    // [bs_aoff[k], bs_aoff[k+1]) with head ba_v (offset in the segment) and weight ba_w
+    std::vector<uint32_t> bs_off, bs_src, bs_aoff, ba_w;
+    std::vector<uint16_t> ba_v;
+    // Bottom chunks: bottom nodes in pre-order cut into chunks of at most
+    // block_limit nodes (members of one chunk share their top ancestors).
+    // Chunk c holds positions ch_pos[ch_off[c] .. ch_off[c+1]); its DOWN arcs
+    // from top-region sources are grouped by source in cs_* / ca_* as above.
+    // For a bottom position p, in-arcs [in_off[p], in_split[p]) come from
+    // bottom sources, [in_split[p], in_off[p+1]) from top sources.
+    std::vector<uint32_t> ch_off, ch_pos, cs_off, cs_src, cs_aoff, ca_w;
+    std::vector<uint16_t> ca_v;
+    uint32_t max_block = 0;
+    // UP arcs leaving segment s, grouped by target: targets ut_tgt[ut_off[s] ..
+    // ut_off[s+1]) (positions), arcs [ut_aoff[k], ut_aoff[k+1]) with tail uta_x
+    // (offset in the segment) and weight uta_w
+    std::vector<uint32_t> ut_off, ut_tgt, ut_aoff, uta_w;
+    std::vector<uint16_t> uta_x;
+    // intra-segment arcs by tail position p (top region): DOWN arcs to later
+    // nodes [sd_off[p], sd_off[p+1]) and UP arcs to earlier nodes
+    // [su_off[p], su_off[p+1]); head as offset from the segment start
+    std::vector<uint32_t> sd_off, sd_w, su_off, su_w;
+    std::vector<uint16_t> sd_tgt, su_tgt;
     // statistics
     uint64_t m_up = 0, m_down = 0, m_up_inf = 0, m_down_inf = 0;
-    uint32_t n_top = 0, n_chains = 0, n_bottom_tasks = 0, max_steps_per_task = 0;
+    uint64_t m_down_intra = 0, m_up_intra = 0;
+    uint32_t n_chains = 0, max_segment = 0;
 };
 
 // Builds the layout; returns false with a message on invalid input (ranks out
 // of range, arcs that do not go up/down, more than 2^32-1 finite arcs).
 bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &out, std::string &err);
 
-// Verifies the property the sweep relies on: for every finite arc the higher
-// endpoint is an etree ancestor of the lower one.  O(m * depth) worst case;
-// used by tests and on request.
-bool check_ancestry(const CchInput &in, const Layout &L, std::string &err);
+// Verifies the properties the kernels rely on: every finite arc joins an
+// etree ancestor/descendant pair, and the schedule relaxes every arc after its
+// tail is final (and, upward, before its head is read).
+bool check_layout(const CchInput &in, const Layout &L, std::string &err);
 
 }  // namespace bfg

@@ -61,7 +61,10 @@ typedef struct bfg_ctx_info {
     uint32_t n_nodes;
     uint64_t m_up, m_down;          /* finite arcs after INF filtering           */
     uint32_t etree_depth;           /* deepest node of the elimination tree      */
-    uint32_t n_levels, n_tasks, n_steps;
+    uint32_t n_top;                 /* nodes in the top region (separator chains) */
+    uint32_t n_segments;            /* chain segments of the top region          */
+    uint32_t n_segment_levels;      /* dependent stages of the top region        */
+    uint32_t n_bottom_levels;       /* height levels of the bottom region        */
     uint32_t lanes;                 /* K: source lanes interleaved per group     */
     uint32_t lane_capacity;         /* sources per device batch                  */
     uint64_t device_bytes;          /* resident CCH + arenas                     */

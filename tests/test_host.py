@@ -53,7 +53,9 @@ def test_validate_broom_and_grids(grid_small, grid_config1):
         rc, info = validate(gg)
         assert rc == 0, _native.bfgpu().bfg_last_error()
         assert info.m_down == gg.m_down and info.m_up == gg.m_up
-        assert info.n_steps >= info.n_tasks >= info.n_levels >= 1
+        assert info.n_bottom_levels >= 1
+        assert info.n_segments >= info.n_segment_levels
+        assert 0 < info.n_top < gg.n
 
 
 def test_rejects_non_rank_aligned_cch():
