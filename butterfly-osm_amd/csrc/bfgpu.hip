@@ -13,17 +13,19 @@
 //   bottom_walk  one wavefront per lane walks the bottom part of its root path
 //                (depth-indexed slots), relaxing UP arcs into bottom ancestors
 //                (slots) and top ancestors (atomic min into the arena)
-//   up_seg       per (segment, group), deepest segment level first: the
-//                segment's labels in LDS, intra-segment UP arcs relaxed from
-//                the last node to the first (sparse lists), then the UP arcs
-//                leaving the segment, grouped by target, reduced per lane and
-//                pushed with one atomic min per (target, lane)
+//   up_tile      per (group, segment), deepest segment level first, as a
+//                register tile (8 nodes x KL/8 lanes per thread): what lower
+//                levels pushed, intra-segment UP arcs as 8x8 blocks from the
+//                last octet to the first, then the UP arcs leaving the
+//                segment as ELL rows reduced per target block and pushed with
+//                one atomic min per (target, lane)
 // Downward phase — the rank-descending DOWN scan (batched_phast.rs:205-250,
 // :538-590) scheduled by the elimination tree:
-//   down_seg     per (segment, group) of a segment level: labels in LDS seeded
-//                with the upward labels, DOWN arcs from outside the segment
-//                grouped by source (one source label read serves all its
-//                arcs), then intra-segment DOWN arcs first node to last
+//   down_tile    per (group, segment) of a segment level, as the same register
+//                tile seeded with the upward labels: DOWN arcs from outside the
+//                segment as ELL rows (min-plus product of the sources' labels
+//                and the tile's weights), then the intra-segment blocks from the
+//                first octet to the last
 //   bottom_chunk per (bottom chunk, group): seeds from the lanes' bottom path
 //                slots, then DOWN arcs from top-region sources, grouped by
 //                source, into the chunk's LDS labels
@@ -70,7 +72,7 @@ int fail(int code, const std::string &msg) {
     } while (0)
 
 constexpr int NT = 256;   // threads per workgroup of the gather kernel
-constexpr int NTB = 512;  // threads per workgroup of the segment and block kernels
+constexpr int NTB = 512;  // threads per workgroup of the bottom chunk kernel
 
 __device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
     return __builtin_elementwise_add_sat(a, b);
@@ -223,149 +225,228 @@ __device__ __forceinline__ void relax_arcs(uint32_t *L, uint32_t d, uint32_t a0,
     }
 }
 
-// Min over the arcs [a0, a1) of one target of L[tail] + w (UP arcs leaving a
-// segment); arc reads as in relax_arcs.
-template <int KL>
-__device__ __forceinline__ uint32_t reduce_arcs(const uint32_t *L, uint32_t a0, uint32_t a1,
-                                                const uint16_t *__restrict__ ax, const uint32_t *__restrict__ aw,
-                                                uint32_t lane) {
-    uint32_t m = INF;
-    if constexpr (KL >= 64) {
-        for (uint32_t base = a0; base < a1; base += 64) {
-            const uint32_t cnt = min(64u, a1 - base);
-            const uint32_t wl = __lane_id();
-            uint32_t mx = 0, mw = INF;
-            if (wl < cnt) { mx = ax[base + wl]; mw = aw[base + wl]; }
-#pragma unroll 4
-            for (uint32_t j = 0; j < cnt; ++j) {
-                const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)mx, (int)j);
-                const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)mw, (int)j);
-                m = min(m, sat_add(L[x * KL + lane], w));
-            }
-        }
+// ---------------------------------------------------------------------------
+// Dense tiles of the top region.  A workgroup of 256 threads owns one
+// (segment, group) pair: thread t holds nodes 8*(t/8) .. +8 of the segment
+// (its octet) for lanes TK*(t%8) .. +TK of the group, TK = KL/8, as an 8 x TK
+// register tile.  Wavefront w thus covers the 64 nodes of wave block w.
+// ---------------------------------------------------------------------------
+constexpr int TILE_THREADS = 256;
+
+template <int N>
+__device__ __forceinline__ void load_u32(const uint32_t *__restrict__ p, uint32_t (&v)[N]) {
+    static_assert(N == 2 || N % 4 == 0, "vector width");
+    if constexpr (N == 2) {
+        const uint2 x = *reinterpret_cast<const uint2 *>(p);
+        v[0] = x.x;
+        v[1] = x.y;
     } else {
-        for (uint32_t base = a0; base < a1; base += KL) {
-            const uint32_t cnt = min((uint32_t)KL, a1 - base);
-            uint32_t mx = 0, mw = INF;
-            if (lane < cnt) { mx = ax[base + lane]; mw = aw[base + lane]; }
-            for (uint32_t j = 0; j < cnt; ++j) {
-                const uint32_t x = (uint32_t)__shfl((int)mx, (int)j, KL);
-                const uint32_t w = (uint32_t)__shfl((int)mw, (int)j, KL);
-                m = min(m, sat_add(L[x * KL + lane], w));
-            }
+#pragma unroll
+        for (int q = 0; q < N / 4; ++q) {
+            const uint4 x = reinterpret_cast<const uint4 *>(p)[q];
+            v[4 * q] = x.x;
+            v[4 * q + 1] = x.y;
+            v[4 * q + 2] = x.z;
+            v[4 * q + 3] = x.w;
         }
     }
-    return m;
 }
 
-// Broadcast of lane j of the slot to every lane of the slot.
-template <int KL>
-__device__ __forceinline__ uint32_t bcast(uint32_t x, uint32_t j) {
-    if constexpr (KL >= 64) return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)j);
-    return (uint32_t)__shfl((int)x, (int)j, KL);
+template <int N>
+__device__ __forceinline__ void store_u32(uint32_t *__restrict__ p, const uint32_t (&v)[N]) {
+    if constexpr (N == 2) {
+        *reinterpret_cast<uint2 *>(p) = make_uint2(v[0], v[1]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < N / 4; ++q)
+            reinterpret_cast<uint4 *>(p)[q] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    }
 }
 
-// Intra-segment relaxation, node by node, over the LDS labels L[len][KL]:
-// DOWN runs first to last over arcs to later nodes, UP last to first over
-// arcs to earlier nodes.  soff (LDS) holds the tails' arc offsets.  A step's
-// arcs are split evenly over the NS slots; each slot fetches its share (at
-// most KL arcs, since a node has fewer than SEG <= NS*KL intra arcs) with one
-// coalesced load and applies it arc by arc; the next step's share is fetched
-// before the current one is applied.  One barrier per step.
-template <int KL, int SEG, int NTH, bool DOWN>
-__device__ __forceinline__ void chain_relax(uint32_t *L, const uint32_t *soff, uint32_t len,
-                                            const uint16_t *__restrict__ at, const uint32_t *__restrict__ aw,
-                                            uint32_t slot, uint32_t lane) {
-    constexpr int NS = NTH / KL;
-    static_assert(SEG <= NS * KL, "a step's share of arcs fits one load per slot");
-    auto fetch = [&](uint32_t step, uint32_t &t, uint32_t &w, uint32_t &cnt) {
-        t = 0;
-        w = INF;
-        cnt = 0;
-        if (step >= len) return;
-        const uint32_t i = DOWN ? step : len - 1 - step;
-        const uint32_t a0 = soff[i], a1 = soff[i + 1];
-        const uint32_t per = (a1 - a0 + NS - 1) / NS;
-        const uint32_t c0 = a0 + slot * per;
-        cnt = c0 < a1 ? min(per, a1 - c0) : 0u;
-        if (lane < cnt) {
-            t = at[c0 + lane];
-            w = aw[c0 + lane];
-        }
-    };
-    uint32_t t0, w0, n0, t1, w1, n1;
-    fetch(0, t0, w0, n0);
-    for (uint32_t step = 0; step < len; ++step) {
-        fetch(step + 1, t1, w1, n1);
-        const uint32_t i = DOWN ? step : len - 1 - step;
-        const uint32_t d = L[i * KL + lane];
-        for (uint32_t j = 0; j < n0; ++j) {
-            const uint32_t v = bcast<KL>(t0, j), w = bcast<KL>(w0, j);
-            if (d != INF) L[v * KL + lane] = min(L[v * KL + lane], sat_add(d, w));
+// acc[j][k] = min(acc[j][k], lab[key][k] + w[j]) over the ELL rows [r0, r1):
+// this thread reads entry 8r + o (its octet o), the key's TK labels of its
+// lanes and the entry's 8 weights.  Two rows per step: min(acc, x0, x1) is
+// one v_min3.  key_base offsets the keys (segment-local tails).
+template <int TK>
+__device__ __forceinline__ void ell_rows(uint32_t (&acc)[8][TK], uint32_t r0, uint32_t r1, uint32_t o,
+                                         uint32_t lane0, const uint32_t *__restrict__ key,
+                                         const uint32_t *__restrict__ wts, const uint32_t *__restrict__ lab,
+                                         uint32_t key_base, uint32_t KL) {
+    uint32_t r = r0;
+    for (; r + 2 <= r1; r += 2) {
+        const size_t e0 = (size_t)r * 8 + o, e1 = e0 + 8;
+        const uint32_t u0 = key[e0] + key_base, u1 = key[e1] + key_base;
+        uint32_t w0[8], w1[8], a0[TK], a1[TK];
+        load_u32<8>(wts + e0 * 8, w0);
+        load_u32<8>(wts + e1 * 8, w1);
+        load_u32<TK>(lab + (size_t)u0 * KL + lane0, a0);
+        load_u32<TK>(lab + (size_t)u1 * KL + lane0, a1);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int k = 0; k < TK; ++k)
+                acc[j][k] = min(acc[j][k], min(sat_add(a0[k], w0[j]), sat_add(a1[k], w1[j])));
+    }
+    if (r < r1) {
+        const size_t e0 = (size_t)r * 8 + o;
+        const uint32_t u0 = key[e0] + key_base;
+        uint32_t w0[8], a0[TK];
+        load_u32<8>(wts + e0 * 8, w0);
+        load_u32<TK>(lab + (size_t)u0 * KL + lane0, a0);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int k = 0; k < TK; ++k) acc[j][k] = min(acc[j][k], sat_add(a0[k], w0[j]));
+    }
+}
+
+// Intra-segment relaxation over 8x8 blocks, octet by octet (DOWN: first to
+// last, UP: last to first).  Step a: the threads of octet a finish it with
+// its diagonal block (a sequential pass over its 8 nodes in registers) and
+// publish its labels in LDS; after one barrier every octet still to come
+// applies block (a, its octet).  xb is double-buffered, so one barrier per
+// step suffices.
+template <int KL, bool DOWN>
+__device__ __forceinline__ void tile_triangle(uint32_t (&acc)[8][KL / 8], uint32_t (*xb)[8][KL], uint32_t len,
+                                              uint32_t ov, uint32_t lane0, const uint32_t *__restrict__ tab,
+                                              const uint32_t *__restrict__ tw) {
+    constexpr int TK = KL / 8;
+    const uint32_t nb = (len + 7) / 8;
+    for (uint32_t step = 0; step < nb; ++step) {
+        const uint32_t a = DOWN ? step : nb - 1 - step;
+        uint32_t(*x)[KL] = xb[step & 1];
+        if (ov == a) {
+            const uint32_t k = tab[32 * a + a];
+            if (k != NONE) {
+                const uint32_t *W = tw + (size_t)k * 64;
+#pragma unroll
+                for (int ii = 0; ii < 8; ++ii) {
+                    const int i = DOWN ? ii : 7 - ii;
+                    uint32_t w[8];
+                    load_u32<8>(W + 8 * i, w);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        if (DOWN ? j <= i : j >= i) continue;
+#pragma unroll
+                        for (int q = 0; q < TK; ++q) acc[j][q] = min(acc[j][q], sat_add(acc[i][q], w[j]));
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) store_u32<TK>(&x[i][lane0], acc[i]);
         }
         __syncthreads();
-        t0 = t1;
-        w0 = w1;
-        n0 = n1;
+        if ((DOWN ? ov > a : ov < a) && ov < nb) {
+            const uint32_t k = tab[32 * a + ov];
+            if (k != NONE) {
+                const uint32_t *W = tw + (size_t)k * 64;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    uint32_t w[8], d[TK];
+                    load_u32<8>(W + 8 * i, w);
+                    load_u32<TK>(&x[i][lane0], d);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+#pragma unroll
+                        for (int q = 0; q < TK; ++q) acc[j][q] = min(acc[j][q], sat_add(d[q], w[j]));
+                }
+            }
+        }
     }
 }
 
-// ---------------------------------------------------------------------------
-// up_seg: segments [seg0, seg0 + gridDim.x) of one segment level, group
-// blockIdx.y.  The segment's labels (pushed from below) go to LDS; upward
-// relaxation inside the segment runs from the last node (lowest rank) to the
-// first; then every UP arc leaving the segment is reduced per target in LDS
-// and pushed with one atomic min per (target, lane).
-// ---------------------------------------------------------------------------
-template <int KL, int SEG, int NTH>
-__global__ __launch_bounds__(NTH) void up_seg_kernel(
-    uint32_t seg0, int push_out, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ su_off,
-    const uint16_t *__restrict__ su_tgt, const uint32_t *__restrict__ su_w, const uint32_t *__restrict__ ut_off, const uint32_t *__restrict__ ut_tgt,
-    const uint32_t *__restrict__ ut_aoff, const uint16_t *__restrict__ uta_x, const uint32_t *__restrict__ uta_w,
-    uint32_t *__restrict__ arena, uint32_t n) {
-    constexpr int NS = NTH / KL;
-    __shared__ uint32_t L[SEG * KL];
-    __shared__ uint32_t soff[SEG + 1];
-    const uint32_t s = seg0 + blockIdx.x, g = blockIdx.y;
-    const uint32_t lane = threadIdx.x % KL, slot = slot_id<KL>();
+template <int KL>
+__device__ __forceinline__ void tile_load(uint32_t (&acc)[8][KL / 8], const uint32_t *__restrict__ lab, uint32_t p0,
+                                          uint32_t len, uint32_t ov, uint32_t lane0) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t i = ov * 8 + j;
+        if (i < len) {
+            load_u32<KL / 8>(lab + (size_t)(p0 + i) * KL + lane0, acc[j]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < KL / 8; ++q) acc[j][q] = INF;
+        }
+    }
+}
+
+template <int KL>
+__device__ __forceinline__ void tile_store(const uint32_t (&acc)[8][KL / 8], uint32_t *__restrict__ lab, uint32_t p0,
+                                           uint32_t len, uint32_t ov, uint32_t lane0) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t i = ov * 8 + j;
+        if (i < len) store_u32<KL / 8>(lab + (size_t)(p0 + i) * KL + lane0, acc[j]);
+    }
+}
+
+// down_tile: group blockIdx.x, segment seg0 + blockIdx.y of one segment level.
+// The tile starts from the upward labels, takes the DOWN arcs from outside the
+// segment (ELL rows of its wave block) and then the intra-segment blocks.
+template <int KL>
+__global__ __launch_bounds__(TILE_THREADS) void down_tile_kernel(
+    uint32_t seg0, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ xr_off,
+    const uint32_t *__restrict__ xr_src, const uint32_t *__restrict__ xr_w, const uint32_t *__restrict__ tb_dn,
+    const uint32_t *__restrict__ tb_w, uint32_t *__restrict__ arena, uint32_t n) {
+    constexpr int TK = KL / 8;
+    __shared__ __align__(16) uint32_t xb[2][8][KL];
+    const uint32_t g = blockIdx.x, s = seg0 + blockIdx.y;
+    const uint32_t ov = threadIdx.x / 8, lane0 = (threadIdx.x % 8) * TK;
+    const uint32_t o = ov % 8, wb = ov / 8;
     const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
     uint32_t *lab = arena + (size_t)g * n * KL;
-    for (uint32_t i = threadIdx.x; i <= len; i += NTH) soff[i] = su_off[p0 + i];
+    uint32_t acc[8][TK];
+    tile_load<KL>(acc, lab, p0, len, ov, lane0);
+    ell_rows<TK>(acc, xr_off[4 * s + wb], xr_off[4 * s + wb + 1], o, lane0, xr_src, xr_w, lab, 0, KL);
+    tile_triangle<KL, true>(acc, xb, len, ov, lane0, tb_dn + (size_t)s * 1024, tb_w);
+    tile_store<KL>(acc, lab, p0, len, ov, lane0);
+}
+
+// up_tile: group blockIdx.x, segment seg0 + blockIdx.y of one segment level
+// (levels run deepest first).  The tile holds what lower levels pushed; a
+// group none of whose lanes reaches the segment stops here.  Intra-segment
+// UP blocks run last to first; then, if push_out, the UP arcs leaving the
+// segment: each wavefront takes target blocks of 64 targets, reduces their
+// ELL rows in registers and pushes one atomic min per finite (target, lane).
+template <int KL>
+__global__ __launch_bounds__(TILE_THREADS) void up_tile_kernel(
+    uint32_t seg0, int push_out, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ tb_up,
+    const uint32_t *__restrict__ tb_w, const uint32_t *__restrict__ pt_blk, const uint32_t *__restrict__ pt_tgt,
+    const uint32_t *__restrict__ pr_off, const uint32_t *__restrict__ pr_x, const uint32_t *__restrict__ pr_w,
+    uint32_t *__restrict__ arena, uint32_t n) {
+    constexpr int TK = KL / 8;
+    __shared__ __align__(16) uint32_t xb[2][8][KL];
+    const uint32_t g = blockIdx.x, s = seg0 + blockIdx.y;
+    const uint32_t ov = threadIdx.x / 8, lane0 = (threadIdx.x % 8) * TK;
+    const uint32_t o = ov % 8, wb = ov / 8;
+    const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
+    uint32_t *lab = arena + (size_t)g * n * KL;
+    uint32_t acc[8][TK];
+    tile_load<KL>(acc, lab, p0, len, ov, lane0);
     int any = 0;
-    for (uint32_t i = slot; i < len; i += NS) {
-        const uint32_t v = lab[(size_t)(p0 + i) * KL + lane];
-        L[i * KL + lane] = v;
-        any |= (v != INF);
-    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+        for (int q = 0; q < TK; ++q) any |= acc[j][q] != INF;
     if (!__syncthreads_or(any)) return;  // no lane of this group reaches the segment
-    chain_relax<KL, SEG, NTH, false>(L, soff, len, su_tgt, su_w, slot, lane);
-    for (uint32_t i = slot; i < len; i += NS) lab[(size_t)(p0 + i) * KL + lane] = L[i * KL + lane];
+    tile_triangle<KL, false>(acc, xb, len, ov, lane0, tb_up + (size_t)s * 1024, tb_w);
+    tile_store<KL>(acc, lab, p0, len, ov, lane0);
     if (!push_out) return;
-    const uint32_t k0 = ut_off[s], k1 = ut_off[s + 1];
-    if constexpr (KL >= 64) {
-        const uint32_t wl = __lane_id();
-        const uint32_t mine = k1 > k0 + slot ? (k1 - k0 - slot + NS - 1) / NS : 0u;
-        for (uint32_t mb = 0; mb < mine; mb += 64) {
-            const uint32_t cntk = min(64u, mine - mb);
-            uint32_t mt = 0, ma0 = 0, ma1 = 0;
-            if (wl < cntk) {
-                const uint32_t k = k0 + slot + (mb + wl) * NS;
-                mt = ut_tgt[k];
-                ma0 = ut_aoff[k];
-                ma1 = ut_aoff[k + 1];
-            }
-            for (uint32_t j = 0; j < cntk; ++j) {
-                const uint32_t a0 = (uint32_t)__builtin_amdgcn_readlane((int)ma0, (int)j);
-                const uint32_t a1 = (uint32_t)__builtin_amdgcn_readlane((int)ma1, (int)j);
-                const uint32_t y = (uint32_t)__builtin_amdgcn_readlane((int)mt, (int)j);
-                const uint32_t m = reduce_arcs<KL>(L, a0, a1, uta_x, uta_w, lane);
-                if (m != INF) atomicMin(lab + (size_t)y * KL + lane, m);
-            }
-        }
-    } else {
-        for (uint32_t k = k0 + slot; k < k1; k += NS) {
-            const uint32_t m = reduce_arcs<KL>(L, ut_aoff[k], ut_aoff[k + 1], uta_x, uta_w, lane);
-            if (m != INF) atomicMin(lab + (size_t)ut_tgt[k] * KL + lane, m);
+    __syncthreads();  // the segment's final labels, written above, are read below by every wavefront
+    for (uint32_t kb = pt_blk[s] + wb; kb < pt_blk[s + 1]; kb += TILE_THREADS / 64) {
+        uint32_t red[8][TK];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int q = 0; q < TK; ++q) red[j][q] = INF;
+        ell_rows<TK>(red, pr_off[kb], pr_off[kb + 1], o, lane0, pr_x, pr_w, lab, p0, KL);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t t = pt_tgt[(size_t)kb * 64 + o * 8 + j];
+            if (t == NONE) continue;
+#pragma unroll
+            for (int q = 0; q < TK; ++q)
+                if (red[j][q] != INF) atomicMin(lab + (size_t)t * KL + lane0 + q, red[j][q]);
         }
     }
 }
@@ -429,34 +510,6 @@ __device__ __forceinline__ void relax_sources(uint32_t *L, const uint32_t *__res
             for (int r = 0; r < UNR; ++r) relax_arcs<KL>(L, d[r], a0[r], a1[r], av, aw, lane);
         }
     }
-}
-
-// ---------------------------------------------------------------------------
-// down_seg: segments [seg0, seg0 + gridDim.x) of one segment level, group
-// blockIdx.y.  Labels in LDS seeded with the upward label; DOWN arcs from
-// outside the segment grouped by source (each source label read once per
-// segment); then the intra-segment arcs, node by node.
-// ---------------------------------------------------------------------------
-template <int KL, int SEG, int NTH>
-__global__ __launch_bounds__(NTH) void down_seg_kernel(
-    uint32_t seg0, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ bs_off,
-    const uint32_t *__restrict__ bs_src, const uint32_t *__restrict__ bs_aoff, const uint16_t *__restrict__ ba_v,
-    const uint32_t *__restrict__ ba_w, const uint32_t *__restrict__ sd_off, const uint16_t *__restrict__ sd_tgt,
-    const uint32_t *__restrict__ sd_w, uint32_t *__restrict__ arena, uint32_t n) {
-    constexpr int NS = NTH / KL;
-    __shared__ uint32_t L[SEG * KL];
-    __shared__ uint32_t soff[SEG + 1];
-    const uint32_t s = seg0 + blockIdx.x, g = blockIdx.y;
-    const uint32_t lane = threadIdx.x % KL, slot = slot_id<KL>();
-    const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
-    uint32_t *lab = arena + (size_t)g * n * KL;
-    for (uint32_t i = threadIdx.x; i <= len; i += NTH) soff[i] = sd_off[p0 + i];
-    for (uint32_t i = slot; i < len; i += NS) L[i * KL + lane] = lab[(size_t)(p0 + i) * KL + lane];
-    __syncthreads();
-    relax_sources<KL, NS>(L, lab, bs_off[s], bs_off[s + 1], bs_src, bs_aoff, ba_v, ba_w, slot, lane);
-    __syncthreads();
-    chain_relax<KL, SEG, NTH, true>(L, soff, len, sd_tgt, sd_w, slot, lane);
-    for (uint32_t i = slot; i < len; i += NS) lab[(size_t)(p0 + i) * KL + lane] = L[i * KL + lane];
 }
 
 // ---------------------------------------------------------------------------
@@ -628,7 +681,7 @@ struct DevBuf {
     }
 };
 
-constexpr int SEG = 256;  // segment limit; LDS = SEG * KL * 4 bytes per chain workgroup
+constexpr int SEG = 256;  // nodes of a segment tile and of a bottom chunk
 
 }  // namespace
 
@@ -642,12 +695,11 @@ struct bfg_ctx {
     bool cap_fixed = false;
     // resident graph
     DevBuf<uint32_t> parent, pos_parent, pos_pre, pos_sub, pos_depth, seg_pos;
-    DevBuf<uint32_t> bs_off, bs_src, bs_aoff, ba_w, sd_off, sd_w, su_off, su_w, in_off, in_split, in_src, in_w;
-    DevBuf<uint16_t> sd_tgt, su_tgt;
+    DevBuf<uint32_t> in_off, in_split, in_src, in_w;
     DevBuf<uint32_t> ch_off, ch_pos, cs_off, cs_src, cs_aoff, ca_w;
     DevBuf<uint16_t> ca_v;
-    DevBuf<uint32_t> up_off, up_tgt, up_w, ut_off, ut_tgt, ut_aoff, uta_w;
-    DevBuf<uint16_t> ba_v, uta_x;
+    DevBuf<uint32_t> up_off, up_tgt, up_w;
+    DevBuf<uint32_t> xr_off, xr_src, xr_w, tb_dn, tb_up, tb_w, pt_blk, pt_tgt, pr_off, pr_x, pr_w;
     // arenas (per batch)
     DevBuf<uint32_t> arena, pathbuf;
     DevBuf<uint32_t> seed_off, seed_rank, seed_pos, seed_cost, seed_depth, seed_pre, lane_limit, row_map;
@@ -726,18 +778,17 @@ int launch_phases(bfg_ctx *c, uint32_t nl, float *up_ms, float *sweep_ms, uint32
                        D1, c->arena.p, n);
     for (int l = (int)nsl - 1; l >= 0; --l) {
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
-        hipLaunchKernelGGL((up_seg_kernel<KL, SEG, NTB>), dim3(s1 - s0, groups), dim3(NTB), 0, s, s0,
-                           l > 0 ? 1 : 0, c->seg_pos.p, c->su_off.p, c->su_tgt.p, c->su_w.p, c->ut_off.p, c->ut_tgt.p,
-                           c->ut_aoff.p, c->uta_x.p, c->uta_w.p, c->arena.p, n);
+        hipLaunchKernelGGL(up_tile_kernel<KL>, dim3(groups, s1 - s0), dim3(TILE_THREADS), 0, s, s0, l > 0 ? 1 : 0,
+                           c->seg_pos.p, c->tb_up.p, c->tb_w.p, c->pt_blk.p, c->pt_tgt.p, c->pr_off.p, c->pr_x.p,
+                           c->pr_w.p, c->arena.p, n);
         ++nlaunch;
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[1], s));
     for (uint32_t l = 0; l < nsl; ++l) {
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
-        hipLaunchKernelGGL((down_seg_kernel<KL, SEG, NTB>), dim3(s1 - s0, groups), dim3(NTB), 0, s, s0, c->seg_pos.p,
-                           c->bs_off.p, c->bs_src.p, c->bs_aoff.p, c->ba_v.p, c->ba_w.p, c->sd_off.p, c->sd_tgt.p,
-                           c->sd_w.p, c->arena.p, n);
+        hipLaunchKernelGGL(down_tile_kernel<KL>, dim3(groups, s1 - s0), dim3(TILE_THREADS), 0, s, s0, c->seg_pos.p,
+                           c->xr_off.p, c->xr_src.p, c->xr_w.p, c->tb_dn.p, c->tb_w.p, c->arena.p, n);
         ++nlaunch;
     }
     const uint32_t nch = (uint32_t)L.ch_off.size() - 1;
@@ -1009,35 +1060,32 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     HIP_TRY(c->cs_aoff.upload(L.cs_aoff, s));
     HIP_TRY(c->ca_v.upload(L.ca_v, s));
     HIP_TRY(c->ca_w.upload(L.ca_w, s));
-    HIP_TRY(c->bs_off.upload(L.bs_off, s));
-    HIP_TRY(c->bs_src.upload(L.bs_src, s));
-    HIP_TRY(c->bs_aoff.upload(L.bs_aoff, s));
-    HIP_TRY(c->ba_v.upload(L.ba_v, s));
-    HIP_TRY(c->ba_w.upload(L.ba_w, s));
-    HIP_TRY(c->sd_off.upload(L.sd_off, s));
-    HIP_TRY(c->sd_tgt.upload(L.sd_tgt, s));
-    HIP_TRY(c->sd_w.upload(L.sd_w, s));
-    HIP_TRY(c->su_off.upload(L.su_off, s));
-    HIP_TRY(c->su_tgt.upload(L.su_tgt, s));
-    HIP_TRY(c->su_w.upload(L.su_w, s));
     HIP_TRY(c->up_off.upload(L.up_off, s));
     HIP_TRY(c->up_tgt.upload(L.up_tgt, s));
     HIP_TRY(c->up_w.upload(L.up_w, s));
-    HIP_TRY(c->ut_off.upload(L.ut_off, s));
-    HIP_TRY(c->ut_tgt.upload(L.ut_tgt, s));
-    HIP_TRY(c->ut_aoff.upload(L.ut_aoff, s));
-    HIP_TRY(c->uta_x.upload(L.uta_x, s));
-    HIP_TRY(c->uta_w.upload(L.uta_w, s));
+    HIP_TRY(c->xr_off.upload(L.xr_off, s));
+    HIP_TRY(c->xr_src.upload(L.xr_src, s));
+    HIP_TRY(c->xr_w.upload(L.xr_w, s));
+    HIP_TRY(c->tb_dn.upload(L.tb_dn, s));
+    HIP_TRY(c->tb_up.upload(L.tb_up, s));
+    HIP_TRY(c->tb_w.upload(L.tb_w, s));
+    HIP_TRY(c->pt_blk.upload(L.pt_blk, s));
+    HIP_TRY(c->pt_tgt.upload(L.pt_tgt, s));
+    HIP_TRY(c->pr_off.upload(L.pr_off, s));
+    HIP_TRY(c->pr_x.upload(L.pr_x, s));
+    HIP_TRY(c->pr_w.upload(L.pr_w, s));
     HIP_TRY(hipStreamSynchronize(s));
-    c->graph_bytes = 4ull * (6ull * L.n + L.seg_pos.size() + L.bs_off.size() + 2ull * L.bs_src.size() +
-                             L.ba_w.size() + L.sd_off.size() + L.sd_w.size() + L.su_off.size() + L.su_w.size() +
-                             (L.n + 1) + 2ull * L.up_tgt.size() + L.ut_off.size() + 2ull * L.ut_tgt.size() +
-                             L.uta_w.size() + 2ull * (L.n + 1) + 2ull * L.in_src.size() + L.ch_off.size() +
-                             L.ch_pos.size() + L.cs_off.size() + 2ull * L.cs_src.size() + L.ca_w.size()) +
-                      2ull * (L.ba_v.size() + L.uta_x.size() + L.ca_v.size() + L.sd_tgt.size() + L.su_tgt.size());
+    c->graph_bytes = 0;
+    for (auto *b : {&c->parent, &c->pos_parent, &c->pos_pre, &c->pos_sub, &c->pos_depth, &c->seg_pos, &c->in_off,
+                    &c->in_split, &c->in_src, &c->in_w, &c->ch_off, &c->ch_pos, &c->cs_off, &c->cs_src, &c->cs_aoff,
+                    &c->ca_w, &c->up_off, &c->up_tgt, &c->up_w, &c->xr_off, &c->xr_src, &c->xr_w, &c->tb_dn,
+                    &c->tb_up, &c->tb_w, &c->pt_blk, &c->pt_tgt, &c->pr_off, &c->pr_x, &c->pr_w})
+        c->graph_bytes += b->n * sizeof(uint32_t);
+    c->graph_bytes += c->ca_v.n * sizeof(uint16_t);
     // host copies the kernels alone need
     for (auto *v : {&L.in_src, &L.in_w, &L.up_tgt, &L.up_w, &L.bs_src, &L.bs_aoff, &L.ba_w, &L.ut_tgt, &L.ut_aoff,
-                    &L.uta_w, &L.sd_w, &L.su_w, &L.cs_src, &L.cs_aoff, &L.ca_w, &L.ch_pos})
+                    &L.uta_w, &L.sd_w, &L.su_w, &L.cs_src, &L.cs_aoff, &L.ca_w, &L.ch_pos, &L.xr_src, &L.xr_w,
+                    &L.tb_dn, &L.tb_up, &L.tb_w, &L.pt_tgt, &L.pr_off, &L.pr_x, &L.pr_w})
         std::vector<uint32_t>().swap(*v);
     for (auto *v : {&L.ba_v, &L.uta_x, &L.ca_v, &L.sd_tgt, &L.su_tgt}) std::vector<uint16_t>().swap(*v);
     *out = c.release();

@@ -9,6 +9,73 @@ namespace bfg {
 
 namespace {
 template <class T>
+void prefix(std::vector<T> &v);
+
+struct Arc3 {
+    uint32_t key, col, w;  // key: source (or tail), col: column in a 64-wide block
+};
+
+// Appends the ELL rows of one 64-column block (columns grouped in octets)
+// from arcs (key, column, weight).  A key with arcs into at least half of the
+// block's octets gets a row of its own (one label row per ELL row, so the
+// label read coalesces); the other keys are packed per octet.  Parallel arcs
+// keep the minimum weight.  Returns the number of rows.
+uint32_t build_ell(std::vector<Arc3> &arcs, uint32_t ncols, std::vector<uint32_t> &ekey,
+                   std::vector<uint32_t> &ew) {
+    if (arcs.empty()) return 0;
+    std::sort(arcs.begin(), arcs.end(), [](const Arc3 &a, const Arc3 &b) {
+        return a.key != b.key ? a.key < b.key : a.col < b.col;
+    });
+    const uint32_t n_oct = (ncols + 7) / 8;
+    struct Ent { uint32_t key; uint32_t w[8]; };
+    std::vector<Ent> packed[8];
+    uint32_t rows = 0;
+    size_t i = 0;
+    while (i < arcs.size()) {
+        const uint32_t key = arcs[i].key;
+        uint32_t w[64];
+        std::fill(w, w + 64, INF);
+        uint32_t mask = 0;
+        for (; i < arcs.size() && arcs[i].key == key; ++i) {
+            const uint32_t c = arcs[i].col;
+            w[c] = std::min(w[c], arcs[i].w);
+            mask |= 1u << (c / 8);
+        }
+        if (2u * (uint32_t)__builtin_popcount(mask) >= n_oct) {
+            for (uint32_t o = 0; o < 8; ++o) {
+                ekey.push_back(key);
+                for (uint32_t j = 0; j < 8; ++j) ew.push_back(w[8 * o + j]);
+            }
+            ++rows;
+        } else {
+            for (uint32_t o = 0; o < 8; ++o)
+                if (mask >> o & 1u) {
+                    Ent e{key, {}};
+                    std::copy(w + 8 * o, w + 8 * o + 8, e.w);
+                    packed[o].push_back(e);
+                }
+        }
+    }
+    size_t np = 0;
+    for (auto &p : packed) np = std::max(np, p.size());
+    for (size_t r = 0; r < np; ++r) {
+        uint32_t pad = INF;
+        for (auto &p : packed)
+            if (r < p.size()) { pad = p[r].key; break; }
+        for (uint32_t o = 0; o < 8; ++o) {
+            if (r < packed[o].size()) {
+                ekey.push_back(packed[o][r].key);
+                for (uint32_t j = 0; j < 8; ++j) ew.push_back(packed[o][r].w[j]);
+            } else {
+                ekey.push_back(pad);
+                for (uint32_t j = 0; j < 8; ++j) ew.push_back(INF);
+            }
+        }
+    }
+    return rows + (uint32_t)np;
+}
+
+template <class T>
 void prefix(std::vector<T> &v) {  // exclusive prefix sum in place
     T run = 0;
     for (auto &x : v) { T c = x; x = run; run += c; }
@@ -385,6 +452,88 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
                 }
             }
     }
+    // ---- dense tiles of the top region ----
+    {
+        const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
+        if (L.max_segment > 256) { err = "segment_limit above 256"; return false; }
+        L.xr_off.assign(4 * (size_t)nseg + 1, 0);
+        L.tb_dn.assign(1024 * (size_t)nseg, NONE);
+        L.tb_up.assign(1024 * (size_t)nseg, NONE);
+        L.pt_blk.assign(nseg + 1, 0);
+        L.xr_src.clear(); L.xr_w.clear(); L.tb_w.clear();
+        L.pt_tgt.clear(); L.pr_off.clear(); L.pr_x.clear(); L.pr_w.clear();
+        std::vector<Arc3> a3;
+        std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> outs;  // (target, tail offset, weight)
+        std::vector<uint32_t> colv;                                   // target column of each arc of outs
+        uint64_t xrows = 0, prows = 0;
+        auto put = [&](std::vector<uint32_t> &tab, uint32_t s, uint32_t a, uint32_t o, uint32_t i, uint32_t j,
+                       uint32_t w) {
+            uint32_t &k = tab[1024 * (size_t)s + 32 * a + o];
+            if (k == NONE) {
+                k = (uint32_t)(L.tb_w.size() / 64);
+                L.tb_w.resize(L.tb_w.size() + 64, INF);
+            }
+            uint32_t &slot = L.tb_w[64 * (size_t)k + 8 * i + j];
+            slot = std::min(slot, w);
+        };
+        for (uint32_t s = 0; s < nseg; ++s) {
+            const uint32_t p0 = L.seg_pos[s], len = L.seg_pos[s + 1] - p0;
+            for (uint32_t b = 0; b < 4; ++b) {
+                L.xr_off[4 * (size_t)s + b] = (uint32_t)xrows;
+                if (64 * b >= len) continue;
+                const uint32_t ncols = std::min<uint32_t>(64, len - 64 * b);
+                a3.clear();
+                for (uint32_t j = 0; j < ncols; ++j) {
+                    const uint32_t p = p0 + 64 * b + j;
+                    for (uint32_t k = L.in_off[p]; k < L.in_split[p]; ++k) a3.push_back({L.in_src[k], j, L.in_w[k]});
+                }
+                xrows += build_ell(a3, ncols, L.xr_src, L.xr_w);
+            }
+            for (uint32_t i = 0; i < len; ++i) {
+                const uint32_t p = p0 + i;
+                for (uint32_t k = L.in_split[p]; k < L.in_off[p + 1]; ++k) {
+                    const uint32_t ti = L.in_src[k] - p0;  // tail before head: ti < i
+                    put(L.tb_dn, s, ti / 8, i / 8, ti % 8, i % 8, L.in_w[k]);
+                }
+                for (uint32_t k = L.up_split[p]; k < L.up_off[p + 1]; ++k) {
+                    const uint32_t t = L.up_tgt[k];  // head offset, t < i
+                    put(L.tb_up, s, i / 8, t / 8, i % 8, t % 8, L.up_w[k]);
+                }
+            }
+            outs.clear();
+            for (uint32_t i = 0; i < len; ++i)
+                for (uint32_t k = L.up_off[p0 + i]; k < L.up_split[p0 + i]; ++k) outs.emplace_back(L.up_tgt[k], i, L.up_w[k]);
+            std::sort(outs.begin(), outs.end());
+            L.pt_blk[s] = (uint32_t)(L.pt_tgt.size() / 64);
+            const size_t t0 = L.pt_tgt.size();
+            for (size_t q = 0; q < outs.size(); ++q)
+                if (q == 0 || std::get<0>(outs[q]) != std::get<0>(outs[q - 1])) L.pt_tgt.push_back(std::get<0>(outs[q]));
+            const uint32_t nt = (uint32_t)(L.pt_tgt.size() - t0);
+            L.pt_tgt.resize(t0 + (size_t)(nt + 63) / 64 * 64, NONE);
+            colv.resize(outs.size());
+            for (size_t q = 0, c = 0; q < outs.size(); ++q) {
+                if (q > 0 && std::get<0>(outs[q]) != std::get<0>(outs[q - 1])) ++c;
+                colv[q] = (uint32_t)c;
+            }
+            size_t q = 0;
+            for (uint32_t kb = 0; kb * 64 < nt; ++kb) {
+                L.pr_off.push_back((uint32_t)prows);
+                a3.clear();
+                for (; q < outs.size() && colv[q] < 64 * (kb + 1); ++q)
+                    a3.push_back({std::get<1>(outs[q]), colv[q] - 64 * kb, std::get<2>(outs[q])});
+                prows += build_ell(a3, std::min<uint32_t>(64, nt - 64 * kb), L.pr_x, L.pr_w);
+            }
+        }
+        L.xr_off[4 * (size_t)nseg] = (uint32_t)xrows;
+        L.pt_blk[nseg] = (uint32_t)(L.pt_tgt.size() / 64);
+        L.pr_off.push_back((uint32_t)prows);
+        L.ell_down_entries = L.xr_w.size();
+        L.ell_up_entries = L.pr_w.size();
+        if (L.xr_w.size() >= (1ull << 32) || L.pr_w.size() >= (1ull << 32) || L.tb_w.size() >= (1ull << 32)) {
+            err = "dense tiles exceed 2^32 weights";
+            return false;
+        }
+    }
     return true;
 }
 
@@ -480,6 +629,112 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
             if (!(y < L.n_top && stage[y] < stage[p0])) { err = "segment up target not an earlier top node"; return false; }
             for (uint32_t a = L.ut_aoff[k]; a < L.ut_aoff[k + 1]; ++a)
                 if (L.uta_x[a] >= p1 - p0) { err = "segment up arc tail out of segment"; return false; }
+        }
+    }
+    // dense tiles: the finite entries of each table are exactly the arcs they
+    // replace (parallel arcs merged to their minimum), keys are readable and
+    // final before the segment's stage
+    {
+        using T3 = std::tuple<uint32_t, uint32_t, uint32_t>;
+        auto merged = [](std::vector<T3> &v) {  // sort, keep the minimum weight per (a, b)
+            std::sort(v.begin(), v.end());
+            size_t o = 0;
+            for (size_t i = 0; i < v.size(); ++i)
+                if (o == 0 || std::get<0>(v[i]) != std::get<0>(v[o - 1]) || std::get<1>(v[i]) != std::get<1>(v[o - 1]))
+                    v[o++] = v[i];
+            v.resize(o);
+        };
+        std::vector<T3> want, got;
+        const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
+        if (L.xr_off.size() != 4 * (size_t)nseg + 1 || L.pt_blk.size() != nseg + 1 ||
+            L.tb_dn.size() != 1024 * (size_t)nseg || L.tb_up.size() != 1024 * (size_t)nseg) {
+            err = "dense tile tables disagree";
+            return false;
+        }
+        for (uint32_t s = 0; s < nseg; ++s) {
+            const uint32_t p0 = L.seg_pos[s], len = L.seg_pos[s + 1] - p0;
+            if (len > 256) { err = "segment longer than a tile"; return false; }
+            // DOWN from outside: (head offset, source, w)
+            want.clear();
+            got.clear();
+            for (uint32_t i = 0; i < len; ++i)
+                for (uint32_t k = L.in_off[p0 + i]; k < L.in_split[p0 + i]; ++k) want.emplace_back(i, L.in_src[k], L.in_w[k]);
+            for (uint32_t b = 0; b < 4; ++b)
+                for (uint32_t r = L.xr_off[4 * s + b]; r < L.xr_off[4 * s + b + 1]; ++r)
+                    for (uint32_t o = 0; o < 8; ++o) {
+                        const size_t e = 8 * (size_t)r + o;
+                        const uint32_t u = L.xr_src[e];
+                        if (u >= n || stage[u] >= stage[p0]) { err = "tile source not final"; return false; }
+                        for (uint32_t j = 0; j < 8; ++j) {
+                            const uint32_t w = L.xr_w[8 * e + j], h = 64 * b + 8 * o + j;
+                            if (w == INF) continue;
+                            if (h >= len) { err = "tile head beyond the segment"; return false; }
+                            got.emplace_back(h, u, w);
+                        }
+                    }
+            merged(want);
+            std::sort(got.begin(), got.end());
+            if (want != got) { err = "DOWN tiles differ from the arcs they replace"; return false; }
+            // intra: (tail, head, w) for both directions
+            for (int dir = 0; dir < 2; ++dir) {
+                want.clear();
+                got.clear();
+                for (uint32_t i = 0; i < len; ++i) {
+                    if (dir == 0)
+                        for (uint32_t k = L.in_split[p0 + i]; k < L.in_off[p0 + i + 1]; ++k)
+                            want.emplace_back(L.in_src[k] - p0, i, L.in_w[k]);
+                    else
+                        for (uint32_t k = L.up_split[p0 + i]; k < L.up_off[p0 + i + 1]; ++k)
+                            want.emplace_back(i, L.up_tgt[k], L.up_w[k]);
+                }
+                const auto &tab = dir == 0 ? L.tb_dn : L.tb_up;
+                for (uint32_t a = 0; a < 32; ++a)
+                    for (uint32_t o = 0; o < 32; ++o) {
+                        const uint32_t k = tab[1024 * (size_t)s + 32 * a + o];
+                        if (k == NONE) continue;
+                        if ((dir == 0 && a > o) || (dir == 1 && a < o) || 64 * (size_t)k + 64 > L.tb_w.size()) {
+                            err = "bad intra block";
+                            return false;
+                        }
+                        for (uint32_t i = 0; i < 8; ++i)
+                            for (uint32_t j = 0; j < 8; ++j) {
+                                const uint32_t w = L.tb_w[64 * (size_t)k + 8 * i + j];
+                                if (w == INF) continue;
+                                const uint32_t t = 8 * a + i, h = 8 * o + j;
+                                if (t >= len || h >= len || (dir == 0 ? t >= h : t <= h)) {
+                                    err = "intra block arc against the sweep order";
+                                    return false;
+                                }
+                                got.emplace_back(t, h, w);
+                            }
+                    }
+                merged(want);
+                std::sort(got.begin(), got.end());
+                if (want != got) { err = "intra blocks differ from the arcs they replace"; return false; }
+            }
+            // UP leaving: (target, tail, w)
+            want.clear();
+            got.clear();
+            for (uint32_t i = 0; i < len; ++i)
+                for (uint32_t k = L.up_off[p0 + i]; k < L.up_split[p0 + i]; ++k) want.emplace_back(L.up_tgt[k], i, L.up_w[k]);
+            for (uint32_t kb = L.pt_blk[s]; kb < L.pt_blk[s + 1]; ++kb) {
+                if (kb + 1 >= L.pr_off.size() || 64 * (size_t)kb + 64 > L.pt_tgt.size()) { err = "bad target block"; return false; }
+                for (uint32_t r = L.pr_off[kb]; r < L.pr_off[kb + 1]; ++r)
+                    for (uint32_t o = 0; o < 8; ++o) {
+                        const size_t e = 8 * (size_t)r + o;
+                        const uint32_t x = L.pr_x[e];
+                        if (x >= len) { err = "push tail beyond the segment"; return false; }
+                        for (uint32_t j = 0; j < 8; ++j) {
+                            const uint32_t w = L.pr_w[8 * e + j], t = L.pt_tgt[64 * (size_t)kb + 8 * o + j];
+                            if (w == INF) continue;
+                            if (t == NONE || t >= L.n_top || stage[t] >= stage[p0]) { err = "push target not an earlier top node"; return false; }
+                            got.emplace_back(t, x, w);
+                        }
+                    }
+            }
+            merged(want);
+            std::sort(got.begin(), got.end());
+            if (want != got) { err = "UP push tiles differ from the arcs they replace"; return false; }
         }
     }
     return true;
