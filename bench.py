@@ -169,7 +169,10 @@ def main():
     cap = info["lane_capacity"]
     groups_per_step = sum((min(cap, S - b) + KL - 1) // KL for b in range(0, S, cap))
     n, m_dn = info["n_nodes"], info["m_down"]
-    bytes_per_group = 4 * (n + 1) + 8 * m_dn + 8 * KL * n
+    # the subgraph one lane group's sweep covers (the matrix sweep skips the
+    # bottom nodes that are no target's ancestor)
+    n_s, m_s = st.nodes_swept, st.arcs_swept
+    bytes_per_group = 4 * (n_s + 1) + 8 * m_s + 8 * KL * n_s
     sweep_bytes = groups_per_step * bytes_per_group * args.steps
     achieved = sweep_bytes / (sweep_ms / 1000.0) / 1e9 if sweep_ms > 0 else 0.0
 
@@ -196,6 +199,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
                      "kernel": "sweep_kernel", "bytes_per_lane_group": bytes_per_group,
+                     "nodes_swept": n_s, "arcs_swept": m_s,
                      "lane_groups_per_step": groups_per_step, "sweep_launches_per_step": launches // args.steps,
                      "sweep_ms_per_step": sweep_ms / args.steps},
         "phases_ms_per_step": {"up": up_ms / args.steps, "sweep": sweep_ms / args.steps,

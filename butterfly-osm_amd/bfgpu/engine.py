@@ -38,6 +38,8 @@ class BatchedPhastStats:
     total_time_ms: float = 0.0
     batches: int = 0
     sweep_launches: int = 0
+    nodes_swept: int = 0  # nodes / DOWN arcs one lane group's sweep covered
+    arcs_swept: int = 0
 
 
 @dataclasses.dataclass
@@ -92,7 +94,8 @@ class BatchedPhastEngine:
         _check(self._lib.bfg_ctx_last_timings(self._h, C.byref(t)))
         return BatchedPhastStats(n_sources=n_sources, upward_time_ms=t.up_ms, downward_time_ms=t.sweep_ms,
                                  output_time_ms=t.output_ms, total_time_ms=t.total_ms, batches=t.batches,
-                                 sweep_launches=t.sweep_launches)
+                                 sweep_launches=t.sweep_launches, nodes_swept=t.nodes_swept,
+                                 arcs_swept=t.arcs_swept)
 
     def configure(self, lanes=0, lane_capacity=0):
         _check(self._lib.bfg_ctx_configure(self._h, lanes, lane_capacity))

@@ -226,229 +226,156 @@ __device__ __forceinline__ void relax_arcs(uint32_t *L, uint32_t d, uint32_t a0,
 }
 
 // ---------------------------------------------------------------------------
-// Dense tiles of the top region.  A workgroup of 256 threads owns one
-// (segment, group) pair: thread t holds nodes 8*(t/8) .. +8 of the segment
-// (its octet) for lanes TK*(t%8) .. +TK of the group, TK = KL/8, as an 8 x TK
-// register tile.  Wavefront w thus covers the 64 nodes of wave block w.
+// Wave tiles of the top region.  A wavefront carries 64 source lanes (64/KL
+// groups of KL; wave group w holds groups w*64/KL ..) and each thread holds,
+// for its lane, the labels of one block of up to 64 positions in registers.
+// A row (key, 64 weights) is wave-uniform: its key and weights come in
+// through scalar loads, each thread reads the key's label of its lane (one
+// coalesced load per row) and does 64 saturating adds and mins.
 // ---------------------------------------------------------------------------
-constexpr int TILE_THREADS = 256;
+constexpr int TILE_THREADS = 256;  // four wavefronts, four independent tiles
 
-template <int N>
-__device__ __forceinline__ void load_u32(const uint32_t *__restrict__ p, uint32_t (&v)[N]) {
-    static_assert(N == 2 || N % 4 == 0, "vector width");
-    if constexpr (N == 2) {
-        const uint2 x = *reinterpret_cast<const uint2 *>(p);
-        v[0] = x.x;
-        v[1] = x.y;
-    } else {
-#pragma unroll
-        for (int q = 0; q < N / 4; ++q) {
-            const uint4 x = reinterpret_cast<const uint4 *>(p)[q];
-            v[4 * q] = x.x;
-            v[4 * q + 1] = x.y;
-            v[4 * q + 2] = x.z;
-            v[4 * q + 3] = x.w;
+// Read-only tables the kernels index with wave-uniform offsets: read through
+// the constant address space, whose uniform loads become scalar loads.
+typedef __attribute__((address_space(4))) const uint32_t cu32;
+__device__ __forceinline__ const cu32 *as_const(const uint32_t *p) { return (const cu32 *)p; }
+
+// Index of this thread's wavefront in the workgroup, as a scalar (the
+// compiler cannot prove threadIdx.x / 64 uniform by itself).
+__device__ __forceinline__ uint32_t wave_id() {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+}
+
+template <int KL>
+__device__ __forceinline__ uint32_t *lane_labels(uint32_t *arena, uint32_t n, uint32_t wg, uint32_t *group) {
+    const uint32_t l = threadIdx.x % 64;
+    const uint32_t g = wg * (64 / KL) + l / KL;
+    if (group) *group = g;
+    return arena + (size_t)g * n * KL + l % KL;
+}
+
+// acc[j] = min(acc[j], lab[key[r]] + wts[64r + j]) over rows [r0, r1); two
+// rows per step (min(acc, x0, x1) is one v_min3), the labels of the next two
+// rows loaded one step ahead.  An odd last row pairs with itself and INF.
+template <int KL>
+__device__ __forceinline__ void rows64(uint32_t (&acc)[64], uint32_t r0, uint32_t r1,
+                                       const uint32_t *__restrict__ key, const uint32_t *__restrict__ wts,
+                                       const uint32_t *lab) {
+    if (r0 >= r1) return;
+    const cu32 *K = as_const(key);
+    uint32_t n0 = lab[(size_t)K[r0] * KL];
+    uint32_t n1 = r0 + 1 < r1 ? lab[(size_t)K[r0 + 1] * KL] : INF;
+    for (uint32_t r = r0; r < r1; r += 2) {
+        const uint32_t a0 = n0, a1 = n1;
+        const cu32 *W0 = as_const(wts) + (size_t)r * 64;
+        const cu32 *W1 = r + 1 < r1 ? W0 + 64 : W0;
+        if (r + 2 < r1) {
+            n0 = lab[(size_t)K[r + 2] * KL];
+            n1 = r + 3 < r1 ? lab[(size_t)K[r + 3] * KL] : INF;
         }
+#pragma unroll
+        for (int j = 0; j < 64; ++j) acc[j] = min(acc[j], min(sat_add(a0, W0[j]), sat_add(a1, W1[j])));
     }
 }
 
-template <int N>
-__device__ __forceinline__ void store_u32(uint32_t *__restrict__ p, const uint32_t (&v)[N]) {
-    if constexpr (N == 2) {
-        *reinterpret_cast<uint2 *>(p) = make_uint2(v[0], v[1]);
+// rows: wavefront w of workgroup (wave group blockIdx.x, blockIdx.y) takes
+// unit u0 + 4*blockIdx.y + w and merges its rows into the arena with one
+// atomic min per finite (position, lane).
+//   DOWN (PUSH = false): unit = rows [r0[u], r1[u]) of segment seg[u] (DOWN
+//     arcs from outside the segment); positions are the segment's nodes,
+//     which hold the upward labels.
+//   UP (PUSH = true): unit = target block u of the level, rows
+//     [r0[u], r0[u+1]) keyed by tails in segment seg[u] (final up labels);
+//     positions are tgt[64u ..]; a wavefront none of whose groups reaches
+//     the segment stops at once.
+template <int KL, bool PUSH>
+__global__ __launch_bounds__(TILE_THREADS) void rows_kernel(
+    uint32_t u0, uint32_t u1, const uint32_t *__restrict__ seg, const uint32_t *__restrict__ r0,
+    const uint32_t *__restrict__ r1, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ tgt,
+    const uint32_t *__restrict__ key, const uint32_t *__restrict__ wts, const uint8_t *__restrict__ active,
+    uint32_t nseg, uint32_t *__restrict__ arena, uint32_t n) {
+    const uint32_t u = u0 + blockIdx.y * (TILE_THREADS / 64) + wave_id();
+    if (u >= u1) return;  // wave-uniform; this kernel has no barrier
+    uint32_t g;
+    uint32_t *lab = lane_labels<KL>(arena, n, blockIdx.x, &g);
+    const uint32_t s = seg[u];
+    if constexpr (PUSH) {
+        if (__ballot(active[(size_t)g * nseg + s] != 0) == 0) return;
+    }
+    uint32_t acc[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) acc[j] = INF;
+    rows64<KL>(acc, r0[u], PUSH ? r0[u + 1] : r1[u], key, wts, lab);
+    if constexpr (PUSH) {
+        const cu32 *t = as_const(tgt) + (size_t)u * 64;
+#pragma unroll
+        for (int j = 0; j < 64; ++j)
+            if (t[j] != NONE && acc[j] != INF) atomicMin(lab + (size_t)t[j] * KL, acc[j]);
     } else {
+        const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
 #pragma unroll
-        for (int q = 0; q < N / 4; ++q)
-            reinterpret_cast<uint4 *>(p)[q] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+        for (int j = 0; j < 64; ++j)
+            if (j < len && acc[j] != INF) atomicMin(lab + (size_t)(p0 + j) * KL, acc[j]);
     }
 }
 
-// acc[j][k] = min(acc[j][k], lab[key][k] + w[j]) over the ELL rows [r0, r1):
-// this thread reads entry 8r + o (its octet o), the key's TK labels of its
-// lanes and the entry's 8 weights.  Two rows per step: min(acc, x0, x1) is
-// one v_min3.  key_base offsets the keys (segment-local tails).
-template <int TK>
-__device__ __forceinline__ void ell_rows(uint32_t (&acc)[8][TK], uint32_t r0, uint32_t r1, uint32_t o,
-                                         uint32_t lane0, const uint32_t *__restrict__ key,
-                                         const uint32_t *__restrict__ wts, const uint32_t *__restrict__ lab,
-                                         uint32_t key_base, uint32_t KL) {
-    uint32_t r = r0;
-    for (; r + 2 <= r1; r += 2) {
-        const size_t e0 = (size_t)r * 8 + o, e1 = e0 + 8;
-        const uint32_t u0 = key[e0] + key_base, u1 = key[e1] + key_base;
-        uint32_t w0[8], w1[8], a0[TK], a1[TK];
-        load_u32<8>(wts + e0 * 8, w0);
-        load_u32<8>(wts + e1 * 8, w1);
-        load_u32<TK>(lab + (size_t)u0 * KL + lane0, a0);
-        load_u32<TK>(lab + (size_t)u1 * KL + lane0, a1);
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-#pragma unroll
-            for (int k = 0; k < TK; ++k)
-                acc[j][k] = min(acc[j][k], min(sat_add(a0[k], w0[j]), sat_add(a1[k], w1[j])));
-    }
-    if (r < r1) {
-        const size_t e0 = (size_t)r * 8 + o;
-        const uint32_t u0 = key[e0] + key_base;
-        uint32_t w0[8], a0[TK];
-        load_u32<8>(wts + e0 * 8, w0);
-        load_u32<TK>(lab + (size_t)u0 * KL + lane0, a0);
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-#pragma unroll
-            for (int k = 0; k < TK; ++k) acc[j][k] = min(acc[j][k], sat_add(a0[k], w0[j]));
-    }
-}
-
-// Intra-segment relaxation over 8x8 blocks, octet by octet (DOWN: first to
-// last, UP: last to first).  Step a: the threads of octet a finish it with
-// its diagonal block (a sequential pass over its 8 nodes in registers) and
-// publish its labels in LDS; after one barrier every octet still to come
-// applies block (a, its octet).  xb is double-buffered, so one barrier per
-// step suffices.
+// tri: wavefront w of workgroup (wave group blockIdx.x, blockIdx.y) takes
+// segment s0 + 4*blockIdx.y + w of one level and relaxes its intra-segment
+// arcs in registers: 8x8 blocks, DOWN from the first octet to the last (the
+// diagonal block of an octet first, i ascending), UP from the last to the
+// first (i descending).  UP also records active[g * nseg + s]: whether any
+// lane of group g reaches the segment (a wavefront that reaches none stops
+// and leaves the segment's labels INF).
 template <int KL, bool DOWN>
-__device__ __forceinline__ void tile_triangle(uint32_t (&acc)[8][KL / 8], uint32_t (*xb)[8][KL], uint32_t len,
-                                              uint32_t ov, uint32_t lane0, const uint32_t *__restrict__ tab,
-                                              const uint32_t *__restrict__ tw) {
-    constexpr int TK = KL / 8;
-    const uint32_t nb = (len + 7) / 8;
-    for (uint32_t step = 0; step < nb; ++step) {
-        const uint32_t a = DOWN ? step : nb - 1 - step;
-        uint32_t(*x)[KL] = xb[step & 1];
-        if (ov == a) {
-            const uint32_t k = tab[32 * a + a];
-            if (k != NONE) {
-                const uint32_t *W = tw + (size_t)k * 64;
+__global__ __launch_bounds__(TILE_THREADS) void tri_kernel(uint32_t s0, uint32_t s1, uint32_t nseg,
+                                                           const uint32_t *__restrict__ seg_pos,
+                                                           const uint32_t *__restrict__ tab,
+                                                           const uint32_t *__restrict__ tw,
+                                                           uint8_t *__restrict__ active,
+                                                           uint32_t *__restrict__ arena, uint32_t n) {
+    const uint32_t s = s0 + blockIdx.y * (TILE_THREADS / 64) + wave_id();
+    if (s >= s1) return;  // wave-uniform; this kernel has no barrier
+    uint32_t g;
+    uint32_t *lab = lane_labels<KL>(arena, n, blockIdx.x, &g);
+    const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
+    uint32_t acc[64];
 #pragma unroll
-                for (int ii = 0; ii < 8; ++ii) {
-                    const int i = DOWN ? ii : 7 - ii;
-                    uint32_t w[8];
-                    load_u32<8>(W + 8 * i, w);
+    for (int j = 0; j < 64; ++j) acc[j] = j < len ? lab[(size_t)(p0 + j) * KL] : INF;
+    if constexpr (!DOWN) {
+        uint32_t m = INF;
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        if (DOWN ? j <= i : j >= i) continue;
+        for (int j = 0; j < 64; ++j) m = min(m, acc[j]);
+        const uint64_t b = __ballot(m != INF);
+        const uint32_t l = threadIdx.x % 64;
+        const uint64_t gm = KL >= 64 ? ~0ull : ((1ull << KL) - 1) << (l / KL * KL);
+        if (l % KL == 0) active[(size_t)g * nseg + s] = (b & gm) ? 1 : 0;
+        if (b == 0) return;
+    }
+    const cu32 *T = as_const(tab) + (size_t)s * 64;
 #pragma unroll
-                        for (int q = 0; q < TK; ++q) acc[j][q] = min(acc[j][q], sat_add(acc[i][q], w[j]));
-                    }
+    for (int aa = 0; aa < 8; ++aa) {
+        const int a = DOWN ? aa : 7 - aa;
+#pragma unroll
+        for (int oo = 0; oo < 8; ++oo) {
+            const int o = DOWN ? a + oo : a - oo;
+            if (o < 0 || o > 7) continue;
+            const uint32_t k = T[a * 8 + o];
+            if (k == NONE) continue;
+            const cu32 *W = as_const(tw) + (size_t)k * 64;
+#pragma unroll
+            for (int ii = 0; ii < 8; ++ii) {
+                const int i = DOWN ? ii : 7 - ii;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (o == a && (DOWN ? j <= i : j >= i)) continue;
+                    acc[o * 8 + j] = min(acc[o * 8 + j], sat_add(acc[a * 8 + i], W[i * 8 + j]));
                 }
             }
-#pragma unroll
-            for (int i = 0; i < 8; ++i) store_u32<TK>(&x[i][lane0], acc[i]);
-        }
-        __syncthreads();
-        if ((DOWN ? ov > a : ov < a) && ov < nb) {
-            const uint32_t k = tab[32 * a + ov];
-            if (k != NONE) {
-                const uint32_t *W = tw + (size_t)k * 64;
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    uint32_t w[8], d[TK];
-                    load_u32<8>(W + 8 * i, w);
-                    load_u32<TK>(&x[i][lane0], d);
-#pragma unroll
-                    for (int j = 0; j < 8; ++j)
-#pragma unroll
-                        for (int q = 0; q < TK; ++q) acc[j][q] = min(acc[j][q], sat_add(d[q], w[j]));
-                }
-            }
         }
     }
-}
-
-template <int KL>
-__device__ __forceinline__ void tile_load(uint32_t (&acc)[8][KL / 8], const uint32_t *__restrict__ lab, uint32_t p0,
-                                          uint32_t len, uint32_t ov, uint32_t lane0) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const uint32_t i = ov * 8 + j;
-        if (i < len) {
-            load_u32<KL / 8>(lab + (size_t)(p0 + i) * KL + lane0, acc[j]);
-        } else {
-#pragma unroll
-            for (int q = 0; q < KL / 8; ++q) acc[j][q] = INF;
-        }
-    }
-}
-
-template <int KL>
-__device__ __forceinline__ void tile_store(const uint32_t (&acc)[8][KL / 8], uint32_t *__restrict__ lab, uint32_t p0,
-                                           uint32_t len, uint32_t ov, uint32_t lane0) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const uint32_t i = ov * 8 + j;
-        if (i < len) store_u32<KL / 8>(lab + (size_t)(p0 + i) * KL + lane0, acc[j]);
-    }
-}
-
-// down_tile: group blockIdx.x, segment seg0 + blockIdx.y of one segment level.
-// The tile starts from the upward labels, takes the DOWN arcs from outside the
-// segment (ELL rows of its wave block) and then the intra-segment blocks.
-template <int KL>
-__global__ __launch_bounds__(TILE_THREADS) void down_tile_kernel(
-    uint32_t seg0, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ xr_off,
-    const uint32_t *__restrict__ xr_src, const uint32_t *__restrict__ xr_w, const uint32_t *__restrict__ tb_dn,
-    const uint32_t *__restrict__ tb_w, uint32_t *__restrict__ arena, uint32_t n) {
-    constexpr int TK = KL / 8;
-    __shared__ __align__(16) uint32_t xb[2][8][KL];
-    const uint32_t g = blockIdx.x, s = seg0 + blockIdx.y;
-    const uint32_t ov = threadIdx.x / 8, lane0 = (threadIdx.x % 8) * TK;
-    const uint32_t o = ov % 8, wb = ov / 8;
-    const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
-    uint32_t *lab = arena + (size_t)g * n * KL;
-    uint32_t acc[8][TK];
-    tile_load<KL>(acc, lab, p0, len, ov, lane0);
-    ell_rows<TK>(acc, xr_off[4 * s + wb], xr_off[4 * s + wb + 1], o, lane0, xr_src, xr_w, lab, 0, KL);
-    tile_triangle<KL, true>(acc, xb, len, ov, lane0, tb_dn + (size_t)s * 1024, tb_w);
-    tile_store<KL>(acc, lab, p0, len, ov, lane0);
-}
-
-// up_tile: group blockIdx.x, segment seg0 + blockIdx.y of one segment level
-// (levels run deepest first).  The tile holds what lower levels pushed; a
-// group none of whose lanes reaches the segment stops here.  Intra-segment
-// UP blocks run last to first; then, if push_out, the UP arcs leaving the
-// segment: each wavefront takes target blocks of 64 targets, reduces their
-// ELL rows in registers and pushes one atomic min per finite (target, lane).
-template <int KL>
-__global__ __launch_bounds__(TILE_THREADS) void up_tile_kernel(
-    uint32_t seg0, int push_out, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ tb_up,
-    const uint32_t *__restrict__ tb_w, const uint32_t *__restrict__ pt_blk, const uint32_t *__restrict__ pt_tgt,
-    const uint32_t *__restrict__ pr_off, const uint32_t *__restrict__ pr_x, const uint32_t *__restrict__ pr_w,
-    uint32_t *__restrict__ arena, uint32_t n) {
-    constexpr int TK = KL / 8;
-    __shared__ __align__(16) uint32_t xb[2][8][KL];
-    const uint32_t g = blockIdx.x, s = seg0 + blockIdx.y;
-    const uint32_t ov = threadIdx.x / 8, lane0 = (threadIdx.x % 8) * TK;
-    const uint32_t o = ov % 8, wb = ov / 8;
-    const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
-    uint32_t *lab = arena + (size_t)g * n * KL;
-    uint32_t acc[8][TK];
-    tile_load<KL>(acc, lab, p0, len, ov, lane0);
-    int any = 0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-#pragma unroll
-        for (int q = 0; q < TK; ++q) any |= acc[j][q] != INF;
-    if (!__syncthreads_or(any)) return;  // no lane of this group reaches the segment
-    tile_triangle<KL, false>(acc, xb, len, ov, lane0, tb_up + (size_t)s * 1024, tb_w);
-    tile_store<KL>(acc, lab, p0, len, ov, lane0);
-    if (!push_out) return;
-    __syncthreads();  // the segment's final labels, written above, are read below by every wavefront
-    for (uint32_t kb = pt_blk[s] + wb; kb < pt_blk[s + 1]; kb += TILE_THREADS / 64) {
-        uint32_t red[8][TK];
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-#pragma unroll
-            for (int q = 0; q < TK; ++q) red[j][q] = INF;
-        ell_rows<TK>(red, pr_off[kb], pr_off[kb + 1], o, lane0, pr_x, pr_w, lab, p0, KL);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const uint32_t t = pt_tgt[(size_t)kb * 64 + o * 8 + j];
-            if (t == NONE) continue;
-#pragma unroll
-            for (int q = 0; q < TK; ++q)
-                if (red[j][q] != INF) atomicMin(lab + (size_t)t * KL + lane0 + q, red[j][q]);
-        }
-    }
+    for (int j = 0; j < 64; ++j)
+        if (j < len) lab[(size_t)(p0 + j) * KL] = acc[j];
 }
 
 // Relaxes the grouped arcs of sources [k0, k1) into the LDS labels: sources
@@ -552,6 +479,60 @@ __global__ __launch_bounds__(NTH) void bottom_chunk_kernel(
     relax_sources<KL, NS>(L, lab, cs_off[c], cs_off[c + 1], cs_src, cs_aoff, ca_v, ca_w, slot, lane);
     __syncthreads();
     for (uint32_t i = slot; i < len; i += NS) lab[(size_t)ch_pos[i0 + i] * KL + lane] = L[i * KL + lane];
+}
+
+// bottom_path: the bottom region restricted to what a matrix reads.  A
+// target's label depends only on its ancestors, so the bottom sweep of a
+// matrix needs the bottom paths from the top region down to its targets
+// alone.  Slot (KL threads, one per lane) `blockIdx.x * NS + slot` of group
+// blockIdx.y walks path_pos[path_off[k] .. path_off[k+1]) top-down: each node
+// starts from the lane's upward label (its bottom path slots) and pulls over
+// all its DOWN in-arcs.  Top sources are final; bottom sources lie above it
+// on the same path and were written by this same thread.  Paths that share
+// nodes write identical values.
+template <int KL>
+__global__ __launch_bounds__(NT) void bottom_path_kernel(
+    uint32_t n_paths, const uint32_t *__restrict__ path_off, const uint32_t *__restrict__ path_pos,
+    const uint32_t *__restrict__ in_off, const uint32_t *__restrict__ in_src, const uint32_t *__restrict__ in_w,
+    const uint32_t *__restrict__ pos_pre, const uint32_t *__restrict__ pos_sub, const uint32_t *__restrict__ pos_depth,
+    const uint32_t *__restrict__ seed_off, const uint32_t *__restrict__ seed_pre,
+    const uint32_t *__restrict__ seed_depth, const uint32_t *__restrict__ pathbuf, uint32_t D1, uint32_t n_lanes,
+    uint32_t *__restrict__ arena, uint32_t n) {
+    constexpr int NS = NT / KL;
+    const uint32_t lane = threadIdx.x % KL, slot = threadIdx.x / KL;
+    const uint32_t k = blockIdx.x * NS + slot;
+    if (k >= n_paths) return;  // this kernel has no barrier
+    const uint32_t g = blockIdx.y;
+    uint32_t *lab = arena + (size_t)g * n * KL + lane;
+    uint32_t sj0 = 0, sj1 = 0;
+    const uint32_t gl = g * KL + lane;
+    if (gl < n_lanes) {
+        sj0 = seed_off[gl];
+        sj1 = seed_off[gl + 1];
+    }
+    for (uint32_t i = path_off[k]; i < path_off[k + 1]; ++i) {
+        const uint32_t p = path_pos[i];
+        uint32_t m0 = INF, m1 = INF, m2 = INF, m3 = INF;
+        if (sj0 < sj1) {
+            const uint32_t pr = pos_pre[p], ps = pos_sub[p], pd = pos_depth[p];
+            for (uint32_t j = sj0; j < sj1; ++j) {
+                const uint32_t sp = seed_pre[j];
+                if (sp >= pr && sp - pr < ps) m0 = min(m0, pathbuf[(size_t)j * D1 + (seed_depth[j] - pd)]);
+            }
+        }
+        uint32_t a = in_off[p];
+        const uint32_t a1 = in_off[p + 1];
+        for (; a + 4 <= a1; a += 4) {
+            const uint32_t u0 = in_src[a], u1 = in_src[a + 1], u2 = in_src[a + 2], u3 = in_src[a + 3];
+            const uint32_t w0 = in_w[a], w1 = in_w[a + 1], w2 = in_w[a + 2], w3 = in_w[a + 3];
+            m0 = min(m0, sat_add(lab[(size_t)u0 * KL], w0));
+            m1 = min(m1, sat_add(lab[(size_t)u1 * KL], w1));
+            m2 = min(m2, sat_add(lab[(size_t)u2 * KL], w2));
+            m3 = min(m3, sat_add(lab[(size_t)u3 * KL], w3));
+        }
+        for (; a < a1; ++a) m0 = min(m0, sat_add(lab[(size_t)in_src[a] * KL], in_w[a]));
+        lab[(size_t)p * KL] = min(min(m0, m1), min(m2, m3));
+    }
 }
 
 // bottom_pull: bottom nodes [p0, p1) of one height level, group blockIdx.y:
@@ -681,7 +662,8 @@ struct DevBuf {
     }
 };
 
-constexpr int SEG = 256;  // nodes of a segment tile and of a bottom chunk
+constexpr int SEG = 64;     // nodes of a segment (one register tile)
+constexpr int CHUNK = 256;  // nodes of a bottom chunk (LDS = CHUNK * KL * 4 bytes)
 
 }  // namespace
 
@@ -691,7 +673,7 @@ struct bfg_ctx {
     std::mutex mu;
     bfg::Layout L;
     uint32_t lanes = 32;        // KL: sources per lane group (16, 32 or 64)
-    uint32_t lane_cap = 0;      // lanes per batch (multiple of KL)
+    uint32_t lane_cap = 0;      // lanes per batch (multiple of 64)
     bool cap_fixed = false;
     // resident graph
     DevBuf<uint32_t> parent, pos_parent, pos_pre, pos_sub, pos_depth, seg_pos;
@@ -700,12 +682,20 @@ struct bfg_ctx {
     DevBuf<uint16_t> ca_v;
     DevBuf<uint32_t> up_off, up_tgt, up_w;
     DevBuf<uint32_t> xr_off, xr_src, xr_w, tb_dn, tb_up, tb_w, pt_blk, pt_tgt, pr_off, pr_x, pr_w;
+    DevBuf<uint32_t> du_seg, du_r0, du_r1, pk_seg;
     // arenas (per batch)
     DevBuf<uint32_t> arena, pathbuf;
+    DevBuf<uint8_t> active;  // [group][segment] reach flags of the upward phase
     DevBuf<uint32_t> seed_off, seed_rank, seed_pos, seed_cost, seed_depth, seed_pre, lane_limit, row_map;
     size_t seeds_cap = 0, lanes_alloc = 0;
     DevBuf<uint32_t> tgt_pos;
     size_t tgt_cap = 0;
+    // bottom paths above the targets of the current matrix call; while
+    // paths_on, the downward sweep covers the top region and these paths only
+    DevBuf<uint32_t> path_off, path_pos;
+    bool paths_on = false;
+    uint32_t n_paths = 0;
+    uint64_t path_nodes = 0, path_arcs = 0, top_in_arcs = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bfg_timings last{};
     uint64_t graph_bytes = 0;
@@ -740,6 +730,9 @@ int ensure_arena(bfg_ctx *c) {
         c->arena.release();
         HIP_TRY(c->arena.alloc(need));
     }
+    // reach flags for the largest group count a batch of lane_cap lanes can form
+    const size_t flags = ((size_t)c->lane_cap + 15) / 16 * (c->L.seg_pos.size() - 1);
+    if (c->active.n < flags) HIP_TRY(c->active.alloc(flags));
     return BFG_SUCCESS;
 }
 
@@ -749,10 +742,11 @@ uint32_t auto_lane_cap(bfg_ctx *c, uint32_t want) {
     const size_t have = free_b + c->arena.n * sizeof(uint32_t);
     const size_t per_lane = (size_t)c->L.n * 4 + (size_t)d1_of(c) * 4 + 64;
     const size_t budget = (size_t)(have * 0.6);
-    size_t cap = std::max<size_t>(budget / std::max<size_t>(per_lane, 1), c->lanes);
+    // a multiple of 64: the top-region kernels carry 64 lanes per wavefront
+    size_t cap = std::max<size_t>(budget / std::max<size_t>(per_lane, 1), 64);
     cap = std::min<size_t>(cap, 16384);
-    cap = std::min<size_t>(cap, ((size_t)want + c->lanes - 1) / c->lanes * c->lanes);
-    cap = std::max<size_t>(cap / c->lanes * c->lanes, c->lanes);
+    cap = std::min<size_t>(cap, ((size_t)want + 63) / 64 * 64);
+    cap = std::max<size_t>(cap / 64 * 64, 64);
     return (uint32_t)cap;
 }
 
@@ -761,50 +755,78 @@ int launch_phases(bfg_ctx *c, uint32_t nl, float *up_ms, float *sweep_ms, uint32
     const bfg::Layout &L = c->L;
     hipStream_t s = c->stream;
     const uint32_t groups = (nl + KL - 1) / KL;
+    const uint32_t wgs = (nl + 63) / 64;            // wave groups of the top-region kernels
+    const uint32_t tile_groups = wgs * (64 / KL);   // groups those touch (lanes past nl stay INF)
     const uint32_t n = L.n, D1 = d1_of(c);
     const uint32_t nsl = (uint32_t)L.seglvl_seg.size() - 1;
+    const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
     uint32_t nlaunch = 0;
     HIP_TRY(hipEventRecord(c->ev[0], s));
     if (L.n_top) {
         const uint64_t words = (uint64_t)L.n_top * KL;
-        const uint64_t total = words * groups;
+        const uint64_t total = words * tile_groups;
         const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
         hipLaunchKernelGGL(fill_top_kernel, dim3(blocks), dim3(256), 0, s, c->arena.p, (uint64_t)n * KL, words,
-                           groups);
+                           tile_groups);
     }
     hipLaunchKernelGGL(bottom_walk_kernel<KL>, dim3(nl), dim3(64), 0, s, c->seed_off.p, c->seed_rank.p,
                        c->seed_pos.p, c->seed_cost.p, c->seed_depth.p, c->lane_limit.p, c->parent.p,
                        c->pos_parent.p, c->pos_depth.p, c->up_off.p, c->up_tgt.p, c->up_w.p, L.n_top, c->pathbuf.p,
                        D1, c->arena.p, n);
+    constexpr uint32_t WPB = TILE_THREADS / 64;  // tiles per workgroup
     for (int l = (int)nsl - 1; l >= 0; --l) {
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
-        hipLaunchKernelGGL(up_tile_kernel<KL>, dim3(groups, s1 - s0), dim3(TILE_THREADS), 0, s, s0, l > 0 ? 1 : 0,
-                           c->seg_pos.p, c->tb_up.p, c->tb_w.p, c->pt_blk.p, c->pt_tgt.p, c->pr_off.p, c->pr_x.p,
-                           c->pr_w.p, c->arena.p, n);
+        hipLaunchKernelGGL((tri_kernel<KL, false>), dim3(wgs, (s1 - s0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s,
+                           s0, s1, nseg, c->seg_pos.p, c->tb_up.p, c->tb_w.p, c->active.p, c->arena.p, n);
         ++nlaunch;
+        const uint32_t k0 = L.pt_blk[s0], k1 = L.pt_blk[s1];
+        if (k1 > k0) {
+            hipLaunchKernelGGL((rows_kernel<KL, true>), dim3(wgs, (k1 - k0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0,
+                               s, k0, k1, c->pk_seg.p, c->pr_off.p, c->pr_off.p, c->seg_pos.p, c->pt_tgt.p, c->pr_x.p,
+                               c->pr_w.p, c->active.p, nseg, c->arena.p, n);
+            ++nlaunch;
+        }
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[1], s));
     for (uint32_t l = 0; l < nsl; ++l) {
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
-        hipLaunchKernelGGL(down_tile_kernel<KL>, dim3(groups, s1 - s0), dim3(TILE_THREADS), 0, s, s0, c->seg_pos.p,
-                           c->xr_off.p, c->xr_src.p, c->xr_w.p, c->tb_dn.p, c->tb_w.p, c->arena.p, n);
+        const uint32_t u0 = L.du_lvl[l], u1 = L.du_lvl[l + 1];
+        if (u1 > u0) {
+            hipLaunchKernelGGL((rows_kernel<KL, false>), dim3(wgs, (u1 - u0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0,
+                               s, u0, u1, c->du_seg.p, c->du_r0.p, c->du_r1.p, c->seg_pos.p, c->pt_tgt.p, c->xr_src.p,
+                               c->xr_w.p, c->active.p, nseg, c->arena.p, n);
+            ++nlaunch;
+        }
+        hipLaunchKernelGGL((tri_kernel<KL, true>), dim3(wgs, (s1 - s0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s,
+                           s0, s1, nseg, c->seg_pos.p, c->tb_dn.p, c->tb_w.p, c->active.p, c->arena.p, n);
         ++nlaunch;
     }
-    const uint32_t nch = (uint32_t)L.ch_off.size() - 1;
-    if (nch) {
-        hipLaunchKernelGGL((bottom_chunk_kernel<KL, SEG, NTB>), dim3(nch, groups), dim3(NTB), 0, s, c->ch_off.p,
-                           c->ch_pos.p, c->cs_off.p, c->cs_src.p, c->cs_aoff.p, c->ca_v.p, c->ca_w.p, c->pos_pre.p,
-                           c->pos_sub.p, c->pos_depth.p, c->seed_off.p, c->seed_pre.p, c->seed_depth.p, c->pathbuf.p,
-                           D1, nl, c->arena.p, n);
-        ++nlaunch;
-    }
-    constexpr int NSP = NT / KL;
-    for (size_t b = 0; b + 1 < L.botlvl_pos.size(); ++b) {
-        const uint32_t p0 = L.botlvl_pos[b], p1 = L.botlvl_pos[b + 1];
-        hipLaunchKernelGGL(bottom_pull_kernel<KL>, dim3((p1 - p0 + NSP - 1) / NSP, groups), dim3(NT), 0, s, p0, p1,
-                           c->in_off.p, c->in_split.p, c->in_src.p, c->in_w.p, c->arena.p, n);
-        ++nlaunch;
+    if (c->paths_on) {
+        constexpr int NSP = NT / KL;
+        if (c->n_paths) {
+            hipLaunchKernelGGL(bottom_path_kernel<KL>, dim3((c->n_paths + NSP - 1) / NSP, groups), dim3(NT), 0, s,
+                               c->n_paths, c->path_off.p, c->path_pos.p, c->in_off.p, c->in_src.p, c->in_w.p,
+                               c->pos_pre.p, c->pos_sub.p, c->pos_depth.p, c->seed_off.p, c->seed_pre.p,
+                               c->seed_depth.p, c->pathbuf.p, D1, nl, c->arena.p, n);
+            ++nlaunch;
+        }
+    } else {
+        const uint32_t nch = (uint32_t)L.ch_off.size() - 1;
+        if (nch) {
+            hipLaunchKernelGGL((bottom_chunk_kernel<KL, CHUNK, NTB>), dim3(nch, groups), dim3(NTB), 0, s,
+                               c->ch_off.p, c->ch_pos.p, c->cs_off.p, c->cs_src.p, c->cs_aoff.p, c->ca_v.p, c->ca_w.p,
+                               c->pos_pre.p, c->pos_sub.p, c->pos_depth.p, c->seed_off.p, c->seed_pre.p,
+                               c->seed_depth.p, c->pathbuf.p, D1, nl, c->arena.p, n);
+            ++nlaunch;
+        }
+        constexpr int NSP = NT / KL;
+        for (size_t b = 0; b + 1 < L.botlvl_pos.size(); ++b) {
+            const uint32_t p0 = L.botlvl_pos[b], p1 = L.botlvl_pos[b + 1];
+            hipLaunchKernelGGL(bottom_pull_kernel<KL>, dim3((p1 - p0 + NSP - 1) / NSP, groups), dim3(NT), 0, s, p0,
+                               p1, c->in_off.p, c->in_split.p, c->in_src.p, c->in_w.p, c->arena.p, n);
+            ++nlaunch;
+        }
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[2], s));
@@ -930,6 +952,8 @@ int run_all(bfg_ctx *c, const Req &req, F &&out) {
         tm.batches++;
     }
     tm.lanes_run = L;
+    tm.nodes_swept = c->paths_on ? c->L.n_top + c->path_nodes : c->L.n;
+    tm.arcs_swept = c->paths_on ? c->top_in_arcs + c->path_arcs : c->L.m_down;
     tm.total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     c->last = tm;
     return BFG_SUCCESS;
@@ -954,6 +978,45 @@ int stage_targets(bfg_ctx *c, const uint32_t *targets, uint32_t T) {
     }
     HIP_TRY(hipMemcpyAsync(c->tgt_pos.p, tp.data(), (size_t)T * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));  // tp dies with this frame
+    return BFG_SUCCESS;
+}
+
+// Bottom paths of a matrix's targets: for every bottom target that is not an
+// ancestor of another target, its bottom ancestors top-down (the top-region
+// boundary first, the target last).  Targets in the top region need none.
+int stage_target_paths(bfg_ctx *c, const uint32_t *targets, uint32_t T) {
+    const bfg::Layout &L = c->L;
+    std::vector<uint32_t> bt;
+    for (uint32_t t = 0; t < T; ++t) {
+        const uint32_t p = L.pos_of_rank[targets[t]];
+        if (p >= L.n_top) bt.push_back(p);
+    }
+    std::sort(bt.begin(), bt.end(), [&](uint32_t a, uint32_t b) { return L.pos_pre[a] < L.pos_pre[b]; });
+    bt.erase(std::unique(bt.begin(), bt.end()), bt.end());
+    std::vector<uint32_t> off{0}, pos, up;
+    uint64_t arcs = 0;
+    for (size_t i = 0; i < bt.size(); ++i) {
+        const uint32_t t = bt[i];
+        // the next target in pre-order lies in t's subtree iff t is an
+        // ancestor of some later target: t is then on that target's path
+        if (i + 1 < bt.size() && L.pos_pre[bt[i + 1]] - L.pos_pre[t] < L.pos_subtree[t]) continue;
+        up.clear();
+        for (uint32_t x = t; x != NONE && x >= L.n_top; x = L.pos_parent[x]) {
+            up.push_back(x);
+            arcs += L.in_off[x + 1] - L.in_off[x];
+        }
+        pos.insert(pos.end(), up.rbegin(), up.rend());
+        off.push_back((uint32_t)pos.size());
+    }
+    if (c->path_off.n < off.size()) HIP_TRY(c->path_off.alloc(off.size()));
+    if (c->path_pos.n < pos.size()) HIP_TRY(c->path_pos.alloc(pos.size()));
+    HIP_TRY(hipMemcpyAsync(c->path_off.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, c->stream));
+    if (!pos.empty())
+        HIP_TRY(hipMemcpyAsync(c->path_pos.p, pos.data(), pos.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));  // off and pos die with this frame
+    c->n_paths = (uint32_t)(off.size() - 1);
+    c->path_nodes = pos.size();
+    c->path_arcs = arcs;
     return BFG_SUCCESS;
 }
 
@@ -1035,6 +1098,7 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     std::string err;
     bfg::LayoutParams lp;
     lp.segment_limit = SEG;
+    lp.block_limit = CHUNK;
     if (!bfg::build_layout(in, lp, c->L, err)) return fail(BFG_INVALID_PARAMETER, err);
     // every slot, position and segment index the kernels derive rests on these
     // properties; refuse the graph rather than launch on a broken layout
@@ -1050,6 +1114,7 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     HIP_TRY(c->pos_depth.upload(L.pos_depth, s));
     HIP_TRY(c->seg_pos.upload(L.seg_pos, s));
     HIP_TRY(c->in_off.upload(L.in_off, s));
+    c->top_in_arcs = L.in_off[L.n_top];
     HIP_TRY(c->in_split.upload(L.in_split, s));
     HIP_TRY(c->in_src.upload(L.in_src, s));
     HIP_TRY(c->in_w.upload(L.in_w, s));
@@ -1074,18 +1139,24 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     HIP_TRY(c->pr_off.upload(L.pr_off, s));
     HIP_TRY(c->pr_x.upload(L.pr_x, s));
     HIP_TRY(c->pr_w.upload(L.pr_w, s));
+    HIP_TRY(c->du_seg.upload(L.du_seg, s));
+    HIP_TRY(c->du_r0.upload(L.du_r0, s));
+    HIP_TRY(c->du_r1.upload(L.du_r1, s));
+    HIP_TRY(c->pk_seg.upload(L.pk_seg, s));
     HIP_TRY(hipStreamSynchronize(s));
     c->graph_bytes = 0;
     for (auto *b : {&c->parent, &c->pos_parent, &c->pos_pre, &c->pos_sub, &c->pos_depth, &c->seg_pos, &c->in_off,
                     &c->in_split, &c->in_src, &c->in_w, &c->ch_off, &c->ch_pos, &c->cs_off, &c->cs_src, &c->cs_aoff,
                     &c->ca_w, &c->up_off, &c->up_tgt, &c->up_w, &c->xr_off, &c->xr_src, &c->xr_w, &c->tb_dn,
-                    &c->tb_up, &c->tb_w, &c->pt_blk, &c->pt_tgt, &c->pr_off, &c->pr_x, &c->pr_w})
+                    &c->tb_up, &c->tb_w, &c->pt_blk, &c->pt_tgt, &c->pr_off, &c->pr_x, &c->pr_w, &c->du_seg,
+                    &c->du_r0, &c->du_r1, &c->pk_seg})
         c->graph_bytes += b->n * sizeof(uint32_t);
     c->graph_bytes += c->ca_v.n * sizeof(uint16_t);
     // host copies the kernels alone need
     for (auto *v : {&L.in_src, &L.in_w, &L.up_tgt, &L.up_w, &L.bs_src, &L.bs_aoff, &L.ba_w, &L.ut_tgt, &L.ut_aoff,
                     &L.uta_w, &L.sd_w, &L.su_w, &L.cs_src, &L.cs_aoff, &L.ca_w, &L.ch_pos, &L.xr_src, &L.xr_w,
-                    &L.tb_dn, &L.tb_up, &L.tb_w, &L.pt_tgt, &L.pr_off, &L.pr_x, &L.pr_w})
+                    &L.tb_dn, &L.tb_up, &L.tb_w, &L.pt_tgt, &L.pr_off, &L.pr_x, &L.pr_w, &L.du_seg, &L.du_r0,
+                    &L.du_r1, &L.pk_seg})
         std::vector<uint32_t>().swap(*v);
     for (auto *v : {&L.ba_v, &L.uta_x, &L.ca_v, &L.sd_tgt, &L.su_tgt}) std::vector<uint16_t>().swap(*v);
     *out = c.release();
@@ -1109,6 +1180,7 @@ int bfg_validate_graph(const bfg_graph_desc *gd, bfg_ctx_info *info) {
     std::string err;
     bfg::LayoutParams lp;
     lp.segment_limit = SEG;
+    lp.block_limit = CHUNK;
     if (!bfg::build_layout(in, lp, L, err)) return fail(BFG_INVALID_PARAMETER, err);
     if (!bfg::check_layout(in, L, err)) return fail(BFG_INTERNAL_ERROR, err);
     fill_info(L, info);
@@ -1139,12 +1211,11 @@ int bfg_ctx_configure(bfg_ctx *c, uint32_t lanes, uint32_t lane_capacity) {
         c->lanes = lanes;
     }
     if (lane_capacity != 0) {
-        c->lane_cap = (lane_capacity + c->lanes - 1) / c->lanes * c->lanes;
+        c->lane_cap = (lane_capacity + 63) / 64 * 64;  // whole wavefronts of lanes
         c->cap_fixed = true;
         return ensure_arena(c);
     }
     c->cap_fixed = false;
-    if (c->lane_cap) c->lane_cap = (c->lane_cap + c->lanes - 1) / c->lanes * c->lanes;
     return BFG_SUCCESS;
 }
 
@@ -1152,7 +1223,13 @@ static int matrix_impl(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32
                        uint32_t *d_out, uint32_t *host_out) {
     Req req = single_source_req(src, S, thr);
     int rc = stage_targets(c, tgt, T);
+    if (!rc) rc = stage_target_paths(c, tgt, T);
     if (rc) return rc;
+    struct PathsOn {  // restrict the bottom sweep for this call only
+        bfg_ctx *c;
+        explicit PathsOn(bfg_ctx *cc) : c(cc) { c->paths_on = true; }
+        ~PathsOn() { c->paths_on = false; }
+    } paths_on(c);
     DevBuf<uint32_t> tmp;
     if (!d_out) {
         HIP_TRY(tmp.alloc((size_t)S * T));

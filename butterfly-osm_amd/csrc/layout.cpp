@@ -15,64 +15,25 @@ struct Arc3 {
     uint32_t key, col, w;  // key: source (or tail), col: column in a 64-wide block
 };
 
-// Appends the ELL rows of one 64-column block (columns grouped in octets)
-// from arcs (key, column, weight).  A key with arcs into at least half of the
-// block's octets gets a row of its own (one label row per ELL row, so the
-// label read coalesces); the other keys are packed per octet.  Parallel arcs
-// keep the minimum weight.  Returns the number of rows.
-uint32_t build_ell(std::vector<Arc3> &arcs, uint32_t ncols, std::vector<uint32_t> &ekey,
-                   std::vector<uint32_t> &ew) {
-    if (arcs.empty()) return 0;
+// Appends the rows of one 64-column block from arcs (key, column, weight):
+// one row per distinct key, ascending, holding the key and the weights of its
+// arcs to the 64 columns (INF where there is none; parallel arcs keep the
+// minimum).  Returns the number of rows.
+uint32_t build_rows(std::vector<Arc3> &arcs, std::vector<uint32_t> &rkey, std::vector<uint32_t> &rw) {
     std::sort(arcs.begin(), arcs.end(), [](const Arc3 &a, const Arc3 &b) {
         return a.key != b.key ? a.key < b.key : a.col < b.col;
     });
-    const uint32_t n_oct = (ncols + 7) / 8;
-    struct Ent { uint32_t key; uint32_t w[8]; };
-    std::vector<Ent> packed[8];
     uint32_t rows = 0;
-    size_t i = 0;
-    while (i < arcs.size()) {
+    for (size_t i = 0; i < arcs.size();) {
         const uint32_t key = arcs[i].key;
-        uint32_t w[64];
-        std::fill(w, w + 64, INF);
-        uint32_t mask = 0;
-        for (; i < arcs.size() && arcs[i].key == key; ++i) {
-            const uint32_t c = arcs[i].col;
-            w[c] = std::min(w[c], arcs[i].w);
-            mask |= 1u << (c / 8);
-        }
-        if (2u * (uint32_t)__builtin_popcount(mask) >= n_oct) {
-            for (uint32_t o = 0; o < 8; ++o) {
-                ekey.push_back(key);
-                for (uint32_t j = 0; j < 8; ++j) ew.push_back(w[8 * o + j]);
-            }
-            ++rows;
-        } else {
-            for (uint32_t o = 0; o < 8; ++o)
-                if (mask >> o & 1u) {
-                    Ent e{key, {}};
-                    std::copy(w + 8 * o, w + 8 * o + 8, e.w);
-                    packed[o].push_back(e);
-                }
-        }
+        const size_t base = rw.size();
+        rw.resize(base + 64, INF);
+        for (; i < arcs.size() && arcs[i].key == key; ++i)
+            rw[base + arcs[i].col] = std::min(rw[base + arcs[i].col], arcs[i].w);
+        rkey.push_back(key);
+        ++rows;
     }
-    size_t np = 0;
-    for (auto &p : packed) np = std::max(np, p.size());
-    for (size_t r = 0; r < np; ++r) {
-        uint32_t pad = INF;
-        for (auto &p : packed)
-            if (r < p.size()) { pad = p[r].key; break; }
-        for (uint32_t o = 0; o < 8; ++o) {
-            if (r < packed[o].size()) {
-                ekey.push_back(packed[o][r].key);
-                for (uint32_t j = 0; j < 8; ++j) ew.push_back(packed[o][r].w[j]);
-            } else {
-                ekey.push_back(pad);
-                for (uint32_t j = 0; j < 8; ++j) ew.push_back(INF);
-            }
-        }
-    }
-    return rows + (uint32_t)np;
+    return rows;
 }
 
 template <class T>
@@ -452,23 +413,23 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
                 }
             }
     }
-    // ---- dense tiles of the top region ----
+    // ---- dense tiles of the top region (segments of at most 64 nodes) ----
     {
         const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
-        if (L.max_segment > 256) { err = "segment_limit above 256"; return false; }
-        L.xr_off.assign(4 * (size_t)nseg + 1, 0);
-        L.tb_dn.assign(1024 * (size_t)nseg, NONE);
-        L.tb_up.assign(1024 * (size_t)nseg, NONE);
+        if (L.max_segment > 64) { err = "segment_limit above 64"; return false; }
+        L.xr_off.assign(nseg + 1, 0);
+        L.tb_dn.assign(64 * (size_t)nseg, NONE);
+        L.tb_up.assign(64 * (size_t)nseg, NONE);
         L.pt_blk.assign(nseg + 1, 0);
         L.xr_src.clear(); L.xr_w.clear(); L.tb_w.clear();
         L.pt_tgt.clear(); L.pr_off.clear(); L.pr_x.clear(); L.pr_w.clear();
         std::vector<Arc3> a3;
-        std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> outs;  // (target, tail offset, weight)
+        std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> outs;  // (target, tail position, weight)
         std::vector<uint32_t> colv;                                   // target column of each arc of outs
         uint64_t xrows = 0, prows = 0;
         auto put = [&](std::vector<uint32_t> &tab, uint32_t s, uint32_t a, uint32_t o, uint32_t i, uint32_t j,
                        uint32_t w) {
-            uint32_t &k = tab[1024 * (size_t)s + 32 * a + o];
+            uint32_t &k = tab[64 * (size_t)s + 8 * a + o];
             if (k == NONE) {
                 k = (uint32_t)(L.tb_w.size() / 64);
                 L.tb_w.resize(L.tb_w.size() + 64, INF);
@@ -478,17 +439,11 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
         };
         for (uint32_t s = 0; s < nseg; ++s) {
             const uint32_t p0 = L.seg_pos[s], len = L.seg_pos[s + 1] - p0;
-            for (uint32_t b = 0; b < 4; ++b) {
-                L.xr_off[4 * (size_t)s + b] = (uint32_t)xrows;
-                if (64 * b >= len) continue;
-                const uint32_t ncols = std::min<uint32_t>(64, len - 64 * b);
-                a3.clear();
-                for (uint32_t j = 0; j < ncols; ++j) {
-                    const uint32_t p = p0 + 64 * b + j;
-                    for (uint32_t k = L.in_off[p]; k < L.in_split[p]; ++k) a3.push_back({L.in_src[k], j, L.in_w[k]});
-                }
-                xrows += build_ell(a3, ncols, L.xr_src, L.xr_w);
-            }
+            L.xr_off[s] = (uint32_t)xrows;
+            a3.clear();
+            for (uint32_t j = 0; j < len; ++j)
+                for (uint32_t k = L.in_off[p0 + j]; k < L.in_split[p0 + j]; ++k) a3.push_back({L.in_src[k], j, L.in_w[k]});
+            xrows += build_rows(a3, L.xr_src, L.xr_w);
             for (uint32_t i = 0; i < len; ++i) {
                 const uint32_t p = p0 + i;
                 for (uint32_t k = L.in_split[p]; k < L.in_off[p + 1]; ++k) {
@@ -502,7 +457,8 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
             }
             outs.clear();
             for (uint32_t i = 0; i < len; ++i)
-                for (uint32_t k = L.up_off[p0 + i]; k < L.up_split[p0 + i]; ++k) outs.emplace_back(L.up_tgt[k], i, L.up_w[k]);
+                for (uint32_t k = L.up_off[p0 + i]; k < L.up_split[p0 + i]; ++k)
+                    outs.emplace_back(L.up_tgt[k], p0 + i, L.up_w[k]);
             std::sort(outs.begin(), outs.end());
             L.pt_blk[s] = (uint32_t)(L.pt_tgt.size() / 64);
             const size_t t0 = L.pt_tgt.size();
@@ -521,12 +477,27 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
                 a3.clear();
                 for (; q < outs.size() && colv[q] < 64 * (kb + 1); ++q)
                     a3.push_back({std::get<1>(outs[q]), colv[q] - 64 * kb, std::get<2>(outs[q])});
-                prows += build_ell(a3, std::min<uint32_t>(64, nt - 64 * kb), L.pr_x, L.pr_w);
+                prows += build_rows(a3, L.pr_x, L.pr_w);
             }
         }
-        L.xr_off[4 * (size_t)nseg] = (uint32_t)xrows;
+        L.xr_off[nseg] = (uint32_t)xrows;
         L.pt_blk[nseg] = (uint32_t)(L.pt_tgt.size() / 64);
         L.pr_off.push_back((uint32_t)prows);
+        const uint32_t R = std::max<uint32_t>(lp.ell_chunk, 1);
+        L.du_lvl.clear(); L.du_seg.clear(); L.du_r0.clear(); L.du_r1.clear();
+        for (size_t l = 0; l + 1 < L.seglvl_seg.size(); ++l) {
+            L.du_lvl.push_back((uint32_t)L.du_seg.size());
+            for (uint32_t s = L.seglvl_seg[l]; s < L.seglvl_seg[l + 1]; ++s)
+                for (uint32_t r = L.xr_off[s]; r < L.xr_off[s + 1]; r += R) {
+                    L.du_seg.push_back(s);
+                    L.du_r0.push_back(r);
+                    L.du_r1.push_back(std::min(r + R, L.xr_off[s + 1]));
+                }
+        }
+        L.du_lvl.push_back((uint32_t)L.du_seg.size());
+        L.pk_seg.assign(L.pt_blk[nseg], 0);
+        for (uint32_t s = 0; s < nseg; ++s)
+            for (uint32_t k = L.pt_blk[s]; k < L.pt_blk[s + 1]; ++k) L.pk_seg[k] = s;
         L.ell_down_entries = L.xr_w.size();
         L.ell_up_entries = L.pr_w.size();
         if (L.xr_w.size() >= (1ull << 32) || L.pr_w.size() >= (1ull << 32) || L.tb_w.size() >= (1ull << 32)) {
@@ -646,32 +617,32 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
         };
         std::vector<T3> want, got;
         const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
-        if (L.xr_off.size() != 4 * (size_t)nseg + 1 || L.pt_blk.size() != nseg + 1 ||
-            L.tb_dn.size() != 1024 * (size_t)nseg || L.tb_up.size() != 1024 * (size_t)nseg) {
+        if (L.xr_off.size() != (size_t)nseg + 1 || L.pt_blk.size() != nseg + 1 ||
+            L.tb_dn.size() != 64 * (size_t)nseg || L.tb_up.size() != 64 * (size_t)nseg ||
+            L.xr_w.size() != 64 * (size_t)L.xr_src.size() || L.pr_w.size() != 64 * (size_t)L.pr_x.size() ||
+            L.xr_off[nseg] != L.xr_src.size() || L.pr_off.size() != (size_t)L.pt_blk[nseg] + 1 ||
+            L.pr_off.back() != L.pr_x.size() || L.pt_tgt.size() != 64 * (size_t)L.pt_blk[nseg]) {
             err = "dense tile tables disagree";
             return false;
         }
         for (uint32_t s = 0; s < nseg; ++s) {
             const uint32_t p0 = L.seg_pos[s], len = L.seg_pos[s + 1] - p0;
-            if (len > 256) { err = "segment longer than a tile"; return false; }
+            if (len > 64) { err = "segment longer than a tile"; return false; }
             // DOWN from outside: (head offset, source, w)
             want.clear();
             got.clear();
             for (uint32_t i = 0; i < len; ++i)
                 for (uint32_t k = L.in_off[p0 + i]; k < L.in_split[p0 + i]; ++k) want.emplace_back(i, L.in_src[k], L.in_w[k]);
-            for (uint32_t b = 0; b < 4; ++b)
-                for (uint32_t r = L.xr_off[4 * s + b]; r < L.xr_off[4 * s + b + 1]; ++r)
-                    for (uint32_t o = 0; o < 8; ++o) {
-                        const size_t e = 8 * (size_t)r + o;
-                        const uint32_t u = L.xr_src[e];
-                        if (u >= n || stage[u] >= stage[p0]) { err = "tile source not final"; return false; }
-                        for (uint32_t j = 0; j < 8; ++j) {
-                            const uint32_t w = L.xr_w[8 * e + j], h = 64 * b + 8 * o + j;
-                            if (w == INF) continue;
-                            if (h >= len) { err = "tile head beyond the segment"; return false; }
-                            got.emplace_back(h, u, w);
-                        }
-                    }
+            for (uint32_t r = L.xr_off[s]; r < L.xr_off[s + 1]; ++r) {
+                const uint32_t u = L.xr_src[r];
+                if (u >= n || stage[u] >= stage[p0]) { err = "tile source not final"; return false; }
+                for (uint32_t j = 0; j < 64; ++j) {
+                    const uint32_t w = L.xr_w[64 * (size_t)r + j];
+                    if (w == INF) continue;
+                    if (j >= len) { err = "tile head beyond the segment"; return false; }
+                    got.emplace_back(j, u, w);
+                }
+            }
             merged(want);
             std::sort(got.begin(), got.end());
             if (want != got) { err = "DOWN tiles differ from the arcs they replace"; return false; }
@@ -688,9 +659,9 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
                             want.emplace_back(i, L.up_tgt[k], L.up_w[k]);
                 }
                 const auto &tab = dir == 0 ? L.tb_dn : L.tb_up;
-                for (uint32_t a = 0; a < 32; ++a)
-                    for (uint32_t o = 0; o < 32; ++o) {
-                        const uint32_t k = tab[1024 * (size_t)s + 32 * a + o];
+                for (uint32_t a = 0; a < 8; ++a)
+                    for (uint32_t o = 0; o < 8; ++o) {
+                        const uint32_t k = tab[64 * (size_t)s + 8 * a + o];
                         if (k == NONE) continue;
                         if ((dir == 0 && a > o) || (dir == 1 && a < o) || 64 * (size_t)k + 64 > L.tb_w.size()) {
                             err = "bad intra block";
@@ -712,30 +683,49 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
                 std::sort(got.begin(), got.end());
                 if (want != got) { err = "intra blocks differ from the arcs they replace"; return false; }
             }
-            // UP leaving: (target, tail, w)
+            // UP leaving: (target, tail position, w)
             want.clear();
             got.clear();
             for (uint32_t i = 0; i < len; ++i)
-                for (uint32_t k = L.up_off[p0 + i]; k < L.up_split[p0 + i]; ++k) want.emplace_back(L.up_tgt[k], i, L.up_w[k]);
-            for (uint32_t kb = L.pt_blk[s]; kb < L.pt_blk[s + 1]; ++kb) {
-                if (kb + 1 >= L.pr_off.size() || 64 * (size_t)kb + 64 > L.pt_tgt.size()) { err = "bad target block"; return false; }
-                for (uint32_t r = L.pr_off[kb]; r < L.pr_off[kb + 1]; ++r)
-                    for (uint32_t o = 0; o < 8; ++o) {
-                        const size_t e = 8 * (size_t)r + o;
-                        const uint32_t x = L.pr_x[e];
-                        if (x >= len) { err = "push tail beyond the segment"; return false; }
-                        for (uint32_t j = 0; j < 8; ++j) {
-                            const uint32_t w = L.pr_w[8 * e + j], t = L.pt_tgt[64 * (size_t)kb + 8 * o + j];
-                            if (w == INF) continue;
-                            if (t == NONE || t >= L.n_top || stage[t] >= stage[p0]) { err = "push target not an earlier top node"; return false; }
-                            got.emplace_back(t, x, w);
-                        }
+                for (uint32_t k = L.up_off[p0 + i]; k < L.up_split[p0 + i]; ++k) want.emplace_back(L.up_tgt[k], p0 + i, L.up_w[k]);
+            for (uint32_t kb = L.pt_blk[s]; kb < L.pt_blk[s + 1]; ++kb)
+                for (uint32_t r = L.pr_off[kb]; r < L.pr_off[kb + 1]; ++r) {
+                    const uint32_t x = L.pr_x[r];
+                    if (x < p0 || x >= p0 + len) { err = "push tail outside the segment"; return false; }
+                    for (uint32_t j = 0; j < 64; ++j) {
+                        const uint32_t w = L.pr_w[64 * (size_t)r + j], t = L.pt_tgt[64 * (size_t)kb + j];
+                        if (w == INF) continue;
+                        if (t == NONE || t >= L.n_top || stage[t] >= stage[p0]) { err = "push target not an earlier top node"; return false; }
+                        got.emplace_back(t, x, w);
                     }
-            }
+                }
             merged(want);
             std::sort(got.begin(), got.end());
             if (want != got) { err = "UP push tiles differ from the arcs they replace"; return false; }
         }
+        // DOWN work units: the units of a level cover the rows of its segments once
+        if (L.du_lvl.size() != L.seglvl_seg.size() || L.du_r0.size() != L.du_seg.size() ||
+            L.du_r1.size() != L.du_seg.size()) {
+            err = "work unit tables disagree";
+            return false;
+        }
+        std::vector<uint8_t> cover(L.xr_off[nseg], 0);
+        for (size_t l = 0; l + 1 < L.du_lvl.size(); ++l)
+            for (uint32_t u = L.du_lvl[l]; u < L.du_lvl[l + 1]; ++u) {
+                const uint32_t s = L.du_seg[u];
+                if (s >= nseg || lvl_of_seg[s] != l || L.du_r0[u] >= L.du_r1[u] || L.du_r0[u] < L.xr_off[s] ||
+                    L.du_r1[u] > L.xr_off[s + 1]) {
+                    err = "bad work unit";
+                    return false;
+                }
+                for (uint32_t r = L.du_r0[u]; r < L.du_r1[u]; ++r) cover[r]++;
+            }
+        for (uint8_t c : cover)
+            if (c != 1) { err = "work units do not cover the rows once"; return false; }
+        if (L.pk_seg.size() != L.pt_blk[nseg]) { err = "target block table disagrees"; return false; }
+        for (uint32_t s = 0; s < nseg; ++s)
+            for (uint32_t k = L.pt_blk[s]; k < L.pt_blk[s + 1]; ++k)
+                if (L.pk_seg[k] != s) { err = "target block of another segment"; return false; }
     }
     return true;
 }

@@ -45,8 +45,9 @@ struct CchInput {
 
 struct LayoutParams {
     uint32_t bottom_limit = 1024;   // subtree size at or below which a node is bottom
-    uint32_t segment_limit = 256;   // max nodes of one chain segment (one workgroup tile, <= 256)
+    uint32_t segment_limit = 64;    // max nodes of one chain segment (one register tile, <= 64)
     uint32_t block_limit = 256;     // max nodes of one bottom block (LDS resident)
+    uint32_t ell_chunk = 64;        // max ELL rows of one DOWN work unit
 };
 
 struct Layout {
@@ -96,25 +97,27 @@ struct Layout {
     // [su_off[p], su_off[p+1]); head as offset from the segment start
     std::vector<uint32_t> sd_off, sd_w, su_off, su_w;
     std::vector<uint16_t> sd_tgt, su_tgt;
-    // Dense tiles of the top region.  A segment is cut into wave blocks of 64
-    // nodes and octets of 8 (a thread of the segment kernels holds one octet
-    // for KL/8 lanes).  An ELL row has one entry per octet: e = 8r + o holds a
-    // key and the weights of the key's arcs to the 8 nodes of octet o (INF
-    // where there is none); padding entries repeat a key of the row with INF
-    // weights.
-    //   DOWN arcs entering segment s from outside: rows [xr_off[4s+b],
-    //   xr_off[4s+b+1]) of wave block b; key xr_src[e] (source position),
-    //   weights xr_w[8e .. 8e+8).
+    // Dense tiles of the top region (segments of at most 64 nodes; a thread of
+    // the segment kernels holds one lane's labels of a whole segment).  A row
+    // is a key (a position) and 64 weights, the key's arcs to the 64 columns
+    // of a block (INF where there is none).
+    //   DOWN arcs entering segment s from outside: rows [xr_off[s],
+    //   xr_off[s+1]), key xr_src[r] (source), weights xr_w[64r ..] to the
+    //   segment's nodes.
     //   Intra-segment arcs: 8x8 blocks W[i][j] (tail i of octet a, head j of
-    //   octet o) at tb_w[64k ..], k = tb_dn[1024s + 32a + o] (a <= o) or
-    //   tb_up[1024s + 32a + o] (a >= o), NONE where the block has no arc.
+    //   octet o) at tb_w[64k ..], k = tb_dn[64s + 8a + o] (a <= o) or
+    //   tb_up[64s + 8a + o] (a >= o), NONE where the block has no arc.
     //   UP arcs leaving segment s: its targets (positions) pt_tgt[64k .. 64k+64)
     //   for target blocks k in [pt_blk[s], pt_blk[s+1]) (NONE pads a block);
-    //   rows [pr_off[k], pr_off[k+1]) with key pr_x[e] (tail offset in the
-    //   segment) and the weights to the 8 targets of octet o, pr_w[8e ..].
+    //   rows [pr_off[k], pr_off[k+1]), key pr_x[r] (tail position), weights
+    //   pr_w[64r ..] to the block's targets.
     std::vector<uint32_t> xr_off, xr_src, xr_w;
     std::vector<uint32_t> tb_dn, tb_up, tb_w;
     std::vector<uint32_t> pt_blk, pt_tgt, pr_off, pr_x, pr_w;
+    // Work units of the DOWN rows: unit u relaxes rows [du_r0[u], du_r1[u]) of
+    // segment du_seg[u] (at most ell_chunk rows); segment level l owns units
+    // [du_lvl[l], du_lvl[l+1]).  pk_seg[k] is the segment of target block k.
+    std::vector<uint32_t> du_lvl, du_seg, du_r0, du_r1, pk_seg;
     uint64_t ell_down_entries = 0, ell_up_entries = 0;
     // statistics
     uint64_t m_up = 0, m_down = 0, m_up_inf = 0, m_down_inf = 0;

@@ -75,6 +75,10 @@ typedef struct bfg_timings {
     float up_ms, sweep_ms, output_ms, total_ms;
     uint32_t batches, sweep_launches;
     uint64_t lanes_run;
+    /* nodes and finite DOWN arcs one lane group's downward sweep covered: the
+     * whole CCH, or for a matrix the top region plus the bottom paths above
+     * its targets (a target's label depends on its ancestors alone) */
+    uint64_t nodes_swept, arcs_swept;
 } bfg_timings;
 
 /* BatchedPhastEngine::new (batched_phast.rs:89): copies the CCH to `device`,
@@ -84,7 +88,8 @@ void bfg_ctx_destroy(bfg_ctx *ctx);
 const char *bfg_last_error(void);
 int bfg_ctx_get_info(const bfg_ctx *ctx, bfg_ctx_info *info);
 int bfg_ctx_last_timings(const bfg_ctx *ctx, bfg_timings *t);
-/* Lane width K in {16, 32, 64} and the batch capacity (0 = keep/auto).
+/* Lane width K in {16, 32, 64} and the batch capacity in lanes (0 = auto;
+ * rounded up to a multiple of 64).
  * Reallocates the arenas; used by the autotuner. */
 int bfg_ctx_configure(bfg_ctx *ctx, uint32_t lanes, uint32_t lane_capacity);
 

@@ -78,6 +78,39 @@ def test_matrix_config1_100x100(engine_cls, grid_config1):
     assert (got != INF).mean() > 0.5
 
 
+def root_path(g, r):
+    """Ranks from r to its elimination-tree root: in a chordal CCH the parent of
+    a node is its lowest-ranked UP neighbour."""
+    out = [int(r)]
+    while True:
+        a, b = int(g.up_off[out[-1]]), int(g.up_off[out[-1] + 1])
+        fin = g.up_tgt[a:b][g.up_w[a:b] != INF]
+        if len(fin) == 0:
+            return out
+        out.append(int(fin.min()))
+
+
+def test_matrix_target_paths(engine_cls, grid_config1):
+    """The matrix sweep restricted to the targets' ancestors: targets that are
+    ancestors of other targets, duplicates, top- and bottom-region targets and
+    a whole root path, against the oracle's full sweep."""
+    g = grid_config1
+    rng = np.random.default_rng(99)
+    deep = [int(x) for x in rng.integers(0, g.n // 4, 3)]
+    t = []
+    for r in deep:
+        t += root_path(g, r)
+    t += [int(x) for x in rng.integers(0, g.n, 40)] + t[:5]
+    t = np.array(t, dtype=np.uint32)
+    s = rng.integers(0, g.n, 70).astype(np.uint32)
+    say("target paths T", len(t))
+    o = Oracle(g)
+    with engine_cls(g) as e:
+        got, st = e.compute_matrix_flat(s, t)
+        assert st.nodes_swept < g.n  # the restricted sweep ran
+    assert np.array_equal(got, o.compute_matrix_flat_mt(s, t))
+
+
 @pytest.mark.parametrize("lanes,cap", [(32, 0), (16, 48), (64, 64)])
 def test_matrix_lanes_and_batches(engine_cls, grid_config1, lanes, cap):
     """K in {16, 32, 64}, and a lane capacity that forces several device batches."""
@@ -93,7 +126,9 @@ def test_matrix_lanes_and_batches(engine_cls, grid_config1, lanes, cap):
         got, st = e.compute_matrix_flat(s, t)
         assert np.array_equal(got, exp)
         if cap:
-            assert st.batches == (200 + cap - 1) // cap
+            eff = -(-cap // 64) * 64  # capacities round up to whole wavefronts of lanes
+            assert e.info()["lane_capacity"] == eff
+            assert st.batches == (200 + eff - 1) // eff
 
 
 def test_query_batch_full_rows_and_edges(engine_cls, grid_small):
