@@ -1153,12 +1153,11 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
         c->graph_bytes += b->n * sizeof(uint32_t);
     c->graph_bytes += c->ca_v.n * sizeof(uint16_t);
     // host copies the kernels alone need
-    for (auto *v : {&L.in_src, &L.in_w, &L.up_tgt, &L.up_w, &L.bs_src, &L.bs_aoff, &L.ba_w, &L.ut_tgt, &L.ut_aoff,
-                    &L.uta_w, &L.sd_w, &L.su_w, &L.cs_src, &L.cs_aoff, &L.ca_w, &L.ch_pos, &L.xr_src, &L.xr_w,
-                    &L.tb_dn, &L.tb_up, &L.tb_w, &L.pt_tgt, &L.pr_off, &L.pr_x, &L.pr_w, &L.du_seg, &L.du_r0,
-                    &L.du_r1, &L.pk_seg})
+    for (auto *v : {&L.in_src, &L.in_w, &L.up_tgt, &L.up_w, &L.cs_src, &L.cs_aoff, &L.ca_w, &L.ch_pos,
+                    &L.xr_src, &L.xr_w, &L.tb_dn, &L.tb_up, &L.tb_w, &L.pt_tgt, &L.pr_off, &L.pr_x, &L.pr_w,
+                    &L.du_seg, &L.du_r0, &L.du_r1, &L.pk_seg})
         std::vector<uint32_t>().swap(*v);
-    for (auto *v : {&L.ba_v, &L.uta_x, &L.ca_v, &L.sd_tgt, &L.su_tgt}) std::vector<uint16_t>().swap(*v);
+    std::vector<uint16_t>().swap(L.ca_v);
     *out = c.release();
     return BFG_SUCCESS;
 }

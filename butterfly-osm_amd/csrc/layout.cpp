@@ -2,7 +2,10 @@
 #include "layout.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <numeric>
+#include <thread>
 #include <tuple>
 
 namespace bfg {
@@ -10,6 +13,40 @@ namespace bfg {
 namespace {
 template <class T>
 void prefix(std::vector<T> &v);
+
+unsigned worker_count() {
+    unsigned nt = std::thread::hardware_concurrency();
+    if (const char *e = std::getenv("OMP_NUM_THREADS")) {
+        const int v = std::atoi(e);
+        if (v > 0) nt = (unsigned)v;
+    }
+    return std::min(std::max(nt, 1u), 64u);
+}
+
+// fn(begin, end) over [0, count) in blocks of `grain`, on worker_count()
+// threads taking blocks in turn.  Every block runs exactly once; the calls
+// must not depend on each other.
+template <class F>
+void parallel_blocks(size_t count, size_t grain, F &&fn) {
+    const unsigned nt = worker_count();
+    grain = std::max<size_t>(grain, 1);
+    if (nt == 1 || count <= grain) {
+        if (count) fn((size_t)0, count);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+        for (;;) {
+            const size_t b = next.fetch_add(grain);
+            if (b >= count) return;
+            fn(b, std::min(count, b + grain));
+        }
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+}
 
 struct Arc3 {
     uint32_t key, col, w;  // key: source (or tail), col: column in a 64-wide block
@@ -59,41 +96,61 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
             return false;
         }
     // ---- validation and INF filtering (bucket_ch.rs:191-424) ----
-    uint64_t mu = 0, md = 0;
-    for (uint32_t r = 0; r < n; ++r) {
-        for (uint64_t i = in.up_off[r]; i < in.up_off[r + 1]; ++i) {
-            uint32_t v = in.up_tgt[i];
-            if (v >= n || v <= r) { err = "UP arc target not above its tail (not a rank-aligned CCH)"; return false; }
-            if (in.up_w[i] != INF) ++mu; else ++L.m_up_inf;
+    std::atomic<uint64_t> a_mu{0}, a_md{0}, a_mui{0}, a_mdi{0};
+    std::atomic<int> bad{0};  // 1: UP arc not upward, 2: DOWN arc not downward
+    parallel_blocks(n, 4096, [&](size_t b, size_t e) {
+        uint64_t mu = 0, md = 0, mui = 0, mdi = 0;
+        for (uint32_t r = (uint32_t)b; r < e; ++r) {
+            for (uint64_t i = in.up_off[r]; i < in.up_off[r + 1]; ++i) {
+                const uint32_t v = in.up_tgt[i];
+                if (v >= n || v <= r) bad = 1;
+                if (in.up_w[i] != INF) ++mu; else ++mui;
+            }
+            for (uint64_t i = in.dn_off[r]; i < in.dn_off[r + 1]; ++i) {
+                if (in.dn_tgt[i] >= r) bad = 2;
+                if (in.dn_w[i] != INF) ++md; else ++mdi;
+            }
         }
-        for (uint64_t i = in.dn_off[r]; i < in.dn_off[r + 1]; ++i) {
-            uint32_t v = in.dn_tgt[i];
-            if (v >= r) { err = "DOWN arc target not below its tail (not a rank-aligned CCH)"; return false; }
-            if (in.dn_w[i] != INF) ++md; else ++L.m_down_inf;
-        }
-    }
+        a_mu += mu; a_md += md; a_mui += mui; a_mdi += mdi;
+    });
+    if (bad == 1) { err = "UP arc target not above its tail (not a rank-aligned CCH)"; return false; }
+    if (bad == 2) { err = "DOWN arc target not below its tail (not a rank-aligned CCH)"; return false; }
+    const uint64_t mu = a_mu, md = a_md;
+    L.m_up_inf = a_mui;
+    L.m_down_inf = a_mdi;
     if (mu >= 0xFFFFFFFFull || md >= 0xFFFFFFFFull) { err = "more than 2^32-1 finite arcs"; return false; }
     L.m_up = mu;
     L.m_down = md;
-    // UP arcs by tail rank, duplicates (tail, target) merged to the minimum weight
+    // UP arcs by tail rank, duplicates (tail, target) merged to the minimum
+    // weight: each rank's finite arcs sorted and merged in place in a copy
+    // laid out like the input, then compacted
     std::vector<uint32_t> uo(n + 1, 0), ut, uw;
-    ut.reserve(mu);
-    uw.reserve(mu);
     {
-        std::vector<std::pair<uint32_t, uint32_t>> tmp;
-        for (uint32_t r = 0; r < n; ++r) {
-            uo[r] = (uint32_t)ut.size();
-            tmp.clear();
-            for (uint64_t i = in.up_off[r]; i < in.up_off[r + 1]; ++i)
-                if (in.up_w[i] != INF) tmp.emplace_back(in.up_tgt[i], in.up_w[i]);
-            std::sort(tmp.begin(), tmp.end());
-            for (size_t i = 0; i < tmp.size(); ++i) {
-                if (i && tmp[i].first == tmp[i - 1].first) continue;
-                ut.push_back(tmp[i].first);
-                uw.push_back(tmp[i].second);
+        const uint64_t mall = in.up_off[n];
+        std::vector<std::pair<uint32_t, uint32_t>> tmp(mall);
+        parallel_blocks(n, 4096, [&](size_t b, size_t e) {
+            for (uint32_t r = (uint32_t)b; r < e; ++r) {
+                const uint64_t a0 = in.up_off[r];
+                uint64_t k = a0;
+                for (uint64_t i = a0; i < in.up_off[r + 1]; ++i)
+                    if (in.up_w[i] != INF) tmp[k++] = {in.up_tgt[i], in.up_w[i]};
+                std::sort(tmp.begin() + a0, tmp.begin() + k);
+                uint64_t o = a0;
+                for (uint64_t i = a0; i < k; ++i)
+                    if (i == a0 || tmp[i].first != tmp[i - 1].first) tmp[o++] = tmp[i];
+                uo[r] = (uint32_t)(o - a0);
             }
-        }
-        uo[n] = (uint32_t)ut.size();
+        });
+        prefix(uo);
+        ut.resize(uo[n]);
+        uw.resize(uo[n]);
+        parallel_blocks(n, 4096, [&](size_t b, size_t e) {
+            for (uint32_t r = (uint32_t)b; r < e; ++r)
+                for (uint32_t j = 0; j < uo[r + 1] - uo[r]; ++j) {
+                    ut[uo[r] + j] = tmp[in.up_off[r] + j].first;
+                    uw[uo[r] + j] = tmp[in.up_off[r] + j].second;
+                }
+        });
     }
     // ---- elimination tree of the symmetrised finite pattern (Liu) ----
     L.parent.assign(n, NONE);
@@ -169,9 +226,6 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
             seg_range.emplace_back(c, i);
         }
     }
-    std::vector<uint32_t> seg_of(n, NONE);
-    for (uint32_t v = 0; v < n; ++v)
-        if (top[v]) seg_of[v] = chain_seg0[chain_of[v]] + idx_in_chain[v] / SEG;
     // bottom heights
     std::vector<uint32_t> height(n, 0);
     for (uint32_t v = 0; v < n; ++v) {
@@ -180,24 +234,46 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
     }
     for (uint32_t v = 0; v < n; ++v)
         if (!top[v]) L.max_height = std::max(L.max_height, height[v]);
-    // ---- positions ----
+    // ---- segment order: the segments of one level in pre-order of their heads
+    // (neighbours in the tree side by side) ----
     uint32_t n_seglvl = 0;
     for (uint32_t l : seg_level) n_seglvl = std::max(n_seglvl, l + 1);
     std::vector<std::vector<uint32_t>> lvl_segs(n_seglvl);
     for (uint32_t s = 0; s < seg_level.size(); ++s) lvl_segs[seg_level[s]].push_back(s);
+    auto seg_len = [&](uint32_t s) {
+        const auto &ch = chains[seg_range[s].first];
+        return std::min<uint32_t>((uint32_t)ch.size() - seg_range[s].second, SEG);
+    };
+    std::vector<uint32_t> seg_tile(seg_level.size());  // segment index in position order
+    uint32_t n_tiles = 0;
+    for (auto &segs : lvl_segs) {
+        std::sort(segs.begin(), segs.end(), [&](uint32_t a, uint32_t b) {
+            return L.pre[chains[seg_range[a].first][seg_range[a].second]] <
+                   L.pre[chains[seg_range[b].first][seg_range[b].second]];
+        });
+        for (uint32_t s : segs) seg_tile[s] = n_tiles++;
+    }
+    std::vector<uint32_t> seg_of(n, NONE);  // segment of a top node, in position order
+    for (uint32_t v = 0; v < n; ++v)
+        if (top[v]) seg_of[v] = seg_tile[chain_seg0[chain_of[v]] + idx_in_chain[v] / SEG];
+    // ---- positions ----
     L.pos_of_rank.assign(n, NONE);
     L.rank_of_pos.assign(n, NONE);
     uint32_t pos = 0;
-    std::vector<uint32_t> seg_start(seg_level.size());
+    std::vector<uint32_t> seg_start(n_tiles);  // first position of a segment
     for (uint32_t l = 0; l < n_seglvl; ++l) {
         L.seglvl_seg.push_back((uint32_t)L.seg_pos.size());
+        uint32_t cur = NONE;
         for (uint32_t s : lvl_segs[l]) {
-            L.seg_pos.push_back(pos);
-            seg_start[s] = pos;
+            if (seg_tile[s] != cur) {
+                cur = seg_tile[s];
+                L.seg_pos.push_back(pos);
+                seg_start[cur] = pos;
+            }
             const auto &ch = chains[seg_range[s].first];
-            uint32_t i0 = seg_range[s].second, i1 = std::min<uint32_t>((uint32_t)ch.size(), i0 + SEG);
-            L.max_segment = std::max(L.max_segment, i1 - i0);
+            const uint32_t i0 = seg_range[s].second, i1 = i0 + seg_len(s);
             for (uint32_t i = i0; i < i1; ++i) { L.pos_of_rank[ch[i]] = pos; L.rank_of_pos[pos] = ch[i]; ++pos; }
+            L.max_segment = std::max(L.max_segment, pos - seg_start[cur]);
         }
     }
     L.seglvl_seg.push_back((uint32_t)L.seg_pos.size());
@@ -230,50 +306,61 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
     // ---- DOWN pull by head position, outside arcs first ----
     {
         std::vector<uint32_t> cnt(n + 1, 0);
-        for (uint32_t r = 0; r < n; ++r)
-            for (uint64_t i = in.dn_off[r]; i < in.dn_off[r + 1]; ++i)
-                if (in.dn_w[i] != INF) cnt[L.pos_of_rank[in.dn_tgt[i]]]++;
+        parallel_blocks(n, 4096, [&](size_t b, size_t e) {
+            for (uint32_t r = (uint32_t)b; r < e; ++r)
+                for (uint64_t i = in.dn_off[r]; i < in.dn_off[r + 1]; ++i)
+                    if (in.dn_w[i] != INF) __atomic_fetch_add(&cnt[L.pos_of_rank[in.dn_tgt[i]]], 1u, __ATOMIC_RELAXED);
+        });
         L.in_off = cnt;
         prefix(L.in_off);
         L.in_src.resize(md);
         L.in_w.resize(md);
+        // fill in any order; the sort below fixes each head's list
         std::vector<uint32_t> cur(L.in_off.begin(), L.in_off.end() - 1);
-        for (uint32_t r = 0; r < n; ++r) {
-            uint32_t pr = L.pos_of_rank[r];
-            for (uint64_t i = in.dn_off[r]; i < in.dn_off[r + 1]; ++i) {
-                if (in.dn_w[i] == INF) continue;
-                uint32_t k = cur[L.pos_of_rank[in.dn_tgt[i]]]++;
-                L.in_src[k] = pr;
-                L.in_w[k] = in.dn_w[i];
+        parallel_blocks(n, 4096, [&](size_t b, size_t e) {
+            for (uint32_t r = (uint32_t)b; r < e; ++r) {
+                const uint32_t pr = L.pos_of_rank[r];
+                for (uint64_t i = in.dn_off[r]; i < in.dn_off[r + 1]; ++i) {
+                    if (in.dn_w[i] == INF) continue;
+                    const uint32_t k = __atomic_fetch_add(&cur[L.pos_of_rank[in.dn_tgt[i]]], 1u, __ATOMIC_RELAXED);
+                    L.in_src[k] = pr;
+                    L.in_w[k] = in.dn_w[i];
+                }
             }
-        }
+        });
         L.in_split.resize(n);
-        std::vector<std::pair<uint32_t, uint32_t>> tmp;
-        for (uint32_t p = 0; p < n; ++p) {
-            uint32_t a = L.in_off[p], b = L.in_off[p + 1];
-            uint32_t v = L.rank_of_pos[p];
-            // second class: top head — arcs from its own segment; bottom head —
-            // arcs from top sources
-            auto second = [&](uint32_t src_pos) {
-                return top[v] ? same_seg(L.rank_of_pos[src_pos], v) : src_pos < L.n_top;
-            };
-            tmp.resize(b - a);
-            for (uint32_t k = a; k < b; ++k) tmp[k - a] = {L.in_src[k], L.in_w[k]};
-            std::sort(tmp.begin(), tmp.end(), [&](const std::pair<uint32_t, uint32_t> &x,
-                                                  const std::pair<uint32_t, uint32_t> &y) {
-                bool ix = second(x.first), iy = second(y.first);
-                if (ix != iy) return !ix;
-                return x < y;
-            });
-            uint32_t split = b;
-            for (uint32_t k = a; k < b; ++k) {
-                L.in_src[k] = tmp[k - a].first;
-                L.in_w[k] = tmp[k - a].second;
-                if (split == b && second(tmp[k - a].first)) split = k;
+        std::atomic<uint64_t> intra{0};
+        parallel_blocks(n, 1024, [&](size_t pb, size_t pe) {
+            std::vector<std::pair<uint32_t, uint32_t>> tmp;
+            uint64_t my_intra = 0;
+            for (uint32_t p = (uint32_t)pb; p < pe; ++p) {
+                const uint32_t a = L.in_off[p], b = L.in_off[p + 1];
+                const uint32_t v = L.rank_of_pos[p];
+                // second class: top head — arcs from its own segment; bottom
+                // head — arcs from top sources
+                auto second = [&](uint32_t src_pos) {
+                    return top[v] ? same_seg(L.rank_of_pos[src_pos], v) : src_pos < L.n_top;
+                };
+                tmp.resize(b - a);
+                for (uint32_t k = a; k < b; ++k) tmp[k - a] = {L.in_src[k], L.in_w[k]};
+                std::sort(tmp.begin(), tmp.end(), [&](const std::pair<uint32_t, uint32_t> &x,
+                                                      const std::pair<uint32_t, uint32_t> &y) {
+                    const bool ix = second(x.first), iy = second(y.first);
+                    if (ix != iy) return !ix;
+                    return x < y;
+                });
+                uint32_t split = b;
+                for (uint32_t k = a; k < b; ++k) {
+                    L.in_src[k] = tmp[k - a].first;
+                    L.in_w[k] = tmp[k - a].second;
+                    if (split == b && second(tmp[k - a].first)) split = k;
+                }
+                L.in_split[p] = split;
+                if (top[v]) my_intra += b - split;
             }
-            L.in_split[p] = split;
-            if (top[v]) L.m_down_intra += b - split;
-        }
+            intra += my_intra;
+        });
+        L.m_down_intra = intra;
     }
     // ---- UP by tail position, outside arcs first ----
     {
@@ -283,206 +370,194 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
         L.up_tgt.resize(ut.size());
         L.up_w.resize(ut.size());
         L.up_split.resize(n);
-        std::vector<std::pair<uint32_t, uint32_t>> outs, ins;
-        for (uint32_t p = 0; p < n; ++p) {
-            uint32_t x = L.rank_of_pos[p];
-            outs.clear();
-            ins.clear();
-            for (uint32_t k = uo[x]; k < uo[x + 1]; ++k) {
-                uint32_t y = ut[k];
-                if (same_seg(x, y)) ins.emplace_back(L.pos_of_rank[y] - seg_start[seg_of[y]], uw[k]);
-                else outs.emplace_back(L.pos_of_rank[y], uw[k]);
+        std::atomic<uint64_t> intra{0};
+        parallel_blocks(n, 1024, [&](size_t pb, size_t pe) {
+            std::vector<std::pair<uint32_t, uint32_t>> outs, ins;
+            uint64_t my_intra = 0;
+            for (uint32_t p = (uint32_t)pb; p < pe; ++p) {
+                const uint32_t x = L.rank_of_pos[p];
+                outs.clear();
+                ins.clear();
+                for (uint32_t k = uo[x]; k < uo[x + 1]; ++k) {
+                    const uint32_t y = ut[k];
+                    if (same_seg(x, y)) ins.emplace_back(L.pos_of_rank[y] - seg_start[seg_of[y]], uw[k]);
+                    else outs.emplace_back(L.pos_of_rank[y], uw[k]);
+                }
+                std::sort(outs.begin(), outs.end());
+                std::sort(ins.begin(), ins.end());
+                uint32_t k = L.up_off[p];
+                for (auto &e : outs) { L.up_tgt[k] = e.first; L.up_w[k] = e.second; ++k; }
+                L.up_split[p] = k;
+                for (auto &e : ins) { L.up_tgt[k] = e.first; L.up_w[k] = e.second; ++k; }
+                my_intra += ins.size();
             }
-            std::sort(outs.begin(), outs.end());
-            std::sort(ins.begin(), ins.end());
-            uint32_t k = L.up_off[p];
-            for (auto &e : outs) { L.up_tgt[k] = e.first; L.up_w[k] = e.second; ++k; }
-            L.up_split[p] = k;
-            for (auto &e : ins) { L.up_tgt[k] = e.first; L.up_w[k] = e.second; ++k; }
-            L.m_up_intra += ins.size();
-        }
+            intra += my_intra;
+        });
+        L.m_up_intra = intra;
     }
-    // ---- grouped DOWN arcs: segments (outside arcs) and bottom chunks (top-sourced arcs) ----
+    // ---- bottom chunks: bottom nodes in pre-order, their top-sourced DOWN arcs
+    // grouped by source ----
     const uint32_t BB = std::max<uint32_t>(lp.block_limit, 1);
-    // groups arcs (source, local head, weight) of one node set by source
-    auto group = [](std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> &arcs, std::vector<uint32_t> &src,
-                    std::vector<uint32_t> &aoff, std::vector<uint16_t> &av, std::vector<uint32_t> &aw) {
-        std::sort(arcs.begin(), arcs.end());
-        for (size_t i = 0; i < arcs.size(); ++i) {
-            if (i == 0 || std::get<0>(arcs[i]) != std::get<0>(arcs[i - 1])) {
-                src.push_back(std::get<0>(arcs[i]));
-                aoff.push_back((uint32_t)av.size());
-            }
-            av.push_back((uint16_t)std::get<1>(arcs[i]));
-            aw.push_back(std::get<2>(arcs[i]));
-        }
-    };
-    std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> arcs;
-    {
-        const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
-        L.bs_off.assign(nseg + 1, 0);
-        for (uint32_t s = 0; s < nseg; ++s) {
-            L.bs_off[s] = (uint32_t)L.bs_src.size();
-            const uint32_t p0 = L.seg_pos[s], p1 = L.seg_pos[s + 1];
-            L.max_block = std::max(L.max_block, p1 - p0);
-            arcs.clear();
-            for (uint32_t p = p0; p < p1; ++p)
-                for (uint32_t k = L.in_off[p]; k < L.in_split[p]; ++k) arcs.emplace_back(L.in_src[k], p - p0, L.in_w[k]);
-            group(arcs, L.bs_src, L.bs_aoff, L.ba_v, L.ba_w);
-        }
-        L.bs_off[nseg] = (uint32_t)L.bs_src.size();
-        L.bs_aoff.push_back((uint32_t)L.ba_v.size());
-    }
     {
         std::vector<uint32_t> bot;
         for (uint32_t p = L.n_top; p < n; ++p) bot.push_back(p);
         std::sort(bot.begin(), bot.end(), [&](uint32_t a, uint32_t b) { return L.pos_pre[a] < L.pos_pre[b]; });
-        L.ch_off.clear();
-        L.ch_pos.clear();
-        for (size_t i = 0; i < bot.size(); i += BB) {
-            L.ch_off.push_back((uint32_t)L.ch_pos.size());
-            const size_t e = std::min(bot.size(), i + BB);
-            arcs.clear();
-            L.cs_off.push_back((uint32_t)L.cs_src.size());
-            for (size_t k = i; k < e; ++k) {
-                const uint32_t p = bot[k];
-                L.ch_pos.push_back(p);
-                for (uint32_t a = L.in_split[p]; a < L.in_off[p + 1]; ++a)
-                    arcs.emplace_back(L.in_src[a], (uint32_t)(k - i), L.in_w[a]);
+        const size_t nch = (bot.size() + BB - 1) / BB;
+        struct Chunk {
+            std::vector<uint32_t> src, aoff, aw;
+            std::vector<uint16_t> av;
+        };
+        std::vector<Chunk> ch(nch);
+        parallel_blocks(nch, 64, [&](size_t cb, size_t ce) {
+            std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> arcs;
+            for (size_t c = cb; c < ce; ++c) {
+                const size_t i = c * BB, e = std::min(bot.size(), i + BB);
+                arcs.clear();
+                for (size_t k = i; k < e; ++k)
+                    for (uint32_t a = L.in_split[bot[k]]; a < L.in_off[bot[k] + 1]; ++a)
+                        arcs.emplace_back(L.in_src[a], (uint32_t)(k - i), L.in_w[a]);
+                std::sort(arcs.begin(), arcs.end());
+                Chunk &q = ch[c];
+                for (size_t j = 0; j < arcs.size(); ++j) {
+                    if (j == 0 || std::get<0>(arcs[j]) != std::get<0>(arcs[j - 1])) {
+                        q.src.push_back(std::get<0>(arcs[j]));
+                        q.aoff.push_back((uint32_t)q.av.size());
+                    }
+                    q.av.push_back((uint16_t)std::get<1>(arcs[j]));
+                    q.aw.push_back(std::get<2>(arcs[j]));
+                }
             }
-            group(arcs, L.cs_src, L.cs_aoff, L.ca_v, L.ca_w);
+        });
+        L.ch_pos = bot;
+        L.ch_off.clear();
+        for (size_t c = 0; c < nch; ++c) {
+            L.ch_off.push_back((uint32_t)(c * BB));
+            L.cs_off.push_back((uint32_t)L.cs_src.size());
+            const uint32_t base = (uint32_t)L.ca_v.size();
+            for (size_t j = 0; j < ch[c].src.size(); ++j) {
+                L.cs_src.push_back(ch[c].src[j]);
+                L.cs_aoff.push_back(base + ch[c].aoff[j]);
+            }
+            L.ca_v.insert(L.ca_v.end(), ch[c].av.begin(), ch[c].av.end());
+            L.ca_w.insert(L.ca_w.end(), ch[c].aw.begin(), ch[c].aw.end());
+            Chunk().src.swap(ch[c].src);
         }
         L.ch_off.push_back((uint32_t)L.ch_pos.size());
         L.cs_off.push_back((uint32_t)L.cs_src.size());
         L.cs_aoff.push_back((uint32_t)L.ca_v.size());
     }
-    // UP arcs leaving each segment, grouped by target position
-    {
-        const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
-        L.ut_off.assign(nseg + 1, 0);
-        L.ut_tgt.clear();
-        L.ut_aoff.clear();
-        L.uta_x.clear();
-        L.uta_w.clear();
-        std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> arcs;  // (target, local tail, weight)
-        for (uint32_t s = 0; s < nseg; ++s) {
-            L.ut_off[s] = (uint32_t)L.ut_tgt.size();
-            const uint32_t p0 = L.seg_pos[s], p1 = L.seg_pos[s + 1];
-            arcs.clear();
-            for (uint32_t p = p0; p < p1; ++p)
-                for (uint32_t k = L.up_off[p]; k < L.up_split[p]; ++k) arcs.emplace_back(L.up_tgt[k], p - p0, L.up_w[k]);
-            std::sort(arcs.begin(), arcs.end());
-            for (size_t i = 0; i < arcs.size(); ++i) {
-                if (i == 0 || std::get<0>(arcs[i]) != std::get<0>(arcs[i - 1])) {
-                    L.ut_tgt.push_back(std::get<0>(arcs[i]));
-                    L.ut_aoff.push_back((uint32_t)L.uta_x.size());
-                }
-                L.uta_x.push_back((uint16_t)std::get<1>(arcs[i]));
-                L.uta_w.push_back(std::get<2>(arcs[i]));
-            }
-        }
-        L.ut_off[nseg] = (uint32_t)L.ut_tgt.size();
-        L.ut_aoff.push_back((uint32_t)L.uta_x.size());
-    }
-    // intra-segment arc lists by tail
-    {
-        const uint32_t nt = L.n_top;
-        L.sd_off.assign(nt + 1, 0);
-        L.su_off.assign(nt + 1, 0);
-        for (uint32_t p = 0; p < nt; ++p) {
-            for (uint32_t k = L.in_split[p]; k < L.in_off[p + 1]; ++k) L.sd_off[L.in_src[k]]++;
-            L.su_off[p] = L.up_off[p + 1] - L.up_split[p];
-        }
-        prefix(L.sd_off);
-        prefix(L.su_off);
-        L.sd_tgt.resize(L.sd_off[nt]);
-        L.sd_w.resize(L.sd_off[nt]);
-        L.su_tgt.resize(L.su_off[nt]);
-        L.su_w.resize(L.su_off[nt]);
-        std::vector<uint32_t> cur(L.sd_off.begin(), L.sd_off.end() - 1);
-        for (size_t s = 0; s + 1 < L.seg_pos.size(); ++s)
-            for (uint32_t p = L.seg_pos[s]; p < L.seg_pos[s + 1]; ++p) {  // heads ascending per tail
-                for (uint32_t k = L.in_split[p]; k < L.in_off[p + 1]; ++k) {
-                    const uint32_t d = cur[L.in_src[k]]++;
-                    L.sd_tgt[d] = (uint16_t)(p - L.seg_pos[s]);
-                    L.sd_w[d] = L.in_w[k];
-                }
-                for (uint32_t k = L.up_split[p], d = L.su_off[p]; k < L.up_off[p + 1]; ++k, ++d) {
-                    L.su_tgt[d] = (uint16_t)L.up_tgt[k];
-                    L.su_w[d] = L.up_w[k];
-                }
-            }
-    }
-    // ---- dense tiles of the top region (segments of at most 64 nodes) ----
+    // ---- dense tiles of the top region (segments of at most 64 nodes), built
+    // per segment in parallel and concatenated in segment order ----
     {
         const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
         if (L.max_segment > 64) { err = "segment_limit above 64"; return false; }
+        struct Tiles {
+            std::vector<uint32_t> xr_src, xr_w;                 // DOWN rows
+            std::vector<std::pair<uint32_t, uint32_t>> blocks;  // (tab 0/1 * 64 + 8a + o, local block)
+            std::vector<uint32_t> tb_w;                         // 64 weights per local block
+            std::vector<uint32_t> pt_tgt, pr_cnt, pr_x, pr_w;   // UP push: targets, rows per block, rows
+        };
+        std::vector<Tiles> tl(nseg);
+        parallel_blocks(nseg, 16, [&](size_t sb, size_t se) {
+            std::vector<Arc3> a3;
+            std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> outs;  // (target, tail position, weight)
+            std::vector<uint32_t> colv;                                   // target column of each arc of outs
+            for (uint32_t s = (uint32_t)sb; s < se; ++s) {
+                Tiles &t = tl[s];
+                const uint32_t p0 = L.seg_pos[s], len = L.seg_pos[s + 1] - p0;
+                a3.clear();
+                for (uint32_t j = 0; j < len; ++j)
+                    for (uint32_t k = L.in_off[p0 + j]; k < L.in_split[p0 + j]; ++k)
+                        a3.push_back({L.in_src[k], j, L.in_w[k]});
+                build_rows(a3, t.xr_src, t.xr_w);
+                uint32_t tab[2][64];
+                std::fill(&tab[0][0], &tab[0][0] + 128, NONE);
+                auto put = [&](int d, uint32_t a, uint32_t o, uint32_t i, uint32_t j, uint32_t w) {
+                    uint32_t &k = tab[d][8 * a + o];
+                    if (k == NONE) {
+                        k = (uint32_t)t.blocks.size();
+                        t.blocks.emplace_back(64 * d + 8 * a + o, k);
+                        t.tb_w.resize(t.tb_w.size() + 64, INF);
+                    }
+                    uint32_t &slot = t.tb_w[64 * (size_t)k + 8 * i + j];
+                    slot = std::min(slot, w);
+                };
+                for (uint32_t i = 0; i < len; ++i) {
+                    const uint32_t p = p0 + i;
+                    for (uint32_t k = L.in_split[p]; k < L.in_off[p + 1]; ++k) {
+                        const uint32_t ti = L.in_src[k] - p0;  // tail before head: ti < i
+                        put(0, ti / 8, i / 8, ti % 8, i % 8, L.in_w[k]);
+                    }
+                    for (uint32_t k = L.up_split[p]; k < L.up_off[p + 1]; ++k) {
+                        const uint32_t h = L.up_tgt[k];  // head offset, h < i
+                        put(1, i / 8, h / 8, i % 8, h % 8, L.up_w[k]);
+                    }
+                }
+                outs.clear();
+                for (uint32_t i = 0; i < len; ++i)
+                    for (uint32_t k = L.up_off[p0 + i]; k < L.up_split[p0 + i]; ++k)
+                        outs.emplace_back(L.up_tgt[k], p0 + i, L.up_w[k]);
+                std::sort(outs.begin(), outs.end());
+                for (size_t q = 0; q < outs.size(); ++q)
+                    if (q == 0 || std::get<0>(outs[q]) != std::get<0>(outs[q - 1]))
+                        t.pt_tgt.push_back(std::get<0>(outs[q]));
+                const uint32_t nt = (uint32_t)t.pt_tgt.size();
+                t.pt_tgt.resize((size_t)(nt + 63) / 64 * 64, NONE);
+                colv.resize(outs.size());
+                for (size_t q = 0, c = 0; q < outs.size(); ++q) {
+                    if (q > 0 && std::get<0>(outs[q]) != std::get<0>(outs[q - 1])) ++c;
+                    colv[q] = (uint32_t)c;
+                }
+                size_t q = 0;
+                for (uint32_t kb = 0; kb * 64 < nt; ++kb) {
+                    a3.clear();
+                    for (; q < outs.size() && colv[q] < 64 * (kb + 1); ++q)
+                        a3.push_back({std::get<1>(outs[q]), colv[q] - 64 * kb, std::get<2>(outs[q])});
+                    t.pr_cnt.push_back(build_rows(a3, t.pr_x, t.pr_w));
+                }
+            }
+        });
         L.xr_off.assign(nseg + 1, 0);
         L.tb_dn.assign(64 * (size_t)nseg, NONE);
         L.tb_up.assign(64 * (size_t)nseg, NONE);
         L.pt_blk.assign(nseg + 1, 0);
         L.xr_src.clear(); L.xr_w.clear(); L.tb_w.clear();
         L.pt_tgt.clear(); L.pr_off.clear(); L.pr_x.clear(); L.pr_w.clear();
-        std::vector<Arc3> a3;
-        std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> outs;  // (target, tail position, weight)
-        std::vector<uint32_t> colv;                                   // target column of each arc of outs
-        uint64_t xrows = 0, prows = 0;
-        auto put = [&](std::vector<uint32_t> &tab, uint32_t s, uint32_t a, uint32_t o, uint32_t i, uint32_t j,
-                       uint32_t w) {
-            uint32_t &k = tab[64 * (size_t)s + 8 * a + o];
-            if (k == NONE) {
-                k = (uint32_t)(L.tb_w.size() / 64);
-                L.tb_w.resize(L.tb_w.size() + 64, INF);
-            }
-            uint32_t &slot = L.tb_w[64 * (size_t)k + 8 * i + j];
-            slot = std::min(slot, w);
-        };
-        for (uint32_t s = 0; s < nseg; ++s) {
-            const uint32_t p0 = L.seg_pos[s], len = L.seg_pos[s + 1] - p0;
-            L.xr_off[s] = (uint32_t)xrows;
-            a3.clear();
-            for (uint32_t j = 0; j < len; ++j)
-                for (uint32_t k = L.in_off[p0 + j]; k < L.in_split[p0 + j]; ++k) a3.push_back({L.in_src[k], j, L.in_w[k]});
-            xrows += build_rows(a3, L.xr_src, L.xr_w);
-            for (uint32_t i = 0; i < len; ++i) {
-                const uint32_t p = p0 + i;
-                for (uint32_t k = L.in_split[p]; k < L.in_off[p + 1]; ++k) {
-                    const uint32_t ti = L.in_src[k] - p0;  // tail before head: ti < i
-                    put(L.tb_dn, s, ti / 8, i / 8, ti % 8, i % 8, L.in_w[k]);
-                }
-                for (uint32_t k = L.up_split[p]; k < L.up_off[p + 1]; ++k) {
-                    const uint32_t t = L.up_tgt[k];  // head offset, t < i
-                    put(L.tb_up, s, i / 8, t / 8, i % 8, t % 8, L.up_w[k]);
-                }
-            }
-            outs.clear();
-            for (uint32_t i = 0; i < len; ++i)
-                for (uint32_t k = L.up_off[p0 + i]; k < L.up_split[p0 + i]; ++k)
-                    outs.emplace_back(L.up_tgt[k], p0 + i, L.up_w[k]);
-            std::sort(outs.begin(), outs.end());
-            L.pt_blk[s] = (uint32_t)(L.pt_tgt.size() / 64);
-            const size_t t0 = L.pt_tgt.size();
-            for (size_t q = 0; q < outs.size(); ++q)
-                if (q == 0 || std::get<0>(outs[q]) != std::get<0>(outs[q - 1])) L.pt_tgt.push_back(std::get<0>(outs[q]));
-            const uint32_t nt = (uint32_t)(L.pt_tgt.size() - t0);
-            L.pt_tgt.resize(t0 + (size_t)(nt + 63) / 64 * 64, NONE);
-            colv.resize(outs.size());
-            for (size_t q = 0, c = 0; q < outs.size(); ++q) {
-                if (q > 0 && std::get<0>(outs[q]) != std::get<0>(outs[q - 1])) ++c;
-                colv[q] = (uint32_t)c;
-            }
-            size_t q = 0;
-            for (uint32_t kb = 0; kb * 64 < nt; ++kb) {
-                L.pr_off.push_back((uint32_t)prows);
-                a3.clear();
-                for (; q < outs.size() && colv[q] < 64 * (kb + 1); ++q)
-                    a3.push_back({std::get<1>(outs[q]), colv[q] - 64 * kb, std::get<2>(outs[q])});
-                prows += build_rows(a3, L.pr_x, L.pr_w);
-            }
+        size_t nx = 0, nxw = 0, ntb = 0, npt = 0, npr = 0, nprw = 0;
+        for (const Tiles &t : tl) {
+            nx += t.xr_src.size(); nxw += t.xr_w.size(); ntb += t.tb_w.size();
+            npt += t.pt_tgt.size(); npr += t.pr_x.size(); nprw += t.pr_w.size();
         }
-        L.xr_off[nseg] = (uint32_t)xrows;
+        if (nxw >= (1ull << 32) || nprw >= (1ull << 32) || ntb >= (1ull << 32)) {
+            err = "dense tiles exceed 2^32 weights";
+            return false;
+        }
+        L.xr_src.reserve(nx); L.xr_w.reserve(nxw); L.tb_w.reserve(ntb);
+        L.pt_tgt.reserve(npt); L.pr_x.reserve(npr); L.pr_w.reserve(nprw);
+        for (uint32_t s = 0; s < nseg; ++s) {
+            Tiles &t = tl[s];
+            L.xr_off[s] = (uint32_t)L.xr_src.size();
+            L.xr_src.insert(L.xr_src.end(), t.xr_src.begin(), t.xr_src.end());
+            L.xr_w.insert(L.xr_w.end(), t.xr_w.begin(), t.xr_w.end());
+            const uint32_t b0 = (uint32_t)(L.tb_w.size() / 64);
+            for (auto &bk : t.blocks)
+                (bk.first < 64 ? L.tb_dn : L.tb_up)[64 * (size_t)s + bk.first % 64] = b0 + bk.second;
+            L.tb_w.insert(L.tb_w.end(), t.tb_w.begin(), t.tb_w.end());
+            L.pt_blk[s] = (uint32_t)(L.pt_tgt.size() / 64);
+            L.pt_tgt.insert(L.pt_tgt.end(), t.pt_tgt.begin(), t.pt_tgt.end());
+            uint32_t r = (uint32_t)L.pr_x.size();
+            for (uint32_t c : t.pr_cnt) {
+                L.pr_off.push_back(r);
+                r += c;
+            }
+            L.pr_x.insert(L.pr_x.end(), t.pr_x.begin(), t.pr_x.end());
+            L.pr_w.insert(L.pr_w.end(), t.pr_w.begin(), t.pr_w.end());
+            Tiles().xr_w.swap(t.xr_w);
+            Tiles().pr_w.swap(t.pr_w);
+        }
+        L.xr_off[nseg] = (uint32_t)L.xr_src.size();
         L.pt_blk[nseg] = (uint32_t)(L.pt_tgt.size() / 64);
-        L.pr_off.push_back((uint32_t)prows);
+        L.pr_off.push_back((uint32_t)L.pr_x.size());
         const uint32_t R = std::max<uint32_t>(lp.ell_chunk, 1);
         L.du_lvl.clear(); L.du_seg.clear(); L.du_r0.clear(); L.du_r1.clear();
         for (size_t l = 0; l + 1 < L.seglvl_seg.size(); ++l) {
@@ -500,78 +575,79 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
             for (uint32_t k = L.pt_blk[s]; k < L.pt_blk[s + 1]; ++k) L.pk_seg[k] = s;
         L.ell_down_entries = L.xr_w.size();
         L.ell_up_entries = L.pr_w.size();
-        if (L.xr_w.size() >= (1ull << 32) || L.pr_w.size() >= (1ull << 32) || L.tb_w.size() >= (1ull << 32)) {
-            err = "dense tiles exceed 2^32 weights";
-            return false;
-        }
     }
     return true;
 }
 
 bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
     const uint32_t n = in.n;
+    // the checks run in parallel blocks; the first failure's message is kept
+    std::atomic<const char *> first{nullptr};
+    auto flag = [&](const char *m) {
+        const char *z = nullptr;
+        first.compare_exchange_strong(z, m);
+    };
+    auto failed = [&] {
+        if (!first.load()) return false;
+        err = first.load();
+        return true;
+    };
     auto is_anc = [&](uint32_t a, uint32_t d) {  // rank a ancestor-or-self of rank d
         return L.pre[a] <= L.pre[d] && L.pre[d] < L.pre[a] + L.subtree[a];
     };
-    for (uint32_t r = 0; r < n; ++r) {
-        for (uint64_t i = in.up_off[r]; i < in.up_off[r + 1]; ++i)
-            if (in.up_w[i] != INF && !is_anc(in.up_tgt[i], r)) { err = "up arc between non-ancestors"; return false; }
-        for (uint64_t i = in.dn_off[r]; i < in.dn_off[r + 1]; ++i)
-            if (in.dn_w[i] != INF && !is_anc(r, in.dn_tgt[i])) { err = "down arc between non-ancestors"; return false; }
-    }
+    parallel_blocks(n, 4096, [&](size_t b, size_t e) {
+        for (uint32_t r = (uint32_t)b; r < e; ++r) {
+            for (uint64_t i = in.up_off[r]; i < in.up_off[r + 1]; ++i)
+                if (in.up_w[i] != INF && !is_anc(in.up_tgt[i], r)) flag("up arc between non-ancestors");
+            for (uint64_t i = in.dn_off[r]; i < in.dn_off[r + 1]; ++i)
+                if (in.dn_w[i] != INF && !is_anc(r, in.dn_tgt[i])) flag("down arc between non-ancestors");
+        }
+    });
+    if (failed()) return false;
     // stage of a position: segment level for top nodes, then the bottom levels
     std::vector<uint32_t> seg_of_pos(L.n_top), lvl_of_seg(L.seg_pos.size() - 1), stage(n);
     for (size_t l = 0; l + 1 < L.seglvl_seg.size(); ++l)
         for (uint32_t s = L.seglvl_seg[l]; s < L.seglvl_seg[l + 1]; ++s) lvl_of_seg[s] = (uint32_t)l;
     for (size_t s = 0; s + 1 < L.seg_pos.size(); ++s)
-        for (uint32_t p = L.seg_pos[s]; p < L.seg_pos[s + 1]; ++p) { seg_of_pos[p] = (uint32_t)s; stage[p] = lvl_of_seg[s]; }
+        for (uint32_t p = L.seg_pos[s]; p < L.seg_pos[s + 1]; ++p) {
+            seg_of_pos[p] = (uint32_t)s;
+            stage[p] = lvl_of_seg[s];
+        }
     const uint32_t nsl = (uint32_t)L.seglvl_seg.size() - 1;
     for (size_t b = 0; b + 1 < L.botlvl_pos.size(); ++b)
         for (uint32_t p = L.botlvl_pos[b]; p < L.botlvl_pos[b + 1]; ++p) stage[p] = nsl + (uint32_t)b;
-    for (uint32_t p = 0; p < n; ++p) {
-        const bool tp = p < L.n_top;
-        if (L.in_split[p] < L.in_off[p] || L.in_split[p] > L.in_off[p + 1]) { err = "bad in_split"; return false; }
-        for (uint32_t k = L.in_off[p]; k < L.in_off[p + 1]; ++k) {
-            uint32_t q = L.in_src[k];
-            bool second = k >= L.in_split[p];
-            bool ok;
-            if (tp) ok = second ? (q < L.n_top && seg_of_pos[q] == seg_of_pos[p] && q < p)
-                                : (q < n && stage[q] < stage[p]);
-            else ok = second ? q < L.n_top : (q >= L.n_top && q < n && stage[q] < stage[p]);
-            if (!ok) { err = "down schedule violates an arc dependency"; return false; }
-        }
-        if (L.up_split[p] < L.up_off[p] || L.up_split[p] > L.up_off[p + 1]) { err = "bad up_split"; return false; }
-        for (uint32_t k = L.up_off[p]; k < L.up_off[p + 1]; ++k) {
-            bool intra = k >= L.up_split[p];
-            if (intra) {
-                if (!tp) { err = "intra up arc at a bottom node"; return false; }
-                uint32_t q = L.seg_pos[seg_of_pos[p]] + L.up_tgt[k];
-                if (!(q < p && seg_of_pos[q] == seg_of_pos[p])) { err = "bad intra up arc"; return false; }
-            } else {
-                uint32_t q = L.up_tgt[k];
-                if (q >= n) { err = "up target out of range"; return false; }
-                if (tp && !(q < L.n_top && stage[q] < stage[p])) { err = "up schedule violates an arc dependency"; return false; }
-                if (!tp && !(stage[q] < stage[p])) { err = "bottom up arc to a non-ancestor level"; return false; }
+    parallel_blocks(n, 4096, [&](size_t pb, size_t pe) {
+        for (uint32_t p = (uint32_t)pb; p < pe; ++p) {
+            const bool tp = p < L.n_top;
+            if (L.in_split[p] < L.in_off[p] || L.in_split[p] > L.in_off[p + 1]) { flag("bad in_split"); return; }
+            for (uint32_t k = L.in_off[p]; k < L.in_off[p + 1]; ++k) {
+                const uint32_t q = L.in_src[k];
+                const bool second = k >= L.in_split[p];
+                bool ok;
+                if (tp) ok = second ? (q < L.n_top && seg_of_pos[q] == seg_of_pos[p] && q < p)
+                                    : (q < n && stage[q] < stage[p]);
+                else ok = second ? q < L.n_top : (q >= L.n_top && q < n && stage[q] < stage[p]);
+                if (!ok) flag("down schedule violates an arc dependency");
             }
+            if (L.up_split[p] < L.up_off[p] || L.up_split[p] > L.up_off[p + 1]) { flag("bad up_split"); return; }
+            for (uint32_t k = L.up_off[p]; k < L.up_off[p + 1]; ++k) {
+                if (k >= L.up_split[p]) {  // intra-segment
+                    if (!tp) { flag("intra up arc at a bottom node"); return; }
+                    const uint32_t q = L.seg_pos[seg_of_pos[p]] + L.up_tgt[k];
+                    if (!(q < p && seg_of_pos[q] == seg_of_pos[p])) flag("bad intra up arc");
+                } else {
+                    const uint32_t q = L.up_tgt[k];
+                    if (q >= n) { flag("up target out of range"); return; }
+                    if (tp && !(q < L.n_top && stage[q] < stage[p])) flag("up schedule violates an arc dependency");
+                    if (!tp && !(stage[q] < stage[p])) flag("bottom up arc to a non-ancestor level");
+                }
+            }
+            if (L.pos_parent[p] != NONE && L.pos_parent[p] >= n) flag("bad parent position");
         }
-        if (L.pos_parent[p] != NONE && L.pos_parent[p] >= n) { err = "bad parent position"; return false; }
-    }
-    // grouped segment arcs: sources final before the segment's stage
-    for (size_t s = 0; s + 1 < L.seg_pos.size(); ++s) {
-        const uint32_t p0 = L.seg_pos[s], p1 = L.seg_pos[s + 1];
-        if (p1 <= p0 || p1 - p0 > 0xFFFF) { err = "bad segment size"; return false; }
-        for (uint32_t p = p0; p < p1; ++p) {
-            for (uint32_t k = L.sd_off[p]; k < L.sd_off[p + 1]; ++k)
-                if (!(L.sd_tgt[k] > p - p0 && L.sd_tgt[k] < p1 - p0)) { err = "intra DOWN arc not to a later node"; return false; }
-            for (uint32_t k = L.su_off[p]; k < L.su_off[p + 1]; ++k)
-                if (!(L.su_tgt[k] < p - p0)) { err = "intra UP arc not to an earlier node"; return false; }
-        }
-        for (uint32_t k = L.bs_off[s]; k < L.bs_off[s + 1]; ++k) {
-            if (!(L.bs_src[k] < n && stage[L.bs_src[k]] < stage[p0])) { err = "segment source not final"; return false; }
-            for (uint32_t a = L.bs_aoff[k]; a < L.bs_aoff[k + 1]; ++a)
-                if (L.ba_v[a] >= p1 - p0) { err = "segment arc head out of segment"; return false; }
-        }
-    }
+    });
+    if (failed()) return false;
+    for (size_t s = 0; s + 1 < L.seg_pos.size(); ++s)
+        if (L.seg_pos[s + 1] <= L.seg_pos[s]) { err = "empty segment"; return false; }
     // bottom chunks: every bottom position once, sources in the top region
     {
         std::vector<uint8_t> seen(n, 0);
@@ -593,15 +669,6 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
         }
         if (L.ch_pos.size() != n - L.n_top) { err = "chunks miss bottom positions"; return false; }
     }
-    for (size_t s = 0; s + 1 < L.seg_pos.size(); ++s) {
-        const uint32_t p0 = L.seg_pos[s], p1 = L.seg_pos[s + 1];
-        for (uint32_t k = L.ut_off[s]; k < L.ut_off[s + 1]; ++k) {
-            const uint32_t y = L.ut_tgt[k];
-            if (!(y < L.n_top && stage[y] < stage[p0])) { err = "segment up target not an earlier top node"; return false; }
-            for (uint32_t a = L.ut_aoff[k]; a < L.ut_aoff[k + 1]; ++a)
-                if (L.uta_x[a] >= p1 - p0) { err = "segment up arc tail out of segment"; return false; }
-        }
-    }
     // dense tiles: the finite entries of each table are exactly the arcs they
     // replace (parallel arcs merged to their minimum), keys are readable and
     // final before the segment's stage
@@ -615,7 +682,6 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
                     v[o++] = v[i];
             v.resize(o);
         };
-        std::vector<T3> want, got;
         const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
         if (L.xr_off.size() != (size_t)nseg + 1 || L.pt_blk.size() != nseg + 1 ||
             L.tb_dn.size() != 64 * (size_t)nseg || L.tb_up.size() != 64 * (size_t)nseg ||
@@ -625,84 +691,93 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
             err = "dense tile tables disagree";
             return false;
         }
-        for (uint32_t s = 0; s < nseg; ++s) {
-            const uint32_t p0 = L.seg_pos[s], len = L.seg_pos[s + 1] - p0;
-            if (len > 64) { err = "segment longer than a tile"; return false; }
-            // DOWN from outside: (head offset, source, w)
-            want.clear();
-            got.clear();
-            for (uint32_t i = 0; i < len; ++i)
-                for (uint32_t k = L.in_off[p0 + i]; k < L.in_split[p0 + i]; ++k) want.emplace_back(i, L.in_src[k], L.in_w[k]);
-            for (uint32_t r = L.xr_off[s]; r < L.xr_off[s + 1]; ++r) {
-                const uint32_t u = L.xr_src[r];
-                if (u >= n || stage[u] >= stage[p0]) { err = "tile source not final"; return false; }
-                for (uint32_t j = 0; j < 64; ++j) {
-                    const uint32_t w = L.xr_w[64 * (size_t)r + j];
-                    if (w == INF) continue;
-                    if (j >= len) { err = "tile head beyond the segment"; return false; }
-                    got.emplace_back(j, u, w);
-                }
-            }
-            merged(want);
-            std::sort(got.begin(), got.end());
-            if (want != got) { err = "DOWN tiles differ from the arcs they replace"; return false; }
-            // intra: (tail, head, w) for both directions
-            for (int dir = 0; dir < 2; ++dir) {
+        parallel_blocks(nseg, 16, [&](size_t sb, size_t se) {
+            std::vector<T3> want, got;
+            for (uint32_t s = (uint32_t)sb; s < se; ++s) {
+                const uint32_t p0 = L.seg_pos[s], len = L.seg_pos[s + 1] - p0;
+                if (len > 64) { flag("segment longer than a tile"); return; }
+                // DOWN from outside: (head offset, source, w)
                 want.clear();
                 got.clear();
-                for (uint32_t i = 0; i < len; ++i) {
-                    if (dir == 0)
-                        for (uint32_t k = L.in_split[p0 + i]; k < L.in_off[p0 + i + 1]; ++k)
-                            want.emplace_back(L.in_src[k] - p0, i, L.in_w[k]);
-                    else
-                        for (uint32_t k = L.up_split[p0 + i]; k < L.up_off[p0 + i + 1]; ++k)
-                            want.emplace_back(i, L.up_tgt[k], L.up_w[k]);
+                for (uint32_t i = 0; i < len; ++i)
+                    for (uint32_t k = L.in_off[p0 + i]; k < L.in_split[p0 + i]; ++k)
+                        want.emplace_back(i, L.in_src[k], L.in_w[k]);
+                for (uint32_t r = L.xr_off[s]; r < L.xr_off[s + 1]; ++r) {
+                    const uint32_t u = L.xr_src[r];
+                    if (u >= n || stage[u] >= stage[p0]) { flag("tile source not final"); return; }
+                    for (uint32_t j = 0; j < 64; ++j) {
+                        const uint32_t w = L.xr_w[64 * (size_t)r + j];
+                        if (w == INF) continue;
+                        if (j >= len) { flag("tile head beyond the segment"); return; }
+                        got.emplace_back(j, u, w);
+                    }
                 }
-                const auto &tab = dir == 0 ? L.tb_dn : L.tb_up;
-                for (uint32_t a = 0; a < 8; ++a)
-                    for (uint32_t o = 0; o < 8; ++o) {
-                        const uint32_t k = tab[64 * (size_t)s + 8 * a + o];
-                        if (k == NONE) continue;
-                        if ((dir == 0 && a > o) || (dir == 1 && a < o) || 64 * (size_t)k + 64 > L.tb_w.size()) {
-                            err = "bad intra block";
-                            return false;
-                        }
-                        for (uint32_t i = 0; i < 8; ++i)
-                            for (uint32_t j = 0; j < 8; ++j) {
-                                const uint32_t w = L.tb_w[64 * (size_t)k + 8 * i + j];
-                                if (w == INF) continue;
-                                const uint32_t t = 8 * a + i, h = 8 * o + j;
-                                if (t >= len || h >= len || (dir == 0 ? t >= h : t <= h)) {
-                                    err = "intra block arc against the sweep order";
-                                    return false;
-                                }
-                                got.emplace_back(t, h, w);
+                merged(want);
+                std::sort(got.begin(), got.end());
+                if (want != got) { flag("DOWN tiles differ from the arcs they replace"); return; }
+                // intra: (tail, head, w) for both directions
+                for (int dir = 0; dir < 2; ++dir) {
+                    want.clear();
+                    got.clear();
+                    for (uint32_t i = 0; i < len; ++i) {
+                        if (dir == 0)
+                            for (uint32_t k = L.in_split[p0 + i]; k < L.in_off[p0 + i + 1]; ++k)
+                                want.emplace_back(L.in_src[k] - p0, i, L.in_w[k]);
+                        else
+                            for (uint32_t k = L.up_split[p0 + i]; k < L.up_off[p0 + i + 1]; ++k)
+                                want.emplace_back(i, L.up_tgt[k], L.up_w[k]);
+                    }
+                    const auto &tab = dir == 0 ? L.tb_dn : L.tb_up;
+                    for (uint32_t a = 0; a < 8; ++a)
+                        for (uint32_t o = 0; o < 8; ++o) {
+                            const uint32_t k = tab[64 * (size_t)s + 8 * a + o];
+                            if (k == NONE) continue;
+                            if ((dir == 0 && a > o) || (dir == 1 && a < o) || 64 * (size_t)k + 64 > L.tb_w.size()) {
+                                flag("bad intra block");
+                                return;
                             }
+                            for (uint32_t i = 0; i < 8; ++i)
+                                for (uint32_t j = 0; j < 8; ++j) {
+                                    const uint32_t w = L.tb_w[64 * (size_t)k + 8 * i + j];
+                                    if (w == INF) continue;
+                                    const uint32_t t = 8 * a + i, h = 8 * o + j;
+                                    if (t >= len || h >= len || (dir == 0 ? t >= h : t <= h)) {
+                                        flag("intra block arc against the sweep order");
+                                        return;
+                                    }
+                                    got.emplace_back(t, h, w);
+                                }
+                        }
+                    merged(want);
+                    std::sort(got.begin(), got.end());
+                    if (want != got) { flag("intra blocks differ from the arcs they replace"); return; }
+                }
+                // UP leaving: (target, tail position, w)
+                want.clear();
+                got.clear();
+                for (uint32_t i = 0; i < len; ++i)
+                    for (uint32_t k = L.up_off[p0 + i]; k < L.up_split[p0 + i]; ++k)
+                        want.emplace_back(L.up_tgt[k], p0 + i, L.up_w[k]);
+                for (uint32_t kb = L.pt_blk[s]; kb < L.pt_blk[s + 1]; ++kb)
+                    for (uint32_t r = L.pr_off[kb]; r < L.pr_off[kb + 1]; ++r) {
+                        const uint32_t x = L.pr_x[r];
+                        if (x < p0 || x >= p0 + len) { flag("push tail outside the segment"); return; }
+                        for (uint32_t j = 0; j < 64; ++j) {
+                            const uint32_t w = L.pr_w[64 * (size_t)r + j], t = L.pt_tgt[64 * (size_t)kb + j];
+                            if (w == INF) continue;
+                            if (t == NONE || t >= L.n_top || stage[t] >= stage[p0]) {
+                                flag("push target not an earlier top node");
+                                return;
+                            }
+                            got.emplace_back(t, x, w);
+                        }
                     }
                 merged(want);
                 std::sort(got.begin(), got.end());
-                if (want != got) { err = "intra blocks differ from the arcs they replace"; return false; }
+                if (want != got) { flag("UP push tiles differ from the arcs they replace"); return; }
             }
-            // UP leaving: (target, tail position, w)
-            want.clear();
-            got.clear();
-            for (uint32_t i = 0; i < len; ++i)
-                for (uint32_t k = L.up_off[p0 + i]; k < L.up_split[p0 + i]; ++k) want.emplace_back(L.up_tgt[k], p0 + i, L.up_w[k]);
-            for (uint32_t kb = L.pt_blk[s]; kb < L.pt_blk[s + 1]; ++kb)
-                for (uint32_t r = L.pr_off[kb]; r < L.pr_off[kb + 1]; ++r) {
-                    const uint32_t x = L.pr_x[r];
-                    if (x < p0 || x >= p0 + len) { err = "push tail outside the segment"; return false; }
-                    for (uint32_t j = 0; j < 64; ++j) {
-                        const uint32_t w = L.pr_w[64 * (size_t)r + j], t = L.pt_tgt[64 * (size_t)kb + j];
-                        if (w == INF) continue;
-                        if (t == NONE || t >= L.n_top || stage[t] >= stage[p0]) { err = "push target not an earlier top node"; return false; }
-                        got.emplace_back(t, x, w);
-                    }
-                }
-            merged(want);
-            std::sort(got.begin(), got.end());
-            if (want != got) { err = "UP push tiles differ from the arcs they replace"; return false; }
-        }
+        });
+        if (failed()) return false;
         // DOWN work units: the units of a level cover the rows of its segments once
         if (L.du_lvl.size() != L.seglvl_seg.size() || L.du_r0.size() != L.du_seg.size() ||
             L.du_r1.size() != L.du_seg.size()) {

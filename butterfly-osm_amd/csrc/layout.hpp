@@ -14,17 +14,18 @@
 // top region T (the separators of the nested dissection and the upper part of
 // its leaves); the rest is the bottom region B, a forest of small subtrees.
 //   * T is cut into chains (maximal single-child paths) and the chains into
-//     segments of at most `segment_limit` nodes.  A segment depends on the
-//     segment holding its head's parent; segments of one segment level are
-//     independent.  Arcs into a segment node from outside the segment are
-//     relaxed in parallel (pull); arcs inside it are relaxed sequentially by
-//     one workgroup from LDS.
+//     segments of at most `segment_limit` (64) nodes.  A segment depends on
+//     the segment holding its head's parent; segments of one segment level
+//     are independent.  A segment is one register tile: arcs into it from
+//     outside are dense rows (source, 64 weights) relaxed in parallel, arcs
+//     inside it 8x8 blocks relaxed in order within each thread.
 //   * B is swept level-synchronously by subtree height (descending for the
 //     downward sweep): nodes of equal height are never ancestor-related.
 //
-// Positions.  Nodes are renumbered: T first (segment levels ascending, a
-// segment's nodes in chain order, parent first), then B by descending height
-// (pre-order within a height).  Label arenas are indexed by position.
+// Positions.  Nodes are renumbered: T first (segment levels ascending, the
+// segments of a level in pre-order of their heads, a segment's nodes in chain
+// order, parent first), then B by descending height (pre-order within a
+// height).  Label arenas are indexed by position.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -73,11 +74,6 @@ struct Layout {
     std::vector<uint32_t> seg_pos;        // n_seg+1: segment s covers [seg_pos[s], seg_pos[s+1])
     std::vector<uint32_t> seglvl_seg;     // n_seglvl+1: level l covers segments [seglvl_seg[l], seglvl_seg[l+1])
     std::vector<uint32_t> botlvl_pos;     // n_botlvl+1: bottom level l covers positions [botlvl_pos[l], botlvl_pos[l+1])
-    // DOWN arcs entering segment s from outside it, grouped by source:
-    // sources bs_src[bs_off[s] .. bs_off[s+1]), arcs of source k are
-    // [bs_aoff[k], bs_aoff[k+1]) with head ba_v (offset in the segment) and weight ba_w
-    std::vector<uint32_t> bs_off, bs_src, bs_aoff, ba_w;
-    std::vector<uint16_t> ba_v;
     // Bottom chunks: bottom nodes in pre-order cut into chunks of at most
     // block_limit nodes (members of one chunk share their top ancestors).
     // Chunk c holds positions ch_pos[ch_off[c] .. ch_off[c+1]); its DOWN arcs
@@ -86,17 +82,6 @@ struct Layout {
     // bottom sources, [in_split[p], in_off[p+1]) from top sources.
     std::vector<uint32_t> ch_off, ch_pos, cs_off, cs_src, cs_aoff, ca_w;
     std::vector<uint16_t> ca_v;
-    uint32_t max_block = 0;
-    // UP arcs leaving segment s, grouped by target: targets ut_tgt[ut_off[s] ..
-    // ut_off[s+1]) (positions), arcs [ut_aoff[k], ut_aoff[k+1]) with tail uta_x
-    // (offset in the segment) and weight uta_w
-    std::vector<uint32_t> ut_off, ut_tgt, ut_aoff, uta_w;
-    std::vector<uint16_t> uta_x;
-    // intra-segment arcs by tail position p (top region): DOWN arcs to later
-    // nodes [sd_off[p], sd_off[p+1]) and UP arcs to earlier nodes
-    // [su_off[p], su_off[p+1]); head as offset from the segment start
-    std::vector<uint32_t> sd_off, sd_w, su_off, su_w;
-    std::vector<uint16_t> sd_tgt, su_tgt;
     // Dense tiles of the top region (segments of at most 64 nodes; a thread of
     // the segment kernels holds one lane's labels of a whole segment).  A row
     // is a key (a position) and 64 weights, the key's arcs to the 64 columns
