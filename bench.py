@@ -103,7 +103,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="matrix10k", choices=sorted(WORKLOADS))
-    ap.add_argument("--lanes", type=int, default=32, help="K lanes per group (16, 32, 64)")
+    ap.add_argument("--lanes", type=int, default=64, help="K lanes per group (16, 32, 64)")
     ap.add_argument("--lane-capacity", type=int, default=0, help="sources per device batch (0 = auto)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of single-thread CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")

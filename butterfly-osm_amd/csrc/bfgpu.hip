@@ -41,6 +41,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -254,27 +255,45 @@ __device__ __forceinline__ uint32_t *lane_labels(uint32_t *arena, uint32_t n, ui
     return arena + (size_t)g * n * KL + l % KL;
 }
 
-// acc[j] = min(acc[j], lab[key[r]] + wts[64r + j]) over rows [r0, r1); two
-// rows per step (min(acc, x0, x1) is one v_min3), the labels of the next two
-// rows loaded one step ahead.  An odd last row pairs with itself and INF.
-template <int KL>
-__device__ __forceinline__ void rows64(uint32_t (&acc)[64], uint32_t r0, uint32_t r1,
-                                       const uint32_t *__restrict__ key, const uint32_t *__restrict__ wts,
+// acc[j] = min(acc[j], lab[key[r]] + wr[W(r - r0) + j]) over rows [r0, r1)
+// of width W; two rows per step (min(acc, x0, x1) is one v_min3), the labels
+// of the next two rows loaded one step ahead.  An odd last row pairs with
+// itself and INF.
+template <int KL, int W>
+__device__ __forceinline__ void rows_w(uint32_t (&acc)[W], uint32_t r0, uint32_t r1, const cu32 *K, const cu32 *wr,
                                        const uint32_t *lab) {
     if (r0 >= r1) return;
-    const cu32 *K = as_const(key);
     uint32_t n0 = lab[(size_t)K[r0] * KL];
     uint32_t n1 = r0 + 1 < r1 ? lab[(size_t)K[r0 + 1] * KL] : INF;
     for (uint32_t r = r0; r < r1; r += 2) {
         const uint32_t a0 = n0, a1 = n1;
-        const cu32 *W0 = as_const(wts) + (size_t)r * 64;
-        const cu32 *W1 = r + 1 < r1 ? W0 + 64 : W0;
+        const cu32 *W0 = wr + (size_t)(r - r0) * W;
+        const cu32 *W1 = r + 1 < r1 ? W0 + W : W0;
         if (r + 2 < r1) {
             n0 = lab[(size_t)K[r + 2] * KL];
             n1 = r + 3 < r1 ? lab[(size_t)K[r + 3] * KL] : INF;
         }
 #pragma unroll
-        for (int j = 0; j < 64; ++j) acc[j] = min(acc[j], min(sat_add(a0, W0[j]), sat_add(a1, W1[j])));
+        for (int j = 0; j < W; ++j) acc[j] = min(acc[j], min(sat_add(a0, W0[j]), sat_add(a1, W1[j])));
+    }
+}
+
+// One unit of rows_kernel at width W: rows [ra, rb) with weights from wr,
+// merged into positions out[j] (PUSH) or p0 + j (j < len) of the arena.
+template <int KL, int W, bool PUSH>
+__device__ __forceinline__ void rows_unit(uint32_t ra, uint32_t rb, const cu32 *K, const cu32 *wr, const cu32 *out,
+                                          uint32_t p0, uint32_t len, uint32_t *lab) {
+    uint32_t acc[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) acc[j] = INF;
+    rows_w<KL, W>(acc, ra, rb, K, wr, lab);
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        if constexpr (PUSH) {
+            if (out[j] != NONE && acc[j] != INF) atomicMin(lab + (size_t)out[j] * KL, acc[j]);
+        } else {
+            if (j < len && acc[j] != INF) atomicMin(lab + (size_t)(p0 + j) * KL, acc[j]);
+        }
     }
 }
 
@@ -282,16 +301,18 @@ __device__ __forceinline__ void rows64(uint32_t (&acc)[64], uint32_t r0, uint32_
 // unit u0 + 4*blockIdx.y + w and merges its rows into the arena with one
 // atomic min per finite (position, lane).
 //   DOWN (PUSH = false): unit = rows [r0[u], r1[u]) of segment seg[u] (DOWN
-//     arcs from outside the segment); positions are the segment's nodes,
-//     which hold the upward labels.
-//   UP (PUSH = true): unit = target block u of the level, rows
-//     [r0[u], r0[u+1]) keyed by tails in segment seg[u] (final up labels);
-//     positions are tgt[64u ..]; a wavefront none of whose groups reaches
-//     the segment stops at once.
+//     arcs from outside the segment), width wid[s], weights from
+//     woff[s] + (r - rbase[s]) * W; positions are the segment's nodes, which
+//     hold the upward labels.
+//   UP (PUSH = true): unit = target block u, rows [r0[u], r0[u+1]) keyed by
+//     tails in segment seg[u] (final up labels), width wid[u], weights from
+//     woff[u]; positions are tgt[64u ..]; a wavefront none of whose groups
+//     reaches the segment stops at once.
 template <int KL, bool PUSH>
 __global__ __launch_bounds__(TILE_THREADS) void rows_kernel(
     uint32_t u0, uint32_t u1, const uint32_t *__restrict__ seg, const uint32_t *__restrict__ r0,
-    const uint32_t *__restrict__ r1, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ tgt,
+    const uint32_t *__restrict__ r1, const uint32_t *__restrict__ rbase, const uint32_t *__restrict__ wid,
+    const uint32_t *__restrict__ woff, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ tgt,
     const uint32_t *__restrict__ key, const uint32_t *__restrict__ wts, const uint8_t *__restrict__ active,
     uint32_t nseg, uint32_t *__restrict__ arena, uint32_t n) {
     const uint32_t u = u0 + blockIdx.y * (TILE_THREADS / 64) + wave_id();
@@ -302,21 +323,26 @@ __global__ __launch_bounds__(TILE_THREADS) void rows_kernel(
     if constexpr (PUSH) {
         if (__ballot(active[(size_t)g * nseg + s] != 0) == 0) return;
     }
-    uint32_t acc[64];
-#pragma unroll
-    for (int j = 0; j < 64; ++j) acc[j] = INF;
-    rows64<KL>(acc, r0[u], PUSH ? r0[u + 1] : r1[u], key, wts, lab);
+    const cu32 *K = as_const(key);
+    uint32_t ra, rb, W, p0 = 0, len = 0;
+    const cu32 *wr, *out = nullptr;
     if constexpr (PUSH) {
-        const cu32 *t = as_const(tgt) + (size_t)u * 64;
-#pragma unroll
-        for (int j = 0; j < 64; ++j)
-            if (t[j] != NONE && acc[j] != INF) atomicMin(lab + (size_t)t[j] * KL, acc[j]);
+        ra = r0[u];
+        rb = r0[u + 1];
+        W = wid[u];
+        wr = as_const(wts) + woff[u];
+        out = as_const(tgt) + (size_t)u * 64;
     } else {
-        const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
-#pragma unroll
-        for (int j = 0; j < 64; ++j)
-            if (j < len && acc[j] != INF) atomicMin(lab + (size_t)(p0 + j) * KL, acc[j]);
+        ra = r0[u];
+        rb = r1[u];
+        W = wid[s];
+        wr = as_const(wts) + woff[s] + (size_t)(ra - rbase[s]) * W;
+        p0 = seg_pos[s];
+        len = seg_pos[s + 1] - p0;
     }
+    if (W == 16) rows_unit<KL, 16, PUSH>(ra, rb, K, wr, out, p0, len, lab);
+    else if (W == 32) rows_unit<KL, 32, PUSH>(ra, rb, K, wr, out, p0, len, lab);
+    else rows_unit<KL, 64, PUSH>(ra, rb, K, wr, out, p0, len, lab);
 }
 
 // tri: wavefront w of workgroup (wave group blockIdx.x, blockIdx.y) takes
@@ -664,6 +690,22 @@ struct DevBuf {
 
 constexpr int SEG = 64;     // nodes of a segment (one register tile)
 constexpr int CHUNK = 256;  // nodes of a bottom chunk (LDS = CHUNK * KL * 4 bytes)
+// Subtree size at or below which a node is in the bottom region.  Small: the
+// dense separator chains of the nested dissection become register tiles and a
+// matrix's target paths through the bottom stay short.
+constexpr uint32_t BOTTOM_LIMIT = 256;
+
+bfg::LayoutParams layout_params() {
+    bfg::LayoutParams lp;
+    lp.segment_limit = SEG;
+    lp.block_limit = CHUNK;
+    lp.bottom_limit = BOTTOM_LIMIT;
+    if (const char *e = std::getenv("BFG_BOTTOM_LIMIT")) {  // tuning override
+        const long v = std::strtol(e, nullptr, 10);
+        if (v > 0 && v < (1L << 30)) lp.bottom_limit = (uint32_t)v;
+    }
+    return lp;
+}
 
 }  // namespace
 
@@ -672,7 +714,7 @@ struct bfg_ctx {
     hipStream_t stream = nullptr;
     std::mutex mu;
     bfg::Layout L;
-    uint32_t lanes = 32;        // KL: sources per lane group (16, 32 or 64)
+    uint32_t lanes = 64;        // KL: sources per lane group (16, 32 or 64)
     uint32_t lane_cap = 0;      // lanes per batch (multiple of 64)
     bool cap_fixed = false;
     // resident graph
@@ -681,7 +723,8 @@ struct bfg_ctx {
     DevBuf<uint32_t> ch_off, ch_pos, cs_off, cs_src, cs_aoff, ca_w;
     DevBuf<uint16_t> ca_v;
     DevBuf<uint32_t> up_off, up_tgt, up_w;
-    DevBuf<uint32_t> xr_off, xr_src, xr_w, tb_dn, tb_up, tb_w, pt_blk, pt_tgt, pr_off, pr_x, pr_w;
+    DevBuf<uint32_t> xr_off, xr_wid, xw_off, xr_src, xr_w, tb_dn, tb_up, tb_w, pt_blk, pt_tgt, pr_off, pr_wid,
+        pw_off, pr_x, pr_w;
     DevBuf<uint32_t> du_seg, du_r0, du_r1, pk_seg;
     // arenas (per batch)
     DevBuf<uint32_t> arena, pathbuf;
@@ -782,8 +825,9 @@ int launch_phases(bfg_ctx *c, uint32_t nl, float *up_ms, float *sweep_ms, uint32
         const uint32_t k0 = L.pt_blk[s0], k1 = L.pt_blk[s1];
         if (k1 > k0) {
             hipLaunchKernelGGL((rows_kernel<KL, true>), dim3(wgs, (k1 - k0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0,
-                               s, k0, k1, c->pk_seg.p, c->pr_off.p, c->pr_off.p, c->seg_pos.p, c->pt_tgt.p, c->pr_x.p,
-                               c->pr_w.p, c->active.p, nseg, c->arena.p, n);
+                               s, k0, k1, c->pk_seg.p, c->pr_off.p, c->pr_off.p, c->pr_off.p, c->pr_wid.p,
+                               c->pw_off.p, c->seg_pos.p, c->pt_tgt.p, c->pr_x.p, c->pr_w.p, c->active.p, nseg,
+                               c->arena.p, n);
             ++nlaunch;
         }
     }
@@ -794,8 +838,9 @@ int launch_phases(bfg_ctx *c, uint32_t nl, float *up_ms, float *sweep_ms, uint32
         const uint32_t u0 = L.du_lvl[l], u1 = L.du_lvl[l + 1];
         if (u1 > u0) {
             hipLaunchKernelGGL((rows_kernel<KL, false>), dim3(wgs, (u1 - u0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0,
-                               s, u0, u1, c->du_seg.p, c->du_r0.p, c->du_r1.p, c->seg_pos.p, c->pt_tgt.p, c->xr_src.p,
-                               c->xr_w.p, c->active.p, nseg, c->arena.p, n);
+                               s, u0, u1, c->du_seg.p, c->du_r0.p, c->du_r1.p, c->xr_off.p, c->xr_wid.p,
+                               c->xw_off.p, c->seg_pos.p, c->pt_tgt.p, c->xr_src.p, c->xr_w.p, c->active.p, nseg,
+                               c->arena.p, n);
             ++nlaunch;
         }
         hipLaunchKernelGGL((tri_kernel<KL, true>), dim3(wgs, (s1 - s0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s,
@@ -1096,9 +1141,7 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     HIP_TRY(hipSetDevice(device));
     const bfg::CchInput in = input_of(gd);
     std::string err;
-    bfg::LayoutParams lp;
-    lp.segment_limit = SEG;
-    lp.block_limit = CHUNK;
+    const bfg::LayoutParams lp = layout_params();
     if (!bfg::build_layout(in, lp, c->L, err)) return fail(BFG_INVALID_PARAMETER, err);
     // every slot, position and segment index the kernels derive rests on these
     // properties; refuse the graph rather than launch on a broken layout
@@ -1129,6 +1172,10 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     HIP_TRY(c->up_tgt.upload(L.up_tgt, s));
     HIP_TRY(c->up_w.upload(L.up_w, s));
     HIP_TRY(c->xr_off.upload(L.xr_off, s));
+    HIP_TRY(c->xr_wid.upload(L.xr_wid, s));
+    HIP_TRY(c->xw_off.upload(L.xw_off, s));
+    HIP_TRY(c->pr_wid.upload(L.pr_wid, s));
+    HIP_TRY(c->pw_off.upload(L.pw_off, s));
     HIP_TRY(c->xr_src.upload(L.xr_src, s));
     HIP_TRY(c->xr_w.upload(L.xr_w, s));
     HIP_TRY(c->tb_dn.upload(L.tb_dn, s));
@@ -1147,7 +1194,8 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     c->graph_bytes = 0;
     for (auto *b : {&c->parent, &c->pos_parent, &c->pos_pre, &c->pos_sub, &c->pos_depth, &c->seg_pos, &c->in_off,
                     &c->in_split, &c->in_src, &c->in_w, &c->ch_off, &c->ch_pos, &c->cs_off, &c->cs_src, &c->cs_aoff,
-                    &c->ca_w, &c->up_off, &c->up_tgt, &c->up_w, &c->xr_off, &c->xr_src, &c->xr_w, &c->tb_dn,
+                    &c->ca_w, &c->up_off, &c->up_tgt, &c->up_w, &c->xr_off, &c->xr_wid, &c->xw_off, &c->pr_wid,
+                    &c->pw_off, &c->xr_src, &c->xr_w, &c->tb_dn,
                     &c->tb_up, &c->tb_w, &c->pt_blk, &c->pt_tgt, &c->pr_off, &c->pr_x, &c->pr_w, &c->du_seg,
                     &c->du_r0, &c->du_r1, &c->pk_seg})
         c->graph_bytes += b->n * sizeof(uint32_t);
@@ -1177,9 +1225,7 @@ int bfg_validate_graph(const bfg_graph_desc *gd, bfg_ctx_info *info) {
     const bfg::CchInput in = input_of(gd);
     bfg::Layout L;
     std::string err;
-    bfg::LayoutParams lp;
-    lp.segment_limit = SEG;
-    lp.block_limit = CHUNK;
+    const bfg::LayoutParams lp = layout_params();
     if (!bfg::build_layout(in, lp, L, err)) return fail(BFG_INVALID_PARAMETER, err);
     if (!bfg::check_layout(in, L, err)) return fail(BFG_INTERNAL_ERROR, err);
     fill_info(L, info);

@@ -52,11 +52,14 @@ struct Arc3 {
     uint32_t key, col, w;  // key: source (or tail), col: column in a 64-wide block
 };
 
-// Appends the rows of one 64-column block from arcs (key, column, weight):
+// Columns of a row for a block of `cols` columns: 16, 32 or 64.
+uint32_t width_class(uint32_t cols) { return cols <= 16 ? 16u : cols <= 32 ? 32u : 64u; }
+
+// Appends the rows of one block of W columns from arcs (key, column, weight):
 // one row per distinct key, ascending, holding the key and the weights of its
-// arcs to the 64 columns (INF where there is none; parallel arcs keep the
+// arcs to the W columns (INF where there is none; parallel arcs keep the
 // minimum).  Returns the number of rows.
-uint32_t build_rows(std::vector<Arc3> &arcs, std::vector<uint32_t> &rkey, std::vector<uint32_t> &rw) {
+uint32_t build_rows(std::vector<Arc3> &arcs, uint32_t W, std::vector<uint32_t> &rkey, std::vector<uint32_t> &rw) {
     std::sort(arcs.begin(), arcs.end(), [](const Arc3 &a, const Arc3 &b) {
         return a.key != b.key ? a.key < b.key : a.col < b.col;
     });
@@ -64,7 +67,7 @@ uint32_t build_rows(std::vector<Arc3> &arcs, std::vector<uint32_t> &rkey, std::v
     for (size_t i = 0; i < arcs.size();) {
         const uint32_t key = arcs[i].key;
         const size_t base = rw.size();
-        rw.resize(base + 64, INF);
+        rw.resize(base + W, INF);
         for (; i < arcs.size() && arcs[i].key == key; ++i)
             rw[base + arcs[i].col] = std::min(rw[base + arcs[i].col], arcs[i].w);
         rkey.push_back(key);
@@ -452,10 +455,10 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
         const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
         if (L.max_segment > 64) { err = "segment_limit above 64"; return false; }
         struct Tiles {
-            std::vector<uint32_t> xr_src, xr_w;                 // DOWN rows
+            std::vector<uint32_t> xr_src, xr_w;                 // DOWN rows (width_class(len) wide)
             std::vector<std::pair<uint32_t, uint32_t>> blocks;  // (tab 0/1 * 64 + 8a + o, local block)
             std::vector<uint32_t> tb_w;                         // 64 weights per local block
-            std::vector<uint32_t> pt_tgt, pr_cnt, pr_x, pr_w;   // UP push: targets, rows per block, rows
+            std::vector<uint32_t> pt_tgt, pr_cnt, pr_wid, pr_x, pr_w;  // UP push: targets, rows and width per block
         };
         std::vector<Tiles> tl(nseg);
         parallel_blocks(nseg, 16, [&](size_t sb, size_t se) {
@@ -469,7 +472,7 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
                 for (uint32_t j = 0; j < len; ++j)
                     for (uint32_t k = L.in_off[p0 + j]; k < L.in_split[p0 + j]; ++k)
                         a3.push_back({L.in_src[k], j, L.in_w[k]});
-                build_rows(a3, t.xr_src, t.xr_w);
+                build_rows(a3, width_class(len), t.xr_src, t.xr_w);
                 uint32_t tab[2][64];
                 std::fill(&tab[0][0], &tab[0][0] + 128, NONE);
                 auto put = [&](int d, uint32_t a, uint32_t o, uint32_t i, uint32_t j, uint32_t w) {
@@ -513,11 +516,17 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
                     a3.clear();
                     for (; q < outs.size() && colv[q] < 64 * (kb + 1); ++q)
                         a3.push_back({std::get<1>(outs[q]), colv[q] - 64 * kb, std::get<2>(outs[q])});
-                    t.pr_cnt.push_back(build_rows(a3, t.pr_x, t.pr_w));
+                    const uint32_t W = width_class(std::min<uint32_t>(64, nt - 64 * kb));
+                    t.pr_wid.push_back(W);
+                    t.pr_cnt.push_back(build_rows(a3, W, t.pr_x, t.pr_w));
                 }
             }
         });
         L.xr_off.assign(nseg + 1, 0);
+        L.xr_wid.assign(nseg, 0);
+        L.xw_off.assign(nseg + 1, 0);
+        L.pr_wid.clear();
+        L.pw_off.clear();
         L.tb_dn.assign(64 * (size_t)nseg, NONE);
         L.tb_up.assign(64 * (size_t)nseg, NONE);
         L.pt_blk.assign(nseg + 1, 0);
@@ -537,6 +546,8 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
         for (uint32_t s = 0; s < nseg; ++s) {
             Tiles &t = tl[s];
             L.xr_off[s] = (uint32_t)L.xr_src.size();
+            L.xr_wid[s] = width_class(L.seg_pos[s + 1] - L.seg_pos[s]);
+            L.xw_off[s] = (uint32_t)L.xr_w.size();
             L.xr_src.insert(L.xr_src.end(), t.xr_src.begin(), t.xr_src.end());
             L.xr_w.insert(L.xr_w.end(), t.xr_w.begin(), t.xr_w.end());
             const uint32_t b0 = (uint32_t)(L.tb_w.size() / 64);
@@ -545,10 +556,13 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
             L.tb_w.insert(L.tb_w.end(), t.tb_w.begin(), t.tb_w.end());
             L.pt_blk[s] = (uint32_t)(L.pt_tgt.size() / 64);
             L.pt_tgt.insert(L.pt_tgt.end(), t.pt_tgt.begin(), t.pt_tgt.end());
-            uint32_t r = (uint32_t)L.pr_x.size();
-            for (uint32_t c : t.pr_cnt) {
+            uint32_t r = (uint32_t)L.pr_x.size(), w = (uint32_t)L.pr_w.size();
+            for (size_t k = 0; k < t.pr_cnt.size(); ++k) {
                 L.pr_off.push_back(r);
-                r += c;
+                L.pw_off.push_back(w);
+                L.pr_wid.push_back(t.pr_wid[k]);
+                r += t.pr_cnt[k];
+                w += t.pr_cnt[k] * t.pr_wid[k];
             }
             L.pr_x.insert(L.pr_x.end(), t.pr_x.begin(), t.pr_x.end());
             L.pr_w.insert(L.pr_w.end(), t.pr_w.begin(), t.pr_w.end());
@@ -556,8 +570,10 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
             Tiles().pr_w.swap(t.pr_w);
         }
         L.xr_off[nseg] = (uint32_t)L.xr_src.size();
+        L.xw_off[nseg] = (uint32_t)L.xr_w.size();
         L.pt_blk[nseg] = (uint32_t)(L.pt_tgt.size() / 64);
         L.pr_off.push_back((uint32_t)L.pr_x.size());
+        L.pw_off.push_back((uint32_t)L.pr_w.size());
         const uint32_t R = std::max<uint32_t>(lp.ell_chunk, 1);
         L.du_lvl.clear(); L.du_seg.clear(); L.du_r0.clear(); L.du_r1.clear();
         for (size_t l = 0; l + 1 < L.seglvl_seg.size(); ++l) {
@@ -683,13 +699,33 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
             v.resize(o);
         };
         const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
+        const size_t nblk = L.pt_blk.empty() ? 0 : L.pt_blk.back();
         if (L.xr_off.size() != (size_t)nseg + 1 || L.pt_blk.size() != nseg + 1 ||
             L.tb_dn.size() != 64 * (size_t)nseg || L.tb_up.size() != 64 * (size_t)nseg ||
-            L.xr_w.size() != 64 * (size_t)L.xr_src.size() || L.pr_w.size() != 64 * (size_t)L.pr_x.size() ||
-            L.xr_off[nseg] != L.xr_src.size() || L.pr_off.size() != (size_t)L.pt_blk[nseg] + 1 ||
-            L.pr_off.back() != L.pr_x.size() || L.pt_tgt.size() != 64 * (size_t)L.pt_blk[nseg]) {
+            L.xr_wid.size() != nseg || L.xw_off.size() != (size_t)nseg + 1 || L.xr_off[nseg] != L.xr_src.size() ||
+            L.xw_off[nseg] != L.xr_w.size() || L.pr_off.size() != nblk + 1 || L.pw_off.size() != nblk + 1 ||
+            L.pr_wid.size() != nblk || L.pr_off.back() != L.pr_x.size() || L.pw_off.back() != L.pr_w.size() ||
+            L.pt_tgt.size() != 64 * nblk) {
             err = "dense tile tables disagree";
             return false;
+        }
+        for (uint32_t s = 0; s < nseg; ++s) {
+            const uint32_t W = L.xr_wid[s], len = L.seg_pos[s + 1] - L.seg_pos[s];
+            if ((W != 16 && W != 32 && W != 64) || len > W ||
+                (uint64_t)L.xw_off[s + 1] - L.xw_off[s] != (uint64_t)W * (L.xr_off[s + 1] - L.xr_off[s])) {
+                err = "bad row width of a segment";
+                return false;
+            }
+        }
+        for (size_t k = 0; k < nblk; ++k) {
+            const uint32_t W = L.pr_wid[k];
+            if ((W != 16 && W != 32 && W != 64) ||
+                (uint64_t)L.pw_off[k + 1] - L.pw_off[k] != (uint64_t)W * (L.pr_off[k + 1] - L.pr_off[k])) {
+                err = "bad row width of a target block";
+                return false;
+            }
+            for (uint32_t j = W; j < 64; ++j)
+                if (L.pt_tgt[64 * k + j] != NONE) { err = "target beyond a block's row width"; return false; }
         }
         parallel_blocks(nseg, 16, [&](size_t sb, size_t se) {
             std::vector<T3> want, got;
@@ -702,11 +738,12 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
                 for (uint32_t i = 0; i < len; ++i)
                     for (uint32_t k = L.in_off[p0 + i]; k < L.in_split[p0 + i]; ++k)
                         want.emplace_back(i, L.in_src[k], L.in_w[k]);
+                const uint32_t W = L.xr_wid[s];
                 for (uint32_t r = L.xr_off[s]; r < L.xr_off[s + 1]; ++r) {
                     const uint32_t u = L.xr_src[r];
                     if (u >= n || stage[u] >= stage[p0]) { flag("tile source not final"); return; }
-                    for (uint32_t j = 0; j < 64; ++j) {
-                        const uint32_t w = L.xr_w[64 * (size_t)r + j];
+                    for (uint32_t j = 0; j < W; ++j) {
+                        const uint32_t w = L.xr_w[L.xw_off[s] + (size_t)(r - L.xr_off[s]) * W + j];
                         if (w == INF) continue;
                         if (j >= len) { flag("tile head beyond the segment"); return; }
                         got.emplace_back(j, u, w);
@@ -760,10 +797,11 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
                         want.emplace_back(L.up_tgt[k], p0 + i, L.up_w[k]);
                 for (uint32_t kb = L.pt_blk[s]; kb < L.pt_blk[s + 1]; ++kb)
                     for (uint32_t r = L.pr_off[kb]; r < L.pr_off[kb + 1]; ++r) {
-                        const uint32_t x = L.pr_x[r];
+                        const uint32_t x = L.pr_x[r], W = L.pr_wid[kb];
                         if (x < p0 || x >= p0 + len) { flag("push tail outside the segment"); return; }
-                        for (uint32_t j = 0; j < 64; ++j) {
-                            const uint32_t w = L.pr_w[64 * (size_t)r + j], t = L.pt_tgt[64 * (size_t)kb + j];
+                        for (uint32_t j = 0; j < W; ++j) {
+                            const uint32_t w = L.pr_w[L.pw_off[kb] + (size_t)(r - L.pr_off[kb]) * W + j];
+                            const uint32_t t = L.pt_tgt[64 * (size_t)kb + j];
                             if (w == INF) continue;
                             if (t == NONE || t >= L.n_top || stage[t] >= stage[p0]) {
                                 flag("push target not an earlier top node");

@@ -84,21 +84,23 @@ struct Layout {
     std::vector<uint16_t> ca_v;
     // Dense tiles of the top region (segments of at most 64 nodes; a thread of
     // the segment kernels holds one lane's labels of a whole segment).  A row
-    // is a key (a position) and 64 weights, the key's arcs to the 64 columns
-    // of a block (INF where there is none).
+    // is a key (a position) and W weights, the key's arcs to the W columns of
+    // a block (INF where there is none); W (16, 32 or 64) is the smallest
+    // width that covers the block.
     //   DOWN arcs entering segment s from outside: rows [xr_off[s],
-    //   xr_off[s+1]), key xr_src[r] (source), weights xr_w[64r ..] to the
-    //   segment's nodes.
+    //   xr_off[s+1]) of width xr_wid[s], key xr_src[r] (source), weights
+    //   xr_w[xw_off[s] + (r - xr_off[s]) * W ..] to the segment's nodes.
     //   Intra-segment arcs: 8x8 blocks W[i][j] (tail i of octet a, head j of
     //   octet o) at tb_w[64k ..], k = tb_dn[64s + 8a + o] (a <= o) or
     //   tb_up[64s + 8a + o] (a >= o), NONE where the block has no arc.
     //   UP arcs leaving segment s: its targets (positions) pt_tgt[64k .. 64k+64)
     //   for target blocks k in [pt_blk[s], pt_blk[s+1]) (NONE pads a block);
-    //   rows [pr_off[k], pr_off[k+1]), key pr_x[r] (tail position), weights
-    //   pr_w[64r ..] to the block's targets.
-    std::vector<uint32_t> xr_off, xr_src, xr_w;
+    //   rows [pr_off[k], pr_off[k+1]) of width pr_wid[k], key pr_x[r] (tail
+    //   position), weights pr_w[pw_off[k] + (r - pr_off[k]) * W ..] to the
+    //   block's targets.
+    std::vector<uint32_t> xr_off, xr_wid, xw_off, xr_src, xr_w;
     std::vector<uint32_t> tb_dn, tb_up, tb_w;
-    std::vector<uint32_t> pt_blk, pt_tgt, pr_off, pr_x, pr_w;
+    std::vector<uint32_t> pt_blk, pt_tgt, pr_off, pr_wid, pw_off, pr_x, pr_w;
     // Work units of the DOWN rows: unit u relaxes rows [du_r0[u], du_r1[u]) of
     // segment du_seg[u] (at most ell_chunk rows); segment level l owns units
     // [du_lvl[l], du_lvl[l+1]).  pk_seg[k] is the segment of target block k.

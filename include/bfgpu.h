@@ -82,7 +82,9 @@ typedef struct bfg_timings {
 } bfg_timings;
 
 /* BatchedPhastEngine::new (batched_phast.rs:89): copies the CCH to `device`,
- * derives the device layout and keeps it resident until destroy. */
+ * derives the device layout and keeps it resident until destroy.  The
+ * environment variable BFG_BOTTOM_LIMIT (a subtree size, default 256) tunes
+ * where the layout switches from register tiles to per-node bottom sweeps. */
 int bfg_ctx_create(const bfg_graph_desc *graph, int device, bfg_ctx **out);
 void bfg_ctx_destroy(bfg_ctx *ctx);
 const char *bfg_last_error(void);
