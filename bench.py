@@ -60,6 +60,44 @@ def make_queries(g, count, seed, hot_frac=0.2, n_hot=5, sigma_frac=0.02):
     return rng.permutation(np.concatenate([uni, idx])).astype(np.uint32)
 
 
+def copy_probe(nbytes):
+    """One device-to-device copy of nbytes by a plain copy kernel (torch's),
+    nothing else on the GPU: its FETCH_SIZE and WRITE_SIZE readings per byte
+    copied scale the same counters read for the path's kernels."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {nbytes} bytes")
+
+
+def summarize_counters(path):
+    import csv
+    import collections
+    agg = collections.defaultdict(lambda: [0, 0.0])
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            k = (r["Kernel_Name"], r["Counter_Name"])
+            agg[k][0] += 1
+            agg[k][1] += float(r["Counter_Value"])
+    for (name, ctr), (cnt, tot) in sorted(agg.items(), key=lambda x: -x[1][1]):
+        print(f"{ctr:12s} dispatches={cnt:6d} mean={tot / cnt:14.1f} total={tot:16.1f}  {name[:110]}")
+
+
+def committed_traffic(workload, lanes):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary for this workload (FETCH_SIZE + WRITE_SIZE, scaled by a copy
+    kernel of known size), or None."""
+    path = os.path.join(REPO, "profiles", f"rows_down_{workload}_k{lanes}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return {"bytes_per_launch": d["hbm_bytes_per_launch"], "source": os.path.relpath(path, REPO)}
+
+
 def cpu_baseline(g, src, tgt, budget_s, gpu_rows):
     """Single-thread oracle (compute_matrix_flat, as the reference runs it) on a
     bounded prefix of the sources; the sample rows are checked against the GPU."""
@@ -109,7 +147,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph-cache", default=os.environ.get("BFG_GRAPH_CACHE", ""))
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run one plain device copy of BYTES (calibrates FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--summarize-counters", default="", metavar="CSV",
+                    help="print the per-dispatch mean of each counter per kernel of a rocprofv3 "
+                         "counter_collection.csv and exit")
     args = ap.parse_args()
+    if args.summarize_counters:
+        summarize_counters(args.summarize_counters)
+        return
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU: --gpus {args.gpus} is not supported by this build "
               f"(the path shards by sources; see DESIGN.md for the multi-GPU design)", file=sys.stderr)
@@ -146,8 +195,8 @@ def main():
         log(f"warmup {i}: up {st.upward_time_ms:.1f} ms sweep {st.downward_time_ms:.1f} ms "
             f"out {st.output_time_ms:.1f} ms batches {st.batches}")
     sync()
-    sweep_ms = up_ms = out_ms = 0.0
-    launches = batches = 0
+    sweep_ms = up_ms = out_ms = dom_ms = 0.0
+    launches = batches = dom_launches = dom_bytes = 0
     t0 = time.perf_counter()
     for i in range(args.steps):
         st = step()
@@ -156,6 +205,9 @@ def main():
         out_ms += st.output_time_ms
         launches += st.sweep_launches
         batches += st.batches
+        dom_ms += st.dom_ms
+        dom_launches += st.dom_launches
+        dom_bytes += st.dom_bytes
         log(f"step {i}: up {st.upward_time_ms:.1f} ms sweep {st.downward_time_ms:.1f} ms "
             f"out {st.output_time_ms:.1f} ms")
     sync()
@@ -174,7 +226,11 @@ def main():
     n_s, m_s = st.nodes_swept, st.arcs_swept
     bytes_per_group = 4 * (n_s + 1) + 8 * m_s + 8 * KL * n_s
     sweep_bytes = groups_per_step * bytes_per_group * args.steps
-    achieved = sweep_bytes / (sweep_ms / 1000.0) / 1e9 if sweep_ms > 0 else 0.0
+    sweep_gbps = sweep_bytes / (sweep_ms / 1000.0) / 1e9 if sweep_ms > 0 else 0.0
+    # the dominant kernel (DOWN rows of the top region): algorithmic bytes of
+    # its launches over their summed duration, both from the library
+    achieved = dom_bytes / (dom_ms / 1000.0) / 1e9 if dom_ms > 0 else 0.0
+    traffic = committed_traffic(args.workload, KL)
 
     n_check = min(S, 64)
     gpu_rows = d_out.to_host(0, n_check * T).reshape(n_check, T)
@@ -197,11 +253,16 @@ def main():
                    "lane_capacity": cap, "n_nodes": n, "m_up": info["m_up"], "m_down": m_dn,
                    "etree_depth": info["etree_depth"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "sweep_kernel", "bytes_per_lane_group": bytes_per_group,
-                     "nodes_swept": n_s, "arcs_swept": m_s,
-                     "lane_groups_per_step": groups_per_step, "sweep_launches_per_step": launches // args.steps,
-                     "sweep_ms_per_step": sweep_ms / args.steps},
+                     "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "kernel": f"rows_kernel<{KL}, false> (DOWN rows of the top region)",
+                     "launches": dom_launches, "avg_launch_ms": dom_ms / max(dom_launches, 1),
+                     "bytes_per_launch": dom_bytes / max(dom_launches, 1),
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "sweep": {"bytes_per_lane_group": bytes_per_group, "nodes_swept": n_s, "arcs_swept": m_s,
+                               "lane_groups_per_step": groups_per_step,
+                               "launches_per_step": launches // args.steps,
+                               "ms_per_step": sweep_ms / args.steps, "achieved_gbps": sweep_gbps}},
         "phases_ms_per_step": {"up": up_ms / args.steps, "sweep": sweep_ms / args.steps,
                                "gather": out_ms / args.steps},
         "cpu_baseline": cpu,

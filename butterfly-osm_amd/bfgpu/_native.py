@@ -31,7 +31,8 @@ class CtxInfo(C.Structure):
 class Timings(C.Structure):
     _fields_ = [("up_ms", C.c_float), ("sweep_ms", C.c_float), ("output_ms", C.c_float),
                 ("total_ms", C.c_float), ("batches", C.c_uint32), ("sweep_launches", C.c_uint32),
-                ("lanes_run", C.c_uint64), ("nodes_swept", C.c_uint64), ("arcs_swept", C.c_uint64)]
+                ("lanes_run", C.c_uint64), ("nodes_swept", C.c_uint64), ("arcs_swept", C.c_uint64),
+                ("dom_ms", C.c_float), ("dom_launches", C.c_uint32), ("dom_bytes", C.c_uint64)]
 
 
 class BfcParams(C.Structure):

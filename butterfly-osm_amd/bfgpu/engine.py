@@ -40,6 +40,9 @@ class BatchedPhastStats:
     sweep_launches: int = 0
     nodes_swept: int = 0  # nodes / DOWN arcs one lane group's sweep covered
     arcs_swept: int = 0
+    dom_ms: float = 0.0  # the DOWN row kernel: summed launch time, launches, algorithmic bytes
+    dom_launches: int = 0
+    dom_bytes: int = 0
 
 
 @dataclasses.dataclass
@@ -95,7 +98,8 @@ class BatchedPhastEngine:
         return BatchedPhastStats(n_sources=n_sources, upward_time_ms=t.up_ms, downward_time_ms=t.sweep_ms,
                                  output_time_ms=t.output_ms, total_time_ms=t.total_ms, batches=t.batches,
                                  sweep_launches=t.sweep_launches, nodes_swept=t.nodes_swept,
-                                 arcs_swept=t.arcs_swept)
+                                 arcs_swept=t.arcs_swept, dom_ms=t.dom_ms, dom_launches=t.dom_launches,
+                                 dom_bytes=t.dom_bytes)
 
     def configure(self, lanes=0, lane_capacity=0):
         _check(self._lib.bfg_ctx_configure(self._h, lanes, lane_capacity))

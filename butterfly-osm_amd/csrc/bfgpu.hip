@@ -740,6 +740,8 @@ struct bfg_ctx {
     uint32_t n_paths = 0;
     uint64_t path_nodes = 0, path_arcs = 0, top_in_arcs = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> dom_ev;     // event pairs around each DOWN row launch of a batch
+    std::vector<uint64_t> lvl_dom_bytes;  // algorithmic bytes of a level's DOWN rows per wave group
     bfg_timings last{};
     uint64_t graph_bytes = 0;
 };
@@ -793,8 +795,14 @@ uint32_t auto_lane_cap(bfg_ctx *c, uint32_t want) {
     return (uint32_t)cap;
 }
 
+struct PhaseTimes {
+    float up_ms = 0, sweep_ms = 0, dom_ms = 0;
+    uint32_t launches = 0, dom_launches = 0;
+    uint64_t dom_bytes = 0;
+};
+
 template <int KL>
-int launch_phases(bfg_ctx *c, uint32_t nl, float *up_ms, float *sweep_ms, uint32_t *launches) {
+int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
     const bfg::Layout &L = c->L;
     hipStream_t s = c->stream;
     const uint32_t groups = (nl + KL - 1) / KL;
@@ -817,6 +825,7 @@ int launch_phases(bfg_ctx *c, uint32_t nl, float *up_ms, float *sweep_ms, uint32
                        c->pos_parent.p, c->pos_depth.p, c->up_off.p, c->up_tgt.p, c->up_w.p, L.n_top, c->pathbuf.p,
                        D1, c->arena.p, n);
     constexpr uint32_t WPB = TILE_THREADS / 64;  // tiles per workgroup
+    uint32_t ndom = 0;                           // DOWN row launches of this batch
     for (int l = (int)nsl - 1; l >= 0; --l) {
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
         hipLaunchKernelGGL((tri_kernel<KL, false>), dim3(wgs, (s1 - s0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s,
@@ -837,10 +846,21 @@ int launch_phases(bfg_ctx *c, uint32_t nl, float *up_ms, float *sweep_ms, uint32
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
         const uint32_t u0 = L.du_lvl[l], u1 = L.du_lvl[l + 1];
         if (u1 > u0) {
+            if (c->dom_ev.size() < 2 * (size_t)(ndom + 1)) {
+                for (int k = 0; k < 2; ++k) {
+                    hipEvent_t e;
+                    HIP_TRY(hipEventCreate(&e));
+                    c->dom_ev.push_back(e);
+                }
+            }
+            HIP_TRY(hipEventRecord(c->dom_ev[2 * ndom], s));
             hipLaunchKernelGGL((rows_kernel<KL, false>), dim3(wgs, (u1 - u0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0,
                                s, u0, u1, c->du_seg.p, c->du_r0.p, c->du_r1.p, c->xr_off.p, c->xr_wid.p,
                                c->xw_off.p, c->seg_pos.p, c->pt_tgt.p, c->xr_src.p, c->xr_w.p, c->active.p, nseg,
                                c->arena.p, n);
+            HIP_TRY(hipEventRecord(c->dom_ev[2 * ndom + 1], s));
+            ++ndom;
+            pt->dom_bytes += (uint64_t)wgs * c->lvl_dom_bytes[l];
             ++nlaunch;
         }
         hipLaunchKernelGGL((tri_kernel<KL, true>), dim3(wgs, (s1 - s0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s,
@@ -879,9 +899,15 @@ int launch_phases(bfg_ctx *c, uint32_t nl, float *up_ms, float *sweep_ms, uint32
     float a = 0, b = 0;
     HIP_TRY(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
     HIP_TRY(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
-    *up_ms += a;
-    *sweep_ms += b;
-    *launches += nlaunch;
+    pt->up_ms += a;
+    pt->sweep_ms += b;
+    pt->launches += nlaunch;
+    for (uint32_t k = 0; k < ndom; ++k) {
+        float d = 0;
+        HIP_TRY(hipEventElapsedTime(&d, c->dom_ev[2 * k], c->dom_ev[2 * k + 1]));
+        pt->dom_ms += d;
+    }
+    pt->dom_launches += ndom;
     return BFG_SUCCESS;
 }
 
@@ -889,7 +915,7 @@ int launch_phases(bfg_ctx *c, uint32_t nl, float *up_ms, float *sweep_ms, uint32
 // upward and downward phases; the output step is done by the caller.
 int run_batch(bfg_ctx *c, uint32_t nl, const std::vector<uint32_t> &soff, const std::vector<uint32_t> &srank,
               const std::vector<uint32_t> &scost, const std::vector<uint32_t> &limits,
-              const std::vector<uint32_t> &rows, float *up_ms, float *sweep_ms, uint32_t *launches) {
+              const std::vector<uint32_t> &rows, PhaseTimes *pt) {
     const size_t ns = srank.size();
     int rc = ensure_lane_buffers(c, nl, ns);
     if (rc) return rc;
@@ -911,9 +937,9 @@ int run_batch(bfg_ctx *c, uint32_t nl, const std::vector<uint32_t> &soff, const 
         HIP_TRY(hipMemcpyAsync(c->seed_pre.p, spre.data(), ns * 4, hipMemcpyHostToDevice, s));
     }
     switch (c->lanes) {
-        case 16: rc = launch_phases<16>(c, nl, up_ms, sweep_ms, launches); break;
-        case 64: rc = launch_phases<64>(c, nl, up_ms, sweep_ms, launches); break;
-        default: rc = launch_phases<32>(c, nl, up_ms, sweep_ms, launches); break;
+        case 16: rc = launch_phases<16>(c, nl, pt); break;
+        case 64: rc = launch_phases<64>(c, nl, pt); break;
+        default: rc = launch_phases<32>(c, nl, pt); break;
     }
     return rc;  // launch_phases synchronised the stream: the staged vectors may go
 }
@@ -976,6 +1002,7 @@ int run_all(bfg_ctx *c, const Req &req, F &&out) {
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
     auto t_start = std::chrono::steady_clock::now();
     bfg_timings tm{};
+    PhaseTimes pt;
     for (uint32_t b0 = 0; b0 < L; b0 += c->lane_cap) {
         const uint32_t nl = std::min(c->lane_cap, L - b0);
         std::vector<uint32_t> soff(nl + 1), srank, scost, lim(nl), rows(nl);
@@ -990,12 +1017,18 @@ int run_all(bfg_ctx *c, const Req &req, F &&out) {
             lim[i] = req.limit[o];
         }
         soff[nl] = (uint32_t)srank.size();
-        rc = run_batch(c, nl, soff, srank, scost, lim, rows, &tm.up_ms, &tm.sweep_ms, &tm.sweep_launches);
+        rc = run_batch(c, nl, soff, srank, scost, lim, rows, &pt);
         if (rc) return rc;
         rc = out(b0, nl, rows, &tm.output_ms);
         if (rc) return rc;
         tm.batches++;
     }
+    tm.up_ms = pt.up_ms;
+    tm.sweep_ms = pt.sweep_ms;
+    tm.sweep_launches = pt.launches;
+    tm.dom_ms = pt.dom_ms;
+    tm.dom_launches = pt.dom_launches;
+    tm.dom_bytes = pt.dom_bytes;
     tm.lanes_run = L;
     tm.nodes_swept = c->paths_on ? c->L.n_top + c->path_nodes : c->L.n;
     tm.arcs_swept = c->paths_on ? c->top_in_arcs + c->path_arcs : c->L.m_down;
@@ -1158,6 +1191,12 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     HIP_TRY(c->seg_pos.upload(L.seg_pos, s));
     HIP_TRY(c->in_off.upload(L.in_off, s));
     c->top_in_arcs = L.in_off[L.n_top];
+    for (size_t l = 0; l + 1 < L.seglvl_seg.size(); ++l) {
+        const uint32_t p0 = L.seg_pos[L.seglvl_seg[l]], p1 = L.seg_pos[L.seglvl_seg[l + 1]];
+        uint64_t m = 0;
+        for (uint32_t p = p0; p < p1; ++p) m += L.in_split[p] - L.in_off[p];
+        c->lvl_dom_bytes.push_back(8 * m + 8ull * 64 * (p1 - p0));
+    }
     HIP_TRY(c->in_split.upload(L.in_split, s));
     HIP_TRY(c->in_src.upload(L.in_src, s));
     HIP_TRY(c->in_w.upload(L.in_w, s));
@@ -1216,6 +1255,7 @@ void bfg_ctx_destroy(bfg_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->dom_ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

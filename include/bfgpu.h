@@ -79,6 +79,14 @@ typedef struct bfg_timings {
      * whole CCH, or for a matrix the top region plus the bottom paths above
      * its targets (a target's label depends on its ancestors alone) */
     uint64_t nodes_swept, arcs_swept;
+    /* the dominant kernel (DOWN rows of the top region): summed durations of
+     * its launches (HIP events on the context stream), launch count, and the
+     * algorithmic bytes those launches move: per 64-lane wavefront group and
+     * segment level, 8 per DOWN arc from outside the level's segments plus
+     * 8*64 per node of the level (its labels written once, read once) */
+    float dom_ms;
+    uint32_t dom_launches;
+    uint64_t dom_bytes;
 } bfg_timings;
 
 /* BatchedPhastEngine::new (batched_phast.rs:89): copies the CCH to `device`,
