@@ -9,23 +9,27 @@
 // Upward phase — the reference's per-lane heap Dijkstra over UP arcs
 // (batched_phast.rs:156-189, isochrone_handler.rs:388-411) as an ancestor
 // walk, since the up search space lies on the source's root path:
-//   fill_top     top-region labels of the batch := INF
+//   fill_reach   labels := INF for the (segment, 64-lane wave group) pairs the
+//                upward phase reaches (a pre-order test on the lanes' seeds;
+//                no other pair is ever read as an upward label)
 //   bottom_walk  one wavefront per lane walks the bottom part of its root path
 //                (depth-indexed slots), relaxing UP arcs into bottom ancestors
 //                (slots) and top ancestors (atomic min into the arena)
-//   up_tile      per (group, segment), deepest segment level first, as a
-//                register tile (8 nodes x KL/8 lanes per thread): what lower
-//                levels pushed, intra-segment UP arcs as 8x8 blocks from the
-//                last octet to the first, then the UP arcs leaving the
-//                segment as ELL rows reduced per target block and pushed with
-//                one atomic min per (target, lane)
+//   tri<UP>      per (segment, wave group), deepest segment level first: the
+//                segment's labels in registers (one lane per thread), its
+//                intra-segment UP arcs as 8x8 blocks, last octet to first
+//   rows<PUSH>   per (target block, wave group): the UP arcs leaving a
+//                segment as rows (tail, W weights) reduced in registers and
+//                pushed with one atomic min per (target, lane)
 // Downward phase — the rank-descending DOWN scan (batched_phast.rs:205-250,
 // :538-590) scheduled by the elimination tree:
-//   down_tile    per (group, segment) of a segment level, as the same register
-//                tile seeded with the upward labels: DOWN arcs from outside the
-//                segment as ELL rows (min-plus product of the sources' labels
-//                and the tile's weights), then the intra-segment blocks from the
-//                first octet to the last
+//   rows<DOWN>   per (unit of <= 128 rows of a segment, wave group): DOWN arcs
+//                from outside the segment as rows (source, W weights), a
+//                min-plus product of the sources' labels and the segment's
+//                weight block; a unit holding all rows of its segment merges
+//                with the upward labels and stores, split units use atomics
+//   tri<DOWN>    per (segment, wave group): intra-segment DOWN blocks
+//   bottom_path  matrices: the bottom paths above the targets, pulled top-down
 //   bottom_chunk per (bottom chunk, group): seeds from the lanes' bottom path
 //                slots, then DOWN arcs from top-region sources, grouped by
 //                source, into the chunk's LDS labels
@@ -83,17 +87,6 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
     return v;
-}
-
-// ---------------------------------------------------------------------------
-__global__ void fill_top_kernel(uint32_t *__restrict__ arena, uint64_t group_stride, uint64_t top_words,
-                                uint32_t groups) {
-    const uint64_t total = top_words * groups;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t g = i / top_words, r = i - g * top_words;
-        arena[g * group_stride + r] = INF;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -255,6 +248,41 @@ __device__ __forceinline__ uint32_t *lane_labels(uint32_t *arena, uint32_t n, ui
     return arena + (size_t)g * n * KL + l % KL;
 }
 
+// Which wave groups reach which segments.  A lane's upward labels are finite
+// exactly on the root paths of its seeds, so wave group wg (lanes wg*64 ..,
+// sorted order) reaches the segment headed by position p iff a seed of one of
+// its lanes lies in p's subtree: a pre-order interval test, no labels read.
+struct Reach {
+    const uint32_t *seed_off, *seed_pre, *pos_pre, *pos_sub;
+    uint32_t n_lanes;
+};
+
+__device__ __forceinline__ bool wave_reaches(const Reach &R, uint32_t wg, uint32_t p) {
+    const uint32_t gl = wg * 64 + threadIdx.x % 64;
+    const uint32_t pr = R.pos_pre[p], ps = R.pos_sub[p];
+    bool r = false;
+    if (gl < R.n_lanes)
+        for (uint32_t j = R.seed_off[gl]; j < R.seed_off[gl + 1]; ++j) r |= R.seed_pre[j] - pr < ps;
+    return __ballot(r) != 0;
+}
+
+// fill_reach: wavefront w of workgroup (wave group blockIdx.x, blockIdx.y)
+// takes segment 4*blockIdx.y + w and, if its wave group reaches it, sets the
+// segment's labels of the wave group to INF before the upward phase pushes
+// into them.  Unreached segments are never read as such (see tri_kernel and
+// rows_kernel).
+template <int KL>
+__global__ __launch_bounds__(TILE_THREADS) void fill_reach_kernel(uint32_t nseg, const uint32_t *__restrict__ seg_pos,
+                                                                  Reach R, uint32_t *__restrict__ arena,
+                                                                  uint32_t n) {
+    const uint32_t s = blockIdx.y * (TILE_THREADS / 64) + wave_id();
+    if (s >= nseg) return;  // wave-uniform; this kernel has no barrier
+    const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
+    if (!wave_reaches(R, blockIdx.x, p0)) return;
+    uint32_t *lab = lane_labels<KL>(arena, n, blockIdx.x, nullptr);
+    for (uint32_t j = 0; j < len; ++j) lab[(size_t)(p0 + j) * KL] = INF;
+}
+
 // acc[j] = min(acc[j], lab[key[r]] + wr[W(r - r0) + j]) over rows [r0, r1)
 // of width W; two rows per step (min(acc, x0, x1) is one v_min3), the labels
 // of the next two rows loaded one step ahead.  An odd last row pairs with
@@ -278,15 +306,55 @@ __device__ __forceinline__ void rows_w(uint32_t (&acc)[W], uint32_t r0, uint32_t
     }
 }
 
+// Intra-segment arcs of a segment of at most 8*NO nodes in registers
+// (acc[j] = label of node j for this lane): 8x8 blocks T[8a + o] of tw, DOWN
+// from the first octet to the last (the diagonal block of an octet first, i
+// ascending), UP from the last to the first (i descending).
+template <bool DOWN, int NO>
+__device__ __forceinline__ void tri_apply(uint32_t (&acc)[8 * NO], const cu32 *T, const cu32 *tw) {
+#pragma unroll
+    for (int aa = 0; aa < NO; ++aa) {
+        const int a = DOWN ? aa : NO - 1 - aa;
+#pragma unroll
+        for (int oo = 0; oo < NO; ++oo) {
+            const int o = DOWN ? a + oo : a - oo;
+            if (o < 0 || o >= NO) continue;
+            const uint32_t k = T[a * 8 + o];
+            if (k == NONE) continue;
+            const cu32 *W = tw + (size_t)k * 64;
+#pragma unroll
+            for (int ii = 0; ii < 8; ++ii) {
+                const int i = DOWN ? ii : 7 - ii;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (o == a && (DOWN ? j <= i : j >= i)) continue;
+                    acc[o * 8 + j] = min(acc[o * 8 + j], sat_add(acc[a * 8 + i], W[i * 8 + j]));
+                }
+            }
+        }
+    }
+}
+
 // One unit of rows_kernel at width W: rows [ra, rb) with weights from wr,
-// merged into positions out[j] (PUSH) or p0 + j (j < len) of the arena.
+// merged into positions out[j] (PUSH) or p0 + j (j < len) of the arena with
+// one return-less atomic min per finite (position, lane).  A DOWN unit that
+// holds all rows of its segment (whole) and whose wave group does not reach
+// the segment stores instead: those upward labels are INF and never written.
 template <int KL, int W, bool PUSH>
 __device__ __forceinline__ void rows_unit(uint32_t ra, uint32_t rb, const cu32 *K, const cu32 *wr, const cu32 *out,
-                                          uint32_t p0, uint32_t len, uint32_t *lab) {
+                                          uint32_t p0, uint32_t len, uint32_t *lab, bool whole, bool reach) {
     uint32_t acc[W];
 #pragma unroll
     for (int j = 0; j < W; ++j) acc[j] = INF;
     rows_w<KL, W>(acc, ra, rb, K, wr, lab);
+    if constexpr (!PUSH) {
+        if (whole && !reach) {  // the upward labels are INF and were never written: store
+#pragma unroll
+            for (int j = 0; j < W; ++j)
+                if (j < len) lab[(size_t)(p0 + j) * KL] = acc[j];
+            return;
+        }
+    }
 #pragma unroll
     for (int j = 0; j < W; ++j) {
         if constexpr (PUSH) {
@@ -298,31 +366,33 @@ __device__ __forceinline__ void rows_unit(uint32_t ra, uint32_t rb, const cu32 *
 }
 
 // rows: wavefront w of workgroup (wave group blockIdx.x, blockIdx.y) takes
-// unit u0 + 4*blockIdx.y + w and merges its rows into the arena with one
-// atomic min per finite (position, lane).
-//   DOWN (PUSH = false): unit = rows [r0[u], r1[u]) of segment seg[u] (DOWN
-//     arcs from outside the segment), width wid[s], weights from
-//     woff[s] + (r - rbase[s]) * W; positions are the segment's nodes, which
-//     hold the upward labels.
+// unit u0 + 4*blockIdx.y + w.
+//   DOWN (PUSH = false): unit = rows [r0[u], r1[u]) of segment
+//     seg[u] & ~UNIT_WHOLE (DOWN arcs from outside the segment), width
+//     wid[s], weights from woff[s] + (r - rbase[s]) * W, merged into the
+//     segment's labels (which hold the upward labels): by plain stores if the
+//     unit is flagged UNIT_WHOLE (it holds all rows of its segment), else by
+//     one atomic min per finite (node, lane).  The intra arcs follow in
+//     tri_kernel.
 //   UP (PUSH = true): unit = target block u, rows [r0[u], r0[u+1]) keyed by
 //     tails in segment seg[u] (final up labels), width wid[u], weights from
-//     woff[u]; positions are tgt[64u ..]; a wavefront none of whose groups
-//     reaches the segment stops at once.
+//     woff[u], merged into targets tgt[64u ..] with atomic min; a wavefront
+//     whose wave group does not reach the segment stops at once.
+constexpr uint32_t UNIT_WHOLE = 0x80000000u;
+
 template <int KL, bool PUSH>
 __global__ __launch_bounds__(TILE_THREADS) void rows_kernel(
     uint32_t u0, uint32_t u1, const uint32_t *__restrict__ seg, const uint32_t *__restrict__ r0,
     const uint32_t *__restrict__ r1, const uint32_t *__restrict__ rbase, const uint32_t *__restrict__ wid,
     const uint32_t *__restrict__ woff, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ tgt,
-    const uint32_t *__restrict__ key, const uint32_t *__restrict__ wts, const uint8_t *__restrict__ active,
-    uint32_t nseg, uint32_t *__restrict__ arena, uint32_t n) {
+    const uint32_t *__restrict__ key, const uint32_t *__restrict__ wts, Reach R, uint32_t *__restrict__ arena,
+    uint32_t n) {
     const uint32_t u = u0 + blockIdx.y * (TILE_THREADS / 64) + wave_id();
     if (u >= u1) return;  // wave-uniform; this kernel has no barrier
-    uint32_t g;
-    uint32_t *lab = lane_labels<KL>(arena, n, blockIdx.x, &g);
-    const uint32_t s = seg[u];
-    if constexpr (PUSH) {
-        if (__ballot(active[(size_t)g * nseg + s] != 0) == 0) return;
-    }
+    uint32_t *lab = lane_labels<KL>(arena, n, blockIdx.x, nullptr);
+    const uint32_t sw = seg[u], s = sw & ~UNIT_WHOLE;
+    const bool reach = wave_reaches(R, blockIdx.x, seg_pos[s]);
+    if (PUSH && !reach) return;  // the segment's labels are all INF: nothing to push
     const cu32 *K = as_const(key);
     uint32_t ra, rb, W, p0 = 0, len = 0;
     const cu32 *wr, *out = nullptr;
@@ -340,65 +410,40 @@ __global__ __launch_bounds__(TILE_THREADS) void rows_kernel(
         p0 = seg_pos[s];
         len = seg_pos[s + 1] - p0;
     }
-    if (W == 16) rows_unit<KL, 16, PUSH>(ra, rb, K, wr, out, p0, len, lab);
-    else if (W == 32) rows_unit<KL, 32, PUSH>(ra, rb, K, wr, out, p0, len, lab);
-    else rows_unit<KL, 64, PUSH>(ra, rb, K, wr, out, p0, len, lab);
+    const bool whole = (sw & UNIT_WHOLE) != 0;
+    if (W == 16) rows_unit<KL, 16, PUSH>(ra, rb, K, wr, out, p0, len, lab, whole, reach);
+    else if (W == 32) rows_unit<KL, 32, PUSH>(ra, rb, K, wr, out, p0, len, lab, whole, reach);
+    else rows_unit<KL, 64, PUSH>(ra, rb, K, wr, out, p0, len, lab, whole, reach);
 }
 
 // tri: wavefront w of workgroup (wave group blockIdx.x, blockIdx.y) takes
-// segment s0 + 4*blockIdx.y + w of one level and relaxes its intra-segment
-// arcs in registers: 8x8 blocks, DOWN from the first octet to the last (the
-// diagonal block of an octet first, i ascending), UP from the last to the
-// first (i descending).  UP also records active[g * nseg + s]: whether any
-// lane of group g reaches the segment (a wavefront that reaches none stops
-// and leaves the segment's labels INF).
+// segment first + 4*blockIdx.y + w < first + count of one level, loads its
+// labels, relaxes its intra-segment arcs (tri_apply) and stores them.  UP: a
+// wave group that does not reach the segment skips it, only writing INF
+// where the DOWN plan merges into the labels with atomics (split[s]).
 template <int KL, bool DOWN>
-__global__ __launch_bounds__(TILE_THREADS) void tri_kernel(uint32_t s0, uint32_t s1, uint32_t nseg,
+__global__ __launch_bounds__(TILE_THREADS) void tri_kernel(uint32_t first, uint32_t count,
                                                            const uint32_t *__restrict__ seg_pos,
                                                            const uint32_t *__restrict__ tab,
                                                            const uint32_t *__restrict__ tw,
-                                                           uint8_t *__restrict__ active,
+                                                           const uint8_t *__restrict__ split, Reach R,
                                                            uint32_t *__restrict__ arena, uint32_t n) {
-    const uint32_t s = s0 + blockIdx.y * (TILE_THREADS / 64) + wave_id();
-    if (s >= s1) return;  // wave-uniform; this kernel has no barrier
-    uint32_t g;
-    uint32_t *lab = lane_labels<KL>(arena, n, blockIdx.x, &g);
+    const uint32_t i = blockIdx.y * (TILE_THREADS / 64) + wave_id();
+    if (i >= count) return;  // wave-uniform; this kernel has no barrier
+    const uint32_t s = first + i;
+    uint32_t *lab = lane_labels<KL>(arena, n, blockIdx.x, nullptr);
     const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
+    if constexpr (!DOWN) {
+        if (!wave_reaches(R, blockIdx.x, p0)) {
+            if (split[s])
+                for (uint32_t j = 0; j < len; ++j) lab[(size_t)(p0 + j) * KL] = INF;
+            return;
+        }
+    }
     uint32_t acc[64];
 #pragma unroll
     for (int j = 0; j < 64; ++j) acc[j] = j < len ? lab[(size_t)(p0 + j) * KL] : INF;
-    if constexpr (!DOWN) {
-        uint32_t m = INF;
-#pragma unroll
-        for (int j = 0; j < 64; ++j) m = min(m, acc[j]);
-        const uint64_t b = __ballot(m != INF);
-        const uint32_t l = threadIdx.x % 64;
-        const uint64_t gm = KL >= 64 ? ~0ull : ((1ull << KL) - 1) << (l / KL * KL);
-        if (l % KL == 0) active[(size_t)g * nseg + s] = (b & gm) ? 1 : 0;
-        if (b == 0) return;
-    }
-    const cu32 *T = as_const(tab) + (size_t)s * 64;
-#pragma unroll
-    for (int aa = 0; aa < 8; ++aa) {
-        const int a = DOWN ? aa : 7 - aa;
-#pragma unroll
-        for (int oo = 0; oo < 8; ++oo) {
-            const int o = DOWN ? a + oo : a - oo;
-            if (o < 0 || o > 7) continue;
-            const uint32_t k = T[a * 8 + o];
-            if (k == NONE) continue;
-            const cu32 *W = as_const(tw) + (size_t)k * 64;
-#pragma unroll
-            for (int ii = 0; ii < 8; ++ii) {
-                const int i = DOWN ? ii : 7 - ii;
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    if (o == a && (DOWN ? j <= i : j >= i)) continue;
-                    acc[o * 8 + j] = min(acc[o * 8 + j], sat_add(acc[a * 8 + i], W[i * 8 + j]));
-                }
-            }
-        }
-    }
+    tri_apply<DOWN, 8>(acc, as_const(tab) + (size_t)s * 64, as_const(tw));
 #pragma unroll
     for (int j = 0; j < 64; ++j)
         if (j < len) lab[(size_t)(p0 + j) * KL] = acc[j];
@@ -511,15 +556,17 @@ __global__ __launch_bounds__(NTH) void bottom_chunk_kernel(
 // target's label depends only on its ancestors, so the bottom sweep of a
 // matrix needs the bottom paths from the top region down to its targets
 // alone.  Slot (KL threads, one per lane) `blockIdx.x * NS + slot` of group
-// blockIdx.y walks path_pos[path_off[k] .. path_off[k+1]) top-down: each node
-// starts from the lane's upward label (its bottom path slots) and pulls over
-// all its DOWN in-arcs.  Top sources are final; bottom sources lie above it
-// on the same path and were written by this same thread.  Paths that share
-// nodes write identical values.
+// blockIdx.y walks path k, path_pos[path_off[k] .. path_off[k+1]) top-down:
+// each node starts from the lane's upward label (its bottom path slots) and
+// pulls over all its DOWN in-arcs.  Labels are compact: top positions as
+// they are, path element i at n_top + i.  Top sources are final; a bottom
+// source lies above the node on the same path (at the element its depth
+// gives) and was written by this same thread.
 template <int KL>
 __global__ __launch_bounds__(NT) void bottom_path_kernel(
-    uint32_t n_paths, const uint32_t *__restrict__ path_off, const uint32_t *__restrict__ path_pos,
-    const uint32_t *__restrict__ in_off, const uint32_t *__restrict__ in_src, const uint32_t *__restrict__ in_w,
+    uint32_t n_paths, const uint32_t *__restrict__ path_off, const uint32_t *__restrict__ path_pos, uint32_t n_top,
+    const uint32_t *__restrict__ in_off, const uint32_t *__restrict__ in_split, const uint16_t *__restrict__ in_dd,
+    const uint32_t *__restrict__ in_src, const uint32_t *__restrict__ in_w,
     const uint32_t *__restrict__ pos_pre, const uint32_t *__restrict__ pos_sub, const uint32_t *__restrict__ pos_depth,
     const uint32_t *__restrict__ seed_off, const uint32_t *__restrict__ seed_pre,
     const uint32_t *__restrict__ seed_depth, const uint32_t *__restrict__ pathbuf, uint32_t D1, uint32_t n_lanes,
@@ -536,8 +583,9 @@ __global__ __launch_bounds__(NT) void bottom_path_kernel(
         sj0 = seed_off[gl];
         sj1 = seed_off[gl + 1];
     }
+    const uint32_t ib = in_off[n_top];  // first arc of a bottom head: in_dd's origin
     for (uint32_t i = path_off[k]; i < path_off[k + 1]; ++i) {
-        const uint32_t p = path_pos[i];
+        const uint32_t p = path_pos[i], li = n_top + i;  // this node's label index
         uint32_t m0 = INF, m1 = INF, m2 = INF, m3 = INF;
         if (sj0 < sj1) {
             const uint32_t pr = pos_pre[p], ps = pos_sub[p], pd = pos_depth[p];
@@ -546,9 +594,17 @@ __global__ __launch_bounds__(NT) void bottom_path_kernel(
                 if (sp >= pr && sp - pr < ps) m0 = min(m0, pathbuf[(size_t)j * D1 + (seed_depth[j] - pd)]);
             }
         }
+        // bottom sources: the ancestors in_dd levels up, path elements li - in_dd
         uint32_t a = in_off[p];
-        const uint32_t a1 = in_off[p + 1];
-        for (; a + 4 <= a1; a += 4) {
+        const uint32_t as = in_split[p], a1 = in_off[p + 1];
+        for (; a + 2 <= as; a += 2) {
+            const uint32_t d0 = in_dd[a - ib], d1 = in_dd[a + 1 - ib];
+            m0 = min(m0, sat_add(lab[(size_t)(li - d0) * KL], in_w[a]));
+            m1 = min(m1, sat_add(lab[(size_t)(li - d1) * KL], in_w[a + 1]));
+        }
+        if (a < as) m0 = min(m0, sat_add(lab[(size_t)(li - in_dd[a - ib]) * KL], in_w[a]));
+        // top sources: final labels at their positions
+        for (a = as; a + 4 <= a1; a += 4) {
             const uint32_t u0 = in_src[a], u1 = in_src[a + 1], u2 = in_src[a + 2], u3 = in_src[a + 3];
             const uint32_t w0 = in_w[a], w1 = in_w[a + 1], w2 = in_w[a + 2], w3 = in_w[a + 3];
             m0 = min(m0, sat_add(lab[(size_t)u0 * KL], w0));
@@ -557,7 +613,7 @@ __global__ __launch_bounds__(NT) void bottom_path_kernel(
             m3 = min(m3, sat_add(lab[(size_t)u3 * KL], w3));
         }
         for (; a < a1; ++a) m0 = min(m0, sat_add(lab[(size_t)in_src[a] * KL], in_w[a]));
-        lab[(size_t)p * KL] = min(min(m0, m1), min(m2, m3));
+        lab[(size_t)li * KL] = min(min(m0, m1), min(m2, m3));
     }
 }
 
@@ -721,14 +777,20 @@ struct bfg_ctx {
     DevBuf<uint32_t> parent, pos_parent, pos_pre, pos_sub, pos_depth, seg_pos;
     DevBuf<uint32_t> in_off, in_split, in_src, in_w;
     DevBuf<uint32_t> ch_off, ch_pos, cs_off, cs_src, cs_aoff, ca_w;
-    DevBuf<uint16_t> ca_v;
+    DevBuf<uint16_t> ca_v, in_dd;
     DevBuf<uint32_t> up_off, up_tgt, up_w;
     DevBuf<uint32_t> xr_off, xr_wid, xw_off, xr_src, xr_w, tb_dn, tb_up, tb_w, pt_blk, pt_tgt, pr_off, pr_wid,
         pw_off, pr_x, pr_w;
-    DevBuf<uint32_t> du_seg, du_r0, du_r1, pk_seg;
+    DevBuf<uint32_t> pk_seg;
+    // DOWN work plan for the current number of wave groups per batch: units
+    // (segment | UNIT_WHOLE, rows) by level
+    uint32_t plan_wgs = 0;
+    std::vector<uint32_t> plan_lvl;
+    std::vector<uint64_t> lvl_dom_bytes;  // algorithmic bytes of a level's DOWN rows per wave group
+    DevBuf<uint32_t> du_seg, du_r0, du_r1;
+    DevBuf<uint8_t> seg_split;  // 1 where the plan merges a segment's rows with atomics
     // arenas (per batch)
     DevBuf<uint32_t> arena, pathbuf;
-    DevBuf<uint8_t> active;  // [group][segment] reach flags of the upward phase
     DevBuf<uint32_t> seed_off, seed_rank, seed_pos, seed_cost, seed_depth, seed_pre, lane_limit, row_map;
     size_t seeds_cap = 0, lanes_alloc = 0;
     DevBuf<uint32_t> tgt_pos;
@@ -736,12 +798,12 @@ struct bfg_ctx {
     // bottom paths above the targets of the current matrix call; while
     // paths_on, the downward sweep covers the top region and these paths only
     DevBuf<uint32_t> path_off, path_pos;
-    bool paths_on = false;
+    bool paths_on = false;  // while on, labels are compact (see label_stride)
     uint32_t n_paths = 0;
     uint64_t path_nodes = 0, path_arcs = 0, top_in_arcs = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> dom_ev;     // event pairs around each DOWN row launch of a batch
-    std::vector<uint64_t> lvl_dom_bytes;  // algorithmic bytes of a level's DOWN rows per wave group
+
     bfg_timings last{};
     uint64_t graph_bytes = 0;
 };
@@ -769,15 +831,19 @@ int ensure_lane_buffers(bfg_ctx *c, size_t n_lanes, size_t n_seeds) {
     return BFG_SUCCESS;
 }
 
+// Labels per lane in the arena: all n positions, or for a matrix the top
+// region [0, n_top) followed by the nodes of its target paths (path_pos
+// element i at n_top + i).
+uint32_t label_stride(const bfg_ctx *c) {
+    return c->paths_on ? c->L.n_top + (uint32_t)c->path_nodes : c->L.n;
+}
+
 int ensure_arena(bfg_ctx *c) {
-    const size_t need = (size_t)c->lane_cap * c->L.n;
+    const size_t need = (size_t)c->lane_cap * label_stride(c);
     if (c->arena.n < need) {
         c->arena.release();
         HIP_TRY(c->arena.alloc(need));
     }
-    // reach flags for the largest group count a batch of lane_cap lanes can form
-    const size_t flags = ((size_t)c->lane_cap + 15) / 16 * (c->L.seg_pos.size() - 1);
-    if (c->active.n < flags) HIP_TRY(c->active.alloc(flags));
     return BFG_SUCCESS;
 }
 
@@ -785,7 +851,7 @@ uint32_t auto_lane_cap(bfg_ctx *c, uint32_t want) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
     const size_t have = free_b + c->arena.n * sizeof(uint32_t);
-    const size_t per_lane = (size_t)c->L.n * 4 + (size_t)d1_of(c) * 4 + 64;
+    const size_t per_lane = (size_t)label_stride(c) * 4 + (size_t)d1_of(c) * 4 + 64;
     const size_t budget = (size_t)(have * 0.6);
     // a multiple of 64: the top-region kernels carry 64 lanes per wavefront
     size_t cap = std::max<size_t>(budget / std::max<size_t>(per_lane, 1), 64);
@@ -793,6 +859,67 @@ uint32_t auto_lane_cap(bfg_ctx *c, uint32_t want) {
     cap = std::min<size_t>(cap, ((size_t)want + 63) / 64 * 64);
     cap = std::max<size_t>(cap / 64 * 64, 64);
     return (uint32_t)cap;
+}
+
+// Wavefronts a level's DOWN rows aim to keep in flight, and the fewest and the
+// most rows one unit takes (the longest unit bounds its level's time).
+constexpr uint64_t PLAN_WAVES = 8192;
+constexpr uint32_t PLAN_MIN_ROWS = 64;
+constexpr uint32_t PLAN_MAX_ROWS = 128;
+
+uint32_t plan_max_rows() {
+    if (const char *e = std::getenv("BFG_PLAN_MAX_ROWS")) {  // tuning override
+        const long v = std::strtol(e, nullptr, 10);
+        if (v >= PLAN_MIN_ROWS && v < (1L << 24)) return (uint32_t)v;
+    }
+    return PLAN_MAX_ROWS;
+}
+
+// DOWN work plan for batches of `wgs` wave groups.  Per level, a row budget
+// R = rows of the level * wgs / PLAN_WAVES, clamped to [PLAN_MIN_ROWS,
+// PLAN_MAX_ROWS]: a segment with at most R rows is one whole unit; a longer
+// one is cut into units of R rows merged with atomics.
+int plan_down(bfg_ctx *c, uint32_t wgs) {
+    if (c->plan_wgs == wgs) return BFG_SUCCESS;
+    const bfg::Layout &L = c->L;
+    const uint32_t nsl = (uint32_t)L.seglvl_seg.size() - 1;
+    const uint64_t rmax = plan_max_rows();
+    std::vector<uint32_t> us, u0, u1;
+    std::vector<uint8_t> split(L.seg_pos.size() - 1, 0);
+    c->plan_lvl.assign(1, 0);
+    uint64_t covered = 0;
+    for (uint32_t l = 0; l < nsl; ++l) {
+        const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
+        const uint64_t rows = L.xr_off[s1] - L.xr_off[s0];
+        const uint64_t R = std::min<uint64_t>(
+            rmax, std::max<uint64_t>(PLAN_MIN_ROWS, (rows * wgs + PLAN_WAVES - 1) / PLAN_WAVES));
+        for (uint32_t sg = s0; sg < s1; ++sg) {
+            const uint32_t a = L.xr_off[sg], b = L.xr_off[sg + 1];
+            if (b - a <= R) {
+                us.push_back(sg | UNIT_WHOLE);
+                u0.push_back(a);
+                u1.push_back(b);
+            } else {
+                for (uint64_t r = a; r < b; r += R) {
+                    us.push_back(sg);
+                    u0.push_back((uint32_t)r);
+                    u1.push_back((uint32_t)std::min<uint64_t>(r + R, b));
+                }
+                split[sg] = 1;
+            }
+            covered += b - a;
+        }
+        c->plan_lvl.push_back((uint32_t)us.size());
+    }
+    if (covered != L.xr_off.back()) return fail(BFG_INTERNAL_ERROR, "DOWN plan does not cover every row");
+    hipStream_t s = c->stream;
+    HIP_TRY(c->du_seg.upload(us, s));
+    HIP_TRY(c->du_r0.upload(u0, s));
+    HIP_TRY(c->du_r1.upload(u1, s));
+    HIP_TRY(c->seg_split.upload(split, s));
+    HIP_TRY(hipStreamSynchronize(s));  // the host vectors die with this frame
+    c->plan_wgs = wgs;
+    return BFG_SUCCESS;
 }
 
 struct PhaseTimes {
@@ -807,44 +934,42 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
     hipStream_t s = c->stream;
     const uint32_t groups = (nl + KL - 1) / KL;
     const uint32_t wgs = (nl + 63) / 64;            // wave groups of the top-region kernels
-    const uint32_t tile_groups = wgs * (64 / KL);   // groups those touch (lanes past nl stay INF)
-    const uint32_t n = L.n, D1 = d1_of(c);
+    const uint32_t n = label_stride(c), D1 = d1_of(c);  // n: labels per lane in the arena
     const uint32_t nsl = (uint32_t)L.seglvl_seg.size() - 1;
     const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
     uint32_t nlaunch = 0;
+    int rc = plan_down(c, wgs);
+    if (rc) return rc;
     HIP_TRY(hipEventRecord(c->ev[0], s));
-    if (L.n_top) {
-        const uint64_t words = (uint64_t)L.n_top * KL;
-        const uint64_t total = words * tile_groups;
-        const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
-        hipLaunchKernelGGL(fill_top_kernel, dim3(blocks), dim3(256), 0, s, c->arena.p, (uint64_t)n * KL, words,
-                           tile_groups);
+    const Reach R{c->seed_off.p, c->seed_pre.p, c->pos_pre.p, c->pos_sub.p, nl};
+    constexpr uint32_t WPB = TILE_THREADS / 64;  // tiles per workgroup
+    if (nseg) {
+        hipLaunchKernelGGL(fill_reach_kernel<KL>, dim3(wgs, (nseg + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, nseg,
+                           c->seg_pos.p, R, c->arena.p, n);
+        ++nlaunch;
     }
     hipLaunchKernelGGL(bottom_walk_kernel<KL>, dim3(nl), dim3(64), 0, s, c->seed_off.p, c->seed_rank.p,
                        c->seed_pos.p, c->seed_cost.p, c->seed_depth.p, c->lane_limit.p, c->parent.p,
                        c->pos_parent.p, c->pos_depth.p, c->up_off.p, c->up_tgt.p, c->up_w.p, L.n_top, c->pathbuf.p,
                        D1, c->arena.p, n);
-    constexpr uint32_t WPB = TILE_THREADS / 64;  // tiles per workgroup
-    uint32_t ndom = 0;                           // DOWN row launches of this batch
+    uint32_t ndom = 0;  // DOWN row launches of this batch
     for (int l = (int)nsl - 1; l >= 0; --l) {
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
         hipLaunchKernelGGL((tri_kernel<KL, false>), dim3(wgs, (s1 - s0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s,
-                           s0, s1, nseg, c->seg_pos.p, c->tb_up.p, c->tb_w.p, c->active.p, c->arena.p, n);
+                           s0, s1 - s0, c->seg_pos.p, c->tb_up.p, c->tb_w.p, c->seg_split.p, R, c->arena.p, n);
         ++nlaunch;
         const uint32_t k0 = L.pt_blk[s0], k1 = L.pt_blk[s1];
         if (k1 > k0) {
             hipLaunchKernelGGL((rows_kernel<KL, true>), dim3(wgs, (k1 - k0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0,
                                s, k0, k1, c->pk_seg.p, c->pr_off.p, c->pr_off.p, c->pr_off.p, c->pr_wid.p,
-                               c->pw_off.p, c->seg_pos.p, c->pt_tgt.p, c->pr_x.p, c->pr_w.p, c->active.p, nseg,
-                               c->arena.p, n);
+                               c->pw_off.p, c->seg_pos.p, c->pt_tgt.p, c->pr_x.p, c->pr_w.p, R, c->arena.p, n);
             ++nlaunch;
         }
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[1], s));
     for (uint32_t l = 0; l < nsl; ++l) {
-        const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
-        const uint32_t u0 = L.du_lvl[l], u1 = L.du_lvl[l + 1];
+        const uint32_t u0 = c->plan_lvl[l], u1 = c->plan_lvl[l + 1];
         if (u1 > u0) {
             if (c->dom_ev.size() < 2 * (size_t)(ndom + 1)) {
                 for (int k = 0; k < 2; ++k) {
@@ -856,24 +981,24 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
             HIP_TRY(hipEventRecord(c->dom_ev[2 * ndom], s));
             hipLaunchKernelGGL((rows_kernel<KL, false>), dim3(wgs, (u1 - u0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0,
                                s, u0, u1, c->du_seg.p, c->du_r0.p, c->du_r1.p, c->xr_off.p, c->xr_wid.p,
-                               c->xw_off.p, c->seg_pos.p, c->pt_tgt.p, c->xr_src.p, c->xr_w.p, c->active.p, nseg,
-                               c->arena.p, n);
+                               c->xw_off.p, c->seg_pos.p, c->pt_tgt.p, c->xr_src.p, c->xr_w.p, R, c->arena.p, n);
             HIP_TRY(hipEventRecord(c->dom_ev[2 * ndom + 1], s));
             ++ndom;
             pt->dom_bytes += (uint64_t)wgs * c->lvl_dom_bytes[l];
             ++nlaunch;
         }
-        hipLaunchKernelGGL((tri_kernel<KL, true>), dim3(wgs, (s1 - s0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s,
-                           s0, s1, nseg, c->seg_pos.p, c->tb_dn.p, c->tb_w.p, c->active.p, c->arena.p, n);
+        const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
+        hipLaunchKernelGGL((tri_kernel<KL, true>), dim3(wgs, (s1 - s0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, s0,
+                           s1 - s0, c->seg_pos.p, c->tb_dn.p, c->tb_w.p, c->seg_split.p, R, c->arena.p, n);
         ++nlaunch;
     }
     if (c->paths_on) {
         constexpr int NSP = NT / KL;
         if (c->n_paths) {
             hipLaunchKernelGGL(bottom_path_kernel<KL>, dim3((c->n_paths + NSP - 1) / NSP, groups), dim3(NT), 0, s,
-                               c->n_paths, c->path_off.p, c->path_pos.p, c->in_off.p, c->in_src.p, c->in_w.p,
-                               c->pos_pre.p, c->pos_sub.p, c->pos_depth.p, c->seed_off.p, c->seed_pre.p,
-                               c->seed_depth.p, c->pathbuf.p, D1, nl, c->arena.p, n);
+                               c->n_paths, c->path_off.p, c->path_pos.p, L.n_top, c->in_off.p, c->in_split.p,
+                               c->in_dd.p, c->in_src.p, c->in_w.p, c->pos_pre.p, c->pos_sub.p, c->pos_depth.p,
+                               c->seed_off.p, c->seed_pre.p, c->seed_depth.p, c->pathbuf.p, D1, nl, c->arena.p, n);
             ++nlaunch;
         }
     } else {
@@ -952,15 +1077,15 @@ int gather_out(bfg_ctx *c, uint32_t nl, const uint32_t *d_tpos, uint32_t T, cons
     HIP_TRY(hipEventRecord(c->ev[0], s));
     switch (c->lanes) {
         case 16:
-            hipLaunchKernelGGL(gather_kernel<16>, grid, dim3(NT), 0, s, c->arena.p, c->L.n, d_tpos, T,
+            hipLaunchKernelGGL(gather_kernel<16>, grid, dim3(NT), 0, s, c->arena.p, label_stride(c), d_tpos, T,
                                c->lane_limit.p, nl, d_rows, d_out, ld, d_bits, bits_ld);
             break;
         case 64:
-            hipLaunchKernelGGL(gather_kernel<64>, grid, dim3(NT), 0, s, c->arena.p, c->L.n, d_tpos, T,
+            hipLaunchKernelGGL(gather_kernel<64>, grid, dim3(NT), 0, s, c->arena.p, label_stride(c), d_tpos, T,
                                c->lane_limit.p, nl, d_rows, d_out, ld, d_bits, bits_ld);
             break;
         default:
-            hipLaunchKernelGGL(gather_kernel<32>, grid, dim3(NT), 0, s, c->arena.p, c->L.n, d_tpos, T,
+            hipLaunchKernelGGL(gather_kernel<32>, grid, dim3(NT), 0, s, c->arena.p, label_stride(c), d_tpos, T,
                                c->lane_limit.p, nl, d_rows, d_out, ld, d_bits, bits_ld);
             break;
     }
@@ -1047,21 +1172,11 @@ Req single_source_req(const uint32_t *src, uint32_t k, uint32_t threshold) {
     return r;
 }
 
-int stage_targets(bfg_ctx *c, const uint32_t *targets, uint32_t T) {
-    std::vector<uint32_t> tp(T);
-    for (uint32_t t = 0; t < T; ++t) tp[t] = c->L.pos_of_rank[targets[t]];
-    if (T > c->tgt_cap) {
-        HIP_TRY(c->tgt_pos.alloc(T));
-        c->tgt_cap = T;
-    }
-    HIP_TRY(hipMemcpyAsync(c->tgt_pos.p, tp.data(), (size_t)T * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));  // tp dies with this frame
-    return BFG_SUCCESS;
-}
-
 // Bottom paths of a matrix's targets: for every bottom target that is not an
 // ancestor of another target, its bottom ancestors top-down (the top-region
-// boundary first, the target last).  Targets in the top region need none.
+// boundary first, the target last).  Also stages the targets' label indices
+// (tgt_pos): top targets by position, bottom targets by their element on a
+// path (compact labels, see label_stride).
 int stage_target_paths(bfg_ctx *c, const uint32_t *targets, uint32_t T) {
     const bfg::Layout &L = c->L;
     std::vector<uint32_t> bt;
@@ -1071,27 +1186,50 @@ int stage_target_paths(bfg_ctx *c, const uint32_t *targets, uint32_t T) {
     }
     std::sort(bt.begin(), bt.end(), [&](uint32_t a, uint32_t b) { return L.pos_pre[a] < L.pos_pre[b]; });
     bt.erase(std::unique(bt.begin(), bt.end()), bt.end());
-    std::vector<uint32_t> off{0}, pos, up;
+    std::vector<uint32_t> off{0}, pos, up, pending;
+    std::vector<std::pair<uint32_t, uint32_t>> elem;  // (bottom target position, path element)
     uint64_t arcs = 0;
     for (size_t i = 0; i < bt.size(); ++i) {
         const uint32_t t = bt[i];
+        pending.push_back(t);
         // the next target in pre-order lies in t's subtree iff t is an
         // ancestor of some later target: t is then on that target's path
+        // (and so are the pending targets before it, its ancestors)
         if (i + 1 < bt.size() && L.pos_pre[bt[i + 1]] - L.pos_pre[t] < L.pos_subtree[t]) continue;
         up.clear();
         for (uint32_t x = t; x != NONE && x >= L.n_top; x = L.pos_parent[x]) {
             up.push_back(x);
             arcs += L.in_off[x + 1] - L.in_off[x];
         }
+        const uint32_t e0 = (uint32_t)pos.size(), d0 = L.pos_depth[up.back()];
         pos.insert(pos.end(), up.rbegin(), up.rend());
         off.push_back((uint32_t)pos.size());
+        for (uint32_t q : pending) elem.emplace_back(q, e0 + L.pos_depth[q] - d0);
+        pending.clear();
+    }
+    std::sort(elem.begin(), elem.end());
+    std::vector<uint32_t> tp(T);
+    for (uint32_t t = 0; t < T; ++t) {
+        const uint32_t p = L.pos_of_rank[targets[t]];
+        if (p < L.n_top) {
+            tp[t] = p;
+        } else {
+            auto it = std::lower_bound(elem.begin(), elem.end(), std::make_pair(p, 0u));
+            if (it == elem.end() || it->first != p) return fail(BFG_INTERNAL_ERROR, "target missing from its path");
+            tp[t] = L.n_top + it->second;
+        }
     }
     if (c->path_off.n < off.size()) HIP_TRY(c->path_off.alloc(off.size()));
     if (c->path_pos.n < pos.size()) HIP_TRY(c->path_pos.alloc(pos.size()));
+    if (T > c->tgt_cap) {
+        HIP_TRY(c->tgt_pos.alloc(T));
+        c->tgt_cap = T;
+    }
     HIP_TRY(hipMemcpyAsync(c->path_off.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, c->stream));
     if (!pos.empty())
         HIP_TRY(hipMemcpyAsync(c->path_pos.p, pos.data(), pos.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));  // off and pos die with this frame
+    HIP_TRY(hipMemcpyAsync(c->tgt_pos.p, tp.data(), (size_t)T * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));  // off, pos and tp die with this frame
     c->n_paths = (uint32_t)(off.size() - 1);
     c->path_nodes = pos.size();
     c->path_arcs = arcs;
@@ -1200,6 +1338,7 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     HIP_TRY(c->in_split.upload(L.in_split, s));
     HIP_TRY(c->in_src.upload(L.in_src, s));
     HIP_TRY(c->in_w.upload(L.in_w, s));
+    HIP_TRY(c->in_dd.upload(L.in_dd, s));
     HIP_TRY(c->ch_off.upload(L.ch_off, s));
     HIP_TRY(c->ch_pos.upload(L.ch_pos, s));
     HIP_TRY(c->cs_off.upload(L.cs_off, s));
@@ -1225,9 +1364,6 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     HIP_TRY(c->pr_off.upload(L.pr_off, s));
     HIP_TRY(c->pr_x.upload(L.pr_x, s));
     HIP_TRY(c->pr_w.upload(L.pr_w, s));
-    HIP_TRY(c->du_seg.upload(L.du_seg, s));
-    HIP_TRY(c->du_r0.upload(L.du_r0, s));
-    HIP_TRY(c->du_r1.upload(L.du_r1, s));
     HIP_TRY(c->pk_seg.upload(L.pk_seg, s));
     HIP_TRY(hipStreamSynchronize(s));
     c->graph_bytes = 0;
@@ -1235,16 +1371,16 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
                     &c->in_split, &c->in_src, &c->in_w, &c->ch_off, &c->ch_pos, &c->cs_off, &c->cs_src, &c->cs_aoff,
                     &c->ca_w, &c->up_off, &c->up_tgt, &c->up_w, &c->xr_off, &c->xr_wid, &c->xw_off, &c->pr_wid,
                     &c->pw_off, &c->xr_src, &c->xr_w, &c->tb_dn,
-                    &c->tb_up, &c->tb_w, &c->pt_blk, &c->pt_tgt, &c->pr_off, &c->pr_x, &c->pr_w, &c->du_seg,
-                    &c->du_r0, &c->du_r1, &c->pk_seg})
+                    &c->tb_up, &c->tb_w, &c->pt_blk, &c->pt_tgt, &c->pr_off, &c->pr_x, &c->pr_w, &c->pk_seg})
         c->graph_bytes += b->n * sizeof(uint32_t);
-    c->graph_bytes += c->ca_v.n * sizeof(uint16_t);
+    c->graph_bytes += (c->ca_v.n + c->in_dd.n) * sizeof(uint16_t);
     // host copies the kernels alone need
     for (auto *v : {&L.in_src, &L.in_w, &L.up_tgt, &L.up_w, &L.cs_src, &L.cs_aoff, &L.ca_w, &L.ch_pos,
                     &L.xr_src, &L.xr_w, &L.tb_dn, &L.tb_up, &L.tb_w, &L.pt_tgt, &L.pr_off, &L.pr_x, &L.pr_w,
-                    &L.du_seg, &L.du_r0, &L.du_r1, &L.pk_seg})
+                    &L.pk_seg})
         std::vector<uint32_t>().swap(*v);
     std::vector<uint16_t>().swap(L.ca_v);
+    std::vector<uint16_t>().swap(L.in_dd);
     *out = c.release();
     return BFG_SUCCESS;
 }
@@ -1307,8 +1443,7 @@ int bfg_ctx_configure(bfg_ctx *c, uint32_t lanes, uint32_t lane_capacity) {
 static int matrix_impl(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *tgt, uint32_t T, uint32_t thr,
                        uint32_t *d_out, uint32_t *host_out) {
     Req req = single_source_req(src, S, thr);
-    int rc = stage_targets(c, tgt, T);
-    if (!rc) rc = stage_target_paths(c, tgt, T);
+    int rc = stage_target_paths(c, tgt, T);
     if (rc) return rc;
     struct PathsOn {  // restrict the bottom sweep for this call only
         bfg_ctx *c;

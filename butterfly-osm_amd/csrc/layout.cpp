@@ -398,6 +398,21 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
         });
         L.m_up_intra = intra;
     }
+    // ---- depth gaps of the bottom-sourced arcs of bottom heads ----
+    {
+        const uint32_t ib = L.in_off[L.n_top];
+        L.in_dd.assign(L.in_off[n] - ib, 0);
+        bool deep = false;
+        parallel_blocks(n - L.n_top, 4096, [&](size_t b, size_t e) {
+            for (uint32_t p = L.n_top + (uint32_t)b; p < L.n_top + e; ++p)
+                for (uint32_t a = L.in_off[p]; a < L.in_split[p]; ++a) {
+                    const uint32_t dd = L.pos_depth[p] - L.pos_depth[L.in_src[a]];
+                    if (dd > 0xFFFF) deep = true;
+                    L.in_dd[a - ib] = (uint16_t)dd;
+                }
+        });
+        if (deep) { err = "bottom subtree deeper than 65535"; return false; }
+    }
     // ---- bottom chunks: bottom nodes in pre-order, their top-sourced DOWN arcs
     // grouped by source ----
     const uint32_t BB = std::max<uint32_t>(lp.block_limit, 1);
@@ -574,18 +589,6 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
         L.pt_blk[nseg] = (uint32_t)(L.pt_tgt.size() / 64);
         L.pr_off.push_back((uint32_t)L.pr_x.size());
         L.pw_off.push_back((uint32_t)L.pr_w.size());
-        const uint32_t R = std::max<uint32_t>(lp.ell_chunk, 1);
-        L.du_lvl.clear(); L.du_seg.clear(); L.du_r0.clear(); L.du_r1.clear();
-        for (size_t l = 0; l + 1 < L.seglvl_seg.size(); ++l) {
-            L.du_lvl.push_back((uint32_t)L.du_seg.size());
-            for (uint32_t s = L.seglvl_seg[l]; s < L.seglvl_seg[l + 1]; ++s)
-                for (uint32_t r = L.xr_off[s]; r < L.xr_off[s + 1]; r += R) {
-                    L.du_seg.push_back(s);
-                    L.du_r0.push_back(r);
-                    L.du_r1.push_back(std::min(r + R, L.xr_off[s + 1]));
-                }
-        }
-        L.du_lvl.push_back((uint32_t)L.du_seg.size());
         L.pk_seg.assign(L.pt_blk[nseg], 0);
         for (uint32_t s = 0; s < nseg; ++s)
             for (uint32_t k = L.pt_blk[s]; k < L.pt_blk[s + 1]; ++k) L.pk_seg[k] = s;
@@ -664,6 +667,18 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
     if (failed()) return false;
     for (size_t s = 0; s + 1 < L.seg_pos.size(); ++s)
         if (L.seg_pos[s + 1] <= L.seg_pos[s]) { err = "empty segment"; return false; }
+    if (L.in_dd.size() != L.in_off[n] - L.in_off[L.n_top]) { err = "depth gap table disagrees"; return false; }
+    parallel_blocks(n - L.n_top, 4096, [&](size_t b, size_t e) {
+        for (uint32_t p = L.n_top + (uint32_t)b; p < L.n_top + e; ++p)
+            for (uint32_t a = L.in_off[p]; a < L.in_split[p]; ++a) {
+                const uint32_t u = L.in_src[a], dd = L.in_dd[a - L.in_off[L.n_top]];
+                // the source is the ancestor dd levels up: follow the parents
+                uint32_t x = p;
+                for (uint32_t k = 0; k < dd && x != NONE; ++k) x = L.pos_parent[x];
+                if (dd == 0 || x != u) flag("depth gap of a bottom arc");
+            }
+    });
+    if (failed()) return false;
     // bottom chunks: every bottom position once, sources in the top region
     {
         std::vector<uint8_t> seen(n, 0);
@@ -816,25 +831,6 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
             }
         });
         if (failed()) return false;
-        // DOWN work units: the units of a level cover the rows of its segments once
-        if (L.du_lvl.size() != L.seglvl_seg.size() || L.du_r0.size() != L.du_seg.size() ||
-            L.du_r1.size() != L.du_seg.size()) {
-            err = "work unit tables disagree";
-            return false;
-        }
-        std::vector<uint8_t> cover(L.xr_off[nseg], 0);
-        for (size_t l = 0; l + 1 < L.du_lvl.size(); ++l)
-            for (uint32_t u = L.du_lvl[l]; u < L.du_lvl[l + 1]; ++u) {
-                const uint32_t s = L.du_seg[u];
-                if (s >= nseg || lvl_of_seg[s] != l || L.du_r0[u] >= L.du_r1[u] || L.du_r0[u] < L.xr_off[s] ||
-                    L.du_r1[u] > L.xr_off[s + 1]) {
-                    err = "bad work unit";
-                    return false;
-                }
-                for (uint32_t r = L.du_r0[u]; r < L.du_r1[u]; ++r) cover[r]++;
-            }
-        for (uint8_t c : cover)
-            if (c != 1) { err = "work units do not cover the rows once"; return false; }
         if (L.pk_seg.size() != L.pt_blk[nseg]) { err = "target block table disagrees"; return false; }
         for (uint32_t s = 0; s < nseg; ++s)
             for (uint32_t k = L.pt_blk[s]; k < L.pt_blk[s + 1]; ++k)

@@ -48,7 +48,6 @@ struct LayoutParams {
     uint32_t bottom_limit = 1024;   // subtree size at or below which a node is bottom
     uint32_t segment_limit = 64;    // max nodes of one chain segment (one register tile, <= 64)
     uint32_t block_limit = 256;     // max nodes of one bottom block (LDS resident)
-    uint32_t ell_chunk = 64;        // max ELL rows of one DOWN work unit
 };
 
 struct Layout {
@@ -82,6 +81,9 @@ struct Layout {
     // bottom sources, [in_split[p], in_off[p+1]) from top sources.
     std::vector<uint32_t> ch_off, ch_pos, cs_off, cs_src, cs_aoff, ca_w;
     std::vector<uint16_t> ca_v;
+    // For a bottom head p, arc a in [in_off[p], in_split[p]) comes from a
+    // bottom ancestor in_dd[a - in_off[n_top]] levels above p.
+    std::vector<uint16_t> in_dd;
     // Dense tiles of the top region (segments of at most 64 nodes; a thread of
     // the segment kernels holds one lane's labels of a whole segment).  A row
     // is a key (a position) and W weights, the key's arcs to the W columns of
@@ -101,10 +103,8 @@ struct Layout {
     std::vector<uint32_t> xr_off, xr_wid, xw_off, xr_src, xr_w;
     std::vector<uint32_t> tb_dn, tb_up, tb_w;
     std::vector<uint32_t> pt_blk, pt_tgt, pr_off, pr_wid, pw_off, pr_x, pr_w;
-    // Work units of the DOWN rows: unit u relaxes rows [du_r0[u], du_r1[u]) of
-    // segment du_seg[u] (at most ell_chunk rows); segment level l owns units
-    // [du_lvl[l], du_lvl[l+1]).  pk_seg[k] is the segment of target block k.
-    std::vector<uint32_t> du_lvl, du_seg, du_r0, du_r1, pk_seg;
+    // pk_seg[k] is the segment of target block k.
+    std::vector<uint32_t> pk_seg;
     uint64_t ell_down_entries = 0, ell_up_entries = 0;
     // statistics
     uint64_t m_up = 0, m_down = 0, m_up_inf = 0, m_down_inf = 0;
