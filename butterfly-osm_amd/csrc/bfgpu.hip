@@ -309,9 +309,19 @@ __device__ __forceinline__ void rows_w(uint32_t (&acc)[W], uint32_t r0, uint32_t
 // Intra-segment arcs of a segment of at most 8*NO nodes in registers
 // (acc[j] = label of node j for this lane): 8x8 blocks T[8a + o] of tw, DOWN
 // from the first octet to the last (the diagonal block of an octet first, i
-// ascending), UP from the last to the first (i descending).
+// ascending), UP from the last to the first (i descending).  The block ids
+// come in with one vector load (lane i holds T[i]) and every block's 64
+// weights with one more each (lane j holds weight j), all issued before the
+// first is used, so a wavefront waits on its loads once; a weight reaches
+// the adds as a scalar through v_readlane.  Blocks without arcs load from
+// tw[0 .. 63] (the table always holds one block) and are skipped.
 template <bool DOWN, int NO>
-__device__ __forceinline__ void tri_apply(uint32_t (&acc)[8 * NO], const cu32 *T, const cu32 *tw) {
+__device__ __forceinline__ void tri_apply(uint32_t (&acc)[8 * NO], const uint32_t *T, const uint32_t *tw) {
+    constexpr int NB = NO * (NO + 1) / 2;
+    const uint32_t lane = __lane_id();
+    const uint32_t kv = T[lane];
+    uint32_t wv[NB];
+    int b = 0;
 #pragma unroll
     for (int aa = 0; aa < NO; ++aa) {
         const int a = DOWN ? aa : NO - 1 - aa;
@@ -319,16 +329,32 @@ __device__ __forceinline__ void tri_apply(uint32_t (&acc)[8 * NO], const cu32 *T
         for (int oo = 0; oo < NO; ++oo) {
             const int o = DOWN ? a + oo : a - oo;
             if (o < 0 || o >= NO) continue;
-            const uint32_t k = T[a * 8 + o];
+            const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)kv, a * 8 + o);
+            wv[b++] = tw[(size_t)(k == NONE ? 0u : k) * 64 + lane];
+        }
+    }
+    // every load above is complete here, and none is moved below
+#pragma unroll
+    for (int i = 0; i < NB; ++i) asm volatile("" ::"v"(wv[i]));
+    b = 0;
+#pragma unroll
+    for (int aa = 0; aa < NO; ++aa) {
+        const int a = DOWN ? aa : NO - 1 - aa;
+#pragma unroll
+        for (int oo = 0; oo < NO; ++oo) {
+            const int o = DOWN ? a + oo : a - oo;
+            if (o < 0 || o >= NO) continue;
+            const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)kv, a * 8 + o);
+            const uint32_t wb = wv[b++];
             if (k == NONE) continue;
-            const cu32 *W = tw + (size_t)k * 64;
 #pragma unroll
             for (int ii = 0; ii < 8; ++ii) {
                 const int i = DOWN ? ii : 7 - ii;
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     if (o == a && (DOWN ? j <= i : j >= i)) continue;
-                    acc[o * 8 + j] = min(acc[o * 8 + j], sat_add(acc[a * 8 + i], W[i * 8 + j]));
+                    const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)wb, i * 8 + j);
+                    acc[o * 8 + j] = min(acc[o * 8 + j], sat_add(acc[a * 8 + i], w));
                 }
             }
         }
@@ -443,7 +469,7 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_kernel(uint32_t first, uint3
     uint32_t acc[64];
 #pragma unroll
     for (int j = 0; j < 64; ++j) acc[j] = j < len ? lab[(size_t)(p0 + j) * KL] : INF;
-    tri_apply<DOWN, 8>(acc, as_const(tab) + (size_t)s * 64, as_const(tw));
+    tri_apply<DOWN, 8>(acc, tab + (size_t)s * 64, tw);
 #pragma unroll
     for (int j = 0; j < 64; ++j)
         if (j < len) lab[(size_t)(p0 + j) * KL] = acc[j];
@@ -552,6 +578,104 @@ __global__ __launch_bounds__(NTH) void bottom_chunk_kernel(
     for (uint32_t i = slot; i < len; i += NS) lab[(size_t)ch_pos[i0 + i] * KL + lane] = L[i * KL + lane];
 }
 
+// bottom_path for KL = 64, where a slot is one wavefront and everything
+// indexed by the path is wave-uniform.  A wavefront walks its path's
+// elements in chunks of 64: lane j gathers element j's position, arc range
+// and pre-order interval once, and each element reads them back by
+// v_readlane.  An element's arcs come in as vectors (lane j: arc a0 + j, its
+// label index and weight), those of the next element while the current
+// element's labels are in flight, so the walk waits once per element: on
+// its labels, the first 32 arcs' in one batch (the rest, rare, in further
+// batches).  Lanes past an element's arcs read label 0 with weight INF.
+__device__ __forceinline__ void path_arcs(uint32_t wl, uint32_t li, uint32_t a, uint32_t as, uint32_t a1, uint32_t ib,
+                                          const uint16_t *__restrict__ in_dd, const uint32_t *__restrict__ in_src,
+                                          const uint32_t *__restrict__ in_w, uint32_t &mi, uint32_t &mw) {
+    mi = 0;
+    mw = INF;
+    a += wl;
+    if (a < a1) {
+        mi = a < as ? li - in_dd[a - ib] : in_src[a];
+        mw = in_w[a];
+    }
+}
+
+__device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t j) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j);
+}
+
+__device__ void bottom_path_wave(uint32_t e0, uint32_t e1, uint32_t n_top, const uint32_t *__restrict__ path_pos,
+                                 const uint32_t *__restrict__ in_off, const uint32_t *__restrict__ in_split,
+                                 const uint16_t *__restrict__ in_dd, const uint32_t *__restrict__ in_src,
+                                 const uint32_t *__restrict__ in_w, const uint32_t *__restrict__ pos_pre,
+                                 const uint32_t *__restrict__ pos_sub, const uint32_t *__restrict__ pos_depth,
+                                 uint32_t sj0, uint32_t sj1, const uint32_t *__restrict__ seed_pre,
+                                 const uint32_t *__restrict__ seed_depth, const uint32_t *__restrict__ pathbuf,
+                                 uint32_t D1, uint32_t *__restrict__ lab) {
+    const uint32_t wl = __lane_id();
+    const uint32_t ib = in_off[n_top];  // first arc of a bottom head: in_dd's origin
+    // the lane's first seed, kept in registers; further seeds (rare) are read per element
+    const uint32_t sp0 = sj0 < sj1 ? seed_pre[sj0] : 0u, sd0 = sj0 < sj1 ? seed_depth[sj0] : 0u;
+    for (uint32_t c0 = e0; c0 < e1; c0 += 64) {
+        const uint32_t cn = min(64u, e1 - c0);
+        uint32_t vp = 0, va0 = 0, vas = 0, va1 = 0, vpr = 0, vps = 0, vpd = 0;
+        if (wl < cn) {
+            vp = path_pos[c0 + wl];
+            va0 = in_off[vp];
+            vas = in_split[vp];
+            va1 = in_off[vp + 1];
+            vpr = pos_pre[vp];
+            vps = pos_sub[vp];
+            vpd = pos_depth[vp];
+        }
+        uint32_t mi, mw;
+        path_arcs(wl, n_top + c0, rl(va0, 0), rl(vas, 0), rl(va1, 0), ib, in_dd, in_src, in_w, mi, mw);
+        for (uint32_t j = 0; j < cn; ++j) {
+            const uint32_t li = n_top + c0 + j;  // this element's label index
+            const uint32_t a0 = rl(va0, j), a1 = rl(va1, j);
+            uint32_t x[32];
+#pragma unroll
+            for (int r = 0; r < 32; ++r) x[r] = lab[(size_t)rl(mi, r) * 64];
+            const uint32_t cw = mw;
+            // the next element's first arcs
+            if (j + 1 < cn)
+                path_arcs(wl, li + 1, rl(va0, j + 1), rl(vas, j + 1), rl(va1, j + 1), ib, in_dd, in_src, in_w, mi,
+                          mw);
+            uint32_t m0 = INF, m1 = INF, m2 = INF, m3 = INF;
+            if (sj0 < sj1) {
+                const uint32_t pr = rl(vpr, j), ps = rl(vps, j), pd = rl(vpd, j);
+                if (sp0 - pr < ps) m0 = pathbuf[(size_t)sj0 * D1 + (sd0 - pd)];
+                for (uint32_t q = sj0 + 1; q < sj1; ++q) {
+                    const uint32_t sp = seed_pre[q];
+                    if (sp - pr < ps) m0 = min(m0, pathbuf[(size_t)q * D1 + (seed_depth[q] - pd)]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 32; r += 4) {
+                m0 = min(m0, sat_add(x[r], rl(cw, r)));
+                m1 = min(m1, sat_add(x[r + 1], rl(cw, r + 1)));
+                m2 = min(m2, sat_add(x[r + 2], rl(cw, r + 2)));
+                m3 = min(m3, sat_add(x[r + 3], rl(cw, r + 3)));
+            }
+            // arcs past the first 32 of this element
+            for (uint32_t base = a0 + 32; base < a1; base += 32) {
+                const uint32_t as = rl(vas, j);
+                uint32_t bi, bw;
+                path_arcs(wl & 31u, li, base, as, a1, ib, in_dd, in_src, in_w, bi, bw);
+#pragma unroll
+                for (int r = 0; r < 32; ++r) x[r] = lab[(size_t)rl(bi, r) * 64];
+#pragma unroll
+                for (int r = 0; r < 32; r += 4) {
+                    m0 = min(m0, sat_add(x[r], rl(bw, r)));
+                    m1 = min(m1, sat_add(x[r + 1], rl(bw, r + 1)));
+                    m2 = min(m2, sat_add(x[r + 2], rl(bw, r + 2)));
+                    m3 = min(m3, sat_add(x[r + 3], rl(bw, r + 3)));
+                }
+            }
+            lab[(size_t)li * 64] = min(min(m0, m1), min(m2, m3));
+        }
+    }
+}
+
 // bottom_path: the bottom region restricted to what a matrix reads.  A
 // target's label depends only on its ancestors, so the bottom sweep of a
 // matrix needs the bottom paths from the top region down to its targets
@@ -573,15 +697,23 @@ __global__ __launch_bounds__(NT) void bottom_path_kernel(
     uint32_t *__restrict__ arena, uint32_t n) {
     constexpr int NS = NT / KL;
     const uint32_t lane = threadIdx.x % KL, slot = threadIdx.x / KL;
-    const uint32_t k = blockIdx.x * NS + slot;
+    // block b: group b % groups, paths (b / groups) * NS + slot, so the
+    // groups of one path run side by side and share its arcs in L2
+    const uint32_t groups = (n_lanes + KL - 1) / KL;
+    const uint32_t g = blockIdx.x % groups;
+    const uint32_t k = blockIdx.x / groups * NS + slot;
     if (k >= n_paths) return;  // this kernel has no barrier
-    const uint32_t g = blockIdx.y;
     uint32_t *lab = arena + (size_t)g * n * KL + lane;
     uint32_t sj0 = 0, sj1 = 0;
     const uint32_t gl = g * KL + lane;
     if (gl < n_lanes) {
         sj0 = seed_off[gl];
         sj1 = seed_off[gl + 1];
+    }
+    if constexpr (KL == 64) {
+        bottom_path_wave(path_off[k], path_off[k + 1], n_top, path_pos, in_off, in_split, in_dd, in_src, in_w, pos_pre,
+                         pos_sub, pos_depth, sj0, sj1, seed_pre, seed_depth, pathbuf, D1, lab);
+        return;
     }
     const uint32_t ib = in_off[n_top];  // first arc of a bottom head: in_dd's origin
     for (uint32_t i = path_off[k]; i < path_off[k + 1]; ++i) {
@@ -988,14 +1120,14 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
             ++nlaunch;
         }
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
-        hipLaunchKernelGGL((tri_kernel<KL, true>), dim3(wgs, (s1 - s0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, s0,
-                           s1 - s0, c->seg_pos.p, c->tb_dn.p, c->tb_w.p, c->seg_split.p, R, c->arena.p, n);
+        hipLaunchKernelGGL((tri_kernel<KL, true>), dim3(wgs, (s1 - s0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s,
+                           s0, s1 - s0, c->seg_pos.p, c->tb_dn.p, c->tb_w.p, c->seg_split.p, R, c->arena.p, n);
         ++nlaunch;
     }
     if (c->paths_on) {
         constexpr int NSP = NT / KL;
         if (c->n_paths) {
-            hipLaunchKernelGGL(bottom_path_kernel<KL>, dim3((c->n_paths + NSP - 1) / NSP, groups), dim3(NT), 0, s,
+            hipLaunchKernelGGL(bottom_path_kernel<KL>, dim3((c->n_paths + NSP - 1) / NSP * groups), dim3(NT), 0, s,
                                c->n_paths, c->path_off.p, c->path_pos.p, L.n_top, c->in_off.p, c->in_split.p,
                                c->in_dd.p, c->in_src.p, c->in_w.p, c->pos_pre.p, c->pos_sub.p, c->pos_depth.p,
                                c->seed_off.p, c->seed_pre.p, c->seed_depth.p, c->pathbuf.p, D1, nl, c->arena.p, n);
@@ -1358,6 +1490,7 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     HIP_TRY(c->xr_w.upload(L.xr_w, s));
     HIP_TRY(c->tb_dn.upload(L.tb_dn, s));
     HIP_TRY(c->tb_up.upload(L.tb_up, s));
+    if (L.tb_w.size() < 64) L.tb_w.resize(64, INF);  // tri_apply reads block 0 for absent blocks
     HIP_TRY(c->tb_w.upload(L.tb_w, s));
     HIP_TRY(c->pt_blk.upload(L.pt_blk, s));
     HIP_TRY(c->pt_tgt.upload(L.pt_tgt, s));

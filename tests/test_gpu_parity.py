@@ -111,6 +111,25 @@ def test_matrix_target_paths(engine_cls, grid_config1):
     assert np.array_equal(got, o.compute_matrix_flat_mt(s, t))
 
 
+@pytest.mark.parametrize("lanes", [64, 32])
+def test_matrix_long_paths(engine_cls, grid_config1, monkeypatch, lanes):
+    """A bottom region that is the whole elimination tree: bottom paths longer
+    than 64 elements and elements with more than 32 in-arcs."""
+    g = grid_config1
+    monkeypatch.setenv("BFG_BOTTOM_LIMIT", str(1 << 29))
+    rng = np.random.default_rng(7)
+    t = np.array(root_path(g, int(rng.integers(0, g.n // 8))) + [int(x) for x in rng.integers(0, g.n, 60)],
+                 dtype=np.uint32)
+    s = rng.integers(0, g.n, 80).astype(np.uint32)
+    say("long paths lanes", lanes, "T", len(t))
+    with engine_cls(g) as e:
+        e.configure(lanes=lanes)
+        info = e.info()
+        assert info["n_top"] == 0
+        got, st = e.compute_matrix_flat(s, t)
+    assert np.array_equal(got, Oracle(g).compute_matrix_flat_mt(s, t))
+
+
 @pytest.mark.parametrize("lanes,cap", [(32, 0), (16, 48), (64, 64)])
 def test_matrix_lanes_and_batches(engine_cls, grid_config1, lanes, cap):
     """K in {16, 32, 64}, and a lane capacity that forces several device batches."""
