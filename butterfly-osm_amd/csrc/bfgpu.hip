@@ -227,7 +227,8 @@ __device__ __forceinline__ void relax_arcs(uint32_t *L, uint32_t d, uint32_t a0,
 // through scalar loads, each thread reads the key's label of its lane (one
 // coalesced load per row) and does 64 saturating adds and mins.
 // ---------------------------------------------------------------------------
-constexpr int TILE_THREADS = 256;  // four wavefronts, four independent tiles
+constexpr int TILE_THREADS = 256;  // four wavefronts: four wave groups on one tile
+constexpr uint32_t MAX_GRID_Y = 65535;  // most workgroups along y in one launch
 
 // Read-only tables the kernels index with wave-uniform offsets: read through
 // the constant address space, whose uniform loads become scalar loads.
@@ -246,6 +247,11 @@ __device__ __forceinline__ uint32_t *lane_labels(uint32_t *arena, uint32_t n, ui
     const uint32_t g = wg * (64 / KL) + l / KL;
     if (group) *group = g;
     return arena + (size_t)g * n * KL + l % KL;
+}
+
+// Lane j's value of v, as a scalar (j wave-uniform).
+__device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t j) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j);
 }
 
 // Which wave groups reach which segments.  A lane's upward labels are finite
@@ -285,24 +291,33 @@ __global__ __launch_bounds__(TILE_THREADS) void fill_reach_kernel(uint32_t nseg,
 
 // acc[j] = min(acc[j], lab[key[r]] + wr[W(r - r0) + j]) over rows [r0, r1)
 // of width W; two rows per step (min(acc, x0, x1) is one v_min3), the labels
-// of the next two rows loaded one step ahead.  An odd last row pairs with
-// itself and INF.
+// of the next two rows loaded one step ahead.  The keys of 64 rows come in
+// with one vector load (lane j: row c0 + j) and reach the label addresses by
+// v_readlane, so a step waits on no scalar load of its own besides the
+// weights.  Reads past a chunk's last row re-read that row (label and
+// weights), so an odd last row pairs with itself: min(acc, x, x) is exact.
 template <int KL, int W>
-__device__ __forceinline__ void rows_w(uint32_t (&acc)[W], uint32_t r0, uint32_t r1, const cu32 *K, const cu32 *wr,
-                                       const uint32_t *lab) {
-    if (r0 >= r1) return;
-    uint32_t n0 = lab[(size_t)K[r0] * KL];
-    uint32_t n1 = r0 + 1 < r1 ? lab[(size_t)K[r0 + 1] * KL] : INF;
-    for (uint32_t r = r0; r < r1; r += 2) {
-        const uint32_t a0 = n0, a1 = n1;
-        const cu32 *W0 = wr + (size_t)(r - r0) * W;
-        const cu32 *W1 = r + 1 < r1 ? W0 + W : W0;
-        if (r + 2 < r1) {
-            n0 = lab[(size_t)K[r + 2] * KL];
-            n1 = r + 3 < r1 ? lab[(size_t)K[r + 3] * KL] : INF;
-        }
+__device__ __forceinline__ void rows_w(uint32_t (&acc)[W], uint32_t r0, uint32_t r1, const uint32_t *key,
+                                       const cu32 *wr, const uint32_t *lab) {
+    const uint32_t wl = __lane_id();
+    for (uint32_t c0 = r0; c0 < r1; c0 += 64) {
+        const uint32_t cl = min(64u, r1 - c0) - 1;  // last row of the chunk, relative
+        const uint32_t kv = key[c0 + min(wl, cl)];
+        uint32_t n0 = lab[(size_t)rl(kv, 0) * KL];
+        uint32_t n1 = lab[(size_t)rl(kv, min(1u, cl)) * KL];
+        const cu32 *W0 = wr + (size_t)(c0 - r0) * W;
+        for (uint32_t j = 0; j <= cl; j += 2) {
+            const uint32_t a0 = n0, a1 = n1;
+            const cu32 *W1 = j < cl ? W0 + W : W0;
+            n0 = lab[(size_t)rl(kv, min(j + 2, cl)) * KL];
+            n1 = lab[(size_t)rl(kv, min(j + 3, cl)) * KL];
 #pragma unroll
-        for (int j = 0; j < W; ++j) acc[j] = min(acc[j], min(sat_add(a0, W0[j]), sat_add(a1, W1[j])));
+            for (int q = 0; q < W; ++q) acc[q] = min(acc[q], min(sat_add(a0, W0[q]), sat_add(a1, W1[q])));
+            // the next labels are waited for here, after this step's adds:
+            // the loads stay one step ahead of their use
+            asm volatile("" : "+v"(n0), "+v"(n1));
+            W0 += 2 * W;
+        }
     }
 }
 
@@ -367,7 +382,7 @@ __device__ __forceinline__ void tri_apply(uint32_t (&acc)[8 * NO], const uint32_
 // holds all rows of its segment (whole) and whose wave group does not reach
 // the segment stores instead: those upward labels are INF and never written.
 template <int KL, int W, bool PUSH>
-__device__ __forceinline__ void rows_unit(uint32_t ra, uint32_t rb, const cu32 *K, const cu32 *wr, const cu32 *out,
+__device__ __forceinline__ void rows_unit(uint32_t ra, uint32_t rb, const uint32_t *K, const cu32 *wr, const cu32 *out,
                                           uint32_t p0, uint32_t len, uint32_t *lab, bool whole, bool reach) {
     uint32_t acc[W];
 #pragma unroll
@@ -391,8 +406,10 @@ __device__ __forceinline__ void rows_unit(uint32_t ra, uint32_t rb, const cu32 *
     }
 }
 
-// rows: wavefront w of workgroup (wave group blockIdx.x, blockIdx.y) takes
-// unit u0 + 4*blockIdx.y + w.
+// rows: wavefront w of workgroup (blockIdx.x, blockIdx.y) takes unit
+// u0 + blockIdx.y for wave group 4*blockIdx.x + w < wgs: the four wavefronts
+// of a workgroup read the same weights, so three of them find the rows in
+// the scalar cache.
 //   DOWN (PUSH = false): unit = rows [r0[u], r1[u]) of segment
 //     seg[u] & ~UNIT_WHOLE (DOWN arcs from outside the segment), width
 //     wid[s], weights from woff[s] + (r - rbase[s]) * W, merged into the
@@ -412,14 +429,14 @@ __global__ __launch_bounds__(TILE_THREADS) void rows_kernel(
     const uint32_t *__restrict__ r1, const uint32_t *__restrict__ rbase, const uint32_t *__restrict__ wid,
     const uint32_t *__restrict__ woff, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ tgt,
     const uint32_t *__restrict__ key, const uint32_t *__restrict__ wts, Reach R, uint32_t *__restrict__ arena,
-    uint32_t n) {
-    const uint32_t u = u0 + blockIdx.y * (TILE_THREADS / 64) + wave_id();
-    if (u >= u1) return;  // wave-uniform; this kernel has no barrier
-    uint32_t *lab = lane_labels<KL>(arena, n, blockIdx.x, nullptr);
+    uint32_t n, uint32_t wgs) {
+    const uint32_t u = u0 + blockIdx.y, wg = blockIdx.x * (TILE_THREADS / 64) + wave_id();
+    if (u >= u1 || wg >= wgs) return;  // wave-uniform; this kernel has no barrier
+    uint32_t *lab = lane_labels<KL>(arena, n, wg, nullptr);
     const uint32_t sw = seg[u], s = sw & ~UNIT_WHOLE;
-    const bool reach = wave_reaches(R, blockIdx.x, seg_pos[s]);
+    const bool reach = wave_reaches(R, wg, seg_pos[s]);
     if (PUSH && !reach) return;  // the segment's labels are all INF: nothing to push
-    const cu32 *K = as_const(key);
+    const uint32_t *K = key;
     uint32_t ra, rb, W, p0 = 0, len = 0;
     const cu32 *wr, *out = nullptr;
     if constexpr (PUSH) {
@@ -442,8 +459,9 @@ __global__ __launch_bounds__(TILE_THREADS) void rows_kernel(
     else rows_unit<KL, 64, PUSH>(ra, rb, K, wr, out, p0, len, lab, whole, reach);
 }
 
-// tri: wavefront w of workgroup (wave group blockIdx.x, blockIdx.y) takes
-// segment first + 4*blockIdx.y + w < first + count of one level, loads its
+// tri: wavefront w of workgroup (blockIdx.x, blockIdx.y) takes segment
+// first + blockIdx.y < first + count of one level for wave group
+// 4*blockIdx.x + w < wgs (the workgroup shares the segment's blocks), loads its
 // labels, relaxes its intra-segment arcs (tri_apply) and stores them.  UP: a
 // wave group that does not reach the segment skips it, only writing INF
 // where the DOWN plan merges into the labels with atomics (split[s]).
@@ -453,14 +471,15 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_kernel(uint32_t first, uint3
                                                            const uint32_t *__restrict__ tab,
                                                            const uint32_t *__restrict__ tw,
                                                            const uint8_t *__restrict__ split, Reach R,
-                                                           uint32_t *__restrict__ arena, uint32_t n) {
-    const uint32_t i = blockIdx.y * (TILE_THREADS / 64) + wave_id();
-    if (i >= count) return;  // wave-uniform; this kernel has no barrier
+                                                           uint32_t *__restrict__ arena, uint32_t n,
+                                                           uint32_t wgs) {
+    const uint32_t i = blockIdx.y, wg = blockIdx.x * (TILE_THREADS / 64) + wave_id();
+    if (i >= count || wg >= wgs) return;  // wave-uniform; this kernel has no barrier
     const uint32_t s = first + i;
-    uint32_t *lab = lane_labels<KL>(arena, n, blockIdx.x, nullptr);
+    uint32_t *lab = lane_labels<KL>(arena, n, wg, nullptr);
     const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
     if constexpr (!DOWN) {
-        if (!wave_reaches(R, blockIdx.x, p0)) {
+        if (!wave_reaches(R, wg, p0)) {
             if (split[s])
                 for (uint32_t j = 0; j < len; ++j) lab[(size_t)(p0 + j) * KL] = INF;
             return;
@@ -597,10 +616,6 @@ __device__ __forceinline__ void path_arcs(uint32_t wl, uint32_t li, uint32_t a, 
         mi = a < as ? li - in_dd[a - ib] : in_src[a];
         mw = in_w[a];
     }
-}
-
-__device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t j) {
-    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j);
 }
 
 __device__ void bottom_path_wave(uint32_t e0, uint32_t e1, uint32_t n_top, const uint32_t *__restrict__ path_pos,
@@ -1074,7 +1089,8 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
     if (rc) return rc;
     HIP_TRY(hipEventRecord(c->ev[0], s));
     const Reach R{c->seed_off.p, c->seed_pre.p, c->pos_pre.p, c->pos_sub.p, nl};
-    constexpr uint32_t WPB = TILE_THREADS / 64;  // tiles per workgroup
+    constexpr uint32_t WPB = TILE_THREADS / 64;  // wave groups per workgroup
+    const uint32_t wgx = (wgs + WPB - 1) / WPB;  // workgroups across the wave groups
     if (nseg) {
         hipLaunchKernelGGL(fill_reach_kernel<KL>, dim3(wgs, (nseg + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, nseg,
                            c->seg_pos.p, R, c->arena.p, n);
@@ -1087,16 +1103,16 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
     uint32_t ndom = 0;  // DOWN row launches of this batch
     for (int l = (int)nsl - 1; l >= 0; --l) {
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
-        hipLaunchKernelGGL((tri_kernel<KL, false>), dim3(wgs, (s1 - s0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s,
-                           s0, s1 - s0, c->seg_pos.p, c->tb_up.p, c->tb_w.p, c->seg_split.p, R, c->arena.p, n);
-        ++nlaunch;
+        for (uint32_t b = s0; b < s1; b += MAX_GRID_Y, ++nlaunch)
+            hipLaunchKernelGGL((tri_kernel<KL, false>), dim3(wgx, std::min(s1 - b, MAX_GRID_Y)), dim3(TILE_THREADS),
+                               0, s, b, std::min(s1 - b, MAX_GRID_Y), c->seg_pos.p, c->tb_up.p, c->tb_w.p,
+                               c->seg_split.p, R, c->arena.p, n, wgs);
         const uint32_t k0 = L.pt_blk[s0], k1 = L.pt_blk[s1];
-        if (k1 > k0) {
-            hipLaunchKernelGGL((rows_kernel<KL, true>), dim3(wgs, (k1 - k0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0,
-                               s, k0, k1, c->pk_seg.p, c->pr_off.p, c->pr_off.p, c->pr_off.p, c->pr_wid.p,
-                               c->pw_off.p, c->seg_pos.p, c->pt_tgt.p, c->pr_x.p, c->pr_w.p, R, c->arena.p, n);
-            ++nlaunch;
-        }
+        for (uint32_t b = k0; b < k1; b += MAX_GRID_Y, ++nlaunch)
+            hipLaunchKernelGGL((rows_kernel<KL, true>), dim3(wgx, std::min(k1 - b, MAX_GRID_Y)), dim3(TILE_THREADS),
+                               0, s, b, std::min(k1, b + MAX_GRID_Y), c->pk_seg.p, c->pr_off.p, c->pr_off.p,
+                               c->pr_off.p, c->pr_wid.p, c->pw_off.p, c->seg_pos.p, c->pt_tgt.p, c->pr_x.p, c->pr_w.p,
+                               R, c->arena.p, n, wgs);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[1], s));
@@ -1111,18 +1127,20 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
                 }
             }
             HIP_TRY(hipEventRecord(c->dom_ev[2 * ndom], s));
-            hipLaunchKernelGGL((rows_kernel<KL, false>), dim3(wgs, (u1 - u0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0,
-                               s, u0, u1, c->du_seg.p, c->du_r0.p, c->du_r1.p, c->xr_off.p, c->xr_wid.p,
-                               c->xw_off.p, c->seg_pos.p, c->pt_tgt.p, c->xr_src.p, c->xr_w.p, R, c->arena.p, n);
+            for (uint32_t b = u0; b < u1; b += MAX_GRID_Y, ++nlaunch)
+                hipLaunchKernelGGL((rows_kernel<KL, false>), dim3(wgx, std::min(u1 - b, MAX_GRID_Y)),
+                                   dim3(TILE_THREADS), 0, s, b, std::min(u1, b + MAX_GRID_Y), c->du_seg.p, c->du_r0.p,
+                                   c->du_r1.p, c->xr_off.p, c->xr_wid.p, c->xw_off.p, c->seg_pos.p, c->pt_tgt.p,
+                                   c->xr_src.p, c->xr_w.p, R, c->arena.p, n, wgs);
             HIP_TRY(hipEventRecord(c->dom_ev[2 * ndom + 1], s));
             ++ndom;
             pt->dom_bytes += (uint64_t)wgs * c->lvl_dom_bytes[l];
-            ++nlaunch;
         }
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
-        hipLaunchKernelGGL((tri_kernel<KL, true>), dim3(wgs, (s1 - s0 + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s,
-                           s0, s1 - s0, c->seg_pos.p, c->tb_dn.p, c->tb_w.p, c->seg_split.p, R, c->arena.p, n);
-        ++nlaunch;
+        for (uint32_t b = s0; b < s1; b += MAX_GRID_Y, ++nlaunch)
+            hipLaunchKernelGGL((tri_kernel<KL, true>), dim3(wgx, std::min(s1 - b, MAX_GRID_Y)), dim3(TILE_THREADS), 0,
+                               s, b, std::min(s1 - b, MAX_GRID_Y), c->seg_pos.p, c->tb_dn.p, c->tb_w.p,
+                               c->seg_split.p, R, c->arena.p, n, wgs);
     }
     if (c->paths_on) {
         constexpr int NSP = NT / KL;
