@@ -289,17 +289,19 @@ __global__ __launch_bounds__(TILE_THREADS) void fill_reach_kernel(uint32_t nseg,
     for (uint32_t j = 0; j < len; ++j) lab[(size_t)(p0 + j) * KL] = INF;
 }
 
-// acc[j] = min(acc[j], lab[key[r]] + wr[W(r - r0) + j]) over rows [r0, r1)
-// of width W; two rows per step (min(acc, x0, x1) is one v_min3), the labels
-// of the next two rows loaded one step ahead.  The keys of 64 rows come in
-// with one vector load (lane j: row c0 + j) and reach the label addresses by
-// v_readlane, so a step waits on no scalar load of its own besides the
-// weights.  Reads past a chunk's last row re-read that row (label and
-// weights), so an odd last row pairs with itself: min(acc, x, x) is exact.
-template <int KL, int W>
-__device__ __forceinline__ void rows_w(uint32_t (&acc)[W], uint32_t r0, uint32_t r1, const uint32_t *key,
-                                       const cu32 *wr, const uint32_t *lab) {
-    const uint32_t wl = __lane_id();
+// acc[j] = min(acc[j], lab[key[r]] + wr[8k(r - r0) + j]) over rows [r0, r1)
+// of k octets (8k weights, the first of the tile's N columns; k is
+// wave-uniform, an octet past it is skipped by a scalar branch); two rows
+// per step (min(acc, x0, x1) is one v_min3), the labels of the next two rows
+// loaded one step ahead.  The keys of 64 rows come in with one vector load
+// (lane j: row c0 + j) and reach the label addresses by v_readlane, so a
+// step waits on no scalar load of its own besides the weights.  Reads past a
+// chunk's last row re-read that row (label and weights), so an odd last row
+// pairs with itself: min(acc, x, x) is exact.
+template <int KL, int N>
+__device__ __forceinline__ void rows_w(uint32_t (&acc)[N], uint32_t r0, uint32_t r1, const uint32_t *key,
+                                       const cu32 *wr, uint32_t k, const uint32_t *lab) {
+    const uint32_t wl = __lane_id(), W = 8 * k;
     for (uint32_t c0 = r0; c0 < r1; c0 += 64) {
         const uint32_t cl = min(64u, r1 - c0) - 1;  // last row of the chunk, relative
         const uint32_t kv = key[c0 + min(wl, cl)];
@@ -312,7 +314,12 @@ __device__ __forceinline__ void rows_w(uint32_t (&acc)[W], uint32_t r0, uint32_t
             n0 = lab[(size_t)rl(kv, min(j + 2, cl)) * KL];
             n1 = lab[(size_t)rl(kv, min(j + 3, cl)) * KL];
 #pragma unroll
-            for (int q = 0; q < W; ++q) acc[q] = min(acc[q], min(sat_add(a0, W0[q]), sat_add(a1, W1[q])));
+            for (int o = 0; o < N / 8; ++o) {
+                if (o > 0 && (uint32_t)o >= k) break;
+#pragma unroll
+                for (int q = 8 * o; q < 8 * o + 8; ++q)
+                    acc[q] = min(acc[q], min(sat_add(a0, W0[q]), sat_add(a1, W1[q])));
+            }
             // the next labels are waited for here, after this step's adds:
             // the loads stay one step ahead of their use
             asm volatile("" : "+v"(n0), "+v"(n1));
@@ -381,38 +388,50 @@ __device__ __forceinline__ void tri_apply(uint32_t (&acc)[8 * NO], const uint32_
 // one return-less atomic min per finite (position, lane).  A DOWN unit that
 // holds all rows of its segment (whole) and whose wave group does not reach
 // the segment stores instead: those upward labels are INF and never written.
-template <int KL, int W, bool PUSH>
-__device__ __forceinline__ void rows_unit(uint32_t ra, uint32_t rb, const uint32_t *K, const cu32 *wr, const cu32 *out,
-                                          uint32_t p0, uint32_t len, uint32_t *lab, bool whole, bool reach) {
+template <int KL, int W>
+__device__ __forceinline__ void push_unit(uint32_t ra, uint32_t rb, const uint32_t *K, const cu32 *wr, const cu32 *out,
+                                          uint32_t *lab) {
     uint32_t acc[W];
 #pragma unroll
     for (int j = 0; j < W; ++j) acc[j] = INF;
-    rows_w<KL, W>(acc, ra, rb, K, wr, lab);
-    if constexpr (!PUSH) {
-        if (whole && !reach) {  // the upward labels are INF and were never written: store
+    rows_w<KL, W>(acc, ra, rb, K, wr, W / 8, lab);
 #pragma unroll
-            for (int j = 0; j < W; ++j)
-                if (j < len) lab[(size_t)(p0 + j) * KL] = acc[j];
-            return;
-        }
+    for (int j = 0; j < W; ++j)
+        if (out[j] != NONE && acc[j] != INF) atomicMin(lab + (size_t)out[j] * KL, acc[j]);
+}
+
+// A DOWN unit of width W: rows [ra, rb) of one run of a segment (8W/8
+// weights each, from wr), reduced into acc[W] and merged into the segment's
+// labels p0 .. p0 + len: stored when the unit holds all rows of its segment
+// (whole) and its wave group does not reach the segment (the upward labels
+// are INF and were never written; columns past W get INF), else merged by
+// one atomic min per finite (node, lane).
+template <int KL, int W>
+__device__ __forceinline__ void down_unit(uint32_t ra, uint32_t rb, const uint32_t *K, const cu32 *wr, uint32_t p0,
+                                          uint32_t len, uint32_t *lab, bool whole, bool reach) {
+    uint32_t acc[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) acc[j] = INF;
+    rows_w<KL, W>(acc, ra, rb, K, wr, W / 8, lab);
+    if (whole && !reach) {
+#pragma unroll
+        for (int j = 0; j < 64; ++j)
+            if (j < len) lab[(size_t)(p0 + j) * KL] = j < W ? acc[j] : INF;
+        return;
     }
 #pragma unroll
-    for (int j = 0; j < W; ++j) {
-        if constexpr (PUSH) {
-            if (out[j] != NONE && acc[j] != INF) atomicMin(lab + (size_t)out[j] * KL, acc[j]);
-        } else {
-            if (j < len && acc[j] != INF) atomicMin(lab + (size_t)(p0 + j) * KL, acc[j]);
-        }
-    }
+    for (int j = 0; j < W; ++j)
+        if (j < len && acc[j] != INF) atomicMin(lab + (size_t)(p0 + j) * KL, acc[j]);
 }
 
 // rows: wavefront w of workgroup (blockIdx.x, blockIdx.y) takes unit
 // u0 + blockIdx.y for wave group 4*blockIdx.x + w < wgs: the four wavefronts
 // of a workgroup read the same weights, so three of them find the rows in
 // the scalar cache.
-//   DOWN (PUSH = false): unit = rows [r0[u], r1[u]) of segment
-//     seg[u] & ~UNIT_WHOLE (DOWN arcs from outside the segment), width
-//     wid[s], weights from woff[s] + (r - rbase[s]) * W, merged into the
+//   DOWN (PUSH = false): unit = rows [r0[u], r1[u]) of one run c of
+//     segment seg[u] & UNIT_SEG_MASK (DOWN arcs from outside the segment;
+//     c = seg[u] >> UNIT_RUN_SHIFT & 7, 8(8 - c) weights a row from woff[u]
+//     on), reduced by the down_unit of that width and merged into the
 //     segment's labels (which hold the upward labels): by plain stores if the
 //     unit is flagged UNIT_WHOLE (it holds all rows of its segment), else by
 //     one atomic min per finite (node, lane).  The intra arcs follow in
@@ -422,41 +441,43 @@ __device__ __forceinline__ void rows_unit(uint32_t ra, uint32_t rb, const uint32
 //     woff[u], merged into targets tgt[64u ..] with atomic min; a wavefront
 //     whose wave group does not reach the segment stops at once.
 constexpr uint32_t UNIT_WHOLE = 0x80000000u;
+constexpr uint32_t UNIT_RUN_SHIFT = 28;  // DOWN: the unit's run c in bits 28..30
+constexpr uint32_t UNIT_SEG_MASK = (1u << UNIT_RUN_SHIFT) - 1;
 
 template <int KL, bool PUSH>
 __global__ __launch_bounds__(TILE_THREADS) void rows_kernel(
     uint32_t u0, uint32_t u1, const uint32_t *__restrict__ seg, const uint32_t *__restrict__ r0,
-    const uint32_t *__restrict__ r1, const uint32_t *__restrict__ rbase, const uint32_t *__restrict__ wid,
+    const uint32_t *__restrict__ r1, const uint32_t *__restrict__ wid,
     const uint32_t *__restrict__ woff, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ tgt,
     const uint32_t *__restrict__ key, const uint32_t *__restrict__ wts, Reach R, uint32_t *__restrict__ arena,
     uint32_t n, uint32_t wgs) {
     const uint32_t u = u0 + blockIdx.y, wg = blockIdx.x * (TILE_THREADS / 64) + wave_id();
     if (u >= u1 || wg >= wgs) return;  // wave-uniform; this kernel has no barrier
     uint32_t *lab = lane_labels<KL>(arena, n, wg, nullptr);
-    const uint32_t sw = seg[u], s = sw & ~UNIT_WHOLE;
+    const uint32_t sw = seg[u], s = sw & UNIT_SEG_MASK;
     const bool reach = wave_reaches(R, wg, seg_pos[s]);
     if (PUSH && !reach) return;  // the segment's labels are all INF: nothing to push
-    const uint32_t *K = key;
-    uint32_t ra, rb, W, p0 = 0, len = 0;
-    const cu32 *wr, *out = nullptr;
     if constexpr (PUSH) {
-        ra = r0[u];
-        rb = r0[u + 1];
-        W = wid[u];
-        wr = as_const(wts) + woff[u];
-        out = as_const(tgt) + (size_t)u * 64;
+        const uint32_t ra = r0[u], rb = r0[u + 1], W = wid[u];
+        const cu32 *wr = as_const(wts) + woff[u], *out = as_const(tgt) + (size_t)u * 64;
+        if (W == 16) push_unit<KL, 16>(ra, rb, key, wr, out, lab);
+        else if (W == 32) push_unit<KL, 32>(ra, rb, key, wr, out, lab);
+        else push_unit<KL, 64>(ra, rb, key, wr, out, lab);
     } else {
-        ra = r0[u];
-        rb = r1[u];
-        W = wid[s];
-        wr = as_const(wts) + woff[s] + (size_t)(ra - rbase[s]) * W;
-        p0 = seg_pos[s];
-        len = seg_pos[s + 1] - p0;
+        const uint32_t ra = r0[u], rb = r1[u], p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
+        const cu32 *wr = as_const(wts) + woff[u];
+        const bool whole = (sw & UNIT_WHOLE) != 0;
+        switch ((sw >> UNIT_RUN_SHIFT) & 7u) {  // the unit's run: width 8(8 - c)
+        case 0: down_unit<KL, 64>(ra, rb, key, wr, p0, len, lab, whole, reach); break;
+        case 1: down_unit<KL, 56>(ra, rb, key, wr, p0, len, lab, whole, reach); break;
+        case 2: down_unit<KL, 48>(ra, rb, key, wr, p0, len, lab, whole, reach); break;
+        case 3: down_unit<KL, 40>(ra, rb, key, wr, p0, len, lab, whole, reach); break;
+        case 4: down_unit<KL, 32>(ra, rb, key, wr, p0, len, lab, whole, reach); break;
+        case 5: down_unit<KL, 24>(ra, rb, key, wr, p0, len, lab, whole, reach); break;
+        case 6: down_unit<KL, 16>(ra, rb, key, wr, p0, len, lab, whole, reach); break;
+        default: down_unit<KL, 8>(ra, rb, key, wr, p0, len, lab, whole, reach); break;
+        }
     }
-    const bool whole = (sw & UNIT_WHOLE) != 0;
-    if (W == 16) rows_unit<KL, 16, PUSH>(ra, rb, K, wr, out, p0, len, lab, whole, reach);
-    else if (W == 32) rows_unit<KL, 32, PUSH>(ra, rb, K, wr, out, p0, len, lab, whole, reach);
-    else rows_unit<KL, 64, PUSH>(ra, rb, K, wr, out, p0, len, lab, whole, reach);
 }
 
 // tri: wavefront w of workgroup (blockIdx.x, blockIdx.y) takes segment
@@ -926,7 +947,7 @@ struct bfg_ctx {
     DevBuf<uint32_t> ch_off, ch_pos, cs_off, cs_src, cs_aoff, ca_w;
     DevBuf<uint16_t> ca_v, in_dd;
     DevBuf<uint32_t> up_off, up_tgt, up_w;
-    DevBuf<uint32_t> xr_off, xr_wid, xw_off, xr_src, xr_w, tb_dn, tb_up, tb_w, pt_blk, pt_tgt, pr_off, pr_wid,
+    DevBuf<uint32_t> xr_off, xr_run, xw_off, xr_src, xr_w, tb_dn, tb_up, tb_w, pt_blk, pt_tgt, pr_off, pr_wid,
         pw_off, pr_x, pr_w;
     DevBuf<uint32_t> pk_seg;
     // DOWN work plan for the current number of wave groups per batch: units
@@ -934,7 +955,7 @@ struct bfg_ctx {
     uint32_t plan_wgs = 0;
     std::vector<uint32_t> plan_lvl;
     std::vector<uint64_t> lvl_dom_bytes;  // algorithmic bytes of a level's DOWN rows per wave group
-    DevBuf<uint32_t> du_seg, du_r0, du_r1;
+    DevBuf<uint32_t> du_seg, du_r0, du_r1, du_w;
     DevBuf<uint8_t> seg_split;  // 1 where the plan merges a segment's rows with atomics
     // arenas (per batch)
     DevBuf<uint32_t> arena, pathbuf;
@@ -1031,7 +1052,8 @@ int plan_down(bfg_ctx *c, uint32_t wgs) {
     const bfg::Layout &L = c->L;
     const uint32_t nsl = (uint32_t)L.seglvl_seg.size() - 1;
     const uint64_t rmax = plan_max_rows();
-    std::vector<uint32_t> us, u0, u1;
+    if (L.seg_pos.size() - 1 > UNIT_SEG_MASK) return fail(BFG_NOT_SUPPORTED, "too many segments for the DOWN plan");
+    std::vector<uint32_t> us, u0, u1, uw;
     std::vector<uint8_t> split(L.seg_pos.size() - 1, 0);
     c->plan_lvl.assign(1, 0);
     uint64_t covered = 0;
@@ -1042,27 +1064,46 @@ int plan_down(bfg_ctx *c, uint32_t wgs) {
             rmax, std::max<uint64_t>(PLAN_MIN_ROWS, (rows * wgs + PLAN_WAVES - 1) / PLAN_WAVES));
         for (uint32_t sg = s0; sg < s1; ++sg) {
             const uint32_t a = L.xr_off[sg], b = L.xr_off[sg + 1];
-            if (b - a <= R) {
-                us.push_back(sg | UNIT_WHOLE);
+            const uint32_t *run = &L.xr_run[8 * (size_t)sg];
+            uint32_t nrun = 0;
+            for (uint32_t c = 0, st = 0; c < 8; st = run[c++]) nrun += run[c] > st;
+            uint64_t w = L.xw_off[sg];
+            if (nrun == 0) {  // no rows: a whole unit still gives unreached wave groups their INF labels
+                us.push_back(sg | 7u << UNIT_RUN_SHIFT | UNIT_WHOLE);
                 u0.push_back(a);
-                u1.push_back(b);
-            } else {
-                for (uint64_t r = a; r < b; r += R) {
-                    us.push_back(sg);
-                    u0.push_back((uint32_t)r);
-                    u1.push_back((uint32_t)std::min<uint64_t>(r + R, b));
-                }
-                split[sg] = 1;
+                u1.push_back(a);
+                uw.push_back((uint32_t)w);
             }
-            covered += b - a;
+            for (uint32_t c = 0, st = 0; c < 8; st = run[c++]) {
+                const uint32_t W = 8 * (8 - c), cs = sg | c << UNIT_RUN_SHIFT;
+                if (run[c] == st) continue;  // an empty run (the increment still advances st)
+                if (nrun == 1 && run[c] - st <= R) {  // the whole segment in one unit
+                    us.push_back(cs | UNIT_WHOLE);
+                    u0.push_back(a + st);
+                    u1.push_back(a + run[c]);
+                    uw.push_back((uint32_t)w);
+                } else {
+                    for (uint64_t r = st; r < run[c]; r += R) {
+                        us.push_back(cs);
+                        u0.push_back((uint32_t)(a + r));
+                        u1.push_back((uint32_t)(a + std::min<uint64_t>(r + R, run[c])));
+                        uw.push_back((uint32_t)(w + (r - st) * W));
+                    }
+                    split[sg] = 1;
+                }
+                w += (uint64_t)(run[c] - st) * W;
+            }
+            if (w != L.xw_off[sg + 1] || a + run[7] != b) return fail(BFG_INTERNAL_ERROR, "row runs disagree");
         }
         c->plan_lvl.push_back((uint32_t)us.size());
     }
+    for (size_t k = 0; k < us.size(); ++k) covered += u1[k] - u0[k];
     if (covered != L.xr_off.back()) return fail(BFG_INTERNAL_ERROR, "DOWN plan does not cover every row");
     hipStream_t s = c->stream;
     HIP_TRY(c->du_seg.upload(us, s));
     HIP_TRY(c->du_r0.upload(u0, s));
     HIP_TRY(c->du_r1.upload(u1, s));
+    HIP_TRY(c->du_w.upload(uw, s));
     HIP_TRY(c->seg_split.upload(split, s));
     HIP_TRY(hipStreamSynchronize(s));  // the host vectors die with this frame
     c->plan_wgs = wgs;
@@ -1111,7 +1152,7 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
         for (uint32_t b = k0; b < k1; b += MAX_GRID_Y, ++nlaunch)
             hipLaunchKernelGGL((rows_kernel<KL, true>), dim3(wgx, std::min(k1 - b, MAX_GRID_Y)), dim3(TILE_THREADS),
                                0, s, b, std::min(k1, b + MAX_GRID_Y), c->pk_seg.p, c->pr_off.p, c->pr_off.p,
-                               c->pr_off.p, c->pr_wid.p, c->pw_off.p, c->seg_pos.p, c->pt_tgt.p, c->pr_x.p, c->pr_w.p,
+                               c->pr_wid.p, c->pw_off.p, c->seg_pos.p, c->pt_tgt.p, c->pr_x.p, c->pr_w.p,
                                R, c->arena.p, n, wgs);
     }
     HIP_TRY(hipGetLastError());
@@ -1130,7 +1171,7 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
             for (uint32_t b = u0; b < u1; b += MAX_GRID_Y, ++nlaunch)
                 hipLaunchKernelGGL((rows_kernel<KL, false>), dim3(wgx, std::min(u1 - b, MAX_GRID_Y)),
                                    dim3(TILE_THREADS), 0, s, b, std::min(u1, b + MAX_GRID_Y), c->du_seg.p, c->du_r0.p,
-                                   c->du_r1.p, c->xr_off.p, c->xr_wid.p, c->xw_off.p, c->seg_pos.p, c->pt_tgt.p,
+                                   c->du_r1.p, nullptr, c->du_w.p, c->seg_pos.p, c->pt_tgt.p,
                                    c->xr_src.p, c->xr_w.p, R, c->arena.p, n, wgs);
             HIP_TRY(hipEventRecord(c->dom_ev[2 * ndom + 1], s));
             ++ndom;
@@ -1500,7 +1541,7 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     HIP_TRY(c->up_tgt.upload(L.up_tgt, s));
     HIP_TRY(c->up_w.upload(L.up_w, s));
     HIP_TRY(c->xr_off.upload(L.xr_off, s));
-    HIP_TRY(c->xr_wid.upload(L.xr_wid, s));
+    HIP_TRY(c->xr_run.upload(L.xr_run, s));
     HIP_TRY(c->xw_off.upload(L.xw_off, s));
     HIP_TRY(c->pr_wid.upload(L.pr_wid, s));
     HIP_TRY(c->pw_off.upload(L.pw_off, s));
@@ -1520,7 +1561,7 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     c->graph_bytes = 0;
     for (auto *b : {&c->parent, &c->pos_parent, &c->pos_pre, &c->pos_sub, &c->pos_depth, &c->seg_pos, &c->in_off,
                     &c->in_split, &c->in_src, &c->in_w, &c->ch_off, &c->ch_pos, &c->cs_off, &c->cs_src, &c->cs_aoff,
-                    &c->ca_w, &c->up_off, &c->up_tgt, &c->up_w, &c->xr_off, &c->xr_wid, &c->xw_off, &c->pr_wid,
+                    &c->ca_w, &c->up_off, &c->up_tgt, &c->up_w, &c->xr_off, &c->xr_run, &c->xw_off, &c->pr_wid,
                     &c->pw_off, &c->xr_src, &c->xr_w, &c->tb_dn,
                     &c->tb_up, &c->tb_w, &c->pt_blk, &c->pt_tgt, &c->pr_off, &c->pr_x, &c->pr_w, &c->pk_seg})
         c->graph_bytes += b->n * sizeof(uint32_t);

@@ -76,6 +76,44 @@ uint32_t build_rows(std::vector<Arc3> &arcs, uint32_t W, std::vector<uint32_t> &
     return rows;
 }
 
+// Appends the DOWN rows of one segment from arcs (source, column, weight):
+// one row per distinct source holding its arcs to the first 8k columns
+// (k octets: the fewest that cover its last arc, or the widest row's of a
+// segment of at most 64 rows; INF where there is none, parallel arcs keep
+// the minimum), rows ordered by k descending and source ascending.
+// run[c] = rows with at least 8 - c octets.
+void build_runs(std::vector<Arc3> &arcs, std::vector<uint32_t> &rkey, std::vector<uint32_t> &rw, uint32_t run[8]) {
+    std::sort(arcs.begin(), arcs.end(), [](const Arc3 &a, const Arc3 &b) {
+        return a.key != b.key ? a.key < b.key : a.col < b.col;
+    });
+    std::vector<std::pair<uint32_t, size_t>> rows;  // (octets, first arc)
+    for (size_t i = 0; i < arcs.size();) {
+        size_t e = i;
+        uint32_t last = 0;
+        for (; e < arcs.size() && arcs[e].key == arcs[i].key; ++e) last = std::max(last, arcs[e].col);
+        rows.emplace_back(last / 8 + 1, i);
+        i = e;
+    }
+    // a segment of at most 64 rows is one unit of the DOWN sweep whatever its
+    // row widths: its rows share the widest width, so it stays one run
+    if (rows.size() <= 64) {
+        uint32_t kmax = 0;
+        for (const auto &r : rows) kmax = std::max(kmax, r.first);
+        for (auto &r : rows) r.first = kmax;
+    }
+    std::stable_sort(rows.begin(), rows.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+    std::fill(run, run + 8, 0u);
+    for (const auto &[k, i] : rows) {
+        const uint32_t key = arcs[i].key, W = 8 * k;
+        const size_t base = rw.size();
+        rw.resize(base + W, INF);
+        for (size_t a = i; a < arcs.size() && arcs[a].key == key; ++a)
+            rw[base + arcs[a].col] = std::min(rw[base + arcs[a].col], arcs[a].w);
+        rkey.push_back(key);
+        for (uint32_t c = 8 - k; c < 8; ++c) ++run[c];
+    }
+}
+
 template <class T>
 void prefix(std::vector<T> &v) {  // exclusive prefix sum in place
     T run = 0;
@@ -470,7 +508,8 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
         const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
         if (L.max_segment > 64) { err = "segment_limit above 64"; return false; }
         struct Tiles {
-            std::vector<uint32_t> xr_src, xr_w;                 // DOWN rows (width_class(len) wide)
+            std::vector<uint32_t> xr_src, xr_w;                 // DOWN rows, runs by width
+            uint32_t run[8];
             std::vector<std::pair<uint32_t, uint32_t>> blocks;  // (tab 0/1 * 64 + 8a + o, local block)
             std::vector<uint32_t> tb_w;                         // 64 weights per local block
             std::vector<uint32_t> pt_tgt, pr_cnt, pr_wid, pr_x, pr_w;  // UP push: targets, rows and width per block
@@ -487,7 +526,7 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
                 for (uint32_t j = 0; j < len; ++j)
                     for (uint32_t k = L.in_off[p0 + j]; k < L.in_split[p0 + j]; ++k)
                         a3.push_back({L.in_src[k], j, L.in_w[k]});
-                build_rows(a3, width_class(len), t.xr_src, t.xr_w);
+                build_runs(a3, t.xr_src, t.xr_w, t.run);
                 uint32_t tab[2][64];
                 std::fill(&tab[0][0], &tab[0][0] + 128, NONE);
                 auto put = [&](int d, uint32_t a, uint32_t o, uint32_t i, uint32_t j, uint32_t w) {
@@ -538,7 +577,7 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
             }
         });
         L.xr_off.assign(nseg + 1, 0);
-        L.xr_wid.assign(nseg, 0);
+        L.xr_run.assign(8 * (size_t)nseg, 0);
         L.xw_off.assign(nseg + 1, 0);
         L.pr_wid.clear();
         L.pw_off.clear();
@@ -561,7 +600,7 @@ bool build_layout(const CchInput &in, const LayoutParams &lp, Layout &L, std::st
         for (uint32_t s = 0; s < nseg; ++s) {
             Tiles &t = tl[s];
             L.xr_off[s] = (uint32_t)L.xr_src.size();
-            L.xr_wid[s] = width_class(L.seg_pos[s + 1] - L.seg_pos[s]);
+            std::copy(t.run, t.run + 8, L.xr_run.begin() + 8 * (size_t)s);
             L.xw_off[s] = (uint32_t)L.xr_w.size();
             L.xr_src.insert(L.xr_src.end(), t.xr_src.begin(), t.xr_src.end());
             L.xr_w.insert(L.xr_w.end(), t.xr_w.begin(), t.xr_w.end());
@@ -717,7 +756,8 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
         const size_t nblk = L.pt_blk.empty() ? 0 : L.pt_blk.back();
         if (L.xr_off.size() != (size_t)nseg + 1 || L.pt_blk.size() != nseg + 1 ||
             L.tb_dn.size() != 64 * (size_t)nseg || L.tb_up.size() != 64 * (size_t)nseg ||
-            L.xr_wid.size() != nseg || L.xw_off.size() != (size_t)nseg + 1 || L.xr_off[nseg] != L.xr_src.size() ||
+            L.xr_run.size() != 8 * (size_t)nseg || L.xw_off.size() != (size_t)nseg + 1 ||
+            L.xr_off[nseg] != L.xr_src.size() ||
             L.xw_off[nseg] != L.xr_w.size() || L.pr_off.size() != nblk + 1 || L.pw_off.size() != nblk + 1 ||
             L.pr_wid.size() != nblk || L.pr_off.back() != L.pr_x.size() || L.pw_off.back() != L.pr_w.size() ||
             L.pt_tgt.size() != 64 * nblk) {
@@ -725,10 +765,14 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
             return false;
         }
         for (uint32_t s = 0; s < nseg; ++s) {
-            const uint32_t W = L.xr_wid[s], len = L.seg_pos[s + 1] - L.seg_pos[s];
-            if ((W != 16 && W != 32 && W != 64) || len > W ||
-                (uint64_t)L.xw_off[s + 1] - L.xw_off[s] != (uint64_t)W * (L.xr_off[s + 1] - L.xr_off[s])) {
-                err = "bad row width of a segment";
+            const uint32_t *run = &L.xr_run[8 * (size_t)s];
+            uint64_t weights = 0;
+            for (uint32_t c = 0, st = 0; c < 8; st = run[c++]) {
+                if (run[c] < st) { err = "row runs of a segment out of order"; return false; }
+                weights += (uint64_t)(run[c] - st) * 8 * (8 - c);
+            }
+            if (run[7] != L.xr_off[s + 1] - L.xr_off[s] || (uint64_t)L.xw_off[s + 1] - L.xw_off[s] != weights) {
+                err = "row runs disagree with a segment's rows";
                 return false;
             }
         }
@@ -753,16 +797,20 @@ bool check_layout(const CchInput &in, const Layout &L, std::string &err) {
                 for (uint32_t i = 0; i < len; ++i)
                     for (uint32_t k = L.in_off[p0 + i]; k < L.in_split[p0 + i]; ++k)
                         want.emplace_back(i, L.in_src[k], L.in_w[k]);
-                const uint32_t W = L.xr_wid[s];
+                size_t wo = L.xw_off[s];
                 for (uint32_t r = L.xr_off[s]; r < L.xr_off[s + 1]; ++r) {
-                    const uint32_t u = L.xr_src[r];
+                    const uint32_t u = L.xr_src[r], i = r - L.xr_off[s];
+                    uint32_t c = 0;
+                    while (i >= L.xr_run[8 * (size_t)s + c]) ++c;  // i < run[7]: ends
+                    const uint32_t W = 8 * (8 - c);
                     if (u >= n || stage[u] >= stage[p0]) { flag("tile source not final"); return; }
                     for (uint32_t j = 0; j < W; ++j) {
-                        const uint32_t w = L.xr_w[L.xw_off[s] + (size_t)(r - L.xr_off[s]) * W + j];
+                        const uint32_t w = L.xr_w[wo + j];
                         if (w == INF) continue;
                         if (j >= len) { flag("tile head beyond the segment"); return; }
                         got.emplace_back(j, u, w);
                     }
+                    wo += W;
                 }
                 merged(want);
                 std::sort(got.begin(), got.end());

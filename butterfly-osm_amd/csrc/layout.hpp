@@ -86,12 +86,15 @@ struct Layout {
     std::vector<uint16_t> in_dd;
     // Dense tiles of the top region (segments of at most 64 nodes; a thread of
     // the segment kernels holds one lane's labels of a whole segment).  A row
-    // is a key (a position) and W weights, the key's arcs to the W columns of
-    // a block (INF where there is none); W (16, 32 or 64) is the smallest
-    // width that covers the block.
+    // is a key (a position) and W weights, the key's arcs to the first W
+    // columns of a block (INF where there is none).
     //   DOWN arcs entering segment s from outside: rows [xr_off[s],
-    //   xr_off[s+1]) of width xr_wid[s], key xr_src[r] (source), weights
-    //   xr_w[xw_off[s] + (r - xr_off[s]) * W ..] to the segment's nodes.
+    //   xr_off[s+1]), key xr_src[r] (source), ordered by width: row i of the
+    //   segment (i = r - xr_off[s]) is in run c (0..7) for xr_run[8s + c - 1]
+    //   <= i < xr_run[8s + c] (xr_run[8s - 1] taken as 0) and holds W = 8(8 - c)
+    //   weights, the smallest multiple of 8 that covers its last arc; a
+    //   segment's rows are stored one after the other from xw_off[s], in key
+    //   order within a run.
     //   Intra-segment arcs: 8x8 blocks W[i][j] (tail i of octet a, head j of
     //   octet o) at tb_w[64k ..], k = tb_dn[64s + 8a + o] (a <= o) or
     //   tb_up[64s + 8a + o] (a >= o), NONE where the block has no arc.
@@ -100,7 +103,7 @@ struct Layout {
     //   rows [pr_off[k], pr_off[k+1]) of width pr_wid[k], key pr_x[r] (tail
     //   position), weights pr_w[pw_off[k] + (r - pr_off[k]) * W ..] to the
     //   block's targets.
-    std::vector<uint32_t> xr_off, xr_wid, xw_off, xr_src, xr_w;
+    std::vector<uint32_t> xr_off, xr_run, xw_off, xr_src, xr_w;
     std::vector<uint32_t> tb_dn, tb_up, tb_w;
     std::vector<uint32_t> pt_blk, pt_tgt, pr_off, pr_wid, pw_off, pr_x, pr_w;
     // pk_seg[k] is the segment of target block k.
