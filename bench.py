@@ -195,7 +195,7 @@ def main():
         log(f"warmup {i}: up {st.upward_time_ms:.1f} ms sweep {st.downward_time_ms:.1f} ms "
             f"out {st.output_time_ms:.1f} ms batches {st.batches}")
     sync()
-    sweep_ms = up_ms = out_ms = dom_ms = 0.0
+    sweep_ms = up_ms = out_ms = dom_ms = engine_ms = 0.0
     launches = batches = dom_launches = dom_bytes = 0
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -203,6 +203,7 @@ def main():
         sweep_ms += st.downward_time_ms
         up_ms += st.upward_time_ms
         out_ms += st.output_time_ms
+        engine_ms += st.total_time_ms
         launches += st.sweep_launches
         batches += st.batches
         dom_ms += st.dom_ms
@@ -264,7 +265,9 @@ def main():
                                "launches_per_step": launches // args.steps,
                                "ms_per_step": sweep_ms / args.steps, "achieved_gbps": sweep_gbps}},
         "phases_ms_per_step": {"up": up_ms / args.steps, "sweep": sweep_ms / args.steps,
-                               "gather": out_ms / args.steps},
+                               "gather": out_ms / args.steps,
+                               "batches_wall": engine_ms / args.steps,
+                               "host_staging": ms_per_step - engine_ms / args.steps},
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": mt,
         "bit_exact_sample": exact,

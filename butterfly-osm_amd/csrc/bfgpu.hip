@@ -969,6 +969,12 @@ struct bfg_ctx {
     bool paths_on = false;  // while on, labels are compact (see label_stride)
     uint32_t n_paths = 0;
     uint64_t path_nodes = 0, path_arcs = 0, top_in_arcs = 0;
+    // bdep[p]: levels between bottom position p and the top region (0 for a
+    // bottom root); host copies of the staged path tables, kept for their
+    // asynchronous uploads; the last element of each path (its target) and
+    // whether path_pos of this call is on the device yet
+    std::vector<uint32_t> bdep, path_off_h, path_pos_h, tgt_pos_h, path_leaf;
+    bool path_pos_staged = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> dom_ev;     // event pairs around each DOWN row launch of a batch
 
@@ -1110,6 +1116,8 @@ int plan_down(bfg_ctx *c, uint32_t wgs) {
     return BFG_SUCCESS;
 }
 
+int stage_path_elements(bfg_ctx *c);
+
 struct PhaseTimes {
     float up_ms = 0, sweep_ms = 0, dom_ms = 0;
     uint32_t launches = 0, dom_launches = 0;
@@ -1186,6 +1194,8 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
     if (c->paths_on) {
         constexpr int NSP = NT / KL;
         if (c->n_paths) {
+            rc = stage_path_elements(c);
+            if (rc) return rc;
             hipLaunchKernelGGL(bottom_path_kernel<KL>, dim3((c->n_paths + NSP - 1) / NSP * groups), dim3(NT), 0, s,
                                c->n_paths, c->path_off.p, c->path_pos.p, L.n_top, c->in_off.p, c->in_split.p,
                                c->in_dd.p, c->in_src.p, c->in_w.p, c->pos_pre.p, c->pos_sub.p, c->pos_depth.p,
@@ -1365,9 +1375,11 @@ Req single_source_req(const uint32_t *src, uint32_t k, uint32_t threshold) {
 
 // Bottom paths of a matrix's targets: for every bottom target that is not an
 // ancestor of another target, its bottom ancestors top-down (the top-region
-// boundary first, the target last).  Also stages the targets' label indices
-// (tgt_pos): top targets by position, bottom targets by their element on a
-// path (compact labels, see label_stride).
+// boundary first, the target last).  Stages the path offsets (a path's
+// length is its target's bdep + 1) and the targets' label indices (tgt_pos):
+// top targets by position, bottom targets by their element on a path
+// (compact labels, see label_stride).  The elements themselves are written
+// by stage_path_elements once the top-region kernels are queued.
 int stage_target_paths(bfg_ctx *c, const uint32_t *targets, uint32_t T) {
     const bfg::Layout &L = c->L;
     std::vector<uint32_t> bt;
@@ -1377,9 +1389,10 @@ int stage_target_paths(bfg_ctx *c, const uint32_t *targets, uint32_t T) {
     }
     std::sort(bt.begin(), bt.end(), [&](uint32_t a, uint32_t b) { return L.pos_pre[a] < L.pos_pre[b]; });
     bt.erase(std::unique(bt.begin(), bt.end()), bt.end());
-    std::vector<uint32_t> off{0}, pos, up, pending;
+    std::vector<uint32_t> &off = c->path_off_h, &leaf = c->path_leaf, pending;
+    off.assign(1, 0);
+    leaf.clear();
     std::vector<std::pair<uint32_t, uint32_t>> elem;  // (bottom target position, path element)
-    uint64_t arcs = 0;
     for (size_t i = 0; i < bt.size(); ++i) {
         const uint32_t t = bt[i];
         pending.push_back(t);
@@ -1387,19 +1400,15 @@ int stage_target_paths(bfg_ctx *c, const uint32_t *targets, uint32_t T) {
         // ancestor of some later target: t is then on that target's path
         // (and so are the pending targets before it, its ancestors)
         if (i + 1 < bt.size() && L.pos_pre[bt[i + 1]] - L.pos_pre[t] < L.pos_subtree[t]) continue;
-        up.clear();
-        for (uint32_t x = t; x != NONE && x >= L.n_top; x = L.pos_parent[x]) {
-            up.push_back(x);
-            arcs += L.in_off[x + 1] - L.in_off[x];
-        }
-        const uint32_t e0 = (uint32_t)pos.size(), d0 = L.pos_depth[up.back()];
-        pos.insert(pos.end(), up.rbegin(), up.rend());
-        off.push_back((uint32_t)pos.size());
+        const uint32_t e0 = off.back(), d0 = L.pos_depth[t] - c->bdep[t];  // depth of the path's first element
+        off.push_back(e0 + c->bdep[t] + 1);
+        leaf.push_back(t);
         for (uint32_t q : pending) elem.emplace_back(q, e0 + L.pos_depth[q] - d0);
         pending.clear();
     }
     std::sort(elem.begin(), elem.end());
-    std::vector<uint32_t> tp(T);
+    std::vector<uint32_t> &tp = c->tgt_pos_h;
+    tp.resize(T);
     for (uint32_t t = 0; t < T; ++t) {
         const uint32_t p = L.pos_of_rank[targets[t]];
         if (p < L.n_top) {
@@ -1411,19 +1420,45 @@ int stage_target_paths(bfg_ctx *c, const uint32_t *targets, uint32_t T) {
         }
     }
     if (c->path_off.n < off.size()) HIP_TRY(c->path_off.alloc(off.size()));
-    if (c->path_pos.n < pos.size()) HIP_TRY(c->path_pos.alloc(pos.size()));
+    if (c->path_pos.n < off.back()) HIP_TRY(c->path_pos.alloc(off.back()));
     if (T > c->tgt_cap) {
         HIP_TRY(c->tgt_pos.alloc(T));
         c->tgt_cap = T;
     }
+    // the host tables live in the context until the next call, which starts
+    // after this one has synchronised
     HIP_TRY(hipMemcpyAsync(c->path_off.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->tgt_pos.p, tp.data(), (size_t)T * 4, hipMemcpyHostToDevice, c->stream));
+    c->n_paths = (uint32_t)leaf.size();
+    c->path_nodes = off.back();
+    c->path_arcs = 0;
+    c->path_pos_staged = false;
+    return BFG_SUCCESS;
+}
+
+// The elements of the staged paths (a walk up from each path's target) and
+// their DOWN in-arc count, uploaded on the stream.  Called with the
+// top-region kernels already queued, so the walk overlaps them.
+int stage_path_elements(bfg_ctx *c) {
+    if (c->path_pos_staged) return BFG_SUCCESS;
+    const bfg::Layout &L = c->L;
+    std::vector<uint32_t> &pos = c->path_pos_h;
+    pos.resize(c->path_nodes);
+    uint64_t arcs = 0;
+    for (uint32_t k = 0; k < c->n_paths; ++k) {
+        uint32_t x = c->path_leaf[k];
+        for (uint32_t i = c->path_off_h[k + 1]; i-- > c->path_off_h[k];) {
+            if (x == NONE || x < L.n_top) return fail(BFG_INTERNAL_ERROR, "bottom path shorter than its depth");
+            pos[i] = x;
+            arcs += L.in_off[x + 1] - L.in_off[x];
+            x = L.pos_parent[x];
+        }
+        if (x != NONE && x >= L.n_top) return fail(BFG_INTERNAL_ERROR, "bottom path longer than its depth");
+    }
     if (!pos.empty())
         HIP_TRY(hipMemcpyAsync(c->path_pos.p, pos.data(), pos.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->tgt_pos.p, tp.data(), (size_t)T * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));  // off, pos and tp die with this frame
-    c->n_paths = (uint32_t)(off.size() - 1);
-    c->path_nodes = pos.size();
     c->path_arcs = arcs;
+    c->path_pos_staged = true;
     return BFG_SUCCESS;
 }
 
@@ -1520,6 +1555,12 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     HIP_TRY(c->seg_pos.upload(L.seg_pos, s));
     HIP_TRY(c->in_off.upload(L.in_off, s));
     c->top_in_arcs = L.in_off[L.n_top];
+    // ranks descend from parents to children (a parent's rank is higher)
+    c->bdep.assign(L.n, 0);
+    for (uint32_t r = L.n; r-- > 0;) {
+        const uint32_t p = L.pos_of_rank[r], q = L.pos_parent[p];
+        if (p >= L.n_top && q != NONE && q >= L.n_top) c->bdep[p] = c->bdep[q] + 1;
+    }
     for (size_t l = 0; l + 1 < L.seglvl_seg.size(); ++l) {
         const uint32_t p0 = L.seg_pos[L.seglvl_seg[l]], p1 = L.seg_pos[L.seglvl_seg[l + 1]];
         uint64_t m = 0;
