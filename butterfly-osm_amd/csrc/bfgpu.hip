@@ -732,7 +732,7 @@ __global__ __launch_bounds__(NT) void bottom_path_kernel(
     const uint32_t *__restrict__ seed_depth, const uint32_t *__restrict__ pathbuf, uint32_t D1, uint32_t n_lanes,
     uint32_t *__restrict__ arena, uint32_t n) {
     constexpr int NS = NT / KL;
-    const uint32_t lane = threadIdx.x % KL, slot = threadIdx.x / KL;
+    const uint32_t lane = threadIdx.x % KL, slot = slot_id<KL>();
     // block b: group b % groups, paths (b / groups) * NS + slot, so the
     // groups of one path run side by side and share its arcs in L2
     const uint32_t groups = (n_lanes + KL - 1) / KL;
@@ -1039,7 +1039,7 @@ uint32_t auto_lane_cap(bfg_ctx *c, uint32_t want) {
 // most rows one unit takes (the longest unit bounds its level's time).
 constexpr uint64_t PLAN_WAVES = 8192;
 constexpr uint32_t PLAN_MIN_ROWS = 64;
-constexpr uint32_t PLAN_MAX_ROWS = 128;
+constexpr uint32_t PLAN_MAX_ROWS = 256;
 
 uint32_t plan_max_rows() {
     if (const char *e = std::getenv("BFG_PLAN_MAX_ROWS")) {  // tuning override
