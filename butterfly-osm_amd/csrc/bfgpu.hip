@@ -388,6 +388,39 @@ __device__ __forceinline__ void tri_apply(uint32_t (&acc)[8 * NO], const uint32_
 // one return-less atomic min per finite (position, lane).  A DOWN unit that
 // holds all rows of its segment (whole) and whose wave group does not reach
 // the segment stores instead: those upward labels are INF and never written.
+// tri_apply for levels of many segments, where enough wavefronts are in
+// flight to hide load latency and VALU issue sets the time: the block ids
+// come in with one vector load, each block's 64 weights through scalar
+// loads (shared in the scalar cache by the workgroup's four wavefronts) and
+// reach the adds as scalar operands, with no v_readlane.  Scheduling
+// barriers keep half a block's weights in SGPRs at a time.
+template <bool DOWN, int NO>
+__device__ __forceinline__ void tri_apply_s(uint32_t (&acc)[8 * NO], const uint32_t *T, const cu32 *tw) {
+    const uint32_t kv = T[__lane_id()];
+#pragma unroll
+    for (int aa = 0; aa < NO; ++aa) {
+        const int a = DOWN ? aa : NO - 1 - aa;
+#pragma unroll
+        for (int oo = 0; oo < NO; ++oo) {
+            const int o = DOWN ? a + oo : a - oo;
+            if (o < 0 || o >= NO) continue;
+            const uint32_t k = rl(kv, a * 8 + o);
+            if (k == NONE) continue;
+            const cu32 *W = tw + (size_t)k * 64;
+#pragma unroll
+            for (int ii = 0; ii < 8; ++ii) {
+                const int i = DOWN ? ii : 7 - ii;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (o == a && (DOWN ? j <= i : j >= i)) continue;
+                    acc[o * 8 + j] = min(acc[o * 8 + j], sat_add(acc[a * 8 + i], W[i * 8 + j]));
+                }
+                if (ii % 4 == 3) __builtin_amdgcn_sched_barrier(0);  // half a block's weights at a time
+            }
+        }
+    }
+}
+
 template <int KL, int W>
 __device__ __forceinline__ void push_unit(uint32_t ra, uint32_t rb, const uint32_t *K, const cu32 *wr, const cu32 *out,
                                           uint32_t *lab) {
@@ -485,8 +518,9 @@ __global__ __launch_bounds__(TILE_THREADS) void rows_kernel(
 // 4*blockIdx.x + w < wgs (the workgroup shares the segment's blocks), loads its
 // labels, relaxes its intra-segment arcs (tri_apply) and stores them.  UP: a
 // wave group that does not reach the segment skips it, only writing INF
-// where the DOWN plan merges into the labels with atomics (split[s]).
-template <int KL, bool DOWN>
+// where the DOWN plan merges into the labels with atomics (split[s]).  SC:
+// scalar-loaded blocks (tri_apply_s) for levels of many segments.
+template <int KL, bool DOWN, bool SC>
 __global__ __launch_bounds__(TILE_THREADS) void tri_kernel(uint32_t first, uint32_t count,
                                                            const uint32_t *__restrict__ seg_pos,
                                                            const uint32_t *__restrict__ tab,
@@ -509,7 +543,8 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_kernel(uint32_t first, uint3
     uint32_t acc[64];
 #pragma unroll
     for (int j = 0; j < 64; ++j) acc[j] = j < len ? lab[(size_t)(p0 + j) * KL] : INF;
-    tri_apply<DOWN, 8>(acc, tab + (size_t)s * 64, tw);
+    if constexpr (SC) tri_apply_s<DOWN, 8>(acc, tab + (size_t)s * 64, as_const(tw));
+    else tri_apply<DOWN, 8>(acc, tab + (size_t)s * 64, tw);
 #pragma unroll
     for (int j = 0; j < 64; ++j)
         if (j < len) lab[(size_t)(p0 + j) * KL] = acc[j];
@@ -1049,6 +1084,21 @@ uint32_t plan_max_rows() {
     return PLAN_MAX_ROWS;
 }
 
+// DOWN levels of more than this many segments relax their intra-segment
+// blocks with scalar-loaded weights (tri_apply_s); fewer segments leave too
+// few wavefronts to hide the loads, and the vector-loaded form waits only
+// once.  The upward phase keeps the vector form: few wave groups reach a
+// wide level's segments, so its levels are short of wavefronts too.
+constexpr uint32_t TRI_SCALAR_SEGS = 64;
+
+uint32_t tri_scalar_segs() {
+    if (const char *e = std::getenv("BFG_TRI_SCALAR_SEGS")) {  // tuning override
+        const long v = std::strtol(e, nullptr, 10);
+        if (v >= 0 && v < (1L << 30)) return (uint32_t)v;
+    }
+    return TRI_SCALAR_SEGS;
+}
+
 // DOWN work plan for batches of `wgs` wave groups.  Per level, a row budget
 // R = rows of the level * wgs / PLAN_WAVES, clamped to [PLAN_MIN_ROWS,
 // PLAN_MAX_ROWS]: a segment with at most R rows is one whole unit; a longer
@@ -1140,6 +1190,7 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
     const Reach R{c->seed_off.p, c->seed_pre.p, c->pos_pre.p, c->pos_sub.p, nl};
     constexpr uint32_t WPB = TILE_THREADS / 64;  // wave groups per workgroup
     const uint32_t wgx = (wgs + WPB - 1) / WPB;  // workgroups across the wave groups
+    const uint32_t tss = tri_scalar_segs();
     if (nseg) {
         hipLaunchKernelGGL(fill_reach_kernel<KL>, dim3(wgs, (nseg + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, nseg,
                            c->seg_pos.p, R, c->arena.p, n);
@@ -1153,9 +1204,9 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
     for (int l = (int)nsl - 1; l >= 0; --l) {
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
         for (uint32_t b = s0; b < s1; b += MAX_GRID_Y, ++nlaunch)
-            hipLaunchKernelGGL((tri_kernel<KL, false>), dim3(wgx, std::min(s1 - b, MAX_GRID_Y)), dim3(TILE_THREADS),
-                               0, s, b, std::min(s1 - b, MAX_GRID_Y), c->seg_pos.p, c->tb_up.p, c->tb_w.p,
-                               c->seg_split.p, R, c->arena.p, n, wgs);
+            hipLaunchKernelGGL((tri_kernel<KL, false, false>), dim3(wgx, std::min(s1 - b, MAX_GRID_Y)),
+                               dim3(TILE_THREADS), 0, s, b, std::min(s1 - b, MAX_GRID_Y), c->seg_pos.p, c->tb_up.p,
+                               c->tb_w.p, c->seg_split.p, R, c->arena.p, n, wgs);
         const uint32_t k0 = L.pt_blk[s0], k1 = L.pt_blk[s1];
         for (uint32_t b = k0; b < k1; b += MAX_GRID_Y, ++nlaunch)
             hipLaunchKernelGGL((rows_kernel<KL, true>), dim3(wgx, std::min(k1 - b, MAX_GRID_Y)), dim3(TILE_THREADS),
@@ -1187,7 +1238,12 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
         }
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
         for (uint32_t b = s0; b < s1; b += MAX_GRID_Y, ++nlaunch)
-            hipLaunchKernelGGL((tri_kernel<KL, true>), dim3(wgx, std::min(s1 - b, MAX_GRID_Y)), dim3(TILE_THREADS), 0,
+            if (s1 - s0 > tss)
+                hipLaunchKernelGGL((tri_kernel<KL, true, true>), dim3(wgx, std::min(s1 - b, MAX_GRID_Y)), dim3(TILE_THREADS), 0,
+                               s, b, std::min(s1 - b, MAX_GRID_Y), c->seg_pos.p, c->tb_dn.p, c->tb_w.p,
+                               c->seg_split.p, R, c->arena.p, n, wgs);
+            else
+                hipLaunchKernelGGL((tri_kernel<KL, true, false>), dim3(wgx, std::min(s1 - b, MAX_GRID_Y)), dim3(TILE_THREADS), 0,
                                s, b, std::min(s1 - b, MAX_GRID_Y), c->seg_pos.p, c->tb_dn.p, c->tb_w.p,
                                c->seg_split.p, R, c->arena.p, n, wgs);
     }

@@ -130,6 +130,26 @@ def test_matrix_long_paths(engine_cls, grid_config1, monkeypatch, lanes):
     assert np.array_equal(got, Oracle(g).compute_matrix_flat_mt(s, t))
 
 
+@pytest.mark.parametrize("lanes", [64, 16])
+def test_matrix_scalar_intra_blocks(engine_cls, grid_config1, monkeypatch, lanes):
+    """Every DOWN level relaxes its intra-segment blocks with scalar-loaded
+    weights (the form wide levels use), against the oracle."""
+    g = grid_config1
+    monkeypatch.setenv("BFG_TRI_SCALAR_SEGS", "0")
+    rng = np.random.default_rng(11)
+    s = rng.integers(0, g.n, 90).astype(np.uint32)
+    t = rng.integers(0, g.n, 70).astype(np.uint32)
+    say("scalar intra blocks lanes", lanes)
+    with engine_cls(g) as e:
+        e.configure(lanes=lanes)
+        got, _ = e.compute_matrix_flat(s, t)
+        rows = e.query_batch(s[:20]).dist
+    o = Oracle(g)
+    assert np.array_equal(got, o.compute_matrix_flat_mt(s, t))
+    all_nodes = np.arange(g.n, dtype=np.uint32)
+    assert np.array_equal(rows.reshape(-1), o.compute_matrix_flat_mt(s[:20], all_nodes))
+
+
 @pytest.mark.parametrize("lanes,cap", [(32, 0), (16, 48), (64, 64)])
 def test_matrix_lanes_and_batches(engine_cls, grid_config1, lanes, cap):
     """K in {16, 32, 64}, and a lane capacity that forces several device batches."""
