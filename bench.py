@@ -86,6 +86,31 @@ def summarize_counters(path):
         print(f"{ctr:12s} dispatches={cnt:6d} mean={tot / cnt:14.1f} total={tot:16.1f}  {name[:110]}")
 
 
+def counter_means(path, counter, match):
+    """Dispatches and mean value per dispatch of `counter` over the kernels
+    whose name contains `match` in a rocprofv3 counter_collection.csv."""
+    import csv
+    vals = [float(r["Counter_Value"]) for r in csv.DictReader(open(path))
+            if r["Counter_Name"] == counter and match in r["Kernel_Name"]]
+    return len(vals), (sum(vals) / len(vals) if vals else 0.0)
+
+
+def traffic_summary(fetch_csv, write_csv, cfetch_csv, cwrite_csv, lanes, workload, copy_bytes=1 << 30):
+    """FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the DOWN-rows kernel,
+    each scaled by what the counter reads per KiB for a plain copy of
+    copy_bytes (read copy_bytes, write copy_bytes)."""
+    kern = f"rows_kernel<{lanes}, false>"
+    n, fetch = counter_means(fetch_csv, "FETCH_SIZE", kern)
+    _, write = counter_means(write_csv, "WRITE_SIZE", kern)
+    _, cf = counter_means(cfetch_csv, "FETCH_SIZE", "copyBuffer")
+    _, cw = counter_means(cwrite_csv, "WRITE_SIZE", "copyBuffer")
+    fs, ws = copy_bytes / 1024 / cf, copy_bytes / 1024 / cw
+    return {"kernel": kern, "workload": workload, "lanes": lanes, "dispatches": n,
+            "fetch_kib_per_launch": fetch, "write_kib_per_launch": write, "copy_bytes": copy_bytes,
+            "copy_fetch_kib": cf, "copy_write_kib": cw, "fetch_scale": fs, "write_scale": ws,
+            "hbm_bytes_per_launch": 1024 * (fetch * fs + write * ws)}
+
+
 def committed_traffic(workload, lanes):
     """HBM bytes per launch of the dominant kernel from the committed counter
     summary for this workload (FETCH_SIZE + WRITE_SIZE, scaled by a copy
@@ -152,16 +177,24 @@ def main():
     ap.add_argument("--summarize-counters", default="", metavar="CSV",
                     help="print the per-dispatch mean of each counter per kernel of a rocprofv3 "
                          "counter_collection.csv and exit")
+    ap.add_argument("--traffic-summary", nargs=4, default=None,
+                    metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="print the DOWN-rows kernel's HBM bytes per launch as JSON: its FETCH_SIZE and "
+                         "WRITE_SIZE runs, scaled by the same counters of a --copy-probe run of 1 GiB")
     args = ap.parse_args()
     if args.summarize_counters:
         summarize_counters(args.summarize_counters)
+        return
+    if args.traffic_summary:
+        print(json.dumps(traffic_summary(*args.traffic_summary, lanes=args.lanes, workload=args.workload),
+                         indent=1))
         return
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU: --gpus {args.gpus} is not supported by this build "
-              f"(the path shards by sources; see DESIGN.md for the multi-GPU design)", file=sys.stderr)
+              f"(the path would shard by sources, one replica of the graph per GPU)", file=sys.stderr)
         sys.exit(2)
 
     from bfgpu import BatchedPhastEngine, GridSpec, build_grid, cached_grid
