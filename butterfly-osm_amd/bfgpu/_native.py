@@ -69,6 +69,8 @@ BFGPU_SYMBOLS = {
     "bfg_query_batch": [C.c_void_p, u32p, C.c_uint32, C.c_uint32, u32p],
     "bfg_isochrone": [C.c_void_p, u32p, u32p, C.c_uint32, u64p, u32p],
     "bfg_phast_seeded": [C.c_void_p, u32p, u32p, u32p, C.c_uint32, u32p, u32p],
+    "bfg_table_seeded": [C.c_void_p, u32p, u32p, u32p, C.c_uint32, u32p, u32p, u32p, C.c_uint32, C.c_uint32,
+                         u32p, C.POINTER(C.c_uint8)],
 }
 
 _bfgpu = None

@@ -185,6 +185,34 @@ class BatchedPhastEngine:
                                           _native.ptr(thr), _native.ptr(out)))
         return out
 
+    def table_seeded_bounded_routed(self, src_seedsets, tgt_seedsets, threshold=INF):
+        """bucket_ch.rs:6535-6574 on its PHAST path, table_phast_lopsided
+        (:6717-6853): one seeded field per source, cell = min over the target's
+        seeds (rank, part, ..) of F[rank] - part.  Seeds are (rank, part, ...)
+        tuples (EngineSeed, :6039); only rank and part take part.  Returns
+        (S*T u32 row-major, S*T u8 conflicts): a conflict is a pair whose
+        source and target share a seed rank, which the reference recomputes
+        with its bucket engine; those cells are INF here."""
+        def flat(seedsets):
+            off = np.zeros(len(seedsets) + 1, dtype=np.uint32)
+            ranks, parts = [], []
+            for i, seeds in enumerate(seedsets):
+                for sd in seeds:
+                    ranks.append(sd[0])
+                    parts.append(sd[1])
+                off[i + 1] = len(ranks)
+            return off, _u32(ranks), _u32(parts)
+        so, sr, sp = flat(src_seedsets)
+        to, tr, tp = flat(tgt_seedsets)
+        S, T = len(src_seedsets), len(tgt_seedsets)
+        out = np.empty(S * T, dtype=np.uint32)
+        conf = np.empty(S * T, dtype=np.uint8)
+        _check(self._lib.bfg_table_seeded(self._h, _native.ptr(so), _native.ptr(sr), _native.ptr(sp), S,
+                                          _native.ptr(to), _native.ptr(tr), _native.ptr(tp), T,
+                                          int(threshold) & INF, _native.ptr(out),
+                                          conf.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out, conf
+
     def run_phast_bounded_fast_seeded(self, seeds, threshold):
         """isochrone_handler.rs:338-344: Vec<(rank, dist)> ascending rank, dist <= threshold."""
         row = self.phast_seeded([list(seeds)], [threshold])[0]

@@ -916,6 +916,50 @@ __global__ __launch_bounds__(NT) void gather_kernel(
     }
 }
 
+// table_gather: seeded table cells of one batch (table_phast_lopsided,
+// bucket_ch.rs:6785-6811).  Tile = 64 targets x KL lanes through LDS, as in
+// gather.  Cell (lane gl, target t): the minimum over t's seeds q of
+// F - tpart[q] (saturating), F the lane's label at tlab[q] if F <= the
+// lane's field bound; INF above thr.  A target seed rank equal to one of the
+// lane's seed ranks makes the cell a conflict: INF, and conf = 1.
+template <int KL>
+__global__ __launch_bounds__(NT) void table_gather_kernel(
+    const uint32_t *__restrict__ arena, uint32_t n, uint32_t T, const uint32_t *__restrict__ toff,
+    const uint32_t *__restrict__ tlab, const uint32_t *__restrict__ trank, const uint32_t *__restrict__ tpart,
+    const uint32_t *__restrict__ seed_off, const uint32_t *__restrict__ seed_rank,
+    const uint32_t *__restrict__ lane_limit, uint32_t n_lanes, const uint32_t *__restrict__ row_map, uint32_t thr,
+    uint32_t *__restrict__ out, uint8_t *__restrict__ conf) {
+    __shared__ uint32_t tile[64][KL + 1];
+    const uint32_t t0 = blockIdx.x * 64;
+    const uint32_t g = blockIdx.y;
+    const uint32_t *lab = arena + (size_t)g * n * KL;
+    for (uint32_t idx = threadIdx.x; idx < 64 * KL; idx += NT) {
+        const uint32_t ti = idx / KL, lane = idx % KL;
+        const uint32_t t = t0 + ti, gl = g * KL + lane;
+        uint32_t best = INF, clash = 0;
+        if (t < T && gl < n_lanes) {
+            const uint32_t lim = lane_limit[gl], s0 = seed_off[gl], s1 = seed_off[gl + 1];
+            for (uint32_t q = toff[t]; q < toff[t + 1]; ++q) {
+                const uint32_t f = lab[(size_t)tlab[q] * KL + lane], part = tpart[q], r = trank[q];
+                if (f != INF && f <= lim) best = min(best, f > part ? f - part : 0u);
+                for (uint32_t k = s0; k < s1; ++k) clash |= seed_rank[k] == r;
+            }
+            if (clash || (thr != INF && best > thr)) best = INF;
+        }
+        tile[ti][lane] = best;
+        if (conf && t < T && gl < n_lanes) conf[(uint64_t)(row_map ? row_map[gl] : gl) * T + t] = (uint8_t)clash;
+    }
+    __syncthreads();
+    for (uint32_t idx = threadIdx.x; idx < 64 * KL; idx += NT) {
+        const uint32_t lane = idx / 64, ti = idx % 64;
+        const uint32_t gl = g * KL + lane, t = t0 + ti;
+        if (gl < n_lanes && t < T) {
+            const uint64_t row = row_map ? row_map[gl] : gl;
+            out[row * T + t] = tile[ti][lane];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host context
 // ---------------------------------------------------------------------------
@@ -1832,6 +1876,110 @@ int bfg_phast_seeded(bfg_ctx *c, const uint32_t *seed_offsets, const uint32_t *s
     req.scost.assign(seed_costs, seed_costs + ns);
     req.limit.assign(thresholds, thresholds + L);
     return dense_rows(c, req, dist, nullptr);
+}
+
+int bfg_table_seeded(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t *src_ranks, const uint32_t *src_parts,
+                     uint32_t S, const uint32_t *tgt_offsets, const uint32_t *tgt_ranks, const uint32_t *tgt_parts,
+                     uint32_t T, uint32_t threshold, uint32_t *out, uint8_t *conflicts) {
+    if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
+    if ((S && !src_offsets) || (T && !tgt_offsets) || (S && T && !out))
+        return fail(BFG_INVALID_PARAMETER, "null buffer");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    if (S == 0 || T == 0) return BFG_SUCCESS;
+    if (src_offsets[0] != 0 || tgt_offsets[0] != 0) return fail(BFG_INVALID_PARAMETER, "seed offsets must start at 0");
+    for (uint32_t i = 0; i < S; ++i) {
+        if (src_offsets[i + 1] < src_offsets[i]) return fail(BFG_INVALID_PARAMETER, "src_offsets not monotone");
+        if (src_offsets[i + 1] - src_offsets[i] > 64) return fail(BFG_NOT_SUPPORTED, "more than 64 seeds in one source");
+    }
+    for (uint32_t j = 0; j < T; ++j)
+        if (tgt_offsets[j + 1] < tgt_offsets[j]) return fail(BFG_INVALID_PARAMETER, "tgt_offsets not monotone");
+    const uint32_t ns = src_offsets[S], nt = tgt_offsets[T];
+    if ((ns && (!src_ranks || !src_parts)) || (nt && (!tgt_ranks || !tgt_parts)))
+        return fail(BFG_INVALID_PARAMETER, "null seed buffer");
+    int rc = check_ranks(c, src_ranks, ns, "source seed");
+    if (!rc) rc = check_ranks(c, tgt_ranks, nt, "target seed");
+    if (rc) return rc;
+    // one field per source, bounded by threshold + its largest part + the
+    // largest target part (bucket_ch.rs:6738-6752)
+    uint32_t max_tgt_part = 0;
+    for (uint32_t q = 0; q < nt; ++q) max_tgt_part = std::max(max_tgt_part, tgt_parts[q]);
+    auto sat = [](uint32_t a, uint32_t b) { return a > INF - b ? INF : a + b; };
+    Req req;
+    req.soff.assign(src_offsets, src_offsets + S + 1);
+    req.srank.assign(src_ranks, src_ranks + ns);
+    req.scost.assign(src_parts, src_parts + ns);
+    req.limit.resize(S);
+    for (uint32_t i = 0; i < S; ++i) {
+        uint32_t mp = 0;
+        for (uint32_t k = src_offsets[i]; k < src_offsets[i + 1]; ++k) mp = std::max(mp, src_parts[k]);
+        req.limit[i] = threshold == INF ? INF : sat(sat(threshold, mp), max_tgt_part);
+    }
+    // the field is read at the target seeds' ranks only: they are the
+    // matrix targets of the restricted sweep
+    std::vector<uint32_t> U(tgt_ranks, tgt_ranks + nt);
+    std::sort(U.begin(), U.end());
+    U.erase(std::unique(U.begin(), U.end()), U.end());
+    if (U.empty()) {  // no target has a seed: every cell is INF, no conflicts
+        std::fill(out, out + (size_t)S * T, INF);
+        if (conflicts) std::memset(conflicts, 0, (size_t)S * T);
+        return BFG_SUCCESS;
+    }
+    rc = stage_target_paths(c, U.data(), (uint32_t)U.size());
+    if (rc) return rc;
+    std::vector<uint32_t> tlab(nt);
+    for (uint32_t q = 0; q < nt; ++q)
+        tlab[q] = c->tgt_pos_h[std::lower_bound(U.begin(), U.end(), tgt_ranks[q]) - U.begin()];
+    DevBuf<uint32_t> d_toff, d_tlab, d_trank, d_tpart, d_out;
+    DevBuf<uint8_t> d_conf;
+    hipStream_t st = c->stream;
+    const std::vector<uint32_t> h_toff(tgt_offsets, tgt_offsets + T + 1), h_trank(tgt_ranks, tgt_ranks + nt),
+        h_tpart(tgt_parts, tgt_parts + nt);
+    HIP_TRY(d_toff.upload(h_toff, st));
+    HIP_TRY(d_tlab.upload(tlab, st));
+    HIP_TRY(d_trank.upload(h_trank, st));
+    HIP_TRY(d_tpart.upload(h_tpart, st));
+    HIP_TRY(d_out.alloc((size_t)S * T));
+    if (conflicts) HIP_TRY(d_conf.alloc((size_t)S * T));
+    HIP_TRY(hipStreamSynchronize(st));  // before the host tables above may go
+    struct PathsOn {  // restrict the bottom sweep for this call only
+        bfg_ctx *c;
+        explicit PathsOn(bfg_ctx *cc) : c(cc) { c->paths_on = true; }
+        ~PathsOn() { c->paths_on = false; }
+    } paths_on(c);
+    rc = run_all(c, req, [&](uint32_t, uint32_t nl, const std::vector<uint32_t> &, float *oms) -> int {
+        const uint32_t groups = (nl + c->lanes - 1) / c->lanes;
+        const dim3 grid((T + 63) / 64, groups);
+        HIP_TRY(hipEventRecord(c->ev[0], st));
+        switch (c->lanes) {
+            case 16:
+                hipLaunchKernelGGL(table_gather_kernel<16>, grid, dim3(NT), 0, st, c->arena.p, label_stride(c), T,
+                                   d_toff.p, d_tlab.p, d_trank.p, d_tpart.p, c->seed_off.p, c->seed_rank.p,
+                                   c->lane_limit.p, nl, c->row_map.p, threshold, d_out.p, d_conf.p);
+                break;
+            case 64:
+                hipLaunchKernelGGL(table_gather_kernel<64>, grid, dim3(NT), 0, st, c->arena.p, label_stride(c), T,
+                                   d_toff.p, d_tlab.p, d_trank.p, d_tpart.p, c->seed_off.p, c->seed_rank.p,
+                                   c->lane_limit.p, nl, c->row_map.p, threshold, d_out.p, d_conf.p);
+                break;
+            default:
+                hipLaunchKernelGGL(table_gather_kernel<32>, grid, dim3(NT), 0, st, c->arena.p, label_stride(c), T,
+                                   d_toff.p, d_tlab.p, d_trank.p, d_tpart.p, c->seed_off.p, c->seed_rank.p,
+                                   c->lane_limit.p, nl, c->row_map.p, threshold, d_out.p, d_conf.p);
+                break;
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(c->ev[1], st));
+        HIP_TRY(hipEventSynchronize(c->ev[1]));
+        float a = 0;
+        HIP_TRY(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+        *oms += a;
+        return BFG_SUCCESS;
+    });
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(out, d_out.p, (size_t)S * T * 4, hipMemcpyDeviceToHost));
+    if (conflicts) HIP_TRY(hipMemcpy(conflicts, d_conf.p, (size_t)S * T, hipMemcpyDeviceToHost));
+    return BFG_SUCCESS;
 }
 
 }  // extern "C"

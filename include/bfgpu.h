@@ -150,6 +150,25 @@ int bfg_phast_seeded(bfg_ctx *ctx, const uint32_t *seed_offsets, const uint32_t 
                      const uint32_t *seed_costs, uint32_t n_lanes, const uint32_t *thresholds,
                      uint32_t *dist);
 
+/* Seeded S x T table through one seeded PHAST field per source: the
+ * table_phast_lopsided path (matrix/bucket_ch.rs:6717-6853) that
+ * table_seeded_bounded_routed (:6535-6574) takes when phast_wins (:6677-6711).
+ * Source i has seeds (src_ranks[k], src_parts[k]) for k in
+ * [src_offsets[i], src_offsets[i+1]) (at most 64), target j the same in
+ * tgt_*.  The field of source i is run_phast_bounded_fast_seeded of its
+ * seeds, bounded by threshold + its largest part + the largest target part
+ * (saturating; unbounded when threshold == BFG_INF, :6738-6752);
+ * out[i * n_tgt + j] = min over target seeds (r, p) with a field value F[r]
+ * of F[r] - p (saturating), BFG_INF above threshold (:6785-6811).  A pair
+ * whose source and target share a seed rank is the reference's same-edge
+ * conflict, which it recomputes with its bucket engine (:6789-6793,
+ * :6828-6845): its cell is BFG_INF here and conflicts[i * n_tgt + j] (when
+ * conflicts is not null) is 1, else 0. */
+int bfg_table_seeded(bfg_ctx *ctx, const uint32_t *src_offsets, const uint32_t *src_ranks,
+                     const uint32_t *src_parts, uint32_t n_src, const uint32_t *tgt_offsets,
+                     const uint32_t *tgt_ranks, const uint32_t *tgt_parts, uint32_t n_tgt,
+                     uint32_t threshold, uint32_t *out, uint8_t *conflicts);
+
 #ifdef __cplusplus
 }
 #endif

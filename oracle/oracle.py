@@ -193,6 +193,47 @@ class Oracle:
                                       threshold, _p(orank), _p(odist))
         return list(zip(orank[:cnt].tolist(), odist[:cnt].tolist()))
 
+    # matrix/bucket_ch.rs:6717-6853 (table_phast_lopsided), the bucket
+    # recomputation of same-edge conflicts (:6828-6845) left out: returns
+    # (S*T cells, S*T conflict flags), a conflicted cell INF
+    def table_phast_lopsided(self, src_seedsets, tgt_seedsets, threshold):
+        S, T = len(src_seedsets), len(tgt_seedsets)
+        cells = np.full(S * T, INF, dtype=np.uint32)
+        conf = np.zeros(S * T, dtype=np.uint8)
+        # :6738-6752 the field bound widens the threshold by both sides' worst parts
+        max_tgt_part = max((s[1] for v in tgt_seedsets for s in v), default=0)
+
+        def field_bound(seeds):
+            if threshold == INF:
+                return INF
+            return min(INF, min(INF, threshold + max((s[1] for s in seeds), default=0)) + max_tgt_part)
+
+        tgt_ranks = {s[0] for v in tgt_seedsets for s in v}
+        for i, seeds in enumerate(src_seedsets):  # :6753
+            if not seeds:  # :6756-6758
+                continue
+            src_ranks = {s[0] for s in seeds}
+            settled = self.seeded([(s[0], s[1]) for s in seeds], field_bound(seeds))  # :6762-6769
+            field = {}
+            for rank, dist in settled:  # :6771-6783
+                if rank in tgt_ranks:
+                    field[rank] = min(dist, field.get(rank, INF))
+            for j, tseeds in enumerate(tgt_seedsets):  # :6785-6811
+                if not tseeds:
+                    continue
+                if any(s[0] in src_ranks for s in tseeds):  # :6789-6793
+                    conf[i * T + j] = 1
+                    continue
+                best = INF
+                for s in tseeds:
+                    f = field.get(s[0])
+                    if f is not None and f != INF:
+                        best = min(best, max(0, f - s[1]))  # saturating_sub
+                if threshold != INF and best != INF and best > threshold:
+                    best = INF
+                cells[i * T + j] = best
+        return cells, conf
+
     def __del__(self):
         st = getattr(self, "_st", None)
         if st:

@@ -247,3 +247,31 @@ def test_chords_graph(engine_cls):
     with engine_cls(g) as e:
         got, _ = e.compute_matrix_flat(s, t)
     assert np.array_equal(got, o.compute_matrix_flat(s, t))
+
+
+@pytest.mark.parametrize("lanes,thr", [(64, INF), (32, 2500)])
+def test_table_seeded(engine_cls, grid_config1, lanes, thr):
+    """Seeded tables (table_phast_lopsided): multi-seed sources and targets with
+    parts, empty seed sets, targets sharing a seed rank with a source
+    (conflicts), unbounded and bounded, against the oracle."""
+    g = grid_config1
+    rng = np.random.default_rng(lanes + 1)
+    ranks = rng.choice(g.n, 400, replace=False)
+
+    def sets(count, pool):
+        out = []
+        for _ in range(count):
+            k = int(rng.integers(0, 4))
+            out.append([(int(r), int(rng.integers(0, 400)), 0, True) for r in rng.choice(pool, k, replace=False)])
+        return out
+
+    src = sets(70, ranks[:200])
+    tgt = sets(90, ranks[150:])  # ranks 150..199 may be shared: conflicts
+    say("seeded table lanes", lanes, "thr", thr)
+    with engine_cls(g) as e:
+        e.configure(lanes=lanes)
+        cells, conf = e.table_seeded_bounded_routed(src, tgt, thr)
+    exp, econf = Oracle(g).table_phast_lopsided(src, tgt, thr)
+    assert conf.any() and (exp != INF).any()
+    assert np.array_equal(conf, econf)
+    assert np.array_equal(cells, exp)

@@ -165,3 +165,57 @@ def test_saturating_add_at_inf_boundary():
     assert d.tolist() == [0, INF, big]  # big + 0xFFFFFFFE saturates to INF
     d1 = o.phast_query(1)
     assert d1.tolist() == [105, 0, 5]
+
+
+def _seedsets(rng, ranks, count, max_seeds, max_part, allow_empty=True):
+    out = []
+    for _ in range(count):
+        k = int(rng.integers(0 if allow_empty else 1, max_seeds + 1))
+        out.append([(int(r), int(rng.integers(0, max_part + 1)), 0, True)
+                    for r in rng.choice(ranks, k, replace=False)])
+    return out
+
+
+@pytest.mark.parametrize("thr", [INF, 0, 60, 150])
+def test_table_phast_lopsided_broom_closed_form(thr):
+    """The seeded table on the broom (bucket_ch.rs:5638-5670): a source's field
+    at leaf t is min over its seeds (s, a) of a + wu[s] + wd[t] (leaves differ),
+    present up to threshold + its largest part + the largest target part; a
+    cell is min over the target's seeds (t, b) of field - b (saturating), INF
+    above threshold; a shared seed rank is a conflict."""
+    g, wu, wd = broom_graph(12)
+    rng = np.random.default_rng(thr & 0xFFFF)
+    leaves = np.arange(12)
+    src = _seedsets(rng, leaves, 9, 3, 40)
+    tgt = _seedsets(rng, leaves, 11, 3, 40)
+    cells, conf = Oracle(g).table_phast_lopsided(src, tgt, thr)
+    max_b = max((s[1] for v in tgt for s in v), default=0)
+    for i, ss in enumerate(src):
+        bound = INF if thr == INF else min(INF, thr + max((s[1] for s in ss), default=0) + max_b)
+        for j, ts in enumerate(tgt):
+            k = i * len(tgt) + j
+            shared = {s[0] for s in ss} & {t[0] for t in ts}
+            assert conf[k] == (1 if ss and ts and shared else 0)
+            if not ss or not ts or shared:
+                assert cells[k] == INF
+                continue
+            best = INF
+            for t, b, _, _ in ts:
+                f = min(int(a + wu[s] + wd[t]) for s, a, _, _ in ss)
+                if f <= bound:
+                    best = min(best, max(0, f - b))
+            if thr != INF and best > thr:
+                best = INF
+            assert cells[k] == best, (i, j)
+
+
+def test_table_phast_lopsided_single_seeds_equal_the_matrix(grid_small):
+    """One seed of part 0 per source and per target, no shared rank: the table
+    is compute_matrix_flat."""
+    g = grid_small
+    rng = np.random.default_rng(5)
+    ranks = rng.choice(g.n, 60, replace=False)
+    s, t = ranks[:25], ranks[25:]
+    cells, conf = Oracle(g).table_phast_lopsided([[(int(r), 0)] for r in s], [[(int(r), 0)] for r in t], INF)
+    assert not conf.any()
+    assert np.array_equal(cells, Oracle(g).compute_matrix_flat(s, t))
