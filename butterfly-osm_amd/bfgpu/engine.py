@@ -68,8 +68,13 @@ class BatchedPhastEngine:
         _check(self._lib.bfg_ctx_create(C.byref(d), device, C.byref(h)))
         self._h = h
         self.n_nodes = graph.n
+        self._device = device
+        self._rev = None
 
     def close(self):
+        if getattr(self, "_rev", None) is not None:
+            self._rev.close()
+            self._rev = None
         if getattr(self, "_h", None):
             self._lib.bfg_ctx_destroy(self._h)
             self._h = None
@@ -212,6 +217,19 @@ class BatchedPhastEngine:
                                           int(threshold) & INF, _native.ptr(out),
                                           conf.ctypes.data_as(C.POINTER(C.c_uint8))))
         return out, conf
+
+    def reverse_engine(self):
+        """The engine over the reverse metric's CCH (CchGraph.transpose), built on
+        first use: a second device context whose rows are all-to-one distances
+        (the reference passes the swapped flats, range/phast.rs:869-885)."""
+        if self._rev is None:
+            self._rev = BatchedPhastEngine(self.graph.transpose(), device=self._device)
+        return self._rev
+
+    def query_block_gated_reverse(self, targets, threshold=INF):
+        """PhastEngine::query_block_gated_reverse (range/phast.rs:879-1108) for
+        many targets: dist[i][v] = d(v -> targets[i]), INF above threshold."""
+        return self.reverse_engine().query_batch_bounded(targets, threshold)
 
     def run_phast_bounded_fast_seeded(self, seeds, threshold):
         """isochrone_handler.rs:338-344: Vec<(rank, dist)> ascending rank, dist <= threshold."""

@@ -126,6 +126,23 @@ class CchGraph:
         do, dt, dw = csr(down_arcs)
         return CchGraph(n=n, up_off=uo, up_tgt=ut, up_w=uw, dn_off=do, dn_tgt=dt, dn_w=dw)
 
+    def transpose(self):
+        """The CCH of the reverse metric: UP arcs are the DOWN arcs reversed
+        (DownReverseAdjFlat, matrix/bucket_ch.rs:191-424, keyed by head) and DOWN
+        arcs the UP arcs reversed, per-tail targets ascending.  A forward PHAST
+        on it is the reference's reverse PHAST (range/phast.rs:879-1108:
+        upward over DOWN arcs reversed, then a rank-descending pull over UP
+        arcs): its label of v for origin t is d(v -> t)."""
+        def flip(off, tgt, w):
+            tails = np.repeat(np.arange(self.n, dtype=np.uint32), np.diff(off.astype(np.int64)))
+            order = np.lexsort((tails, tgt))  # by new tail (old head), then new head
+            noff = np.zeros(self.n + 1, dtype=np.uint64)
+            noff[1:] = np.cumsum(np.bincount(tgt, minlength=self.n), dtype=np.uint64)
+            return noff, tails[order], w[order]
+        uo, ut, uw = flip(self.dn_off, self.dn_tgt, self.dn_w)
+        do, dt, dw = flip(self.up_off, self.up_tgt, self.up_w)
+        return CchGraph(n=self.n, up_off=uo, up_tgt=ut, up_w=uw, dn_off=do, dn_tgt=dt, dn_w=dw)
+
     def desc(self):
         """bfg_graph_desc pointing into this object's arrays (keep self alive)."""
         d = _native.GraphDesc()

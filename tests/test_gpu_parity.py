@@ -275,3 +275,30 @@ def test_table_seeded(engine_cls, grid_config1, lanes, thr):
     assert conf.any() and (exp != INF).any()
     assert np.array_equal(conf, econf)
     assert np.array_equal(cells, exp)
+
+
+def test_reverse_phast_and_arrive_isochrones(engine_cls, grid_config1):
+    """Reverse PHAST (range/phast.rs:879-1108) through the engine over the
+    transposed CCH: all-to-one rows, bounded, and the reverse matrix equals the
+    forward one transposed."""
+    g = grid_config1
+    rng = np.random.default_rng(3)
+    tg = rng.integers(0, g.n, 12).astype(np.uint32)
+    o = Oracle(g)
+    say("reverse PHAST targets", len(tg))
+    with engine_cls(g) as e:
+        r = e.query_block_gated_reverse(tg).dist
+        rb = e.query_block_gated_reverse(tg, 1500).dist
+        s = rng.integers(0, g.n, 30).astype(np.uint32)
+        fwd, _ = e.compute_matrix_flat(s, tg)
+        rev, _ = e.reverse_engine().compute_matrix_flat(tg, s)
+        lim = np.array([600, 1200, 2400, 4800] * 3, dtype=np.uint32)
+        bits, times = e.reverse_engine().isochrones(tg, lim)
+    for i, t in enumerate(tg):
+        exp = o.phast_query_reverse(int(t))
+        assert np.array_equal(r[i], exp)
+        assert np.array_equal(rb[i], mask(exp, 1500))
+        assert np.array_equal(times[i], mask(exp, lim[i]))  # arrive isochrone: who reaches t within the limit
+        inside = np.nonzero(exp <= lim[i])[0]
+        assert np.array_equal(np.nonzero(np.unpackbits(bits[i].view(np.uint8), bitorder="little")[:g.n])[0], inside)
+    assert np.array_equal(rev.reshape(tg.size, s.size).T, fwd.reshape(s.size, tg.size))

@@ -85,3 +85,27 @@ def test_weight_width_codec():
     # sentinel of each width widens to u32::MAX (WeightArray::get)
     assert weight_decode(np.array([0xFFFF, 7], dtype=np.uint16), "u16").tolist() == [INF, 7]
     assert weight_decode(np.array([0xFF, 0xFF, 0xFF, 1, 0, 0], dtype=np.uint8), "u24").tolist() == [INF, 1]
+
+
+def test_transpose_is_an_involution_and_keeps_rank_alignment(grid_small):
+    """CchGraph.transpose (the reverse metric's CCH): UP targets stay above their
+    tail, DOWN targets below, and transposing twice gives the CCH back."""
+    g = grid_small
+    r = g.transpose()
+    tails_up = np.repeat(np.arange(g.n), np.diff(r.up_off.astype(np.int64)))
+    tails_dn = np.repeat(np.arange(g.n), np.diff(r.dn_off.astype(np.int64)))
+    assert (r.up_tgt > tails_up).all() and (r.dn_tgt < tails_dn).all()
+    assert r.m_up == g.m_down and r.m_down == g.m_up
+    rr = r.transpose()
+    for f in ("up_off", "up_tgt", "up_w", "dn_off", "dn_tgt", "dn_w"):
+        a, b = getattr(g, f), getattr(rr, f)
+        if f.endswith("tgt") or f.endswith("_w"):  # per-tail order may differ: compare sorted (tail, head, w)
+            continue
+        assert np.array_equal(a, b), f
+    for off, tgt, w, off2, tgt2, w2 in ((g.up_off, g.up_tgt, g.up_w, rr.up_off, rr.up_tgt, rr.up_w),
+                                        (g.dn_off, g.dn_tgt, g.dn_w, rr.dn_off, rr.dn_tgt, rr.dn_w)):
+        t = np.repeat(np.arange(g.n), np.diff(off.astype(np.int64)))
+        k1 = np.lexsort((w, tgt, t))
+        k2 = np.lexsort((w2, tgt2, t))
+        assert np.array_equal(tgt[k1], tgt2[k2]) and np.array_equal(w[k1], w2[k2])
+

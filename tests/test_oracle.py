@@ -219,3 +219,16 @@ def test_table_phast_lopsided_single_seeds_equal_the_matrix(grid_small):
     cells, conf = Oracle(g).table_phast_lopsided([[(int(r), 0)] for r in s], [[(int(r), 0)] for r in t], INF)
     assert not conf.any()
     assert np.array_equal(cells, Oracle(g).compute_matrix_flat(s, t))
+
+
+def test_transposed_phast_is_reverse_phast(grid_small):
+    """Forward PHAST on the transposed CCH (CchGraph.transpose) equals the
+    reverse PHAST restatement (range/phast.rs:879-1108) and Dijkstra towards
+    the target on the transposed union graph."""
+    g = grid_small
+    o, ro = Oracle(g), Oracle(g.transpose())
+    off, head, w = union_graph(g.transpose())
+    for t in (0, 7, g.n // 2, g.n - 1):
+        rev = o.phast_query_reverse(t)
+        assert np.array_equal(ro.phast_query(t), rev)
+        assert np.array_equal(rev, dijkstra(g.n, off, head, w, t))
