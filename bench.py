@@ -160,6 +160,88 @@ def cpu_baseline_mt(g, src, tgt, budget_s):
             "sample": f"{k} sources x {len(tgt)} targets, one K=8 batch per thread"}
 
 
+def serving_requests(g, count, seed):
+    """The mixed stream: 70 % matrices with S and T each log-uniform in
+    [10, 2000] (uniform random nodes), 30 % isochrones from a uniform random
+    node with a limit of 5, 10, 15, 30, 45 or 60 minutes (decisecond weights)."""
+    from bfgpu.serving import IsochroneRequest, MatrixRequest
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        if rng.random() < 0.7:
+            S, T = (int(round(np.exp(rng.uniform(np.log(10), np.log(2000))))) for _ in range(2))
+            out.append(MatrixRequest(rng.integers(0, g.n, S).astype(np.uint32),
+                                     rng.integers(0, g.n, T).astype(np.uint32)))
+        else:
+            out.append(IsochroneRequest(int(rng.integers(0, g.n)), int(rng.choice([5, 10, 15, 30, 45, 60])) * 600))
+    return out
+
+
+def serve_stream(loop_cls, eng, reqs, arrivals):
+    """Submits reqs at their arrival offsets (seconds) and returns per-request
+    latencies (s), the span from the first arrival to the last completion (s),
+    the results and the loop's round count."""
+    with loop_cls(eng) as loop:
+        t0 = time.perf_counter()
+        futs = []
+        for r, a in zip(reqs, arrivals):
+            wait = t0 + a - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            futs.append(loop.submit(r))
+        res = [f.result() for f in futs]
+        rounds = loop.rounds
+    lat = np.array([f.completed - f.submitted for f in futs])
+    span = max(f.completed for f in futs) - t0
+    return lat, span, res, rounds
+
+
+def serving_bench(eng, g, args, rows, chords):
+    from bfgpu.serving import PlainLoop, ServingLoop
+    n = args.serving_requests
+    warm = serving_requests(g, 8, args.seed + 7)
+    t0 = time.perf_counter()
+    for r in warm:
+        serve_stream(PlainLoop, eng, [r], [0.0])
+    cal = serving_requests(g, 24, args.seed + 8)
+    t0 = time.perf_counter()
+    serve_stream(PlainLoop, eng, cal, [0.0] * len(cal))
+    plain_rate = len(cal) / (time.perf_counter() - t0)
+    log(f"plain capacity ~{plain_rate:.1f} requests/s")
+    levels = []
+    exact = True
+    for k, load in enumerate((0.5, 1.0, 2.0)):
+        lam = load * plain_rate
+        reqs = serving_requests(g, n, args.seed + 10 + k)
+        arrivals = np.cumsum(np.random.default_rng(args.seed + 20 + k).exponential(1.0 / lam, n))
+        arrivals -= arrivals[0]
+        row = {"offered_rps": lam, "load_vs_plain_capacity": load}
+        results = {}
+        for name, cls in (("plain", PlainLoop), ("serving", ServingLoop)):
+            lat, span, res, rounds = serve_stream(cls, eng, reqs, arrivals)
+            results[name] = res
+            row[name] = {"p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+                         "throughput_rps": n / span, "gpu_rounds": rounds}
+            log(f"load {load}: {name} p50 {row[name]['p50_ms']:.1f} ms p99 {row[name]['p99_ms']:.1f} ms "
+                f"{row[name]['throughput_rps']:.1f} req/s in {rounds} rounds")
+        same = all(np.array_equal(a, b) for a, b in zip(results["plain"], results["serving"]))
+        exact &= same
+        row["bit_exact_vs_plain"] = same
+        levels.append(row)
+    top = levels[-1]
+    res = {
+        "metric": "serving p50/p99 vs plain (mixed matrix + isochrone stream, continuous batching)",
+        "value": top["serving"]["throughput_rps"], "unit": "requests/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"Poisson mixed stream, {n} requests per load: 70% matrices S,T log-uniform in "
+                               f"[10, 2000], 30% isochrones with limits 5-60 min; {rows}x{rows} jittered grid "
+                               f"+ {int(chords * 100)}% chords, n={g.n}",
+                   "loads_vs_plain_capacity": [0.5, 1.0, 2.0], "plain_capacity_rps": plain_rate},
+        "levels": levels, "bit_exact": exact,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +259,11 @@ def main():
     ap.add_argument("--summarize-counters", default="", metavar="CSV",
                     help="print the per-dispatch mean of each counter per kernel of a rocprofv3 "
                          "counter_collection.csv and exit")
+    ap.add_argument("--serving", action="store_true",
+                    help="instead of the matrix line: Poisson streams of mixed matrix and isochrone requests at "
+                         "three loads, served by the continuous-batching loop and as plain per-request calls "
+                         "(p50/p99 latency, throughput, bit-exact per request); prints its own JSON line")
+    ap.add_argument("--serving-requests", type=int, default=100, help="requests per load level")
     ap.add_argument("--traffic-summary", nargs=4, default=None,
                     metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="print the DOWN-rows kernel's HBM bytes per launch as JSON: its FETCH_SIZE and "
@@ -215,6 +302,9 @@ def main():
     eng.configure(lanes=args.lanes, lane_capacity=args.lane_capacity)
     info = eng.info()
     log(f"context ready in {time.perf_counter() - t0:.1f}s: {info}")
+    if args.serving:
+        serving_bench(eng, g, args, rows, chords)
+        return
 
     d_src = DeviceArray.from_host(src)
     d_tgt = DeviceArray.from_host(tgt)

@@ -1,0 +1,128 @@
+"""The serving loop's host logic: admission, coalescing into shared sweeps and
+the split back into per-request results.  CPU tests run it over an engine
+whose sweeps are the oracle's; the GPU test runs it over the HIP engine and
+checks every result against the same request issued alone."""
+import numpy as np
+import pytest
+
+from bfgpu.graph import INF
+from bfgpu.serving import IsochroneRequest, MatrixRequest, PlainLoop, ServingLoop, run_round
+from oracle.oracle import Oracle
+
+
+class OracleEngine:
+    """The engine calls the serving loop makes, answered by the oracle."""
+
+    def __init__(self, g):
+        self.o, self.n_nodes = Oracle(g), g.n
+        self.calls = 0
+
+    def compute_matrix_block_gated(self, sources, targets, threshold):
+        self.calls += 1
+        m = self.o.compute_matrix_flat_mt(sources, targets)
+        if threshold != INF:
+            m = np.where(m > threshold, INF, m).astype(np.uint32)
+        return m, None
+
+    def isochrones(self, origins, limits, want_times=True):
+        self.calls += 1
+        words = (self.n_nodes + 63) // 64
+        bits = np.zeros((len(origins), words), dtype=np.uint64)
+        for i, (o, lim) in enumerate(zip(origins, limits)):
+            inside = self.o.phast_query(int(o)) <= lim
+            padded = np.zeros(words * 64, dtype=bool)
+            padded[:self.n_nodes] = inside
+            bits[i] = np.packbits(padded, bitorder="little").view(np.uint64)
+        return bits, None
+
+
+def mixed_stream(g, rng, count):
+    reqs = []
+    for _ in range(count):
+        if rng.random() < 0.7:
+            s = rng.integers(0, g.n, int(rng.integers(1, 40))).astype(np.uint32)
+            t = rng.integers(0, g.n, int(rng.integers(1, 50))).astype(np.uint32)
+            thr = INF if rng.random() < 0.7 else 1500
+            reqs.append(MatrixRequest(s, t, thr))
+        else:
+            reqs.append(IsochroneRequest(int(rng.integers(0, g.n)), int(rng.choice([300, 600, 1800]))))
+    return reqs
+
+
+def alone(engine, reqs):
+    return [run_round(engine, [r])[0] for r in reqs]
+
+
+def test_round_results_equal_requests_alone(grid_small):
+    eng = OracleEngine(grid_small)
+    reqs = mixed_stream(grid_small, np.random.default_rng(1), 16)
+    together = run_round(eng, reqs)
+    assert eng.calls <= 3  # one matrix sweep per threshold present, one isochrone sweep
+    for a, b in zip(together, alone(eng, reqs)):
+        assert np.array_equal(a, b)
+
+
+def test_matrix_request_against_the_oracle(grid_small):
+    eng = OracleEngine(grid_small)
+    rng = np.random.default_rng(2)
+    r = MatrixRequest(rng.integers(0, grid_small.n, 9).astype(np.uint32),
+                      np.array([5, 3, 5, 0], dtype=np.uint32))  # duplicate target columns
+    got = run_round(eng, [r, MatrixRequest(r.targets, r.sources)])[0]
+    assert np.array_equal(got, eng.o.compute_matrix_flat(r.sources, r.targets))
+
+
+@pytest.mark.parametrize("loop_cls", [ServingLoop, PlainLoop])
+def test_loops_serve_a_burst(grid_small, loop_cls):
+    eng = OracleEngine(grid_small)
+    reqs = mixed_stream(grid_small, np.random.default_rng(3), 24)
+    want = alone(OracleEngine(grid_small), reqs)
+    with loop_cls(eng) as loop:
+        futs = [loop.submit(r) for r in reqs]
+        got = [f.result(timeout=120) for f in futs]
+        rounds = loop.rounds
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+    if loop_cls is ServingLoop:
+        assert rounds < len(reqs)  # requests that waited were coalesced
+    else:
+        assert rounds == len(reqs)
+
+
+def test_admission_limits_a_round(grid_small):
+    """A round takes the oldest requests while their lanes fit the budget."""
+    from concurrent.futures import Future
+    eng = OracleEngine(grid_small)
+    reqs = [MatrixRequest(np.arange(10, dtype=np.uint32), np.arange(10, dtype=np.uint32)) for _ in range(6)]
+    with ServingLoop(eng, max_lanes=25) as loop:
+        with loop._cv:  # the worker needs this lock to take requests: queue all six first
+            pending = [(r, Future()) for r in reqs]
+            loop._pending.extend(pending)
+            batch = loop._take()
+            loop._pending.extendleft(reversed(batch))
+            loop._cv.notify()
+        assert len(batch) == 2  # 10 + 10 lanes fit 25, a third request would not
+        for _, f in pending:
+            f.result(timeout=60)
+
+
+@pytest.mark.gpu
+def test_serving_on_gpu_equals_requests_alone(grid_config1):
+    from bfgpu import BatchedPhastEngine
+    g = grid_config1
+    reqs = mixed_stream(g, np.random.default_rng(4), 30)
+    print("[gpu-test] serving burst", len(reqs), flush=True)
+    with BatchedPhastEngine(g) as e:
+        want = alone(e, reqs)
+        with ServingLoop(e) as loop:
+            got = [f.result(timeout=120) for f in [loop.submit(r) for r in reqs]]
+            rounds = loop.rounds
+    assert rounds < len(reqs)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+    o = Oracle(g)
+    for r, res in zip(reqs, got):
+        if isinstance(r, MatrixRequest):
+            m = o.compute_matrix_flat_mt(r.sources, r.targets)
+            if r.threshold != INF:
+                m = np.where(m > r.threshold, INF, m)
+            assert np.array_equal(res, m)
