@@ -1,0 +1,126 @@
+"""Row streaming of matrices as Arrow IPC tiles.
+
+The wire format of the reference's matrix tiles (route/src/matrix/arrow_stream.rs:
+6-15, schema :102-110): one record per tile of (src_block_start u32,
+dst_block_start u32, src_block_len u16, dst_block_len u16, durations_ms
+binary), the durations row-major little-endian u32.  A writer sends the schema,
+then record batches as tiles complete (the streaming protocol, :17-22).
+
+stream_matrix_tiles computes the matrix in source chunks on the GPU and yields
+each chunk's tiles while the next chunk runs: a worker thread keeps the device
+busy with chunk i + 1 while the caller packs and sends chunk i.  Values are
+the engine's u32 durations times `scale` (saturating, INF kept), so a caller
+holding decisecond weights passes scale=100 for milliseconds.
+"""
+import queue
+import threading
+
+import numpy as np
+
+from .graph import INF
+
+
+def matrix_tile_schema():
+    import pyarrow as pa
+    return pa.schema([
+        pa.field("src_block_start", pa.uint32(), nullable=False),
+        pa.field("dst_block_start", pa.uint32(), nullable=False),
+        pa.field("src_block_len", pa.uint16(), nullable=False),
+        pa.field("dst_block_len", pa.uint16(), nullable=False),
+        pa.field("durations_ms", pa.binary(), nullable=False),
+    ])
+
+
+def _scaled(block, scale):
+    if scale == 1:
+        return block
+    wide = block.astype(np.uint64) * np.uint64(scale)
+    return np.where((block == INF) | (wide >= INF), INF, wide).astype(np.uint32)
+
+
+def stream_matrix_tiles(engine, sources, targets, chunk_sources=2048, tile_sources=256, tile_targets=4096,
+                        threshold=INF, scale=1):
+    """Yields (src_start, dst_start, src_len, dst_len, row-major u32 block) tiles
+    covering the S x T matrix, source chunk by source chunk, as they complete."""
+    if tile_sources > 0xFFFF or tile_targets > 0xFFFF:
+        raise ValueError("tile sides are u16 in the wire format")
+    src = np.ascontiguousarray(sources, dtype=np.uint32)
+    tgt = np.ascontiguousarray(targets, dtype=np.uint32)
+    chunks = [(c0, src[c0:c0 + chunk_sources]) for c0 in range(0, src.size, chunk_sources)]
+    done = queue.Queue(maxsize=2)  # bounded: the GPU runs at most one chunk ahead of the consumer
+
+    def produce():
+        try:
+            for c0, s in chunks:
+                m, _ = engine.compute_matrix_block_gated(s, tgt, threshold)
+                done.put((c0, m.reshape(s.size, tgt.size)))
+        except Exception as e:  # handed to the consumer
+            done.put(e)
+        done.put(None)
+
+    worker = threading.Thread(target=produce, daemon=True)
+    worker.start()
+    try:
+        while True:
+            item = done.get()
+            if item is None:
+                break
+            if isinstance(item, Exception):
+                raise item
+            c0, m = item
+            for r0 in range(0, m.shape[0], tile_sources):
+                for t0 in range(0, m.shape[1], tile_targets):
+                    block = _scaled(np.ascontiguousarray(m[r0:r0 + tile_sources, t0:t0 + tile_targets]), scale)
+                    yield c0 + r0, t0, block.shape[0], block.shape[1], block
+    finally:
+        worker.join()
+
+
+def tiles_to_record_batch(tiles):
+    """arrow_stream.rs:112-142: one record per tile."""
+    import pyarrow as pa
+    return pa.record_batch([
+        pa.array([t[0] for t in tiles], pa.uint32()),
+        pa.array([t[1] for t in tiles], pa.uint32()),
+        pa.array([t[2] for t in tiles], pa.uint16()),
+        pa.array([t[3] for t in tiles], pa.uint16()),
+        pa.array([t[4].astype("<u4").tobytes() for t in tiles], pa.binary()),
+    ], schema=matrix_tile_schema())
+
+
+def write_matrix_ipc(engine, sources, targets, sink, tiles_per_batch=8, **kw):
+    """ArrowMatrixWriter (arrow_stream.rs:145-200): the schema, then a record
+    batch of up to tiles_per_batch tiles as they complete.  Returns
+    (tiles written, duration bytes written)."""
+    import pyarrow as pa
+    n_tiles = n_bytes = 0
+    pending = []
+    with pa.ipc.new_stream(sink, matrix_tile_schema()) as w:
+        for tile in stream_matrix_tiles(engine, sources, targets, **kw):
+            pending.append(tile)
+            if len(pending) == tiles_per_batch:
+                w.write_batch(tiles_to_record_batch(pending))
+                n_tiles += len(pending)
+                n_bytes += sum(t[4].nbytes for t in pending)
+                pending = []
+        if pending:
+            w.write_batch(tiles_to_record_batch(pending))
+            n_tiles += len(pending)
+            n_bytes += sum(t[4].nbytes for t in pending)
+    return n_tiles, n_bytes
+
+
+def read_matrix_ipc(source, n_src, n_tgt):
+    """Reassembles the S x T matrix from a tile stream (the consumer side)."""
+    import pyarrow as pa
+    out = np.full((n_src, n_tgt), INF, dtype=np.uint32)
+    seen = np.zeros((n_src, n_tgt), dtype=bool)
+    for batch in pa.ipc.open_stream(source):
+        cols = batch.to_pydict()
+        for s0, t0, sl, tl, buf in zip(cols["src_block_start"], cols["dst_block_start"], cols["src_block_len"],
+                                       cols["dst_block_len"], cols["durations_ms"]):
+            out[s0:s0 + sl, t0:t0 + tl] = np.frombuffer(buf, dtype="<u4").reshape(sl, tl)
+            seen[s0:s0 + sl, t0:t0 + tl] = True
+    if not seen.all():
+        raise ValueError("tile stream does not cover the matrix")
+    return out
