@@ -28,6 +28,7 @@ for _p in (REPO, PKG):
 
 METRIC = "matrix cells/s 10k×10k u32 bit-exact, % HBM roofline; serving p50/p99 vs plain"
 HBM_PEAK_GBPS = 8000.0
+INF_U32 = 0xFFFFFFFF
 
 WORKLOADS = {
     # name: (grid rows, chord fraction, S, T)
@@ -242,6 +243,56 @@ def serving_bench(eng, g, args, rows, chords):
     print(json.dumps(res), flush=True)
 
 
+def isochrone_bench(eng, g, args, rows, chords):
+    """L isochrones from uniform random origins with uneven limits in one call:
+    reach bits of every node for every lane (times stay on the device except
+    for the checked sample)."""
+    L = args.isochrones
+    rng = np.random.default_rng(args.seed + 30)
+    org = rng.integers(0, g.n, L).astype(np.uint32)
+    lim = (rng.choice([5, 10, 15, 30, 45, 60], L) * 600).astype(np.uint32)
+    eng.isochrones(org[:64], lim[:64], want_times=False)  # warm-up
+    times = []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        bits, _ = eng.isochrones(org, lim, want_times=False)
+        times.append(time.perf_counter() - t0)
+        st = eng.last_stats(L)
+        log(f"isochrones {i}: {1e3 * times[-1]:.1f} ms wall, up {st.upward_time_ms:.1f} sweep "
+            f"{st.downward_time_ms:.1f} out {st.output_time_ms:.1f} ms, batches {st.batches}")
+    wall = float(np.median(times))
+    # bit-exact sample: the first lanes' times and bits against the oracle's seeded PHAST
+    k = 8
+    sb, stimes = eng.isochrones(org[:k], lim[:k], want_times=True)
+    exact = None
+    if not args.no_cpu:
+        sys.path.insert(0, REPO)
+        from oracle.oracle import Oracle
+        o = Oracle(g)
+        exact = True
+        for i in range(k):
+            row = np.full(g.n, INF_U32, dtype=np.uint32)
+            for r, d in o.seeded([(int(org[i]), 0)], int(lim[i])):
+                row[r] = d
+            exact &= bool(np.array_equal(stimes[i], row))
+            inside = np.zeros(((g.n + 63) // 64) * 64, dtype=bool)
+            inside[:g.n] = row != INF_U32
+            exact &= bool(np.array_equal(bits[i], np.packbits(inside, bitorder="little").view(np.uint64)))
+        log(f"isochrone sample of {k} lanes bit-exact vs the oracle: {exact}")
+    res = {
+        "metric": "isochrone node-lanes/s (block-gated, uneven limits, one call)",
+        "value": L * g.n / wall, "unit": "node-lanes/s", "n_gpus": 1, "steps": args.steps,
+        "ms_per_step": 1e3 * wall, "higher_is_better": True, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"{L} isochrones, limits from {{5,10,15,30,45,60}} min, uniform random origins; "
+                               f"{rows}x{rows} jittered grid + {int(chords * 100)}% chords, n={g.n}",
+                   "lanes": L, "output": "reach bits of every node per lane, copied to host"},
+        "phases_ms": {"up": st.upward_time_ms, "sweep": st.downward_time_ms, "output": st.output_time_ms,
+                      "batches": st.batches},
+        "bit_exact_sample": exact,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,6 +315,9 @@ def main():
                          "three loads, served by the continuous-batching loop and as plain per-request calls "
                          "(p50/p99 latency, throughput, bit-exact per request); prints its own JSON line")
     ap.add_argument("--serving-requests", type=int, default=100, help="requests per load level")
+    ap.add_argument("--isochrones", type=int, default=0, metavar="L",
+                    help="instead of the matrix line: L block-gated isochrone sweeps in one call, limits drawn "
+                         "from {5,10,15,30,45,60} min (uneven per lane); prints its own JSON line")
     ap.add_argument("--traffic-summary", nargs=4, default=None,
                     metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="print the DOWN-rows kernel's HBM bytes per launch as JSON: its FETCH_SIZE and "
@@ -304,6 +358,9 @@ def main():
     log(f"context ready in {time.perf_counter() - t0:.1f}s: {info}")
     if args.serving:
         serving_bench(eng, g, args, rows, chords)
+        return
+    if args.isochrones:
+        isochrone_bench(eng, g, args, rows, chords)
         return
 
     d_src = DeviceArray.from_host(src)

@@ -1566,39 +1566,41 @@ int stage_path_elements(bfg_ctx *c) {
 // by original lane.
 int dense_rows(bfg_ctx *c, const Req &req, uint32_t *dist_host, uint64_t *bits_host) {
     const uint32_t n = c->L.n;
+    const uint32_t L = (uint32_t)req.limit.size();
     const uint64_t words = (n + 63) / 64;
     DevBuf<uint32_t> d_rows, d_all_pos;
     DevBuf<unsigned long long> d_bits;
     std::vector<uint32_t> all_pos(c->L.pos_of_rank);
     HIP_TRY(d_all_pos.upload(all_pos, c->stream));
+    // reach bits of every lane land in request order on the device (rows
+    // through row_map) and come back in one copy
+    if (bits_host) HIP_TRY(d_bits.alloc((size_t)L * words));
     HIP_TRY(hipStreamSynchronize(c->stream));
     std::vector<uint32_t> hrow;
-    std::vector<uint64_t> hbits;
     uint32_t cap = 0;
-    return run_all(c, req, [&](uint32_t b0, uint32_t nl, const std::vector<uint32_t> &rows, float *oms) -> int {
+    int rc = run_all(c, req, [&](uint32_t b0, uint32_t nl, const std::vector<uint32_t> &rows, float *oms) -> int {
         (void)b0;
-        if (cap < nl) {
-            if (dist_host) HIP_TRY(d_rows.alloc((size_t)nl * n));
-            if (bits_host) HIP_TRY(d_bits.alloc((size_t)nl * words));
+        if (dist_host && cap < nl) {
+            HIP_TRY(d_rows.alloc((size_t)nl * n));
             cap = nl;
         }
-        int rc = gather_out(c, nl, d_all_pos.p, n, nullptr, dist_host ? d_rows.p : nullptr, n,
-                            bits_host ? d_bits.p : nullptr, words, oms);
-        if (rc) return rc;
-        if (dist_host) {
+        if (dist_host) {  // times: this batch's rows, in sorted lane order
+            int rc = gather_out(c, nl, d_all_pos.p, n, nullptr, d_rows.p, n, nullptr, 0, oms);
+            if (rc) return rc;
             hrow.resize((size_t)nl * n);
             HIP_TRY(hipMemcpy(hrow.data(), d_rows.p, (size_t)nl * n * 4, hipMemcpyDeviceToHost));
             for (uint32_t i = 0; i < nl; ++i)
                 memcpy(dist_host + (size_t)rows[i] * n, hrow.data() + (size_t)i * n, (size_t)n * 4);
         }
         if (bits_host) {
-            hbits.resize((size_t)nl * words);
-            HIP_TRY(hipMemcpy(hbits.data(), d_bits.p, (size_t)nl * words * 8, hipMemcpyDeviceToHost));
-            for (uint32_t i = 0; i < nl; ++i)
-                memcpy(bits_host + (size_t)rows[i] * words, hbits.data() + (size_t)i * words, (size_t)words * 8);
+            int rc = gather_out(c, nl, d_all_pos.p, n, c->row_map.p, nullptr, n, d_bits.p, words, oms);
+            if (rc) return rc;
         }
         return BFG_SUCCESS;
     });
+    if (rc) return rc;
+    if (bits_host) HIP_TRY(hipMemcpy(bits_host, d_bits.p, (size_t)L * words * 8, hipMemcpyDeviceToHost));
+    return BFG_SUCCESS;
 }
 
 void fill_info(const bfg::Layout &L, bfg_ctx_info *info) {
