@@ -303,6 +303,8 @@ def main():
     ap.add_argument("--lane-capacity", type=int, default=0, help="sources per device batch (0 = auto)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of single-thread CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="choose K and the DOWN unit size on this workload before the timed steps")
     ap.add_argument("--graph-cache", default=os.environ.get("BFG_GRAPH_CACHE", ""))
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -370,6 +372,14 @@ def main():
     def step():
         return eng.compute_matrix_device(d_src.ptr, S, d_tgt.ptr, T, d_out.ptr)
 
+    tuned = None
+    if args.autotune:
+        from bfgpu.autotune import autotune_matrix
+        best, timings = autotune_matrix(eng, d_src.ptr, S, d_tgt.ptr, T, d_out.ptr, log=log)
+        tuned = {"lanes": best[0], "plan_rows": best[1],
+                 "candidates_ms": {f"K={k},rows={r}": round(v, 2) for (k, r), v in timings.items()}}
+        log(f"autotune chose K={best[0]} plan_rows={best[1]}")
+
     for i in range(args.warmup):
         st = step()
         log(f"warmup {i}: up {st.upward_time_ms:.1f} ms sweep {st.downward_time_ms:.1f} ms "
@@ -432,7 +442,7 @@ def main():
                                f"{' + 2% motorway chords' if chords else ''}, edge-based CCH n={n}",
                    "sources": S, "targets": T, "graph_seed": args.seed, "lanes": KL,
                    "lane_capacity": cap, "n_nodes": n, "m_up": info["m_up"], "m_down": m_dn,
-                   "etree_depth": info["etree_depth"], "parallelism": "single GPU"},
+                   "etree_depth": info["etree_depth"], "parallelism": "single GPU", "autotune": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,

@@ -109,6 +109,11 @@ class BatchedPhastEngine:
     def configure(self, lanes=0, lane_capacity=0):
         _check(self._lib.bfg_ctx_configure(self._h, lanes, lane_capacity))
 
+    def tune(self, plan_rows=0, tri_scalar_segs=None):
+        """DOWN unit size and the level width from which intra-segment blocks
+        are scalar-loaded (bfg_ctx_tune); None / 0 keep the defaults."""
+        _check(self._lib.bfg_ctx_tune(self._h, plan_rows, 0 if tri_scalar_segs is None else tri_scalar_segs + 1))
+
     def rank_to_filtered(self):
         return self.graph.rank_to_filtered
 

@@ -1033,6 +1033,8 @@ struct bfg_ctx {
     // (segment | UNIT_WHOLE, rows) by level
     uint32_t plan_wgs = 0;
     std::vector<uint32_t> plan_lvl;
+    // tuning set by bfg_ctx_tune (0: the defaults or their environment overrides)
+    uint32_t plan_rows = 0, tri_segs = 0;
     std::vector<uint64_t> lvl_dom_bytes;  // algorithmic bytes of a level's DOWN rows per wave group
     DevBuf<uint32_t> du_seg, du_r0, du_r1, du_w;
     DevBuf<uint8_t> seg_split;  // 1 where the plan merges a segment's rows with atomics
@@ -1151,7 +1153,7 @@ int plan_down(bfg_ctx *c, uint32_t wgs) {
     if (c->plan_wgs == wgs) return BFG_SUCCESS;
     const bfg::Layout &L = c->L;
     const uint32_t nsl = (uint32_t)L.seglvl_seg.size() - 1;
-    const uint64_t rmax = plan_max_rows();
+    const uint64_t rmax = c->plan_rows ? c->plan_rows : plan_max_rows();
     if (L.seg_pos.size() - 1 > UNIT_SEG_MASK) return fail(BFG_NOT_SUPPORTED, "too many segments for the DOWN plan");
     std::vector<uint32_t> us, u0, u1, uw;
     std::vector<uint8_t> split(L.seg_pos.size() - 1, 0);
@@ -1234,7 +1236,7 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
     const Reach R{c->seed_off.p, c->seed_pre.p, c->pos_pre.p, c->pos_sub.p, nl};
     constexpr uint32_t WPB = TILE_THREADS / 64;  // wave groups per workgroup
     const uint32_t wgx = (wgs + WPB - 1) / WPB;  // workgroups across the wave groups
-    const uint32_t tss = tri_scalar_segs();
+    const uint32_t tss = c->tri_segs ? c->tri_segs - 1 : tri_scalar_segs();
     if (nseg) {
         hipLaunchKernelGGL(fill_reach_kernel<KL>, dim3(wgs, (nseg + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, nseg,
                            c->seg_pos.p, R, c->arena.p, n);
@@ -1755,6 +1757,17 @@ int bfg_ctx_get_info(const bfg_ctx *c, bfg_ctx_info *info) {
 int bfg_ctx_last_timings(const bfg_ctx *c, bfg_timings *t) {
     if (!c || !t) return fail(BFG_INVALID_PARAMETER, "null argument");
     *t = c->last;
+    return BFG_SUCCESS;
+}
+
+int bfg_ctx_tune(bfg_ctx *c, uint32_t plan_rows, uint32_t tri_scalar_segs) {
+    if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
+    if (plan_rows != 0 && (plan_rows < PLAN_MIN_ROWS || plan_rows >= (1u << 24)))
+        return fail(BFG_INVALID_PARAMETER, "plan_rows out of range");
+    std::lock_guard<std::mutex> g(c->mu);
+    c->plan_rows = plan_rows;
+    c->tri_segs = tri_scalar_segs;
+    c->plan_wgs = 0;  // the DOWN plan is rebuilt by the next call
     return BFG_SUCCESS;
 }
 

@@ -103,6 +103,13 @@ int bfg_ctx_last_timings(const bfg_ctx *ctx, bfg_timings *t);
  * Reallocates the arenas; used by the autotuner. */
 int bfg_ctx_configure(bfg_ctx *ctx, uint32_t lanes, uint32_t lane_capacity);
 
+/* Tuning of the downward sweep (no counterpart in the reference; set by the
+ * autotuner, bfgpu/autotune.py): plan_rows = the most rows one DOWN unit
+ * takes (>= 64; 0 = default 256 or BFG_PLAN_MAX_ROWS), tri_scalar_segs =
+ * 1 + the most segments a DOWN level may have and still use the vector-loaded
+ * intra-segment blocks (0 = default 64 or BFG_TRI_SCALAR_SEGS). */
+int bfg_ctx_tune(bfg_ctx *ctx, uint32_t plan_rows, uint32_t tri_scalar_segs);
+
 /* Host-only check of a CCH before upload (no device is touched): derives the
  * device layout, verifies that every finite arc joins an elimination-tree
  * ancestor/descendant pair and that the sweep schedule respects every arc,

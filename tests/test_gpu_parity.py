@@ -302,3 +302,26 @@ def test_reverse_phast_and_arrive_isochrones(engine_cls, grid_config1):
         inside = np.nonzero(exp <= lim[i])[0]
         assert np.array_equal(np.nonzero(np.unpackbits(bits[i].view(np.uint8), bitorder="little")[:g.n])[0], inside)
     assert np.array_equal(rev.reshape(tg.size, s.size).T, fwd.reshape(s.size, tg.size))
+
+
+def test_autotune_and_tuning_keep_results(engine_cls, grid_config1):
+    """The autotuner's candidates (K, DOWN unit size) and the scalar-block
+    threshold change timings only: the matrix stays bit-exact."""
+    from bfgpu.autotune import autotune_matrix
+    from bfgpu.hip import DeviceArray
+    g = grid_config1
+    rng = np.random.default_rng(12)
+    s = rng.integers(0, g.n, 150).astype(np.uint32)
+    t = rng.integers(0, g.n, 120).astype(np.uint32)
+    exp = Oracle(g).compute_matrix_flat_mt(s, t)
+    say("autotune 150x120")
+    with engine_cls(g) as e:
+        ds, dt = DeviceArray.from_host(s), DeviceArray.from_host(t)
+        dout = DeviceArray(s.size * t.size, np.uint32)
+        best, timings = autotune_matrix(e, ds.ptr, s.size, dt.ptr, t.size, dout.ptr, lanes=(64, 16),
+                                        plan_rows=(64, 512), reps=1)
+        assert len(timings) == 4 and e.info()["lanes"] == best[0]
+        assert np.array_equal(dout.to_host(), exp)
+        e.tune(plan_rows=64, tri_scalar_segs=0)
+        got, _ = e.compute_matrix_flat(s, t)
+        assert np.array_equal(got, exp)
