@@ -229,6 +229,7 @@ __device__ __forceinline__ void relax_arcs(uint32_t *L, uint32_t d, uint32_t a0,
 // ---------------------------------------------------------------------------
 constexpr int TILE_THREADS = 256;  // four wavefronts: four wave groups on one tile
 constexpr uint32_t MAX_GRID_Y = 65535;  // most workgroups along y in one launch
+constexpr uint32_t XCDS = 8;            // MI355X: accelerator complex dies, each with its own L2
 
 // Read-only tables the kernels index with wave-uniform offsets: read through
 // the constant address space, whose uniform loads become scalar loads.
@@ -768,12 +769,16 @@ __global__ __launch_bounds__(NT) void bottom_path_kernel(
     uint32_t *__restrict__ arena, uint32_t n) {
     constexpr int NS = NT / KL;
     const uint32_t lane = threadIdx.x % KL, slot = slot_id<KL>();
-    // block b: group b % groups, paths (b / groups) * NS + slot, so the
-    // groups of one path run side by side and share its arcs in L2
-    const uint32_t groups = (n_lanes + KL - 1) / KL;
-    const uint32_t g = blockIdx.x % groups;
-    const uint32_t k = blockIdx.x / groups * NS + slot;
-    if (k >= n_paths) return;  // this kernel has no barrier
+    // Workgroups go to the XCDs round robin (XCD b % XCDS).  XCD x walks the
+    // groups g = x (mod XCDS), GPX of them side by side on each path chunk,
+    // chunk after chunk: one XCD's L2 then holds the top labels of its own
+    // groups only, for the few neighbouring paths in flight.  Blocks past the
+    // last group (the padding of GPX) return at once.
+    const uint32_t groups = (n_lanes + KL - 1) / KL, gpx = (groups + XCDS - 1) / XCDS;
+    const uint32_t x = blockIdx.x % XCDS, j = blockIdx.x / XCDS;
+    const uint32_t g = x + XCDS * (j % gpx);
+    const uint32_t k = j / gpx * NS + slot;
+    if (g >= groups || k >= n_paths) return;  // this kernel has no barrier
     uint32_t *lab = arena + (size_t)g * n * KL + lane;
     uint32_t sj0 = 0, sj1 = 0;
     const uint32_t gl = g * KL + lane;
@@ -1298,7 +1303,8 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
         if (c->n_paths) {
             rc = stage_path_elements(c);
             if (rc) return rc;
-            hipLaunchKernelGGL(bottom_path_kernel<KL>, dim3((c->n_paths + NSP - 1) / NSP * groups), dim3(NT), 0, s,
+            const uint32_t gpx = (groups + XCDS - 1) / XCDS;
+            hipLaunchKernelGGL(bottom_path_kernel<KL>, dim3((c->n_paths + NSP - 1) / NSP * gpx * XCDS), dim3(NT), 0, s,
                                c->n_paths, c->path_off.p, c->path_pos.p, L.n_top, c->in_off.p, c->in_split.p,
                                c->in_dd.p, c->in_src.p, c->in_w.p, c->pos_pre.p, c->pos_sub.p, c->pos_depth.p,
                                c->seed_off.p, c->seed_pre.p, c->seed_depth.p, c->pathbuf.p, D1, nl, c->arena.p, n);
