@@ -195,6 +195,45 @@ class BatchedPhastEngine:
                                           _native.ptr(thr), _native.ptr(out)))
         return out
 
+    def attach_length(self, up_len, dn_len):
+        """Length-along-time weights, one per CCH arc like the graph's up_w /
+        dn_w (the reference's cch.lat weights, server/state.rs:2880-2891), for
+        the two-channel calls."""
+        up_len, dn_len = _u32(up_len), _u32(dn_len)
+        if up_len.size != self.graph.m_up or dn_len.size != self.graph.m_down:
+            raise ValueError("one length weight per arc")
+        d = self.graph.desc()
+        _check(self._lib.bfg_ctx_attach_length(self._h, C.byref(d), _native.ptr(up_len), _native.ptr(dn_len)))
+
+    def phast_seeded_2ch(self, seed_lists, thresholds):
+        """Many run_phast_bounded_fast_seeded_2ch calls (isochrone_handler.rs:
+        500-625) in one launch; seeds are (rank, time, len).  Returns dense
+        (times [L, n], lengths [L, n]), INF beyond each lane's limit."""
+        L = len(seed_lists)
+        off = np.zeros(L + 1, dtype=np.uint32)
+        ranks, ts, ls = [], [], []
+        for i, seeds in enumerate(seed_lists):
+            for r, t, ln in seeds:
+                ranks.append(r)
+                ts.append(t)
+                ls.append(ln)
+            off[i + 1] = len(ranks)
+        ranks, ts, ls, thr = _u32(ranks), _u32(ts), _u32(ls), _u32(thresholds)
+        if thr.size != L:
+            raise ValueError("one threshold per lane")
+        times = np.empty((L, self.n_nodes), dtype=np.uint32)
+        lens = np.empty((L, self.n_nodes), dtype=np.uint32)
+        _check(self._lib.bfg_phast_seeded_2ch(self._h, _native.ptr(off), _native.ptr(ranks), _native.ptr(ts),
+                                              _native.ptr(ls), L, _native.ptr(thr), _native.ptr(times),
+                                              _native.ptr(lens)))
+        return times, lens
+
+    def run_phast_bounded_fast_seeded_2ch(self, seeds, threshold):
+        """isochrone_handler.rs:500-509: Vec<(rank, time, len)> ascending rank, time <= threshold."""
+        t, ln = self.phast_seeded_2ch([list(seeds)], [threshold])
+        idx = np.nonzero(t[0] != INF)[0]
+        return [(int(r), int(t[0][r]), int(ln[0][r])) for r in idx]
+
     def table_seeded_bounded_routed(self, src_seedsets, tgt_seedsets, threshold=INF):
         """bucket_ch.rs:6535-6574 on its PHAST path, table_phast_lopsided
         (:6717-6853): one seeded field per source, cell = min over the target's

@@ -1015,6 +1015,29 @@ bfg::LayoutParams layout_params() {
     return lp;
 }
 
+// Two-channel sweeps (time, length along the time-optimal path): arcs as
+// packed (time << 32 | length) weights in pull and push lists of their own,
+// built by bfg_ctx_attach_length.
+struct TwoChannel {
+    // top region: per segment s, DOWN arcs from outside [x_off[s], x_off[s+1])
+    // grouped by source, x_sj = source position << 6 | head offset in the
+    // segment; per top head position p, arcs from its own segment
+    // [i_off[p], i_off[p+1]), i_k = tail offset in the segment
+    DevBuf<uint32_t> x_off, x_sj, i_off, i_k;
+    DevBuf<uint64_t> x_w, i_w;
+    // bottom heads p >= n_top: DOWN in-arcs [b_off[p - n_top], ..), sources b_src
+    DevBuf<uint32_t> b_off, b_src;
+    DevBuf<uint64_t> b_w;
+    // UP arcs by tail position (targets are positions)
+    DevBuf<uint32_t> u_off, u_tgt;
+    DevBuf<uint64_t> u_w;
+    DevBuf<uint32_t> pos_of_rank;
+    // per call: labels [group][position][64 lanes], seeds, limits, outputs
+    DevBuf<uint64_t> lab, s_lab;
+    DevBuf<uint32_t> s_off, s_pos, cur, lim, o_t, o_l;
+    uint64_t bytes = 0;
+};
+
 }  // namespace
 
 struct bfg_ctx {
@@ -1066,6 +1089,7 @@ struct bfg_ctx {
 
     bfg_timings last{};
     uint64_t graph_bytes = 0;
+    std::unique_ptr<TwoChannel> t2;  // set by bfg_ctx_attach_length
 };
 
 namespace {
@@ -1611,6 +1635,335 @@ int dense_rows(bfg_ctx *c, const Req &req, uint32_t *dist_host, uint64_t *bits_h
     return BFG_SUCCESS;
 }
 
+// ---------------------------------------------------------------------------
+// Two-channel seeded PHAST (time, length along the time-optimal path), the
+// semantics of run_phast_bounded_fast_seeded_2ch (isochrone_handler.rs:500-625).
+// A label is (time << 32 | length): lexicographic order is integer order, so
+// the reference's "strictly faster, or equally fast and strictly shorter"
+// relaxation (:546-565, :586-602) is a min, and a node's final label is the
+// least (time, length) over the up-down paths from the lane's seeds.  Lanes
+// are grouped by 64; labels live in [group][position][64 lanes].  A lane
+// with limit T relaxes only from labels of time <= T; every label of time
+// <= T is exact, the rest are masked to INF on output.
+// ---------------------------------------------------------------------------
+constexpr uint64_t INF2 = ~0ull;
+
+__device__ __forceinline__ uint64_t add2(uint64_t a, uint64_t w) {
+    const uint32_t t = sat_add((uint32_t)(a >> 32), (uint32_t)(w >> 32));
+    const uint32_t l = sat_add((uint32_t)a, (uint32_t)w);
+    return (uint64_t)t << 32 | l;
+}
+
+// a label relaxes its arcs only when reached within the lane's limit
+__device__ __forceinline__ bool live2(uint64_t x, uint32_t thr) {
+    const uint32_t t = (uint32_t)(x >> 32);
+    return t != INF && t <= thr;
+}
+
+__global__ __launch_bounds__(NT) void t2_fill_kernel(uint64_t *__restrict__ p, size_t count) {
+    for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < count; i += (size_t)gridDim.x * NT) p[i] = INF2;
+}
+
+// One thread per lane: the lane's seeds (distinct positions), then its
+// upward phase (isochrone_handler.rs:527-568) as a walk over the union of the
+// seeds' root paths in descending position (an ancestor always has a smaller
+// position than its descendants, so every push into a node precedes its own
+// pushes).  cur[k] is the walker of seed k; walkers meeting at a node merge.
+// Only this thread touches the lane's labels in this kernel.
+__global__ __launch_bounds__(64) void t2_up_kernel(uint32_t nl, uint32_t n, const uint32_t *__restrict__ s_off,
+                                                   const uint32_t *__restrict__ s_pos,
+                                                   const uint64_t *__restrict__ s_lab, uint32_t *__restrict__ cur,
+                                                   const uint32_t *__restrict__ lim,
+                                                   const uint32_t *__restrict__ pos_parent,
+                                                   const uint32_t *__restrict__ u_off,
+                                                   const uint32_t *__restrict__ u_tgt,
+                                                   const uint64_t *__restrict__ u_w, uint64_t *__restrict__ lab) {
+    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= nl) return;
+    uint64_t *col = lab + (size_t)(i / 64) * n * 64 + (i % 64);
+    const uint32_t a = s_off[i], b = s_off[i + 1], thr = lim[i];
+    for (uint32_t k = a; k < b; ++k) {
+        col[(size_t)s_pos[k] * 64] = s_lab[k];
+        cur[k] = s_pos[k];
+    }
+    for (;;) {
+        uint32_t u = NONE;
+        for (uint32_t k = a; k < b; ++k)
+            if (cur[k] != NONE && (u == NONE || cur[k] > u)) u = cur[k];
+        if (u == NONE) break;
+        const uint64_t lu = col[(size_t)u * 64];
+        if (live2(lu, thr))
+            for (uint32_t e = u_off[u]; e < u_off[u + 1]; ++e) {
+                uint64_t &x = col[(size_t)u_tgt[e] * 64];
+                x = min(x, add2(lu, u_w[e]));
+            }
+        const uint32_t par = pos_parent[u];
+        for (uint32_t k = a; k < b; ++k)
+            if (cur[k] == u) cur[k] = par;
+    }
+}
+
+// Downward phase over one segment level of the top region: a workgroup per
+// (segment, lane group).  The four wavefronts split the segment's arcs from
+// outside (grouped by source, so a source label is loaded once per run) and
+// merge into the LDS tile with atomic min; wavefront 0 then walks the chain,
+// parent first, adding the arcs inside the segment.
+__global__ __launch_bounds__(NT) void t2_seg_kernel(uint32_t s0, uint32_t n, uint32_t nl,
+                                                    const uint32_t *__restrict__ seg_pos,
+                                                    const uint32_t *__restrict__ x_off,
+                                                    const uint32_t *__restrict__ x_sj,
+                                                    const uint64_t *__restrict__ x_w,
+                                                    const uint32_t *__restrict__ i_off,
+                                                    const uint32_t *__restrict__ i_k,
+                                                    const uint64_t *__restrict__ i_w,
+                                                    const uint32_t *__restrict__ lim, uint64_t *__restrict__ lab) {
+    __shared__ uint64_t acc[SEG][64];
+    const uint32_t s = s0 + blockIdx.x, g = blockIdx.y, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t gl = g * 64 + lane, thr = gl < nl ? lim[gl] : 0;
+    const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
+    uint64_t *base = lab + (size_t)g * n * 64 + lane;
+    for (uint32_t j = w; j < len; j += NT / 64) acc[j][lane] = base[(size_t)(p0 + j) * 64];
+    __syncthreads();
+    const uint32_t a0 = x_off[s], a1 = x_off[s + 1], per = (a1 - a0 + NT / 64 - 1) / (NT / 64);
+    const uint32_t e0 = min(a1, a0 + w * per), e1 = min(a1, e0 + per);
+    uint32_t last = NONE;
+    uint64_t ls = INF2;
+    for (uint32_t e = e0; e < e1; ++e) {
+        const uint32_t sj = x_sj[e], src = sj >> 6;
+        if (src != last) {
+            ls = base[(size_t)src * 64];
+            last = src;
+        }
+        if (live2(ls, thr)) atomicMin((unsigned long long *)&acc[sj & 63][lane], (unsigned long long)add2(ls, x_w[e]));
+    }
+    __syncthreads();
+    if (w != 0) return;
+    for (uint32_t i = 0; i < len; ++i) {
+        uint64_t v = acc[i][lane];
+        for (uint32_t e = i_off[p0 + i]; e < i_off[p0 + i + 1]; ++e) {
+            const uint64_t lk = acc[i_k[e]][lane];
+            if (live2(lk, thr)) v = min(v, add2(lk, i_w[e]));
+        }
+        acc[i][lane] = v;
+        base[(size_t)(p0 + i) * 64] = v;
+    }
+}
+
+// Downward phase over one height level of the bottom region: a wavefront per
+// (node, lane group) pulls the node's DOWN in-arcs.
+__global__ __launch_bounds__(NT) void t2_pull_kernel(uint32_t p0, uint32_t p1, uint32_t n, uint32_t n_top, uint32_t nl,
+                                                     const uint32_t *__restrict__ b_off,
+                                                     const uint32_t *__restrict__ b_src,
+                                                     const uint64_t *__restrict__ b_w,
+                                                     const uint32_t *__restrict__ lim, uint64_t *__restrict__ lab) {
+    const uint32_t p = p0 + blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    if (p >= p1) return;
+    const uint32_t lane = threadIdx.x & 63, g = blockIdx.y, gl = g * 64 + lane, thr = gl < nl ? lim[gl] : 0;
+    uint64_t *base = lab + (size_t)g * n * 64 + lane;
+    uint64_t v = base[(size_t)p * 64];
+    for (uint32_t e = b_off[p - n_top]; e < b_off[p - n_top + 1]; ++e) {
+        const uint64_t ls = base[(size_t)b_src[e] * 64];
+        if (live2(ls, thr)) v = min(v, add2(ls, b_w[e]));
+    }
+    base[(size_t)p * 64] = v;
+}
+
+// Output rows by rank: a workgroup per (64 ranks, lane group) transposes
+// through LDS; labels of time above the lane's limit (or unreached) are INF
+// in both channels (isochrone_handler.rs:605-622 keeps time <= threshold).
+__global__ __launch_bounds__(NT) void t2_out_kernel(uint32_t n, uint32_t nl, const uint32_t *__restrict__ pos_of_rank,
+                                                    const uint32_t *__restrict__ lim, const uint64_t *__restrict__ lab,
+                                                    uint32_t *__restrict__ o_t, uint32_t *__restrict__ o_l) {
+    __shared__ uint64_t tile[64][65];
+    const uint32_t v0 = blockIdx.x * 64, g = blockIdx.y, t = threadIdx.x;
+    const uint64_t *base = lab + (size_t)g * n * 64;
+    for (uint32_t r = t >> 6; r < 64; r += NT / 64)
+        if (v0 + r < n) tile[r][t & 63] = base[(size_t)pos_of_rank[v0 + r] * 64 + (t & 63)];
+    __syncthreads();
+    const uint32_t r = t & 63;
+    if (v0 + r >= n) return;
+    for (uint32_t l = t >> 6; l < 64; l += NT / 64) {
+        const uint32_t gl = g * 64 + l;
+        if (gl >= nl) break;
+        const uint64_t x = tile[r][l];
+        const uint32_t tm = (uint32_t)(x >> 32);
+        const bool ok = tm != INF && tm <= lim[gl];
+        o_t[(size_t)gl * n + v0 + r] = ok ? tm : INF;
+        o_l[(size_t)gl * n + v0 + r] = ok ? (uint32_t)x : INF;
+    }
+}
+
+// Builds the two-channel arc lists from the CSR the context was created from
+// and length weights aligned with it (arcs of INF time are dropped, as the
+// time flats drop them).
+int attach_length(bfg_ctx *c, const bfg_graph_desc *gd, const uint32_t *up_len, const uint32_t *dn_len) {
+    const bfg::Layout &L = c->L;
+    const uint32_t n = L.n, nt = L.n_top;
+    if (n >= (1u << 26)) return fail(BFG_NOT_SUPPORTED, "two-channel lists pack positions in 26 bits");
+    // the upward walk visits a lane's nodes in descending position: every
+    // ancestor must sit at a smaller position than its descendants
+    for (uint32_t p = 0; p < n; ++p)
+        if (L.pos_parent[p] != NONE && L.pos_parent[p] >= p)
+            return fail(BFG_INTERNAL_ERROR, "a parent follows its child in position order");
+    const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
+    for (uint32_t s = 0; s < nseg; ++s)
+        if (L.seg_pos[s + 1] - L.seg_pos[s] > SEG) return fail(BFG_INTERNAL_ERROR, "segment longer than a tile");
+    std::vector<uint32_t> seg_of(nt);
+    for (uint32_t s = 0; s < nseg; ++s)
+        for (uint32_t p = L.seg_pos[s]; p < L.seg_pos[s + 1]; ++p) seg_of[p] = s;
+    auto pack = [](uint32_t t, uint32_t l) { return (uint64_t)t << 32 | l; };
+    uint64_t mu = 0, md = 0;
+    for (uint64_t i = 0; i < gd->up_offsets[n]; ++i) mu += gd->up_weights[i] != INF;
+    for (uint64_t i = 0; i < gd->down_offsets[n]; ++i) md += gd->down_weights[i] != INF;
+    if (mu != L.m_up || md != L.m_down) return fail(BFG_INVALID_PARAMETER, "graph differs from the context's");
+    if ((mu && !up_len) || (md && !dn_len)) return fail(BFG_INVALID_PARAMETER, "null length weights");
+    // DOWN arcs: count per list, then fill with tails in ascending position so
+    // each segment's outside arcs come grouped by source
+    std::vector<uint32_t> x_off(nseg + 1, 0), i_off(nt + 1, 0), b_off(n - nt + 1, 0);
+    for (uint32_t r = 0; r < n; ++r) {
+        const uint32_t ps = L.pos_of_rank[r];
+        for (uint64_t i = gd->down_offsets[r]; i < gd->down_offsets[r + 1]; ++i) {
+            if (gd->down_weights[i] == INF) continue;
+            const uint32_t ph = L.pos_of_rank[gd->down_targets[i]];
+            if (ph >= nt) ++b_off[ph - nt + 1];
+            else if (ps < nt && seg_of[ps] == seg_of[ph]) ++i_off[ph + 1];
+            else ++x_off[seg_of[ph] + 1];
+        }
+    }
+    for (size_t k = 1; k < x_off.size(); ++k) x_off[k] += x_off[k - 1];
+    for (size_t k = 1; k < i_off.size(); ++k) i_off[k] += i_off[k - 1];
+    for (size_t k = 1; k < b_off.size(); ++k) b_off[k] += b_off[k - 1];
+    std::vector<uint32_t> x_sj(x_off[nseg]), i_k(i_off[nt]), b_src(b_off[n - nt]);
+    std::vector<uint64_t> x_w(x_sj.size()), i_w(i_k.size()), b_w(b_src.size());
+    {
+        std::vector<uint32_t> xc(x_off.begin(), x_off.end() - 1), ic(i_off.begin(), i_off.end() - 1),
+            bc(b_off.begin(), b_off.end() - 1);
+        for (uint32_t ps = 0; ps < n; ++ps) {
+            const uint32_t r = L.rank_of_pos[ps];
+            for (uint64_t i = gd->down_offsets[r]; i < gd->down_offsets[r + 1]; ++i) {
+                if (gd->down_weights[i] == INF) continue;
+                const uint32_t ph = L.pos_of_rank[gd->down_targets[i]];
+                const uint64_t w2 = pack(gd->down_weights[i], dn_len[i]);
+                if (ph >= nt) {
+                    const uint32_t k = bc[ph - nt]++;
+                    b_src[k] = ps;
+                    b_w[k] = w2;
+                } else if (ps < nt && seg_of[ps] == seg_of[ph]) {
+                    const uint32_t k = ic[ph]++;
+                    i_k[k] = ps - L.seg_pos[seg_of[ph]];
+                    i_w[k] = w2;
+                } else {
+                    const uint32_t k = xc[seg_of[ph]]++;
+                    x_sj[k] = ps << 6 | (ph - L.seg_pos[seg_of[ph]]);
+                    x_w[k] = w2;
+                }
+            }
+        }
+    }
+    // UP arcs by tail position
+    std::vector<uint32_t> u_off(n + 1, 0);
+    for (uint32_t r = 0; r < n; ++r) {
+        uint32_t k = 0;
+        for (uint64_t i = gd->up_offsets[r]; i < gd->up_offsets[r + 1]; ++i) k += gd->up_weights[i] != INF;
+        u_off[L.pos_of_rank[r] + 1] = k;
+    }
+    for (size_t k = 1; k < u_off.size(); ++k) u_off[k] += u_off[k - 1];
+    std::vector<uint32_t> u_tgt(u_off[n]);
+    std::vector<uint64_t> u_w(u_tgt.size());
+    for (uint32_t r = 0; r < n; ++r) {
+        uint32_t k = u_off[L.pos_of_rank[r]];
+        for (uint64_t i = gd->up_offsets[r]; i < gd->up_offsets[r + 1]; ++i) {
+            if (gd->up_weights[i] == INF) continue;
+            u_tgt[k] = L.pos_of_rank[gd->up_targets[i]];
+            u_w[k++] = pack(gd->up_weights[i], up_len[i]);
+        }
+    }
+    std::unique_ptr<TwoChannel> t(new TwoChannel());
+    hipStream_t s = c->stream;
+    HIP_TRY(t->x_off.upload(x_off, s));
+    HIP_TRY(t->x_sj.upload(x_sj, s));
+    HIP_TRY(t->x_w.upload(x_w, s));
+    HIP_TRY(t->i_off.upload(i_off, s));
+    HIP_TRY(t->i_k.upload(i_k, s));
+    HIP_TRY(t->i_w.upload(i_w, s));
+    HIP_TRY(t->b_off.upload(b_off, s));
+    HIP_TRY(t->b_src.upload(b_src, s));
+    HIP_TRY(t->b_w.upload(b_w, s));
+    HIP_TRY(t->u_off.upload(u_off, s));
+    HIP_TRY(t->u_tgt.upload(u_tgt, s));
+    HIP_TRY(t->u_w.upload(u_w, s));
+    HIP_TRY(t->pos_of_rank.upload(L.pos_of_rank, s));
+    HIP_TRY(hipStreamSynchronize(s));  // the host vectors die with this frame
+    t->bytes = 4 * (x_off.size() + x_sj.size() + i_off.size() + i_k.size() + b_off.size() + b_src.size() +
+                    u_off.size() + u_tgt.size() + n) +
+               8 * (x_w.size() + i_w.size() + b_w.size() + u_w.size());
+    c->t2 = std::move(t);
+    return BFG_SUCCESS;
+}
+
+// Runs lanes [0, nl) of a two-channel request in batches of 64-lane groups;
+// seeds per lane are distinct positions with their (time, length) labels.
+int phast_2ch(bfg_ctx *c, const std::vector<uint32_t> &soff, const std::vector<uint32_t> &spos,
+              const std::vector<uint64_t> &slab, const uint32_t *limits, uint32_t nl, uint32_t *times,
+              uint32_t *lens) {
+    TwoChannel &t = *c->t2;
+    const bfg::Layout &L = c->L;
+    const uint32_t n = L.n;
+    hipStream_t s = c->stream;
+    // groups per batch: labels and both output channels fit a share of free memory
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
+    const size_t per_group = (size_t)n * 64 * 16;
+    const uint32_t want = (nl + 63) / 64;
+    const uint32_t gmax = (uint32_t)std::max<size_t>(1, std::min<size_t>(64, (free_b + t.lab.n * 8) / 3 / per_group));
+    const uint32_t G = std::min(want, gmax);
+    if (t.lab.n < (size_t)G * n * 64) {
+        t.lab.release();
+        t.o_t.release();
+        t.o_l.release();
+        HIP_TRY(t.lab.alloc((size_t)G * n * 64));
+        HIP_TRY(t.o_t.alloc((size_t)G * 64 * n));
+        HIP_TRY(t.o_l.alloc((size_t)G * 64 * n));
+    }
+    for (uint32_t l0 = 0; l0 < nl; l0 += G * 64) {
+        const uint32_t bl = std::min(nl - l0, G * 64), bg = (bl + 63) / 64;
+        // this batch's seeds, offsets rebased
+        std::vector<uint32_t> off(bl + 1);
+        for (uint32_t i = 0; i <= bl; ++i) off[i] = soff[l0 + i] - soff[l0];
+        const uint32_t k0 = soff[l0], k1 = soff[l0 + bl];
+        std::vector<uint32_t> pos(spos.begin() + k0, spos.begin() + k1), lim(limits + l0, limits + l0 + bl);
+        std::vector<uint64_t> lb(slab.begin() + k0, slab.begin() + k1);
+        HIP_TRY(t.s_off.upload(off, s));
+        HIP_TRY(t.s_pos.upload(pos, s));
+        HIP_TRY(t.s_lab.upload(lb, s));
+        HIP_TRY(t.lim.upload(lim, s));
+        HIP_TRY(t.cur.alloc(std::max<size_t>(pos.size(), 1)));
+        const size_t cells = (size_t)bg * n * 64;
+        t2_fill_kernel<<<(uint32_t)std::min<size_t>((cells + NT - 1) / NT, 1u << 20), NT, 0, s>>>(t.lab.p, cells);
+        t2_up_kernel<<<bg, 64, 0, s>>>(bl, n, t.s_off.p, t.s_pos.p, t.s_lab.p, t.cur.p, t.lim.p, c->pos_parent.p,
+                                       t.u_off.p, t.u_tgt.p, t.u_w.p, t.lab.p);
+        for (size_t lv = 0; lv + 1 < L.seglvl_seg.size(); ++lv) {
+            const uint32_t s0 = L.seglvl_seg[lv], s1 = L.seglvl_seg[lv + 1];
+            if (s1 > s0)
+                t2_seg_kernel<<<dim3(s1 - s0, bg), NT, 0, s>>>(s0, n, bl, c->seg_pos.p, t.x_off.p, t.x_sj.p,
+                                                               t.x_w.p, t.i_off.p, t.i_k.p, t.i_w.p, t.lim.p, t.lab.p);
+        }
+        for (size_t lv = 0; lv + 1 < L.botlvl_pos.size(); ++lv) {
+            const uint32_t p0 = L.botlvl_pos[lv], p1 = L.botlvl_pos[lv + 1];
+            if (p1 > p0)
+                t2_pull_kernel<<<dim3((p1 - p0 + NT / 64 - 1) / (NT / 64), bg), NT, 0, s>>>(
+                    p0, p1, n, L.n_top, bl, t.b_off.p, t.b_src.p, t.b_w.p, t.lim.p, t.lab.p);
+        }
+        t2_out_kernel<<<dim3((n + 63) / 64, bg), NT, 0, s>>>(n, bl, t.pos_of_rank.p, t.lim.p, t.lab.p, t.o_t.p,
+                                                            t.o_l.p);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(times + (size_t)l0 * n, t.o_t.p, (size_t)bl * n * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(lens + (size_t)l0 * n, t.o_l.p, (size_t)bl * n * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    return BFG_SUCCESS;
+}
+
 void fill_info(const bfg::Layout &L, bfg_ctx_info *info) {
     memset(info, 0, sizeof(*info));
     info->n_nodes = L.n;
@@ -1897,6 +2250,55 @@ int bfg_phast_seeded(bfg_ctx *c, const uint32_t *seed_offsets, const uint32_t *s
     req.scost.assign(seed_costs, seed_costs + ns);
     req.limit.assign(thresholds, thresholds + L);
     return dense_rows(c, req, dist, nullptr);
+}
+
+int bfg_ctx_attach_length(bfg_ctx *c, const bfg_graph_desc *gd, const uint32_t *up_len, const uint32_t *down_len) {
+    if (!c || !gd) return fail(BFG_INVALID_PARAMETER, "null argument");
+    if (gd->n_nodes != c->L.n) return fail(BFG_INVALID_PARAMETER, "graph differs from the context's");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    c->t2.reset();
+    return attach_length(c, gd, up_len, down_len);
+}
+
+int bfg_phast_seeded_2ch(bfg_ctx *c, const uint32_t *seed_offsets, const uint32_t *seed_ranks,
+                         const uint32_t *seed_times, const uint32_t *seed_lens, uint32_t L,
+                         const uint32_t *thresholds, uint32_t *times, uint32_t *lens) {
+    if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
+    if (L && (!seed_offsets || !thresholds || !times || !lens)) return fail(BFG_INVALID_PARAMETER, "null buffer");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    if (!c->t2) return fail(BFG_INVALID_PARAMETER, "no length weights attached (bfg_ctx_attach_length)");
+    if (L == 0) return BFG_SUCCESS;
+    if (seed_offsets[0] != 0) return fail(BFG_INVALID_PARAMETER, "seed_offsets[0] must be 0");
+    for (uint32_t i = 0; i < L; ++i)
+        if (seed_offsets[i + 1] < seed_offsets[i]) return fail(BFG_INVALID_PARAMETER, "seed_offsets not monotone");
+    const uint32_t ns = seed_offsets[L];
+    if (ns && (!seed_ranks || !seed_times || !seed_lens)) return fail(BFG_INVALID_PARAMETER, "null seed buffer");
+    int rc = check_ranks(c, seed_ranks, ns, "seed");
+    if (rc) return rc;
+    // isochrone_handler.rs:522-526: a seed applies while strictly faster than
+    // what its rank holds, so per rank the first of the fastest seeds wins
+    std::vector<uint32_t> soff(1, 0), spos;
+    std::vector<uint64_t> slab;
+    for (uint32_t i = 0; i < L; ++i) {
+        const size_t first = spos.size();
+        for (uint32_t k = seed_offsets[i]; k < seed_offsets[i + 1]; ++k) {
+            if (seed_times[k] == INF) continue;
+            const uint32_t p = c->L.pos_of_rank[seed_ranks[k]];
+            const uint64_t x = (uint64_t)seed_times[k] << 32 | seed_lens[k];
+            size_t j = first;
+            while (j < spos.size() && spos[j] != p) ++j;
+            if (j == spos.size()) {
+                spos.push_back(p);
+                slab.push_back(x);
+            } else if (seed_times[k] < (uint32_t)(slab[j] >> 32)) {
+                slab[j] = x;
+            }
+        }
+        soff.push_back((uint32_t)spos.size());
+    }
+    return phast_2ch(c, soff, spos, slab, thresholds, L, times, lens);
 }
 
 int bfg_table_seeded(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t *src_ranks, const uint32_t *src_parts,

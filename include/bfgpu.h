@@ -176,6 +176,28 @@ int bfg_table_seeded(bfg_ctx *ctx, const uint32_t *src_offsets, const uint32_t *
                      const uint32_t *tgt_ranks, const uint32_t *tgt_parts, uint32_t n_tgt,
                      uint32_t threshold, uint32_t *out, uint8_t *conflicts);
 
+/* Length-along-time weights for the two-channel calls: the CchWeights the
+ * reference loads beside the time weights (cch.lat.<mode>.u32,
+ * server/state.rs:2880-2891) and reads through flats that share the time
+ * flats' topology (state.rs:157-165, isochrone_handler.rs:493-499).
+ * up_len / down_len hold one weight per arc of `graph`, which must describe
+ * the CCH the context was created from; arcs of INF time are dropped with
+ * their length.  Replaces any lengths attached before. */
+int bfg_ctx_attach_length(bfg_ctx *ctx, const bfg_graph_desc *graph, const uint32_t *up_len,
+                          const uint32_t *down_len);
+
+/* run_phast_bounded_fast_seeded_2ch (isochrone_handler.rs:500-625) for n_lanes
+ * independent queries: lane i has seeds (seed_ranks[k], seed_times[k],
+ * seed_lens[k]) for k in [seed_offsets[i], seed_offsets[i+1]) and time limit
+ * thresholds[i].  times[i * n_nodes + v] is the bounded time of the PHAST
+ * above and lens[i * n_nodes + v] the length along it: the least length over
+ * the fastest up-down paths, with per-rank seeds taken as the reference takes
+ * them (the first of the fastest, :522-526); both BFG_INF where the time
+ * exceeds the limit.  Needs bfg_ctx_attach_length. */
+int bfg_phast_seeded_2ch(bfg_ctx *ctx, const uint32_t *seed_offsets, const uint32_t *seed_ranks,
+                         const uint32_t *seed_times, const uint32_t *seed_lens, uint32_t n_lanes,
+                         const uint32_t *thresholds, uint32_t *times, uint32_t *lens);
+
 #ifdef __cplusplus
 }
 #endif

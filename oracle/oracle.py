@@ -67,6 +67,9 @@ def lib():
         _lib.orc_seeded_state_free.restype = None
         _lib.orc_phast_seeded.argtypes = [C.c_void_p, fp, fp, u32p, u32p, C.c_uint32, C.c_uint32, u32p, u32p]
         _lib.orc_phast_seeded.restype = C.c_uint64
+        _lib.orc_phast_seeded_2ch.argtypes = [C.c_void_p, fp, fp, fp, fp, u32p, u32p, u32p, C.c_uint32,
+                                              C.c_uint32, u32p, u32p, u32p]
+        _lib.orc_phast_seeded_2ch.restype = C.c_uint64
     return _lib
 
 
@@ -233,6 +236,99 @@ class Oracle:
                     best = INF
                 cells[i * T + j] = best
         return cells, conf
+
+    def attach_length(self, up_len, dn_len):
+        """Length-along-time weights, one per CCH arc like up_w / dn_w.  The
+        reference builds their flats from the length weights
+        (UpAdjFlat::build / DownAdjFlat::build, server/state.rs:178-183,
+        :2890-2891) and indexes them with the time flats' arc index, so both
+        must drop the same arcs."""
+        self.seeded([], INF)  # builds the time flats and the state
+        lg = _Cch(self.n, self._c.up_off, self._c.up_tgt, _p(_u32(up_len)), self._c.dn_off, self._c.dn_tgt,
+                  _p(_u32(dn_len)))
+        self._len_src = [_u32(up_len), _u32(dn_len)]
+        lg.up_w, lg.dn_w = _p(self._len_src[0]), _p(self._len_src[1])
+        saved = self._c
+        self._c = lg
+        try:
+            luo, lut, luw = self.up_flat()
+            ldo, ldt, ldw = self.down_flat()
+        finally:
+            self._c = saved
+        uo, ut, _, do, dt, _ = self._flats
+        if not (np.array_equal(luo, uo) and np.array_equal(lut, ut) and np.array_equal(ldo, do)
+                and np.array_equal(ldt, dt)):
+            raise ValueError("length flats do not share the time flats' topology")
+        self._lflats = [luw, ldw]
+        self._flu = _Flat(self.n, _p(uo, u64p), _p(ut), _p(luw))
+        self._fld = _Flat(self.n, _p(do, u64p), _p(dt), _p(ldw))
+
+    def seeded_2ch(self, seeds, threshold):
+        """run_phast_bounded_fast_seeded_2ch (isochrone_handler.rs:500-625):
+        seeds (rank, time, len) -> list of (rank, time, len), time <= threshold."""
+        r = _u32([s[0] for s in seeds])
+        t = _u32([s[1] for s in seeds])
+        ln = _u32([s[2] for s in seeds])
+        orank = np.empty(self.n, dtype=np.uint32)
+        otime = np.empty(self.n, dtype=np.uint32)
+        olen = np.empty(self.n, dtype=np.uint32)
+        cnt = self.L.orc_phast_seeded_2ch(self._st, C.byref(self._fu), C.byref(self._fd), C.byref(self._flu),
+                                          C.byref(self._fld), _p(r), _p(t), _p(ln), r.size, threshold,
+                                          _p(orank), _p(otime), _p(olen))
+        return list(zip(orank[:cnt].tolist(), otime[:cnt].tolist(), olen[:cnt].tolist()))
+
+    def seeded_2ch_rows(self, seeds, threshold):
+        """seeded_2ch as dense (time, len) rows over all ranks, INF where unsettled."""
+        tr = np.full(self.n, INF, dtype=np.uint32)
+        lr = np.full(self.n, INF, dtype=np.uint32)
+        for rank, t, ln in self.seeded_2ch(seeds, threshold):
+            tr[rank] = t
+            lr[rank] = ln
+        return tr, lr
+
+    # matrix/bucket_ch.rs:6381-6520 (table_phast_lopsided_2ch), the bucket
+    # recomputation of same-edge conflicts (:6496-6516) left out: returns
+    # (S*T times, S*T lengths, S*T conflict flags), a conflicted pair INF in both
+    def table_phast_lopsided_2ch(self, src_seedsets, tgt_seedsets, threshold):
+        S, T = len(src_seedsets), len(tgt_seedsets)
+        tm = np.full(S * T, INF, dtype=np.uint32)
+        lm = np.full(S * T, INF, dtype=np.uint32)
+        conf = np.zeros(S * T, dtype=np.uint8)
+        max_tgt_part = max((s[1] for v in tgt_seedsets for s in v), default=0)
+
+        def field_bound(seeds):  # :6407-6415
+            if threshold == INF:
+                return INF
+            return min(INF, min(INF, threshold + max((s[1] for s in seeds), default=0)) + max_tgt_part)
+
+        tgt_ranks = {s[0] for v in tgt_seedsets for s in v}
+        for i, seeds in enumerate(src_seedsets):
+            if not seeds:
+                continue
+            src_ranks = {s[0] for s in seeds}
+            settled = self.seeded_2ch([(s[0], s[1], s[2]) for s in seeds], field_bound(seeds))  # :6431-6439
+            field = {}
+            for rank, t, ln in settled:  # :6440-6453
+                if rank in tgt_ranks and (rank not in field or t < field[rank][0]):
+                    field[rank] = (t, ln)
+            for j, tseeds in enumerate(tgt_seedsets):  # :6454-6481
+                if not tseeds:
+                    continue
+                if any(s[0] in src_ranks for s in tseeds):
+                    conf[i * T + j] = 1
+                    continue
+                bt, bl = INF, INF
+                for s in tseeds:
+                    f = field.get(s[0])
+                    if f is not None and f[0] != INF:
+                        at = max(0, f[0] - s[1])
+                        if at < bt:  # strict: the first target seed of the best time wins
+                            bt, bl = at, max(0, f[1] - s[2])
+                if threshold != INF and bt != INF and bt > threshold:
+                    bt, bl = INF, INF
+                tm[i * T + j] = bt
+                lm[i * T + j] = bl
+        return tm, lm, conf
 
     def __del__(self):
         st = getattr(self, "_st", None)

@@ -21,6 +21,8 @@
  *                              matrix/bucket_ch.rs:191-424 (INF arcs dropped)
  *   orc_phast_seeded           server/isochrone_handler.rs:338-491
  *                              (generation stamps, 4096-rank block gating)
+ *   orc_phast_seeded_2ch       server/isochrone_handler.rs:500-625 (time and
+ *                              length-along-time, lexicographic tie-break)
  *   orc_dijkstra               plain Dijkstra, the brute-force pin
  *
  * Parity pins (tests/): the reference's hand vectors (range/phast.rs:1198-1554),
@@ -517,6 +519,7 @@ int orc_phast_query_reverse(const orc_cch *g, uint32_t target, uint32_t *dist) {
 typedef struct {
     uint32_t n, n_blocks, gen;
     uint32_t *dist, *version, *block_active;
+    uint32_t *len; /* length channel, allocated by the first 2-channel query */
 } orc_seeded_state;
 
 orc_seeded_state *orc_seeded_state_new(uint32_t n) {
@@ -531,7 +534,7 @@ orc_seeded_state *orc_seeded_state_new(uint32_t n) {
 }
 void orc_seeded_state_free(orc_seeded_state *s) {
     if (!s) return;
-    free(s->dist); free(s->version); free(s->block_active); free(s);
+    free(s->dist); free(s->version); free(s->block_active); free(s->len); free(s);
 }
 static inline uint32_t st_get(const orc_seeded_state *s, uint32_t v) { return s->version[v] == s->gen ? s->dist[v] : INF; }
 static inline void st_set(orc_seeded_state *s, uint32_t v, uint32_t d) {
@@ -587,6 +590,96 @@ uint64_t orc_phast_seeded(orc_seeded_state *s, const orc_flat *up, const orc_fla
             if (s->version[r] == s->gen && s->dist[r] <= threshold) {
                 if (out_rank) out_rank[cnt] = r;
                 if (out_dist) out_dist[cnt] = s->dist[r];
+                ++cnt;
+            }
+    }
+    return cnt;
+}
+
+/* server/isochrone_handler.rs:500-625 — 2-channel seeded bounded PHAST: a
+ * length-along-time channel carried with the time field.  The *_len flats
+ * share the time flats' topology (index i is the same arc).  Time is primary;
+ * a strict time improvement takes the new length, an equal finite time keeps
+ * the shorter length (the lazy lexicographic tie-break of :551-565 in the
+ * upward phase, which re-pushes the node, and :591-602 in the downward scan).
+ * Seeds apply in order while strictly faster (:522-526), so among seeds of one
+ * rank the first of the fastest wins.  Output: (rank, time, len) with
+ * time <= threshold in ascending rank; returns the count. */
+static inline uint32_t st_len(const orc_seeded_state *s, uint32_t v) { return s->version[v] == s->gen ? s->len[v] : INF; }
+static inline void st_set2(orc_seeded_state *s, uint32_t v, uint32_t d, uint32_t l) {
+    st_set(s, v, d);
+    s->len[v] = l;
+}
+
+uint64_t orc_phast_seeded_2ch(orc_seeded_state *s, const orc_flat *up, const orc_flat *down, const orc_flat *up_len,
+                              const orc_flat *down_len, const uint32_t *seed_rank, const uint32_t *seed_time,
+                              const uint32_t *seed_len, uint32_t n_seeds, uint32_t threshold, uint32_t *out_rank,
+                              uint32_t *out_time, uint32_t *out_len) {
+    const uint32_t n = s->n;
+    s->gen += 1;
+    if (s->gen == 0) {
+        memset(s->version, 0, sizeof(uint32_t) * n);
+        memset(s->block_active, 0, sizeof(uint32_t) * s->n_blocks);
+        s->gen = 1;
+    }
+    if (!s->len) { /* ensure_len, :167-171 */
+        s->len = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+        for (uint32_t i = 0; i < n; ++i) s->len[i] = INF;
+    }
+    for (uint32_t i = 0; i < n_seeds; ++i) /* :522-526 */
+        if (seed_time[i] < st_get(s, seed_rank[i])) st_set2(s, seed_rank[i], seed_time[i], seed_len[i]);
+    heap_t h = {0};
+    for (uint32_t i = 0; i < n_seeds; ++i) /* :528-532 */
+        if (st_get(s, seed_rank[i]) == seed_time[i]) hp_push(&h, seed_time[i], seed_rank[i]);
+    uint32_t d, u;
+    while (hp_pop(&h, &d, &u)) { /* :533-568 */
+        if (d > threshold) break;
+        if (d > st_get(s, u)) continue;
+        const uint32_t lu = st_len(s, u);
+        for (uint64_t i = up->off[u]; i < up->off[u + 1]; ++i) {
+            const uint32_t v = up->tgt[i];
+            const uint32_t nt = sat_add(d, up->w[i]), ct = st_get(s, v);
+            if (nt < ct) {
+                st_set2(s, v, nt, sat_add(lu, up_len->w[i]));
+                hp_push(&h, nt, v);
+            } else if (nt == ct && ct != INF) {
+                const uint32_t nl = sat_add(lu, up_len->w[i]);
+                if (nl < st_len(s, v)) {
+                    st_set2(s, v, nt, nl);
+                    hp_push(&h, nt, v);
+                }
+            }
+        }
+    }
+    hp_free(&h);
+    for (int64_t b = (int64_t)s->n_blocks - 1; b >= 0; --b) { /* :569-604 */
+        if (s->block_active[b] != s->gen) continue;
+        uint32_t r0 = (uint32_t)b * 4096, r1 = r0 + 4096 < n ? r0 + 4096 : n;
+        for (int64_t r = (int64_t)r1 - 1; r >= (int64_t)r0; --r) {
+            const uint32_t du = st_get(s, (uint32_t)r);
+            if (du == INF || du > threshold) continue;
+            const uint32_t lu = st_len(s, (uint32_t)r);
+            for (uint64_t i = down->off[r]; i < down->off[r + 1]; ++i) {
+                const uint32_t v = down->tgt[i];
+                const uint32_t nt = sat_add(du, down->w[i]), ct = st_get(s, v);
+                if (nt < ct) {
+                    st_set2(s, v, nt, sat_add(lu, down_len->w[i]));
+                } else if (nt == ct && ct != INF) {
+                    const uint32_t nl = sat_add(lu, down_len->w[i]);
+                    if (nl < st_len(s, v)) st_set2(s, v, nt, nl);
+                }
+            }
+        }
+    }
+    uint64_t cnt = 0;
+    for (uint32_t b = 0; b < s->n_blocks; ++b) { /* :605-622 */
+        if (s->block_active[b] != s->gen) continue;
+        uint32_t r0 = b * 4096, r1 = r0 + 4096 < n ? r0 + 4096 : n;
+        for (uint32_t r = r0; r < r1; ++r)
+            if (s->version[r] == s->gen && s->dist[r] <= threshold) {
+                if (out_rank) out_rank[cnt] = r;
+                if (out_time) out_time[cnt] = s->dist[r];
+                if (out_len) out_len[cnt] = s->len[r];
                 ++cnt;
             }
     }

@@ -1,0 +1,258 @@
+"""Two-channel seeded PHAST: time and the length along the time-optimal path
+(isochrone_handler.rs:500-625, the 2-channel table of bucket_ch.rs:6381-6520).
+
+The oracle's restatement is pinned first, on the CPU:
+  * the reference's own tie vectors (isochrone_handler.rs:1809-1876, committed
+    as tests/golden/two_channel_vectors.json);
+  * the broom with lengths = 2 x times (bucket_ch.rs:5801-5845): length is
+    exactly twice the time;
+  * brute force: a lexicographic Dijkstra over the two-layer graph (up layer,
+    then down layer) of the CCH, a different algorithm from the PHAST walk;
+  * the time channel equals the one-channel seeded PHAST, and lengths equal to
+    the times give lengths equal to the times.
+The GPU path is then checked bit-exact against the oracle (tests marked gpu).
+"""
+import heapq
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, broom_graph, grid_graph
+from oracle.oracle import INF, Oracle
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def sat(a):
+    return min(a, INF)
+
+
+def tie_cases():
+    with open(os.path.join(GOLDEN, "two_channel_vectors.json")) as f:
+        return json.load(f)["cases"]
+
+
+def tie_graph(case):
+    from bfgpu import CchGraph
+    g = CchGraph.from_arcs(case["n"], [(a, b, t) for a, b, t, _ in case["up"]],
+                           [(a, b, t) for a, b, t, _ in case["down"]])
+    # from_arcs sorts by (tail, head); lengths follow the same order
+    up_len = [ln for _, _, _, ln in sorted(case["up"])]
+    dn_len = [ln for _, _, _, ln in sorted(case["down"])]
+    return g, up_len, dn_len
+
+
+def lengths_for(g, seed, scale=3):
+    """Length weights with the time flats' INF pattern: finite where the time is."""
+    rng = np.random.default_rng(seed)
+    ul = rng.integers(0, 4000, g.m_up).astype(np.uint32)
+    dl = rng.integers(0, 4000, g.m_down).astype(np.uint32)
+    ul[g.up_w == INF] = INF
+    dl[g.dn_w == INF] = INF
+    return ul, dl
+
+
+def tie_heavy(g, seed):
+    """The same topology with times in {1, 2, 3}: equal-time paths everywhere."""
+    from bfgpu import CchGraph
+    rng = np.random.default_rng(seed)
+    uw = rng.integers(1, 4, g.m_up).astype(np.uint32)
+    dw = rng.integers(1, 4, g.m_down).astype(np.uint32)
+    uw[g.up_w == INF] = INF
+    dw[g.dn_w == INF] = INF
+    return CchGraph(n=g.n, up_off=g.up_off, up_tgt=g.up_tgt, up_w=uw, dn_off=g.dn_off, dn_tgt=g.dn_tgt, dn_w=dw)
+
+
+def first_fastest(seeds):
+    """isochrone_handler.rs:522-526: per rank the first of the fastest seeds."""
+    best = {}
+    for r, t, ln in seeds:
+        if t != INF and (r not in best or t < best[r][0]):
+            best[r] = (t, ln)
+    return best
+
+
+def brute_2ch(g, ul, dl, seeds, thr):
+    """Least (time, length) over the up-down paths from the seeds, by a
+    lexicographic Dijkstra over two layers: layer 0 follows UP arcs, layer 1
+    DOWN arcs, and a node passes from layer 0 to 1 at no cost."""
+    n = g.n
+    lab = {}
+    pq = []
+    for r, (t, ln) in first_fastest(seeds).items():
+        lab[(0, r)] = (t, ln)
+        heapq.heappush(pq, (t, ln, 0, r))
+    done = set()
+    while pq:
+        t, ln, layer, u = heapq.heappop(pq)
+        if (layer, u) in done or lab.get((layer, u)) != (t, ln):
+            continue
+        done.add((layer, u))
+        nxt = []
+        if layer == 0:
+            nxt.append((1, u, 0, 0))
+            for i in range(int(g.up_off[u]), int(g.up_off[u + 1])):
+                if g.up_w[i] != INF:
+                    nxt.append((0, int(g.up_tgt[i]), int(g.up_w[i]), int(ul[i])))
+        else:
+            for i in range(int(g.dn_off[u]), int(g.dn_off[u + 1])):
+                if g.dn_w[i] != INF:
+                    nxt.append((1, int(g.dn_tgt[i]), int(g.dn_w[i]), int(dl[i])))
+        for lay, v, wt, wl in nxt:
+            c = (sat(t + wt), sat(ln + wl))
+            if c[0] == INF:
+                continue
+            if (lay, v) not in lab or c < lab[(lay, v)]:
+                lab[(lay, v)] = c
+                heapq.heappush(pq, (c[0], c[1], lay, v))
+    out = {}
+    for v in range(n):
+        c = lab.get((1, v))
+        if c is not None and c[0] <= thr:
+            out[v] = c
+    return out
+
+
+# ---------------------------------------------------------------- CPU: oracle pins
+
+@pytest.mark.parametrize("case", tie_cases(), ids=lambda c: c["name"])
+def test_reference_tie_vectors(case):
+    g, ul, dl = tie_graph(case)
+    o = Oracle(g)
+    o.attach_length(ul, dl)
+    got = {r: (t, ln) for r, t, ln in o.seeded_2ch([tuple(s) for s in case["seeds"]], case["threshold"])}
+    for r, (t, ln) in case["expect"].items():
+        assert got[int(r)] == (t, ln)
+
+
+@pytest.mark.parametrize("thr", [INF, 200, 55])
+def test_broom_length_is_twice_the_time(thr):
+    g, wu, wd = broom_graph(12)
+    o = Oracle(g)
+    o.attach_length(2 * g.up_w, 2 * g.dn_w)
+    for s in range(12):
+        got = o.seeded_2ch([(s, 0, 0)], thr)
+        assert got, "the seed itself is settled"
+        for r, t, ln in got:
+            assert ln == 2 * t and t <= thr
+        t1 = dict(o.seeded([(s, 0)], thr))
+        assert {r: t for r, t, _ in got} == t1
+
+
+@pytest.mark.parametrize("thr", [INF, 400])
+def test_matches_lexicographic_bruteforce(grid_small, thr):
+    for gi, g in enumerate([grid_small, tie_heavy(grid_small, 7)]):
+        ul, dl = lengths_for(g, 11 + gi)
+        o = Oracle(g)
+        o.attach_length(ul, dl)
+        rng = np.random.default_rng(3 + gi)
+        for _ in range(4):
+            k = int(rng.integers(1, 4))
+            seeds = [(int(r), int(c), int(c) * 2) for r, c in zip(rng.integers(0, g.n, k), rng.integers(0, 30, k))]
+            seeds.append((seeds[0][0], seeds[0][1], 0))  # same rank and time: the first seed keeps its length
+            got = {r: (t, ln) for r, t, ln in o.seeded_2ch(seeds, thr)}
+            assert got == brute_2ch(g, ul, dl, seeds, thr)
+
+
+def test_time_channel_and_equal_lengths(grid_small):
+    g = grid_small
+    o = Oracle(g)
+    o.attach_length(g.up_w, g.dn_w)
+    for s in (0, g.n // 3, g.n - 1):
+        got = o.seeded_2ch([(s, 5, 5)], 900)
+        assert all(t == ln for _, t, ln in got)
+        assert [(r, t) for r, t, _ in got] == o.seeded([(s, 5)], 900)
+
+
+def test_length_flats_must_share_topology(grid_small):
+    g = grid_small
+    ul, dl = lengths_for(g, 1)
+    ul[np.nonzero(g.up_w != INF)[0][0]] = INF  # a finite-time arc without a length
+    with pytest.raises(ValueError):
+        Oracle(g).attach_length(ul, dl)
+
+
+def test_table_2ch_broom():
+    """table_phast_lopsided_2ch on the broom (bucket_ch.rs:5880-5912): the
+    length matrix is twice the time matrix within the bound."""
+    g, wu, wd = broom_graph(12)
+    o = Oracle(g)
+    o.attach_length(2 * g.up_w, 2 * g.dn_w)
+    seeds = [[(i, 0, 0)] for i in range(12)]
+    for thr in (INF, 200, 55):
+        tm, lm, conf = o.table_phast_lopsided_2ch(seeds, seeds, thr)
+        t1, c1 = o.table_phast_lopsided([[(i, 0)] for i in range(12)], [[(i, 0)] for i in range(12)], thr)
+        assert np.array_equal(tm, t1) and np.array_equal(conf, c1)
+        fin = tm != INF
+        assert np.array_equal(lm[fin], 2 * tm[fin]) and np.all(lm[~fin] == INF)
+
+
+# ---------------------------------------------------------------- GPU parity
+
+def oracle_rows(o, seeds, thr, n):
+    return o.seeded_2ch_rows(seeds, thr)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", tie_cases(), ids=lambda c: c["name"])
+def test_gpu_reference_tie_vectors(case):
+    from bfgpu import BatchedPhastEngine
+    say("2ch tie", case["name"])
+    g, ul, dl = tie_graph(case)
+    with BatchedPhastEngine(g) as e:
+        e.attach_length(ul, dl)
+        got = {r: (t, ln) for r, t, ln in e.run_phast_bounded_fast_seeded_2ch(
+            [tuple(s) for s in case["seeds"]], case["threshold"])}
+    for r, (t, ln) in case["expect"].items():
+        assert got[int(r)] == (t, ln)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ties", [False, True])
+def test_gpu_2ch_matches_oracle(grid_config1, ties):
+    from bfgpu import BatchedPhastEngine
+    g = tie_heavy(grid_config1, 5) if ties else grid_config1
+    ul, dl = lengths_for(g, 21)
+    say("2ch grid", g.n, "ties" if ties else "times")
+    o = Oracle(g)
+    o.attach_length(ul, dl)
+    rng = np.random.default_rng(9)
+    lanes, thrs = [], []
+    for i in range(70):  # two lane groups, the second partial
+        k = int(rng.integers(1, 4))
+        seeds = [(int(r), int(c), int(c) * 3) for r, c in zip(rng.integers(0, g.n, k), rng.integers(0, 60, k))]
+        if i % 5 == 0:
+            seeds.append((seeds[0][0], seeds[0][1], 0))  # duplicate rank, equal time
+        if i % 7 == 0:
+            seeds.append((seeds[0][0], INF, 0))  # an INF seed applies nowhere
+        lanes.append(seeds)
+        thrs.append(INF if i % 3 == 0 else int(rng.integers(20, 400 if ties else 3000)))
+    with BatchedPhastEngine(g) as e:
+        e.attach_length(ul, dl)
+        t, ln = e.phast_seeded_2ch(lanes, thrs)
+        t1 = e.phast_seeded([[(r, c) for r, c, _ in s] for s in lanes], thrs)
+    assert np.array_equal(t, t1), "time channel differs from the one-channel PHAST"
+    for i in range(len(lanes)):
+        et, el = o.seeded_2ch_rows(lanes[i], thrs[i])
+        assert np.array_equal(t[i], et), f"lane {i}: times"
+        assert np.array_equal(ln[i], el), f"lane {i}: lengths"
+
+
+@pytest.mark.gpu
+def test_gpu_2ch_needs_lengths(grid_small):
+    from bfgpu import BatchedPhastEngine
+    from bfgpu.engine import BfgError
+    with BatchedPhastEngine(grid_small) as e:
+        with pytest.raises(BfgError):
+            e.phast_seeded_2ch([[(0, 0, 0)]], [INF])
+        ul, dl = lengths_for(grid_small, 2)
+        e.attach_length(ul, dl)
+        t, ln = e.phast_seeded_2ch([[(0, 0, 0)], []], [INF, INF])
+        assert t[0][0] == 0 and ln[0][0] == 0
+        assert np.all(t[1] == INF) and np.all(ln[1] == INF)
