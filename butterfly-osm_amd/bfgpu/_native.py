@@ -74,6 +74,8 @@ BFGPU_SYMBOLS = {
                          u32p, C.POINTER(C.c_uint8)],
     "bfg_ctx_attach_length": [C.c_void_p, C.POINTER(GraphDesc), u32p, u32p],
     "bfg_phast_seeded_2ch": [C.c_void_p, u32p, u32p, u32p, u32p, C.c_uint32, u32p, u32p, u32p],
+    "bfg_table_seeded_2ch": [C.c_void_p, u32p, u32p, u32p, u32p, C.c_uint32, u32p, u32p, u32p, u32p, C.c_uint32,
+                             C.c_uint32, u32p, u32p, C.POINTER(C.c_uint8)],
 }
 
 _bfgpu = None

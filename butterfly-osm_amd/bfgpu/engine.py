@@ -234,6 +234,34 @@ class BatchedPhastEngine:
         idx = np.nonzero(t[0] != INF)[0]
         return [(int(r), int(t[0][r]), int(ln[0][r])) for r in idx]
 
+    def table_seeded_bounded_routed_2ch(self, src_seedsets, tgt_seedsets, threshold=INF):
+        """bucket_ch.rs:6329-6379 on its PHAST path, table_phast_lopsided_2ch
+        (:6381-6520).  Seeds are (rank, part_time, part_len, ...) tuples.
+        Returns (S*T times, S*T lengths, S*T u8 conflicts), row-major; a
+        conflict (a shared seed rank, which the reference recomputes with its
+        two-channel bucket engine) is INF in both."""
+        def flat(seedsets):
+            off = np.zeros(len(seedsets) + 1, dtype=np.uint32)
+            ranks, pt, pl = [], [], []
+            for i, seeds in enumerate(seedsets):
+                for sd in seeds:
+                    ranks.append(sd[0])
+                    pt.append(sd[1])
+                    pl.append(sd[2])
+                off[i + 1] = len(ranks)
+            return off, _u32(ranks), _u32(pt), _u32(pl)
+        so, sr, st, sl = flat(src_seedsets)
+        to, tr, tt, tl = flat(tgt_seedsets)
+        S, T = len(src_seedsets), len(tgt_seedsets)
+        times = np.empty(S * T, dtype=np.uint32)
+        lens = np.empty(S * T, dtype=np.uint32)
+        conf = np.empty(S * T, dtype=np.uint8)
+        P = _native.ptr
+        _check(self._lib.bfg_table_seeded_2ch(self._h, P(so), P(sr), P(st), P(sl), S, P(to), P(tr), P(tt), P(tl),
+                                              T, int(threshold) & INF, P(times), P(lens),
+                                              conf.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return times, lens, conf
+
     def table_seeded_bounded_routed(self, src_seedsets, tgt_seedsets, threshold=INF):
         """bucket_ch.rs:6535-6574 on its PHAST path, table_phast_lopsided
         (:6717-6853): one seeded field per source, cell = min over the target's

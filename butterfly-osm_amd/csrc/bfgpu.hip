@@ -1793,6 +1793,60 @@ __global__ __launch_bounds__(NT) void t2_out_kernel(uint32_t n, uint32_t nl, con
     }
 }
 
+// Two-channel seeded table (bucket_ch.rs:6454-6481): a workgroup per (64
+// targets, lane group).  Target j's cell for source lane i takes, over j's
+// seeds (rank, part_time, part_len) in order, the first seed of least
+// field time - part_time (saturating) whose rank the field settled (time <=
+// the lane's field bound), and that seed's field length - part_len; above
+// `thr` both are INF.  A target sharing a seed rank with the source is a
+// conflict: flagged, cells INF.
+__global__ __launch_bounds__(NT) void t2_table_kernel(uint32_t n, uint32_t T, uint32_t nl,
+                                                      const uint32_t *__restrict__ toff,
+                                                      const uint32_t *__restrict__ tpos,
+                                                      const uint32_t *__restrict__ trank,
+                                                      const uint32_t *__restrict__ tpt,
+                                                      const uint32_t *__restrict__ tpl,
+                                                      const uint32_t *__restrict__ soff,
+                                                      const uint32_t *__restrict__ srank,
+                                                      const uint32_t *__restrict__ lim, uint32_t thr,
+                                                      const uint64_t *__restrict__ lab, uint32_t *__restrict__ o_t,
+                                                      uint32_t *__restrict__ o_l, uint8_t *__restrict__ conf) {
+    __shared__ uint32_t tt[64][65], tl[64][65];
+    const uint32_t t0 = blockIdx.x * 64, g = blockIdx.y;
+    const uint64_t *base = lab + (size_t)g * n * 64;
+    for (uint32_t idx = threadIdx.x; idx < 64 * 64; idx += NT) {
+        const uint32_t ti = idx / 64, lane = idx % 64, t = t0 + ti, gl = g * 64 + lane;
+        uint32_t bt = INF, bl = INF, clash = 0;
+        if (t < T && gl < nl) {
+            const uint32_t lm = lim[gl];
+            for (uint32_t q = toff[t]; q < toff[t + 1]; ++q) {
+                const uint64_t x = base[(size_t)tpos[q] * 64 + lane];
+                const uint32_t ft = (uint32_t)(x >> 32), fl = (uint32_t)x;
+                if (ft != INF && ft <= lm) {
+                    const uint32_t at = ft > tpt[q] ? ft - tpt[q] : 0u;
+                    if (at < bt) {
+                        bt = at;
+                        bl = fl > tpl[q] ? fl - tpl[q] : 0u;
+                    }
+                }
+                for (uint32_t k = soff[gl]; k < soff[gl + 1]; ++k) clash |= srank[k] == trank[q];
+            }
+            if (clash || (thr != INF && bt != INF && bt > thr)) bt = bl = INF;
+            if (conf) conf[(size_t)gl * T + t] = (uint8_t)clash;
+        }
+        tt[ti][lane] = bt;
+        tl[ti][lane] = bl;
+    }
+    __syncthreads();
+    for (uint32_t idx = threadIdx.x; idx < 64 * 64; idx += NT) {
+        const uint32_t lane = idx / 64, ti = idx % 64, gl = g * 64 + lane, t = t0 + ti;
+        if (gl < nl && t < T) {
+            o_t[(size_t)gl * T + t] = tt[ti][lane];
+            o_l[(size_t)gl * T + t] = tl[ti][lane];
+        }
+    }
+}
+
 // Builds the two-channel arc lists from the CSR the context was created from
 // and length weights aligned with it (arcs of INF time are dropped, as the
 // time flats drop them).
@@ -1903,9 +1957,11 @@ int attach_length(bfg_ctx *c, const bfg_graph_desc *gd, const uint32_t *up_len, 
 
 // Runs lanes [0, nl) of a two-channel request in batches of 64-lane groups;
 // seeds per lane are distinct positions with their (time, length) labels.
+// After each batch's sweep, emit(first lane, lanes, groups) reads the labels
+// (t.lab, limits in t.lim) on the context stream.
+template <class Emit>
 int phast_2ch(bfg_ctx *c, const std::vector<uint32_t> &soff, const std::vector<uint32_t> &spos,
-              const std::vector<uint64_t> &slab, const uint32_t *limits, uint32_t nl, uint32_t *times,
-              uint32_t *lens) {
+              const std::vector<uint64_t> &slab, const uint32_t *limits, uint32_t nl, Emit emit) {
     TwoChannel &t = *c->t2;
     const bfg::Layout &L = c->L;
     const uint32_t n = L.n;
@@ -1954,14 +2010,38 @@ int phast_2ch(bfg_ctx *c, const std::vector<uint32_t> &soff, const std::vector<u
                 t2_pull_kernel<<<dim3((p1 - p0 + NT / 64 - 1) / (NT / 64), bg), NT, 0, s>>>(
                     p0, p1, n, L.n_top, bl, t.b_off.p, t.b_src.p, t.b_w.p, t.lim.p, t.lab.p);
         }
-        t2_out_kernel<<<dim3((n + 63) / 64, bg), NT, 0, s>>>(n, bl, t.pos_of_rank.p, t.lim.p, t.lab.p, t.o_t.p,
-                                                            t.o_l.p);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(times + (size_t)l0 * n, t.o_t.p, (size_t)bl * n * 4, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(lens + (size_t)l0 * n, t.o_l.p, (size_t)bl * n * 4, hipMemcpyDeviceToHost, s));
+        const int rc = emit(l0, bl, bg);
+        if (rc) return rc;
         HIP_TRY(hipStreamSynchronize(s));
     }
     return BFG_SUCCESS;
+}
+
+// Per lane, its seeds as the reference applies them (isochrone_handler.rs:
+// 522-526: a seed applies while strictly faster than what its rank holds, so
+// per rank the first of the fastest wins; an INF seed never applies).
+void seeds_2ch(const bfg::Layout &L, const uint32_t *off, const uint32_t *rank, const uint32_t *tm,
+               const uint32_t *ln, uint32_t nl, std::vector<uint32_t> &soff, std::vector<uint32_t> &spos,
+               std::vector<uint64_t> &slab) {
+    soff.assign(1, 0);
+    for (uint32_t i = 0; i < nl; ++i) {
+        const size_t first = spos.size();
+        for (uint32_t k = off[i]; k < off[i + 1]; ++k) {
+            if (tm[k] == INF) continue;
+            const uint32_t p = L.pos_of_rank[rank[k]];
+            const uint64_t x = (uint64_t)tm[k] << 32 | ln[k];
+            size_t j = first;
+            while (j < spos.size() && spos[j] != p) ++j;
+            if (j == spos.size()) {
+                spos.push_back(p);
+                slab.push_back(x);
+            } else if (tm[k] < (uint32_t)(slab[j] >> 32)) {
+                slab[j] = x;
+            }
+        }
+        soff.push_back((uint32_t)spos.size());
+    }
 }
 
 void fill_info(const bfg::Layout &L, bfg_ctx_info *info) {
@@ -2277,28 +2357,93 @@ int bfg_phast_seeded_2ch(bfg_ctx *c, const uint32_t *seed_offsets, const uint32_
     if (ns && (!seed_ranks || !seed_times || !seed_lens)) return fail(BFG_INVALID_PARAMETER, "null seed buffer");
     int rc = check_ranks(c, seed_ranks, ns, "seed");
     if (rc) return rc;
-    // isochrone_handler.rs:522-526: a seed applies while strictly faster than
-    // what its rank holds, so per rank the first of the fastest seeds wins
-    std::vector<uint32_t> soff(1, 0), spos;
+    std::vector<uint32_t> soff, spos;
     std::vector<uint64_t> slab;
-    for (uint32_t i = 0; i < L; ++i) {
-        const size_t first = spos.size();
-        for (uint32_t k = seed_offsets[i]; k < seed_offsets[i + 1]; ++k) {
-            if (seed_times[k] == INF) continue;
-            const uint32_t p = c->L.pos_of_rank[seed_ranks[k]];
-            const uint64_t x = (uint64_t)seed_times[k] << 32 | seed_lens[k];
-            size_t j = first;
-            while (j < spos.size() && spos[j] != p) ++j;
-            if (j == spos.size()) {
-                spos.push_back(p);
-                slab.push_back(x);
-            } else if (seed_times[k] < (uint32_t)(slab[j] >> 32)) {
-                slab[j] = x;
-            }
-        }
-        soff.push_back((uint32_t)spos.size());
+    seeds_2ch(c->L, seed_offsets, seed_ranks, seed_times, seed_lens, L, soff, spos, slab);
+    TwoChannel &t = *c->t2;
+    const uint32_t n = c->L.n;
+    return phast_2ch(c, soff, spos, slab, thresholds, L, [&](uint32_t l0, uint32_t bl, uint32_t bg) -> int {
+        hipStream_t s = c->stream;
+        t2_out_kernel<<<dim3((n + 63) / 64, bg), NT, 0, s>>>(n, bl, t.pos_of_rank.p, t.lim.p, t.lab.p, t.o_t.p,
+                                                            t.o_l.p);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(times + (size_t)l0 * n, t.o_t.p, (size_t)bl * n * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(lens + (size_t)l0 * n, t.o_l.p, (size_t)bl * n * 4, hipMemcpyDeviceToHost, s));
+        return BFG_SUCCESS;
+    });
+}
+
+int bfg_table_seeded_2ch(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t *src_ranks,
+                         const uint32_t *src_part_times, const uint32_t *src_part_lens, uint32_t S,
+                         const uint32_t *tgt_offsets, const uint32_t *tgt_ranks, const uint32_t *tgt_part_times,
+                         const uint32_t *tgt_part_lens, uint32_t T, uint32_t threshold, uint32_t *times,
+                         uint32_t *lens, uint8_t *conflicts) {
+    if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
+    if ((S && !src_offsets) || (T && !tgt_offsets) || (S && T && (!times || !lens)))
+        return fail(BFG_INVALID_PARAMETER, "null buffer");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    if (!c->t2) return fail(BFG_INVALID_PARAMETER, "no length weights attached (bfg_ctx_attach_length)");
+    if (S == 0 || T == 0) return BFG_SUCCESS;
+    if (src_offsets[0] != 0 || tgt_offsets[0] != 0) return fail(BFG_INVALID_PARAMETER, "seed offsets must start at 0");
+    for (uint32_t i = 0; i < S; ++i)
+        if (src_offsets[i + 1] < src_offsets[i]) return fail(BFG_INVALID_PARAMETER, "src_offsets not monotone");
+    for (uint32_t j = 0; j < T; ++j)
+        if (tgt_offsets[j + 1] < tgt_offsets[j]) return fail(BFG_INVALID_PARAMETER, "tgt_offsets not monotone");
+    const uint32_t ns = src_offsets[S], nt = tgt_offsets[T];
+    if ((ns && (!src_ranks || !src_part_times || !src_part_lens)) ||
+        (nt && (!tgt_ranks || !tgt_part_times || !tgt_part_lens)))
+        return fail(BFG_INVALID_PARAMETER, "null seed buffer");
+    int rc = check_ranks(c, src_ranks, ns, "source seed");
+    if (!rc) rc = check_ranks(c, tgt_ranks, nt, "target seed");
+    if (rc) return rc;
+    // one field per source, bounded by threshold + its largest time part + the
+    // largest target time part (bucket_ch.rs:6400-6415)
+    uint32_t max_tgt_part = 0;
+    for (uint32_t q = 0; q < nt; ++q) max_tgt_part = std::max(max_tgt_part, tgt_part_times[q]);
+    auto sat = [](uint32_t a, uint32_t b) { return a > INF - b ? INF : a + b; };
+    std::vector<uint32_t> bound(S);
+    for (uint32_t i = 0; i < S; ++i) {
+        uint32_t mp = 0;
+        for (uint32_t k = src_offsets[i]; k < src_offsets[i + 1]; ++k) mp = std::max(mp, src_part_times[k]);
+        bound[i] = threshold == INF ? INF : sat(sat(threshold, mp), max_tgt_part);
     }
-    return phast_2ch(c, soff, spos, slab, thresholds, L, times, lens);
+    std::vector<uint32_t> soff, spos;
+    std::vector<uint64_t> slab;
+    seeds_2ch(c->L, src_offsets, src_ranks, src_part_times, src_part_lens, S, soff, spos, slab);
+    std::vector<uint32_t> tpos(nt);
+    for (uint32_t q = 0; q < nt; ++q) tpos[q] = c->L.pos_of_rank[tgt_ranks[q]];
+    const std::vector<uint32_t> h_toff(tgt_offsets, tgt_offsets + T + 1), h_trank(tgt_ranks, tgt_ranks + nt),
+        h_tpt(tgt_part_times, tgt_part_times + nt), h_tpl(tgt_part_lens, tgt_part_lens + nt),
+        h_soff(src_offsets, src_offsets + S + 1), h_srank(src_ranks, src_ranks + ns);
+    DevBuf<uint32_t> d_toff, d_tpos, d_trank, d_tpt, d_tpl, d_soff, d_srank, d_t, d_l;
+    DevBuf<uint8_t> d_conf;
+    hipStream_t st = c->stream;
+    HIP_TRY(d_toff.upload(h_toff, st));
+    HIP_TRY(d_tpos.upload(tpos, st));
+    HIP_TRY(d_trank.upload(h_trank, st));
+    HIP_TRY(d_tpt.upload(h_tpt, st));
+    HIP_TRY(d_tpl.upload(h_tpl, st));
+    HIP_TRY(d_soff.upload(h_soff, st));
+    HIP_TRY(d_srank.upload(h_srank, st));
+    HIP_TRY(d_t.alloc((size_t)S * T));
+    HIP_TRY(d_l.alloc((size_t)S * T));
+    if (conflicts) HIP_TRY(d_conf.alloc((size_t)S * T));
+    HIP_TRY(hipStreamSynchronize(st));  // before the host tables above may go
+    rc = phast_2ch(c, soff, spos, slab, bound.data(), S, [&](uint32_t l0, uint32_t bl, uint32_t bg) -> int {
+        // this batch's lanes are sources [l0, l0 + bl): offset the per-source tables
+        t2_table_kernel<<<dim3((T + 63) / 64, bg), NT, 0, st>>>(
+            c->L.n, T, bl, d_toff.p, d_tpos.p, d_trank.p, d_tpt.p, d_tpl.p, d_soff.p + l0, d_srank.p,
+            c->t2->lim.p, threshold, c->t2->lab.p, d_t.p + (size_t)l0 * T, d_l.p + (size_t)l0 * T,
+            conflicts ? d_conf.p + (size_t)l0 * T : nullptr);
+        HIP_TRY(hipGetLastError());
+        return BFG_SUCCESS;
+    });
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(times, d_t.p, (size_t)S * T * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(lens, d_l.p, (size_t)S * T * 4, hipMemcpyDeviceToHost));
+    if (conflicts) HIP_TRY(hipMemcpy(conflicts, d_conf.p, (size_t)S * T, hipMemcpyDeviceToHost));
+    return BFG_SUCCESS;
 }
 
 int bfg_table_seeded(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t *src_ranks, const uint32_t *src_parts,

@@ -198,6 +198,23 @@ int bfg_phast_seeded_2ch(bfg_ctx *ctx, const uint32_t *seed_offsets, const uint3
                          const uint32_t *seed_times, const uint32_t *seed_lens, uint32_t n_lanes,
                          const uint32_t *thresholds, uint32_t *times, uint32_t *lens);
 
+/* Two-channel seeded S x T table: table_phast_lopsided_2ch
+ * (matrix/bucket_ch.rs:6381-6520), the path table_seeded_bounded_routed_2ch
+ * (:6329-6379) takes when phast_wins.  Seeds are (rank, part_time, part_len).
+ * The field of source i is bfg_phast_seeded_2ch of its seeds, bounded by
+ * threshold + its largest part_time + the largest target part_time
+ * (:6400-6415).  times/lens[i * n_tgt + j]: over target j's seeds in order,
+ * the first seed of least field time - part_time (saturating) among the ranks
+ * the field settled, with that seed's field length - part_len; BFG_INF in
+ * both above threshold (:6454-6481).  Same-edge conflicts (a seed rank shared
+ * by source and target) are BFG_INF and flagged in conflicts (nullable), as
+ * in bfg_table_seeded.  Needs bfg_ctx_attach_length. */
+int bfg_table_seeded_2ch(bfg_ctx *ctx, const uint32_t *src_offsets, const uint32_t *src_ranks,
+                         const uint32_t *src_part_times, const uint32_t *src_part_lens, uint32_t n_src,
+                         const uint32_t *tgt_offsets, const uint32_t *tgt_ranks, const uint32_t *tgt_part_times,
+                         const uint32_t *tgt_part_lens, uint32_t n_tgt, uint32_t threshold, uint32_t *times,
+                         uint32_t *lens, uint8_t *conflicts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,7 +20,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, broom_graph, grid_graph
+from conftest import GOLDEN, broom_graph
 from oracle.oracle import INF, Oracle
 
 
@@ -48,7 +48,7 @@ def tie_graph(case):
     return g, up_len, dn_len
 
 
-def lengths_for(g, seed, scale=3):
+def lengths_for(g, seed):
     """Length weights with the time flats' INF pattern: finite where the time is."""
     rng = np.random.default_rng(seed)
     ul = rng.integers(0, 4000, g.m_up).astype(np.uint32)
@@ -195,10 +195,6 @@ def test_table_2ch_broom():
 
 # ---------------------------------------------------------------- GPU parity
 
-def oracle_rows(o, seeds, thr, n):
-    return o.seeded_2ch_rows(seeds, thr)
-
-
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", tie_cases(), ids=lambda c: c["name"])
 def test_gpu_reference_tie_vectors(case):
@@ -242,6 +238,52 @@ def test_gpu_2ch_matches_oracle(grid_config1, ties):
         et, el = o.seeded_2ch_rows(lanes[i], thrs[i])
         assert np.array_equal(t[i], et), f"lane {i}: times"
         assert np.array_equal(ln[i], el), f"lane {i}: lengths"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("thr", [INF, 1500])
+def test_gpu_table_2ch_matches_oracle(grid_config1, thr):
+    from bfgpu import BatchedPhastEngine
+    g = grid_config1
+    ul, dl = lengths_for(g, 31)
+    say("2ch table", g.n, thr)
+    o = Oracle(g)
+    o.attach_length(ul, dl)
+    rng = np.random.default_rng(17)
+
+    def seedset(k):
+        return [(int(r), int(p), int(q)) for r, p, q in zip(rng.integers(0, g.n, k), rng.integers(0, 90, k),
+                                                             rng.integers(0, 900, k))]
+    src = [seedset(int(rng.integers(1, 3))) for _ in range(70)]
+    tgt = [seedset(int(rng.integers(1, 3))) for _ in range(90)]
+    src[3], tgt[5] = [], []  # empty seed sets: INF rows / columns, no conflicts
+    tgt[7] = [(src[0][0][0], 3, 4)]  # a same-edge conflict with source 0
+    tgt[8] = [(tgt[9][0][0], 0, 50), (tgt[9][0][0], 0, 10)]  # equal times: the first seed's length
+    with BatchedPhastEngine(g) as e:
+        e.attach_length(ul, dl)
+        t, ln, conf = e.table_seeded_bounded_routed_2ch(src, tgt, thr)
+        t1, c1 = e.table_seeded_bounded_routed([[s[:2] for s in v] for v in src], [[s[:2] for s in v] for v in tgt],
+                                               thr)
+    et, el, ec = o.table_phast_lopsided_2ch(src, tgt, thr)
+    assert conf[7] == 1 and np.array_equal(conf, ec)
+    assert np.array_equal(t, et)
+    assert np.array_equal(ln, el)
+    assert np.array_equal(t, t1) and np.array_equal(conf, c1)
+
+
+@pytest.mark.gpu
+def test_gpu_table_2ch_broom():
+    from bfgpu import BatchedPhastEngine
+    g, wu, wd = broom_graph(12)
+    seeds = [[(i, 0, 0)] for i in range(12)]
+    o = Oracle(g)
+    o.attach_length(2 * g.up_w, 2 * g.dn_w)
+    with BatchedPhastEngine(g) as e:
+        e.attach_length(2 * g.up_w, 2 * g.dn_w)
+        for thr in (INF, 200, 55):
+            t, ln, conf = e.table_seeded_bounded_routed_2ch(seeds, seeds, thr)
+            et, el, ec = o.table_phast_lopsided_2ch(seeds, seeds, thr)
+            assert np.array_equal(t, et) and np.array_equal(ln, el) and np.array_equal(conf, ec)
 
 
 @pytest.mark.gpu
