@@ -293,6 +293,82 @@ def isochrone_bench(eng, g, args, rows, chords):
     print(json.dumps(res), flush=True)
 
 
+def synthetic_lengths(g, seed):
+    """Length-along-time weights for the synthetic CCH: 13.9 m per second of
+    time (50 km/h) with +-20 % per-arc jitter, INF where the time is INF."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for w in (g.up_w, g.dn_w):
+        ln = (w.astype(np.float64) * 13.9 * rng.uniform(0.8, 1.2, w.size)).astype(np.uint64)
+        ln = np.minimum(ln, INF_U32 - 1).astype(np.uint32)
+        ln[w == INF_U32] = INF_U32
+        out.append(ln)
+    return out
+
+
+def table2ch_bench(eng, g, args, rows, chords, src, tgt):
+    """An S x S two-channel seeded table (times and lengths along them,
+    table_phast_lopsided_2ch semantics): each query point is one or two seeds
+    (rank, part_time, part_len), the second on an UP neighbour, as a snapped
+    phantom node's two edge ends."""
+    S = args.table_2ch
+    rng = np.random.default_rng(args.seed + 40)
+
+    def seedsets(pts):
+        out = []
+        for r in pts[:S]:
+            r = int(r)
+            sets = [(r, int(rng.integers(0, 20)), int(rng.integers(0, 300)))]
+            a, b = int(g.up_off[r]), int(g.up_off[r + 1])
+            if b > a and rng.random() < 0.5:
+                sets.append((int(g.up_tgt[rng.integers(a, b)]), int(rng.integers(0, 20)), int(rng.integers(0, 300))))
+            out.append(sets)
+        return out
+    ss, ts = seedsets(src), seedsets(tgt)
+    ul, dl = synthetic_lengths(g, args.seed + 41)
+    t0 = time.perf_counter()
+    eng.attach_length(ul, dl)
+    log(f"length weights attached in {time.perf_counter() - t0:.1f}s")
+    eng.table_seeded_bounded_routed_2ch(ss[:64], ts, INF_U32)  # warm-up
+    times = []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        tm, lm, conf = eng.table_seeded_bounded_routed_2ch(ss, ts, INF_U32)
+        times.append(time.perf_counter() - t0)
+        log(f"2ch table {i}: {1e3 * times[-1]:.1f} ms wall")
+    wall = float(np.median(times))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, REPO)
+        from oracle.oracle import Oracle
+        o = Oracle(g)
+        o.attach_length(ul, dl)
+        t0 = time.perf_counter()
+        k = 0
+        exact = True
+        while k < S and time.perf_counter() - t0 < args.cpu_budget:
+            et, el, ec = o.table_phast_lopsided_2ch(ss[k:k + 1], ts, INF_U32)
+            exact &= bool(np.array_equal(et, tm[k * S:(k + 1) * S]) and np.array_equal(el, lm[k * S:(k + 1) * S])
+                          and np.array_equal(ec, conf[k * S:(k + 1) * S]))
+            k += 1
+        el_s = time.perf_counter() - t0
+        cpu = {"value": k * S / el_s, "unit": "cells/s", "cores": 1, "kind": "port",
+               "sample": f"oracle table_phast_lopsided_2ch, first {k} of {S} sources x {S} targets, single thread",
+               "bit_exact_sample": exact}
+        log(f"CPU 2ch table: {k} sources in {el_s:.1f}s, bit-exact vs GPU: {exact}")
+    res = {
+        "metric": "two-channel seeded table cells/s (time + length along it)",
+        "value": S * S / wall, "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
+        "ms_per_step": 1e3 * wall, "higher_is_better": True, "dtype": "u32x2", "data": "synthetic",
+        "config": {"workload": f"{S}x{S} seeded 2-channel table, 1-2 seeds per point, unbounded; "
+                               f"{rows}x{rows} jittered grid + {int(chords * 100)}% chords, n={g.n}; "
+                               f"synthetic lengths (13.9 m/s +-20 %)"},
+        "conflicts": int(conf.sum()),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -320,6 +396,9 @@ def main():
     ap.add_argument("--isochrones", type=int, default=0, metavar="L",
                     help="instead of the matrix line: L block-gated isochrone sweeps in one call, limits drawn "
                          "from {5,10,15,30,45,60} min (uneven per lane); prints its own JSON line")
+    ap.add_argument("--table-2ch", type=int, default=0, metavar="S",
+                    help="instead of the matrix line: an S x S two-channel seeded table (times and lengths "
+                         "along them) on the workload's graph; prints its own JSON line")
     ap.add_argument("--traffic-summary", nargs=4, default=None,
                     metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="print the DOWN-rows kernel's HBM bytes per launch as JSON: its FETCH_SIZE and "
@@ -363,6 +442,9 @@ def main():
         return
     if args.isochrones:
         isochrone_bench(eng, g, args, rows, chords)
+        return
+    if args.table_2ch:
+        table2ch_bench(eng, g, args, rows, chords, src, tgt)
         return
 
     d_src = DeviceArray.from_host(src)

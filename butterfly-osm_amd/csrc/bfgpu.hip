@@ -1034,7 +1034,7 @@ struct TwoChannel {
     DevBuf<uint32_t> pos_of_rank;
     // per call: labels [group][position][64 lanes], seeds, limits, outputs
     DevBuf<uint64_t> lab, s_lab;
-    DevBuf<uint32_t> s_off, s_pos, cur, lim, o_t, o_l;
+    DevBuf<uint32_t> s_off, s_pos, lim, o_t, o_l;
     uint64_t bytes = 0;
 };
 
@@ -1647,6 +1647,7 @@ int dense_rows(bfg_ctx *c, const Req &req, uint32_t *dist_host, uint64_t *bits_h
 // <= T is exact, the rest are masked to INF on output.
 // ---------------------------------------------------------------------------
 constexpr uint64_t INF2 = ~0ull;
+constexpr int T2_SEG_WAVES = 16;  // wavefronts sharing one segment tile
 
 __device__ __forceinline__ uint64_t add2(uint64_t a, uint64_t w) {
     const uint32_t t = sat_add((uint32_t)(a >> 32), (uint32_t)(w >> 32));
@@ -1664,51 +1665,72 @@ __global__ __launch_bounds__(NT) void t2_fill_kernel(uint64_t *__restrict__ p, s
     for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < count; i += (size_t)gridDim.x * NT) p[i] = INF2;
 }
 
-// One thread per lane: the lane's seeds (distinct positions), then its
+// One wavefront per lane: the lane's seeds (distinct positions), then its
 // upward phase (isochrone_handler.rs:527-568) as a walk over the union of the
 // seeds' root paths in descending position (an ancestor always has a smaller
 // position than its descendants, so every push into a node precedes its own
-// pushes).  cur[k] is the walker of seed k; walkers meeting at a node merge.
-// Only this thread touches the lane's labels in this kernel.
+// pushes).  wk[k] is the walker of seed k; walkers meeting at a node merge.
+// The wavefront's threads split a node's UP arcs (their heads are distinct);
+// a barrier orders one node's pushes before the next node's label is read.
 __global__ __launch_bounds__(64) void t2_up_kernel(uint32_t nl, uint32_t n, const uint32_t *__restrict__ s_off,
                                                    const uint32_t *__restrict__ s_pos,
-                                                   const uint64_t *__restrict__ s_lab, uint32_t *__restrict__ cur,
+                                                   const uint64_t *__restrict__ s_lab,
                                                    const uint32_t *__restrict__ lim,
                                                    const uint32_t *__restrict__ pos_parent,
                                                    const uint32_t *__restrict__ u_off,
                                                    const uint32_t *__restrict__ u_tgt,
                                                    const uint64_t *__restrict__ u_w, uint64_t *__restrict__ lab) {
-    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+    __shared__ uint32_t wk[64];
+    const uint32_t i = blockIdx.x, tid = threadIdx.x;
     if (i >= nl) return;
     uint64_t *col = lab + (size_t)(i / 64) * n * 64 + (i % 64);
-    const uint32_t a = s_off[i], b = s_off[i + 1], thr = lim[i];
-    for (uint32_t k = a; k < b; ++k) {
-        col[(size_t)s_pos[k] * 64] = s_lab[k];
-        cur[k] = s_pos[k];
+    const uint32_t a = s_off[i], ns = s_off[i + 1] - a, thr = lim[i];
+    if (tid < ns) {  // at most 64 seeds per lane (checked on the host)
+        col[(size_t)s_pos[a + tid] * 64] = s_lab[a + tid];
+        wk[tid] = s_pos[a + tid];
     }
+    __syncthreads();
     for (;;) {
-        uint32_t u = NONE;
-        for (uint32_t k = a; k < b; ++k)
-            if (cur[k] != NONE && (u == NONE || cur[k] > u)) u = cur[k];
+        uint32_t u = NONE;  // the same for every thread: all read the same walkers
+        for (uint32_t k = 0; k < ns; ++k) {
+            const uint32_t ck = wk[k];
+            if (ck != NONE && (u == NONE || ck > u)) u = ck;
+        }
         if (u == NONE) break;
+        const uint32_t par = pos_parent[u], e0 = u_off[u], e1 = u_off[u + 1];
         const uint64_t lu = col[(size_t)u * 64];
-        if (live2(lu, thr))
-            for (uint32_t e = u_off[u]; e < u_off[u + 1]; ++e) {
-                uint64_t &x = col[(size_t)u_tgt[e] * 64];
-                x = min(x, add2(lu, u_w[e]));
+        if (live2(lu, thr)) {
+            // eight arcs per thread and step: their loads in flight together
+            constexpr uint32_t U = 8;
+            for (uint32_t e = e0 + tid; e < e1; e += 64 * U) {
+                uint32_t v[U];
+                uint64_t w[U], x[U];
+#pragma unroll
+                for (uint32_t q = 0; q < U; ++q) {
+                    const uint32_t f = e + 64 * q;
+                    v[q] = f < e1 ? u_tgt[f] : NONE;
+                    w[q] = f < e1 ? u_w[f] : INF2;
+                }
+#pragma unroll
+                for (uint32_t q = 0; q < U; ++q) x[q] = v[q] != NONE ? col[(size_t)v[q] * 64] : INF2;
+#pragma unroll
+                for (uint32_t q = 0; q < U; ++q)
+                    if (v[q] != NONE) col[(size_t)v[q] * 64] = min(x[q], add2(lu, w[q]));
             }
-        const uint32_t par = pos_parent[u];
-        for (uint32_t k = a; k < b; ++k)
-            if (cur[k] == u) cur[k] = par;
+        }
+        __syncthreads();
+        if (tid < ns && wk[tid] == u) wk[tid] = par;
+        __syncthreads();
     }
 }
 
 // Downward phase over one segment level of the top region: a workgroup per
-// (segment, lane group).  The four wavefronts split the segment's arcs from
+// (segment, lane group).  Its NW wavefronts split the segment's arcs from
 // outside (grouped by source, so a source label is loaded once per run) and
 // merge into the LDS tile with atomic min; wavefront 0 then walks the chain,
 // parent first, adding the arcs inside the segment.
-__global__ __launch_bounds__(NT) void t2_seg_kernel(uint32_t s0, uint32_t n, uint32_t nl,
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void t2_seg_kernel(uint32_t s0, uint32_t n, uint32_t nl,
                                                     const uint32_t *__restrict__ seg_pos,
                                                     const uint32_t *__restrict__ x_off,
                                                     const uint32_t *__restrict__ x_sj,
@@ -1718,31 +1740,56 @@ __global__ __launch_bounds__(NT) void t2_seg_kernel(uint32_t s0, uint32_t n, uin
                                                     const uint64_t *__restrict__ i_w,
                                                     const uint32_t *__restrict__ lim, uint64_t *__restrict__ lab) {
     __shared__ uint64_t acc[SEG][64];
-    const uint32_t s = s0 + blockIdx.x, g = blockIdx.y, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t s = s0 + blockIdx.x, g = blockIdx.y, lane = threadIdx.x & 63;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t gl = g * 64 + lane, thr = gl < nl ? lim[gl] : 0;
     const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
     uint64_t *base = lab + (size_t)g * n * 64 + lane;
-    for (uint32_t j = w; j < len; j += NT / 64) acc[j][lane] = base[(size_t)(p0 + j) * 64];
+    for (uint32_t j = w; j < len; j += NW) acc[j][lane] = base[(size_t)(p0 + j) * 64];
     __syncthreads();
-    const uint32_t a0 = x_off[s], a1 = x_off[s + 1], per = (a1 - a0 + NT / 64 - 1) / (NT / 64);
+    const uint32_t a0 = x_off[s], a1 = x_off[s + 1], per = (a1 - a0 + NW - 1) / NW;
     const uint32_t e0 = min(a1, a0 + w * per), e1 = min(a1, e0 + per);
-    uint32_t last = NONE;
-    uint64_t ls = INF2;
-    for (uint32_t e = e0; e < e1; ++e) {
-        const uint32_t sj = x_sj[e], src = sj >> 6;
-        if (src != last) {
-            ls = base[(size_t)src * 64];
-            last = src;
+    // eight arcs per step, their label loads in flight together (arcs of one
+    // source hit the same line)
+    constexpr int U = 8;
+    uint32_t e = e0;
+    for (; e + U <= e1; e += U) {
+        uint32_t sj[U];
+        uint64_t ww[U], ls[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            sj[q] = x_sj[e + q];
+            ww[q] = x_w[e + q];
         }
+#pragma unroll
+        for (int q = 0; q < U; ++q) ls[q] = base[(size_t)(sj[q] >> 6) * 64];
+#pragma unroll
+        for (int q = 0; q < U; ++q)
+            if (live2(ls[q], thr))
+                atomicMin((unsigned long long *)&acc[sj[q] & 63][lane], (unsigned long long)add2(ls[q], ww[q]));
+    }
+    for (; e < e1; ++e) {
+        const uint32_t sj = x_sj[e];
+        const uint64_t ls = base[(size_t)(sj >> 6) * 64];
         if (live2(ls, thr)) atomicMin((unsigned long long *)&acc[sj & 63][lane], (unsigned long long)add2(ls, x_w[e]));
     }
     __syncthreads();
     if (w != 0) return;
     for (uint32_t i = 0; i < len; ++i) {
         uint64_t v = acc[i][lane];
-        for (uint32_t e = i_off[p0 + i]; e < i_off[p0 + i + 1]; ++e) {
-            const uint64_t lk = acc[i_k[e]][lane];
-            if (live2(lk, thr)) v = min(v, add2(lk, i_w[e]));
+        const uint32_t f0 = i_off[p0 + i], f1 = i_off[p0 + i + 1];
+        uint32_t f = f0;
+        for (; f + 4 <= f1; f += 4) {
+            uint64_t lk[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) lk[q] = acc[i_k[f + q]][lane];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (live2(lk[q], thr)) v = min(v, add2(lk[q], i_w[f + q]));
+        }
+        for (; f < f1; ++f) {
+            const uint64_t lk = acc[i_k[f]][lane];
+            if (live2(lk, thr)) v = min(v, add2(lk, i_w[f]));
         }
         acc[i][lane] = v;
         base[(size_t)(p0 + i) * 64] = v;
@@ -1993,16 +2040,16 @@ int phast_2ch(bfg_ctx *c, const std::vector<uint32_t> &soff, const std::vector<u
         HIP_TRY(t.s_pos.upload(pos, s));
         HIP_TRY(t.s_lab.upload(lb, s));
         HIP_TRY(t.lim.upload(lim, s));
-        HIP_TRY(t.cur.alloc(std::max<size_t>(pos.size(), 1)));
         const size_t cells = (size_t)bg * n * 64;
         t2_fill_kernel<<<(uint32_t)std::min<size_t>((cells + NT - 1) / NT, 1u << 20), NT, 0, s>>>(t.lab.p, cells);
-        t2_up_kernel<<<bg, 64, 0, s>>>(bl, n, t.s_off.p, t.s_pos.p, t.s_lab.p, t.cur.p, t.lim.p, c->pos_parent.p,
+        t2_up_kernel<<<bl, 64, 0, s>>>(bl, n, t.s_off.p, t.s_pos.p, t.s_lab.p, t.lim.p, c->pos_parent.p,
                                        t.u_off.p, t.u_tgt.p, t.u_w.p, t.lab.p);
         for (size_t lv = 0; lv + 1 < L.seglvl_seg.size(); ++lv) {
             const uint32_t s0 = L.seglvl_seg[lv], s1 = L.seglvl_seg[lv + 1];
             if (s1 > s0)
-                t2_seg_kernel<<<dim3(s1 - s0, bg), NT, 0, s>>>(s0, n, bl, c->seg_pos.p, t.x_off.p, t.x_sj.p,
-                                                               t.x_w.p, t.i_off.p, t.i_k.p, t.i_w.p, t.lim.p, t.lab.p);
+                t2_seg_kernel<T2_SEG_WAVES><<<dim3(s1 - s0, bg), T2_SEG_WAVES * 64, 0, s>>>(
+                    s0, n, bl, c->seg_pos.p, t.x_off.p, t.x_sj.p, t.x_w.p, t.i_off.p, t.i_k.p, t.i_w.p, t.lim.p,
+                    t.lab.p);
         }
         for (size_t lv = 0; lv + 1 < L.botlvl_pos.size(); ++lv) {
             const uint32_t p0 = L.botlvl_pos[lv], p1 = L.botlvl_pos[lv + 1];
@@ -2021,9 +2068,9 @@ int phast_2ch(bfg_ctx *c, const std::vector<uint32_t> &soff, const std::vector<u
 // Per lane, its seeds as the reference applies them (isochrone_handler.rs:
 // 522-526: a seed applies while strictly faster than what its rank holds, so
 // per rank the first of the fastest wins; an INF seed never applies).
-void seeds_2ch(const bfg::Layout &L, const uint32_t *off, const uint32_t *rank, const uint32_t *tm,
-               const uint32_t *ln, uint32_t nl, std::vector<uint32_t> &soff, std::vector<uint32_t> &spos,
-               std::vector<uint64_t> &slab) {
+int seeds_2ch(const bfg::Layout &L, const uint32_t *off, const uint32_t *rank, const uint32_t *tm,
+              const uint32_t *ln, uint32_t nl, std::vector<uint32_t> &soff, std::vector<uint32_t> &spos,
+              std::vector<uint64_t> &slab) {
     soff.assign(1, 0);
     for (uint32_t i = 0; i < nl; ++i) {
         const size_t first = spos.size();
@@ -2040,8 +2087,10 @@ void seeds_2ch(const bfg::Layout &L, const uint32_t *off, const uint32_t *rank, 
                 slab[j] = x;
             }
         }
+        if (spos.size() - first > 64) return fail(BFG_NOT_SUPPORTED, "more than 64 seed ranks in one lane");
         soff.push_back((uint32_t)spos.size());
     }
+    return BFG_SUCCESS;
 }
 
 void fill_info(const bfg::Layout &L, bfg_ctx_info *info) {
@@ -2359,7 +2408,8 @@ int bfg_phast_seeded_2ch(bfg_ctx *c, const uint32_t *seed_offsets, const uint32_
     if (rc) return rc;
     std::vector<uint32_t> soff, spos;
     std::vector<uint64_t> slab;
-    seeds_2ch(c->L, seed_offsets, seed_ranks, seed_times, seed_lens, L, soff, spos, slab);
+    rc = seeds_2ch(c->L, seed_offsets, seed_ranks, seed_times, seed_lens, L, soff, spos, slab);
+    if (rc) return rc;
     TwoChannel &t = *c->t2;
     const uint32_t n = c->L.n;
     return phast_2ch(c, soff, spos, slab, thresholds, L, [&](uint32_t l0, uint32_t bl, uint32_t bg) -> int {
@@ -2410,7 +2460,8 @@ int bfg_table_seeded_2ch(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t
     }
     std::vector<uint32_t> soff, spos;
     std::vector<uint64_t> slab;
-    seeds_2ch(c->L, src_offsets, src_ranks, src_part_times, src_part_lens, S, soff, spos, slab);
+    rc = seeds_2ch(c->L, src_offsets, src_ranks, src_part_times, src_part_lens, S, soff, spos, slab);
+    if (rc) return rc;
     std::vector<uint32_t> tpos(nt);
     for (uint32_t q = 0; q < nt; ++q) tpos[q] = c->L.pos_of_rank[tgt_ranks[q]];
     const std::vector<uint32_t> h_toff(tgt_offsets, tgt_offsets + T + 1), h_trank(tgt_ranks, tgt_ranks + nt),

@@ -188,8 +188,8 @@ int bfg_ctx_attach_length(bfg_ctx *ctx, const bfg_graph_desc *graph, const uint3
 
 /* run_phast_bounded_fast_seeded_2ch (isochrone_handler.rs:500-625) for n_lanes
  * independent queries: lane i has seeds (seed_ranks[k], seed_times[k],
- * seed_lens[k]) for k in [seed_offsets[i], seed_offsets[i+1]) and time limit
- * thresholds[i].  times[i * n_nodes + v] is the bounded time of the PHAST
+ * seed_lens[k]) for k in [seed_offsets[i], seed_offsets[i+1]) (at most 64
+ * distinct ranks) and time limit thresholds[i].  times[i * n_nodes + v] is the bounded time of the PHAST
  * above and lens[i * n_nodes + v] the length along it: the least length over
  * the fastest up-down paths, with per-rank seeds taken as the reference takes
  * them (the first of the fastest, :522-526); both BFG_INF where the time
@@ -200,7 +200,8 @@ int bfg_phast_seeded_2ch(bfg_ctx *ctx, const uint32_t *seed_offsets, const uint3
 
 /* Two-channel seeded S x T table: table_phast_lopsided_2ch
  * (matrix/bucket_ch.rs:6381-6520), the path table_seeded_bounded_routed_2ch
- * (:6329-6379) takes when phast_wins.  Seeds are (rank, part_time, part_len).
+ * (:6329-6379) takes when phast_wins.  Seeds are (rank, part_time, part_len),
+ * at most 64 distinct ranks per source.
  * The field of source i is bfg_phast_seeded_2ch of its seeds, bounded by
  * threshold + its largest part_time + the largest target part_time
  * (:6400-6415).  times/lens[i * n_tgt + j]: over target j's seeds in order,
