@@ -1,4 +1,4 @@
-"""Row streaming of matrices as Arrow IPC tiles.
+"""Row streaming of matrices: Arrow IPC tiles and Flight matrix rows.
 
 The wire format of the reference's matrix tiles (route/src/matrix/arrow_stream.rs:
 6-15, schema :102-110): one record per tile of (src_block_start u32,
@@ -11,6 +11,10 @@ each chunk's tiles while the next chunk runs: a worker thread keeps the device
 busy with chunk i + 1 while the caller packs and sends chunk i.  Values are
 the engine's u32 durations times `scale` (saturating, INF kept), so a caller
 holding decisecond weights passes scale=100 for milliseconds.
+
+stream_flight_matrix emits the Flight matrix rows (source_idx, target_idx,
+duration_ms, distance_m) from two-channel seeded tables, one record batch per
+block of sources.
 """
 import queue
 import threading
@@ -108,6 +112,60 @@ def write_matrix_ipc(engine, sources, targets, sink, tiles_per_batch=8, **kw):
             n_tiles += len(pending)
             n_bytes += sum(t[4].nbytes for t in pending)
     return n_tiles, n_bytes
+
+
+def flight_matrix_schema():
+    """The Flight matrix rows (route/src/server/flight.rs:265-272)."""
+    import pyarrow as pa
+    return pa.schema([
+        pa.field("source_idx", pa.uint32(), nullable=False),
+        pa.field("target_idx", pa.uint32(), nullable=False),
+        pa.field("duration_ms", pa.uint32(), nullable=False),
+        pa.field("distance_m", pa.uint32(), nullable=False),
+    ])
+
+
+def flight_matrix_batch(times, lens, src_idx, dst_idx, threshold=INF, sparse=False):
+    """build_matrix_tile_batch (flight.rs:505-576) for one tile: times and lens
+    are row-major len(src_idx) x len(dst_idx) (lens may be None: no length
+    weights).  A cell above threshold is INF in both columns; a finite time
+    goes out as milliseconds (saturating); sparse drops INF cells.  Returns a
+    record batch, or None when sparse drops every cell."""
+    import pyarrow as pa
+    src_idx = np.asarray(src_idx, dtype=np.uint32)
+    dst_idx = np.asarray(dst_idx, dtype=np.uint32)
+    t = np.asarray(times, dtype=np.uint32).reshape(src_idx.size, dst_idx.size)
+    t = np.where(t > np.uint32(threshold), np.uint32(INF), t)
+    fin = t != INF
+    ms = np.where(fin, np.minimum(t.astype(np.uint64) * 1000, INF), INF).astype(np.uint32)
+    if lens is None:
+        dist = np.full(t.shape, INF, dtype=np.uint32)
+    else:
+        dist = np.where(fin, np.asarray(lens, dtype=np.uint32).reshape(t.shape), np.uint32(INF))
+    si = np.repeat(src_idx, dst_idx.size).reshape(t.shape)
+    di = np.tile(dst_idx, src_idx.size).reshape(t.shape)
+    keep = fin if sparse else np.ones(t.shape, dtype=bool)
+    if not keep.any():
+        return None
+    return pa.record_batch([pa.array(si[keep], pa.uint32()), pa.array(di[keep], pa.uint32()),
+                            pa.array(ms[keep], pa.uint32()), pa.array(dist[keep], pa.uint32())],
+                           schema=flight_matrix_schema())
+
+
+def stream_flight_matrix(engine, src_seedsets, tgt_seedsets, threshold=INF, tile_sources=256, sparse=False):
+    """Yields the Flight matrix rows tile by tile: each block of sources runs as
+    one two-channel seeded table on the GPU (bfg_table_seeded_2ch; times and
+    lengths along them) and goes out as one record batch.  Same-edge
+    conflicts are INF here (the reference recomputes them with its bucket
+    engine)."""
+    T = len(tgt_seedsets)
+    dst = np.arange(T, dtype=np.uint32)
+    for s0 in range(0, len(src_seedsets), tile_sources):
+        block = src_seedsets[s0:s0 + tile_sources]
+        t, ln, _ = engine.table_seeded_bounded_routed_2ch(block, tgt_seedsets, threshold)
+        b = flight_matrix_batch(t, ln, np.arange(s0, s0 + len(block), dtype=np.uint32), dst, threshold, sparse)
+        if b is not None:
+            yield b
 
 
 def read_matrix_ipc(source, n_src, n_tgt):

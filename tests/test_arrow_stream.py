@@ -65,3 +65,92 @@ def test_ipc_stream_on_gpu(grid_config1):
         write_matrix_ipc(e, s, t, sink, chunk_sources=128, tile_sources=64, tile_targets=100)
     got = read_matrix_ipc(io.BytesIO(sink.getvalue()), 300, 250)
     assert np.array_equal(got.reshape(-1), Oracle(g).compute_matrix_flat_mt(s, t))
+
+
+# ---------------------------------------------------------------- Flight matrix rows
+
+class Oracle2chEngine:
+    """The oracle behind the engine's two-channel table method (CPU tests)."""
+
+    def __init__(self, g, ul, dl):
+        from oracle.oracle import Oracle
+        self.o = Oracle(g)
+        self.o.attach_length(ul, dl)
+        self.calls = 0
+
+    def table_seeded_bounded_routed_2ch(self, src, tgt, threshold=INF):
+        self.calls += 1
+        return self.o.table_phast_lopsided_2ch(src, tgt, threshold)
+
+
+def lengths(g, seed):
+    rng = np.random.default_rng(seed)
+    ul = rng.integers(0, 5000, g.m_up).astype(np.uint32)
+    dl = rng.integers(0, 5000, g.m_down).astype(np.uint32)
+    ul[g.up_w == INF] = INF
+    dl[g.dn_w == INF] = INF
+    return ul, dl
+
+
+def seedsets(g, k, seed):
+    rng = np.random.default_rng(seed)
+    return [[(int(r), int(p), int(q))] for r, p, q in zip(rng.integers(0, g.n, k), rng.integers(0, 20, k),
+                                                           rng.integers(0, 200, k))]
+
+
+def test_flight_schema_matches_the_reference():
+    from bfgpu.arrow_stream import flight_matrix_schema
+    assert [(f.name, str(f.type), f.nullable) for f in flight_matrix_schema()] == [
+        ("source_idx", "uint32", False), ("target_idx", "uint32", False),
+        ("duration_ms", "uint32", False), ("distance_m", "uint32", False)]
+
+
+def test_flight_batch_cells():
+    """flight.rs:532-560: > threshold is INF in both columns, seconds to
+    saturating milliseconds, sparse drops INF cells, no lengths -> INF."""
+    from bfgpu.arrow_stream import flight_matrix_batch
+    t = np.array([[5, INF], [5_000_000, 300]], dtype=np.uint32)
+    ln = np.array([[70, 80], [90, 4000]], dtype=np.uint32)
+    b = flight_matrix_batch(t, ln, [10, 11], [3, 4], threshold=INF).to_pydict()
+    assert b["source_idx"] == [10, 10, 11, 11] and b["target_idx"] == [3, 4, 3, 4]
+    assert b["duration_ms"] == [5000, INF, INF, 300000]  # 5e9 ms saturates
+    assert b["distance_m"] == [70, INF, 90, 4000]
+    b = flight_matrix_batch(t, ln, [10, 11], [3, 4], threshold=200, sparse=True).to_pydict()
+    assert b == {"source_idx": [10], "target_idx": [3], "duration_ms": [5000], "distance_m": [70]}
+    b = flight_matrix_batch(t, None, [10, 11], [3, 4]).to_pydict()
+    assert b["distance_m"] == [INF] * 4
+    assert flight_matrix_batch(np.full((1, 2), INF, np.uint32), None, [0], [0, 1], sparse=True) is None
+
+
+def test_flight_stream_equals_the_table(grid_small):
+    from bfgpu.arrow_stream import stream_flight_matrix
+    ul, dl = lengths(grid_small, 4)
+    eng = Oracle2chEngine(grid_small, ul, dl)
+    src, tgt = seedsets(grid_small, 13, 1), seedsets(grid_small, 9, 2)
+    et, el, _ = eng.o.table_phast_lopsided_2ch(src, tgt, 600)
+    rows = {}
+    for b in stream_flight_matrix(eng, src, tgt, threshold=600, tile_sources=5, sparse=True):
+        d = b.to_pydict()
+        for s, t, ms, m in zip(d["source_idx"], d["target_idx"], d["duration_ms"], d["distance_m"]):
+            assert (s, t) not in rows
+            rows[(s, t)] = (ms, m)
+    assert eng.calls == 3
+    exp = {(i, j): (int(et[i * 9 + j]) * 1000, int(el[i * 9 + j]))
+           for i in range(13) for j in range(9) if et[i * 9 + j] != INF}
+    assert rows == exp
+
+
+@pytest.mark.gpu
+def test_gpu_flight_stream_equals_the_oracle(grid_small):
+    from bfgpu import BatchedPhastEngine
+    from bfgpu.arrow_stream import stream_flight_matrix
+    ul, dl = lengths(grid_small, 5)
+    ref = Oracle2chEngine(grid_small, ul, dl)
+    src, tgt = seedsets(grid_small, 70, 3), seedsets(grid_small, 40, 4)
+    with BatchedPhastEngine(grid_small) as e:
+        e.attach_length(ul, dl)
+        got = list(stream_flight_matrix(e, src, tgt, threshold=INF, tile_sources=64))
+    exp = list(stream_flight_matrix(ref, src, tgt, threshold=INF, tile_sources=64))
+    assert len(got) == len(exp) == 2
+    for a, b in zip(got, exp):
+        assert a.equals(b)
