@@ -1034,6 +1034,7 @@ struct TwoChannel {
     DevBuf<uint32_t> pos_of_rank;
     // per call: labels [group][position][64 lanes], seeds, limits, outputs
     DevBuf<uint64_t> lab, s_lab;
+    DevBuf<uint64_t> top;  // the upward phase's top-region labels, [lane][top position]
     DevBuf<uint32_t> s_off, s_pos, lim, o_t, o_l;
     uint64_t bytes = 0;
 };
@@ -1672,21 +1673,28 @@ __global__ __launch_bounds__(NT) void t2_fill_kernel(uint64_t *__restrict__ p, s
 // pushes).  wk[k] is the walker of seed k; walkers meeting at a node merge.
 // The wavefront's threads split a node's UP arcs (their heads are distinct);
 // a barrier orders one node's pushes before the next node's label is read.
-__global__ __launch_bounds__(64) void t2_up_kernel(uint32_t nl, uint32_t n, const uint32_t *__restrict__ s_off,
+// Top-region labels go to the lane's own row of `top` (a node's ancestors in
+// one segment are 64 adjacent words there, so a node's pushes touch few
+// lines and the walk's labels stay in cache); bottom labels to the arena.
+__global__ __launch_bounds__(64) void t2_up_kernel(uint32_t nl, uint32_t n, uint32_t n_top,
+                                                   const uint32_t *__restrict__ s_off,
                                                    const uint32_t *__restrict__ s_pos,
                                                    const uint64_t *__restrict__ s_lab,
                                                    const uint32_t *__restrict__ lim,
                                                    const uint32_t *__restrict__ pos_parent,
                                                    const uint32_t *__restrict__ u_off,
                                                    const uint32_t *__restrict__ u_tgt,
-                                                   const uint64_t *__restrict__ u_w, uint64_t *__restrict__ lab) {
+                                                   const uint64_t *__restrict__ u_w, uint64_t *__restrict__ lab,
+                                                   uint64_t *__restrict__ top) {
     __shared__ uint32_t wk[64];
     const uint32_t i = blockIdx.x, tid = threadIdx.x;
     if (i >= nl) return;
     uint64_t *col = lab + (size_t)(i / 64) * n * 64 + (i % 64);
+    uint64_t *row = top + (size_t)i * n_top;
+    auto at = [&](uint32_t p) -> uint64_t & { return p < n_top ? row[p] : col[(size_t)p * 64]; };
     const uint32_t a = s_off[i], ns = s_off[i + 1] - a, thr = lim[i];
     if (tid < ns) {  // at most 64 seeds per lane (checked on the host)
-        col[(size_t)s_pos[a + tid] * 64] = s_lab[a + tid];
+        at(s_pos[a + tid]) = s_lab[a + tid];
         wk[tid] = s_pos[a + tid];
     }
     __syncthreads();
@@ -1698,7 +1706,7 @@ __global__ __launch_bounds__(64) void t2_up_kernel(uint32_t nl, uint32_t n, cons
         }
         if (u == NONE) break;
         const uint32_t par = pos_parent[u], e0 = u_off[u], e1 = u_off[u + 1];
-        const uint64_t lu = col[(size_t)u * 64];
+        const uint64_t lu = at(u);
         if (live2(lu, thr)) {
             // eight arcs per thread and step: their loads in flight together
             constexpr uint32_t U = 8;
@@ -1712,15 +1720,33 @@ __global__ __launch_bounds__(64) void t2_up_kernel(uint32_t nl, uint32_t n, cons
                     w[q] = f < e1 ? u_w[f] : INF2;
                 }
 #pragma unroll
-                for (uint32_t q = 0; q < U; ++q) x[q] = v[q] != NONE ? col[(size_t)v[q] * 64] : INF2;
+                for (uint32_t q = 0; q < U; ++q) x[q] = v[q] != NONE ? at(v[q]) : INF2;
 #pragma unroll
                 for (uint32_t q = 0; q < U; ++q)
-                    if (v[q] != NONE) col[(size_t)v[q] * 64] = min(x[q], add2(lu, w[q]));
+                    if (v[q] != NONE) at(v[q]) = min(x[q], add2(lu, w[q]));
             }
         }
         __syncthreads();
         if (tid < ns && wk[tid] == u) wk[tid] = par;
         __syncthreads();
+    }
+}
+
+// The upward phase's top-region rows into the arena: a workgroup per (64 top
+// positions, lane group), transposed through LDS.
+__global__ __launch_bounds__(NT) void t2_top_kernel(uint32_t n, uint32_t n_top, const uint64_t *__restrict__ top,
+                                                    uint64_t *__restrict__ lab) {
+    __shared__ uint64_t tile[64][65];
+    const uint32_t p0 = blockIdx.x * 64, g = blockIdx.y;
+    for (uint32_t idx = threadIdx.x; idx < 64 * 64; idx += NT) {
+        const uint32_t l = idx / 64, j = idx % 64;
+        if (p0 + j < n_top) tile[j][l] = top[(size_t)(g * 64 + l) * n_top + p0 + j];
+    }
+    __syncthreads();
+    uint64_t *base = lab + (size_t)g * n * 64;
+    for (uint32_t idx = threadIdx.x; idx < 64 * 64; idx += NT) {
+        const uint32_t j = idx / 64, l = idx % 64;
+        if (p0 + j < n_top) base[(size_t)(p0 + j) * 64 + l] = tile[j][l];
     }
 }
 
@@ -1739,57 +1765,75 @@ __global__ __launch_bounds__(NW * 64) void t2_seg_kernel(uint32_t s0, uint32_t n
                                                     const uint32_t *__restrict__ i_k,
                                                     const uint64_t *__restrict__ i_w,
                                                     const uint32_t *__restrict__ lim, uint64_t *__restrict__ lab) {
+    constexpr uint32_t MAXI = SEG * (SEG - 1) / 2;  // arcs inside a segment (one per ordered pair)
     __shared__ uint64_t acc[SEG][64];
+    __shared__ uint64_t iw[MAXI];
+    __shared__ uint32_t ik[MAXI], io[SEG + 1];
     const uint32_t s = s0 + blockIdx.x, g = blockIdx.y, lane = threadIdx.x & 63;
     const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t gl = g * 64 + lane, thr = gl < nl ? lim[gl] : 0;
     const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
     uint64_t *base = lab + (size_t)g * n * 64 + lane;
     for (uint32_t j = w; j < len; j += NW) acc[j][lane] = base[(size_t)(p0 + j) * 64];
+    // the chain's arcs into LDS, so wavefront 0 walks it without memory waits
+    const uint32_t i0 = i_off[p0], ni = i_off[p0 + len] - i0;
+    for (uint32_t k = threadIdx.x; k < ni; k += NW * 64) {
+        ik[k] = i_k[i0 + k];
+        iw[k] = i_w[i0 + k];
+    }
+    for (uint32_t k = threadIdx.x; k <= len; k += NW * 64) io[k] = i_off[p0 + k] - i0;
     __syncthreads();
     const uint32_t a0 = x_off[s], a1 = x_off[s + 1], per = (a1 - a0 + NW - 1) / NW;
     const uint32_t e0 = min(a1, a0 + w * per), e1 = min(a1, e0 + per);
-    // eight arcs per step, their label loads in flight together (arcs of one
-    // source hit the same line)
-    constexpr int U = 8;
-    uint32_t e = e0;
-    for (; e + U <= e1; e += U) {
-        uint32_t sj[U];
-        uint64_t ww[U], ls[U];
+    // 64 arcs' words per vector load, broadcast by v_readlane; eight arcs per
+    // step with their label loads in flight together (arcs of one source hit
+    // the same line)
+    constexpr uint32_t U = 8;
+    for (uint32_t c0 = e0; c0 < e1; c0 += 64) {
+        const uint32_t cl = min(64u, e1 - c0);
+        const uint32_t my_sj = lane < cl ? x_sj[c0 + lane] : 0u;
+        const uint64_t my_w = lane < cl ? x_w[c0 + lane] : INF2;
+        const uint32_t my_wt = (uint32_t)(my_w >> 32), my_wl = (uint32_t)my_w;
+        uint32_t j = 0;
+        for (; j + U <= cl; j += U) {
+            uint32_t sj[U];
+            uint64_t ww[U], ls[U];
 #pragma unroll
-        for (int q = 0; q < U; ++q) {
-            sj[q] = x_sj[e + q];
-            ww[q] = x_w[e + q];
+            for (uint32_t q = 0; q < U; ++q) {
+                sj[q] = rl(my_sj, j + q);
+                ww[q] = (uint64_t)rl(my_wt, j + q) << 32 | rl(my_wl, j + q);
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < U; ++q) ls[q] = base[(size_t)(sj[q] >> 6) * 64];
+#pragma unroll
+            for (uint32_t q = 0; q < U; ++q)
+                if (live2(ls[q], thr))
+                    atomicMin((unsigned long long *)&acc[sj[q] & 63][lane], (unsigned long long)add2(ls[q], ww[q]));
         }
-#pragma unroll
-        for (int q = 0; q < U; ++q) ls[q] = base[(size_t)(sj[q] >> 6) * 64];
-#pragma unroll
-        for (int q = 0; q < U; ++q)
-            if (live2(ls[q], thr))
-                atomicMin((unsigned long long *)&acc[sj[q] & 63][lane], (unsigned long long)add2(ls[q], ww[q]));
-    }
-    for (; e < e1; ++e) {
-        const uint32_t sj = x_sj[e];
-        const uint64_t ls = base[(size_t)(sj >> 6) * 64];
-        if (live2(ls, thr)) atomicMin((unsigned long long *)&acc[sj & 63][lane], (unsigned long long)add2(ls, x_w[e]));
+        for (; j < cl; ++j) {
+            const uint32_t sj = rl(my_sj, j);
+            const uint64_t ww = (uint64_t)rl(my_wt, j) << 32 | rl(my_wl, j);
+            const uint64_t ls = base[(size_t)(sj >> 6) * 64];
+            if (live2(ls, thr)) atomicMin((unsigned long long *)&acc[sj & 63][lane], (unsigned long long)add2(ls, ww));
+        }
     }
     __syncthreads();
     if (w != 0) return;
     for (uint32_t i = 0; i < len; ++i) {
         uint64_t v = acc[i][lane];
-        const uint32_t f0 = i_off[p0 + i], f1 = i_off[p0 + i + 1];
-        uint32_t f = f0;
+        const uint32_t f1 = io[i + 1];
+        uint32_t f = io[i];
         for (; f + 4 <= f1; f += 4) {
             uint64_t lk[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) lk[q] = acc[i_k[f + q]][lane];
+            for (int q = 0; q < 4; ++q) lk[q] = acc[ik[f + q]][lane];
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if (live2(lk[q], thr)) v = min(v, add2(lk[q], i_w[f + q]));
+                if (live2(lk[q], thr)) v = min(v, add2(lk[q], iw[f + q]));
         }
         for (; f < f1; ++f) {
-            const uint64_t lk = acc[i_k[f]][lane];
-            if (live2(lk, thr)) v = min(v, add2(lk, i_w[f]));
+            const uint64_t lk = acc[ik[f]][lane];
+            if (live2(lk, thr)) v = min(v, add2(lk, iw[f]));
         }
         acc[i][lane] = v;
         base[(size_t)(p0 + i) * 64] = v;
@@ -1797,14 +1841,18 @@ __global__ __launch_bounds__(NW * 64) void t2_seg_kernel(uint32_t s0, uint32_t n
 }
 
 // Downward phase over one height level of the bottom region: a wavefront per
-// (node, lane group) pulls the node's DOWN in-arcs.
-__global__ __launch_bounds__(NT) void t2_pull_kernel(uint32_t p0, uint32_t p1, uint32_t n, uint32_t n_top, uint32_t nl,
+// (node, lane group) pulls the node's DOWN in-arcs.  The level's nodes are
+// positions [p0, p1), or plist[p0 .. p1) when a table needs only the bottom
+// nodes above its target seeds.
+__global__ __launch_bounds__(NT) void t2_pull_kernel(uint32_t p0, uint32_t p1, const uint32_t *__restrict__ plist,
+                                                     uint32_t n, uint32_t n_top, uint32_t nl,
                                                      const uint32_t *__restrict__ b_off,
                                                      const uint32_t *__restrict__ b_src,
                                                      const uint64_t *__restrict__ b_w,
                                                      const uint32_t *__restrict__ lim, uint64_t *__restrict__ lab) {
-    const uint32_t p = p0 + blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
-    if (p >= p1) return;
+    const uint32_t k = p0 + blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    if (k >= p1) return;
+    const uint32_t p = plist ? plist[k] : k;
     const uint32_t lane = threadIdx.x & 63, g = blockIdx.y, gl = g * 64 + lane, thr = gl < nl ? lim[gl] : 0;
     uint64_t *base = lab + (size_t)g * n * 64 + lane;
     uint64_t v = base[(size_t)p * 64];
@@ -1934,6 +1982,9 @@ int attach_length(bfg_ctx *c, const bfg_graph_desc *gd, const uint32_t *up_len, 
     for (size_t k = 1; k < x_off.size(); ++k) x_off[k] += x_off[k - 1];
     for (size_t k = 1; k < i_off.size(); ++k) i_off[k] += i_off[k - 1];
     for (size_t k = 1; k < b_off.size(); ++k) b_off[k] += b_off[k - 1];
+    for (uint32_t sg = 0; sg < nseg; ++sg)
+        if (i_off[L.seg_pos[sg + 1]] - i_off[L.seg_pos[sg]] > SEG * (SEG - 1) / 2)
+            return fail(BFG_INTERNAL_ERROR, "more arcs inside a segment than node pairs");
     std::vector<uint32_t> x_sj(x_off[nseg]), i_k(i_off[nt]), b_src(b_off[n - nt]);
     std::vector<uint64_t> x_w(x_sj.size()), i_w(i_k.size()), b_w(b_src.size());
     {
@@ -2006,25 +2057,41 @@ int attach_length(bfg_ctx *c, const bfg_graph_desc *gd, const uint32_t *up_len, 
 // seeds per lane are distinct positions with their (time, length) labels.
 // After each batch's sweep, emit(first lane, lanes, groups) reads the labels
 // (t.lab, limits in t.lim) on the context stream.
+// The bottom nodes a sweep covers: all of them, or per bottom level l the
+// positions d_pos[off[l] .. off[l+1]) (off on the host, d_pos on the device).
+struct BottomSet {
+    const std::vector<uint32_t> *off = nullptr;
+    const uint32_t *d_pos = nullptr;
+};
+
 template <class Emit>
 int phast_2ch(bfg_ctx *c, const std::vector<uint32_t> &soff, const std::vector<uint32_t> &spos,
-              const std::vector<uint64_t> &slab, const uint32_t *limits, uint32_t nl, Emit emit) {
+              const std::vector<uint64_t> &slab, const uint32_t *limits, uint32_t nl, bool dense, BottomSet bs,
+              Emit emit) {
     TwoChannel &t = *c->t2;
     const bfg::Layout &L = c->L;
     const uint32_t n = L.n;
     hipStream_t s = c->stream;
-    // groups per batch: labels and both output channels fit a share of free memory
+    // groups per batch: the labels (and for dense rows both output channels)
+    // fit a share of free memory; a sweep's time barely grows with its groups
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
-    const size_t per_group = (size_t)n * 64 * 16;
+    const size_t per_group = (size_t)n * 64 * (dense ? 16 : 8) + (size_t)L.n_top * 64 * 8;
+    const size_t held = (t.lab.n + t.top.n) * 8 + (t.o_t.n + t.o_l.n) * 4;
     const uint32_t want = (nl + 63) / 64;
-    const uint32_t gmax = (uint32_t)std::max<size_t>(1, std::min<size_t>(64, (free_b + t.lab.n * 8) / 3 / per_group));
+    const uint32_t gmax = (uint32_t)std::max<size_t>(1, std::min<size_t>(64, (free_b + held) / 2 / per_group));
     const uint32_t G = std::min(want, gmax);
     if (t.lab.n < (size_t)G * n * 64) {
         t.lab.release();
+        t.top.release();
         t.o_t.release();
         t.o_l.release();
         HIP_TRY(t.lab.alloc((size_t)G * n * 64));
+        HIP_TRY(t.top.alloc((size_t)G * 64 * std::max<uint32_t>(L.n_top, 1)));
+    }
+    if (dense && t.o_t.n < (size_t)G * 64 * n) {
+        t.o_t.release();
+        t.o_l.release();
         HIP_TRY(t.o_t.alloc((size_t)G * 64 * n));
         HIP_TRY(t.o_l.alloc((size_t)G * 64 * n));
     }
@@ -2041,9 +2108,15 @@ int phast_2ch(bfg_ctx *c, const std::vector<uint32_t> &soff, const std::vector<u
         HIP_TRY(t.s_lab.upload(lb, s));
         HIP_TRY(t.lim.upload(lim, s));
         const size_t cells = (size_t)bg * n * 64;
+        const size_t tcells = (size_t)bg * 64 * L.n_top;
         t2_fill_kernel<<<(uint32_t)std::min<size_t>((cells + NT - 1) / NT, 1u << 20), NT, 0, s>>>(t.lab.p, cells);
-        t2_up_kernel<<<bl, 64, 0, s>>>(bl, n, t.s_off.p, t.s_pos.p, t.s_lab.p, t.lim.p, c->pos_parent.p,
-                                       t.u_off.p, t.u_tgt.p, t.u_w.p, t.lab.p);
+        if (tcells)
+            t2_fill_kernel<<<(uint32_t)std::min<size_t>((tcells + NT - 1) / NT, 1u << 20), NT, 0, s>>>(t.top.p,
+                                                                                                     tcells);
+        t2_up_kernel<<<bl, 64, 0, s>>>(bl, n, L.n_top, t.s_off.p, t.s_pos.p, t.s_lab.p, t.lim.p, c->pos_parent.p,
+                                       t.u_off.p, t.u_tgt.p, t.u_w.p, t.lab.p, t.top.p);
+        if (L.n_top)
+            t2_top_kernel<<<dim3((L.n_top + 63) / 64, bg), NT, 0, s>>>(n, L.n_top, t.top.p, t.lab.p);
         for (size_t lv = 0; lv + 1 < L.seglvl_seg.size(); ++lv) {
             const uint32_t s0 = L.seglvl_seg[lv], s1 = L.seglvl_seg[lv + 1];
             if (s1 > s0)
@@ -2052,10 +2125,11 @@ int phast_2ch(bfg_ctx *c, const std::vector<uint32_t> &soff, const std::vector<u
                     t.lab.p);
         }
         for (size_t lv = 0; lv + 1 < L.botlvl_pos.size(); ++lv) {
-            const uint32_t p0 = L.botlvl_pos[lv], p1 = L.botlvl_pos[lv + 1];
+            const uint32_t p0 = bs.off ? (*bs.off)[lv] : L.botlvl_pos[lv];
+            const uint32_t p1 = bs.off ? (*bs.off)[lv + 1] : L.botlvl_pos[lv + 1];
             if (p1 > p0)
                 t2_pull_kernel<<<dim3((p1 - p0 + NT / 64 - 1) / (NT / 64), bg), NT, 0, s>>>(
-                    p0, p1, n, L.n_top, bl, t.b_off.p, t.b_src.p, t.b_w.p, t.lim.p, t.lab.p);
+                    p0, p1, bs.d_pos, n, L.n_top, bl, t.b_off.p, t.b_src.p, t.b_w.p, t.lim.p, t.lab.p);
         }
         HIP_TRY(hipGetLastError());
         const int rc = emit(l0, bl, bg);
@@ -2412,7 +2486,8 @@ int bfg_phast_seeded_2ch(bfg_ctx *c, const uint32_t *seed_offsets, const uint32_
     if (rc) return rc;
     TwoChannel &t = *c->t2;
     const uint32_t n = c->L.n;
-    return phast_2ch(c, soff, spos, slab, thresholds, L, [&](uint32_t l0, uint32_t bl, uint32_t bg) -> int {
+    return phast_2ch(c, soff, spos, slab, thresholds, L, true, BottomSet{},
+                     [&](uint32_t l0, uint32_t bl, uint32_t bg) -> int {
         hipStream_t s = c->stream;
         t2_out_kernel<<<dim3((n + 63) / 64, bg), NT, 0, s>>>(n, bl, t.pos_of_rank.p, t.lim.p, t.lab.p, t.o_t.p,
                                                             t.o_l.p);
@@ -2464,10 +2539,23 @@ int bfg_table_seeded_2ch(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t
     if (rc) return rc;
     std::vector<uint32_t> tpos(nt);
     for (uint32_t q = 0; q < nt; ++q) tpos[q] = c->L.pos_of_rank[tgt_ranks[q]];
+    // the field is read at the target seeds only: their labels depend on their
+    // ancestors alone, so the bottom sweep covers the bottom nodes above them
+    const bfg::Layout &Ly = c->L;
+    std::vector<uint8_t> need(Ly.n - Ly.n_top, 0);
+    for (uint32_t q = 0; q < nt; ++q)
+        for (uint32_t p = tpos[q]; p != NONE && p >= Ly.n_top && !need[p - Ly.n_top]; p = Ly.pos_parent[p])
+            need[p - Ly.n_top] = 1;
+    std::vector<uint32_t> bpos, boff(1, 0);
+    for (size_t lv = 0; lv + 1 < Ly.botlvl_pos.size(); ++lv) {
+        for (uint32_t p = Ly.botlvl_pos[lv]; p < Ly.botlvl_pos[lv + 1]; ++p)
+            if (need[p - Ly.n_top]) bpos.push_back(p);
+        boff.push_back((uint32_t)bpos.size());
+    }
     const std::vector<uint32_t> h_toff(tgt_offsets, tgt_offsets + T + 1), h_trank(tgt_ranks, tgt_ranks + nt),
         h_tpt(tgt_part_times, tgt_part_times + nt), h_tpl(tgt_part_lens, tgt_part_lens + nt),
         h_soff(src_offsets, src_offsets + S + 1), h_srank(src_ranks, src_ranks + ns);
-    DevBuf<uint32_t> d_toff, d_tpos, d_trank, d_tpt, d_tpl, d_soff, d_srank, d_t, d_l;
+    DevBuf<uint32_t> d_toff, d_tpos, d_trank, d_tpt, d_tpl, d_soff, d_srank, d_t, d_l, d_bpos;
     DevBuf<uint8_t> d_conf;
     hipStream_t st = c->stream;
     HIP_TRY(d_toff.upload(h_toff, st));
@@ -2477,11 +2565,13 @@ int bfg_table_seeded_2ch(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t
     HIP_TRY(d_tpl.upload(h_tpl, st));
     HIP_TRY(d_soff.upload(h_soff, st));
     HIP_TRY(d_srank.upload(h_srank, st));
+    HIP_TRY(d_bpos.upload(bpos, st));
     HIP_TRY(d_t.alloc((size_t)S * T));
     HIP_TRY(d_l.alloc((size_t)S * T));
     if (conflicts) HIP_TRY(d_conf.alloc((size_t)S * T));
     HIP_TRY(hipStreamSynchronize(st));  // before the host tables above may go
-    rc = phast_2ch(c, soff, spos, slab, bound.data(), S, [&](uint32_t l0, uint32_t bl, uint32_t bg) -> int {
+    rc = phast_2ch(c, soff, spos, slab, bound.data(), S, false, BottomSet{&boff, d_bpos.p},
+                   [&](uint32_t l0, uint32_t bl, uint32_t bg) -> int {
         // this batch's lanes are sources [l0, l0 + bl): offset the per-source tables
         t2_table_kernel<<<dim3((T + 63) / 64, bg), NT, 0, st>>>(
             c->L.n, T, bl, d_toff.p, d_tpos.p, d_trank.p, d_tpt.p, d_tpl.p, d_soff.p + l0, d_srank.p,
