@@ -1698,37 +1698,53 @@ __global__ __launch_bounds__(64) void t2_up_kernel(uint32_t nl, uint32_t n, uint
         wk[tid] = s_pos[a + tid];
     }
     __syncthreads();
-    for (;;) {
-        uint32_t u = NONE;  // the same for every thread: all read the same walkers
+    // the same for every thread: all read the same walkers
+    auto next = [&](uint32_t at_u, uint32_t to) {  // the walk's next node once walkers at at_u move to `to`
+        uint32_t m = NONE;
         for (uint32_t k = 0; k < ns; ++k) {
-            const uint32_t ck = wk[k];
-            if (ck != NONE && (u == NONE || ck > u)) u = ck;
+            uint32_t ck = wk[k];
+            if (ck == at_u) ck = to;
+            if (ck != NONE && (m == NONE || ck > m)) m = ck;
         }
-        if (u == NONE) break;
-        const uint32_t par = pos_parent[u], e0 = u_off[u], e1 = u_off[u + 1];
-        const uint64_t lu = at(u);
-        if (live2(lu, thr)) {
-            // eight arcs per thread and step: their loads in flight together
-            constexpr uint32_t U = 8;
-            for (uint32_t e = e0 + tid; e < e1; e += 64 * U) {
-                uint32_t v[U];
-                uint64_t w[U], x[U];
+        return m;
+    };
+    uint32_t u = next(NONE, NONE);
+    if (u == NONE) return;
+    uint32_t par = pos_parent[u], e0 = u_off[u], e1 = u_off[u + 1];
+    constexpr uint32_t U = 8;  // arcs per thread and step, their loads in flight together
+    while (u != NONE) {
+        // the next node and its tables do not depend on labels: load them now
+        const uint32_t un = next(u, par);
+        uint32_t parn = NONE, e0n = 0, e1n = 0;
+        if (un != NONE) {
+            parn = pos_parent[un];
+            e0n = u_off[un];
+            e1n = u_off[un + 1];
+        }
+        for (uint32_t e = e0 + tid; e < e1; e += 64 * U) {
+            uint32_t v[U];
+            uint64_t w[U], x[U];
 #pragma unroll
-                for (uint32_t q = 0; q < U; ++q) {
-                    const uint32_t f = e + 64 * q;
-                    v[q] = f < e1 ? u_tgt[f] : NONE;
-                    w[q] = f < e1 ? u_w[f] : INF2;
-                }
-#pragma unroll
-                for (uint32_t q = 0; q < U; ++q) x[q] = v[q] != NONE ? at(v[q]) : INF2;
-#pragma unroll
-                for (uint32_t q = 0; q < U; ++q)
-                    if (v[q] != NONE) at(v[q]) = min(x[q], add2(lu, w[q]));
+            for (uint32_t q = 0; q < U; ++q) {
+                const uint32_t f = e + 64 * q;
+                v[q] = f < e1 ? u_tgt[f] : NONE;
+                w[q] = f < e1 ? u_w[f] : INF2;
             }
+            const uint64_t lu = at(u);
+            if (!live2(lu, thr)) break;
+#pragma unroll
+            for (uint32_t q = 0; q < U; ++q) x[q] = v[q] != NONE ? at(v[q]) : INF2;
+#pragma unroll
+            for (uint32_t q = 0; q < U; ++q)
+                if (v[q] != NONE) at(v[q]) = min(x[q], add2(lu, w[q]));
         }
         __syncthreads();
         if (tid < ns && wk[tid] == u) wk[tid] = par;
         __syncthreads();
+        u = un;
+        par = parn;
+        e0 = e0n;
+        e1 = e1n;
     }
 }
 
@@ -1802,6 +1818,15 @@ __global__ __launch_bounds__(NW * 64) void t2_seg_kernel(uint32_t s0, uint32_t n
             for (uint32_t q = 0; q < U; ++q) {
                 sj[q] = rl(my_sj, j + q);
                 ww[q] = (uint64_t)rl(my_wt, j + q) << 32 | rl(my_wl, j + q);
+            }
+            if ((sj[0] >> 6) == (sj[U - 1] >> 6)) {  // one source (arcs come grouped by source): one load
+                const uint64_t l1 = base[(size_t)(sj[0] >> 6) * 64];
+                if (live2(l1, thr)) {
+#pragma unroll
+                    for (uint32_t q = 0; q < U; ++q)
+                        atomicMin((unsigned long long *)&acc[sj[q] & 63][lane], (unsigned long long)add2(l1, ww[q]));
+                }
+                continue;
             }
 #pragma unroll
             for (uint32_t q = 0; q < U; ++q) ls[q] = base[(size_t)(sj[q] >> 6) * 64];
