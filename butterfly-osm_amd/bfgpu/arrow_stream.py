@@ -14,7 +14,7 @@ holding decisecond weights passes scale=100 for milliseconds.
 
 stream_flight_matrix emits the Flight matrix rows (source_idx, target_idx,
 duration_ms, distance_m) from two-channel seeded tables, one record batch per
-block of sources.
+block of sources; write_flight_matrix_ipc adds the completeness trailer.
 """
 import queue
 import threading
@@ -166,6 +166,51 @@ def stream_flight_matrix(engine, src_seedsets, tgt_seedsets, threshold=INF, tile
         b = flight_matrix_batch(t, ln, np.arange(s0, s0 + len(block), dtype=np.uint32), dst, threshold, sparse)
         if b is not None:
             yield b
+
+
+def write_flight_matrix_ipc(engine, src_seedsets, tgt_seedsets, sink, threshold=INF, tile_sources=256,
+                            sparse=False):
+    """The Flight matrix rows as an Arrow IPC stream with the completeness
+    trailer of completed_flight_stream (flight.rs:3612-3658): once every tile is
+    out, one empty record batch whose custom metadata `app_metadata` is
+    {"complete":true,"total_rows":N,"contract":"dense"|"sparse"}.  A producer
+    that fails writes no trailer (its exception propagates), so a reader can
+    tell a truncated stream from a complete one.  Returns the row count."""
+    import pyarrow as pa
+    schema = flight_matrix_schema()
+    rows = 0
+    with pa.ipc.new_stream(sink, schema) as w:
+        for b in stream_flight_matrix(engine, src_seedsets, tgt_seedsets, threshold, tile_sources, sparse):
+            w.write_batch(b)
+            rows += b.num_rows
+        meta = '{"complete":true,"total_rows":%d,"contract":"%s"}' % (rows, "sparse" if sparse else "dense")
+        empty = pa.record_batch([pa.array([], f.type) for f in schema], schema=schema)
+        w.write_batch(empty, custom_metadata={"app_metadata": meta})
+    return rows
+
+
+def read_flight_matrix_ipc(source):
+    """Reads a stream written by write_flight_matrix_ipc: returns (record
+    batches, trailer dict).  Raises if the trailer is missing or its row count
+    disagrees with the rows received."""
+    import json
+    import pyarrow as pa
+    batches, trailer = [], None
+    reader = pa.ipc.open_stream(source)
+    while True:
+        try:
+            b, meta = reader.read_next_batch_with_custom_metadata()
+        except StopIteration:
+            break
+        if meta is not None and b"app_metadata" in meta:
+            trailer = json.loads(meta[b"app_metadata"])
+        else:
+            batches.append(b)
+    if trailer is None or not trailer.get("complete"):
+        raise ValueError("flight matrix stream truncated: no completeness trailer")
+    if trailer["total_rows"] != sum(b.num_rows for b in batches):
+        raise ValueError("flight matrix stream: trailer row count disagrees with the rows received")
+    return batches, trailer
 
 
 def read_matrix_ipc(source, n_src, n_tgt):

@@ -154,3 +154,33 @@ def test_gpu_flight_stream_equals_the_oracle(grid_small):
     assert len(got) == len(exp) == 2
     for a, b in zip(got, exp):
         assert a.equals(b)
+
+
+@pytest.mark.parametrize("sparse", [False, True])
+def test_flight_ipc_completeness_trailer(grid_small, sparse):
+    """flight.rs:3612-3658: a complete stream ends with
+    {"complete":true,"total_rows":N,"contract":...}; a producer failure leaves
+    no trailer and the reader refuses the stream."""
+    from bfgpu.arrow_stream import read_flight_matrix_ipc, write_flight_matrix_ipc
+    ul, dl = lengths(grid_small, 6)
+    eng = Oracle2chEngine(grid_small, ul, dl)
+    src, tgt = seedsets(grid_small, 11, 7), seedsets(grid_small, 8, 8)
+    buf = io.BytesIO()
+    n = write_flight_matrix_ipc(eng, src, tgt, buf, threshold=500, tile_sources=4, sparse=sparse)
+    batches, trailer = read_flight_matrix_ipc(io.BytesIO(buf.getvalue()))
+    assert trailer == {"complete": True, "total_rows": n, "contract": "sparse" if sparse else "dense"}
+    assert sum(b.num_rows for b in batches) == n
+    if not sparse:
+        assert n == 11 * 8
+
+    class Failing(Oracle2chEngine):
+        def table_seeded_bounded_routed_2ch(self, s, t, threshold=INF):
+            if self.calls == 1:
+                raise RuntimeError("device lost")
+            return super().table_seeded_bounded_routed_2ch(s, t, threshold)
+
+    bad = io.BytesIO()
+    with pytest.raises(RuntimeError):
+        write_flight_matrix_ipc(Failing(grid_small, ul, dl), src, tgt, bad, tile_sources=4)
+    with pytest.raises(ValueError):
+        read_flight_matrix_ipc(io.BytesIO(bad.getvalue()))
