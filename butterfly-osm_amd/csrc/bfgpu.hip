@@ -2104,7 +2104,11 @@ int phast_2ch(bfg_ctx *c, const std::vector<uint32_t> &soff, const std::vector<u
     const size_t per_group = (size_t)n * 64 * (dense ? 16 : 8) + (size_t)L.n_top * 64 * 8;
     const size_t held = (t.lab.n + t.top.n) * 8 + (t.o_t.n + t.o_l.n) * 4;
     const uint32_t want = (nl + 63) / 64;
-    const uint32_t gmax = (uint32_t)std::max<size_t>(1, std::min<size_t>(64, (free_b + held) / 2 / per_group));
+    uint32_t gmax = (uint32_t)std::max<size_t>(1, std::min<size_t>(64, (free_b + held) / 2 / per_group));
+    if (const char *e = std::getenv("BFG_T2_GROUPS")) {  // tuning override: groups per batch
+        const long v = std::strtol(e, nullptr, 10);
+        if (v >= 1 && v <= 64) gmax = std::min<uint32_t>(gmax, (uint32_t)v);
+    }
     const uint32_t G = std::min(want, gmax);
     if (t.lab.n < (size_t)G * n * 64) {
         t.lab.release();

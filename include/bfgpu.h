@@ -193,7 +193,9 @@ int bfg_ctx_attach_length(bfg_ctx *ctx, const bfg_graph_desc *graph, const uint3
  * above and lens[i * n_nodes + v] the length along it: the least length over
  * the fastest up-down paths, with per-rank seeds taken as the reference takes
  * them (the first of the fastest, :522-526); both BFG_INF where the time
- * exceeds the limit.  Needs bfg_ctx_attach_length. */
+ * exceeds the limit.  Needs bfg_ctx_attach_length.  Lanes run in batches of
+ * 64-lane groups sized to free HBM; the environment variable BFG_T2_GROUPS
+ * caps the groups per batch (a tuning override, as for the table below). */
 int bfg_phast_seeded_2ch(bfg_ctx *ctx, const uint32_t *seed_offsets, const uint32_t *seed_ranks,
                          const uint32_t *seed_times, const uint32_t *seed_lens, uint32_t n_lanes,
                          const uint32_t *thresholds, uint32_t *times, uint32_t *lens);

@@ -287,6 +287,32 @@ def test_gpu_table_2ch_broom():
 
 
 @pytest.mark.gpu
+def test_gpu_2ch_batches(grid_small, monkeypatch):
+    """One 64-lane group per batch (BFG_T2_GROUPS=1): 150 lanes run as three
+    batches, rows and table cells land at their lanes' offsets."""
+    from bfgpu import BatchedPhastEngine
+    monkeypatch.setenv("BFG_T2_GROUPS", "1")
+    g = grid_small
+    ul, dl = lengths_for(g, 41)
+    say("2ch batches", g.n)
+    o = Oracle(g)
+    o.attach_length(ul, dl)
+    rng = np.random.default_rng(43)
+    lanes = [[(int(r), int(c), int(c))] for r, c in zip(rng.integers(0, g.n, 150), rng.integers(0, 40, 150))]
+    thrs = [INF if i % 2 else 700 for i in range(150)]
+    tgt = [[(int(r), 2, 5)] for r in rng.integers(0, g.n, 33)]
+    with BatchedPhastEngine(g) as e:
+        e.attach_length(ul, dl)
+        t, ln = e.phast_seeded_2ch(lanes, thrs)
+        tt, tl, tc = e.table_seeded_bounded_routed_2ch(lanes, tgt, 900)
+    for i in (0, 63, 64, 127, 128, 149):
+        et, el = o.seeded_2ch_rows(lanes[i], thrs[i])
+        assert np.array_equal(t[i], et) and np.array_equal(ln[i], el), f"lane {i}"
+    et, el, ec = o.table_phast_lopsided_2ch(lanes, tgt, 900)
+    assert np.array_equal(tt, et) and np.array_equal(tl, el) and np.array_equal(tc, ec)
+
+
+@pytest.mark.gpu
 def test_gpu_2ch_needs_lengths(grid_small):
     from bfgpu import BatchedPhastEngine
     from bfgpu.engine import BfgError
