@@ -1986,10 +1986,34 @@ int attach_length(bfg_ctx *c, const bfg_graph_desc *gd, const uint32_t *up_len, 
     for (uint32_t s = 0; s < nseg; ++s)
         for (uint32_t p = L.seg_pos[s]; p < L.seg_pos[s + 1]; ++p) seg_of[p] = s;
     auto pack = [](uint32_t t, uint32_t l) { return (uint64_t)t << 32 | l; };
+    if (!gd->up_offsets || !gd->up_targets || !gd->up_weights || !gd->down_offsets || !gd->down_targets ||
+        !gd->down_weights)
+        return fail(BFG_INVALID_PARAMETER, "null graph array");
+    // the graph must be the context's: every finite arc in range and in its
+    // direction, and as many finite UP arcs per tail and DOWN arcs per head as
+    // the context's layout holds
+    std::vector<uint32_t> cu(n, 0), cd(n, 0);
     uint64_t mu = 0, md = 0;
-    for (uint64_t i = 0; i < gd->up_offsets[n]; ++i) mu += gd->up_weights[i] != INF;
-    for (uint64_t i = 0; i < gd->down_offsets[n]; ++i) md += gd->down_weights[i] != INF;
+    for (uint32_t r = 0; r < n; ++r) {
+        if (gd->up_offsets[r + 1] < gd->up_offsets[r] || gd->down_offsets[r + 1] < gd->down_offsets[r])
+            return fail(BFG_INVALID_PARAMETER, "offsets not monotone");
+        for (uint64_t i = gd->up_offsets[r]; i < gd->up_offsets[r + 1]; ++i) {
+            if (gd->up_weights[i] == INF) continue;
+            if (gd->up_targets[i] >= n || gd->up_targets[i] <= r) return fail(BFG_INVALID_PARAMETER, "bad UP arc");
+            ++cu[L.pos_of_rank[r]];
+            ++mu;
+        }
+        for (uint64_t i = gd->down_offsets[r]; i < gd->down_offsets[r + 1]; ++i) {
+            if (gd->down_weights[i] == INF) continue;
+            if (gd->down_targets[i] >= r) return fail(BFG_INVALID_PARAMETER, "bad DOWN arc");
+            ++cd[L.pos_of_rank[gd->down_targets[i]]];
+            ++md;
+        }
+    }
     if (mu != L.m_up || md != L.m_down) return fail(BFG_INVALID_PARAMETER, "graph differs from the context's");
+    for (uint32_t p = 0; p < n; ++p)
+        if (cu[p] != L.up_off[p + 1] - L.up_off[p] || cd[p] != L.in_off[p + 1] - L.in_off[p])
+            return fail(BFG_INVALID_PARAMETER, "graph differs from the context's");
     if ((mu && !up_len) || (md && !dn_len)) return fail(BFG_INVALID_PARAMETER, "null length weights");
     // DOWN arcs: count per list, then fill with tails in ascending position so
     // each segment's outside arcs come grouped by source

@@ -181,7 +181,9 @@ int bfg_table_seeded(bfg_ctx *ctx, const uint32_t *src_offsets, const uint32_t *
  * server/state.rs:2880-2891) and reads through flats that share the time
  * flats' topology (state.rs:157-165, isochrone_handler.rs:493-499).
  * up_len / down_len hold one weight per arc of `graph`, which must describe
- * the CCH the context was created from; arcs of INF time are dropped with
+ * the CCH the context was created from (checked arc by arc against the
+ * context's layout: BFG_INVALID_PARAMETER otherwise); the two-channel calls
+ * use the times of `graph` as given here.  Arcs of INF time are dropped with
  * their length.  Replaces any lengths attached before. */
 int bfg_ctx_attach_length(bfg_ctx *ctx, const bfg_graph_desc *graph, const uint32_t *up_len,
                           const uint32_t *down_len);

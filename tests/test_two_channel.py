@@ -12,6 +12,7 @@ The oracle's restatement is pinned first, on the CPU:
     the times give lengths equal to the times.
 The GPU path is then checked bit-exact against the oracle (tests marked gpu).
 """
+import ctypes as C
 import heapq
 import json
 import os
@@ -320,6 +321,13 @@ def test_gpu_2ch_needs_lengths(grid_small):
         with pytest.raises(BfgError):
             e.phast_seeded_2ch([[(0, 0, 0)]], [INF])
         ul, dl = lengths_for(grid_small, 2)
+        with pytest.raises(ValueError):  # one length per arc
+            e.attach_length(ul[:-1], dl)
+        other = grid_small.transpose()  # another CCH: per-node arc counts differ
+        d = other.desc()
+        rc = e._lib.bfg_ctx_attach_length(e._h, C.byref(d), other.up_w.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                          other.dn_w.ctypes.data_as(C.POINTER(C.c_uint32)))
+        assert rc == 1, "a different graph must be refused"
         e.attach_length(ul, dl)
         t, ln = e.phast_seeded_2ch([[(0, 0, 0)], []], [INF, INF])
         assert t[0][0] == 0 and ln[0][0] == 0
