@@ -1641,7 +1641,7 @@ int dense_rows(bfg_ctx *c, const Req &req, uint32_t *dist_host, uint64_t *bits_h
 // semantics of run_phast_bounded_fast_seeded_2ch (isochrone_handler.rs:500-625).
 // A label is (time << 32 | length): lexicographic order is integer order, so
 // the reference's "strictly faster, or equally fast and strictly shorter"
-// relaxation (:546-565, :586-602) is a min, and a node's final label is the
+// relaxation (:543-567, :584-602) is a min, and a node's final label is the
 // least (time, length) over the up-down paths from the lane's seeds.  Lanes
 // are grouped by 64; labels live in [group][position][64 lanes].  A lane
 // with limit T relaxes only from labels of time <= T; every label of time
