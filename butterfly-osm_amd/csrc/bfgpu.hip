@@ -1037,6 +1037,8 @@ struct TwoChannel {
     DevBuf<uint64_t> top;  // the upward phase's top-region labels, [lane][top position]
     DevBuf<uint32_t> s_off, s_pos, lim, o_t, o_l;
     uint64_t bytes = 0;
+    // per segment level, the most outside arcs one of its segments has (host)
+    std::vector<uint32_t> lvl_xmax;
 };
 
 }  // namespace
@@ -1649,6 +1651,18 @@ int dense_rows(bfg_ctx *c, const Req &req, uint32_t *dist_host, uint64_t *bits_h
 // ---------------------------------------------------------------------------
 constexpr uint64_t INF2 = ~0ull;
 constexpr int T2_SEG_WAVES = 16;  // wavefronts sharing one segment tile
+// Levels whose segments have at most this many outside arcs run 8-wavefront
+// tiles: their time goes to wavefront 0's chain walk, and three such tiles
+// fit a CU (two of 16 wavefronts).
+constexpr uint32_t T2_SMALL_ARCS = 4096;
+
+uint32_t t2_small_arcs() {
+    if (const char *e = std::getenv("BFG_T2_SMALL_ARCS")) {  // tuning override
+        const long v = std::strtol(e, nullptr, 10);
+        if (v >= 0 && v < (1L << 31)) return (uint32_t)v;
+    }
+    return T2_SMALL_ARCS;
+}
 
 __device__ __forceinline__ uint64_t add2(uint64_t a, uint64_t w) {
     const uint32_t t = sat_add((uint32_t)(a >> 32), (uint32_t)(w >> 32));
@@ -1784,7 +1798,8 @@ __global__ __launch_bounds__(NW * 64) void t2_seg_kernel(uint32_t s0, uint32_t n
     constexpr uint32_t MAXI = SEG * (SEG - 1) / 2;  // arcs inside a segment (one per ordered pair)
     __shared__ uint64_t acc[SEG][64];
     __shared__ uint64_t iw[MAXI];
-    __shared__ uint32_t ik[MAXI], io[SEG + 1];
+    __shared__ uint32_t io[SEG + 1];
+    __shared__ uint8_t ik[MAXI];  // tail offsets (< 64): 51 KB of LDS in all, three tiles per CU
     const uint32_t s = s0 + blockIdx.x, g = blockIdx.y, lane = threadIdx.x & 63;
     const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t gl = g * 64 + lane, thr = gl < nl ? lim[gl] : 0;
@@ -1794,7 +1809,7 @@ __global__ __launch_bounds__(NW * 64) void t2_seg_kernel(uint32_t s0, uint32_t n
     // the chain's arcs into LDS, so wavefront 0 walks it without memory waits
     const uint32_t i0 = i_off[p0], ni = i_off[p0 + len] - i0;
     for (uint32_t k = threadIdx.x; k < ni; k += NW * 64) {
-        ik[k] = i_k[i0 + k];
+        ik[k] = (uint8_t)i_k[i0 + k];
         iw[k] = i_w[i0 + k];
     }
     for (uint32_t k = threadIdx.x; k <= len; k += NW * 64) io[k] = i_off[p0 + k] - i0;
@@ -2098,6 +2113,11 @@ int attach_length(bfg_ctx *c, const bfg_graph_desc *gd, const uint32_t *up_len, 
     t->bytes = 4 * (x_off.size() + x_sj.size() + i_off.size() + i_k.size() + b_off.size() + b_src.size() +
                     u_off.size() + u_tgt.size() + n) +
                8 * (x_w.size() + i_w.size() + b_w.size() + u_w.size());
+    for (size_t lv = 0; lv + 1 < L.seglvl_seg.size(); ++lv) {
+        uint32_t m = 0;
+        for (uint32_t sg = L.seglvl_seg[lv]; sg < L.seglvl_seg[lv + 1]; ++sg) m = std::max(m, x_off[sg + 1] - x_off[sg]);
+        t->lvl_xmax.push_back(m);
+    }
     c->t2 = std::move(t);
     return BFG_SUCCESS;
 }
@@ -2170,9 +2190,15 @@ int phast_2ch(bfg_ctx *c, const std::vector<uint32_t> &soff, const std::vector<u
                                        t.u_off.p, t.u_tgt.p, t.u_w.p, t.lab.p, t.top.p);
         if (L.n_top)
             t2_top_kernel<<<dim3((L.n_top + 63) / 64, bg), NT, 0, s>>>(n, L.n_top, t.top.p, t.lab.p);
+        const uint32_t small = t2_small_arcs();
         for (size_t lv = 0; lv + 1 < L.seglvl_seg.size(); ++lv) {
             const uint32_t s0 = L.seglvl_seg[lv], s1 = L.seglvl_seg[lv + 1];
-            if (s1 > s0)
+            if (s1 == s0) continue;
+            if (t.lvl_xmax[lv] <= small)
+                t2_seg_kernel<8><<<dim3(s1 - s0, bg), 8 * 64, 0, s>>>(s0, n, bl, c->seg_pos.p, t.x_off.p, t.x_sj.p,
+                                                                     t.x_w.p, t.i_off.p, t.i_k.p, t.i_w.p, t.lim.p,
+                                                                     t.lab.p);
+            else
                 t2_seg_kernel<T2_SEG_WAVES><<<dim3(s1 - s0, bg), T2_SEG_WAVES * 64, 0, s>>>(
                     s0, n, bl, c->seg_pos.p, t.x_off.p, t.x_sj.p, t.x_w.p, t.i_off.p, t.i_k.p, t.i_w.p, t.lim.p,
                     t.lab.p);
