@@ -194,6 +194,27 @@ def test_table_2ch_broom():
         assert np.array_equal(lm[fin], 2 * tm[fin]) and np.all(lm[~fin] == INF)
 
 
+def test_table_2ch_first_fastest_target_seed_gives_the_length():
+    """bucket_ch.rs:6462-6473: over a target's seeds in order, a strictly
+    smaller time replaces the best, so among seeds of equal time the first
+    one's length part applies (not the smallest length).  On the broom with
+    lengths = 2 x times the field at leaf j is (t, 2t), so the cell length is
+    2t - (first seed's length part), saturating."""
+    g, wu, wd = broom_graph(12)
+    o = Oracle(g)
+    o.attach_length(2 * g.up_w, 2 * g.dn_w)
+    src = [[(0, 0, 0)], [(5, 0, 0)]]
+    tgt = [[(7, 0, 5), (7, 0, 1)], [(7, 0, 1), (7, 0, 5)], [(9, 3, 1 << 31), (9, 3, 0)]]
+    tm, lm, conf = o.table_phast_lopsided_2ch(src, tgt, INF)
+    for i, s in enumerate((0, 5)):
+        t7 = int(wu[s] + wd[7])
+        t9 = int(wu[s] + wd[9])
+        assert tm[i * 3 + 0] == t7 and lm[i * 3 + 0] == 2 * t7 - 5
+        assert tm[i * 3 + 1] == t7 and lm[i * 3 + 1] == 2 * t7 - 1
+        assert tm[i * 3 + 2] == t9 - 3 and lm[i * 3 + 2] == 0  # 2t - 2^31 saturates at 0
+    assert not conf.any()
+
+
 # ---------------------------------------------------------------- GPU parity
 
 @pytest.mark.gpu
