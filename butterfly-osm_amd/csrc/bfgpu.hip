@@ -2126,6 +2126,14 @@ int attach_length(bfg_ctx *c, const bfg_graph_desc *gd, const uint32_t *up_len, 
 // seeds per lane are distinct positions with their (time, length) labels.
 // After each batch's sweep, emit(first lane, lanes, groups) reads the labels
 // (t.lab, limits in t.lim) on the context stream.
+// Waits for the stream when it goes out of scope: declared after host vectors
+// that asynchronous uploads read, it keeps them alive until the copies are
+// done, also when an error returns early.
+struct StreamDrain {
+    hipStream_t s;
+    ~StreamDrain() { (void)hipStreamSynchronize(s); }
+};
+
 // The bottom nodes a sweep covers: all of them, or per bottom level l the
 // positions d_pos[off[l] .. off[l+1]) (off on the host, d_pos on the device).
 struct BottomSet {
@@ -2176,6 +2184,7 @@ int phast_2ch(bfg_ctx *c, const std::vector<uint32_t> &soff, const std::vector<u
         const uint32_t k0 = soff[l0], k1 = soff[l0 + bl];
         std::vector<uint32_t> pos(spos.begin() + k0, spos.begin() + k1), lim(limits + l0, limits + l0 + bl);
         std::vector<uint64_t> lb(slab.begin() + k0, slab.begin() + k1);
+        const StreamDrain drain{s};
         HIP_TRY(t.s_off.upload(off, s));
         HIP_TRY(t.s_pos.upload(pos, s));
         HIP_TRY(t.s_lab.upload(lb, s));
@@ -2637,6 +2646,7 @@ int bfg_table_seeded_2ch(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t
     DevBuf<uint32_t> d_toff, d_tpos, d_trank, d_tpt, d_tpl, d_soff, d_srank, d_t, d_l, d_bpos;
     DevBuf<uint8_t> d_conf;
     hipStream_t st = c->stream;
+    const StreamDrain drain{st};  // before the device buffers and host tables above go
     HIP_TRY(d_toff.upload(h_toff, st));
     HIP_TRY(d_tpos.upload(tpos, st));
     HIP_TRY(d_trank.upload(h_trank, st));
