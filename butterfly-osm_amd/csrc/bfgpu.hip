@@ -996,6 +996,14 @@ struct DevBuf {
     }
 };
 
+// Waits for the stream when it goes out of scope: declared after host vectors
+// that asynchronous uploads read, it keeps them alive until the copies are
+// done, also when an error returns early.
+struct StreamDrain {
+    hipStream_t s;
+    ~StreamDrain() { (void)hipStreamSynchronize(s); }
+};
+
 constexpr int SEG = 64;     // nodes of a segment (one register tile)
 constexpr int CHUNK = 256;  // nodes of a bottom chunk (LDS = CHUNK * KL * 4 bytes)
 // Subtree size at or below which a node is in the bottom region.  Small: the
@@ -1234,6 +1242,7 @@ int plan_down(bfg_ctx *c, uint32_t wgs) {
     for (size_t k = 0; k < us.size(); ++k) covered += u1[k] - u0[k];
     if (covered != L.xr_off.back()) return fail(BFG_INTERNAL_ERROR, "DOWN plan does not cover every row");
     hipStream_t s = c->stream;
+    const StreamDrain drain{s};
     HIP_TRY(c->du_seg.upload(us, s));
     HIP_TRY(c->du_r0.upload(u0, s));
     HIP_TRY(c->du_r1.upload(u1, s));
@@ -1387,6 +1396,7 @@ int run_batch(bfg_ctx *c, uint32_t nl, const std::vector<uint32_t> &soff, const 
         spos[j] = c->L.pos_of_rank[srank[j]];
     }
     hipStream_t s = c->stream;
+    const StreamDrain drain{s};  // a no-op once launch_phases has synchronised
     HIP_TRY(hipMemcpyAsync(c->seed_off.p, soff.data(), (nl + 1) * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->lane_limit.p, limits.data(), nl * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->row_map.p, rows.data(), nl * 4, hipMemcpyHostToDevice, s));
@@ -1606,6 +1616,7 @@ int dense_rows(bfg_ctx *c, const Req &req, uint32_t *dist_host, uint64_t *bits_h
     DevBuf<uint32_t> d_rows, d_all_pos;
     DevBuf<unsigned long long> d_bits;
     std::vector<uint32_t> all_pos(c->L.pos_of_rank);
+    const StreamDrain drain{c->stream};
     HIP_TRY(d_all_pos.upload(all_pos, c->stream));
     // reach bits of every lane land in request order on the device (rows
     // through row_map) and come back in one copy
@@ -2096,6 +2107,7 @@ int attach_length(bfg_ctx *c, const bfg_graph_desc *gd, const uint32_t *up_len, 
     }
     std::unique_ptr<TwoChannel> t(new TwoChannel());
     hipStream_t s = c->stream;
+    const StreamDrain drain{s};
     HIP_TRY(t->x_off.upload(x_off, s));
     HIP_TRY(t->x_sj.upload(x_sj, s));
     HIP_TRY(t->x_w.upload(x_w, s));
@@ -2126,14 +2138,6 @@ int attach_length(bfg_ctx *c, const bfg_graph_desc *gd, const uint32_t *up_len, 
 // seeds per lane are distinct positions with their (time, length) labels.
 // After each batch's sweep, emit(first lane, lanes, groups) reads the labels
 // (t.lab, limits in t.lim) on the context stream.
-// Waits for the stream when it goes out of scope: declared after host vectors
-// that asynchronous uploads read, it keeps them alive until the copies are
-// done, also when an error returns early.
-struct StreamDrain {
-    hipStream_t s;
-    ~StreamDrain() { (void)hipStreamSynchronize(s); }
-};
-
 // The bottom nodes a sweep covers: all of them, or per bottom level l the
 // positions d_pos[off[l] .. off[l+1]) (off on the host, d_pos on the device).
 struct BottomSet {
@@ -2733,6 +2737,7 @@ int bfg_table_seeded(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t *sr
     hipStream_t st = c->stream;
     const std::vector<uint32_t> h_toff(tgt_offsets, tgt_offsets + T + 1), h_trank(tgt_ranks, tgt_ranks + nt),
         h_tpart(tgt_parts, tgt_parts + nt);
+    const StreamDrain drain{st};
     HIP_TRY(d_toff.upload(h_toff, st));
     HIP_TRY(d_tlab.upload(tlab, st));
     HIP_TRY(d_trank.upload(h_trank, st));
