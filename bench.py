@@ -7,10 +7,14 @@ edge-based CCH of ~6.6M nodes), sources/targets 80 % uniform + 20 % in five
 Gaussian hot-spots (sigma = 2 % of the extent) snapped to the nearest node.
 A step is one full matrix with queries and results resident in HBM.
 
-Prints one JSON line (the contract of this repository's driver) with
-`roofline` (the sweep kernel against the 8 TB/s HBM peak) and `cpu_baseline`
-(the oracle's restatement of the reference's compute_matrix_flat, timed on a
-bounded sample on this host).
+Prints one JSON line with `roofline` (the sweep kernel against the 8 TB/s HBM
+peak) and `cpu_baseline` (the oracle's restatement of the reference's
+compute_matrix_flat, timed on a bounded sample on this host).
+
+Other modes print their own JSON line instead: --serving (continuous batching
+against plain per-request calls), --isochrones L (block-gated isochrones with
+uneven limits), --table-2ch S (a two-channel seeded table: times and lengths
+along them), --autotune (K and the DOWN unit size).
 """
 import argparse
 import json
