@@ -20,7 +20,7 @@ def pytest_configure(config):
 
 
 def _ensure_built():
-    """Build only what is missing (the driver builds before running tests)."""
+    """Build only what is missing (a fresh checkout builds everything here)."""
     if not os.path.exists(os.path.join(REPO, "oracle", "liborc.so")):
         from oracle import build as ob
         ob.build()
