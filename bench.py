@@ -30,40 +30,11 @@ for _p in (REPO, PKG):
     if _p not in sys.path:
         sys.path.insert(0, _p)
 
+from bfgpu.workload import WORKLOADS, isochrone_lanes, make_queries, serving_requests  # noqa: E402
+
 METRIC = "matrix cells/s 10k×10k u32 bit-exact, % HBM roofline; serving p50/p99 vs plain"
 HBM_PEAK_GBPS = 8000.0
 INF_U32 = 0xFFFFFFFF
-
-WORKLOADS = {
-    # name: (grid rows, chord fraction, S, T)
-    "matrix10k": (1414, 0.02, 10000, 10000),
-    "matrix1k": (1414, 0.02, 1000, 1000),
-    "config1": (224, 0.0, 100, 100),
-    "mid": (448, 0.02, 2000, 2000),
-}
-
-
-def log(*a):
-    print("[bench]", *a, file=sys.stderr, flush=True)
-
-
-def make_queries(g, count, seed, hot_frac=0.2, n_hot=5, sigma_frac=0.02):
-    """80 % uniform ranks + 20 % around 5 Gaussian hot-spots, snapped to nodes."""
-    rng = np.random.default_rng(seed)
-    n_hot_pts = int(round(hot_frac * count))
-    uni = rng.integers(0, g.n, count - n_hot_pts)
-    if n_hot_pts == 0 or g.x is None:
-        return rng.permutation(np.concatenate([uni, rng.integers(0, g.n, n_hot_pts)])).astype(np.uint32)
-    from scipy.spatial import cKDTree
-    xy = np.c_[g.x, g.y].astype(np.float64)
-    lo, hi = xy.min(0), xy.max(0)
-    ext = hi - lo
-    centers = lo + rng.random((n_hot, 2)) * ext
-    which = rng.integers(0, n_hot, n_hot_pts)
-    pts = centers[which] + rng.normal(0.0, 1.0, (n_hot_pts, 2)) * (sigma_frac * ext)
-    _, idx = cKDTree(xy).query(pts)
-    return rng.permutation(np.concatenate([uni, idx])).astype(np.uint32)
-
 
 def copy_probe(nbytes):
     """One device-to-device copy of nbytes by a plain copy kernel (torch's),
@@ -165,23 +136,6 @@ def cpu_baseline_mt(g, src, tgt, budget_s):
             "sample": f"{k} sources x {len(tgt)} targets, one K=8 batch per thread"}
 
 
-def serving_requests(g, count, seed):
-    """The mixed stream: 70 % matrices with S and T each log-uniform in
-    [10, 2000] (uniform random nodes), 30 % isochrones from a uniform random
-    node with a limit of 5, 10, 15, 30, 45 or 60 minutes (decisecond weights)."""
-    from bfgpu.serving import IsochroneRequest, MatrixRequest
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(count):
-        if rng.random() < 0.7:
-            S, T = (int(round(np.exp(rng.uniform(np.log(10), np.log(2000))))) for _ in range(2))
-            out.append(MatrixRequest(rng.integers(0, g.n, S).astype(np.uint32),
-                                     rng.integers(0, g.n, T).astype(np.uint32)))
-        else:
-            out.append(IsochroneRequest(int(rng.integers(0, g.n)), int(rng.choice([5, 10, 15, 30, 45, 60])) * 600))
-    return out
-
-
 def serve_stream(loop_cls, eng, reqs, arrivals):
     """Submits reqs at their arrival offsets (seconds) and returns per-request
     latencies (s), the span from the first arrival to the last completion (s),
@@ -252,9 +206,7 @@ def isochrone_bench(eng, g, args, rows, chords):
     reach bits of every node for every lane (times stay on the device except
     for the checked sample)."""
     L = args.isochrones
-    rng = np.random.default_rng(args.seed + 30)
-    org = rng.integers(0, g.n, L).astype(np.uint32)
-    lim = (rng.choice([5, 10, 15, 30, 45, 60], L) * 600).astype(np.uint32)
+    org, lim = isochrone_lanes(g, L, args.seed + 30)
     eng.isochrones(org[:64], lim[:64], want_times=False)  # warm-up
     times = []
     for i in range(args.steps):

@@ -10,7 +10,7 @@ stream_matrix_tiles computes the matrix in source chunks on the GPU and yields
 each chunk's tiles while the next chunk runs: a worker thread keeps the device
 busy with chunk i + 1 while the caller packs and sends chunk i.  Values are
 the engine's u32 durations times `scale` (saturating, INF kept), so a caller
-holding decisecond weights passes scale=100 for milliseconds.
+holding second weights passes scale=1000 for milliseconds.
 
 stream_flight_matrix emits the Flight matrix rows (source_idx, target_idx,
 duration_ms, distance_m) from two-channel seeded tables, one record batch per
@@ -52,15 +52,31 @@ def stream_matrix_tiles(engine, sources, targets, chunk_sources=2048, tile_sourc
     tgt = np.ascontiguousarray(targets, dtype=np.uint32)
     chunks = [(c0, src[c0:c0 + chunk_sources]) for c0 in range(0, src.size, chunk_sources)]
     done = queue.Queue(maxsize=2)  # bounded: the GPU runs at most one chunk ahead of the consumer
+    stop = threading.Event()       # set when the consumer goes away (close, break or an error)
+
+    def put(item):
+        # a bounded put that gives up once the consumer has stopped, so the
+        # producer never blocks on a queue nobody drains
+        while not stop.is_set():
+            try:
+                done.put(item, timeout=0.05)
+                return True
+            except queue.Full:
+                continue
+        return False
 
     def produce():
         try:
             for c0, s in chunks:
+                if stop.is_set():
+                    return
                 m, _ = engine.compute_matrix_block_gated(s, tgt, threshold)
-                done.put((c0, m.reshape(s.size, tgt.size)))
+                if not put((c0, m.reshape(s.size, tgt.size))):
+                    return
         except Exception as e:  # handed to the consumer
-            done.put(e)
-        done.put(None)
+            put(e)
+            return
+        put(None)
 
     worker = threading.Thread(target=produce, daemon=True)
     worker.start()
@@ -77,6 +93,12 @@ def stream_matrix_tiles(engine, sources, targets, chunk_sources=2048, tile_sourc
                     block = _scaled(np.ascontiguousarray(m[r0:r0 + tile_sources, t0:t0 + tile_targets]), scale)
                     yield c0 + r0, t0, block.shape[0], block.shape[1], block
     finally:
+        stop.set()
+        while True:  # free a producer blocked on a full queue
+            try:
+                done.get_nowait()
+            except queue.Empty:
+                break
         worker.join()
 
 

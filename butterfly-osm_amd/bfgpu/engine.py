@@ -190,6 +190,8 @@ class BatchedPhastEngine:
                 costs.append(c)
             off[i + 1] = len(ranks)
         ranks, costs, thr = _u32(ranks), _u32(costs), _u32(thresholds)
+        if thr.size != L:
+            raise ValueError("one threshold per lane")
         out = np.empty((L, self.n_nodes), dtype=np.uint32)
         _check(self._lib.bfg_phast_seeded(self._h, _native.ptr(off), _native.ptr(ranks), _native.ptr(costs), L,
                                           _native.ptr(thr), _native.ptr(out)))

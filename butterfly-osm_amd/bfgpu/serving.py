@@ -131,17 +131,24 @@ class _Loop:
                 if not self._pending:
                     return
                 batch = self._take()
+            # a request its client cancelled before the round starts leaves the
+            # round; the rest can no longer be cancelled
+            batch = [(r, f) for r, f in batch if f.set_running_or_notify_cancel()]
+            if not batch:
+                continue
             try:
                 res = run_round(self.engine, [r for r, _ in batch])
-            except Exception as e:  # a failed round fails its requests, the loop goes on
+            except BaseException as e:  # a failed round fails its requests, the loop goes on
                 for _, f in batch:
-                    f.set_exception(e)
+                    if not f.done():
+                        f.set_exception(e)
                 continue
             self.rounds += 1
             done = time.perf_counter()
             for (_, f), r in zip(batch, res):
                 f.completed = done
-                f.set_result(r)
+                if not f.done():
+                    f.set_result(r)
 
 
 class ServingLoop(_Loop):

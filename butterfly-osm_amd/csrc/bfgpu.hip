@@ -1101,6 +1101,21 @@ struct bfg_ctx {
     bfg_timings last{};
     uint64_t graph_bytes = 0;
     std::unique_ptr<TwoChannel> t2;  // set by bfg_ctx_attach_length
+
+    bfg_ctx() = default;
+    bfg_ctx(const bfg_ctx &) = delete;
+    bfg_ctx &operator=(const bfg_ctx &) = delete;
+    // Drains the stream, then releases the events and the stream, also for a
+    // context that bfg_ctx_create abandons half-built; the device buffers
+    // free themselves afterwards (members are destroyed after this body).
+    ~bfg_ctx() {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto &e : dom_ev) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
 };
 
 namespace {
@@ -2376,16 +2391,7 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     return BFG_SUCCESS;
 }
 
-void bfg_ctx_destroy(bfg_ctx *c) {
-    if (!c) return;
-    (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto &e : c->ev)
-        if (e) (void)hipEventDestroy(e);
-    for (auto &e : c->dom_ev) (void)hipEventDestroy(e);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    delete c;
-}
+void bfg_ctx_destroy(bfg_ctx *c) { delete c; }
 
 int bfg_validate_graph(const bfg_graph_desc *gd, bfg_ctx_info *info) {
     if (!gd || !info) return fail(BFG_INVALID_PARAMETER, "null argument");

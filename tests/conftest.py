@@ -82,3 +82,32 @@ def grid_small():
 def grid_config1():
     """224x224 jittered grid, the reference's plain CPU configuration."""
     return grid_graph(224)
+
+
+def progress(config, msg):
+    """A progress line that bypasses output capture, so a long GPU step shows
+    it is alive while a fixture builds or a test runs."""
+    cm = config.pluginmanager.getplugin("capturemanager")
+    if cm is None:
+        print(f"[progress] {msg}", flush=True)
+        return
+    with cm.global_and_fixture_disabled():
+        print(f"[progress] {msg}", flush=True)
+
+
+@pytest.fixture(scope="session")
+def country_graph(request):
+    """The 1414 x 1414 jittered grid with 2 % motorway chords (about 2M NBG
+    nodes, an edge-based CCH of about 6.7M nodes): the graph of the matrix,
+    isochrone and serving configurations, built once per session (about
+    100 s on 16 cores)."""
+    import time
+    from bfgpu.workload import GRAPH_SEED
+    t0 = time.perf_counter()
+    progress(request.config, "building the 1414x1414 country graph ...")
+    from bfgpu import build_grid
+    from bfgpu.workload import grid_spec
+    g = build_grid(grid_spec(1414, 0.02, GRAPH_SEED), keep_ebg=False)  # 7 GB: not cached on disk
+    progress(request.config, f"country graph n={g.n} m_up={g.m_up} m_down={g.m_down} "
+                             f"in {time.perf_counter() - t0:.0f} s")
+    return g

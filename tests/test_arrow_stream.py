@@ -184,3 +184,47 @@ def test_flight_ipc_completeness_trailer(grid_small, sparse):
         write_flight_matrix_ipc(Failing(grid_small, ul, dl), src, tgt, bad, tile_sources=4)
     with pytest.raises(ValueError):
         read_flight_matrix_ipc(io.BytesIO(bad.getvalue()))
+
+
+def test_consumer_closing_early_does_not_hang(grid_small):
+    """Closing the tile generator after its first tile (a consumer that stops
+    reading) returns at once and leaves no producer blocked on the queue."""
+    import threading
+    import time
+    eng = OracleEngine(grid_small)
+    rng = np.random.default_rng(11)
+    s = rng.integers(0, grid_small.n, 64).astype(np.uint32)
+    t = rng.integers(0, grid_small.n, 16).astype(np.uint32)
+    before = threading.active_count()
+    gen = stream_matrix_tiles(eng, s, t, chunk_sources=4, tile_sources=4, tile_targets=16)
+    next(gen)
+    t0 = time.perf_counter()
+    gen.close()
+    assert time.perf_counter() - t0 < 5.0
+    assert threading.active_count() <= before
+    assert eng.calls < 16  # the producer stopped instead of computing every chunk
+
+
+def test_writer_error_stops_the_producer(grid_small):
+    """A sink that fails mid-stream surfaces its error and ends the producer."""
+    eng = OracleEngine(grid_small)
+    rng = np.random.default_rng(12)
+    s = rng.integers(0, grid_small.n, 64).astype(np.uint32)
+    t = rng.integers(0, grid_small.n, 16).astype(np.uint32)
+
+    class Broken(io.RawIOBase):
+        def __init__(self):
+            self.n = 0
+
+        def writable(self):
+            return True
+
+        def write(self, b):
+            self.n += 1
+            if self.n > 3:
+                raise OSError("sink closed")
+            return len(b)
+
+    with pytest.raises(OSError):
+        write_matrix_ipc(eng, s, t, Broken(), chunk_sources=4, tile_sources=4, tile_targets=16, tiles_per_batch=1)
+    assert eng.calls < 16

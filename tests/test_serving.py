@@ -126,3 +126,65 @@ def test_serving_on_gpu_equals_requests_alone(grid_config1):
             if r.threshold != INF:
                 m = np.where(m > r.threshold, INF, m)
             assert np.array_equal(res, m)
+
+
+class FailingEngine(OracleEngine):
+    """An engine whose calls fail while `fail` is set, the way a device call
+    fails (BFG_OUT_OF_MEMORY forced from the host)."""
+
+    def __init__(self, g):
+        super().__init__(g)
+        self.fail = False
+
+    def _maybe_fail(self):
+        if self.fail:
+            from bfgpu.engine import BfgError
+            raise BfgError(2, "forced out of memory")
+
+    def compute_matrix_block_gated(self, sources, targets, threshold):
+        self._maybe_fail()
+        return super().compute_matrix_block_gated(sources, targets, threshold)
+
+    def isochrones(self, origins, limits, want_times=True):
+        self._maybe_fail()
+        return super().isochrones(origins, limits, want_times)
+
+
+@pytest.mark.parametrize("loop_cls", [ServingLoop, PlainLoop])
+def test_cancelled_request_leaves_the_loop_running(grid_small, loop_cls):
+    """A request cancelled by its client before its round never reaches the
+    engine, and the loop keeps serving the requests after it."""
+    eng = OracleEngine(grid_small)
+    reqs = mixed_stream(grid_small, np.random.default_rng(5), 6)
+    want = alone(OracleEngine(grid_small), reqs)
+    with loop_cls(eng) as loop:
+        with loop._cv:  # hold the worker off while the requests queue up
+            futs = [loop.submit(r) for r in reqs]
+            assert futs[2].cancel()
+        got = [f.result(timeout=60) if k != 2 else None for k, f in enumerate(futs)]
+        assert futs[2].cancelled()
+        late = loop.submit(reqs[0]).result(timeout=60)
+    for k, (a, b) in enumerate(zip(got, want)):
+        if k != 2:
+            assert np.array_equal(a, b)
+    assert np.array_equal(late, want[0])
+
+
+@pytest.mark.parametrize("loop_cls", [ServingLoop, PlainLoop])
+def test_failed_round_fails_only_its_requests(grid_small, loop_cls):
+    """A round whose device call fails (out of memory forced from the host)
+    fails the requests of that round with the engine's error; requests
+    submitted afterwards are served normally."""
+    from bfgpu.engine import BfgError
+    eng = FailingEngine(grid_small)
+    reqs = mixed_stream(grid_small, np.random.default_rng(6), 4)
+    want = alone(OracleEngine(grid_small), reqs)
+    with loop_cls(eng) as loop:
+        eng.fail = True
+        bad = loop.submit(reqs[0])
+        with pytest.raises(BfgError):
+            bad.result(timeout=60)
+        eng.fail = False
+        got = [f.result(timeout=60) for f in [loop.submit(r) for r in reqs]]
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)

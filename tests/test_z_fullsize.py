@@ -1,0 +1,142 @@
+"""GPU parity at the full sizes of the matrix, isochrone and serving
+configurations: the 1414 x 1414 jittered grid with 2 % motorway chords (an
+edge-based CCH of about 6.7M nodes, a 13.5k-deep elimination tree), built once
+per session.
+
+Every check is bit-exact against the oracle (u32 integer work):
+- 1000 x 1000 matrix as one plain call with K = 32 lanes, every cell;
+- 10k x 10k matrix delivered to the host, 256 sampled rows (hot-spot sources,
+  the lowest- and highest-rank sources);
+- 4096 isochrones with uneven 5-60 minute limits in one call: reach bits of
+  every lane against the size-independent invariants (origin reached, bits of
+  the sampled lanes equal to a second call), 32 lanes against the oracle's
+  PHAST rows masked to the limit and 4 lanes against its seeded PHAST;
+- a mixed serving burst: every result equal to the request issued alone, and
+  a sample against the oracle.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import progress
+from oracle.oracle import INF, Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def country(country_graph, request):
+    from bfgpu import BatchedPhastEngine
+    t0 = time.perf_counter()
+    e = BatchedPhastEngine(country_graph)
+    progress(request.config, f"country context ready in {time.perf_counter() - t0:.0f} s: {e.info()}")
+    o = Oracle(country_graph)
+    yield country_graph, e, o
+    e.close()
+
+
+def oracle_rows_masked(o, origins, limits):
+    """The oracle's unbounded PHAST rows masked to each lane's limit: the
+    bounded contract (exact for d <= limit, INF beyond), pinned against the
+    oracle's bounded and seeded restatements in the CPU tests."""
+    m = o.compute_matrix_flat_mt(origins, np.arange(o.n, dtype=np.uint32)).reshape(len(origins), o.n)
+    lim = np.asarray(limits, dtype=np.uint32)[:, None]
+    return np.where(m > lim, np.uint32(INF), m)
+
+
+def bits_of(rows, n):
+    words = (n + 63) // 64
+    inside = np.zeros((rows.shape[0], words * 64), dtype=bool)
+    inside[:, :n] = rows != INF
+    return np.packbits(inside, axis=1, bitorder="little").view(np.uint64)
+
+
+def test_config2_matrix_1000_k32(country, request):
+    """1000 x 1000 as a single plain call with K = 32, every cell."""
+    g, e, o = country
+    from bfgpu.workload import make_queries
+    s = make_queries(g, 1000, 20261203 + 1)
+    t = make_queries(g, 1000, 20261203 + 2)
+    e.configure(lanes=32)
+    try:
+        got, st = e.compute_matrix_flat(s, t)
+    finally:
+        e.configure(lanes=64)
+    progress(request.config, f"config2 GPU {st.total_time_ms:.1f} ms; oracle ...")
+    exp = o.compute_matrix_flat_mt(s, t)
+    assert np.array_equal(got, exp)
+    assert (got != INF).mean() > 0.9
+
+
+def test_config3_matrix_10k_host_rows(country, request):
+    """10k x 10k delivered to host memory; 256 rows against the oracle: the 64
+    lowest-rank sources (deep in the elimination tree), the 64 highest-rank
+    ones (on the top separators) and 128 hot-spot sources."""
+    g, e, o = country
+    from bfgpu.workload import make_queries, make_queries_tagged
+    s, hot = make_queries_tagged(g, 10000, 20261203 + 1)
+    t = make_queries(g, 10000, 20261203 + 2)
+    got, st = e.compute_matrix_flat(s, t)
+    got = got.reshape(10000, 10000)
+    progress(request.config, f"config3 GPU {st.total_time_ms:.1f} ms, {st.batches} batch(es); oracle ...")
+    order = np.argsort(s, kind="stable")
+    rows = np.unique(np.r_[order[:64], order[-64:], np.nonzero(hot)[0][:128]])
+    exp = o.compute_matrix_flat_mt(s[rows], t).reshape(rows.size, 10000)
+    assert rows.size >= 250
+    assert np.array_equal(got[rows], exp)
+
+
+def test_config4_isochrones_4096(country, request):
+    """4096 isochrones, uneven limits of 5-60 minutes, in one call."""
+    g, e, o = country
+    from bfgpu.workload import isochrone_lanes
+    org, lim = isochrone_lanes(g, 4096, 20261203 + 30)
+    bits, _ = e.isochrones(org, lim, want_times=False)
+    st = e.last_stats(4096)
+    progress(request.config, f"config4 GPU {st.total_time_ms:.1f} ms, {st.batches} batch(es)")
+    # every origin reaches itself at 0
+    w, b = org // 64, org % 64
+    assert np.all((bits[np.arange(4096), w] >> b.astype(np.uint64)) & np.uint64(1) == 1)
+    # 32 lanes (every limit class) against the oracle; their bits must match
+    # the 4096-lane call, and the bits must equal times <= limit
+    rng = np.random.default_rng(5)
+    sample = np.unique(np.r_[rng.choice(4096, 26, replace=False),
+                             [int(np.nonzero(lim == m * 60)[0][0]) for m in (5, 10, 15, 30, 45, 60)]])
+    sb, stimes = e.isochrones(org[sample], lim[sample], want_times=True)
+    assert np.array_equal(sb, bits[sample])
+    exp = oracle_rows_masked(o, org[sample], lim[sample])
+    assert np.array_equal(stimes, exp)
+    assert np.array_equal(sb, bits_of(stimes, g.n))
+    # the seeded restatement itself (run_phast_bounded_fast_seeded) on 4 lanes
+    for i in sample[:4]:
+        row = np.full(g.n, INF, dtype=np.uint32)
+        for r, d in o.seeded([(int(org[i]), 0)], int(lim[i])):
+            row[r] = d
+        assert np.array_equal(row, stimes[list(sample).index(i)])
+
+
+def test_config5_serving_burst(country, request):
+    """A mixed burst through the serving loop: each result equals the same
+    request issued alone; matrix results and isochrones sampled against the
+    oracle."""
+    g, e, o = country
+    from bfgpu.serving import IsochroneRequest, MatrixRequest, ServingLoop, run_round
+    from bfgpu.workload import serving_requests
+    reqs = serving_requests(g, 40, 20261203 + 50, s_range=(10, 600))
+    t0 = time.perf_counter()
+    with ServingLoop(e) as loop:
+        got = [f.result(timeout=300) for f in [loop.submit(r) for r in reqs]]
+    progress(request.config, f"config5 burst of {len(reqs)} in {time.perf_counter() - t0:.1f} s; alone ...")
+    for r, res in zip(reqs, got):
+        assert np.array_equal(res, run_round(e, [r])[0])
+    mats = [k for k, r in enumerate(reqs) if isinstance(r, MatrixRequest)][:4]
+    isos = [k for k, r in enumerate(reqs) if isinstance(r, IsochroneRequest)][:3]
+    for k in mats:
+        assert np.array_equal(got[k], o.compute_matrix_flat_mt(reqs[k].sources, reqs[k].targets))
+    if isos:
+        org = np.array([reqs[k].origin for k in isos], dtype=np.uint32)
+        lim = np.array([reqs[k].limit for k in isos], dtype=np.uint32)
+        exp = bits_of(oracle_rows_masked(o, org, lim), g.n)
+        for j, k in enumerate(isos):
+            assert np.array_equal(got[k], exp[j])
