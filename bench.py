@@ -36,6 +36,10 @@ METRIC = "matrix cells/s 10k×10k u32 bit-exact, % HBM roofline; serving p50/p99
 HBM_PEAK_GBPS = 8000.0
 INF_U32 = 0xFFFFFFFF
 
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
 def copy_probe(nbytes):
     """One device-to-device copy of nbytes by a plain copy kernel (torch's),
     nothing else on the GPU: its FETCH_SIZE and WRITE_SIZE readings per byte
