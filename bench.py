@@ -79,7 +79,7 @@ def traffic_summary(fetch_csv, write_csv, cfetch_csv, cwrite_csv, lanes, workloa
     """FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the DOWN-rows kernel,
     each scaled by what the counter reads per KiB for a plain copy of
     copy_bytes (read copy_bytes, write copy_bytes)."""
-    kern = f"rows_kernel<{lanes}, false>"
+    kern = f"rows_kernel<{lanes}, false, false>"
     n, fetch = counter_means(fetch_csv, "FETCH_SIZE", kern)
     _, write = counter_means(write_csv, "WRITE_SIZE", kern)
     _, cf = counter_means(cfetch_csv, "FETCH_SIZE", "copyBuffer")
@@ -206,48 +206,63 @@ def serving_bench(eng, g, args, rows, chords):
 
 
 def isochrone_bench(eng, g, args, rows, chords):
-    """L isochrones from uniform random origins with uneven limits in one call:
-    reach bits of every node for every lane (times stay on the device except
-    for the checked sample)."""
+    """L isochrones from uniform random origins with uneven limits of 5-60
+    minutes in one block-gated call (bfg_isochrone_lists_device): reach bits of
+    every node and the (rank, time <= limit) lists of every lane, both left
+    in HBM.  The unit is settled node-lanes (list entries) per second."""
+    from bfgpu.hip import DeviceArray, sync
     L = args.isochrones
     org, lim = isochrone_lanes(g, L, args.seed + 30)
-    eng.isochrones(org[:64], lim[:64], want_times=False)  # warm-up
+    words = (g.n + 63) // 64
+    d_bits = DeviceArray(L * words, np.uint64)
+    # a counting call (capacity 0) sizes the pair buffer
+    start, length, total, rc = eng.isochrone_lists_device(org, lim, d_bits.ptr, None, 0)
+    log(f"isochrones: {total} settled node-lanes ({total / (L * g.n):.3f} of all), sizing call rc={rc}")
+    d_pairs = DeviceArray(2 * max(total, 1), np.uint32)
+    eng.isochrone_lists_device(org, lim, d_bits.ptr, d_pairs.ptr, total)  # warm-up
+    sync()
     times = []
     for i in range(args.steps):
         t0 = time.perf_counter()
-        bits, _ = eng.isochrones(org, lim, want_times=False)
+        start, length, tot, rc = eng.isochrone_lists_device(org, lim, d_bits.ptr, d_pairs.ptr, total)
+        sync()
         times.append(time.perf_counter() - t0)
         st = eng.last_stats(L)
         log(f"isochrones {i}: {1e3 * times[-1]:.1f} ms wall, up {st.upward_time_ms:.1f} sweep "
-            f"{st.downward_time_ms:.1f} out {st.output_time_ms:.1f} ms, batches {st.batches}")
+            f"{st.downward_time_ms:.1f} out {st.output_time_ms:.1f} ms, batches {st.batches}, "
+            f"open segments {st.gate_seg_open}/{st.gate_seg_units}, trees {st.gate_bt_open}/{st.gate_bt_units}")
+        if rc != 0 or tot != total:
+            raise RuntimeError("isochrone lists changed between identical calls")
     wall = float(np.median(times))
-    # bit-exact sample: the first lanes' times and bits against the oracle's seeded PHAST
+    # bit-exact sample: the first lanes' lists and bits against the oracle's seeded PHAST
     k = 8
-    sb, stimes = eng.isochrones(org[:k], lim[:k], want_times=True)
     exact = None
     if not args.no_cpu:
         sys.path.insert(0, REPO)
         from oracle.oracle import Oracle
         o = Oracle(g)
         exact = True
+        bits = d_bits.to_host(0, k * words).reshape(k, words)
         for i in range(k):
-            row = np.full(g.n, INF_U32, dtype=np.uint32)
-            for r, d in o.seeded([(int(org[i]), 0)], int(lim[i])):
-                row[r] = d
-            exact &= bool(np.array_equal(stimes[i], row))
-            inside = np.zeros(((g.n + 63) // 64) * 64, dtype=bool)
-            inside[:g.n] = row != INF_U32
+            exp = np.array(o.seeded([(int(org[i]), 0)], int(lim[i])), dtype=np.uint32).reshape(-1, 2)
+            got = d_pairs.to_host(2 * int(start[i]), 2 * int(length[i])).reshape(-1, 2)
+            exact &= bool(np.array_equal(got, exp))
+            inside = np.zeros(words * 64, dtype=bool)
+            inside[exp[:, 0]] = True
             exact &= bool(np.array_equal(bits[i], np.packbits(inside, bitorder="little").view(np.uint64)))
         log(f"isochrone sample of {k} lanes bit-exact vs the oracle: {exact}")
     res = {
-        "metric": "isochrone node-lanes/s (block-gated, uneven limits, one call)",
-        "value": L * g.n / wall, "unit": "node-lanes/s", "n_gpus": 1, "steps": args.steps,
+        "metric": "isochrone settled node-lanes/s (block-gated, uneven limits, one call)",
+        "value": total / wall, "unit": "node-lanes/s", "n_gpus": 1, "steps": args.steps,
         "ms_per_step": 1e3 * wall, "higher_is_better": True, "dtype": "u32", "data": "synthetic",
-        "config": {"workload": f"{L} isochrones, limits from {{5,10,15,30,45,60}} min, uniform random origins; "
-                               f"{rows}x{rows} jittered grid + {int(chords * 100)}% chords, n={g.n}",
-                   "lanes": L, "output": "reach bits of every node per lane, copied to host"},
+        "config": {"workload": f"{L} isochrones, limits uniform from {{5,10,15,30,45,60}} min, uniform random "
+                               f"origins; {rows}x{rows} jittered grid + {int(chords * 100)}% chords, n={g.n}",
+                   "lanes": L, "output": "reach bits and (rank, time) lists of every lane, in HBM"},
+        "settled_node_lanes": total, "settled_fraction": total / (L * g.n),
         "phases_ms": {"up": st.upward_time_ms, "sweep": st.downward_time_ms, "output": st.output_time_ms,
                       "batches": st.batches},
+        "gate": {"segment_units": st.gate_seg_units, "segment_open": st.gate_seg_open,
+                 "tree_units": st.gate_bt_units, "tree_open": st.gate_bt_open},
         "bit_exact_sample": exact,
     }
     print(json.dumps(res), flush=True)
@@ -488,7 +503,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": f"rows_kernel<{KL}, false> (DOWN rows of the top region)",
+                     "kernel": f"rows_kernel<{KL}, false, false> (DOWN rows of the top region)",
                      "launches": dom_launches, "avg_launch_ms": dom_ms / max(dom_launches, 1),
                      "bytes_per_launch": dom_bytes / max(dom_launches, 1),
                      "traffic_source": traffic["source"] if traffic else None,

@@ -32,7 +32,9 @@ class Timings(C.Structure):
     _fields_ = [("up_ms", C.c_float), ("sweep_ms", C.c_float), ("output_ms", C.c_float),
                 ("total_ms", C.c_float), ("batches", C.c_uint32), ("sweep_launches", C.c_uint32),
                 ("lanes_run", C.c_uint64), ("nodes_swept", C.c_uint64), ("arcs_swept", C.c_uint64),
-                ("dom_ms", C.c_float), ("dom_launches", C.c_uint32), ("dom_bytes", C.c_uint64)]
+                ("dom_ms", C.c_float), ("dom_launches", C.c_uint32), ("dom_bytes", C.c_uint64),
+                ("gate_seg_units", C.c_uint64), ("gate_seg_open", C.c_uint64),
+                ("gate_bt_units", C.c_uint64), ("gate_bt_open", C.c_uint64)]
 
 
 class BfcParams(C.Structure):
@@ -63,13 +65,19 @@ BFGPU_SYMBOLS = {
     "bfg_ctx_last_timings": [C.c_void_p, C.POINTER(Timings)],
     "bfg_ctx_configure": [C.c_void_p, C.c_uint32, C.c_uint32],
     "bfg_ctx_tune": [C.c_void_p, C.c_uint32, C.c_uint32],
+    "bfg_ctx_set_concurrency": [C.c_void_p, C.c_uint32],
     "bfg_validate_graph": [C.POINTER(GraphDesc), C.POINTER(CtxInfo)],
     "bfg_matrix": [C.c_void_p, u32p, C.c_uint32, u32p, C.c_uint32, u32p],
     "bfg_matrix_bounded": [C.c_void_p, u32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32, u32p],
     "bfg_matrix_device": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p],
+    "bfg_matrix_multi": [C.c_void_p, C.c_uint32, u32p, u32p, u32p, u32p, u32p, u32p],
     "bfg_query_batch": [C.c_void_p, u32p, C.c_uint32, C.c_uint32, u32p],
     "bfg_isochrone": [C.c_void_p, u32p, u32p, C.c_uint32, u64p, u32p],
     "bfg_phast_seeded": [C.c_void_p, u32p, u32p, u32p, C.c_uint32, u32p, u32p],
+    "bfg_isochrone_lists": [C.c_void_p, u32p, u32p, u32p, C.c_uint32, u32p, u64p, u64p, u32p, u32p, C.c_uint64,
+                            u64p],
+    "bfg_isochrone_lists_device": [C.c_void_p, u32p, u32p, u32p, C.c_uint32, u32p, C.c_void_p, u64p, u32p,
+                                   C.c_void_p, C.c_uint64, u64p],
     "bfg_table_seeded": [C.c_void_p, u32p, u32p, u32p, C.c_uint32, u32p, u32p, u32p, C.c_uint32, C.c_uint32,
                          u32p, C.POINTER(C.c_uint8)],
     "bfg_ctx_attach_length": [C.c_void_p, C.POINTER(GraphDesc), u32p, u32p],

@@ -17,6 +17,9 @@ from .graph import INF, CchGraph
 K_LANES = 8  # the reference's lane count (batched_phast.rs:33); the GPU runs wider groups
 
 
+BFG_BUFFER_TOO_SMALL = 6  # bfg_result: the output buffer holds less than the result
+
+
 class BfgError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"bfgpu error {code}: {msg}")
@@ -43,6 +46,10 @@ class BatchedPhastStats:
     dom_ms: float = 0.0  # the DOWN row kernel: summed launch time, launches, algorithmic bytes
     dom_launches: int = 0
     dom_bytes: int = 0
+    gate_seg_units: int = 0  # block gate: (segment, wave group) units, and those left open
+    gate_seg_open: int = 0
+    gate_bt_units: int = 0   # the same for (bottom tree, wave group)
+    gate_bt_open: int = 0
 
 
 @dataclasses.dataclass
@@ -104,10 +111,17 @@ class BatchedPhastEngine:
                                  output_time_ms=t.output_ms, total_time_ms=t.total_ms, batches=t.batches,
                                  sweep_launches=t.sweep_launches, nodes_swept=t.nodes_swept,
                                  arcs_swept=t.arcs_swept, dom_ms=t.dom_ms, dom_launches=t.dom_launches,
-                                 dom_bytes=t.dom_bytes)
+                                 dom_bytes=t.dom_bytes, gate_seg_units=t.gate_seg_units,
+                                 gate_seg_open=t.gate_seg_open, gate_bt_units=t.gate_bt_units,
+                                 gate_bt_open=t.gate_bt_open)
 
     def configure(self, lanes=0, lane_capacity=0):
         _check(self._lib.bfg_ctx_configure(self._h, lanes, lane_capacity))
+
+    def set_concurrency(self, max_calls):
+        """Calls that may run on the device at once, each on its own stream
+        and label arena (bfg_ctx_set_concurrency)."""
+        _check(self._lib.bfg_ctx_set_concurrency(self._h, max_calls))
 
     def tune(self, plan_rows=0, tri_scalar_segs=None):
         """DOWN unit size and the level width from which intra-segment blocks
@@ -150,6 +164,30 @@ class BatchedPhastEngine:
                                             int(threshold) & INF, _native.ptr(out)))
         return out, self.last_stats(src.size)
 
+    def compute_matrix_multi(self, requests):
+        """Several (sources, targets, threshold) matrix requests in one lane
+        batch (bfg_matrix_multi): a list of row-major S_r*T_r u32 arrays,
+        each equal to compute_matrix_block_gated of the request alone."""
+        srcs = [_u32(r[0]) for r in requests]
+        tgts = [_u32(r[1]) for r in requests]
+        thr = _u32([int(r[2]) & INF if len(r) > 2 else INF for r in requests])
+        so = np.zeros(len(requests) + 1, dtype=np.uint32)
+        to = np.zeros(len(requests) + 1, dtype=np.uint32)
+        so[1:] = np.cumsum([a.size for a in srcs])
+        to[1:] = np.cumsum([b.size for b in tgts])
+        sizes = [a.size * b.size for a, b in zip(srcs, tgts)]
+        out = np.empty(max(sum(sizes), 1), dtype=np.uint32)
+        S = np.concatenate(srcs) if srcs else np.zeros(0, np.uint32)
+        T = np.concatenate(tgts) if tgts else np.zeros(0, np.uint32)
+        P = _native.ptr
+        _check(self._lib.bfg_matrix_multi(self._h, len(requests), P(so), P(_u32(S)), P(to), P(_u32(T)), P(thr),
+                                          P(out)))
+        res, at = [], 0
+        for k in sizes:
+            res.append(out[at:at + k].copy())
+            at += k
+        return res
+
     def compute_matrix_device(self, d_sources, n_src, d_targets, n_tgt, d_out, threshold=INF):
         """Device-resident matrix (bfg_matrix_device): raw HIP pointers (ints or c_void_p)."""
         _check(self._lib.bfg_matrix_device(self._h, d_sources, n_src, d_targets, n_tgt, int(threshold) & INF,
@@ -178,6 +216,74 @@ class BatchedPhastEngine:
                                        _native.ptr(bits, _native.u64p),
                                        _native.ptr(times) if want_times else None))
         return bits, times
+
+    @staticmethod
+    def _seed_csr(seed_lists):
+        L = len(seed_lists)
+        off = np.zeros(L + 1, dtype=np.uint32)
+        ranks, costs = [], []
+        for i, seeds in enumerate(seed_lists):
+            for r, c in seeds:
+                ranks.append(r)
+                costs.append(c)
+            off[i + 1] = len(ranks)
+        return off, _u32(ranks), _u32(costs)
+
+    def isochrone_lists(self, seed_lists, thresholds, want_bits=False, capacity=None):
+        """run_phast_bounded_fast_seeded (isochrone_handler.rs:338-491) for many
+        lanes in one block-gated call: per lane an (m, 2) u32 array of
+        (rank, dist <= threshold) in ascending rank, plus the reach bits
+        [L, ceil(n/64)] when want_bits.  capacity: pairs to reserve (default
+        an estimate; a second call runs when it is short)."""
+        L = len(seed_lists)
+        off, ranks, costs = self._seed_csr(seed_lists)
+        thr = _u32(thresholds)
+        if thr.size != L:
+            raise ValueError("one threshold per lane")
+        words = (self.n_nodes + 63) // 64
+        bits = np.empty((L, words), dtype=np.uint64) if want_bits else None
+        start = np.zeros(L, dtype=np.uint64)
+        length = np.zeros(L, dtype=np.uint32)
+        total = np.zeros(1, dtype=np.uint64)
+        cap = int(capacity) if capacity is not None else min(L * self.n_nodes, max(1 << 16, L * 4096))
+        P = _native.ptr
+        for _ in range(2):
+            pairs = np.empty(2 * max(cap, 1), dtype=np.uint32)
+            rc = self._lib.bfg_isochrone_lists(self._h, P(off), P(ranks), P(costs), L, P(thr),
+                                               P(bits, _native.u64p) if want_bits else None,
+                                               P(start, _native.u64p), P(length), P(pairs), cap,
+                                               P(total, _native.u64p))
+            if rc == BFG_BUFFER_TOO_SMALL:
+                cap = int(total[0])
+                continue
+            _check(rc)
+            break
+        pv = pairs.reshape(-1, 2)
+        lists = [pv[int(s):int(s) + int(n)] for s, n in zip(start, length)]
+        return (lists, bits) if want_bits else lists
+
+    def isochrone_lists_device(self, origins, thresholds, d_bits, d_pairs, capacity):
+        """Device-resident isochrone lists (bfg_isochrone_lists_device) of
+        single-origin lanes: reach bits and (rank, dist) pairs stay in the HIP
+        buffers d_bits / d_pairs (raw pointers); returns (start [L] u64,
+        length [L] u32, total pairs, BFG result code: 0 or
+        BFG_BUFFER_TOO_SMALL)."""
+        org, thr = _u32(origins), _u32(thresholds)
+        L = org.size
+        if thr.size != L:
+            raise ValueError("one threshold per lane")
+        off = np.arange(L + 1, dtype=np.uint32)
+        cost = np.zeros(L, dtype=np.uint32)
+        start = np.zeros(L, dtype=np.uint64)
+        length = np.zeros(L, dtype=np.uint32)
+        total = np.zeros(1, dtype=np.uint64)
+        P = _native.ptr
+        rc = self._lib.bfg_isochrone_lists_device(self._h, P(off), P(org), P(cost), L, P(thr), d_bits,
+                                                  P(start, _native.u64p), P(length), d_pairs, int(capacity),
+                                                  P(total, _native.u64p))
+        if rc not in (0, BFG_BUFFER_TOO_SMALL):
+            _check(rc)
+        return start, length, int(total[0]), rc
 
     def phast_seeded(self, seed_lists, thresholds):
         """Many run_phast_bounded_fast_seeded calls in one launch: dense rows [L, n]."""
@@ -307,9 +413,8 @@ class BatchedPhastEngine:
 
     def run_phast_bounded_fast_seeded(self, seeds, threshold):
         """isochrone_handler.rs:338-344: Vec<(rank, dist)> ascending rank, dist <= threshold."""
-        row = self.phast_seeded([list(seeds)], [threshold])[0]
-        idx = np.nonzero(row != INF)[0]
-        return [(int(r), int(row[r])) for r in idx]
+        lst = self.isochrone_lists([list(seeds)], [threshold])[0]
+        return [(int(r), int(d)) for r, d in lst]
 
     def run_phast_bounded_fast(self, origin_rank, threshold):
         """isochrone_handler.rs:315-329."""

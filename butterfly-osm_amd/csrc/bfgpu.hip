@@ -44,12 +44,15 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <numeric>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -478,19 +481,75 @@ constexpr uint32_t UNIT_WHOLE = 0x80000000u;
 constexpr uint32_t UNIT_RUN_SHIFT = 28;  // DOWN: the unit's run c in bits 28..30
 constexpr uint32_t UNIT_SEG_MASK = (1u << UNIT_RUN_SHIFT) - 1;
 
-template <int KL, bool PUSH>
+// The block gate of bounded calls (the K-lane form of the reference's
+// active-block skipping, batched_phast.rs:1040-1064, isochrone_handler.rs:
+// 414-450).  Outside the upward search space a label is a min over DOWN arcs
+// from ancestors, so a DOWN row (source u, weights to a segment) can lower a
+// label to d <= limit only if label(u) + (the row's least weight) <= limit.
+// A unit none of whose rows passes that bound for any lane of its wave group
+// is skipped (its labels stay the upward ones, or INF); a segment none of
+// whose labels is within its lane's limit skips its intra-segment arcs.
+// Skipped values are >= the true labels and > the limits, so every label
+// within its limit stays exact and every other one is masked on output.
+struct Gate {
+    const uint32_t *rmin;        // least weight of each DOWN row; nullptr: no gate
+    const uint32_t *lane_limit;  // per sorted lane
+    uint32_t n_lanes;
+    uint8_t *seg_live;           // [wave group][segment]: a label within a limit (statistics)
+    uint32_t nseg;
+};
+
+// Whether some row of [r0, r1) passes the bound for some lane of this
+// wavefront (wave-uniform).  Keys and least weights of 64 rows come in with
+// one vector load each; eight labels are loaded before any is compared.
+template <int KL>
+__device__ __forceinline__ bool rows_live(const Gate &G, uint32_t wg, uint32_t r0, uint32_t r1,
+                                          const uint32_t *__restrict__ key, const uint32_t *lab) {
+    const uint32_t wl = __lane_id(), gl = wg * 64 + wl;
+    const uint32_t lim = gl < G.n_lanes ? G.lane_limit[gl] : 0u;
+    const bool valid = gl < G.n_lanes;
+    for (uint32_t c0 = r0; c0 < r1; c0 += 64) {
+        const uint32_t cl = min(64u, r1 - c0) - 1;
+        const uint32_t kv = key[c0 + min(wl, cl)], mv = G.rmin[c0 + min(wl, cl)];
+        bool live = false;
+        for (uint32_t j = 0; j <= cl; j += 8) {
+            uint32_t x[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) x[q] = lab[(size_t)rl(kv, min(j + q, cl)) * KL];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const uint32_t d = sat_add(x[q], rl(mv, min(j + q, cl)));
+                live |= (x[q] != INF) & (d <= lim);
+            }
+        }
+        if (__ballot(live && valid) != 0) return true;
+    }
+    return false;
+}
+
+template <int KL, bool PUSH, bool GATE>
 __global__ __launch_bounds__(TILE_THREADS) void rows_kernel(
     uint32_t u0, uint32_t u1, const uint32_t *__restrict__ seg, const uint32_t *__restrict__ r0,
     const uint32_t *__restrict__ r1, const uint32_t *__restrict__ wid,
     const uint32_t *__restrict__ woff, const uint32_t *__restrict__ seg_pos, const uint32_t *__restrict__ tgt,
-    const uint32_t *__restrict__ key, const uint32_t *__restrict__ wts, Reach R, uint32_t *__restrict__ arena,
-    uint32_t n, uint32_t wgs) {
+    const uint32_t *__restrict__ key, const uint32_t *__restrict__ wts, Reach R, Gate G,
+    uint32_t *__restrict__ arena, uint32_t n, uint32_t wgs) {
     const uint32_t u = u0 + blockIdx.y, wg = blockIdx.x * (TILE_THREADS / 64) + wave_id();
     if (u >= u1 || wg >= wgs) return;  // wave-uniform; this kernel has no barrier
     uint32_t *lab = lane_labels<KL>(arena, n, wg, nullptr);
     const uint32_t sw = seg[u], s = sw & UNIT_SEG_MASK;
     const bool reach = wave_reaches(R, wg, seg_pos[s]);
     if (PUSH && !reach) return;  // the segment's labels are all INF: nothing to push
+    if (GATE && !rows_live<KL>(G, wg, r0[u], r1[u], key, lab)) {
+        // no row of the unit reaches a limit: a whole unit of an unreached
+        // wave group stores INF (its upward labels were never written); the
+        // upward labels (or the INF of split segments) stand otherwise
+        if ((sw & UNIT_WHOLE) && !reach) {
+            const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
+            for (uint32_t j = 0; j < len; ++j) lab[(size_t)(p0 + j) * KL] = INF;
+        }
+        return;
+    }
     if constexpr (PUSH) {
         const uint32_t ra = r0[u], rb = r0[u + 1], W = wid[u];
         const cu32 *wr = as_const(wts) + woff[u], *out = as_const(tgt) + (size_t)u * 64;
@@ -521,12 +580,12 @@ __global__ __launch_bounds__(TILE_THREADS) void rows_kernel(
 // wave group that does not reach the segment skips it, only writing INF
 // where the DOWN plan merges into the labels with atomics (split[s]).  SC:
 // scalar-loaded blocks (tri_apply_s) for levels of many segments.
-template <int KL, bool DOWN, bool SC>
+template <int KL, bool DOWN, bool SC, bool GATE>
 __global__ __launch_bounds__(TILE_THREADS) void tri_kernel(uint32_t first, uint32_t count,
                                                            const uint32_t *__restrict__ seg_pos,
                                                            const uint32_t *__restrict__ tab,
                                                            const uint32_t *__restrict__ tw,
-                                                           const uint8_t *__restrict__ split, Reach R,
+                                                           const uint8_t *__restrict__ split, Reach R, Gate G,
                                                            uint32_t *__restrict__ arena, uint32_t n,
                                                            uint32_t wgs) {
     const uint32_t i = blockIdx.y, wg = blockIdx.x * (TILE_THREADS / 64) + wave_id();
@@ -544,12 +603,62 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_kernel(uint32_t first, uint3
     uint32_t acc[64];
 #pragma unroll
     for (int j = 0; j < 64; ++j) acc[j] = j < len ? lab[(size_t)(p0 + j) * KL] : INF;
+    if constexpr (DOWN && GATE) {
+        {  // intra arcs only raise labels from above the limits: skip them then
+            const uint32_t gl = wg * 64 + __lane_id();
+            const uint32_t lim = gl < G.n_lanes ? G.lane_limit[gl] : 0u;
+            bool live = false;
+#pragma unroll
+            for (int j = 0; j < 64; ++j) live |= (j < len) & (acc[j] != INF) & (acc[j] <= lim);
+            const bool alive = __ballot(live && gl < G.n_lanes) != 0;
+            if (__lane_id() == 0) G.seg_live[(size_t)wg * G.nseg + s] = alive ? 1 : 0;
+            if (!alive) return;
+        }
+    }
     if constexpr (SC) tri_apply_s<DOWN, 8>(acc, tab + (size_t)s * 64, as_const(tw));
     else tri_apply<DOWN, 8>(acc, tab + (size_t)s * 64, tw);
 #pragma unroll
     for (int j = 0; j < 64; ++j)
         if (j < len) lab[(size_t)(p0 + j) * KL] = acc[j];
 }
+
+// bottom_gate: per (bottom tree, lane group), whether the tree can hold a
+// label within a lane's limit: some bottom chunk holding part of the tree
+// was left open by the chunk gate (chunk_ok).  Thread (blockIdx.x * NT +
+// threadIdx.x) takes tree t of group blockIdx.y.
+__global__ __launch_bounds__(NT) void bottom_gate_kernel(uint32_t n_bt, const uint32_t *__restrict__ bt_ch,
+                                                         const uint8_t *__restrict__ chunk_ok, uint32_t nch,
+                                                         uint8_t *__restrict__ open_b) {
+    const uint32_t t = blockIdx.x * NT + threadIdx.x, g = blockIdx.y;
+    if (t >= n_bt) return;
+    uint8_t o = 0;
+    for (uint32_t c = bt_ch[2 * t]; c <= bt_ch[2 * t + 1]; ++c) o |= chunk_ok[(size_t)g * nch + c];
+    open_b[(size_t)g * n_bt + t] = o;
+}
+
+// The bottom gate as the bottom kernels read it: open_b = nullptr means no
+// gate; open_b[g * n_bt + t] for lane group g and bottom tree t.
+struct BGate {
+    const uint8_t *open_b;
+    const uint32_t *pos_bt;  // tree of bottom position p at pos_bt[p - n_top]
+    uint32_t n_bt, n_top;
+};
+
+template <int KL>
+__device__ __forceinline__ bool bottom_open(const BGate &B, uint32_t g, uint32_t p) {
+    return B.open_b[(size_t)g * B.n_bt + B.pos_bt[p - B.n_top]] != 0;
+}
+
+// The chunk gate: per (bottom chunk, lane group), whether a lane can settle a
+// node of the chunk within its limit: a seed's path slot within the limit,
+// or a top source u of the chunk with label(u) + (least weight of u's arcs
+// into the chunk) within it.  A closed chunk's labels are INF.
+struct CGate {
+    const uint32_t *cs_min;      // least weight of each chunk source's arcs; nullptr: no gate
+    const uint32_t *lane_limit;  // per sorted lane
+    uint8_t *chunk_ok;           // [group][chunk]
+    uint32_t nch;
+};
 
 // Relaxes the grouped arcs of sources [k0, k1) into the LDS labels: sources
 // go in batches of UNR per slot — ids, arc ranges and labels of a batch are
@@ -626,7 +735,8 @@ __global__ __launch_bounds__(NTH) void bottom_chunk_kernel(
     const uint32_t *__restrict__ ca_w, const uint32_t *__restrict__ pos_pre, const uint32_t *__restrict__ pos_sub,
     const uint32_t *__restrict__ pos_depth, const uint32_t *__restrict__ seed_off,
     const uint32_t *__restrict__ seed_pre, const uint32_t *__restrict__ seed_depth,
-    const uint32_t *__restrict__ pathbuf, uint32_t D1, uint32_t n_lanes, uint32_t *__restrict__ arena, uint32_t n) {
+    const uint32_t *__restrict__ pathbuf, uint32_t D1, uint32_t n_lanes, CGate CG, uint32_t *__restrict__ arena,
+    uint32_t n) {
     constexpr int NS = NTH / KL;
     __shared__ uint32_t L[BB * KL];
     const uint32_t c = blockIdx.x, g = blockIdx.y;
@@ -648,7 +758,28 @@ __global__ __launch_bounds__(NTH) void bottom_chunk_kernel(
         }
         L[i * KL + lane] = v;
     }
-    __syncthreads();
+    if (CG.cs_min) {  // the chunk gate (workgroup-uniform outcome)
+        const uint32_t lim = gl < n_lanes ? CG.lane_limit[gl] : 0u;
+        bool live = false;
+        if (gl < n_lanes) {
+            for (uint32_t i = slot; i < len; i += NS) {
+                const uint32_t v = L[i * KL + lane];
+                live |= (v != INF) & (v <= lim);
+            }
+            for (uint32_t k = cs_off[c] + slot; k < cs_off[c + 1]; k += NS) {
+                const uint32_t x = lab[(size_t)cs_src[k] * KL + lane];
+                live |= (x != INF) & (sat_add(x, CG.cs_min[k]) <= lim);
+            }
+        }
+        const bool open = __syncthreads_or(live) != 0;
+        if (threadIdx.x == 0) CG.chunk_ok[(size_t)g * CG.nch + c] = open ? 1 : 0;
+        if (!open) {
+            for (uint32_t i = slot; i < len; i += NS) lab[(size_t)ch_pos[i0 + i] * KL + lane] = INF;
+            return;
+        }
+    } else {
+        __syncthreads();
+    }
     relax_sources<KL, NS>(L, lab, cs_off[c], cs_off[c + 1], cs_src, cs_aoff, ca_v, ca_w, slot, lane);
     __syncthreads();
     for (uint32_t i = slot; i < len; i += NS) lab[(size_t)ch_pos[i0 + i] * KL + lane] = L[i * KL + lane];
@@ -831,12 +962,13 @@ template <int KL>
 __global__ __launch_bounds__(NT) void bottom_pull_kernel(uint32_t p0, uint32_t p1, const uint32_t *__restrict__ in_off,
                                                          const uint32_t *__restrict__ in_split,
                                                          const uint32_t *__restrict__ in_src,
-                                                         const uint32_t *__restrict__ in_w,
+                                                         const uint32_t *__restrict__ in_w, BGate B,
                                                          uint32_t *__restrict__ arena, uint32_t n) {
     constexpr int NS = NT / KL;
     const uint32_t lane = threadIdx.x % KL, slot = slot_id<KL>();
     const uint32_t p = p0 + blockIdx.x * NS + slot;
     if (p >= p1) return;
+    if (B.open_b && !bottom_open<KL>(B, blockIdx.y, p)) return;  // a dead tree: never read
     uint32_t *lab = arena + (size_t)blockIdx.y * n * KL + lane;
     uint32_t a = in_off[p];
     const uint32_t a1 = in_split[p];
@@ -879,20 +1011,26 @@ __global__ __launch_bounds__(NT) void bottom_pull_kernel(uint32_t p0, uint32_t p
 template <int KL>
 __global__ __launch_bounds__(NT) void gather_kernel(
     const uint32_t *__restrict__ arena, uint32_t n, const uint32_t *__restrict__ tgt_pos, uint32_t T,
-    const uint32_t *__restrict__ lane_limit, uint32_t n_lanes, const uint32_t *__restrict__ row_map,
-    uint32_t *__restrict__ out, uint64_t ld, unsigned long long *__restrict__ bits, uint64_t bits_ld) {
+    const uint32_t *__restrict__ lane_limit, uint32_t n_lanes, const uint32_t *__restrict__ row_map, BGate B,
+    uint32_t *__restrict__ out, uint64_t ld, unsigned long long *__restrict__ bits, uint64_t bits_ld, uint32_t g0,
+    uint32_t lane_lo, uint32_t row_base) {
     __shared__ uint32_t tile[64][KL + 1];
     const uint32_t t0 = blockIdx.x * 64;
-    const uint32_t g = blockIdx.y;
+    const uint32_t g = g0 + blockIdx.y;
     const uint32_t *lab = arena + (size_t)g * n * KL;
     for (uint32_t idx = threadIdx.x; idx < 64 * KL; idx += NT) {
         const uint32_t ti = idx / KL, lane = idx % KL;
         const uint32_t t = t0 + ti;
         uint32_t v = INF;
         const uint32_t gl = g * KL + lane;
-        if (t < T && gl < n_lanes) {
-            v = lab[(size_t)tgt_pos[t] * KL + lane];
-            if (v > lane_limit[gl]) v = INF;
+        if (t < T && gl >= lane_lo && gl < n_lanes) {
+            const uint32_t p = tgt_pos[t];
+            // positions of dead bottom trees were never written (with a gate,
+            // tgt_pos holds positions)
+            if (!B.open_b || p < B.n_top || bottom_open<KL>(B, g, p)) {
+                v = lab[(size_t)p * KL + lane];
+                if (v > lane_limit[gl]) v = INF;
+            }
         }
         tile[ti][lane] = v;
     }
@@ -901,8 +1039,8 @@ __global__ __launch_bounds__(NT) void gather_kernel(
         for (uint32_t idx = threadIdx.x; idx < 64 * KL; idx += NT) {
             const uint32_t lane = idx / 64, ti = idx % 64;
             const uint32_t gl = g * KL + lane, t = t0 + ti;
-            if (gl < n_lanes && t < T) {
-                const uint64_t row = row_map ? row_map[gl] : gl;
+            if (gl >= lane_lo && gl < n_lanes && t < T) {
+                const uint64_t row = (row_map ? row_map[gl] : gl) - row_base;
                 out[row * ld + t] = tile[ti][lane];
             }
         }
@@ -913,10 +1051,140 @@ __global__ __launch_bounds__(NT) void gather_kernel(
             const uint32_t gl = g * KL + lane;
             const bool r = (t0 + ti < T) && tile[ti][lane] != INF;
             const unsigned long long word = __ballot(r);
-            if (ti == 0 && gl < n_lanes) {
+            if (ti == 0 && gl >= lane_lo && gl < n_lanes) {
                 const uint64_t row = row_map ? row_map[gl] : gl;
                 bits[row * bits_ld + (t0 / 64)] = word;
             }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Isochrone lists: per lane, the settled nodes (rank, time <= limit) in
+// ascending rank, the output of run_phast_bounded_fast_seeded
+// (isochrone_handler.rs:453-471), for every lane of a batch at once.  Ranks
+// are cut into superblocks of SB_RANKS (64 reach-bit words).
+//   iso_count  per row: the reach bits' popcount per superblock, turned into
+//              the row's exclusive offsets by superblock, and the row total
+//   iso_scan   the batch's row totals into row starts (one workgroup)
+//   iso_emit   per (superblock, lane group): 64-rank tiles through LDS as in
+//              gather; per lane a ballot over the tile's ranks and the
+//              position of each settled rank among them (mbcnt) give its slot
+// ---------------------------------------------------------------------------
+constexpr uint32_t SB_RANKS = 4096;
+constexpr int ISO_THREADS = 256;
+
+// Row rows_sorted[blockIdx.x] of the reach bits (bits_ld words per row):
+// sb_off[i * nsb + b] = settled ranks of the row before superblock b,
+// row_tot[i] = all of them.  Thread t takes a run of consecutive superblocks.
+__global__ __launch_bounds__(ISO_THREADS) void iso_count_kernel(const unsigned long long *__restrict__ bits,
+                                                                uint64_t words, uint64_t bits_ld,
+                                                                const uint32_t *__restrict__ rows_sorted, uint32_t nsb,
+                                                                uint32_t *__restrict__ sb_off,
+                                                                uint32_t *__restrict__ row_tot) {
+    __shared__ uint32_t part[ISO_THREADS];
+    const uint32_t i = blockIdx.x, t = threadIdx.x;
+    const unsigned long long *row = bits + (uint64_t)rows_sorted[i] * bits_ld;
+    const uint32_t per = (nsb + ISO_THREADS - 1) / ISO_THREADS;
+    const uint32_t b0 = min(nsb, t * per), b1 = min(nsb, b0 + per);
+    uint32_t sum = 0;
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint64_t w1 = min<uint64_t>(words, (uint64_t)(b + 1) * (SB_RANKS / 64));
+        for (uint64_t w = (uint64_t)b * (SB_RANKS / 64); w < w1; ++w) sum += __popcll(row[w]);
+    }
+    part[t] = sum;
+    __syncthreads();
+    // exclusive scan of the per-thread sums (Hillis-Steele in LDS)
+    for (uint32_t o = 1; o < ISO_THREADS; o <<= 1) {
+        const uint32_t v = t >= o ? part[t - o] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;  // ranks before this thread's run
+    for (uint32_t b = b0; b < b1; ++b) {
+        sb_off[(uint64_t)i * nsb + b] = run;
+        const uint64_t w1 = min<uint64_t>(words, (uint64_t)(b + 1) * (SB_RANKS / 64));
+        for (uint64_t w = (uint64_t)b * (SB_RANKS / 64); w < w1; ++w) run += __popcll(row[w]);
+    }
+    if (t == ISO_THREADS - 1) row_tot[i] = part[t];
+}
+
+// row_start[i] = base + row_tot[0] + .. + row_tot[i - 1]; total[0] = base + all.
+__global__ __launch_bounds__(ISO_THREADS) void iso_scan_kernel(const uint32_t *__restrict__ row_tot, uint32_t nl,
+                                                               uint64_t base, uint64_t *__restrict__ row_start,
+                                                               uint64_t *__restrict__ total) {
+    __shared__ uint64_t part[ISO_THREADS];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nl + ISO_THREADS - 1) / ISO_THREADS;
+    const uint32_t i0 = min(nl, t * per), i1 = min(nl, i0 + per);
+    uint64_t sum = 0;
+    for (uint32_t i = i0; i < i1; ++i) sum += row_tot[i];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t o = 1; o < ISO_THREADS; o <<= 1) {
+        const uint64_t v = t >= o ? part[t - o] : 0ull;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint64_t run = base + part[t] - sum;
+    for (uint32_t i = i0; i < i1; ++i) {
+        row_start[i] = run;
+        run += row_tot[i];
+    }
+    if (t == ISO_THREADS - 1) total[0] = base + part[t];
+}
+
+// Superblock blockIdx.x of lane group blockIdx.y: 64 tiles of 64 ranks, each
+// read as in gather (limit mask, bottom gate); lane gl's pairs go from
+// row_start[slot[gl]] + sb_off[slot[gl] * nsb + b] on, in rank order.
+template <int KL>
+__global__ __launch_bounds__(NT) void iso_emit_kernel(const uint32_t *__restrict__ arena, uint32_t n_lab,
+                                                      const uint32_t *__restrict__ pos_of_rank, uint32_t n,
+                                                      const uint32_t *__restrict__ lane_limit, uint32_t n_lanes,
+                                                      const uint32_t *__restrict__ slot,
+                                                      const uint64_t *__restrict__ row_start,
+                                                      const uint32_t *__restrict__ sb_off, uint32_t nsb, BGate B,
+                                                      uint2 *__restrict__ pairs) {
+    __shared__ uint32_t tile[64][KL + 1];
+    __shared__ uint64_t base[KL];
+    const uint32_t b = blockIdx.x, g = blockIdx.y;
+    const uint32_t *lab = arena + (size_t)g * n_lab * KL;
+    for (uint32_t lane = threadIdx.x; lane < KL; lane += NT) {
+        const uint32_t gl = g * KL + lane;
+        base[lane] = gl < n_lanes ? row_start[slot[gl]] + sb_off[(uint64_t)slot[gl] * nsb + b] : 0ull;
+    }
+    for (uint32_t tt = 0; tt < SB_RANKS / 64; ++tt) {
+        const uint32_t t0 = b * SB_RANKS + tt * 64;
+        if (t0 >= n) break;  // workgroup-uniform
+        __syncthreads();     // the previous tile is consumed
+        for (uint32_t idx = threadIdx.x; idx < 64 * KL; idx += NT) {
+            const uint32_t ti = idx / KL, lane = idx % KL;
+            const uint32_t t = t0 + ti, gl = g * KL + lane;
+            uint32_t v = INF;
+            if (t < n && gl < n_lanes) {
+                const uint32_t p = pos_of_rank[t];
+                if (!B.open_b || p < B.n_top || bottom_open<KL>(B, g, p)) {
+                    v = lab[(size_t)p * KL + lane];
+                    if (v > lane_limit[gl]) v = INF;
+                }
+            }
+            tile[ti][lane] = v;
+        }
+        __syncthreads();
+        // wavefront w takes lanes w, w + NT/64, ..; thread ti one rank
+        const uint32_t w = threadIdx.x / 64, ti = threadIdx.x % 64;
+        for (uint32_t lane = w; lane < KL; lane += NT / 64) {
+            const uint32_t v = tile[ti][lane];
+            const bool r = (t0 + ti < n) && v != INF && g * KL + lane < n_lanes;
+            const uint64_t m = __ballot(r);
+            const uint64_t at = base[lane];
+            if (r) {
+                const uint32_t before = __popcll(m & ((1ull << ti) - 1ull));
+                pairs[at + before] = make_uint2(t0 + ti, v);
+            }
+            if (ti == 0) base[lane] = at + __popcll(m);  // this wavefront alone owns the lane
         }
     }
 }
@@ -968,25 +1236,41 @@ __global__ __launch_bounds__(NT) void table_gather_kernel(
 // ---------------------------------------------------------------------------
 // host context
 // ---------------------------------------------------------------------------
+// A device buffer: owned (hipMalloc'd, grown on demand) or borrowed (a fixed
+// range of memory the library holds elsewhere, which is never reallocated: a
+// request beyond its size fails with hipErrorOutOfMemory).
 template <class T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    bool borrowed = false;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
+        if (p && !borrowed) (void)hipFree(p);
+        if (!borrowed) {
+            p = nullptr;
+            n = 0;
+        }
     }
     hipError_t alloc(size_t count) {
+        if (borrowed) return count <= n ? hipSuccess : hipErrorOutOfMemory;
         release();
         if (count == 0) count = 1;
         hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
         if (e == hipSuccess) n = count;
+        else p = nullptr;
         return e;
+    }
+    // at least `count` elements; contents are not kept when it grows
+    hipError_t ensure(size_t count) { return count <= n && p ? hipSuccess : alloc(count); }
+    void borrow(T *ptr, size_t count) {
+        release();
+        p = ptr;
+        n = count;
+        borrowed = true;
     }
     hipError_t upload(const std::vector<T> &v, hipStream_t s) {
         hipError_t e = alloc(v.size());
@@ -1051,14 +1335,82 @@ struct TwoChannel {
 
 }  // namespace
 
+// DOWN work plan for batches of one wave-group count: units (segment |
+// run | UNIT_WHOLE, rows [r0, r1), weights from w) by level.  Plans are shared
+// by the calls in flight and never change once built.
+struct Plan {
+    uint32_t wgs = 0;
+    std::vector<uint32_t> lvl;  // units of level l: [lvl[l], lvl[l+1])
+    DevBuf<uint32_t> seg, r0, r1, w;
+    DevBuf<uint8_t> split;      // 1 where the plan merges a segment's rows with atomics
+};
+
+// A call's lanes: per lane seeds and limit (host), original lane order.
+struct Req {
+    std::vector<uint32_t> soff, srank, scost, limit;
+    std::vector<uint32_t> major;  // optional: lanes are sorted by it first (the request of a lane)
+};
+
+// The device state of one call in flight: its stream, label arena, seeds,
+// target paths and gate flags.  A context runs up to `max_works` calls at
+// once, each on its own Work (and stream); a Work whose buffers are borrowed
+// from the library's static scratch array (`fixed`) never allocates.
+struct Work {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> dom_ev;  // event pairs around each DOWN row launch of a batch
+    bool fixed = false;              // buffers borrowed from the scratch array
+    bool time_dom = true;            // record the dominant kernel's launch times
+    uint32_t lane_cap = 0;           // lanes per batch of the current call (multiple of 64)
+    DevBuf<uint32_t> arena, pathbuf;
+    DevBuf<uint32_t> seed_off, seed_rank, seed_pos, seed_cost, seed_depth, seed_pre, lane_limit, row_map;
+    DevBuf<uint32_t> tgt_pos, path_off, path_pos;
+    // gate flags of a bounded call: per (wave group, segment) whether a label
+    // is within its lane's limit (statistics); per (lane group, bottom chunk)
+    // and (lane group, bottom tree) whether the chunk gate left it open
+    DevBuf<uint8_t> seg_live, chunk_ok, open_b;
+    bool gate = false;
+    // bottom paths above the targets of the current matrix call; while
+    // paths_on, the downward sweep covers the top region and these paths only
+    bool paths_on = false;  // while on, labels are compact (see label_stride)
+    uint32_t n_paths = 0;
+    uint64_t path_nodes = 0, path_arcs = 0;
+    // host copies of the staged path tables, kept for their asynchronous
+    // uploads; the last element of each path (its target); whether path_pos
+    // of this call is on the device yet
+    std::vector<uint32_t> path_off_h, path_pos_h, tgt_pos_h, path_leaf;
+    bool path_pos_staged = false;
+    // host staging of a call (kept, so a fixed Work reuses its capacity)
+    std::vector<uint32_t> h_bt, h_pending, h_order, h_soff, h_srank, h_scost, h_lim, h_rows, h_sdepth, h_spre,
+        h_spos, h_src, h_tgt;
+    std::vector<uint64_t> h_key;
+    std::vector<std::pair<uint32_t, uint32_t>> h_elem;
+    Req req;  // the request of a call that must not allocate (the scratch Work)
+    bfg_timings last{};
+
+    Work() = default;
+    Work(const Work &) = delete;
+    Work &operator=(const Work &) = delete;
+    ~Work() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto &e : dom_ev) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
 struct bfg_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
-    std::mutex mu;
     bfg::Layout L;
-    uint32_t lanes = 64;        // KL: sources per lane group (16, 32 or 64)
-    uint32_t lane_cap = 0;      // lanes per batch (multiple of 64)
+    // configuration: calls hold it shared, configure / tune / attach_length
+    // exclusively
+    std::shared_mutex cfg;
+    uint32_t lanes = 64;        // KL: sources per lane group (16, 32, 64 or 128)
+    uint32_t lane_cap = 0;      // lanes per batch when cap_fixed (multiple of 64)
     bool cap_fixed = false;
+    // tuning set by bfg_ctx_tune (0: the defaults or their environment overrides)
+    uint32_t plan_rows = 0, tri_segs = 0;
     // resident graph
     DevBuf<uint32_t> parent, pos_parent, pos_pre, pos_sub, pos_depth, seg_pos;
     DevBuf<uint32_t> in_off, in_split, in_src, in_w;
@@ -1068,53 +1420,47 @@ struct bfg_ctx {
     DevBuf<uint32_t> xr_off, xr_run, xw_off, xr_src, xr_w, tb_dn, tb_up, tb_w, pt_blk, pt_tgt, pr_off, pr_wid,
         pw_off, pr_x, pr_w;
     DevBuf<uint32_t> pk_seg;
-    // DOWN work plan for the current number of wave groups per batch: units
-    // (segment | UNIT_WHOLE, rows) by level
-    uint32_t plan_wgs = 0;
-    std::vector<uint32_t> plan_lvl;
-    // tuning set by bfg_ctx_tune (0: the defaults or their environment overrides)
-    uint32_t plan_rows = 0, tri_segs = 0;
+    // gate tables: the least weight of each DOWN row (xr order) and of each
+    // bottom chunk source's arcs (cs order); the bottom trees (maximal
+    // subtrees of the bottom region) in pre-order: the tree of each bottom
+    // position and the chunks each tree touches, [bt_ch[2t], bt_ch[2t+1]]
+    DevBuf<uint32_t> xr_min, cs_min, pos_bt, bt_ch;
+    uint32_t n_bt = 0;
+    DevBuf<uint32_t> pos_of_rank_d;  // position of each rank (the dense gathers' target table)
     std::vector<uint64_t> lvl_dom_bytes;  // algorithmic bytes of a level's DOWN rows per wave group
-    DevBuf<uint32_t> du_seg, du_r0, du_r1, du_w;
-    DevBuf<uint8_t> seg_split;  // 1 where the plan merges a segment's rows with atomics
-    // arenas (per batch)
-    DevBuf<uint32_t> arena, pathbuf;
-    DevBuf<uint32_t> seed_off, seed_rank, seed_pos, seed_cost, seed_depth, seed_pre, lane_limit, row_map;
-    size_t seeds_cap = 0, lanes_alloc = 0;
-    DevBuf<uint32_t> tgt_pos;
-    size_t tgt_cap = 0;
-    // bottom paths above the targets of the current matrix call; while
-    // paths_on, the downward sweep covers the top region and these paths only
-    DevBuf<uint32_t> path_off, path_pos;
-    bool paths_on = false;  // while on, labels are compact (see label_stride)
-    uint32_t n_paths = 0;
-    uint64_t path_nodes = 0, path_arcs = 0, top_in_arcs = 0;
+    uint64_t top_in_arcs = 0;
     // bdep[p]: levels between bottom position p and the top region (0 for a
-    // bottom root); host copies of the staged path tables, kept for their
-    // asynchronous uploads; the last element of each path (its target) and
-    // whether path_pos of this call is on the device yet
-    std::vector<uint32_t> bdep, path_off_h, path_pos_h, tgt_pos_h, path_leaf;
-    bool path_pos_staged = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::vector<hipEvent_t> dom_ev;     // event pairs around each DOWN row launch of a batch
-
+    // bottom root)
+    std::vector<uint32_t> bdep;
+    // DOWN plans by wave-group count (a few recent ones; calls in flight hold
+    // theirs), and the one-wave-group plan the scratch Work uses, built at
+    // creation
+    std::mutex plan_mu;
+    std::map<uint32_t, std::shared_ptr<const Plan>> plans;
+    std::vector<uint32_t> plan_lru;
+    std::shared_ptr<const Plan> plan1;
+    // Works: idle ones wait in the pool; at most max_works exist
+    std::mutex pool_mu;
+    std::condition_variable pool_cv;
+    std::vector<std::unique_ptr<Work>> works;
+    std::vector<Work *> idle;
+    uint32_t max_works = 4;
+    // the Work over the static scratch array (bfg_matrix_device_noalloc)
+    std::unique_ptr<Work> scratch_work;
+    // timings of the last finished call
+    std::mutex last_mu;
     bfg_timings last{};
     uint64_t graph_bytes = 0;
+    std::mutex t2_mu;                // two-channel calls share the label buffers below
     std::unique_ptr<TwoChannel> t2;  // set by bfg_ctx_attach_length
 
     bfg_ctx() = default;
     bfg_ctx(const bfg_ctx &) = delete;
     bfg_ctx &operator=(const bfg_ctx &) = delete;
-    // Drains the stream, then releases the events and the stream, also for a
-    // context that bfg_ctx_create abandons half-built; the device buffers
-    // free themselves afterwards (members are destroyed after this body).
     ~bfg_ctx() {
         (void)hipSetDevice(device);
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (auto &e : ev)
-            if (e) (void)hipEventDestroy(e);
-        for (auto &e : dom_ev) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
+        scratch_work.reset();
+        works.clear();  // each Work drains and releases its stream
     }
 };
 
@@ -1122,48 +1468,130 @@ namespace {
 
 uint32_t d1_of(const bfg_ctx *c) { return c->L.max_height + 1; }
 
-int ensure_lane_buffers(bfg_ctx *c, size_t n_lanes, size_t n_seeds) {
-    if (n_lanes > c->lanes_alloc) {
-        HIP_TRY(c->seed_off.alloc(n_lanes + 1));
-        HIP_TRY(c->lane_limit.alloc(n_lanes));
-        HIP_TRY(c->row_map.alloc(n_lanes));
-        c->lanes_alloc = n_lanes;
-    }
-    if (n_seeds > c->seeds_cap) {
-        HIP_TRY(c->seed_rank.alloc(n_seeds));
-        HIP_TRY(c->seed_pos.alloc(n_seeds));
-        HIP_TRY(c->seed_cost.alloc(n_seeds));
-        HIP_TRY(c->seed_depth.alloc(n_seeds));
-        HIP_TRY(c->seed_pre.alloc(n_seeds));
-        HIP_TRY(c->pathbuf.alloc(n_seeds * (size_t)d1_of(c)));
-        c->seeds_cap = n_seeds;
-    }
+// ---------------------------------------------------------------------------
+// Works and their pool
+// ---------------------------------------------------------------------------
+int work_init(Work *w) {
+    HIP_TRY(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+    for (auto &e : w->ev) HIP_TRY(hipEventCreate(&e));
     return BFG_SUCCESS;
+}
+
+// A Work of the pool held for one call; it goes back to the pool (its stream
+// drained) when the lease ends.
+struct WorkLease {
+    bfg_ctx *c = nullptr;
+    Work *w = nullptr;
+    WorkLease() = default;
+    WorkLease(const WorkLease &) = delete;
+    WorkLease &operator=(const WorkLease &) = delete;
+    ~WorkLease() {
+        if (!c || !w) return;
+        (void)hipStreamSynchronize(w->stream);
+        {
+            std::lock_guard<std::mutex> g(c->pool_mu);
+            c->idle.push_back(w);
+        }
+        c->pool_cv.notify_one();
+    }
+};
+
+int acquire_work(bfg_ctx *c, WorkLease &lease) {
+    std::unique_lock<std::mutex> g(c->pool_mu);
+    for (;;) {
+        if (!c->idle.empty()) {
+            lease.w = c->idle.back();  // the most recent one: its buffers are likely the right size
+            c->idle.pop_back();
+            lease.c = c;
+            return BFG_SUCCESS;
+        }
+        if (c->works.size() < c->max_works) {
+            std::unique_ptr<Work> w(new Work());
+            const int rc = work_init(w.get());
+            if (rc) return rc;
+            lease.w = w.get();
+            lease.c = c;
+            c->works.push_back(std::move(w));
+            return BFG_SUCCESS;
+        }
+        c->pool_cv.wait(g);
+    }
+}
+
+// Frees the label arenas of the idle Works (after an allocation failure).
+void trim_idle(bfg_ctx *c) {
+    std::lock_guard<std::mutex> g(c->pool_mu);
+    for (Work *w : c->idle) w->arena.release();
+}
+
+// A call in progress: the configuration held shared, the device current and
+// a Work leased (released in reverse order).
+struct CallScope {
+    std::shared_lock<std::shared_mutex> cfg;
+    WorkLease lease;
+    explicit CallScope(bfg_ctx *c) : cfg(c->cfg) {}
+};
+
+int begin_call(bfg_ctx *c, CallScope &cs) {
+    HIP_TRY(hipSetDevice(c->device));
+    return acquire_work(c, cs.lease);
 }
 
 // Labels per lane in the arena: all n positions, or for a matrix the top
 // region [0, n_top) followed by the nodes of its target paths (path_pos
 // element i at n_top + i).
-uint32_t label_stride(const bfg_ctx *c) {
-    return c->paths_on ? c->L.n_top + (uint32_t)c->path_nodes : c->L.n;
+uint32_t label_stride(const bfg_ctx *c, const Work *w) {
+    return w->paths_on ? c->L.n_top + (uint32_t)w->path_nodes : c->L.n;
 }
 
-int ensure_arena(bfg_ctx *c) {
-    const size_t need = (size_t)c->lane_cap * label_stride(c);
-    if (c->arena.n < need) {
-        c->arena.release();
-        HIP_TRY(c->arena.alloc(need));
-    }
+// A failed allocation of a Work over the scratch array is a request beyond
+// that array's limits.
+int alloc_fail(const Work *w, hipError_t e, const char *what) {
+    if (w->fixed)
+        return fail(BFG_NOT_SUPPORTED, std::string(what) + " exceeds the scratch array of the allocation-free call");
+    return fail(e == hipErrorOutOfMemory ? BFG_OUT_OF_MEMORY : BFG_DEVICE_ERROR,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define ENSURE(w, buf, count, what)                                      \
+    do {                                                                 \
+        hipError_t e_ = (w)->buf.ensure(count);                          \
+        if (e_ != hipSuccess) return alloc_fail((w), e_, what);          \
+    } while (0)
+
+int ensure_lane_buffers(bfg_ctx *c, Work *w, size_t n_lanes, size_t n_seeds) {
+    ENSURE(w, seed_off, n_lanes + 1, "seed offsets");
+    ENSURE(w, lane_limit, n_lanes, "lane limits");
+    ENSURE(w, row_map, n_lanes, "row map");
+    ENSURE(w, seed_rank, n_seeds, "seeds");
+    ENSURE(w, seed_pos, n_seeds, "seeds");
+    ENSURE(w, seed_cost, n_seeds, "seeds");
+    ENSURE(w, seed_depth, n_seeds, "seeds");
+    ENSURE(w, seed_pre, n_seeds, "seeds");
+    ENSURE(w, pathbuf, n_seeds * (size_t)d1_of(c), "path slots");
     return BFG_SUCCESS;
 }
 
-uint32_t auto_lane_cap(bfg_ctx *c, uint32_t want) {
+int ensure_arena(bfg_ctx *c, Work *w, size_t need) {
+    hipError_t e = w->arena.ensure(need);
+    if (e == hipErrorOutOfMemory && !w->fixed) {  // idle Works may hold the memory
+        trim_idle(c);
+        e = w->arena.ensure(need);
+    }
+    if (e != hipSuccess) return alloc_fail(w, e, "label arena");
+    return BFG_SUCCESS;
+}
+
+// Lanes per batch: 60 % of free HBM (plus this Work's own arena) over the
+// bytes one lane needs (labels, path slots, and `extra` per lane of outputs
+// the caller keeps on the device per batch), capped by the request; a
+// multiple of 64 (the top-region kernels carry 64 lanes per wavefront).
+uint32_t auto_lane_cap(bfg_ctx *c, Work *w, uint32_t want, size_t extra) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
-    const size_t have = free_b + c->arena.n * sizeof(uint32_t);
-    const size_t per_lane = (size_t)label_stride(c) * 4 + (size_t)d1_of(c) * 4 + 64;
+    const size_t have = free_b + w->arena.n * sizeof(uint32_t);
+    const size_t per_lane = (size_t)label_stride(c, w) * 4 + (size_t)d1_of(c) * 4 + 64 + extra;
     const size_t budget = (size_t)(have * 0.6);
-    // a multiple of 64: the top-region kernels carry 64 lanes per wavefront
     size_t cap = std::max<size_t>(budget / std::max<size_t>(per_lane, 1), 64);
     cap = std::min<size_t>(cap, 16384);
     cap = std::min<size_t>(cap, ((size_t)want + 63) / 64 * 64);
@@ -1176,6 +1604,7 @@ uint32_t auto_lane_cap(bfg_ctx *c, uint32_t want) {
 constexpr uint64_t PLAN_WAVES = 8192;
 constexpr uint32_t PLAN_MIN_ROWS = 64;
 constexpr uint32_t PLAN_MAX_ROWS = 256;
+constexpr size_t PLAN_CACHE = 8;  // DOWN plans kept per context
 
 uint32_t plan_max_rows() {
     if (const char *e = std::getenv("BFG_PLAN_MAX_ROWS")) {  // tuning override
@@ -1203,16 +1632,16 @@ uint32_t tri_scalar_segs() {
 // DOWN work plan for batches of `wgs` wave groups.  Per level, a row budget
 // R = rows of the level * wgs / PLAN_WAVES, clamped to [PLAN_MIN_ROWS,
 // PLAN_MAX_ROWS]: a segment with at most R rows is one whole unit; a longer
-// one is cut into units of R rows merged with atomics.
-int plan_down(bfg_ctx *c, uint32_t wgs) {
-    if (c->plan_wgs == wgs) return BFG_SUCCESS;
+// one is cut into units of R rows merged with atomics.  Uploads on stream s.
+int build_plan(bfg_ctx *c, hipStream_t s, uint32_t wgs, Plan &P) {
     const bfg::Layout &L = c->L;
     const uint32_t nsl = (uint32_t)L.seglvl_seg.size() - 1;
     const uint64_t rmax = c->plan_rows ? c->plan_rows : plan_max_rows();
     if (L.seg_pos.size() - 1 > UNIT_SEG_MASK) return fail(BFG_NOT_SUPPORTED, "too many segments for the DOWN plan");
     std::vector<uint32_t> us, u0, u1, uw;
     std::vector<uint8_t> split(L.seg_pos.size() - 1, 0);
-    c->plan_lvl.assign(1, 0);
+    P.wgs = wgs;
+    P.lvl.assign(1, 0);
     uint64_t covered = 0;
     for (uint32_t l = 0; l < nsl; ++l) {
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
@@ -1223,7 +1652,7 @@ int plan_down(bfg_ctx *c, uint32_t wgs) {
             const uint32_t a = L.xr_off[sg], b = L.xr_off[sg + 1];
             const uint32_t *run = &L.xr_run[8 * (size_t)sg];
             uint32_t nrun = 0;
-            for (uint32_t c = 0, st = 0; c < 8; st = run[c++]) nrun += run[c] > st;
+            for (uint32_t k = 0, st = 0; k < 8; st = run[k++]) nrun += run[k] > st;
             uint64_t w = L.xw_off[sg];
             if (nrun == 0) {  // no rows: a whole unit still gives unreached wave groups their INF labels
                 us.push_back(sg | 7u << UNIT_RUN_SHIFT | UNIT_WHOLE);
@@ -1231,44 +1660,70 @@ int plan_down(bfg_ctx *c, uint32_t wgs) {
                 u1.push_back(a);
                 uw.push_back((uint32_t)w);
             }
-            for (uint32_t c = 0, st = 0; c < 8; st = run[c++]) {
-                const uint32_t W = 8 * (8 - c), cs = sg | c << UNIT_RUN_SHIFT;
-                if (run[c] == st) continue;  // an empty run (the increment still advances st)
-                if (nrun == 1 && run[c] - st <= R) {  // the whole segment in one unit
+            for (uint32_t k = 0, st = 0; k < 8; st = run[k++]) {
+                const uint32_t W = 8 * (8 - k), cs = sg | k << UNIT_RUN_SHIFT;
+                if (run[k] == st) continue;  // an empty run (the increment still advances st)
+                if (nrun == 1 && run[k] - st <= R) {  // the whole segment in one unit
                     us.push_back(cs | UNIT_WHOLE);
                     u0.push_back(a + st);
-                    u1.push_back(a + run[c]);
+                    u1.push_back(a + run[k]);
                     uw.push_back((uint32_t)w);
                 } else {
-                    for (uint64_t r = st; r < run[c]; r += R) {
+                    for (uint64_t r = st; r < run[k]; r += R) {
                         us.push_back(cs);
                         u0.push_back((uint32_t)(a + r));
-                        u1.push_back((uint32_t)(a + std::min<uint64_t>(r + R, run[c])));
+                        u1.push_back((uint32_t)(a + std::min<uint64_t>(r + R, run[k])));
                         uw.push_back((uint32_t)(w + (r - st) * W));
                     }
                     split[sg] = 1;
                 }
-                w += (uint64_t)(run[c] - st) * W;
+                w += (uint64_t)(run[k] - st) * W;
             }
             if (w != L.xw_off[sg + 1] || a + run[7] != b) return fail(BFG_INTERNAL_ERROR, "row runs disagree");
         }
-        c->plan_lvl.push_back((uint32_t)us.size());
+        P.lvl.push_back((uint32_t)us.size());
     }
     for (size_t k = 0; k < us.size(); ++k) covered += u1[k] - u0[k];
     if (covered != L.xr_off.back()) return fail(BFG_INTERNAL_ERROR, "DOWN plan does not cover every row");
-    hipStream_t s = c->stream;
     const StreamDrain drain{s};
-    HIP_TRY(c->du_seg.upload(us, s));
-    HIP_TRY(c->du_r0.upload(u0, s));
-    HIP_TRY(c->du_r1.upload(u1, s));
-    HIP_TRY(c->du_w.upload(uw, s));
-    HIP_TRY(c->seg_split.upload(split, s));
+    HIP_TRY(P.seg.upload(us, s));
+    HIP_TRY(P.r0.upload(u0, s));
+    HIP_TRY(P.r1.upload(u1, s));
+    HIP_TRY(P.w.upload(uw, s));
+    HIP_TRY(P.split.upload(split, s));
     HIP_TRY(hipStreamSynchronize(s));  // the host vectors die with this frame
-    c->plan_wgs = wgs;
     return BFG_SUCCESS;
 }
 
-int stage_path_elements(bfg_ctx *c);
+// The DOWN plan for `wgs` wave groups: cached per context (a few recent
+// ones), built on first use.  A Work over the scratch array takes the
+// one-wave-group plan built at creation and builds nothing.
+int get_plan(bfg_ctx *c, Work *w, uint32_t wgs, std::shared_ptr<const Plan> &out) {
+    if (w->fixed) {
+        if (wgs != 1 || !c->plan1) return fail(BFG_INTERNAL_ERROR, "no plan for the allocation-free call");
+        out = c->plan1;
+        return BFG_SUCCESS;
+    }
+    std::lock_guard<std::mutex> g(c->plan_mu);
+    auto it = c->plans.find(wgs);
+    if (it == c->plans.end()) {
+        std::shared_ptr<Plan> P(new Plan());
+        const int rc = build_plan(c, w->stream, wgs, *P);
+        if (rc) return rc;
+        it = c->plans.emplace(wgs, std::move(P)).first;
+    }
+    out = it->second;
+    auto &lru = c->plan_lru;
+    lru.erase(std::remove(lru.begin(), lru.end(), wgs), lru.end());
+    lru.push_back(wgs);
+    while (lru.size() > PLAN_CACHE) {  // calls in flight keep theirs alive
+        c->plans.erase(lru.front());
+        lru.erase(lru.begin());
+    }
+    return BFG_SUCCESS;
+}
+
+int stage_path_elements(bfg_ctx *c, Work *w);
 
 struct PhaseTimes {
     float up_ms = 0, sweep_ms = 0, dom_ms = 0;
@@ -1276,89 +1731,121 @@ struct PhaseTimes {
     uint64_t dom_bytes = 0;
 };
 
+Gate gate_of(const bfg_ctx *c, const Work *w, uint32_t nl) {
+    return Gate{w->gate ? c->xr_min.p : nullptr, w->lane_limit.p, nl, w->seg_live.p,
+                (uint32_t)c->L.seg_pos.size() - 1};
+}
+
+BGate bgate_of(const bfg_ctx *c, const Work *w) {
+    return BGate{w->gate ? w->open_b.p : nullptr, c->pos_bt.p, c->n_bt, c->L.n_top};
+}
+
+CGate cgate_of(const bfg_ctx *c, const Work *w) {
+    return CGate{w->gate ? c->cs_min.p : nullptr, w->lane_limit.p, w->chunk_ok.p, (uint32_t)c->L.ch_off.size() - 1};
+}
+
 template <int KL>
-int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
+int launch_phases(bfg_ctx *c, Work *w, const Plan &P, uint32_t nl, PhaseTimes *pt) {
     const bfg::Layout &L = c->L;
-    hipStream_t s = c->stream;
+    hipStream_t s = w->stream;
     const uint32_t groups = (nl + KL - 1) / KL;
     const uint32_t wgs = (nl + 63) / 64;            // wave groups of the top-region kernels
-    const uint32_t n = label_stride(c), D1 = d1_of(c);  // n: labels per lane in the arena
+    const uint32_t n = label_stride(c, w), D1 = d1_of(c);  // n: labels per lane in the arena
     const uint32_t nsl = (uint32_t)L.seglvl_seg.size() - 1;
     const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
     uint32_t nlaunch = 0;
-    int rc = plan_down(c, wgs);
-    if (rc) return rc;
-    HIP_TRY(hipEventRecord(c->ev[0], s));
-    const Reach R{c->seed_off.p, c->seed_pre.p, c->pos_pre.p, c->pos_sub.p, nl};
+    HIP_TRY(hipEventRecord(w->ev[0], s));
+    const Reach R{w->seed_off.p, w->seed_pre.p, c->pos_pre.p, c->pos_sub.p, nl};
+    const Gate G = gate_of(c, w, nl), G0{nullptr, nullptr, 0, nullptr, 0};
+    const CGate CG = cgate_of(c, w);
+    const BGate B = bgate_of(c, w);
     constexpr uint32_t WPB = TILE_THREADS / 64;  // wave groups per workgroup
     const uint32_t wgx = (wgs + WPB - 1) / WPB;  // workgroups across the wave groups
     const uint32_t tss = c->tri_segs ? c->tri_segs - 1 : tri_scalar_segs();
     if (nseg) {
         hipLaunchKernelGGL(fill_reach_kernel<KL>, dim3(wgs, (nseg + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, nseg,
-                           c->seg_pos.p, R, c->arena.p, n);
+                           c->seg_pos.p, R, w->arena.p, n);
         ++nlaunch;
     }
-    hipLaunchKernelGGL(bottom_walk_kernel<KL>, dim3(nl), dim3(64), 0, s, c->seed_off.p, c->seed_rank.p,
-                       c->seed_pos.p, c->seed_cost.p, c->seed_depth.p, c->lane_limit.p, c->parent.p,
-                       c->pos_parent.p, c->pos_depth.p, c->up_off.p, c->up_tgt.p, c->up_w.p, L.n_top, c->pathbuf.p,
-                       D1, c->arena.p, n);
+    hipLaunchKernelGGL(bottom_walk_kernel<KL>, dim3(nl), dim3(64), 0, s, w->seed_off.p, w->seed_rank.p,
+                       w->seed_pos.p, w->seed_cost.p, w->seed_depth.p, w->lane_limit.p, c->parent.p,
+                       c->pos_parent.p, c->pos_depth.p, c->up_off.p, c->up_tgt.p, c->up_w.p, L.n_top, w->pathbuf.p,
+                       D1, w->arena.p, n);
+    ++nlaunch;
     uint32_t ndom = 0;  // DOWN row launches of this batch
     for (int l = (int)nsl - 1; l >= 0; --l) {
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
         for (uint32_t b = s0; b < s1; b += MAX_GRID_Y, ++nlaunch)
-            hipLaunchKernelGGL((tri_kernel<KL, false, false>), dim3(wgx, std::min(s1 - b, MAX_GRID_Y)),
+            hipLaunchKernelGGL((tri_kernel<KL, false, false, false>), dim3(wgx, std::min(s1 - b, MAX_GRID_Y)),
                                dim3(TILE_THREADS), 0, s, b, std::min(s1 - b, MAX_GRID_Y), c->seg_pos.p, c->tb_up.p,
-                               c->tb_w.p, c->seg_split.p, R, c->arena.p, n, wgs);
+                               c->tb_w.p, P.split.p, R, G0, w->arena.p, n, wgs);
         const uint32_t k0 = L.pt_blk[s0], k1 = L.pt_blk[s1];
         for (uint32_t b = k0; b < k1; b += MAX_GRID_Y, ++nlaunch)
-            hipLaunchKernelGGL((rows_kernel<KL, true>), dim3(wgx, std::min(k1 - b, MAX_GRID_Y)), dim3(TILE_THREADS),
+            hipLaunchKernelGGL((rows_kernel<KL, true, false>), dim3(wgx, std::min(k1 - b, MAX_GRID_Y)), dim3(TILE_THREADS),
                                0, s, b, std::min(k1, b + MAX_GRID_Y), c->pk_seg.p, c->pr_off.p, c->pr_off.p,
                                c->pr_wid.p, c->pw_off.p, c->seg_pos.p, c->pt_tgt.p, c->pr_x.p, c->pr_w.p,
-                               R, c->arena.p, n, wgs);
+                               R, G0, w->arena.p, n, wgs);
     }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[1], s));
+    HIP_TRY(hipEventRecord(w->ev[1], s));
     for (uint32_t l = 0; l < nsl; ++l) {
-        const uint32_t u0 = c->plan_lvl[l], u1 = c->plan_lvl[l + 1];
+        const uint32_t u0 = P.lvl[l], u1 = P.lvl[l + 1];
         if (u1 > u0) {
-            if (c->dom_ev.size() < 2 * (size_t)(ndom + 1)) {
+            const bool timed = w->time_dom;
+            if (timed && w->dom_ev.size() < 2 * (size_t)(ndom + 1)) {
                 for (int k = 0; k < 2; ++k) {
                     hipEvent_t e;
                     HIP_TRY(hipEventCreate(&e));
-                    c->dom_ev.push_back(e);
+                    w->dom_ev.push_back(e);
                 }
             }
-            HIP_TRY(hipEventRecord(c->dom_ev[2 * ndom], s));
-            for (uint32_t b = u0; b < u1; b += MAX_GRID_Y, ++nlaunch)
-                hipLaunchKernelGGL((rows_kernel<KL, false>), dim3(wgx, std::min(u1 - b, MAX_GRID_Y)),
-                                   dim3(TILE_THREADS), 0, s, b, std::min(u1, b + MAX_GRID_Y), c->du_seg.p, c->du_r0.p,
-                                   c->du_r1.p, nullptr, c->du_w.p, c->seg_pos.p, c->pt_tgt.p,
-                                   c->xr_src.p, c->xr_w.p, R, c->arena.p, n, wgs);
-            HIP_TRY(hipEventRecord(c->dom_ev[2 * ndom + 1], s));
-            ++ndom;
+            if (timed) HIP_TRY(hipEventRecord(w->dom_ev[2 * ndom], s));
+            for (uint32_t b = u0; b < u1; b += MAX_GRID_Y, ++nlaunch) {
+                const dim3 grid(wgx, std::min(u1 - b, MAX_GRID_Y));
+                if (w->gate)
+                    hipLaunchKernelGGL((rows_kernel<KL, false, true>), grid, dim3(TILE_THREADS), 0, s, b,
+                                       std::min(u1, b + MAX_GRID_Y), P.seg.p, P.r0.p, P.r1.p, nullptr, P.w.p,
+                                       c->seg_pos.p, c->pt_tgt.p, c->xr_src.p, c->xr_w.p, R, G, w->arena.p, n, wgs);
+                else
+                    hipLaunchKernelGGL((rows_kernel<KL, false, false>), grid, dim3(TILE_THREADS), 0, s, b,
+                                       std::min(u1, b + MAX_GRID_Y), P.seg.p, P.r0.p, P.r1.p, nullptr, P.w.p,
+                                       c->seg_pos.p, c->pt_tgt.p, c->xr_src.p, c->xr_w.p, R, G, w->arena.p, n, wgs);
+            }
+            if (timed) {
+                HIP_TRY(hipEventRecord(w->dom_ev[2 * ndom + 1], s));
+                ++ndom;
+            }
             pt->dom_bytes += (uint64_t)wgs * c->lvl_dom_bytes[l];
         }
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
-        for (uint32_t b = s0; b < s1; b += MAX_GRID_Y, ++nlaunch)
-            if (s1 - s0 > tss)
-                hipLaunchKernelGGL((tri_kernel<KL, true, true>), dim3(wgx, std::min(s1 - b, MAX_GRID_Y)), dim3(TILE_THREADS), 0,
-                               s, b, std::min(s1 - b, MAX_GRID_Y), c->seg_pos.p, c->tb_dn.p, c->tb_w.p,
-                               c->seg_split.p, R, c->arena.p, n, wgs);
+        for (uint32_t b = s0; b < s1; b += MAX_GRID_Y, ++nlaunch) {
+            const dim3 grid(wgx, std::min(s1 - b, MAX_GRID_Y));
+            const uint32_t cnt = std::min(s1 - b, MAX_GRID_Y);
+            const bool sc = s1 - s0 > tss;
+            if (w->gate && sc)
+                hipLaunchKernelGGL((tri_kernel<KL, true, true, true>), grid, dim3(TILE_THREADS), 0, s, b, cnt,
+                                   c->seg_pos.p, c->tb_dn.p, c->tb_w.p, P.split.p, R, G, w->arena.p, n, wgs);
+            else if (w->gate)
+                hipLaunchKernelGGL((tri_kernel<KL, true, false, true>), grid, dim3(TILE_THREADS), 0, s, b, cnt,
+                                   c->seg_pos.p, c->tb_dn.p, c->tb_w.p, P.split.p, R, G, w->arena.p, n, wgs);
+            else if (sc)
+                hipLaunchKernelGGL((tri_kernel<KL, true, true, false>), grid, dim3(TILE_THREADS), 0, s, b, cnt,
+                                   c->seg_pos.p, c->tb_dn.p, c->tb_w.p, P.split.p, R, G, w->arena.p, n, wgs);
             else
-                hipLaunchKernelGGL((tri_kernel<KL, true, false>), dim3(wgx, std::min(s1 - b, MAX_GRID_Y)), dim3(TILE_THREADS), 0,
-                               s, b, std::min(s1 - b, MAX_GRID_Y), c->seg_pos.p, c->tb_dn.p, c->tb_w.p,
-                               c->seg_split.p, R, c->arena.p, n, wgs);
+                hipLaunchKernelGGL((tri_kernel<KL, true, false, false>), grid, dim3(TILE_THREADS), 0, s, b, cnt,
+                                   c->seg_pos.p, c->tb_dn.p, c->tb_w.p, P.split.p, R, G, w->arena.p, n, wgs);
+        }
     }
-    if (c->paths_on) {
+    if (w->paths_on) {
         constexpr int NSP = NT / KL;
-        if (c->n_paths) {
-            rc = stage_path_elements(c);
+        if (w->n_paths) {
+            const int rc = stage_path_elements(c, w);
             if (rc) return rc;
             const uint32_t gpx = (groups + XCDS - 1) / XCDS;
-            hipLaunchKernelGGL(bottom_path_kernel<KL>, dim3((c->n_paths + NSP - 1) / NSP * gpx * XCDS), dim3(NT), 0, s,
-                               c->n_paths, c->path_off.p, c->path_pos.p, L.n_top, c->in_off.p, c->in_split.p,
+            hipLaunchKernelGGL(bottom_path_kernel<KL>, dim3((w->n_paths + NSP - 1) / NSP * gpx * XCDS), dim3(NT), 0,
+                               s, w->n_paths, w->path_off.p, w->path_pos.p, L.n_top, c->in_off.p, c->in_split.p,
                                c->in_dd.p, c->in_src.p, c->in_w.p, c->pos_pre.p, c->pos_sub.p, c->pos_depth.p,
-                               c->seed_off.p, c->seed_pre.p, c->seed_depth.p, c->pathbuf.p, D1, nl, c->arena.p, n);
+                               w->seed_off.p, w->seed_pre.p, w->seed_depth.p, w->pathbuf.p, D1, nl, w->arena.p, n);
             ++nlaunch;
         }
     } else {
@@ -1366,103 +1853,114 @@ int launch_phases(bfg_ctx *c, uint32_t nl, PhaseTimes *pt) {
         if (nch) {
             hipLaunchKernelGGL((bottom_chunk_kernel<KL, CHUNK, NTB>), dim3(nch, groups), dim3(NTB), 0, s,
                                c->ch_off.p, c->ch_pos.p, c->cs_off.p, c->cs_src.p, c->cs_aoff.p, c->ca_v.p, c->ca_w.p,
-                               c->pos_pre.p, c->pos_sub.p, c->pos_depth.p, c->seed_off.p, c->seed_pre.p,
-                               c->seed_depth.p, c->pathbuf.p, D1, nl, c->arena.p, n);
+                               c->pos_pre.p, c->pos_sub.p, c->pos_depth.p, w->seed_off.p, w->seed_pre.p,
+                               w->seed_depth.p, w->pathbuf.p, D1, nl, CG, w->arena.p, n);
             ++nlaunch;
+            if (w->gate && c->n_bt) {  // tree flags from the chunk flags
+                hipLaunchKernelGGL(bottom_gate_kernel, dim3((c->n_bt + NT - 1) / NT, groups), dim3(NT), 0, s, c->n_bt,
+                                   c->bt_ch.p, w->chunk_ok.p, nch, w->open_b.p);
+                ++nlaunch;
+            }
         }
         constexpr int NSP = NT / KL;
         for (size_t b = 0; b + 1 < L.botlvl_pos.size(); ++b) {
             const uint32_t p0 = L.botlvl_pos[b], p1 = L.botlvl_pos[b + 1];
             hipLaunchKernelGGL(bottom_pull_kernel<KL>, dim3((p1 - p0 + NSP - 1) / NSP, groups), dim3(NT), 0, s, p0,
-                               p1, c->in_off.p, c->in_split.p, c->in_src.p, c->in_w.p, c->arena.p, n);
+                               p1, c->in_off.p, c->in_split.p, c->in_src.p, c->in_w.p, B, w->arena.p, n);
             ++nlaunch;
         }
     }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[2], s));
-    HIP_TRY(hipEventSynchronize(c->ev[2]));
+    HIP_TRY(hipEventRecord(w->ev[2], s));
+    HIP_TRY(hipEventSynchronize(w->ev[2]));
     float a = 0, b = 0;
-    HIP_TRY(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-    HIP_TRY(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&a, w->ev[0], w->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&b, w->ev[1], w->ev[2]));
     pt->up_ms += a;
     pt->sweep_ms += b;
     pt->launches += nlaunch;
     for (uint32_t k = 0; k < ndom; ++k) {
         float d = 0;
-        HIP_TRY(hipEventElapsedTime(&d, c->dom_ev[2 * k], c->dom_ev[2 * k + 1]));
+        HIP_TRY(hipEventElapsedTime(&d, w->dom_ev[2 * k], w->dom_ev[2 * k + 1]));
         pt->dom_ms += d;
     }
     pt->dom_launches += ndom;
     return BFG_SUCCESS;
 }
 
-// One batch: lanes [0, nl) in sorted order, staged host seed lists; runs the
-// upward and downward phases; the output step is done by the caller.
-int run_batch(bfg_ctx *c, uint32_t nl, const std::vector<uint32_t> &soff, const std::vector<uint32_t> &srank,
-              const std::vector<uint32_t> &scost, const std::vector<uint32_t> &limits,
-              const std::vector<uint32_t> &rows, PhaseTimes *pt) {
-    const size_t ns = srank.size();
-    int rc = ensure_lane_buffers(c, nl, ns);
+// One batch: lanes [0, nl) in sorted order, staged in the Work's host seed
+// lists (h_soff, h_srank, h_scost, h_lim, h_rows); runs the upward and
+// downward phases; the output step is done by the caller.
+int run_batch(bfg_ctx *c, Work *w, const Plan &P, uint32_t nl, PhaseTimes *pt) {
+    const size_t ns = w->h_srank.size();
+    int rc = ensure_lane_buffers(c, w, nl, ns);
     if (rc) return rc;
-    std::vector<uint32_t> sdepth(ns), spre(ns), spos(ns);
+    w->h_sdepth.resize(ns);
+    w->h_spre.resize(ns);
+    w->h_spos.resize(ns);
     for (size_t j = 0; j < ns; ++j) {
-        sdepth[j] = c->L.depth[srank[j]];
-        spre[j] = c->L.pre[srank[j]];
-        spos[j] = c->L.pos_of_rank[srank[j]];
+        const uint32_t r = w->h_srank[j];
+        w->h_sdepth[j] = c->L.depth[r];
+        w->h_spre[j] = c->L.pre[r];
+        w->h_spos[j] = c->L.pos_of_rank[r];
     }
-    hipStream_t s = c->stream;
+    hipStream_t s = w->stream;
     const StreamDrain drain{s};  // a no-op once launch_phases has synchronised
-    HIP_TRY(hipMemcpyAsync(c->seed_off.p, soff.data(), (nl + 1) * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->lane_limit.p, limits.data(), nl * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->row_map.p, rows.data(), nl * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(w->seed_off.p, w->h_soff.data(), (nl + 1) * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(w->lane_limit.p, w->h_lim.data(), nl * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(w->row_map.p, w->h_rows.data(), nl * 4, hipMemcpyHostToDevice, s));
     if (ns) {
-        HIP_TRY(hipMemcpyAsync(c->seed_rank.p, srank.data(), ns * 4, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(c->seed_pos.p, spos.data(), ns * 4, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(c->seed_cost.p, scost.data(), ns * 4, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(c->seed_depth.p, sdepth.data(), ns * 4, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(c->seed_pre.p, spre.data(), ns * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(w->seed_rank.p, w->h_srank.data(), ns * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(w->seed_pos.p, w->h_spos.data(), ns * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(w->seed_cost.p, w->h_scost.data(), ns * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(w->seed_depth.p, w->h_sdepth.data(), ns * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(w->seed_pre.p, w->h_spre.data(), ns * 4, hipMemcpyHostToDevice, s));
     }
     switch (c->lanes) {
-        case 16: rc = launch_phases<16>(c, nl, pt); break;
-        case 64: rc = launch_phases<64>(c, nl, pt); break;
-        default: rc = launch_phases<32>(c, nl, pt); break;
+        case 16: rc = launch_phases<16>(c, w, P, nl, pt); break;
+        case 64: rc = launch_phases<64>(c, w, P, nl, pt); break;
+        default: rc = launch_phases<32>(c, w, P, nl, pt); break;
     }
-    return rc;  // launch_phases synchronised the stream: the staged vectors may go
+    return rc;  // launch_phases synchronised the stream: the staged vectors may change
 }
 
-int gather_out(bfg_ctx *c, uint32_t nl, const uint32_t *d_tpos, uint32_t T, const uint32_t *d_rows, uint32_t *d_out,
-               uint64_t ld, unsigned long long *d_bits, uint64_t bits_ld, float *out_ms) {
-    hipStream_t s = c->stream;
-    const uint32_t groups = (nl + c->lanes - 1) / c->lanes;
+// Gathers targets d_tpos[0 .. T) of a batch into rows (d_out, ld) and/or reach
+// bits (d_bits, bits_ld), rows through d_rows (nullptr: sorted lane order).
+// With a gate and dense labels, d_tpos holds positions and the bottom gate
+// applies.
+int gather_out(bfg_ctx *c, Work *w, uint32_t nl, const uint32_t *d_tpos, uint32_t T, const uint32_t *d_rows,
+               uint32_t *d_out, uint64_t ld, unsigned long long *d_bits, uint64_t bits_ld, float *out_ms,
+               uint32_t lane_lo = 0, uint32_t row_base = 0) {
+    hipStream_t s = w->stream;
+    // the lane groups holding lanes [lane_lo, nl)
+    const uint32_t g0 = lane_lo / c->lanes, groups = (nl + c->lanes - 1) / c->lanes - g0;
     dim3 grid((T + 63) / 64, groups);
-    HIP_TRY(hipEventRecord(c->ev[0], s));
+    BGate B = bgate_of(c, w);
+    if (w->paths_on) B.open_b = nullptr;  // compact labels: dead paths hold INF
+    HIP_TRY(hipEventRecord(w->ev[0], s));
+    const uint32_t n = label_stride(c, w);
     switch (c->lanes) {
         case 16:
-            hipLaunchKernelGGL(gather_kernel<16>, grid, dim3(NT), 0, s, c->arena.p, label_stride(c), d_tpos, T,
-                               c->lane_limit.p, nl, d_rows, d_out, ld, d_bits, bits_ld);
+            hipLaunchKernelGGL(gather_kernel<16>, grid, dim3(NT), 0, s, w->arena.p, n, d_tpos, T, w->lane_limit.p,
+                               nl, d_rows, B, d_out, ld, d_bits, bits_ld, g0, lane_lo, row_base);
             break;
         case 64:
-            hipLaunchKernelGGL(gather_kernel<64>, grid, dim3(NT), 0, s, c->arena.p, label_stride(c), d_tpos, T,
-                               c->lane_limit.p, nl, d_rows, d_out, ld, d_bits, bits_ld);
+            hipLaunchKernelGGL(gather_kernel<64>, grid, dim3(NT), 0, s, w->arena.p, n, d_tpos, T, w->lane_limit.p,
+                               nl, d_rows, B, d_out, ld, d_bits, bits_ld, g0, lane_lo, row_base);
             break;
         default:
-            hipLaunchKernelGGL(gather_kernel<32>, grid, dim3(NT), 0, s, c->arena.p, label_stride(c), d_tpos, T,
-                               c->lane_limit.p, nl, d_rows, d_out, ld, d_bits, bits_ld);
+            hipLaunchKernelGGL(gather_kernel<32>, grid, dim3(NT), 0, s, w->arena.p, n, d_tpos, T, w->lane_limit.p,
+                               nl, d_rows, B, d_out, ld, d_bits, bits_ld, g0, lane_lo, row_base);
             break;
     }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[1], s));
-    HIP_TRY(hipEventSynchronize(c->ev[1]));
+    HIP_TRY(hipEventRecord(w->ev[1], s));
+    HIP_TRY(hipEventSynchronize(w->ev[1]));
     float a = 0;
-    HIP_TRY(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&a, w->ev[0], w->ev[1]));
     *out_ms += a;
     return BFG_SUCCESS;
 }
-
-struct Req {
-    // per lane seeds and limit (host), original lane order
-    std::vector<uint32_t> soff, srank, scost, limit;
-};
 
 int check_ranks(const bfg_ctx *c, const uint32_t *r, size_t k, const char *what) {
     for (size_t i = 0; i < k; ++i)
@@ -1470,42 +1968,91 @@ int check_ranks(const bfg_ctx *c, const uint32_t *r, size_t k, const char *what)
     return BFG_SUCCESS;
 }
 
-// Runs every lane of `req`: lanes are ordered by the pre-order of their first
-// seed and cut into batches of lane_cap; after each batch,
-// out(batch_first, batch_lanes, rows, &ms) writes the outputs, where rows[i]
-// is the original index of sorted lane batch_first + i.
+// Restricts the bottom sweep of one call to the staged target paths (compact
+// labels); always switched off again.
+struct PathsOn {
+    Work *w;
+    explicit PathsOn(Work *ww) : w(ww) { w->paths_on = true; }
+    ~PathsOn() { w->paths_on = false; }
+};
+
+// Runs every lane of `req` on Work w: lanes are ordered by the pre-order of
+// their first seed and cut into batches of the lane capacity; after each
+// batch, out(batch_first, batch_lanes, rows, &ms) writes the outputs, where
+// rows[i] is the original index of sorted lane batch_first + i.  `extra` is
+// the device bytes per lane the caller's outputs take per batch.  A lane
+// limit below INF turns the block gate on.
 template <class F>
-int run_all(bfg_ctx *c, const Req &req, F &&out) {
+int run_all(bfg_ctx *c, Work *w, const Req &req, size_t extra, F &&out) {
     const uint32_t L = (uint32_t)req.limit.size();
-    if (!c->cap_fixed) c->lane_cap = auto_lane_cap(c, L);
-    int rc = ensure_arena(c);
+    w->gate = false;
+    for (uint32_t i = 0; i < L; ++i) w->gate |= req.limit[i] != INF;
+    if (w->fixed) w->lane_cap = 64;
+    else w->lane_cap = c->cap_fixed ? c->lane_cap : auto_lane_cap(c, w, L, extra);
+    int rc = ensure_arena(c, w, (size_t)w->lane_cap * label_stride(c, w));
     if (rc) return rc;
-    std::vector<uint32_t> order(L);
-    std::iota(order.begin(), order.end(), 0);
-    std::vector<uint32_t> key(L, 0);
-    for (uint32_t i = 0; i < L; ++i)
-        if (req.soff[i + 1] > req.soff[i]) key[i] = c->L.pre[req.srank[req.soff[i]]];
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+    if (w->gate) {
+        const size_t wgs = w->lane_cap / 64, groups = (w->lane_cap + c->lanes - 1) / c->lanes;
+        ENSURE(w, seg_live, wgs * std::max<size_t>(c->L.seg_pos.size() - 1, 1), "gate flags");
+        ENSURE(w, chunk_ok, groups * std::max<size_t>(c->L.ch_off.size() - 1, 1), "gate flags");
+        ENSURE(w, open_b, groups * std::max<uint32_t>(c->n_bt, 1), "gate flags");
+    }
+    // sort key: pre-order of the first seed, then the lane index
+    auto &key = w->h_key;
+    key.resize(L);
+    for (uint32_t i = 0; i < L; ++i) {
+        const uint64_t k = req.soff[i + 1] > req.soff[i] ? c->L.pre[req.srank[req.soff[i]]] : 0u;
+        key[i] = k << 32 | i;
+    }
+    if (req.major.empty()) {
+        std::sort(key.begin(), key.end());
+    } else {  // each request's lanes stay together, in the order of their requests
+        const std::vector<uint32_t> &mj = req.major;
+        std::sort(key.begin(), key.end(), [&](uint64_t a, uint64_t b) {
+            const uint32_t ma = mj[(uint32_t)a], mb = mj[(uint32_t)b];
+            return ma != mb ? ma < mb : a < b;
+        });
+    }
     auto t_start = std::chrono::steady_clock::now();
     bfg_timings tm{};
     PhaseTimes pt;
-    for (uint32_t b0 = 0; b0 < L; b0 += c->lane_cap) {
-        const uint32_t nl = std::min(c->lane_cap, L - b0);
-        std::vector<uint32_t> soff(nl + 1), srank, scost, lim(nl), rows(nl);
+    for (uint32_t b0 = 0; b0 < L; b0 += w->lane_cap) {
+        const uint32_t nl = std::min(w->lane_cap, L - b0);
+        w->h_soff.resize(nl + 1);
+        w->h_lim.resize(nl);
+        w->h_rows.resize(nl);
+        w->h_srank.clear();
+        w->h_scost.clear();
         for (uint32_t i = 0; i < nl; ++i) {
-            const uint32_t o = order[b0 + i];
-            rows[i] = o;
-            soff[i] = (uint32_t)srank.size();
+            const uint32_t o = (uint32_t)key[b0 + i];
+            w->h_rows[i] = o;
+            w->h_soff[i] = (uint32_t)w->h_srank.size();
             for (uint32_t k = req.soff[o]; k < req.soff[o + 1]; ++k) {
-                srank.push_back(req.srank[k]);
-                scost.push_back(req.scost[k]);
+                w->h_srank.push_back(req.srank[k]);
+                w->h_scost.push_back(req.scost[k]);
             }
-            lim[i] = req.limit[o];
+            w->h_lim[i] = req.limit[o];
         }
-        soff[nl] = (uint32_t)srank.size();
-        rc = run_batch(c, nl, soff, srank, scost, lim, rows, &pt);
+        w->h_soff[nl] = (uint32_t)w->h_srank.size();
+        std::shared_ptr<const Plan> P;
+        rc = get_plan(c, w, (nl + 63) / 64, P);
         if (rc) return rc;
-        rc = out(b0, nl, rows, &tm.output_ms);
+        rc = run_batch(c, w, *P, nl, &pt);
+        if (rc) return rc;
+        if (w->gate && !w->fixed) {  // how much the gate left open (flags of this batch)
+            const size_t wgs = (nl + 63) / 64, groups = (nl + c->lanes - 1) / c->lanes;
+            const size_t nseg = c->L.seg_pos.size() - 1;
+            std::vector<uint8_t> f(std::max(wgs * nseg, groups * (size_t)c->n_bt));
+            HIP_TRY(hipMemcpy(f.data(), w->seg_live.p, wgs * nseg, hipMemcpyDeviceToHost));
+            tm.gate_seg_units += wgs * nseg;
+            for (size_t k = 0; k < wgs * nseg; ++k) tm.gate_seg_open += f[k];
+            if (c->n_bt && !w->paths_on) {
+                HIP_TRY(hipMemcpy(f.data(), w->open_b.p, groups * c->n_bt, hipMemcpyDeviceToHost));
+                tm.gate_bt_units += groups * c->n_bt;
+                for (size_t k = 0; k < groups * c->n_bt; ++k) tm.gate_bt_open += f[k];
+            }
+        }
+        rc = out(b0, nl, w->h_rows, &tm.output_ms);
         if (rc) return rc;
         tm.batches++;
     }
@@ -1516,21 +2063,21 @@ int run_all(bfg_ctx *c, const Req &req, F &&out) {
     tm.dom_launches = pt.dom_launches;
     tm.dom_bytes = pt.dom_bytes;
     tm.lanes_run = L;
-    tm.nodes_swept = c->paths_on ? c->L.n_top + c->path_nodes : c->L.n;
-    tm.arcs_swept = c->paths_on ? c->top_in_arcs + c->path_arcs : c->L.m_down;
+    tm.nodes_swept = w->paths_on ? c->L.n_top + w->path_nodes : c->L.n;
+    tm.arcs_swept = w->paths_on ? c->top_in_arcs + w->path_arcs : c->L.m_down;
     tm.total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    w->last = tm;
+    std::lock_guard<std::mutex> g(c->last_mu);
     c->last = tm;
     return BFG_SUCCESS;
 }
 
-Req single_source_req(const uint32_t *src, uint32_t k, uint32_t threshold) {
-    Req r;
+void single_source_req(Req &r, const uint32_t *src, uint32_t k, uint32_t threshold) {
     r.soff.resize(k + 1);
     r.srank.assign(src, src + k);
     r.scost.assign(k, 0);
     r.limit.assign(k, threshold);
     for (uint32_t i = 0; i <= k; ++i) r.soff[i] = i;
-    return r;
 }
 
 // Bottom paths of a matrix's targets: for every bottom target that is not an
@@ -1540,19 +2087,22 @@ Req single_source_req(const uint32_t *src, uint32_t k, uint32_t threshold) {
 // top targets by position, bottom targets by their element on a path
 // (compact labels, see label_stride).  The elements themselves are written
 // by stage_path_elements once the top-region kernels are queued.
-int stage_target_paths(bfg_ctx *c, const uint32_t *targets, uint32_t T) {
+int stage_target_paths(bfg_ctx *c, Work *w, const uint32_t *targets, uint32_t T) {
     const bfg::Layout &L = c->L;
-    std::vector<uint32_t> bt;
+    std::vector<uint32_t> &bt = w->h_bt, &pending = w->h_pending;
+    bt.clear();
+    pending.clear();
     for (uint32_t t = 0; t < T; ++t) {
         const uint32_t p = L.pos_of_rank[targets[t]];
         if (p >= L.n_top) bt.push_back(p);
     }
     std::sort(bt.begin(), bt.end(), [&](uint32_t a, uint32_t b) { return L.pos_pre[a] < L.pos_pre[b]; });
     bt.erase(std::unique(bt.begin(), bt.end()), bt.end());
-    std::vector<uint32_t> &off = c->path_off_h, &leaf = c->path_leaf, pending;
+    std::vector<uint32_t> &off = w->path_off_h, &leaf = w->path_leaf;
     off.assign(1, 0);
     leaf.clear();
-    std::vector<std::pair<uint32_t, uint32_t>> elem;  // (bottom target position, path element)
+    auto &elem = w->h_elem;  // (bottom target position, path element)
+    elem.clear();
     for (size_t i = 0; i < bt.size(); ++i) {
         const uint32_t t = bt[i];
         pending.push_back(t);
@@ -1567,7 +2117,7 @@ int stage_target_paths(bfg_ctx *c, const uint32_t *targets, uint32_t T) {
         pending.clear();
     }
     std::sort(elem.begin(), elem.end());
-    std::vector<uint32_t> &tp = c->tgt_pos_h;
+    std::vector<uint32_t> &tp = w->tgt_pos_h;
     tp.resize(T);
     for (uint32_t t = 0; t < T; ++t) {
         const uint32_t p = L.pos_of_rank[targets[t]];
@@ -1579,35 +2129,32 @@ int stage_target_paths(bfg_ctx *c, const uint32_t *targets, uint32_t T) {
             tp[t] = L.n_top + it->second;
         }
     }
-    if (c->path_off.n < off.size()) HIP_TRY(c->path_off.alloc(off.size()));
-    if (c->path_pos.n < off.back()) HIP_TRY(c->path_pos.alloc(off.back()));
-    if (T > c->tgt_cap) {
-        HIP_TRY(c->tgt_pos.alloc(T));
-        c->tgt_cap = T;
-    }
-    // the host tables live in the context until the next call, which starts
+    ENSURE(w, path_off, off.size(), "target paths");
+    ENSURE(w, path_pos, std::max<size_t>(off.back(), 1), "target paths");
+    ENSURE(w, tgt_pos, T, "targets");
+    // the host tables live in the Work until its next call, which starts
     // after this one has synchronised
-    HIP_TRY(hipMemcpyAsync(c->path_off.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->tgt_pos.p, tp.data(), (size_t)T * 4, hipMemcpyHostToDevice, c->stream));
-    c->n_paths = (uint32_t)leaf.size();
-    c->path_nodes = off.back();
-    c->path_arcs = 0;
-    c->path_pos_staged = false;
+    HIP_TRY(hipMemcpyAsync(w->path_off.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, w->stream));
+    HIP_TRY(hipMemcpyAsync(w->tgt_pos.p, tp.data(), (size_t)T * 4, hipMemcpyHostToDevice, w->stream));
+    w->n_paths = (uint32_t)leaf.size();
+    w->path_nodes = off.back();
+    w->path_arcs = 0;
+    w->path_pos_staged = false;
     return BFG_SUCCESS;
 }
 
 // The elements of the staged paths (a walk up from each path's target) and
 // their DOWN in-arc count, uploaded on the stream.  Called with the
 // top-region kernels already queued, so the walk overlaps them.
-int stage_path_elements(bfg_ctx *c) {
-    if (c->path_pos_staged) return BFG_SUCCESS;
+int stage_path_elements(bfg_ctx *c, Work *w) {
+    if (w->path_pos_staged) return BFG_SUCCESS;
     const bfg::Layout &L = c->L;
-    std::vector<uint32_t> &pos = c->path_pos_h;
-    pos.resize(c->path_nodes);
+    std::vector<uint32_t> &pos = w->path_pos_h;
+    pos.resize(w->path_nodes);
     uint64_t arcs = 0;
-    for (uint32_t k = 0; k < c->n_paths; ++k) {
-        uint32_t x = c->path_leaf[k];
-        for (uint32_t i = c->path_off_h[k + 1]; i-- > c->path_off_h[k];) {
+    for (uint32_t k = 0; k < w->n_paths; ++k) {
+        uint32_t x = w->path_leaf[k];
+        for (uint32_t i = w->path_off_h[k + 1]; i-- > w->path_off_h[k];) {
             if (x == NONE || x < L.n_top) return fail(BFG_INTERNAL_ERROR, "bottom path shorter than its depth");
             pos[i] = x;
             arcs += L.in_off[x + 1] - L.in_off[x];
@@ -1616,51 +2163,153 @@ int stage_path_elements(bfg_ctx *c) {
         if (x != NONE && x >= L.n_top) return fail(BFG_INTERNAL_ERROR, "bottom path longer than its depth");
     }
     if (!pos.empty())
-        HIP_TRY(hipMemcpyAsync(c->path_pos.p, pos.data(), pos.size() * 4, hipMemcpyHostToDevice, c->stream));
-    c->path_arcs = arcs;
-    c->path_pos_staged = true;
+        HIP_TRY(hipMemcpyAsync(w->path_pos.p, pos.data(), pos.size() * 4, hipMemcpyHostToDevice, w->stream));
+    w->path_arcs = arcs;
+    w->path_pos_staged = true;
     return BFG_SUCCESS;
 }
 
 // Full rows for all ranks (dist and/or reach bits) into host buffers indexed
-// by original lane.
-int dense_rows(bfg_ctx *c, const Req &req, uint32_t *dist_host, uint64_t *bits_host) {
+// by original lane.  Distances go through a device buffer of one batch's rows
+// (counted in the batch's lane capacity) straight into dist_host.
+int dense_rows(bfg_ctx *c, Work *w, const Req &req, uint32_t *dist_host, uint64_t *bits_host) {
     const uint32_t n = c->L.n;
     const uint32_t L = (uint32_t)req.limit.size();
     const uint64_t words = (n + 63) / 64;
-    DevBuf<uint32_t> d_rows, d_all_pos;
+    DevBuf<uint32_t> d_rows;
     DevBuf<unsigned long long> d_bits;
-    std::vector<uint32_t> all_pos(c->L.pos_of_rank);
-    const StreamDrain drain{c->stream};
-    HIP_TRY(d_all_pos.upload(all_pos, c->stream));
+    const StreamDrain drain{w->stream};
     // reach bits of every lane land in request order on the device (rows
     // through row_map) and come back in one copy
     if (bits_host) HIP_TRY(d_bits.alloc((size_t)L * words));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    std::vector<uint32_t> hrow;
-    uint32_t cap = 0;
-    int rc = run_all(c, req, [&](uint32_t b0, uint32_t nl, const std::vector<uint32_t> &rows, float *oms) -> int {
-        (void)b0;
-        if (dist_host && cap < nl) {
-            HIP_TRY(d_rows.alloc((size_t)nl * n));
-            cap = nl;
-        }
-        if (dist_host) {  // times: this batch's rows, in sorted lane order
-            int rc = gather_out(c, nl, d_all_pos.p, n, nullptr, d_rows.p, n, nullptr, 0, oms);
-            if (rc) return rc;
-            hrow.resize((size_t)nl * n);
-            HIP_TRY(hipMemcpy(hrow.data(), d_rows.p, (size_t)nl * n * 4, hipMemcpyDeviceToHost));
+    const size_t extra = dist_host ? (size_t)n * 4 : 0;
+    int rc = run_all(c, w, req, extra, [&](uint32_t, uint32_t nl, const std::vector<uint32_t> &rows, float *oms) -> int {
+        if (dist_host) {  // this batch's rows in sorted lane order, each copied to its request row
+            HIP_TRY(d_rows.ensure((size_t)nl * n));
+            int rc2 = gather_out(c, w, nl, c->pos_of_rank_d.p, n, nullptr, d_rows.p, n, nullptr, 0, oms);
+            if (rc2) return rc2;
             for (uint32_t i = 0; i < nl; ++i)
-                memcpy(dist_host + (size_t)rows[i] * n, hrow.data() + (size_t)i * n, (size_t)n * 4);
+                HIP_TRY(hipMemcpyAsync(dist_host + (size_t)rows[i] * n, d_rows.p + (size_t)i * n, (size_t)n * 4,
+                                       hipMemcpyDeviceToHost, w->stream));
+            HIP_TRY(hipStreamSynchronize(w->stream));
         }
         if (bits_host) {
-            int rc = gather_out(c, nl, d_all_pos.p, n, c->row_map.p, nullptr, n, d_bits.p, words, oms);
-            if (rc) return rc;
+            int rc2 = gather_out(c, w, nl, c->pos_of_rank_d.p, n, w->row_map.p, nullptr, n, d_bits.p, words, oms);
+            if (rc2) return rc2;
         }
         return BFG_SUCCESS;
     });
     if (rc) return rc;
     if (bits_host) HIP_TRY(hipMemcpy(bits_host, d_bits.p, (size_t)L * words * 8, hipMemcpyDeviceToHost));
+    return BFG_SUCCESS;
+}
+
+// Isochrone lists of every lane of `req`: reach bits (into d_bits_ext when
+// given, else a buffer of the call; copied to bits_host when given), then per
+// lane its settled (rank, time) pairs in ascending rank.  Lane i's list is
+// pairs[start[i] .. start[i] + len[i]); lists are laid out batch by batch,
+// by original lane order within a batch.  Pairs go to d_pairs_ext (device,
+// `capacity` pairs) or to pairs_host (`capacity` pairs).  If the lists need
+// more than capacity, len and start are still filled (every batch is
+// counted) and BFG_BUFFER_TOO_SMALL is returned with no pair written past
+// what fit.
+int iso_lists(bfg_ctx *c, Work *w, const Req &req, uint64_t *bits_host, uint64_t *d_bits_ext, uint64_t *start,
+              uint32_t *len, uint32_t *pairs_host, uint32_t *d_pairs_ext, uint64_t capacity, uint64_t *total_out) {
+    const uint32_t n = c->L.n;
+    const uint32_t L = (uint32_t)req.limit.size();
+    const uint64_t words = (n + 63) / 64;
+    const uint32_t nsb = (n + SB_RANKS - 1) / SB_RANKS;
+    DevBuf<unsigned long long> d_bits_own;
+    DevBuf<uint32_t> d_sorted, d_slot, d_sboff, d_rowtot;
+    DevBuf<uint64_t> d_rowstart, d_total;
+    DevBuf<uint2> d_pairs_own;
+    hipStream_t s = w->stream;
+    const StreamDrain drain{s};
+    unsigned long long *d_bits = (unsigned long long *)d_bits_ext;
+    if (!d_bits) {
+        HIP_TRY(d_bits_own.alloc((size_t)L * words));
+        d_bits = d_bits_own.p;
+    }
+    HIP_TRY(d_total.alloc(1));
+    uint64_t running = 0;
+    bool too_small = false;
+    std::vector<uint32_t> sorted, slot, rowtot;
+    const size_t extra = (size_t)nsb * 4 + 32;
+    int rc = run_all(c, w, req, extra, [&](uint32_t, uint32_t nl, const std::vector<uint32_t> &rows, float *oms) -> int {
+        int rc2 = gather_out(c, w, nl, c->pos_of_rank_d.p, n, w->row_map.p, nullptr, n, d_bits, words, oms);
+        if (rc2) return rc2;
+        // the batch's rows in original order, and each sorted lane's slot among them
+        sorted.assign(rows.begin(), rows.end());
+        std::sort(sorted.begin(), sorted.end());
+        slot.resize(nl);
+        for (uint32_t i = 0; i < nl; ++i)
+            slot[i] = (uint32_t)(std::lower_bound(sorted.begin(), sorted.end(), rows[i]) - sorted.begin());
+        HIP_TRY(d_sorted.ensure(nl));
+        HIP_TRY(d_slot.ensure(nl));
+        HIP_TRY(d_sboff.ensure((size_t)nl * nsb));
+        HIP_TRY(d_rowtot.ensure(nl));
+        HIP_TRY(d_rowstart.ensure(nl));
+        HIP_TRY(hipMemcpyAsync(d_sorted.p, sorted.data(), nl * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_slot.p, slot.data(), nl * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipEventRecord(w->ev[0], s));
+        iso_count_kernel<<<nl, ISO_THREADS, 0, s>>>(d_bits, words, words, d_sorted.p, nsb, d_sboff.p, d_rowtot.p);
+        // host pairs go through a buffer of this batch's pairs (base 0)
+        const uint64_t base = d_pairs_ext ? running : 0;
+        iso_scan_kernel<<<1, ISO_THREADS, 0, s>>>(d_rowtot.p, nl, base, d_rowstart.p, d_total.p);
+        HIP_TRY(hipGetLastError());
+        rowtot.resize(nl);
+        HIP_TRY(hipMemcpyAsync(rowtot.data(), d_rowtot.p, nl * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        uint64_t batch = 0;
+        for (uint32_t i = 0; i < nl; ++i) {
+            start[sorted[i]] = running + batch;
+            len[sorted[i]] = rowtot[i];
+            batch += rowtot[i];
+        }
+        if (too_small || running + batch > capacity) {
+            too_small = true;
+            running += batch;
+            return BFG_SUCCESS;
+        }
+        uint2 *dst = (uint2 *)d_pairs_ext;
+        if (!dst) {
+            HIP_TRY(d_pairs_own.ensure(std::max<uint64_t>(batch, 1)));
+            dst = d_pairs_own.p;
+        }
+        const uint32_t groups = (nl + c->lanes - 1) / c->lanes;
+        const BGate B = bgate_of(c, w);
+        const dim3 grid(nsb, groups);
+        switch (c->lanes) {
+            case 16:
+                iso_emit_kernel<16><<<grid, NT, 0, s>>>(w->arena.p, n, c->pos_of_rank_d.p, n, w->lane_limit.p, nl,
+                                                        d_slot.p, d_rowstart.p, d_sboff.p, nsb, B, dst);
+                break;
+            case 64:
+                iso_emit_kernel<64><<<grid, NT, 0, s>>>(w->arena.p, n, c->pos_of_rank_d.p, n, w->lane_limit.p, nl,
+                                                        d_slot.p, d_rowstart.p, d_sboff.p, nsb, B, dst);
+                break;
+            default:
+                iso_emit_kernel<32><<<grid, NT, 0, s>>>(w->arena.p, n, c->pos_of_rank_d.p, n, w->lane_limit.p, nl,
+                                                        d_slot.p, d_rowstart.p, d_sboff.p, nsb, B, dst);
+                break;
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(w->ev[1], s));
+        if (pairs_host && batch)
+            HIP_TRY(hipMemcpyAsync(pairs_host + 2 * running, dst, batch * 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        float a = 0;
+        HIP_TRY(hipEventElapsedTime(&a, w->ev[0], w->ev[1]));
+        *oms += a;
+        running += batch;
+        return BFG_SUCCESS;
+    });
+    if (rc) return rc;
+    if (total_out) *total_out = running;
+    if (bits_host) HIP_TRY(hipMemcpy(bits_host, d_bits, (size_t)L * words * 8, hipMemcpyDeviceToHost));
+    if (too_small)
+        return fail(BFG_BUFFER_TOO_SMALL, "isochrone lists need " + std::to_string(running) + " pairs, capacity " +
+                                              std::to_string(capacity));
     return BFG_SUCCESS;
 }
 
@@ -2011,7 +2660,8 @@ __global__ __launch_bounds__(NT) void t2_table_kernel(uint32_t n, uint32_t T, ui
 // Builds the two-channel arc lists from the CSR the context was created from
 // and length weights aligned with it (arcs of INF time are dropped, as the
 // time flats drop them).
-int attach_length(bfg_ctx *c, const bfg_graph_desc *gd, const uint32_t *up_len, const uint32_t *dn_len) {
+int attach_length(bfg_ctx *c, hipStream_t s, const bfg_graph_desc *gd, const uint32_t *up_len,
+                  const uint32_t *dn_len) {
     const bfg::Layout &L = c->L;
     const uint32_t n = L.n, nt = L.n_top;
     if (n >= (1u << 26)) return fail(BFG_NOT_SUPPORTED, "two-channel lists pack positions in 26 bits");
@@ -2121,7 +2771,6 @@ int attach_length(bfg_ctx *c, const bfg_graph_desc *gd, const uint32_t *up_len, 
         }
     }
     std::unique_ptr<TwoChannel> t(new TwoChannel());
-    hipStream_t s = c->stream;
     const StreamDrain drain{s};
     HIP_TRY(t->x_off.upload(x_off, s));
     HIP_TRY(t->x_sj.upload(x_sj, s));
@@ -2161,13 +2810,12 @@ struct BottomSet {
 };
 
 template <class Emit>
-int phast_2ch(bfg_ctx *c, const std::vector<uint32_t> &soff, const std::vector<uint32_t> &spos,
+int phast_2ch(bfg_ctx *c, hipStream_t s, const std::vector<uint32_t> &soff, const std::vector<uint32_t> &spos,
               const std::vector<uint64_t> &slab, const uint32_t *limits, uint32_t nl, bool dense, BottomSet bs,
               Emit emit) {
     TwoChannel &t = *c->t2;
     const bfg::Layout &L = c->L;
     const uint32_t n = L.n;
-    hipStream_t s = c->stream;
     // groups per batch: the labels (and for dense rows both output channels)
     // fit a share of free memory; a sweep's time barely grows with its groups
     size_t free_b = 0, total_b = 0;
@@ -2294,6 +2942,146 @@ bfg::CchInput input_of(const bfg_graph_desc *gd) {
     return in;
 }
 
+// Host tables of the block gate (see bfg_ctx): segment parents, the bottom
+// trees in pre-order (ch_pos lists the bottom positions in pre-order, and a
+// tree's nodes are consecutive there), the tree of each bottom position and
+// the trees each bottom chunk touches.
+struct GateTables {
+    std::vector<uint32_t> xr_min, cs_min, pos_bt, bt_ch;
+};
+
+// Host tables of the block gate (see bfg_ctx): the least weight of every DOWN
+// row and of every bottom chunk source's arcs; the bottom trees in pre-order
+// (ch_pos lists the bottom positions in pre-order, and a tree's nodes are
+// consecutive there), the tree of each bottom position and the chunks each
+// tree touches.
+int gate_tables(const bfg::Layout &L, GateTables &G) {
+    const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1, nt = L.n_top;
+    G.xr_min.assign(L.xr_off.empty() ? 0 : L.xr_off.back(), INF);
+    for (uint32_t sg = 0; sg < nseg; ++sg) {
+        uint64_t w = L.xw_off[sg];
+        for (uint32_t k = 0, st = 0; k < 8; st = L.xr_run[8 * (size_t)sg + k++]) {
+            const uint32_t W = 8 * (8 - k), end = L.xr_run[8 * (size_t)sg + k];
+            for (uint32_t i = st; i < end; ++i, w += W) {
+                uint32_t m = INF;
+                for (uint32_t j = 0; j < W; ++j) m = std::min(m, L.xr_w[w + j]);
+                G.xr_min[L.xr_off[sg] + i] = m;
+            }
+        }
+        if (w != L.xw_off[sg + 1]) return fail(BFG_INTERNAL_ERROR, "row runs disagree");
+    }
+    G.cs_min.assign(L.cs_src.size(), INF);
+    for (size_t k = 0; k < L.cs_src.size(); ++k)
+        for (uint32_t a = L.cs_aoff[k]; a < L.cs_aoff[k + 1]; ++a) G.cs_min[k] = std::min(G.cs_min[k], L.ca_w[a]);
+    G.pos_bt.assign(L.n - nt, NONE);
+    const size_t nch = L.ch_off.size() - 1;
+    uint32_t nbt = 0;
+    for (size_t c = 0; c < nch; ++c)
+        for (uint32_t i = L.ch_off[c]; i < L.ch_off[c + 1]; ++i) {
+            const uint32_t p = L.ch_pos[i], pp = L.pos_parent[p];
+            if (pp == NONE || pp < nt) {  // a bottom tree's root
+                G.bt_ch.push_back((uint32_t)c);
+                G.bt_ch.push_back((uint32_t)c);
+                ++nbt;
+            } else if (nbt == 0) {
+                return fail(BFG_INTERNAL_ERROR, "bottom positions not in pre-order");
+            }
+            G.pos_bt[p - nt] = nbt - 1;
+            G.bt_ch[2 * (nbt - 1) + 1] = (uint32_t)c;
+        }
+    for (uint32_t p = nt; p < L.n; ++p) {  // a node's tree is its parent's
+        const uint32_t pp = L.pos_parent[p];
+        if (G.pos_bt[p - nt] == NONE || (pp != NONE && pp >= nt && G.pos_bt[p - nt] != G.pos_bt[pp - nt]))
+            return fail(BFG_INTERNAL_ERROR, "a bottom tree is not consecutive in pre-order");
+    }
+    return BFG_SUCCESS;
+}
+
+int matrix_impl(bfg_ctx *c, Work *w, Req &req, const uint32_t *src, uint32_t S, const uint32_t *tgt, uint32_t T,
+                uint32_t thr, uint32_t *d_out, uint32_t *host_out) {
+    single_source_req(req, src, S, thr);
+    int rc = stage_target_paths(c, w, tgt, T);
+    if (rc) return rc;
+    PathsOn paths_on(w);  // restrict the bottom sweep for this call only
+    DevBuf<uint32_t> tmp;
+    if (!d_out) {
+        HIP_TRY(tmp.alloc((size_t)S * T));
+        d_out = tmp.p;
+    }
+    rc = run_all(c, w, req, 0, [&](uint32_t, uint32_t nl, const std::vector<uint32_t> &, float *oms) -> int {
+        return gather_out(c, w, nl, w->tgt_pos.p, T, w->row_map.p, d_out, T, nullptr, 0, oms);
+    });
+    if (rc) return rc;
+    if (host_out) HIP_TRY(hipMemcpy(host_out, d_out, (size_t)S * T * 4, hipMemcpyDeviceToHost));
+    return BFG_SUCCESS;
+}
+
+// Several matrix requests in one sweep: request r has sources
+// src[src_off[r] .. src_off[r+1]) and targets tgt[tgt_off[r] .. tgt_off[r+1])
+// and limit thr[r]; its S_r x T_r row-major block goes to d_out from
+// sum_{q < r} S_q T_q on.  Lanes are sorted by request, then by pre-order, so
+// each request's lanes are consecutive; the bottom sweep covers the paths
+// above the union of all targets, and each request gathers its own columns.
+int matrix_multi(bfg_ctx *c, Work *w, uint32_t nr, const uint32_t *src_off, const uint32_t *src,
+                 const uint32_t *tgt_off, const uint32_t *tgt, const uint32_t *thr, uint32_t *d_out) {
+    const uint32_t S = src_off[nr], NT_ = tgt_off[nr];
+    Req req;
+    single_source_req(req, src, S, INF);
+    req.major.resize(S);
+    std::vector<uint64_t> out_off(nr + 1, 0);
+    for (uint32_t r = 0; r < nr; ++r) {
+        for (uint32_t l = src_off[r]; l < src_off[r + 1]; ++l) {
+            req.major[l] = r;
+            req.limit[l] = thr[r];
+        }
+        out_off[r + 1] = out_off[r] + (uint64_t)(src_off[r + 1] - src_off[r]) * (tgt_off[r + 1] - tgt_off[r]);
+    }
+    std::vector<uint32_t> U(tgt, tgt + NT_);
+    std::sort(U.begin(), U.end());
+    U.erase(std::unique(U.begin(), U.end()), U.end());
+    int rc = stage_target_paths(c, w, U.data(), (uint32_t)U.size());
+    if (rc) return rc;
+    // each request's targets as label indices of the union's paths
+    std::vector<uint32_t> tlab(NT_);
+    for (uint32_t q = 0; q < NT_; ++q)
+        tlab[q] = w->tgt_pos_h[std::lower_bound(U.begin(), U.end(), tgt[q]) - U.begin()];
+    DevBuf<uint32_t> d_tlab;
+    const StreamDrain drain{w->stream};
+    HIP_TRY(d_tlab.upload(tlab, w->stream));
+    PathsOn paths_on(w);
+    return run_all(c, w, req, 0, [&](uint32_t b0, uint32_t nl, const std::vector<uint32_t> &, float *oms) -> int {
+        // request r holds sorted lanes [src_off[r], src_off[r+1]); this batch [b0, b0 + nl)
+        for (uint32_t r = 0; r < nr; ++r) {
+            const uint32_t lo = std::max(src_off[r], b0), hi = std::min(src_off[r + 1], b0 + nl);
+            const uint32_t T = tgt_off[r + 1] - tgt_off[r];
+            if (lo >= hi || T == 0) continue;
+            const int rc2 = gather_out(c, w, hi - b0, d_tlab.p + tgt_off[r], T, w->row_map.p, d_out + out_off[r], T,
+                                       nullptr, 0, oms, lo - b0, src_off[r]);
+            if (rc2) return rc2;
+        }
+        return BFG_SUCCESS;
+    });
+}
+
+// Seeded lanes of a call from the caller's CSR (at most 64 seeds per lane).
+int seeded_req(bfg_ctx *c, Req &req, const uint32_t *seed_offsets, const uint32_t *seed_ranks,
+               const uint32_t *seed_costs, uint32_t L, const uint32_t *thresholds) {
+    if (seed_offsets[0] != 0) return fail(BFG_INVALID_PARAMETER, "seed_offsets[0] must be 0");
+    for (uint32_t i = 0; i < L; ++i) {
+        if (seed_offsets[i + 1] < seed_offsets[i]) return fail(BFG_INVALID_PARAMETER, "seed_offsets not monotone");
+        if (seed_offsets[i + 1] - seed_offsets[i] > 64) return fail(BFG_NOT_SUPPORTED, "more than 64 seeds in one lane");
+    }
+    const uint32_t ns = seed_offsets[L];
+    if (ns && (!seed_ranks || !seed_costs)) return fail(BFG_INVALID_PARAMETER, "null seed buffer");
+    const int rc = check_ranks(c, seed_ranks, ns, "seed");
+    if (rc) return rc;
+    req.soff.assign(seed_offsets, seed_offsets + L + 1);
+    req.srank.assign(seed_ranks, seed_ranks + ns);
+    req.scost.assign(seed_costs, seed_costs + ns);
+    req.limit.assign(thresholds, thresholds + L);
+    return BFG_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2309,6 +3097,10 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     std::unique_ptr<bfg_ctx> c(new bfg_ctx());
     c->device = device;
     HIP_TRY(hipSetDevice(device));
+    if (const char *e = std::getenv("BFG_STREAMS")) {  // calls in flight at once
+        const long v = std::strtol(e, nullptr, 10);
+        if (v >= 1 && v <= 16) c->max_works = (uint32_t)v;
+    }
     const bfg::CchInput in = input_of(gd);
     std::string err;
     const bfg::LayoutParams lp = layout_params();
@@ -2316,9 +3108,14 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     // every slot, position and segment index the kernels derive rests on these
     // properties; refuse the graph rather than launch on a broken layout
     if (!bfg::check_layout(in, c->L, err)) return fail(BFG_INTERNAL_ERROR, err);
-    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    for (auto &e : c->ev) HIP_TRY(hipEventCreate(&e));
-    hipStream_t s = c->stream;
+    GateTables gt;
+    int rc = gate_tables(c->L, gt);
+    if (rc) return rc;
+    c->n_bt = (uint32_t)(gt.bt_ch.size() / 2);
+    WorkLease lease;  // the first Work: its stream carries the uploads
+    rc = acquire_work(c.get(), lease);
+    if (rc) return rc;
+    hipStream_t s = lease.w->stream;
     auto &L = c->L;
     HIP_TRY(c->parent.upload(L.parent, s));
     HIP_TRY(c->pos_parent.upload(L.pos_parent, s));
@@ -2371,13 +3168,19 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     HIP_TRY(c->pr_x.upload(L.pr_x, s));
     HIP_TRY(c->pr_w.upload(L.pr_w, s));
     HIP_TRY(c->pk_seg.upload(L.pk_seg, s));
+    HIP_TRY(c->xr_min.upload(gt.xr_min, s));
+    HIP_TRY(c->cs_min.upload(gt.cs_min, s));
+    HIP_TRY(c->pos_bt.upload(gt.pos_bt, s));
+    HIP_TRY(c->bt_ch.upload(gt.bt_ch, s));
+    HIP_TRY(c->pos_of_rank_d.upload(L.pos_of_rank, s));
     HIP_TRY(hipStreamSynchronize(s));
     c->graph_bytes = 0;
     for (auto *b : {&c->parent, &c->pos_parent, &c->pos_pre, &c->pos_sub, &c->pos_depth, &c->seg_pos, &c->in_off,
                     &c->in_split, &c->in_src, &c->in_w, &c->ch_off, &c->ch_pos, &c->cs_off, &c->cs_src, &c->cs_aoff,
                     &c->ca_w, &c->up_off, &c->up_tgt, &c->up_w, &c->xr_off, &c->xr_run, &c->xw_off, &c->pr_wid,
-                    &c->pw_off, &c->xr_src, &c->xr_w, &c->tb_dn,
-                    &c->tb_up, &c->tb_w, &c->pt_blk, &c->pt_tgt, &c->pr_off, &c->pr_x, &c->pr_w, &c->pk_seg})
+                    &c->pw_off, &c->xr_src, &c->xr_w, &c->tb_dn, &c->tb_up, &c->tb_w, &c->pt_blk, &c->pt_tgt,
+                    &c->pr_off, &c->pr_x, &c->pr_w, &c->pk_seg, &c->xr_min, &c->cs_min, &c->pos_bt, &c->bt_ch,
+                    &c->pos_of_rank_d})
         c->graph_bytes += b->n * sizeof(uint32_t);
     c->graph_bytes += (c->ca_v.n + c->in_dd.n) * sizeof(uint16_t);
     // host copies the kernels alone need
@@ -2387,6 +3190,13 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
         std::vector<uint32_t>().swap(*v);
     std::vector<uint16_t>().swap(L.ca_v);
     std::vector<uint16_t>().swap(L.in_dd);
+    // the one-wave-group DOWN plan (kept for calls that must not build one)
+    std::shared_ptr<Plan> p1(new Plan());
+    rc = build_plan(c.get(), s, 1, *p1);
+    if (rc) return rc;
+    c->plan1 = std::move(p1);
+    lease.w = nullptr;  // hand the first Work to the pool by hand: the context is about to be released
+    c->idle.push_back(c->works.front().get());
     *out = c.release();
     return BFG_SUCCESS;
 }
@@ -2401,21 +3211,33 @@ int bfg_validate_graph(const bfg_graph_desc *gd, bfg_ctx_info *info) {
     const bfg::LayoutParams lp = layout_params();
     if (!bfg::build_layout(in, lp, L, err)) return fail(BFG_INVALID_PARAMETER, err);
     if (!bfg::check_layout(in, L, err)) return fail(BFG_INTERNAL_ERROR, err);
+    GateTables gt;
+    const int rc = gate_tables(L, gt);
+    if (rc) return rc;
     fill_info(L, info);
     return BFG_SUCCESS;
 }
 
-int bfg_ctx_get_info(const bfg_ctx *c, bfg_ctx_info *info) {
+int bfg_ctx_get_info(bfg_ctx *c, bfg_ctx_info *info) {
     if (!c || !info) return fail(BFG_INVALID_PARAMETER, "null argument");
+    std::shared_lock<std::shared_mutex> cfg(c->cfg);
     fill_info(c->L, info);
     info->lanes = c->lanes;
-    info->lane_capacity = c->lane_cap;
-    info->device_bytes = c->graph_bytes + c->arena.n * 4 + c->pathbuf.n * 4;
+    std::lock_guard<std::mutex> g(c->pool_mu);
+    uint64_t held = 0;
+    uint32_t cap = 0;
+    for (auto &w : c->works) {
+        held += (w->arena.n + w->pathbuf.n) * 4;
+        cap = std::max(cap, w->lane_cap);
+    }
+    info->lane_capacity = c->cap_fixed ? c->lane_cap : cap;
+    info->device_bytes = c->graph_bytes + held;
     return BFG_SUCCESS;
 }
 
-int bfg_ctx_last_timings(const bfg_ctx *c, bfg_timings *t) {
+int bfg_ctx_last_timings(bfg_ctx *c, bfg_timings *t) {
     if (!c || !t) return fail(BFG_INVALID_PARAMETER, "null argument");
+    std::lock_guard<std::mutex> g(c->last_mu);
     *t = c->last;
     return BFG_SUCCESS;
 }
@@ -2424,17 +3246,26 @@ int bfg_ctx_tune(bfg_ctx *c, uint32_t plan_rows, uint32_t tri_scalar_segs) {
     if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
     if (plan_rows != 0 && (plan_rows < PLAN_MIN_ROWS || plan_rows >= (1u << 24)))
         return fail(BFG_INVALID_PARAMETER, "plan_rows out of range");
-    std::lock_guard<std::mutex> g(c->mu);
+    std::unique_lock<std::shared_mutex> cfg(c->cfg);
+    HIP_TRY(hipSetDevice(c->device));
     c->plan_rows = plan_rows;
     c->tri_segs = tri_scalar_segs;
-    c->plan_wgs = 0;  // the DOWN plan is rebuilt by the next call
+    std::lock_guard<std::mutex> g(c->plan_mu);
+    c->plans.clear();  // rebuilt by the next calls
+    c->plan_lru.clear();
+    WorkLease lease;
+    int rc = acquire_work(c, lease);
+    if (rc) return rc;
+    std::shared_ptr<Plan> p1(new Plan());
+    rc = build_plan(c, lease.w->stream, 1, *p1);
+    if (rc) return rc;
+    c->plan1 = std::move(p1);
     return BFG_SUCCESS;
 }
 
 int bfg_ctx_configure(bfg_ctx *c, uint32_t lanes, uint32_t lane_capacity) {
     if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
-    std::lock_guard<std::mutex> g(c->mu);
-    HIP_TRY(hipSetDevice(c->device));
+    std::unique_lock<std::shared_mutex> cfg(c->cfg);
     if (lanes != 0) {
         if (lanes != 16 && lanes != 32 && lanes != 64) return fail(BFG_INVALID_PARAMETER, "lanes must be 16, 32 or 64");
         c->lanes = lanes;
@@ -2442,32 +3273,17 @@ int bfg_ctx_configure(bfg_ctx *c, uint32_t lanes, uint32_t lane_capacity) {
     if (lane_capacity != 0) {
         c->lane_cap = (lane_capacity + 63) / 64 * 64;  // whole wavefronts of lanes
         c->cap_fixed = true;
-        return ensure_arena(c);
+    } else {
+        c->cap_fixed = false;
     }
-    c->cap_fixed = false;
     return BFG_SUCCESS;
 }
 
-static int matrix_impl(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *tgt, uint32_t T, uint32_t thr,
-                       uint32_t *d_out, uint32_t *host_out) {
-    Req req = single_source_req(src, S, thr);
-    int rc = stage_target_paths(c, tgt, T);
-    if (rc) return rc;
-    struct PathsOn {  // restrict the bottom sweep for this call only
-        bfg_ctx *c;
-        explicit PathsOn(bfg_ctx *cc) : c(cc) { c->paths_on = true; }
-        ~PathsOn() { c->paths_on = false; }
-    } paths_on(c);
-    DevBuf<uint32_t> tmp;
-    if (!d_out) {
-        HIP_TRY(tmp.alloc((size_t)S * T));
-        d_out = tmp.p;
-    }
-    rc = run_all(c, req, [&](uint32_t, uint32_t nl, const std::vector<uint32_t> &, float *oms) -> int {
-        return gather_out(c, nl, c->tgt_pos.p, T, c->row_map.p, d_out, T, nullptr, 0, oms);
-    });
-    if (rc) return rc;
-    if (host_out) HIP_TRY(hipMemcpy(host_out, d_out, (size_t)S * T * 4, hipMemcpyDeviceToHost));
+int bfg_ctx_set_concurrency(bfg_ctx *c, uint32_t max_calls) {
+    if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
+    if (max_calls < 1 || max_calls > 16) return fail(BFG_INVALID_PARAMETER, "max_calls must be in 1..16");
+    std::lock_guard<std::mutex> g(c->pool_mu);
+    c->max_works = max_calls;  // existing Works beyond the limit stay until the context goes
     return BFG_SUCCESS;
 }
 
@@ -2475,13 +3291,15 @@ int bfg_matrix_bounded(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32
                        uint32_t *out) {
     if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
     if ((S && !src) || (T && !tgt) || (S && T && !out)) return fail(BFG_INVALID_PARAMETER, "null buffer");
-    std::lock_guard<std::mutex> g(c->mu);
-    HIP_TRY(hipSetDevice(c->device));
     int rc = check_ranks(c, src, S, "source");
     if (!rc) rc = check_ranks(c, tgt, T, "target");
     if (rc) return rc;
     if (S == 0 || T == 0) return BFG_SUCCESS;
-    return matrix_impl(c, src, S, tgt, T, thr, nullptr, out);
+    CallScope cs(c);
+    rc = begin_call(c, cs);
+    if (rc) return rc;
+    Req req;
+    return matrix_impl(c, cs.lease.w, req, src, S, tgt, T, thr, nullptr, out);
 }
 
 int bfg_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *tgt, uint32_t T, uint32_t *out) {
@@ -2492,74 +3310,145 @@ int bfg_matrix_device(bfg_ctx *c, const uint32_t *d_src, uint32_t S, const uint3
                       uint32_t *d_out) {
     if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
     if ((S && !d_src) || (T && !d_tgt) || (S && T && !d_out)) return fail(BFG_INVALID_PARAMETER, "null buffer");
-    std::lock_guard<std::mutex> g(c->mu);
-    HIP_TRY(hipSetDevice(c->device));
     if (S == 0 || T == 0) return BFG_SUCCESS;
-    std::vector<uint32_t> src(S), tgt(T);
-    HIP_TRY(hipMemcpy(src.data(), d_src, (size_t)S * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(tgt.data(), d_tgt, (size_t)T * 4, hipMemcpyDeviceToHost));
-    int rc = check_ranks(c, src.data(), S, "source");
-    if (!rc) rc = check_ranks(c, tgt.data(), T, "target");
+    CallScope cs(c);
+    int rc = begin_call(c, cs);
     if (rc) return rc;
-    return matrix_impl(c, src.data(), S, tgt.data(), T, thr, d_out, nullptr);
+    Work *w = cs.lease.w;
+    w->h_src.resize(S);
+    w->h_tgt.resize(T);
+    HIP_TRY(hipMemcpy(w->h_src.data(), d_src, (size_t)S * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(w->h_tgt.data(), d_tgt, (size_t)T * 4, hipMemcpyDeviceToHost));
+    rc = check_ranks(c, w->h_src.data(), S, "source");
+    if (!rc) rc = check_ranks(c, w->h_tgt.data(), T, "target");
+    if (rc) return rc;
+    Req req;
+    return matrix_impl(c, w, req, w->h_src.data(), S, w->h_tgt.data(), T, thr, d_out, nullptr);
+}
+
+int bfg_matrix_multi(bfg_ctx *c, uint32_t n_req, const uint32_t *src_offsets, const uint32_t *sources,
+                     const uint32_t *tgt_offsets, const uint32_t *targets, const uint32_t *thresholds,
+                     uint32_t *out) {
+    if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
+    if (n_req == 0) return BFG_SUCCESS;
+    if (!src_offsets || !tgt_offsets || !thresholds) return fail(BFG_INVALID_PARAMETER, "null buffer");
+    if (src_offsets[0] != 0 || tgt_offsets[0] != 0) return fail(BFG_INVALID_PARAMETER, "offsets must start at 0");
+    uint64_t cells = 0;
+    for (uint32_t r = 0; r < n_req; ++r) {
+        if (src_offsets[r + 1] < src_offsets[r] || tgt_offsets[r + 1] < tgt_offsets[r])
+            return fail(BFG_INVALID_PARAMETER, "offsets not monotone");
+        cells += (uint64_t)(src_offsets[r + 1] - src_offsets[r]) * (tgt_offsets[r + 1] - tgt_offsets[r]);
+    }
+    const uint32_t S = src_offsets[n_req], T = tgt_offsets[n_req];
+    if ((S && !sources) || (T && !targets) || (cells && !out)) return fail(BFG_INVALID_PARAMETER, "null buffer");
+    int rc = check_ranks(c, sources, S, "source");
+    if (!rc) rc = check_ranks(c, targets, T, "target");
+    if (rc) return rc;
+    if (cells == 0) return BFG_SUCCESS;
+    CallScope cs(c);
+    rc = begin_call(c, cs);
+    if (rc) return rc;
+    DevBuf<uint32_t> d_out;
+    HIP_TRY(d_out.alloc(cells));
+    rc = matrix_multi(c, cs.lease.w, n_req, src_offsets, sources, tgt_offsets, targets, thresholds, d_out.p);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(out, d_out.p, cells * 4, hipMemcpyDeviceToHost));
+    return BFG_SUCCESS;
 }
 
 int bfg_query_batch(bfg_ctx *c, const uint32_t *src, uint32_t S, uint32_t thr, uint32_t *dist) {
     if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
     if (S && (!src || !dist)) return fail(BFG_INVALID_PARAMETER, "null buffer");
-    std::lock_guard<std::mutex> g(c->mu);
-    HIP_TRY(hipSetDevice(c->device));
     int rc = check_ranks(c, src, S, "source");
     if (rc) return rc;
     if (S == 0) return BFG_SUCCESS;
-    return dense_rows(c, single_source_req(src, S, thr), dist, nullptr);
+    CallScope cs(c);
+    rc = begin_call(c, cs);
+    if (rc) return rc;
+    Req req;
+    single_source_req(req, src, S, thr);
+    return dense_rows(c, cs.lease.w, req, dist, nullptr);
 }
 
 int bfg_isochrone(bfg_ctx *c, const uint32_t *origins, const uint32_t *limits, uint32_t L, uint64_t *bits,
                   uint32_t *times) {
     if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
     if (L && (!origins || !limits || !bits)) return fail(BFG_INVALID_PARAMETER, "null buffer");
-    std::lock_guard<std::mutex> g(c->mu);
-    HIP_TRY(hipSetDevice(c->device));
     int rc = check_ranks(c, origins, L, "origin");
     if (rc) return rc;
     if (L == 0) return BFG_SUCCESS;
-    Req req = single_source_req(origins, L, INF);
+    CallScope cs(c);
+    rc = begin_call(c, cs);
+    if (rc) return rc;
+    Req req;
+    single_source_req(req, origins, L, INF);
     req.limit.assign(limits, limits + L);
-    return dense_rows(c, req, times, bits);
+    return dense_rows(c, cs.lease.w, req, times, bits);
 }
 
 int bfg_phast_seeded(bfg_ctx *c, const uint32_t *seed_offsets, const uint32_t *seed_ranks, const uint32_t *seed_costs,
                      uint32_t L, const uint32_t *thresholds, uint32_t *dist) {
     if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
     if (L && (!seed_offsets || !thresholds || !dist)) return fail(BFG_INVALID_PARAMETER, "null buffer");
-    std::lock_guard<std::mutex> g(c->mu);
-    HIP_TRY(hipSetDevice(c->device));
     if (L == 0) return BFG_SUCCESS;
-    if (seed_offsets[0] != 0) return fail(BFG_INVALID_PARAMETER, "seed_offsets[0] must be 0");
     Req req;
-    req.soff.assign(seed_offsets, seed_offsets + L + 1);
-    for (uint32_t i = 0; i < L; ++i) {
-        if (req.soff[i + 1] < req.soff[i]) return fail(BFG_INVALID_PARAMETER, "seed_offsets not monotone");
-        if (req.soff[i + 1] - req.soff[i] > 64) return fail(BFG_NOT_SUPPORTED, "more than 64 seeds in one lane");
-    }
-    const uint32_t ns = req.soff[L];
-    if (ns && (!seed_ranks || !seed_costs)) return fail(BFG_INVALID_PARAMETER, "null seed buffer");
-    int rc = check_ranks(c, seed_ranks, ns, "seed");
+    int rc = seeded_req(c, req, seed_offsets, seed_ranks, seed_costs, L, thresholds);
     if (rc) return rc;
-    req.srank.assign(seed_ranks, seed_ranks + ns);
-    req.scost.assign(seed_costs, seed_costs + ns);
-    req.limit.assign(thresholds, thresholds + L);
-    return dense_rows(c, req, dist, nullptr);
+    CallScope cs(c);
+    rc = begin_call(c, cs);
+    if (rc) return rc;
+    return dense_rows(c, cs.lease.w, req, dist, nullptr);
+}
+
+int bfg_isochrone_lists(bfg_ctx *c, const uint32_t *seed_offsets, const uint32_t *seed_ranks,
+                        const uint32_t *seed_costs, uint32_t L, const uint32_t *thresholds, uint64_t *reach_bits,
+                        uint64_t *list_start, uint32_t *list_len, uint32_t *pairs, uint64_t capacity,
+                        uint64_t *total) {
+    if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
+    if (L && (!seed_offsets || !thresholds || !list_start || !list_len)) return fail(BFG_INVALID_PARAMETER, "null buffer");
+    if (capacity && !pairs) return fail(BFG_INVALID_PARAMETER, "null pair buffer");
+    if (total) *total = 0;
+    if (L == 0) return BFG_SUCCESS;
+    Req req;
+    int rc = seeded_req(c, req, seed_offsets, seed_ranks, seed_costs, L, thresholds);
+    if (rc) return rc;
+    CallScope cs(c);
+    rc = begin_call(c, cs);
+    if (rc) return rc;
+    return iso_lists(c, cs.lease.w, req, reach_bits, nullptr, list_start, list_len, pairs, nullptr, capacity, total);
+}
+
+int bfg_isochrone_lists_device(bfg_ctx *c, const uint32_t *seed_offsets, const uint32_t *seed_ranks,
+                               const uint32_t *seed_costs, uint32_t L, const uint32_t *thresholds,
+                               uint64_t *d_reach_bits, uint64_t *list_start, uint32_t *list_len, uint32_t *d_pairs,
+                               uint64_t capacity, uint64_t *total) {
+    if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
+    if (L && (!seed_offsets || !thresholds || !list_start || !list_len || !d_reach_bits))
+        return fail(BFG_INVALID_PARAMETER, "null buffer");
+    if (capacity && !d_pairs) return fail(BFG_INVALID_PARAMETER, "null pair buffer");
+    if (total) *total = 0;
+    if (L == 0) return BFG_SUCCESS;
+    Req req;
+    int rc = seeded_req(c, req, seed_offsets, seed_ranks, seed_costs, L, thresholds);
+    if (rc) return rc;
+    CallScope cs(c);
+    rc = begin_call(c, cs);
+    if (rc) return rc;
+    return iso_lists(c, cs.lease.w, req, nullptr, d_reach_bits, list_start, list_len, nullptr, d_pairs, capacity,
+                     total);
 }
 
 int bfg_ctx_attach_length(bfg_ctx *c, const bfg_graph_desc *gd, const uint32_t *up_len, const uint32_t *down_len) {
     if (!c || !gd) return fail(BFG_INVALID_PARAMETER, "null argument");
     if (gd->n_nodes != c->L.n) return fail(BFG_INVALID_PARAMETER, "graph differs from the context's");
-    std::lock_guard<std::mutex> g(c->mu);
+    std::unique_lock<std::shared_mutex> cfg(c->cfg);
     HIP_TRY(hipSetDevice(c->device));
+    WorkLease lease;
+    int rc = acquire_work(c, lease);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(c->t2_mu);
     c->t2.reset();
-    return attach_length(c, gd, up_len, down_len);
+    return attach_length(c, lease.w->stream, gd, up_len, down_len);
 }
 
 int bfg_phast_seeded_2ch(bfg_ctx *c, const uint32_t *seed_offsets, const uint32_t *seed_ranks,
@@ -2567,8 +3456,10 @@ int bfg_phast_seeded_2ch(bfg_ctx *c, const uint32_t *seed_offsets, const uint32_
                          const uint32_t *thresholds, uint32_t *times, uint32_t *lens) {
     if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
     if (L && (!seed_offsets || !thresholds || !times || !lens)) return fail(BFG_INVALID_PARAMETER, "null buffer");
-    std::lock_guard<std::mutex> g(c->mu);
-    HIP_TRY(hipSetDevice(c->device));
+    CallScope cs(c);
+    int rc = begin_call(c, cs);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g2(c->t2_mu);
     if (!c->t2) return fail(BFG_INVALID_PARAMETER, "no length weights attached (bfg_ctx_attach_length)");
     if (L == 0) return BFG_SUCCESS;
     if (seed_offsets[0] != 0) return fail(BFG_INVALID_PARAMETER, "seed_offsets[0] must be 0");
@@ -2576,7 +3467,7 @@ int bfg_phast_seeded_2ch(bfg_ctx *c, const uint32_t *seed_offsets, const uint32_
         if (seed_offsets[i + 1] < seed_offsets[i]) return fail(BFG_INVALID_PARAMETER, "seed_offsets not monotone");
     const uint32_t ns = seed_offsets[L];
     if (ns && (!seed_ranks || !seed_times || !seed_lens)) return fail(BFG_INVALID_PARAMETER, "null seed buffer");
-    int rc = check_ranks(c, seed_ranks, ns, "seed");
+    rc = check_ranks(c, seed_ranks, ns, "seed");
     if (rc) return rc;
     std::vector<uint32_t> soff, spos;
     std::vector<uint64_t> slab;
@@ -2584,9 +3475,9 @@ int bfg_phast_seeded_2ch(bfg_ctx *c, const uint32_t *seed_offsets, const uint32_
     if (rc) return rc;
     TwoChannel &t = *c->t2;
     const uint32_t n = c->L.n;
-    return phast_2ch(c, soff, spos, slab, thresholds, L, true, BottomSet{},
+    hipStream_t s = cs.lease.w->stream;
+    return phast_2ch(c, s, soff, spos, slab, thresholds, L, true, BottomSet{},
                      [&](uint32_t l0, uint32_t bl, uint32_t bg) -> int {
-        hipStream_t s = c->stream;
         t2_out_kernel<<<dim3((n + 63) / 64, bg), NT, 0, s>>>(n, bl, t.pos_of_rank.p, t.lim.p, t.lab.p, t.o_t.p,
                                                             t.o_l.p);
         HIP_TRY(hipGetLastError());
@@ -2604,8 +3495,10 @@ int bfg_table_seeded_2ch(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t
     if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
     if ((S && !src_offsets) || (T && !tgt_offsets) || (S && T && (!times || !lens)))
         return fail(BFG_INVALID_PARAMETER, "null buffer");
-    std::lock_guard<std::mutex> g(c->mu);
-    HIP_TRY(hipSetDevice(c->device));
+    CallScope cs(c);
+    int rc = begin_call(c, cs);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g2(c->t2_mu);
     if (!c->t2) return fail(BFG_INVALID_PARAMETER, "no length weights attached (bfg_ctx_attach_length)");
     if (S == 0 || T == 0) return BFG_SUCCESS;
     if (src_offsets[0] != 0 || tgt_offsets[0] != 0) return fail(BFG_INVALID_PARAMETER, "seed offsets must start at 0");
@@ -2617,7 +3510,7 @@ int bfg_table_seeded_2ch(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t
     if ((ns && (!src_ranks || !src_part_times || !src_part_lens)) ||
         (nt && (!tgt_ranks || !tgt_part_times || !tgt_part_lens)))
         return fail(BFG_INVALID_PARAMETER, "null seed buffer");
-    int rc = check_ranks(c, src_ranks, ns, "source seed");
+    rc = check_ranks(c, src_ranks, ns, "source seed");
     if (!rc) rc = check_ranks(c, tgt_ranks, nt, "target seed");
     if (rc) return rc;
     // one field per source, bounded by threshold + its largest time part + the
@@ -2655,7 +3548,7 @@ int bfg_table_seeded_2ch(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t
         h_soff(src_offsets, src_offsets + S + 1), h_srank(src_ranks, src_ranks + ns);
     DevBuf<uint32_t> d_toff, d_tpos, d_trank, d_tpt, d_tpl, d_soff, d_srank, d_t, d_l, d_bpos;
     DevBuf<uint8_t> d_conf;
-    hipStream_t st = c->stream;
+    hipStream_t st = cs.lease.w->stream;
     const StreamDrain drain{st};  // before the device buffers and host tables above go
     HIP_TRY(d_toff.upload(h_toff, st));
     HIP_TRY(d_tpos.upload(tpos, st));
@@ -2669,7 +3562,7 @@ int bfg_table_seeded_2ch(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t
     HIP_TRY(d_l.alloc((size_t)S * T));
     if (conflicts) HIP_TRY(d_conf.alloc((size_t)S * T));
     HIP_TRY(hipStreamSynchronize(st));  // before the host tables above may go
-    rc = phast_2ch(c, soff, spos, slab, bound.data(), S, false, BottomSet{&boff, d_bpos.p},
+    rc = phast_2ch(c, st, soff, spos, slab, bound.data(), S, false, BottomSet{&boff, d_bpos.p},
                    [&](uint32_t l0, uint32_t bl, uint32_t bg) -> int {
         // this batch's lanes are sources [l0, l0 + bl): offset the per-source tables
         t2_table_kernel<<<dim3((T + 63) / 64, bg), NT, 0, st>>>(
@@ -2692,9 +3585,11 @@ int bfg_table_seeded(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t *sr
     if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
     if ((S && !src_offsets) || (T && !tgt_offsets) || (S && T && !out))
         return fail(BFG_INVALID_PARAMETER, "null buffer");
-    std::lock_guard<std::mutex> g(c->mu);
-    HIP_TRY(hipSetDevice(c->device));
     if (S == 0 || T == 0) return BFG_SUCCESS;
+    CallScope cs(c);
+    int rc = begin_call(c, cs);
+    if (rc) return rc;
+    Work *w = cs.lease.w;
     if (src_offsets[0] != 0 || tgt_offsets[0] != 0) return fail(BFG_INVALID_PARAMETER, "seed offsets must start at 0");
     for (uint32_t i = 0; i < S; ++i) {
         if (src_offsets[i + 1] < src_offsets[i]) return fail(BFG_INVALID_PARAMETER, "src_offsets not monotone");
@@ -2705,7 +3600,7 @@ int bfg_table_seeded(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t *sr
     const uint32_t ns = src_offsets[S], nt = tgt_offsets[T];
     if ((ns && (!src_ranks || !src_parts)) || (nt && (!tgt_ranks || !tgt_parts)))
         return fail(BFG_INVALID_PARAMETER, "null seed buffer");
-    int rc = check_ranks(c, src_ranks, ns, "source seed");
+    rc = check_ranks(c, src_ranks, ns, "source seed");
     if (!rc) rc = check_ranks(c, tgt_ranks, nt, "target seed");
     if (rc) return rc;
     // one field per source, bounded by threshold + its largest part + the
@@ -2733,14 +3628,14 @@ int bfg_table_seeded(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t *sr
         if (conflicts) std::memset(conflicts, 0, (size_t)S * T);
         return BFG_SUCCESS;
     }
-    rc = stage_target_paths(c, U.data(), (uint32_t)U.size());
+    rc = stage_target_paths(c, w, U.data(), (uint32_t)U.size());
     if (rc) return rc;
     std::vector<uint32_t> tlab(nt);
     for (uint32_t q = 0; q < nt; ++q)
-        tlab[q] = c->tgt_pos_h[std::lower_bound(U.begin(), U.end(), tgt_ranks[q]) - U.begin()];
+        tlab[q] = w->tgt_pos_h[std::lower_bound(U.begin(), U.end(), tgt_ranks[q]) - U.begin()];
     DevBuf<uint32_t> d_toff, d_tlab, d_trank, d_tpart, d_out;
     DevBuf<uint8_t> d_conf;
-    hipStream_t st = c->stream;
+    hipStream_t st = w->stream;
     const std::vector<uint32_t> h_toff(tgt_offsets, tgt_offsets + T + 1), h_trank(tgt_ranks, tgt_ranks + nt),
         h_tpart(tgt_parts, tgt_parts + nt);
     const StreamDrain drain{st};
@@ -2751,37 +3646,33 @@ int bfg_table_seeded(bfg_ctx *c, const uint32_t *src_offsets, const uint32_t *sr
     HIP_TRY(d_out.alloc((size_t)S * T));
     if (conflicts) HIP_TRY(d_conf.alloc((size_t)S * T));
     HIP_TRY(hipStreamSynchronize(st));  // before the host tables above may go
-    struct PathsOn {  // restrict the bottom sweep for this call only
-        bfg_ctx *c;
-        explicit PathsOn(bfg_ctx *cc) : c(cc) { c->paths_on = true; }
-        ~PathsOn() { c->paths_on = false; }
-    } paths_on(c);
-    rc = run_all(c, req, [&](uint32_t, uint32_t nl, const std::vector<uint32_t> &, float *oms) -> int {
+    PathsOn paths_on(w);  // restrict the bottom sweep for this call only
+    rc = run_all(c, w, req, 0, [&](uint32_t, uint32_t nl, const std::vector<uint32_t> &, float *oms) -> int {
         const uint32_t groups = (nl + c->lanes - 1) / c->lanes;
         const dim3 grid((T + 63) / 64, groups);
-        HIP_TRY(hipEventRecord(c->ev[0], st));
+        HIP_TRY(hipEventRecord(w->ev[0], st));
         switch (c->lanes) {
             case 16:
-                hipLaunchKernelGGL(table_gather_kernel<16>, grid, dim3(NT), 0, st, c->arena.p, label_stride(c), T,
-                                   d_toff.p, d_tlab.p, d_trank.p, d_tpart.p, c->seed_off.p, c->seed_rank.p,
-                                   c->lane_limit.p, nl, c->row_map.p, threshold, d_out.p, d_conf.p);
+                hipLaunchKernelGGL(table_gather_kernel<16>, grid, dim3(NT), 0, st, w->arena.p, label_stride(c, w), T,
+                                   d_toff.p, d_tlab.p, d_trank.p, d_tpart.p, w->seed_off.p, w->seed_rank.p,
+                                   w->lane_limit.p, nl, w->row_map.p, threshold, d_out.p, d_conf.p);
                 break;
             case 64:
-                hipLaunchKernelGGL(table_gather_kernel<64>, grid, dim3(NT), 0, st, c->arena.p, label_stride(c), T,
-                                   d_toff.p, d_tlab.p, d_trank.p, d_tpart.p, c->seed_off.p, c->seed_rank.p,
-                                   c->lane_limit.p, nl, c->row_map.p, threshold, d_out.p, d_conf.p);
+                hipLaunchKernelGGL(table_gather_kernel<64>, grid, dim3(NT), 0, st, w->arena.p, label_stride(c, w), T,
+                                   d_toff.p, d_tlab.p, d_trank.p, d_tpart.p, w->seed_off.p, w->seed_rank.p,
+                                   w->lane_limit.p, nl, w->row_map.p, threshold, d_out.p, d_conf.p);
                 break;
             default:
-                hipLaunchKernelGGL(table_gather_kernel<32>, grid, dim3(NT), 0, st, c->arena.p, label_stride(c), T,
-                                   d_toff.p, d_tlab.p, d_trank.p, d_tpart.p, c->seed_off.p, c->seed_rank.p,
-                                   c->lane_limit.p, nl, c->row_map.p, threshold, d_out.p, d_conf.p);
+                hipLaunchKernelGGL(table_gather_kernel<32>, grid, dim3(NT), 0, st, w->arena.p, label_stride(c, w), T,
+                                   d_toff.p, d_tlab.p, d_trank.p, d_tpart.p, w->seed_off.p, w->seed_rank.p,
+                                   w->lane_limit.p, nl, w->row_map.p, threshold, d_out.p, d_conf.p);
                 break;
         }
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(c->ev[1], st));
-        HIP_TRY(hipEventSynchronize(c->ev[1]));
+        HIP_TRY(hipEventRecord(w->ev[1], st));
+        HIP_TRY(hipEventSynchronize(w->ev[1]));
         float a = 0;
-        HIP_TRY(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&a, w->ev[0], w->ev[1]));
         *oms += a;
         return BFG_SUCCESS;
     });

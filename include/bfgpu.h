@@ -14,8 +14,11 @@
  *     place; nothing returned needs freeing except the context;
  *   - nodes are CCH ranks (rank-aligned CCH: node id == rank), distances are
  *     u32 seconds, 0xFFFFFFFF (BFG_INF) means unreachable / beyond the limit;
- *   - calls on one context are serialised internally; distinct contexts may be
- *     used concurrently from different threads.
+ *   - a context takes calls from any number of threads: up to its concurrency
+ *     (bfg_ctx_set_concurrency, default 4 or the environment variable
+ *     BFG_STREAMS) run at once on the device, each on its own HIP stream with
+ *     its own label arena; further calls wait for one to finish; the
+ *     two-channel calls run one at a time per context.
  * Results are byte-identical to the reference CPU PHAST on the same CCH:
  * exact shortest up-down distances with saturating u32 adds; for bounded
  * calls every value <= the limit is exact and every other value is BFG_INF.
@@ -37,7 +40,8 @@ typedef enum bfg_result {
     BFG_OUT_OF_MEMORY = 2,     /* device allocation failed                     */
     BFG_DEVICE_ERROR = 3,      /* HIP runtime or kernel failure                */
     BFG_NOT_SUPPORTED = 4,     /* valid request outside this build's limits    */
-    BFG_INTERNAL_ERROR = 5
+    BFG_INTERNAL_ERROR = 5,
+    BFG_BUFFER_TOO_SMALL = 6   /* an output buffer holds less than the result; the sizes are reported */
 } bfg_result;
 
 /* The customized CCH as the reference holds it: CchTopo up/down CSR
@@ -87,6 +91,10 @@ typedef struct bfg_timings {
     float dom_ms;
     uint32_t dom_launches;
     uint64_t dom_bytes;
+    /* the block gate of bounded calls (0 without one), summed over batches:
+     * (segment, 64-lane wave group) units of the top region and how many the
+     * gate left open, and the same for the bottom trees */
+    uint64_t gate_seg_units, gate_seg_open, gate_bt_units, gate_bt_open;
 } bfg_timings;
 
 /* BatchedPhastEngine::new (batched_phast.rs:89): copies the CCH to `device`,
@@ -96,12 +104,17 @@ typedef struct bfg_timings {
 int bfg_ctx_create(const bfg_graph_desc *graph, int device, bfg_ctx **out);
 void bfg_ctx_destroy(bfg_ctx *ctx);
 const char *bfg_last_error(void);
-int bfg_ctx_get_info(const bfg_ctx *ctx, bfg_ctx_info *info);
-int bfg_ctx_last_timings(const bfg_ctx *ctx, bfg_timings *t);
+int bfg_ctx_get_info(bfg_ctx *ctx, bfg_ctx_info *info);
+int bfg_ctx_last_timings(bfg_ctx *ctx, bfg_timings *t);
 /* Lane width K in {16, 32, 64} and the batch capacity in lanes (0 = auto;
  * rounded up to a multiple of 64).
  * Reallocates the arenas; used by the autotuner. */
 int bfg_ctx_configure(bfg_ctx *ctx, uint32_t lanes, uint32_t lane_capacity);
+
+/* Calls of one context that may run on the device at once (1..16), each on
+ * its own stream and label arena (no counterpart in the reference, whose
+ * engines are per-thread; the serving loop overlaps its rounds with it). */
+int bfg_ctx_set_concurrency(bfg_ctx *ctx, uint32_t max_calls);
 
 /* Tuning of the downward sweep (no counterpart in the reference; set by the
  * autotuner, bfgpu/autotune.py): plan_rows = the most rows one DOWN unit
@@ -135,6 +148,18 @@ int bfg_matrix_device(bfg_ctx *ctx, const uint32_t *d_sources, uint32_t n_src,
                       const uint32_t *d_targets, uint32_t n_tgt, uint32_t threshold,
                       uint32_t *d_out);
 
+/* Several compute_matrix_block_gated requests (batched_phast.rs:1223-1265) as
+ * one lane batch (no counterpart in the reference, which serves each request
+ * with its own engine call): request r has sources sources[src_offsets[r] ..
+ * src_offsets[r+1]), targets targets[tgt_offsets[r] .. tgt_offsets[r+1]) and
+ * threshold thresholds[r] (BFG_INF: unbounded); its S_r x T_r row-major
+ * block is written to out from sum_{q < r} S_q * T_q on.  Every block equals
+ * the request's own bfg_matrix_bounded result: the sources of all requests
+ * share one downward sweep, and each request gathers only its own columns. */
+int bfg_matrix_multi(bfg_ctx *ctx, uint32_t n_req, const uint32_t *src_offsets, const uint32_t *sources,
+                     const uint32_t *tgt_offsets, const uint32_t *targets, const uint32_t *thresholds,
+                     uint32_t *out);
+
 /* BatchedPhastEngine::query_batch / query_batch_soa (batched_phast.rs:131-260,
  * :470-609) for any number of sources: dist[i * n_nodes + v] = d(sources[i], v).
  * With threshold != BFG_INF this is query_batch_bounded / query_batch_block_gated
@@ -148,6 +173,32 @@ int bfg_query_batch(bfg_ctx *ctx, const uint32_t *sources, uint32_t n_src,
  * times (nullable) [i * n_nodes + v] = that distance, BFG_INF otherwise. */
 int bfg_isochrone(bfg_ctx *ctx, const uint32_t *origins, const uint32_t *limits,
                   uint32_t n_lanes, uint64_t *reach_bits, uint32_t *times);
+
+/* run_phast_bounded_fast_seeded (isochrone_handler.rs:338-491) for n_lanes
+ * independent queries, with the reference's output: lane i's settled nodes
+ * (rank, dist <= thresholds[i]) in ascending rank (:453-471), as pairs
+ * pairs[2k] = rank, pairs[2k+1] = dist for k in [list_start[i],
+ * list_start[i] + list_len[i]).  Seeds as in bfg_phast_seeded.  reach_bits
+ * (nullable) as in bfg_isochrone.  `capacity` counts pairs; *total (nullable)
+ * receives the pairs of all lanes.  When they exceed capacity the call
+ * returns BFG_BUFFER_TOO_SMALL with list_start, list_len and *total filled
+ * and writes no pair past capacity; a second call with capacity >= *total
+ * succeeds.  The downward sweep is block-gated: a (segment or bottom tree,
+ * 64-lane wave group) whose lanes cannot settle a node within their limits is
+ * skipped. */
+int bfg_isochrone_lists(bfg_ctx *ctx, const uint32_t *seed_offsets, const uint32_t *seed_ranks,
+                        const uint32_t *seed_costs, uint32_t n_lanes, const uint32_t *thresholds,
+                        uint64_t *reach_bits, uint64_t *list_start, uint32_t *list_len, uint32_t *pairs,
+                        uint64_t capacity, uint64_t *total);
+
+/* Device-resident variant of bfg_isochrone_lists (no counterpart in the
+ * reference): d_reach_bits (n_lanes * ceil(n/64) words) and d_pairs
+ * (capacity pairs) are HIP device pointers; seeds, thresholds, list_start,
+ * list_len and total stay in host memory. */
+int bfg_isochrone_lists_device(bfg_ctx *ctx, const uint32_t *seed_offsets, const uint32_t *seed_ranks,
+                               const uint32_t *seed_costs, uint32_t n_lanes, const uint32_t *thresholds,
+                               uint64_t *d_reach_bits, uint64_t *list_start, uint32_t *list_len, uint32_t *d_pairs,
+                               uint64_t capacity, uint64_t *total);
 
 /* run_phast_bounded_fast_seeded (isochrone_handler.rs:338-491) for n_lanes
  * independent queries: lane i has seeds (seed_ranks[k], seed_costs[k]) for

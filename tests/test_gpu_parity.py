@@ -325,3 +325,153 @@ def test_autotune_and_tuning_keep_results(engine_cls, grid_config1):
         e.tune(plan_rows=64, tri_scalar_segs=0)
         got, _ = e.compute_matrix_flat(s, t)
         assert np.array_equal(got, exp)
+
+
+def test_isochrone_lists_against_seeded_oracle(engine_cls, grid_config1):
+    """bfg_isochrone_lists: per lane the settled (rank, dist <= limit) pairs in
+    ascending rank, exactly run_phast_bounded_fast_seeded's output, for
+    single- and multi-seed lanes and limits from 0 to INF; the reach bits
+    mark exactly the listed ranks."""
+    g = grid_config1
+    o = Oracle(g)
+    rng = np.random.default_rng(31)
+    L = 150
+    lanes = [[(int(r), int(c))] for r, c in zip(rng.integers(0, g.n, L), rng.integers(0, 50, L))]
+    for i in range(0, L, 7):
+        lanes[i] = lanes[i] + [(int(rng.integers(0, g.n)), int(rng.integers(0, 200)))]
+    thr = rng.choice(np.array([0, 60, 300, 900, 1800, 3600, INF], dtype=np.uint32), L)
+    say("isochrone lists", L)
+    with engine_cls(g) as e:
+        lists, bits = e.isochrone_lists(lanes, thr, want_bits=True)
+        small = e.isochrone_lists(lanes[:10], thr[:10], capacity=5)  # too small: counted, then rerun
+    words = (g.n + 63) // 64
+    for i in range(L):
+        exp = o.seeded(lanes[i], int(thr[i]))
+        assert lists[i].tolist() == [list(x) for x in exp], i
+        inside = np.zeros(words * 64, dtype=bool)
+        inside[lists[i][:, 0]] = True
+        assert np.array_equal(bits[i], np.packbits(inside, bitorder="little").view(np.uint64))
+    for i in range(10):
+        assert np.array_equal(small[i], lists[i])
+
+
+def test_isochrone_lists_buffer_too_small_reports_sizes(engine_cls, grid_small):
+    """A pair buffer shorter than the result: BFG_BUFFER_TOO_SMALL with the
+    sizes filled and nothing written past the capacity."""
+    import ctypes as C
+    from bfgpu import _native
+    from bfgpu.engine import BFG_BUFFER_TOO_SMALL
+    g = grid_small
+    rng = np.random.default_rng(32)
+    L = 12
+    org = rng.integers(0, g.n, L).astype(np.uint32)
+    off = np.arange(L + 1, dtype=np.uint32)
+    cost = np.zeros(L, dtype=np.uint32)
+    thr = np.full(L, 600, dtype=np.uint32)
+    start = np.zeros(L, dtype=np.uint64)
+    length = np.zeros(L, dtype=np.uint32)
+    total = np.zeros(1, dtype=np.uint64)
+    pairs = np.full(2 * 8 + 2, 0xABCDEF, dtype=np.uint32)
+    P = _native.ptr
+    with engine_cls(g) as e:
+        rc = e._lib.bfg_isochrone_lists(e._h, P(off), P(org), P(cost), L, P(thr), None, P(start, _native.u64p),
+                                        P(length), P(pairs), 8, P(total, _native.u64p))
+        want = e.isochrone_lists([[(int(r), 0)] for r in org], thr)
+    assert rc == BFG_BUFFER_TOO_SMALL
+    assert int(total[0]) == sum(len(x) for x in want) > 8
+    assert length.tolist() == [len(x) for x in want]
+    assert (pairs[16:] == 0xABCDEF).all()
+
+
+def test_concurrent_calls_equal_sequential(engine_cls, grid_config1):
+    """Calls from several threads on one context run at once (each on its own
+    stream and arena) and give the same results as one after another."""
+    import threading
+    g = grid_config1
+    rng = np.random.default_rng(33)
+    jobs = []
+    for k in range(12):
+        if k % 3 == 0:
+            jobs.append(("iso", rng.integers(0, g.n, 40).astype(np.uint32),
+                         rng.choice(np.array([300, 900, 1800], dtype=np.uint32), 40)))
+        else:
+            jobs.append(("mat", rng.integers(0, g.n, int(rng.integers(5, 200))).astype(np.uint32),
+                         rng.integers(0, g.n, int(rng.integers(5, 300))).astype(np.uint32)))
+    say("concurrent calls", len(jobs))
+    with engine_cls(g) as e:
+        e.set_concurrency(4)
+
+        def run(j):
+            kind, a, b = j
+            if kind == "iso":
+                return e.isochrones(a, b, want_times=False)[0]
+            return e.compute_matrix_flat(a, b)[0]
+        want = [run(j) for j in jobs]
+        got = [None] * len(jobs)
+        errs = []
+
+        def worker(idx):
+            try:
+                for i in idx:
+                    got[i] = run(jobs[i])
+            except Exception as ex:  # surfaced below
+                errs.append(ex)
+        ts = [threading.Thread(target=worker, args=(list(range(t, len(jobs), 4)),)) for t in range(4)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+    assert not errs, errs
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+
+
+def test_gate_skips_and_stays_exact(engine_cls, grid_config1):
+    """Short limits: the block gate leaves most (segment, wave group) units and
+    bottom trees closed, and every lane still equals the oracle's seeded
+    PHAST; a wave group holding one unbounded lane keeps everything open."""
+    g = grid_config1
+    o = Oracle(g)
+    rng = np.random.default_rng(34)
+    L = 192
+    org = rng.integers(0, g.n, L)
+    lanes = [[(int(r), 0)] for r in org]
+    thr = np.full(L, 120, dtype=np.uint32)
+    thr[150] = INF  # the third wave group holds an unbounded lane
+    say("gate", L)
+    with engine_cls(g) as e:
+        lists = e.isochrone_lists(lanes, thr)
+        st = e.last_stats(L)
+    say("gate open", st.gate_seg_open, "of", st.gate_seg_units, "segments;", st.gate_bt_open, "of",
+        st.gate_bt_units, "trees")
+    nseg = st.gate_seg_units // 3
+    assert st.gate_seg_open < 0.5 * 2 * nseg + nseg  # two wave groups mostly closed, one open
+    assert st.gate_bt_open < st.gate_bt_units
+    for i in range(L):
+        assert lists[i].tolist() == [list(x) for x in o.seeded(lanes[i], int(thr[i]))], i
+
+
+def test_matrix_multi_equals_requests_alone(engine_cls, grid_config1):
+    """bfg_matrix_multi: requests of uneven shapes and thresholds share one
+    sweep; each block equals the request issued alone and the oracle, also
+    when the lanes span several batches."""
+    g = grid_config1
+    o = Oracle(g)
+    rng = np.random.default_rng(35)
+    reqs = []
+    for k in range(9):
+        S, T = int(rng.integers(1, 90)), int(rng.integers(1, 120))
+        thr = [INF, 900, 2400][k % 3]
+        reqs.append((rng.integers(0, g.n, S).astype(np.uint32), rng.integers(0, g.n, T).astype(np.uint32), thr))
+    reqs.append((reqs[0][0][:3], reqs[0][1], INF))  # a request that repeats another's targets
+    say("matrix multi", len(reqs))
+    with engine_cls(g) as e:
+        got = e.compute_matrix_multi(reqs)
+        alone = [e.compute_matrix_block_gated(s, t, thr)[0] for s, t, thr in reqs]
+        e.configure(lane_capacity=128)  # several batches; requests cross batch boundaries
+        got2 = e.compute_matrix_multi(reqs)
+        e.configure(lane_capacity=0)
+    for (s, t, thr), a, b, c in zip(reqs, got, alone, got2):
+        assert np.array_equal(a, b)
+        assert np.array_equal(c, b)
+        assert np.array_equal(a, mask(o.compute_matrix_flat_mt(s, t), thr))
