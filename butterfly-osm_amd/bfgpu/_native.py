@@ -70,6 +70,8 @@ BFGPU_SYMBOLS = {
     "bfg_matrix": [C.c_void_p, u32p, C.c_uint32, u32p, C.c_uint32, u32p],
     "bfg_matrix_bounded": [C.c_void_p, u32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32, u32p],
     "bfg_matrix_device": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p],
+    "bfg_matrix_device_noalloc": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32,
+                                  C.c_void_p],
     "bfg_matrix_multi": [C.c_void_p, C.c_uint32, u32p, u32p, u32p, u32p, u32p, u32p],
     "bfg_query_batch": [C.c_void_p, u32p, C.c_uint32, C.c_uint32, u32p],
     "bfg_isochrone": [C.c_void_p, u32p, u32p, C.c_uint32, u64p, u32p],

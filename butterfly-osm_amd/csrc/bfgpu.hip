@@ -2997,6 +2997,99 @@ int gate_tables(const bfg::Layout &L, GateTables &G) {
     return BFG_SUCCESS;
 }
 
+// ---------------------------------------------------------------------------
+// The allocation-free matrix call.  Its device memory is one file-scope
+// array per GPU, `scratch` (SCRATCH_BYTES), which every context on that GPU
+// carves the same way for its own graph.  Its address on a device comes from
+// hipGetSymbolAddress, once per device, at context creation with that device
+// current; host code uses only that address, never the array's name.  Calls
+// on one GPU are serialised by a per-device lock; each call leaves its own
+// contents behind and clears or overwrites everything it reads.
+// ---------------------------------------------------------------------------
+constexpr size_t SCRATCH_BYTES = (size_t)512 << 20;
+constexpr uint32_t NOALLOC_LANES = 64;               // one wave group
+constexpr uint32_t NOALLOC_MAX_TARGETS = 1u << 18;
+constexpr uint32_t NOALLOC_MAX_PATH = 1u << 21;      // bottom path elements above the targets
+constexpr int MAX_DEVICES = 64;
+
+static __device__ __align__(256) uint32_t scratch[SCRATCH_BYTES / 4];
+
+std::mutex g_scratch_init_mu;
+uint32_t *g_scratch_ptr[MAX_DEVICES] = {};
+std::mutex g_scratch_mu[MAX_DEVICES];
+
+// The scratch array's device address on `device` (current), looked up once.
+int scratch_setup(int device, uint32_t **out) {
+    *out = nullptr;
+    if (device >= MAX_DEVICES) return BFG_SUCCESS;  // the allocation-free call is then unavailable
+    std::lock_guard<std::mutex> g(g_scratch_init_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return fail(BFG_DEVICE_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = (uint32_t *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return BFG_SUCCESS;
+}
+
+// The scratch Work of a context: its buffers borrowed from the array at
+// `base`, the label arena taking what the fixed tables leave; its host
+// staging reserved to the call's limits.  Leaves scratch_work null when the
+// graph's tables do not fit (the call then reports it).
+int scratch_work_init(bfg_ctx *c, uint32_t *base) {
+    if (!base) return BFG_SUCCESS;
+    const size_t nseg = c->L.seg_pos.size() - 1, D1 = d1_of(c);
+    std::unique_ptr<Work> w(new Work());
+    int rc = work_init(w.get());
+    if (rc) return rc;
+    w->fixed = true;
+    w->time_dom = false;
+    size_t at = 0;                          // in u32 words, each piece 256-byte aligned
+    const size_t total = SCRATCH_BYTES / 4;
+    bool fits = true;
+    auto take = [&](size_t words) -> uint32_t * {
+        at = (at + 63) / 64 * 64;
+        if (at + words > total) {
+            fits = false;
+            return base;
+        }
+        uint32_t *p = base + at;
+        at += words;
+        return p;
+    };
+    auto take32 = [&](DevBuf<uint32_t> &b, size_t count) { b.borrow(take(count), count); };
+    auto take8 = [&](DevBuf<uint8_t> &b, size_t count) { b.borrow((uint8_t *)take((count + 3) / 4), count); };
+    take32(w->seed_off, NOALLOC_LANES + 1);
+    take32(w->lane_limit, NOALLOC_LANES);
+    take32(w->row_map, NOALLOC_LANES);
+    for (DevBuf<uint32_t> *b : {&w->seed_rank, &w->seed_pos, &w->seed_cost, &w->seed_depth, &w->seed_pre})
+        take32(*b, NOALLOC_LANES);
+    take32(w->pathbuf, NOALLOC_LANES * D1);
+    take32(w->tgt_pos, NOALLOC_MAX_TARGETS);
+    take32(w->path_off, NOALLOC_MAX_TARGETS + 1);
+    take32(w->path_pos, NOALLOC_MAX_PATH);
+    // gate flags of bounded calls: one wave group, at most 64 / 16 lane groups
+    take8(w->seg_live, std::max<size_t>(nseg, 1));
+    take8(w->chunk_ok, 4 * std::max<size_t>(c->L.ch_off.size() - 1, 1));
+    take8(w->open_b, 4 * (size_t)std::max<uint32_t>(c->n_bt, 1));
+    at = (at + 63) / 64 * 64;
+    if (!fits || at >= total) return BFG_SUCCESS;
+    w->arena.borrow(base + at, total - at);
+    // host staging: capacity for the largest call, so the call never grows it
+    for (auto *v : {&w->h_soff, &w->h_lim, &w->h_rows, &w->h_srank, &w->h_scost, &w->h_sdepth, &w->h_spre,
+                    &w->h_spos, &w->h_src, &w->req.soff, &w->req.srank, &w->req.scost, &w->req.limit})
+        v->reserve(NOALLOC_LANES + 1);
+    w->h_key.reserve(NOALLOC_LANES);
+    for (auto *v : {&w->h_tgt, &w->h_bt, &w->h_pending, &w->path_off_h, &w->path_leaf, &w->tgt_pos_h})
+        v->reserve(NOALLOC_MAX_TARGETS + 1);
+    w->h_elem.reserve(NOALLOC_MAX_TARGETS);
+    w->path_pos_h.reserve(NOALLOC_MAX_PATH);
+    c->scratch_work = std::move(w);
+    return BFG_SUCCESS;
+}
+
 int matrix_impl(bfg_ctx *c, Work *w, Req &req, const uint32_t *src, uint32_t S, const uint32_t *tgt, uint32_t T,
                 uint32_t thr, uint32_t *d_out, uint32_t *host_out) {
     single_source_req(req, src, S, thr);
@@ -3190,11 +3283,16 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
         std::vector<uint32_t>().swap(*v);
     std::vector<uint16_t>().swap(L.ca_v);
     std::vector<uint16_t>().swap(L.in_dd);
-    // the one-wave-group DOWN plan (kept for calls that must not build one)
+    // the one-wave-group DOWN plan and the scratch Work of the allocation-free call
     std::shared_ptr<Plan> p1(new Plan());
     rc = build_plan(c.get(), s, 1, *p1);
     if (rc) return rc;
     c->plan1 = std::move(p1);
+    uint32_t *sp = nullptr;
+    rc = scratch_setup(device, &sp);
+    if (rc) return rc;
+    rc = scratch_work_init(c.get(), sp);
+    if (rc) return rc;
     lease.w = nullptr;  // hand the first Work to the pool by hand: the context is about to be released
     c->idle.push_back(c->works.front().get());
     *out = c.release();
@@ -3354,6 +3452,34 @@ int bfg_matrix_multi(bfg_ctx *c, uint32_t n_req, const uint32_t *src_offsets, co
     if (rc) return rc;
     HIP_TRY(hipMemcpy(out, d_out.p, cells * 4, hipMemcpyDeviceToHost));
     return BFG_SUCCESS;
+}
+
+int bfg_matrix_device_noalloc(bfg_ctx *c, const uint32_t *d_src, uint32_t S, const uint32_t *d_tgt, uint32_t T,
+                              uint32_t thr, uint32_t *d_out) {
+    if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
+    if ((S && !d_src) || (T && !d_tgt) || (S && T && !d_out)) return fail(BFG_INVALID_PARAMETER, "null buffer");
+    if (S > NOALLOC_LANES)
+        return fail(BFG_NOT_SUPPORTED, "the allocation-free matrix takes at most 64 sources per call");
+    if (T > NOALLOC_MAX_TARGETS)
+        return fail(BFG_NOT_SUPPORTED, "the allocation-free matrix takes at most 262144 targets per call");
+    if (S == 0 || T == 0) return BFG_SUCCESS;
+    std::shared_lock<std::shared_mutex> cfg(c->cfg);
+    HIP_TRY(hipSetDevice(c->device));
+    if (!c->scratch_work)
+        return fail(BFG_NOT_SUPPORTED, "the graph's tables do not fit the scratch array of the allocation-free call");
+    std::lock_guard<std::mutex> g(g_scratch_mu[c->device]);
+    Work *w = c->scratch_work.get();
+    w->h_src.resize(S);
+    w->h_tgt.resize(T);
+    HIP_TRY(hipMemcpy(w->h_src.data(), d_src, (size_t)S * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(w->h_tgt.data(), d_tgt, (size_t)T * 4, hipMemcpyDeviceToHost));
+    int rc = check_ranks(c, w->h_src.data(), S, "source");
+    if (!rc) rc = check_ranks(c, w->h_tgt.data(), T, "target");
+    if (rc) return rc;
+    rc = matrix_impl(c, w, w->req, w->h_src.data(), S, w->h_tgt.data(), T, thr, d_out, nullptr);
+    const hipError_t e = hipStreamSynchronize(w->stream);  // the next call may reuse the array
+    if (!rc && e != hipSuccess) return fail(BFG_DEVICE_ERROR, hipGetErrorString(e));
+    return rc;
 }
 
 int bfg_query_batch(bfg_ctx *c, const uint32_t *src, uint32_t S, uint32_t thr, uint32_t *dist) {

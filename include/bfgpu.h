@@ -160,6 +160,20 @@ int bfg_matrix_multi(bfg_ctx *ctx, uint32_t n_req, const uint32_t *src_offsets, 
                      const uint32_t *tgt_offsets, const uint32_t *targets, const uint32_t *thresholds,
                      uint32_t *out);
 
+/* bfg_matrix_device for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The call allocates nothing:
+ * its device state lives in one static array of 512 MiB per GPU, which every
+ * process using this library holds on each GPU it runs on until it exits.
+ * Limits: at most 64 sources and 262144 targets per call, and the graph's
+ * top region plus the bottom paths above the targets must fit the array
+ * (64 lanes x about 2M labels); larger requests get BFG_NOT_SUPPORTED.
+ * Calls on one GPU are serialised inside the library (they share the
+ * array, whose contents each call overwrites; calls from other processes on
+ * the same GPU have arrays of their own). */
+int bfg_matrix_device_noalloc(bfg_ctx *ctx, const uint32_t *d_sources, uint32_t n_src, const uint32_t *d_targets,
+                              uint32_t n_tgt, uint32_t threshold, uint32_t *d_out);
+
 /* BatchedPhastEngine::query_batch / query_batch_soa (batched_phast.rs:131-260,
  * :470-609) for any number of sources: dist[i * n_nodes + v] = d(sources[i], v).
  * With threshold != BFG_INF this is query_batch_bounded / query_batch_block_gated
