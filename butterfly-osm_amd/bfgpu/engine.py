@@ -118,6 +118,12 @@ class BatchedPhastEngine:
     def configure(self, lanes=0, lane_capacity=0):
         _check(self._lib.bfg_ctx_configure(self._h, lanes, lane_capacity))
 
+    def free_device_bytes(self):
+        """Free memory of the engine's device (hipMemGetInfo)."""
+        from . import hip
+        hip.hip().hipSetDevice(self._device)
+        return hip.mem_info()[0]
+
     def set_concurrency(self, max_calls):
         """Calls that may run on the device at once, each on its own stream
         and label arena (bfg_ctx_set_concurrency)."""

@@ -2,19 +2,22 @@
 and isochrone requests.
 
 The graph and the label arenas stay on the GPU for the life of the engine.
-Requests arrive on any thread (submit returns a Future).  One worker thread
-runs rounds; a round admits the oldest pending requests while they fit a lane
-budget and a device-memory budget, coalesces them and runs them as shared
-sweeps:
-- every admitted matrix request's sources become lanes of one matrix sweep
-  (per threshold) over the union of their targets, and each request reads
-  back its own rows and columns;
-- every admitted isochrone becomes one lane of one isochrone sweep with its
-  own limit.
-Requests that arrive while a round is on the GPU are admitted by the next
-round, which starts as soon as the device is free.  A cell or a reach bit
-does not depend on what else shares its sweep (labels are exact distances),
-so every result is bit-identical to the same request issued alone.
+Requests arrive on any thread (submit returns a Future).  The loop has
+`slots` workers; each runs one round at a time, and the engine runs each
+worker's calls on a stream and label arena of their own, so up to `slots`
+rounds overlap on the device.  Whenever a slot is free and requests wait, it
+admits the oldest pending requests while their lanes fit `max_lanes` and
+their estimated device memory fits what the rounds in flight leave of the
+budget (a fraction of the device's free memory when the loop starts), and
+runs them as one round:
+- every matrix request's sources become lanes of one sweep
+  (bfg_matrix_multi), each request gathering only its own rows and columns;
+- every isochrone becomes one lane of one block-gated sweep with its own
+  limit (bfg_isochrone_lists), answered with its settled (rank, time) list.
+Requests that arrive while rounds are on the GPU are admitted as soon as a
+slot and memory free up.  A cell or a settled node does not depend on what
+else shares its sweep (labels are exact distances), so every result is
+bit-identical to the same request issued alone.
 
 PlainLoop serves the same stream the plain way, one GPU call per request in
 arrival order; the serving benchmark (bench.py --serving) compares the two.
@@ -43,7 +46,8 @@ class MatrixRequest:
 
 @dataclasses.dataclass
 class IsochroneRequest:
-    """One block-gated isochrone: reach bits of the nodes within limit."""
+    """One isochrone: the settled (rank, time <= limit) pairs in ascending rank
+    (run_phast_bounded_fast_seeded's output)."""
     origin: int
     limit: int
 
@@ -53,52 +57,60 @@ class IsochroneRequest:
 
 def run_round(engine, reqs):
     """Results of one round of requests, in the order given: a row-major S x T
-    u32 array per matrix request, a ceil(n/64) u64 bit row per isochrone."""
+    u32 array per matrix request, an (m, 2) u32 array of (rank, time) per
+    isochrone."""
     out = [None] * len(reqs)
-    by_thr = collections.defaultdict(list)
-    isos = []
-    for i, r in enumerate(reqs):
-        if isinstance(r, MatrixRequest):
-            by_thr[int(r.threshold)].append(i)
-        else:
-            isos.append(i)
-    for thr, idx in by_thr.items():
-        srcs = [np.asarray(reqs[i].sources, dtype=np.uint32) for i in idx]
-        tgts = [np.asarray(reqs[i].targets, dtype=np.uint32) for i in idx]
-        S = np.concatenate(srcs)
-        T, inv = np.unique(np.concatenate(tgts), return_inverse=True)
-        m, _ = engine.compute_matrix_block_gated(S, T, thr)
-        m = m.reshape(S.size, T.size)
-        r0 = c0 = 0
-        for i, s, t in zip(idx, srcs, tgts):
-            out[i] = m[r0:r0 + s.size][:, inv[c0:c0 + t.size]].reshape(-1).copy()
-            r0 += s.size
-            c0 += t.size
+    mats = [i for i, r in enumerate(reqs) if isinstance(r, MatrixRequest)]
+    isos = [i for i, r in enumerate(reqs) if not isinstance(r, MatrixRequest)]
+    if mats:
+        res = engine.compute_matrix_multi([(reqs[i].sources, reqs[i].targets, reqs[i].threshold) for i in mats])
+        for i, m in zip(mats, res):
+            out[i] = m
     if isos:
-        org = np.array([reqs[i].origin for i in isos], dtype=np.uint32)
-        lim = np.array([reqs[i].limit for i in isos], dtype=np.uint32)
-        bits, _ = engine.isochrones(org, lim, want_times=False)
-        for k, i in enumerate(isos):
-            out[i] = bits[k].copy()
+        lists = engine.isochrone_lists([[(int(reqs[i].origin), 0)] for i in isos],
+                                       np.array([reqs[i].limit for i in isos], dtype=np.uint32))
+        for i, lst in zip(isos, lists):
+            out[i] = lst
     return out
 
 
-def request_bytes(req, n_nodes):
-    """Device output bytes a request adds to a round (the admission measure)."""
+def run_plain(engine, req):
+    """One request as its own plain engine call (the per-request baseline)."""
     if isinstance(req, MatrixRequest):
-        return 4 * len(req.sources) * len(req.targets)
-    return 8 * ((n_nodes + 63) // 64)
+        return engine.compute_matrix_block_gated(req.sources, req.targets, req.threshold)[0]
+    return engine.isochrone_lists([[(int(req.origin), 0)]], np.array([req.limit], dtype=np.uint32))[0]
+
+
+# Estimated settled fraction of an isochrone's nodes (sizes its list buffer).
+_ISO_SETTLED = 0.25
+
+
+def request_bytes(req, n_nodes, n_top):
+    """Device bytes a request adds to a round: its label lanes (a matrix's
+    lanes hold the top region plus the bottom paths above its targets,
+    about 24 nodes per target; an isochrone's lane holds every node) and its
+    outputs."""
+    if isinstance(req, MatrixRequest):
+        S, T = len(req.sources), len(req.targets)
+        stride = min(n_nodes, n_top + 24 * T)
+        return 4 * S * stride + 4 * S * T
+    words = (n_nodes + 63) // 64
+    return 4 * n_nodes + 8 * words + int(8 * _ISO_SETTLED * n_nodes)
 
 
 class _Loop:
-    def __init__(self, engine):
+    """Shared machinery: a pending queue, `slots` worker threads, futures
+    that fail (not hang) when a round fails, and cancelled requests dropped."""
+
+    def __init__(self, engine, slots):
         self.engine = engine
         self._pending = collections.deque()
         self._cv = threading.Condition()
         self._stop = False
         self.rounds = 0
-        self._worker = threading.Thread(target=self._run, daemon=True)
-        self._worker.start()
+        self._workers = [threading.Thread(target=self._run, daemon=True) for _ in range(slots)]
+        for t in self._workers:
+            t.start()
 
     def submit(self, req):
         fut = Future()
@@ -111,8 +123,9 @@ class _Loop:
     def close(self):
         with self._cv:
             self._stop = True
-            self._cv.notify()
-        self._worker.join()
+            self._cv.notify_all()
+        for t in self._workers:
+            t.join()
 
     def __enter__(self):
         return self
@@ -121,60 +134,110 @@ class _Loop:
         self.close()
 
     def _take(self):
+        """Called with the lock held: the next round's (request, future) pairs
+        and its reservation, or (None, 0) when nothing can be admitted now."""
         raise NotImplementedError
+
+    def _release(self, reservation):
+        pass
+
+    def _serve(self, reqs):
+        return run_round(self.engine, reqs)
 
     def _run(self):
         while True:
             with self._cv:
-                while not self._pending and not self._stop:
-                    self._cv.wait()
-                if not self._pending:
-                    return
-                batch = self._take()
-            # a request its client cancelled before the round starts leaves the
-            # round; the rest can no longer be cancelled
+                while True:
+                    if not self._pending:
+                        if self._stop:
+                            return
+                        self._cv.wait()
+                        continue
+                    batch, reservation = self._take()
+                    if batch:
+                        break
+                    self._cv.wait()  # a round in flight holds what this one needs
+            # a request its client cancelled before the round starts leaves
+            # the round; the rest can no longer be cancelled
             batch = [(r, f) for r, f in batch if f.set_running_or_notify_cancel()]
-            if not batch:
-                continue
             try:
-                res = run_round(self.engine, [r for r, _ in batch])
-            except BaseException as e:  # a failed round fails its requests, the loop goes on
-                for _, f in batch:
-                    if not f.done():
-                        f.set_exception(e)
-                continue
-            self.rounds += 1
-            done = time.perf_counter()
-            for (_, f), r in zip(batch, res):
-                f.completed = done
-                if not f.done():
-                    f.set_result(r)
+                if batch:
+                    try:
+                        res = self._serve([r for r, _ in batch])
+                    except BaseException as e:  # a failed round fails its requests, the loop goes on
+                        for _, f in batch:
+                            if not f.done():
+                                f.set_exception(e)
+                        continue
+                    done = time.perf_counter()
+                    with self._cv:
+                        self.rounds += 1
+                    for (_, f), r in zip(batch, res):
+                        f.completed = done
+                        if not f.done():
+                            f.set_result(r)
+            finally:
+                with self._cv:
+                    self._release(reservation)
+                    self._cv.notify_all()
 
 
 class ServingLoop(_Loop):
-    """Continuous batching: each round takes the oldest pending requests while
-    their lanes fit max_lanes and their outputs fit max_bytes (the oldest
-    request is always taken)."""
+    """Continuous batching: `slots` rounds may be on the device at once; a
+    round takes the oldest pending requests while their lanes fit max_lanes
+    and their estimated device bytes fit the budget left by the rounds in
+    flight (with no round in flight the oldest request is always taken).
+    The budget is mem_fraction of the device's free memory when the loop
+    starts, or mem_budget bytes when given."""
 
-    def __init__(self, engine, max_lanes=8192, max_bytes=4 << 30):
-        self.max_lanes, self.max_bytes = max_lanes, max_bytes
-        super().__init__(engine)
+    def __init__(self, engine, slots=3, max_lanes=8192, mem_fraction=0.5, mem_budget=None):
+        self.max_lanes = max_lanes
+        if hasattr(engine, "set_concurrency"):
+            engine.set_concurrency(slots)
+        info = engine.info() if hasattr(engine, "info") else {}
+        self._n = int(info.get("n_nodes", getattr(engine, "n_nodes", 0)))
+        self._n_top = int(info.get("n_top", self._n))
+        if mem_budget is None:
+            free = engine.free_device_bytes() if hasattr(engine, "free_device_bytes") else 8 << 30
+            mem_budget = int(mem_fraction * free)
+        self.mem_budget = mem_budget
+        self._inflight_bytes = 0
+        self._inflight = 0
+        super().__init__(engine, slots)
 
     def _take(self):
         batch, lanes, nbytes = [], 0, 0
-        while self._pending:
-            req, fut = self._pending[0]
-            l, b = req.lanes(), request_bytes(req, self.engine.n_nodes)
-            if batch and (lanes + l > self.max_lanes or nbytes + b > self.max_bytes):
+        for req, fut in self._pending:
+            l, b = req.lanes(), request_bytes(req, self._n, self._n_top)
+            fits = lanes + l <= self.max_lanes and self._inflight_bytes + nbytes + b <= self.mem_budget
+            if not fits and (batch or self._inflight):
                 break
-            batch.append(self._pending.popleft())
+            batch.append((req, fut))
             lanes += l
             nbytes += b
-        return batch
+        if not batch:
+            return None, 0
+        for _ in batch:
+            self._pending.popleft()
+        self._inflight_bytes += nbytes
+        self._inflight += 1
+        return batch, nbytes
+
+    def _release(self, reservation):
+        if reservation is None:
+            return
+        self._inflight_bytes -= reservation
+        self._inflight -= 1
 
 
 class PlainLoop(_Loop):
-    """One GPU call per request, in arrival order."""
+    """One GPU call per request, in arrival order (one slot)."""
+
+    def __init__(self, engine):
+        super().__init__(engine, 1)
 
     def _take(self):
-        return [self._pending.popleft()]
+        return [self._pending.popleft()], None
+
+    def _serve(self, reqs):
+        return [run_plain(self.engine, r) for r in reqs]

@@ -1142,13 +1142,15 @@ __global__ __launch_bounds__(ISO_THREADS) void iso_scan_kernel(const uint32_t *_
 template <int KL>
 __global__ __launch_bounds__(NT) void iso_emit_kernel(const uint32_t *__restrict__ arena, uint32_t n_lab,
                                                       const uint32_t *__restrict__ pos_of_rank, uint32_t n,
-                                                      const uint32_t *__restrict__ lane_limit, uint32_t n_lanes,
+                                                      const unsigned long long *__restrict__ bits, uint64_t bits_ld,
+                                                      const uint32_t *__restrict__ row_map, uint32_t n_lanes,
                                                       const uint32_t *__restrict__ slot,
                                                       const uint64_t *__restrict__ row_start,
-                                                      const uint32_t *__restrict__ sb_off, uint32_t nsb, BGate B,
+                                                      const uint32_t *__restrict__ sb_off, uint32_t nsb,
                                                       uint2 *__restrict__ pairs) {
     __shared__ uint32_t tile[64][KL + 1];
     __shared__ uint64_t base[KL];
+    __shared__ unsigned long long word[KL];
     const uint32_t b = blockIdx.x, g = blockIdx.y;
     const uint32_t *lab = arena + (size_t)g * n_lab * KL;
     for (uint32_t lane = threadIdx.x; lane < KL; lane += NT) {
@@ -1159,18 +1161,15 @@ __global__ __launch_bounds__(NT) void iso_emit_kernel(const uint32_t *__restrict
         const uint32_t t0 = b * SB_RANKS + tt * 64;
         if (t0 >= n) break;  // workgroup-uniform
         __syncthreads();     // the previous tile is consumed
+        for (uint32_t lane = threadIdx.x; lane < KL; lane += NT) {  // this tile's reach bits per lane
+            const uint32_t gl = g * KL + lane;
+            word[lane] = gl < n_lanes ? bits[(uint64_t)row_map[gl] * bits_ld + t0 / 64] : 0ull;
+        }
+        __syncthreads();
         for (uint32_t idx = threadIdx.x; idx < 64 * KL; idx += NT) {
             const uint32_t ti = idx / KL, lane = idx % KL;
-            const uint32_t t = t0 + ti, gl = g * KL + lane;
-            uint32_t v = INF;
-            if (t < n && gl < n_lanes) {
-                const uint32_t p = pos_of_rank[t];
-                if (!B.open_b || p < B.n_top || bottom_open<KL>(B, g, p)) {
-                    v = lab[(size_t)p * KL + lane];
-                    if (v > lane_limit[gl]) v = INF;
-                }
-            }
-            tile[ti][lane] = v;
+            // a set bit marks a settled rank (label within the limit): only those are read
+            tile[ti][lane] = (word[lane] >> ti) & 1ull ? lab[(size_t)pos_of_rank[t0 + ti] * KL + lane] : INF;
         }
         __syncthreads();
         // wavefront w takes lanes w, w + NT/64, ..; thread ti one rank
@@ -1976,6 +1975,15 @@ struct PathsOn {
     ~PathsOn() { w->paths_on = false; }
 };
 
+// Limit classes of about 19 % width (a quarter of a binary order of
+// magnitude): lanes of one class share wave groups when the gate is on.
+uint32_t limit_class(uint32_t lim) {
+    if (lim == INF) return 0xFFFFu;
+    const uint32_t lg = 31 - (uint32_t)__builtin_clz(lim | 1u);  // floor(log2)
+    const uint32_t frac = lg >= 2 ? (lim >> (lg - 2)) & 3u : (lim << (2 - lg)) & 3u;  // next two bits
+    return lg * 4 + frac;
+}
+
 // Runs every lane of `req` on Work w: lanes are ordered by the pre-order of
 // their first seed and cut into batches of the lane capacity; after each
 // batch, out(batch_first, batch_lanes, rows, &ms) writes the outputs, where
@@ -2004,7 +2012,15 @@ int run_all(bfg_ctx *c, Work *w, const Req &req, size_t extra, F &&out) {
         const uint64_t k = req.soff[i + 1] > req.soff[i] ? c->L.pre[req.srank[req.soff[i]]] : 0u;
         key[i] = k << 32 | i;
     }
-    if (req.major.empty()) {
+    if (req.major.empty() && w->gate) {
+        // bounded lanes by limit class first: the block gate skips a unit only
+        // when every lane of its wave group is out of reach, and lanes of
+        // similar limits from neighbouring origins reach similar units
+        std::sort(key.begin(), key.end(), [&](uint64_t a, uint64_t b) {
+            const uint32_t ca = limit_class(req.limit[(uint32_t)a]), cb = limit_class(req.limit[(uint32_t)b]);
+            return ca != cb ? ca < cb : a < b;
+        });
+    } else if (req.major.empty()) {
         std::sort(key.begin(), key.end());
     } else {  // each request's lanes stay together, in the order of their requests
         const std::vector<uint32_t> &mj = req.major;
@@ -2277,20 +2293,22 @@ int iso_lists(bfg_ctx *c, Work *w, const Req &req, uint64_t *bits_host, uint64_t
             dst = d_pairs_own.p;
         }
         const uint32_t groups = (nl + c->lanes - 1) / c->lanes;
-        const BGate B = bgate_of(c, w);
         const dim3 grid(nsb, groups);
         switch (c->lanes) {
             case 16:
-                iso_emit_kernel<16><<<grid, NT, 0, s>>>(w->arena.p, n, c->pos_of_rank_d.p, n, w->lane_limit.p, nl,
-                                                        d_slot.p, d_rowstart.p, d_sboff.p, nsb, B, dst);
+                iso_emit_kernel<16><<<grid, NT, 0, s>>>(w->arena.p, n, c->pos_of_rank_d.p, n, d_bits, words,
+                                                        w->row_map.p, nl, d_slot.p, d_rowstart.p, d_sboff.p, nsb,
+                                                        dst);
                 break;
             case 64:
-                iso_emit_kernel<64><<<grid, NT, 0, s>>>(w->arena.p, n, c->pos_of_rank_d.p, n, w->lane_limit.p, nl,
-                                                        d_slot.p, d_rowstart.p, d_sboff.p, nsb, B, dst);
+                iso_emit_kernel<64><<<grid, NT, 0, s>>>(w->arena.p, n, c->pos_of_rank_d.p, n, d_bits, words,
+                                                        w->row_map.p, nl, d_slot.p, d_rowstart.p, d_sboff.p, nsb,
+                                                        dst);
                 break;
             default:
-                iso_emit_kernel<32><<<grid, NT, 0, s>>>(w->arena.p, n, c->pos_of_rank_d.p, n, w->lane_limit.p, nl,
-                                                        d_slot.p, d_rowstart.p, d_sboff.p, nsb, B, dst);
+                iso_emit_kernel<32><<<grid, NT, 0, s>>>(w->arena.p, n, c->pos_of_rank_d.p, n, d_bits, words,
+                                                        w->row_map.p, nl, d_slot.p, d_rowstart.p, d_sboff.p, nsb,
+                                                        dst);
                 break;
         }
         HIP_TRY(hipGetLastError());

@@ -2,11 +2,13 @@
 the split back into per-request results.  CPU tests run it over an engine
 whose sweeps are the oracle's; the GPU test runs it over the HIP engine and
 checks every result against the same request issued alone."""
+import time
+
 import numpy as np
 import pytest
 
 from bfgpu.graph import INF
-from bfgpu.serving import IsochroneRequest, MatrixRequest, PlainLoop, ServingLoop, run_round
+from bfgpu.serving import IsochroneRequest, MatrixRequest, PlainLoop, ServingLoop, run_plain, run_round
 from oracle.oracle import Oracle
 
 
@@ -17,6 +19,12 @@ class OracleEngine:
         self.o, self.n_nodes = Oracle(g), g.n
         self.calls = 0
 
+    def info(self):
+        return {"n_nodes": self.n_nodes, "n_top": self.n_nodes // 4}
+
+    def free_device_bytes(self):
+        return 1 << 34
+
     def compute_matrix_block_gated(self, sources, targets, threshold):
         self.calls += 1
         m = self.o.compute_matrix_flat_mt(sources, targets)
@@ -24,16 +32,20 @@ class OracleEngine:
             m = np.where(m > threshold, INF, m).astype(np.uint32)
         return m, None
 
-    def isochrones(self, origins, limits, want_times=True):
+    def compute_matrix_multi(self, requests):
         self.calls += 1
-        words = (self.n_nodes + 63) // 64
-        bits = np.zeros((len(origins), words), dtype=np.uint64)
-        for i, (o, lim) in enumerate(zip(origins, limits)):
-            inside = self.o.phast_query(int(o)) <= lim
-            padded = np.zeros(words * 64, dtype=bool)
-            padded[:self.n_nodes] = inside
-            bits[i] = np.packbits(padded, bitorder="little").view(np.uint64)
-        return bits, None
+        out = []
+        for s, t, thr in requests:
+            m = self.o.compute_matrix_flat_mt(s, t)
+            if thr != INF:
+                m = np.where(m > thr, INF, m).astype(np.uint32)
+            out.append(m)
+        return out
+
+    def isochrone_lists(self, seed_lists, thresholds):
+        self.calls += 1
+        return [np.array(self.o.seeded(seeds, int(thr)), dtype=np.uint32).reshape(-1, 2)
+                for seeds, thr in zip(seed_lists, thresholds)]
 
 
 def mixed_stream(g, rng, count):
@@ -57,7 +69,7 @@ def test_round_results_equal_requests_alone(grid_small):
     eng = OracleEngine(grid_small)
     reqs = mixed_stream(grid_small, np.random.default_rng(1), 16)
     together = run_round(eng, reqs)
-    assert eng.calls <= 3  # one matrix sweep per threshold present, one isochrone sweep
+    assert eng.calls <= 2  # one matrix sweep, one isochrone sweep
     for a, b in zip(together, alone(eng, reqs)):
         assert np.array_equal(a, b)
 
@@ -93,11 +105,12 @@ def test_admission_limits_a_round(grid_small):
     from concurrent.futures import Future
     eng = OracleEngine(grid_small)
     reqs = [MatrixRequest(np.arange(10, dtype=np.uint32), np.arange(10, dtype=np.uint32)) for _ in range(6)]
-    with ServingLoop(eng, max_lanes=25) as loop:
+    with ServingLoop(eng, slots=1, max_lanes=25) as loop:
         with loop._cv:  # the worker needs this lock to take requests: queue all six first
             pending = [(r, Future()) for r in reqs]
             loop._pending.extend(pending)
-            batch = loop._take()
+            batch, res = loop._take()
+            loop._release(res)
             loop._pending.extendleft(reversed(batch))
             loop._cv.notify()
         assert len(batch) == 2  # 10 + 10 lanes fit 25, a third request would not
@@ -112,7 +125,7 @@ def test_serving_on_gpu_equals_requests_alone(grid_config1):
     reqs = mixed_stream(g, np.random.default_rng(4), 30)
     print("[gpu-test] serving burst", len(reqs), flush=True)
     with BatchedPhastEngine(g) as e:
-        want = alone(e, reqs)
+        want = [run_plain(e, r) for r in reqs]
         with ServingLoop(e) as loop:
             got = [f.result(timeout=120) for f in [loop.submit(r) for r in reqs]]
             rounds = loop.rounds
@@ -126,6 +139,8 @@ def test_serving_on_gpu_equals_requests_alone(grid_config1):
             if r.threshold != INF:
                 m = np.where(m > r.threshold, INF, m)
             assert np.array_equal(res, m)
+        else:
+            assert res.tolist() == [list(x) for x in o.seeded([(r.origin, 0)], r.limit)]
 
 
 class FailingEngine(OracleEngine):
@@ -145,9 +160,13 @@ class FailingEngine(OracleEngine):
         self._maybe_fail()
         return super().compute_matrix_block_gated(sources, targets, threshold)
 
-    def isochrones(self, origins, limits, want_times=True):
+    def compute_matrix_multi(self, requests):
         self._maybe_fail()
-        return super().isochrones(origins, limits, want_times)
+        return super().compute_matrix_multi(requests)
+
+    def isochrone_lists(self, seed_lists, thresholds):
+        self._maybe_fail()
+        return super().isochrone_lists(seed_lists, thresholds)
 
 
 @pytest.mark.parametrize("loop_cls", [ServingLoop, PlainLoop])
@@ -188,3 +207,65 @@ def test_failed_round_fails_only_its_requests(grid_small, loop_cls):
         got = [f.result(timeout=60) for f in [loop.submit(r) for r in reqs]]
     for a, b in zip(got, want):
         assert np.array_equal(a, b)
+
+
+class SlowEngine(OracleEngine):
+    """Records when each engine call starts and ends; each call takes `delay`
+    seconds (a round on the device)."""
+
+    def __init__(self, g, delay=0.3):
+        super().__init__(g)
+        self.delay = delay
+        self.spans = []
+
+    def _timed(self, fn, *a):
+        t0 = time.perf_counter()
+        time.sleep(self.delay)
+        r = fn(*a)
+        self.spans.append((t0, time.perf_counter()))
+        return r
+
+    def compute_matrix_multi(self, requests):
+        return self._timed(super().compute_matrix_multi, requests)
+
+    def isochrone_lists(self, seed_lists, thresholds):
+        return self._timed(super().isochrone_lists, seed_lists, thresholds)
+
+
+def test_request_arriving_mid_round_is_admitted_and_overlaps(grid_small):
+    """Continuous batching: a request that arrives while a round runs starts
+    in a free slot at once, and the two rounds overlap."""
+    eng = SlowEngine(grid_small)
+    rng = np.random.default_rng(7)
+    a = MatrixRequest(rng.integers(0, grid_small.n, 5).astype(np.uint32), rng.integers(0, grid_small.n, 7).astype(np.uint32))
+    b = MatrixRequest(rng.integers(0, grid_small.n, 3).astype(np.uint32), rng.integers(0, grid_small.n, 4).astype(np.uint32))
+    with ServingLoop(eng, slots=2) as loop:
+        fa = loop.submit(a)
+        time.sleep(0.1)  # a's round is on the "device"
+        fb = loop.submit(b)
+        ra, rb = fa.result(timeout=30), fb.result(timeout=30)
+    (a0, a1), (b0, b1) = sorted(eng.spans)
+    assert b0 < a1  # b started before a's round ended
+    assert np.array_equal(ra, eng.o.compute_matrix_flat(a.sources, a.targets))
+    assert np.array_equal(rb, eng.o.compute_matrix_flat(b.sources, b.targets))
+
+
+def test_memory_budget_holds_a_request_until_a_round_frees(grid_small):
+    """Admission by device memory: a request whose bytes do not fit what the
+    round in flight leaves of the budget waits for that round, even with a
+    free slot."""
+    from bfgpu.serving import request_bytes
+    eng = SlowEngine(grid_small)
+    rng = np.random.default_rng(8)
+    a = MatrixRequest(rng.integers(0, grid_small.n, 20).astype(np.uint32), rng.integers(0, grid_small.n, 20).astype(np.uint32))
+    b = MatrixRequest(rng.integers(0, grid_small.n, 20).astype(np.uint32), rng.integers(0, grid_small.n, 20).astype(np.uint32))
+    info = eng.info()
+    need = request_bytes(a, info["n_nodes"], info["n_top"])
+    with ServingLoop(eng, slots=2, mem_budget=int(1.5 * need)) as loop:
+        fa = loop.submit(a)
+        time.sleep(0.1)
+        fb = loop.submit(b)
+        fa.result(timeout=30)
+        fb.result(timeout=30)
+    (a0, a1), (b0, b1) = sorted(eng.spans)
+    assert b0 >= a1  # b waited for a's memory

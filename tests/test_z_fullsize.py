@@ -7,10 +7,11 @@ Every check is bit-exact against the oracle (u32 integer work):
 - 1000 x 1000 matrix as one plain call with K = 32 lanes, every cell;
 - 10k x 10k matrix delivered to the host, 256 sampled rows (hot-spot sources,
   the lowest- and highest-rank sources);
-- 4096 isochrones with uneven 5-60 minute limits in one call: reach bits of
-  every lane against the size-independent invariants (origin reached, bits of
-  the sampled lanes equal to a second call), 32 lanes against the oracle's
-  PHAST rows masked to the limit and 4 lanes against its seeded PHAST;
+- 4096 isochrones with uneven 5-60 minute limits in one call: on every lane
+  the settled list equals the reach bits (ascending ranks, times within the
+  limit, the origin at 0); 32 lanes against the oracle's PHAST rows masked to
+  the limit (and the bits of a second call) and 4 lanes against its seeded
+  PHAST;
 - a mixed serving burst: every result equal to the request issued alone, and
   a sample against the oracle.
 """
@@ -92,9 +93,19 @@ def test_config4_isochrones_4096(country, request):
     g, e, o = country
     from bfgpu.workload import isochrone_lanes
     org, lim = isochrone_lanes(g, 4096, 20261203 + 30)
-    bits, _ = e.isochrones(org, lim, want_times=False)
+    lists, bits = e.isochrone_lists([[(int(r), 0)] for r in org], lim, want_bits=True)
     st = e.last_stats(4096)
-    progress(request.config, f"config4 GPU {st.total_time_ms:.1f} ms, {st.batches} batch(es)")
+    progress(request.config, f"config4 GPU {st.total_time_ms:.1f} ms, {st.batches} batch(es), "
+                             f"{sum(len(x) for x in lists)} settled node-lanes")
+    # on every lane: the list's ranks are exactly the set reach bits, ascending,
+    # every time within the lane's limit, the origin settled at 0
+    for i in range(4096):
+        lst = lists[i]
+        assert int(np.unpackbits(bits[i].view(np.uint8)).sum()) == len(lst)
+        r = lst[:, 0]
+        assert np.all(np.diff(r.astype(np.int64)) > 0) and np.all(lst[:, 1] <= lim[i])
+        assert np.all((bits[i][r // 64] >> (r % 64).astype(np.uint64)) & np.uint64(1) == 1)
+        assert lst[np.searchsorted(r, org[i]), 1] == 0
     # every origin reaches itself at 0
     w, b = org // 64, org % 64
     assert np.all((bits[np.arange(4096), w] >> b.astype(np.uint64)) & np.uint64(1) == 1)
@@ -110,18 +121,19 @@ def test_config4_isochrones_4096(country, request):
     assert np.array_equal(sb, bits_of(stimes, g.n))
     # the seeded restatement itself (run_phast_bounded_fast_seeded) on 4 lanes
     for i in sample[:4]:
-        row = np.full(g.n, INF, dtype=np.uint32)
-        for r, d in o.seeded([(int(org[i]), 0)], int(lim[i])):
-            row[r] = d
-        assert np.array_equal(row, stimes[list(sample).index(i)])
+        assert lists[i].tolist() == [list(x) for x in o.seeded([(int(org[i]), 0)], int(lim[i]))]
+    # the sampled lanes' lists against the oracle's masked rows
+    for j, i in enumerate(sample):
+        nz = np.nonzero(exp[j] != INF)[0]
+        assert np.array_equal(lists[i][:, 0], nz) and np.array_equal(lists[i][:, 1], exp[j][nz])
 
 
 def test_config5_serving_burst(country, request):
     """A mixed burst through the serving loop: each result equals the same
-    request issued alone; matrix results and isochrones sampled against the
-    oracle."""
+    request issued as its own plain call; matrices and isochrone lists
+    sampled against the oracle."""
     g, e, o = country
-    from bfgpu.serving import IsochroneRequest, MatrixRequest, ServingLoop, run_round
+    from bfgpu.serving import IsochroneRequest, MatrixRequest, ServingLoop, run_plain
     from bfgpu.workload import serving_requests
     reqs = serving_requests(g, 40, 20261203 + 50, s_range=(10, 600))
     t0 = time.perf_counter()
@@ -129,14 +141,11 @@ def test_config5_serving_burst(country, request):
         got = [f.result(timeout=300) for f in [loop.submit(r) for r in reqs]]
     progress(request.config, f"config5 burst of {len(reqs)} in {time.perf_counter() - t0:.1f} s; alone ...")
     for r, res in zip(reqs, got):
-        assert np.array_equal(res, run_round(e, [r])[0])
+        assert np.array_equal(res, run_plain(e, r))
     mats = [k for k, r in enumerate(reqs) if isinstance(r, MatrixRequest)][:4]
     isos = [k for k, r in enumerate(reqs) if isinstance(r, IsochroneRequest)][:3]
     for k in mats:
         assert np.array_equal(got[k], o.compute_matrix_flat_mt(reqs[k].sources, reqs[k].targets))
-    if isos:
-        org = np.array([reqs[k].origin for k in isos], dtype=np.uint32)
-        lim = np.array([reqs[k].limit for k in isos], dtype=np.uint32)
-        exp = bits_of(oracle_rows_masked(o, org, lim), g.n)
-        for j, k in enumerate(isos):
-            assert np.array_equal(got[k], exp[j])
+    for k in isos:
+        exp = o.seeded([(reqs[k].origin, 0)], reqs[k].limit)
+        assert got[k].tolist() == [list(x) for x in exp]
