@@ -79,7 +79,7 @@ def traffic_summary(fetch_csv, write_csv, cfetch_csv, cwrite_csv, lanes, workloa
     """FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the DOWN-rows kernel,
     each scaled by what the counter reads per KiB for a plain copy of
     copy_bytes (read copy_bytes, write copy_bytes)."""
-    kern = f"rows_kernel<{lanes}, false, false>"
+    kern = f"rows16_kernel<{lanes}, false>"
     n, fetch = counter_means(fetch_csv, "FETCH_SIZE", kern)
     _, write = counter_means(write_csv, "WRITE_SIZE", kern)
     _, cf = counter_means(cfetch_csv, "FETCH_SIZE", "copyBuffer")
@@ -503,7 +503,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": f"rows_kernel<{KL}, false, false> (DOWN rows of the top region)",
+                     "kernel": f"rows16_kernel<{KL}, false> (DOWN rows of the top region, packed 16-bit where exact)",
                      "launches": dom_launches, "avg_launch_ms": dom_ms / max(dom_launches, 1),
                      "bytes_per_launch": dom_bytes / max(dom_launches, 1),
                      "traffic_source": traffic["source"] if traffic else None,

@@ -66,6 +66,7 @@ BFGPU_SYMBOLS = {
     "bfg_ctx_configure": [C.c_void_p, C.c_uint32, C.c_uint32],
     "bfg_ctx_tune": [C.c_void_p, C.c_uint32, C.c_uint32],
     "bfg_ctx_set_concurrency": [C.c_void_p, C.c_uint32],
+    "bfg_ctx_set_label_width": [C.c_void_p, C.c_uint32],
     "bfg_validate_graph": [C.POINTER(GraphDesc), C.POINTER(CtxInfo)],
     "bfg_matrix": [C.c_void_p, u32p, C.c_uint32, u32p, C.c_uint32, u32p],
     "bfg_matrix_bounded": [C.c_void_p, u32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32, u32p],

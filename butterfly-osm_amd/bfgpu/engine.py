@@ -124,6 +124,11 @@ class BatchedPhastEngine:
         hip.hip().hipSetDevice(self._device)
         return hip.mem_info()[0]
 
+    def set_label_width(self, bits):
+        """16 (packed 16-bit DOWN rows where exact, the default) or 32
+        (bfg_ctx_set_label_width); results are identical."""
+        _check(self._lib.bfg_ctx_set_label_width(self._h, bits))
+
     def set_concurrency(self, max_calls):
         """Calls that may run on the device at once, each on its own stream
         and label arena (bfg_ctx_set_concurrency)."""

@@ -573,6 +573,175 @@ __global__ __launch_bounds__(TILE_THREADS) void rows_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Packed 16-bit DOWN rows.  One wavefront carries two wave groups, A and B
+// (128 lanes): thread l holds lane l of A in the low and lane l of B in the
+// high half of each accumulator, and v_pk_add_u16 (clamp) + v_pk_min_u16 relax
+// both with one instruction each, a weight broadcast from a scalar half-word
+// (weights packed two per dword, min(w, 0xFFFF)).  Arithmetic saturating at
+// 0xFFFF equals the u32 arithmetic as long as every finite candidate stays
+// below 0xFFFF: a finite source label x of a row r must satisfy x <= 0xFFFE -
+// (largest finite weight of r), i.e. x + 1 <= cap[r] with cap[r] = 0xFFFF -
+// that weight (0 when it is 0xFFFF or more).  Then 0xFFFF means INF and every
+// other value is the exact u32 result.  A unit with a label past its row's
+// cap (any lane of A or B) is redone with the u32 rows (down_unit), so the
+// results are bit-identical to the u32 kernel either way.
+// ---------------------------------------------------------------------------
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
+
+// min(x, 0xFFFF) of two u32 labels into one packed word (A low, B high)
+__device__ __forceinline__ uint32_t pack16(uint32_t a, uint32_t b) {
+    return min(a, 0xFFFFu) | (min(b, 0xFFFFu) << 16);
+}
+
+// acc[j] = min(acc[j], {labA, labB}[key[r]] + w16[..]) over rows [r0, r1) of
+// k octets, two rows per step, labels one step ahead (as rows_w).  Returns
+// (per lane) whether a finite label passed its row's cap.
+template <int KL, int N>
+__device__ __forceinline__ bool rows_w16(u16x2 (&acc)[N], uint32_t r0, uint32_t r1, const uint32_t *key,
+                                         const uint32_t *cap, const cu32 *wr, uint32_t k, const uint32_t *labA,
+                                         const uint32_t *labB) {
+    const uint32_t wl = __lane_id(), W2 = 4 * k;  // dwords per row
+    bool bad = false;
+    for (uint32_t c0 = r0; c0 < r1; c0 += 64) {
+        const uint32_t cl = min(64u, r1 - c0) - 1;
+        const uint32_t kv = key[c0 + min(wl, cl)], cv = cap[c0 + min(wl, cl)];
+        uint32_t nA0 = labA[(size_t)rl(kv, 0) * KL], nB0 = labB[(size_t)rl(kv, 0) * KL];
+        uint32_t nA1 = labA[(size_t)rl(kv, min(1u, cl)) * KL], nB1 = labB[(size_t)rl(kv, min(1u, cl)) * KL];
+        const cu32 *W0 = wr + (size_t)(c0 - r0) * W2;
+        for (uint32_t j = 0; j <= cl; j += 2) {
+            const uint32_t c0v = rl(cv, j), c1v = rl(cv, min(j + 1, cl));
+            bad |= (nA0 + 1u > c0v) | (nB0 + 1u > c0v) | (nA1 + 1u > c1v) | (nB1 + 1u > c1v);
+            const u16x2 a0 = as_u16x2(pack16(nA0, nB0)), a1 = as_u16x2(pack16(nA1, nB1));
+            const cu32 *W1 = j < cl ? W0 + W2 : W0;
+            const uint32_t k2 = rl(kv, min(j + 2, cl)), k3 = rl(kv, min(j + 3, cl));
+            nA0 = labA[(size_t)k2 * KL];
+            nB0 = labB[(size_t)k2 * KL];
+            nA1 = labA[(size_t)k3 * KL];
+            nB1 = labB[(size_t)k3 * KL];
+#pragma unroll
+            for (int o = 0; o < N / 8; ++o) {
+                if (o > 0 && (uint32_t)o >= k) break;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const u16x2 w = as_u16x2(W0[o * 4 + q]), v = as_u16x2(W1[o * 4 + q]);
+                    u16x2 &x0 = acc[o * 8 + 2 * q], &x1 = acc[o * 8 + 2 * q + 1];
+                    x0 = __builtin_elementwise_min(
+                        x0, __builtin_elementwise_min(__builtin_elementwise_add_sat(a0, w.xx),
+                                                      __builtin_elementwise_add_sat(a1, v.xx)));
+                    x1 = __builtin_elementwise_min(
+                        x1, __builtin_elementwise_min(__builtin_elementwise_add_sat(a0, w.yy),
+                                                      __builtin_elementwise_add_sat(a1, v.yy)));
+                }
+            }
+            asm volatile("" : "+v"(nA0), "+v"(nB0), "+v"(nA1), "+v"(nB1));
+            W0 += 2 * W2;
+        }
+    }
+    return bad;
+}
+
+// The u32 fallback of a 16-bit unit, compact rather than fast (it runs only
+// where a label does not fit 16 bits): column j of the unit is the min over
+// its rows of label + weight j, the rows' labels re-read per column.  Merges
+// as down_unit.
+template <int KL>
+__device__ __noinline__ void down_unit_u32(uint32_t ra, uint32_t rb, const uint32_t *K, const cu32 *wr, uint32_t W,
+                                           uint32_t p0, uint32_t len, uint32_t *lab, bool whole, bool reach) {
+    for (uint32_t j = 0; j < len; ++j) {
+        uint32_t m = INF;
+        if (j < W)
+            for (uint32_t r = ra; r < rb; ++r) m = min(m, sat_add(lab[(size_t)K[r] * KL], wr[(size_t)(r - ra) * W + j]));
+        if (whole && !reach) lab[(size_t)(p0 + j) * KL] = m;
+        else if (m != INF) atomicMin(lab + (size_t)(p0 + j) * KL, m);
+    }
+}
+
+// A DOWN unit of width W for wave groups A and B (B absent when !hasB: its
+// pointer then repeats A's and nothing is written to it).  Merges as
+// down_unit, per wave group: stored when the unit is whole and the wave
+// group does not reach the segment, else one atomic min per finite value.
+template <int KL, int W>
+__device__ __forceinline__ void down_unit16(uint32_t ra, uint32_t rb, const uint32_t *K, const uint32_t *cap,
+                                            const cu32 *wr16, const cu32 *wr32, uint32_t p0, uint32_t len,
+                                            uint32_t *labA, uint32_t *labB, bool hasB, bool whole, bool reachA,
+                                            bool reachB) {
+    u16x2 acc[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) acc[j] = (u16x2){0xFFFF, 0xFFFF};
+    const bool bad = rows_w16<KL, W>(acc, ra, rb, K, cap, wr16, W / 8, labA, labB);
+    if (__ballot(bad) != 0) {  // a label too large for 16 bits: u32 arithmetic, per wave group
+        down_unit_u32<KL>(ra, rb, K, wr32, W, p0, len, labA, whole, reachA);
+        if (hasB) down_unit_u32<KL>(ra, rb, K, wr32, W, p0, len, labB, whole, reachB);
+        return;
+    }
+    const bool stA = whole && !reachA, stB = hasB && whole && !reachB;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const uint32_t vA = acc[j].x == 0xFFFF ? INF : (uint32_t)acc[j].x;
+        const uint32_t vB = acc[j].y == 0xFFFF ? INF : (uint32_t)acc[j].y;
+        if (j < len) {
+            if (stA) labA[(size_t)(p0 + j) * KL] = vA;
+            else if (vA != INF) atomicMin(labA + (size_t)(p0 + j) * KL, vA);
+            if (stB) labB[(size_t)(p0 + j) * KL] = vB;
+            else if (hasB && vB != INF) atomicMin(labB + (size_t)(p0 + j) * KL, vB);
+        }
+    }
+    // columns past W of a stored unit: INF
+    for (uint32_t j = W; j < len; ++j) {
+        if (stA) labA[(size_t)(p0 + j) * KL] = INF;
+        if (stB) labB[(size_t)(p0 + j) * KL] = INF;
+    }
+}
+
+// rows16: wavefront w of workgroup (blockIdx.x, blockIdx.y) takes DOWN unit
+// u0 + blockIdx.y for the wave groups 2p and 2p + 1, p = 4 blockIdx.x + w <
+// ceil(wgs / 2).  As rows_kernel<KL, false, GATE> otherwise; the weights come
+// from the packed u16 copy, the u32 ones serve the fallback.
+template <int KL, bool GATE>
+__global__ __launch_bounds__(TILE_THREADS) void rows16_kernel(
+    uint32_t u0, uint32_t u1, const uint32_t *__restrict__ seg, const uint32_t *__restrict__ r0,
+    const uint32_t *__restrict__ r1, const uint32_t *__restrict__ woff, const uint32_t *__restrict__ seg_pos,
+    const uint32_t *__restrict__ key, const uint32_t *__restrict__ cap, const uint32_t *__restrict__ wts16,
+    const uint32_t *__restrict__ wts, Reach R, Gate G, uint32_t *__restrict__ arena, uint32_t n, uint32_t wgs) {
+    const uint32_t u = u0 + blockIdx.y, pr = blockIdx.x * (TILE_THREADS / 64) + wave_id();
+    const uint32_t wgA = 2 * pr, wgB = 2 * pr + 1;
+    if (u >= u1 || wgA >= wgs) return;  // wave-uniform; this kernel has no barrier
+    const bool hasB = wgB < wgs;
+    uint32_t *labA = lane_labels<KL>(arena, n, wgA, nullptr);
+    uint32_t *labB = hasB ? lane_labels<KL>(arena, n, wgB, nullptr) : labA;
+    const uint32_t sw = seg[u], s = sw & UNIT_SEG_MASK;
+    const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
+    const bool reachA = wave_reaches(R, wgA, p0), reachB = hasB && wave_reaches(R, wgB, p0);
+    const bool whole = (sw & UNIT_WHOLE) != 0;
+    if constexpr (GATE) {
+        const bool liveA = rows_live<KL>(G, wgA, r0[u], r1[u], key, labA);
+        const bool liveB = hasB && rows_live<KL>(G, wgB, r0[u], r1[u], key, labB);
+        if (!liveA && !liveB) {  // as rows_kernel: INF where a whole unit's wave group is unreached
+            if (whole)
+                for (uint32_t j = 0; j < len; ++j) {
+                    if (!reachA) labA[(size_t)(p0 + j) * KL] = INF;
+                    if (hasB && !reachB) labB[(size_t)(p0 + j) * KL] = INF;
+                }
+            return;
+        }
+    }
+    const uint32_t ra = r0[u], rb = r1[u];
+    const cu32 *wr32 = as_const(wts) + woff[u], *wr16 = as_const(wts16) + woff[u] / 2;
+    switch ((sw >> UNIT_RUN_SHIFT) & 7u) {  // the unit's run: width 8(8 - c)
+    case 0: down_unit16<KL, 64>(ra, rb, key, cap, wr16, wr32, p0, len, labA, labB, hasB, whole, reachA, reachB); break;
+    case 1: down_unit16<KL, 56>(ra, rb, key, cap, wr16, wr32, p0, len, labA, labB, hasB, whole, reachA, reachB); break;
+    case 2: down_unit16<KL, 48>(ra, rb, key, cap, wr16, wr32, p0, len, labA, labB, hasB, whole, reachA, reachB); break;
+    case 3: down_unit16<KL, 40>(ra, rb, key, cap, wr16, wr32, p0, len, labA, labB, hasB, whole, reachA, reachB); break;
+    case 4: down_unit16<KL, 32>(ra, rb, key, cap, wr16, wr32, p0, len, labA, labB, hasB, whole, reachA, reachB); break;
+    case 5: down_unit16<KL, 24>(ra, rb, key, cap, wr16, wr32, p0, len, labA, labB, hasB, whole, reachA, reachB); break;
+    case 6: down_unit16<KL, 16>(ra, rb, key, cap, wr16, wr32, p0, len, labA, labB, hasB, whole, reachA, reachB); break;
+    default: down_unit16<KL, 8>(ra, rb, key, cap, wr16, wr32, p0, len, labA, labB, hasB, whole, reachA, reachB); break;
+    }
+}
+
 // tri: wavefront w of workgroup (blockIdx.x, blockIdx.y) takes segment
 // first + blockIdx.y < first + count of one level for wave group
 // 4*blockIdx.x + w < wgs (the workgroup shares the segment's blocks), loads its
@@ -1425,6 +1594,11 @@ struct bfg_ctx {
     // position and the chunks each tree touches, [bt_ch[2t], bt_ch[2t+1]]
     DevBuf<uint32_t> xr_min, cs_min, pos_bt, bt_ch;
     uint32_t n_bt = 0;
+    // the packed 16-bit DOWN rows: weights min(w, 0xFFFF) two per dword (xr_w
+    // order) and per row the cap 0xFFFF - (its largest finite weight), 0 when
+    // that is 0xFFFF or more; label_bits 16 uses them, 32 keeps the u32 rows
+    DevBuf<uint32_t> xr_w16, xr_cap;
+    uint32_t label_bits = 16;
     DevBuf<uint32_t> pos_of_rank_d;  // position of each rank (the dense gathers' target table)
     std::vector<uint64_t> lvl_dom_bytes;  // algorithmic bytes of a level's DOWN rows per wave group
     uint64_t top_in_arcs = 0;
@@ -1799,9 +1973,18 @@ int launch_phases(bfg_ctx *c, Work *w, const Plan &P, uint32_t nl, PhaseTimes *p
                 }
             }
             if (timed) HIP_TRY(hipEventRecord(w->dom_ev[2 * ndom], s));
+            const uint32_t pairs = (wgs + 1) / 2, px = (pairs + WPB - 1) / WPB;  // the 16-bit kernel's wave-group pairs
             for (uint32_t b = u0; b < u1; b += MAX_GRID_Y, ++nlaunch) {
-                const dim3 grid(wgx, std::min(u1 - b, MAX_GRID_Y));
-                if (w->gate)
+                const dim3 grid(wgx, std::min(u1 - b, MAX_GRID_Y)), grid16(px, std::min(u1 - b, MAX_GRID_Y));
+                if (c->label_bits == 16 && w->gate)
+                    hipLaunchKernelGGL((rows16_kernel<KL, true>), grid16, dim3(TILE_THREADS), 0, s, b,
+                                       std::min(u1, b + MAX_GRID_Y), P.seg.p, P.r0.p, P.r1.p, P.w.p, c->seg_pos.p,
+                                       c->xr_src.p, c->xr_cap.p, c->xr_w16.p, c->xr_w.p, R, G, w->arena.p, n, wgs);
+                else if (c->label_bits == 16)
+                    hipLaunchKernelGGL((rows16_kernel<KL, false>), grid16, dim3(TILE_THREADS), 0, s, b,
+                                       std::min(u1, b + MAX_GRID_Y), P.seg.p, P.r0.p, P.r1.p, P.w.p, c->seg_pos.p,
+                                       c->xr_src.p, c->xr_cap.p, c->xr_w16.p, c->xr_w.p, R, G, w->arena.p, n, wgs);
+                else if (w->gate)
                     hipLaunchKernelGGL((rows_kernel<KL, false, true>), grid, dim3(TILE_THREADS), 0, s, b,
                                        std::min(u1, b + MAX_GRID_Y), P.seg.p, P.r0.p, P.r1.p, nullptr, P.w.p,
                                        c->seg_pos.p, c->pt_tgt.p, c->xr_src.p, c->xr_w.p, R, G, w->arena.p, n, wgs);
@@ -2965,7 +3148,7 @@ bfg::CchInput input_of(const bfg_graph_desc *gd) {
 // tree's nodes are consecutive there), the tree of each bottom position and
 // the trees each bottom chunk touches.
 struct GateTables {
-    std::vector<uint32_t> xr_min, cs_min, pos_bt, bt_ch;
+    std::vector<uint32_t> xr_min, cs_min, pos_bt, bt_ch, xr_w16, xr_cap;
 };
 
 // Host tables of the block gate (see bfg_ctx): the least weight of every DOWN
@@ -2976,18 +3159,28 @@ struct GateTables {
 int gate_tables(const bfg::Layout &L, GateTables &G) {
     const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1, nt = L.n_top;
     G.xr_min.assign(L.xr_off.empty() ? 0 : L.xr_off.back(), INF);
+    G.xr_cap.assign(G.xr_min.size(), 0);
     for (uint32_t sg = 0; sg < nseg; ++sg) {
         uint64_t w = L.xw_off[sg];
         for (uint32_t k = 0, st = 0; k < 8; st = L.xr_run[8 * (size_t)sg + k++]) {
             const uint32_t W = 8 * (8 - k), end = L.xr_run[8 * (size_t)sg + k];
             for (uint32_t i = st; i < end; ++i, w += W) {
-                uint32_t m = INF;
-                for (uint32_t j = 0; j < W; ++j) m = std::min(m, L.xr_w[w + j]);
+                uint32_t m = INF, mx = 0;
+                for (uint32_t j = 0; j < W; ++j) {
+                    const uint32_t x = L.xr_w[w + j];
+                    m = std::min(m, x);
+                    if (x != INF) mx = std::max(mx, x);
+                }
                 G.xr_min[L.xr_off[sg] + i] = m;
+                G.xr_cap[L.xr_off[sg] + i] = mx >= 0xFFFFu ? 0u : 0xFFFFu - mx;
             }
         }
         if (w != L.xw_off[sg + 1]) return fail(BFG_INTERNAL_ERROR, "row runs disagree");
     }
+    if (L.xr_w.size() % 2) return fail(BFG_INTERNAL_ERROR, "row weights not in whole octets");
+    G.xr_w16.resize(L.xr_w.size() / 2);
+    for (size_t i = 0; i < G.xr_w16.size(); ++i)
+        G.xr_w16[i] = std::min(L.xr_w[2 * i], 0xFFFFu) | std::min(L.xr_w[2 * i + 1], 0xFFFFu) << 16;
     G.cs_min.assign(L.cs_src.size(), INF);
     for (size_t k = 0; k < L.cs_src.size(); ++k)
         for (uint32_t a = L.cs_aoff[k]; a < L.cs_aoff[k + 1]; ++a) G.cs_min[k] = std::min(G.cs_min[k], L.ca_w[a]);
@@ -3280,6 +3473,8 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     HIP_TRY(c->pr_w.upload(L.pr_w, s));
     HIP_TRY(c->pk_seg.upload(L.pk_seg, s));
     HIP_TRY(c->xr_min.upload(gt.xr_min, s));
+    HIP_TRY(c->xr_w16.upload(gt.xr_w16, s));
+    HIP_TRY(c->xr_cap.upload(gt.xr_cap, s));
     HIP_TRY(c->cs_min.upload(gt.cs_min, s));
     HIP_TRY(c->pos_bt.upload(gt.pos_bt, s));
     HIP_TRY(c->bt_ch.upload(gt.bt_ch, s));
@@ -3290,7 +3485,7 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
                     &c->in_split, &c->in_src, &c->in_w, &c->ch_off, &c->ch_pos, &c->cs_off, &c->cs_src, &c->cs_aoff,
                     &c->ca_w, &c->up_off, &c->up_tgt, &c->up_w, &c->xr_off, &c->xr_run, &c->xw_off, &c->pr_wid,
                     &c->pw_off, &c->xr_src, &c->xr_w, &c->tb_dn, &c->tb_up, &c->tb_w, &c->pt_blk, &c->pt_tgt,
-                    &c->pr_off, &c->pr_x, &c->pr_w, &c->pk_seg, &c->xr_min, &c->cs_min, &c->pos_bt, &c->bt_ch,
+                    &c->pr_off, &c->pr_x, &c->pr_w, &c->pk_seg, &c->xr_min, &c->cs_min, &c->pos_bt, &c->bt_ch, &c->xr_w16, &c->xr_cap,
                     &c->pos_of_rank_d})
         c->graph_bytes += b->n * sizeof(uint32_t);
     c->graph_bytes += (c->ca_v.n + c->in_dd.n) * sizeof(uint16_t);
@@ -3392,6 +3587,14 @@ int bfg_ctx_configure(bfg_ctx *c, uint32_t lanes, uint32_t lane_capacity) {
     } else {
         c->cap_fixed = false;
     }
+    return BFG_SUCCESS;
+}
+
+int bfg_ctx_set_label_width(bfg_ctx *c, uint32_t bits) {
+    if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
+    if (bits != 16 && bits != 32) return fail(BFG_INVALID_PARAMETER, "label width must be 16 or 32");
+    std::unique_lock<std::shared_mutex> cfg(c->cfg);
+    c->label_bits = bits;
     return BFG_SUCCESS;
 }
 

@@ -111,6 +111,13 @@ int bfg_ctx_last_timings(bfg_ctx *ctx, bfg_timings *t);
  * Reallocates the arenas; used by the autotuner. */
 int bfg_ctx_configure(bfg_ctx *ctx, uint32_t lanes, uint32_t lane_capacity);
 
+/* Arithmetic width of the downward sweep's dense rows (no counterpart in the
+ * reference): 16 (the default) relaxes two 64-lane groups per instruction in
+ * packed 16-bit saturating arithmetic wherever every finite candidate of a
+ * unit stays below 0xFFFF, and falls back to 32 bits elsewhere; 32 always
+ * uses 32 bits.  Results are bit-identical either way; set by the autotuner. */
+int bfg_ctx_set_label_width(bfg_ctx *ctx, uint32_t bits);
+
 /* Calls of one context that may run on the device at once (1..16), each on
  * its own stream and label arena (no counterpart in the reference, whose
  * engines are per-thread; the serving loop overlaps its rounds with it). */

@@ -475,3 +475,40 @@ def test_matrix_multi_equals_requests_alone(engine_cls, grid_config1):
         assert np.array_equal(a, b)
         assert np.array_equal(c, b)
         assert np.array_equal(a, mask(o.compute_matrix_flat_mt(s, t), thr))
+
+
+@pytest.mark.parametrize("scale", [1, 40])
+def test_label_width_16_equals_32(engine_cls, grid_config1, scale):
+    """The packed 16-bit DOWN rows give the u32 rows' results bit for bit: on
+    the grid as it is (labels fit 16 bits), and with every weight times 40 so
+    that labels pass 0xFFFF and units fall back to 32 bits; both against the
+    oracle, unbounded and bounded, K = 64 and 16."""
+    from bfgpu import CchGraph
+    g0 = grid_config1
+    g = g0
+    if scale != 1:
+        def sc(w):
+            w = w.astype(np.uint64) * scale
+            return np.where(w >= INF, INF, w).astype(np.uint32)
+        g = CchGraph(n=g0.n, up_off=g0.up_off, up_tgt=g0.up_tgt, up_w=sc(g0.up_w),
+                     dn_off=g0.dn_off, dn_tgt=g0.dn_tgt, dn_w=sc(g0.dn_w))
+    rng = np.random.default_rng(36 + scale)
+    s = rng.integers(0, g.n, 200).astype(np.uint32)
+    t = rng.integers(0, g.n, 150).astype(np.uint32)
+    o = Oracle(g)
+    exp = o.compute_matrix_flat_mt(s, t)
+    if scale != 1:
+        assert exp[exp != INF].max() > 0xFFFF  # the fallback is exercised
+    say("label width", scale)
+    with engine_cls(g) as e:
+        for lanes in (64, 16):
+            e.configure(lanes=lanes)
+            res = {}
+            for bits in (16, 32):
+                e.set_label_width(bits)
+                res[bits] = e.compute_matrix_flat(s, t)[0]
+                assert np.array_equal(res[bits], exp), (lanes, bits)
+                mb = e.compute_matrix_block_gated(s, t, 20000 * scale)[0]
+                assert np.array_equal(mb, mask(exp, 20000 * scale)), (lanes, bits)
+        e.configure(lanes=64)
+        e.set_label_width(16)
