@@ -599,17 +599,45 @@ __device__ __forceinline__ uint32_t pack16(uint32_t a, uint32_t b) {
 // acc[j] = min(acc[j], {labA, labB}[key[r]] + w16[..]) over rows [r0, r1) of
 // k octets, two rows per step, labels one step ahead (as rows_w).  Returns
 // (per lane) whether a finite label passed its row's cap.
+// A pointer that is the same in every lane, as a scalar (the compiler cannot
+// prove it uniform by itself).
+typedef __attribute__((address_space(1))) const uint32_t gu32;  // global memory
+
+__device__ __forceinline__ const gu32 *uniform_ptr(const uint32_t *p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return (const gu32 *)((uint64_t)hi << 32 | lo);
+}
+
+// Label loads of one row for wave groups A and B.  KL = 64: a row's labels
+// of a wave group are one 256-byte line at a wave-uniform address, so the
+// address is a scalar base (wave group base + key * 256) and the lane's
+// constant offset, with no per-row vector address arithmetic.
+template <int KL>
+struct RowLab {
+    const gu32 *a, *b;  // KL = 64: the wave groups' label bases (wave-uniform); else the lane's pointers
+    uint32_t lane;
+    __device__ __forceinline__ uint32_t A(uint32_t key) const {
+        if constexpr (KL == 64) return (a + (size_t)key * 64)[lane];
+        return a[(size_t)key * KL];
+    }
+    __device__ __forceinline__ uint32_t B(uint32_t key) const {
+        if constexpr (KL == 64) return (b + (size_t)key * 64)[lane];
+        return b[(size_t)key * KL];
+    }
+};
+
 template <int KL, int N>
 __device__ __forceinline__ bool rows_w16(u16x2 (&acc)[N], uint32_t r0, uint32_t r1, const uint32_t *key,
-                                         const uint32_t *cap, const cu32 *wr, uint32_t k, const uint32_t *labA,
-                                         const uint32_t *labB) {
+                                         const uint32_t *cap, const cu32 *wr, uint32_t k, const RowLab<KL> &LB) {
     const uint32_t wl = __lane_id(), W2 = 4 * k;  // dwords per row
     bool bad = false;
     for (uint32_t c0 = r0; c0 < r1; c0 += 64) {
         const uint32_t cl = min(64u, r1 - c0) - 1;
         const uint32_t kv = key[c0 + min(wl, cl)], cv = cap[c0 + min(wl, cl)];
-        uint32_t nA0 = labA[(size_t)rl(kv, 0) * KL], nB0 = labB[(size_t)rl(kv, 0) * KL];
-        uint32_t nA1 = labA[(size_t)rl(kv, min(1u, cl)) * KL], nB1 = labB[(size_t)rl(kv, min(1u, cl)) * KL];
+        uint32_t nA0 = LB.A(rl(kv, 0)), nB0 = LB.B(rl(kv, 0));
+        uint32_t nA1 = LB.A(rl(kv, min(1u, cl))), nB1 = LB.B(rl(kv, min(1u, cl)));
         const cu32 *W0 = wr + (size_t)(c0 - r0) * W2;
         for (uint32_t j = 0; j <= cl; j += 2) {
             const uint32_t c0v = rl(cv, j), c1v = rl(cv, min(j + 1, cl));
@@ -617,10 +645,10 @@ __device__ __forceinline__ bool rows_w16(u16x2 (&acc)[N], uint32_t r0, uint32_t 
             const u16x2 a0 = as_u16x2(pack16(nA0, nB0)), a1 = as_u16x2(pack16(nA1, nB1));
             const cu32 *W1 = j < cl ? W0 + W2 : W0;
             const uint32_t k2 = rl(kv, min(j + 2, cl)), k3 = rl(kv, min(j + 3, cl));
-            nA0 = labA[(size_t)k2 * KL];
-            nB0 = labB[(size_t)k2 * KL];
-            nA1 = labA[(size_t)k3 * KL];
-            nB1 = labB[(size_t)k3 * KL];
+            nA0 = LB.A(k2);
+            nB0 = LB.B(k2);
+            nA1 = LB.A(k3);
+            nB1 = LB.B(k3);
 #pragma unroll
             for (int o = 0; o < N / 8; ++o) {
                 if (o > 0 && (uint32_t)o >= k) break;
@@ -628,12 +656,14 @@ __device__ __forceinline__ bool rows_w16(u16x2 (&acc)[N], uint32_t r0, uint32_t 
                 for (int q = 0; q < 4; ++q) {
                     const u16x2 w = as_u16x2(W0[o * 4 + q]), v = as_u16x2(W1[o * 4 + q]);
                     u16x2 &x0 = acc[o * 8 + 2 * q], &x1 = acc[o * 8 + 2 * q + 1];
-                    x0 = __builtin_elementwise_min(
-                        x0, __builtin_elementwise_min(__builtin_elementwise_add_sat(a0, w.xx),
-                                                      __builtin_elementwise_add_sat(a1, v.xx)));
-                    x1 = __builtin_elementwise_min(
-                        x1, __builtin_elementwise_min(__builtin_elementwise_add_sat(a0, w.yy),
-                                                      __builtin_elementwise_add_sat(a1, v.yy)));
+                    // four independent adds, then the mins (a packed op reading the
+                    // result of the one just before it waits a cycle)
+                    const u16x2 t0 = __builtin_elementwise_add_sat(a0, w.xx);
+                    const u16x2 t1 = __builtin_elementwise_add_sat(a1, v.xx);
+                    const u16x2 t2 = __builtin_elementwise_add_sat(a0, w.yy);
+                    const u16x2 t3 = __builtin_elementwise_add_sat(a1, v.yy);
+                    x0 = __builtin_elementwise_min(x0, __builtin_elementwise_min(t0, t1));
+                    x1 = __builtin_elementwise_min(x1, __builtin_elementwise_min(t2, t3));
                 }
             }
             asm volatile("" : "+v"(nA0), "+v"(nB0), "+v"(nA1), "+v"(nB1));
@@ -671,7 +701,13 @@ __device__ __forceinline__ void down_unit16(uint32_t ra, uint32_t rb, const uint
     u16x2 acc[W];
 #pragma unroll
     for (int j = 0; j < W; ++j) acc[j] = (u16x2){0xFFFF, 0xFFFF};
-    const bool bad = rows_w16<KL, W>(acc, ra, rb, K, cap, wr16, W / 8, labA, labB);
+    RowLab<KL> LB{(const gu32 *)labA, (const gu32 *)labB, 0};
+    if constexpr (KL == 64) {  // wave-uniform bases (lane_labels added the lane)
+        LB.lane = __lane_id();
+        LB.a = uniform_ptr(labA - LB.lane);
+        LB.b = uniform_ptr(labB - LB.lane);
+    }
+    const bool bad = rows_w16<KL, W>(acc, ra, rb, K, cap, wr16, W / 8, LB);
     if (__ballot(bad) != 0) {  // a label too large for 16 bits: u32 arithmetic, per wave group
         down_unit_u32<KL>(ra, rb, K, wr32, W, p0, len, labA, whole, reachA);
         if (hasB) down_unit_u32<KL>(ra, rb, K, wr32, W, p0, len, labB, whole, reachB);
@@ -701,7 +737,7 @@ __device__ __forceinline__ void down_unit16(uint32_t ra, uint32_t rb, const uint
 // ceil(wgs / 2).  As rows_kernel<KL, false, GATE> otherwise; the weights come
 // from the packed u16 copy, the u32 ones serve the fallback.
 template <int KL, bool GATE>
-__global__ __launch_bounds__(TILE_THREADS) void rows16_kernel(
+__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void rows16_kernel(
     uint32_t u0, uint32_t u1, const uint32_t *__restrict__ seg, const uint32_t *__restrict__ r0,
     const uint32_t *__restrict__ r1, const uint32_t *__restrict__ woff, const uint32_t *__restrict__ seg_pos,
     const uint32_t *__restrict__ key, const uint32_t *__restrict__ cap, const uint32_t *__restrict__ wts16,
