@@ -14,7 +14,8 @@ compute_matrix_flat, timed on a bounded sample on this host).
 Other modes print their own JSON line instead: --serving (continuous batching
 against plain per-request calls), --isochrones L (block-gated isochrones with
 uneven limits), --table-2ch S (a two-channel seeded table: times and lengths
-along them), --autotune (K and the DOWN unit size).
+along them).  The matrix line runs the autotuner first (label width, K,
+the DOWN unit size, the source tile; --no-autotune skips it).
 """
 import argparse
 import json
@@ -354,8 +355,10 @@ def main():
     ap.add_argument("--lane-capacity", type=int, default=0, help="sources per device batch (0 = auto)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of single-thread CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--autotune", action="store_true",
-                    help="choose K and the DOWN unit size on this workload before the timed steps")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner (by default it chooses the label width, K, the DOWN unit size and "
+                         "the source tile on this workload before the timed steps)")
+    ap.add_argument("--autotune", action="store_true", help="(the default; kept for older command lines)")
     ap.add_argument("--graph-cache", default=os.environ.get("BFG_GRAPH_CACHE", ""))
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -430,12 +433,12 @@ def main():
         return eng.compute_matrix_device(d_src.ptr, S, d_tgt.ptr, T, d_out.ptr)
 
     tuned = None
-    if args.autotune:
+    if not args.no_autotune:
         from bfgpu.autotune import autotune_matrix
         best, timings = autotune_matrix(eng, d_src.ptr, S, d_tgt.ptr, T, d_out.ptr, log=log)
-        tuned = {"lanes": best[0], "plan_rows": best[1],
-                 "candidates_ms": {f"K={k},rows={r}": round(v, 2) for (k, r), v in timings.items()}}
-        log(f"autotune chose K={best[0]} plan_rows={best[1]}")
+        tuned = dict(best)
+        tuned["candidates_ms"] = {k: round(v, 2) for k, v in timings.items()}
+        log(f"autotune chose {best}")
 
     for i in range(args.warmup):
         st = step()

@@ -305,8 +305,9 @@ def test_reverse_phast_and_arrive_isochrones(engine_cls, grid_config1):
 
 
 def test_autotune_and_tuning_keep_results(engine_cls, grid_config1):
-    """The autotuner's candidates (K, DOWN unit size) and the scalar-block
-    threshold change timings only: the matrix stays bit-exact."""
+    """The autotuner's candidates (label width, K, DOWN unit size, source
+    tile) and the scalar-block threshold change timings only: the matrix
+    stays bit-exact."""
     from bfgpu.autotune import autotune_matrix
     from bfgpu.hip import DeviceArray
     g = grid_config1
@@ -318,10 +319,15 @@ def test_autotune_and_tuning_keep_results(engine_cls, grid_config1):
     with engine_cls(g) as e:
         ds, dt = DeviceArray.from_host(s), DeviceArray.from_host(t)
         dout = DeviceArray(s.size * t.size, np.uint32)
-        best, timings = autotune_matrix(e, ds.ptr, s.size, dt.ptr, t.size, dout.ptr, lanes=(64, 16),
-                                        plan_rows=(64, 512), reps=1)
-        assert len(timings) == 4 and e.info()["lanes"] == best[0]
+        knobs = (("label_bits", (16, 32)), ("lanes", (64, 16)), ("plan_rows", (64, 512)), ("source_tile", (1, 2)))
+        best, timings = autotune_matrix(e, ds.ptr, s.size, dt.ptr, t.size, dout.ptr, reps=1, knobs=knobs)
+        assert len(timings) == 5 and e.info()["lanes"] == best["lanes"]
         assert np.array_equal(dout.to_host(), exp)
+        for src_tile in (2, 4):  # several sweeps per matrix, each a tile of the sources
+            e.configure(lane_capacity=(s.size + src_tile - 1) // src_tile)
+            got, st = e.compute_matrix_flat(s, t)
+            assert np.array_equal(got, exp) and st.batches == src_tile
+        e.configure(lane_capacity=0)
         e.tune(plan_rows=64, tri_scalar_segs=0)
         got, _ = e.compute_matrix_flat(s, t)
         assert np.array_equal(got, exp)
