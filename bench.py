@@ -359,6 +359,9 @@ def main():
                     help="skip the autotuner (by default it chooses the label width, K, the DOWN unit size and "
                          "the source tile on this workload before the timed steps)")
     ap.add_argument("--autotune", action="store_true", help="(the default; kept for older command lines)")
+    ap.add_argument("--host-steps", type=int, default=3,
+                    help="matrices also delivered to pinned host memory through bfg_matrix_bounded after the "
+                         "timed device-resident steps (reported as host_delivered; 0 = skip)")
     ap.add_argument("--graph-cache", default=os.environ.get("BFG_GRAPH_CACHE", ""))
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -481,10 +484,27 @@ def main():
     # the dominant kernel (DOWN rows of the top region): algorithmic bytes of
     # its launches over their summed duration, both from the library
     achieved = dom_bytes / (dom_ms / 1000.0) / 1e9 if dom_ms > 0 else 0.0
-    traffic = committed_traffic(args.workload, KL)
+    label_bits = tuned["label_bits"] if tuned else 16
+    traffic = committed_traffic(args.workload, KL) if label_bits == 16 else None
 
     n_check = min(S, 64)
     gpu_rows = d_out.to_host(0, n_check * T).reshape(n_check, T)
+    host = None
+    if args.host_steps > 0:
+        # the same matrix delivered to host memory (PCIe included), the way a
+        # caller without device buffers gets it
+        from bfgpu.hip import PinnedArray
+        pinned = PinnedArray(S * T)
+        eng.compute_matrix_block_gated(src, tgt, INF_U32, out=pinned.a)  # untimed
+        t1 = time.perf_counter()
+        for _ in range(args.host_steps):
+            eng.compute_matrix_block_gated(src, tgt, INF_U32, out=pinned.a)
+        host_ms = 1000.0 * (time.perf_counter() - t1) / args.host_steps
+        host = {"ms_per_matrix": host_ms, "cells_per_s": S * T / (host_ms / 1000.0), "steps": args.host_steps,
+                "entry_point": "bfg_matrix_bounded", "memory": "pinned host (hipHostMalloc)",
+                "rows_equal_device_rows": bool(np.array_equal(pinned.a[:n_check * T], gpu_rows.reshape(-1)))}
+        log(f"host-delivered {host_ms:.1f} ms per matrix")
+        pinned.free()
     cpu = mt = None
     exact = None
     if not args.no_cpu:
@@ -506,7 +526,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": f"rows16_kernel<{KL}, false> (DOWN rows of the top region, packed 16-bit where exact)",
+                     "kernel": (f"rows16_kernel<{KL}, false> (DOWN rows of the top region, packed 16-bit where exact)"
+                                if label_bits == 16 else f"rows_kernel<{KL}, false, false> (DOWN rows of the top region)"),
                      "launches": dom_launches, "avg_launch_ms": dom_ms / max(dom_launches, 1),
                      "bytes_per_launch": dom_bytes / max(dom_launches, 1),
                      "traffic_source": traffic["source"] if traffic else None,
@@ -518,6 +539,7 @@ def main():
                                "gather": out_ms / args.steps,
                                "batches_wall": engine_ms / args.steps,
                                "host_staging": ms_per_step - engine_ms / args.steps},
+        "host_delivered": host,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": mt,
         "bit_exact_sample": exact,

@@ -167,10 +167,15 @@ class BatchedPhastEngine:
     compute_matrix_flat_soa = compute_matrix_flat      # :756-797
     compute_matrix_flat_blocked = compute_matrix_flat  # :710-751
 
-    def compute_matrix_block_gated(self, sources, targets, threshold):
-        """batched_phast.rs:1223-1265: cells > threshold are INF."""
+    def compute_matrix_block_gated(self, sources, targets, threshold, out=None):
+        """batched_phast.rs:1223-1265: cells > threshold are INF.  `out`
+        (optional) is a caller-owned contiguous u32 array of S*T cells, e.g.
+        the `a` of a hip.PinnedArray; it is filled and returned."""
         src, tgt = _u32(sources), _u32(targets)
-        out = np.empty(src.size * tgt.size, dtype=np.uint32)
+        if out is None:
+            out = np.empty(src.size * tgt.size, dtype=np.uint32)
+        elif out.dtype != np.uint32 or out.size != src.size * tgt.size or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous u32 array of S*T cells")
         _check(self._lib.bfg_matrix_bounded(self._h, _native.ptr(src), src.size, _native.ptr(tgt), tgt.size,
                                             int(threshold) & INF, _native.ptr(out)))
         return out, self.last_stats(src.size)

@@ -30,8 +30,10 @@ def hip():
         _hip.hipSetDevice.argtypes = [C.c_int]
         _hip.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         _hip.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
+        _hip.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
+        _hip.hipHostFree.argtypes = [C.c_void_p]
         for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipDeviceSynchronize", "hipSetDevice",
-                  "hipMemGetInfo", "hipGetDeviceCount"):
+                  "hipMemGetInfo", "hipGetDeviceCount", "hipHostMalloc", "hipHostFree"):
             getattr(_hip, f).restype = C.c_int
     return _hip
 
@@ -86,6 +88,34 @@ class DeviceArray:
     def free(self):
         if self.ptr is not None and self.ptr.value:
             hip().hipFree(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """Page-locked host memory (hipHostMalloc) of `count` elements, seen as a
+    numpy array `a`: device-to-host copies into it run at the link's DMA rate
+    instead of going through the runtime's staging buffers."""
+
+    def __init__(self, count, dtype=np.uint32):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        nbytes = max(self.count * self.dtype.itemsize, 1)
+        p = C.c_void_p()
+        _ok(hip().hipHostMalloc(C.byref(p), nbytes, 0), "hipHostMalloc")
+        self.ptr = p
+        buf = (C.c_uint8 * nbytes).from_address(p.value)
+        self.a = np.frombuffer(buf, dtype=self.dtype, count=self.count)
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            self.a = None
+            hip().hipHostFree(self.ptr)
         self.ptr = None
 
     def __del__(self):
