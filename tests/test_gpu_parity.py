@@ -312,10 +312,10 @@ def test_autotune_and_tuning_keep_results(engine_cls, grid_config1):
     from bfgpu.hip import DeviceArray
     g = grid_config1
     rng = np.random.default_rng(12)
-    s = rng.integers(0, g.n, 150).astype(np.uint32)
+    s = rng.integers(0, g.n, 256).astype(np.uint32)
     t = rng.integers(0, g.n, 120).astype(np.uint32)
     exp = Oracle(g).compute_matrix_flat_mt(s, t)
-    say("autotune 150x120")
+    say("autotune 256x120")
     with engine_cls(g) as e:
         ds, dt = DeviceArray.from_host(s), DeviceArray.from_host(t)
         dout = DeviceArray(s.size * t.size, np.uint32)
@@ -324,9 +324,11 @@ def test_autotune_and_tuning_keep_results(engine_cls, grid_config1):
         assert len(timings) == 5 and e.info()["lanes"] == best["lanes"]
         assert np.array_equal(dout.to_host(), exp)
         for src_tile in (2, 4):  # several sweeps per matrix, each a tile of the sources
-            e.configure(lane_capacity=(s.size + src_tile - 1) // src_tile)
+            cap = (s.size + src_tile - 1) // src_tile
+            e.configure(lane_capacity=cap)
             got, st = e.compute_matrix_flat(s, t)
-            assert np.array_equal(got, exp) and st.batches == src_tile
+            per_batch = (cap + 63) // 64 * 64  # capacities round up to whole wavefronts of lanes
+            assert np.array_equal(got, exp) and st.batches == (s.size + per_batch - 1) // per_batch
         e.configure(lane_capacity=0)
         e.tune(plan_rows=64, tri_scalar_segs=0)
         got, _ = e.compute_matrix_flat(s, t)
