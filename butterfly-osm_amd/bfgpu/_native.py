@@ -84,6 +84,10 @@ BFGPU_SYMBOLS = {
     "bfg_table_seeded": [C.c_void_p, u32p, u32p, u32p, C.c_uint32, u32p, u32p, u32p, C.c_uint32, C.c_uint32,
                          u32p, C.POINTER(C.c_uint8)],
     "bfg_ctx_attach_length": [C.c_void_p, C.POINTER(GraphDesc), u32p, u32p],
+    "bfg_ctx_pair_reverse": [C.c_void_p, C.c_void_p],
+    "bfg_table_bucket": [C.c_void_p, u32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32, u32p],
+    "bfg_table_bucket_device": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32,
+                                C.c_void_p],
     "bfg_phast_seeded_2ch": [C.c_void_p, u32p, u32p, u32p, u32p, C.c_uint32, u32p, u32p, u32p],
     "bfg_table_seeded_2ch": [C.c_void_p, u32p, u32p, u32p, u32p, C.c_uint32, u32p, u32p, u32p, u32p, C.c_uint32,
                              C.c_uint32, u32p, u32p, C.POINTER(C.c_uint8)],

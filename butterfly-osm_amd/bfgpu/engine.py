@@ -79,6 +79,9 @@ class BatchedPhastEngine:
         self._rev = None
 
     def close(self):
+        if getattr(self, "_paired", False) and getattr(self, "_h", None):
+            self._lib.bfg_ctx_pair_reverse(self._h, None)
+            self._paired = False
         if getattr(self, "_rev", None) is not None:
             self._rev.close()
             self._rev = None
@@ -421,6 +424,42 @@ class BatchedPhastEngine:
         if self._rev is None:
             self._rev = BatchedPhastEngine(self.graph.transpose(), device=self._device)
         return self._rev
+
+    def pair_reverse(self):
+        """Pairs this context with reverse_engine()'s (bfg_ctx_pair_reverse),
+        the backward side of the bucket many-to-many; idempotent."""
+        if not getattr(self, "_paired", False):
+            rev = self.reverse_engine()
+            _check(self._lib.bfg_ctx_pair_reverse(self._h, rev._h))
+            self._paired = True
+        return self._rev
+
+    def table_bucket_parallel_bounded(self, sources, targets, threshold=INF, out=None):
+        """bucket_ch.rs:3436-3520 on the GPU (bfg_table_bucket): the forward
+        UP searches of the sources and the backward searches of the targets
+        (upward phase on the transposed CCH) joined at their common
+        elimination-tree ancestors -> (row-major S*T u32, stats); the cells
+        equal compute_matrix_block_gated's."""
+        self.pair_reverse()
+        src, tgt = _u32(sources), _u32(targets)
+        if out is None:
+            out = np.empty(src.size * tgt.size, dtype=np.uint32)
+        elif out.dtype != np.uint32 or out.size != src.size * tgt.size or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous u32 array of S*T cells")
+        _check(self._lib.bfg_table_bucket(self._h, _native.ptr(src), src.size, _native.ptr(tgt), tgt.size,
+                                          int(threshold) & INF, _native.ptr(out)))
+        return out, self.last_stats(src.size)
+
+    def table_bucket_parallel(self, sources, targets):
+        """bucket_ch.rs:3436-3451: the unbounded bucket matrix."""
+        return self.table_bucket_parallel_bounded(sources, targets, INF)
+
+    def table_bucket_device(self, d_sources, n_src, d_targets, n_tgt, d_out, threshold=INF):
+        """Device-resident bucket matrix (bfg_table_bucket_device): raw HIP pointers."""
+        self.pair_reverse()
+        _check(self._lib.bfg_table_bucket_device(self._h, d_sources, n_src, d_targets, n_tgt,
+                                                 int(threshold) & INF, d_out))
+        return self.last_stats(n_src)
 
     def query_block_gated_reverse(self, targets, threshold=INF):
         """PhastEngine::query_block_gated_reverse (range/phast.rs:879-1108) for

@@ -1,6 +1,7 @@
 """Builds the native libraries in-tree.
 
 libbfgpu.so  — HIP kernels + C ABI (hipcc, gfx950)        -> butterfly-osm_amd/lib/
+               (bfgpu.hip: PHAST engine and C ABI; meet.hip: the many-to-many join)
 libbfcch.so  — synthetic road network -> CCH builder (g++)  -> butterfly-osm_amd/lib/
 """
 import os
@@ -32,11 +33,14 @@ def build_bfgpu():
     os.makedirs(BUILD, exist_ok=True)
     lay_o = os.path.join(BUILD, "layout.o")
     _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "layout.cpp"), "-o", lay_o])
-    hip_o = os.path.join(BUILD, "bfgpu.o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "bfgpu.hip"), "-o", hip_o])
+    objs = []
+    for name in ("bfgpu", "meet"):
+        o = os.path.join(BUILD, name + ".o")
+        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, name + ".hip"), "-o", o])
+        objs.append(o)
     out = os.path.join(LIB, "libbfgpu.so")
     tmp = out + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", hip_o, lay_o, "-o", tmp])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, lay_o, "-o", tmp])
     os.replace(tmp, out)
     return out
 

@@ -54,10 +54,12 @@
 #include <numeric>
 #include <shared_mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/bfgpu.h"
 #include "layout.hpp"
+#include "meet.hpp"
 
 using bfg::INF;
 using bfg::NONE;
@@ -1584,6 +1586,13 @@ struct Work {
     // of this call is on the device yet
     std::vector<uint32_t> path_off_h, path_pos_h, tgt_pos_h, path_leaf;
     bool path_pos_staged = false;
+    // upward phase only (the many-to-many join): labels of the top region
+    // alone (see label_stride), path slots for the bottom; the join's reach
+    // bits and bottom pairs
+    bool up_only = false;
+    DevBuf<unsigned long long> reach;
+    DevBuf<uint32_t> pairs;
+    std::vector<uint32_t> h_pairs;
     // host staging of a call (kept, so a fixed Work reuses its capacity)
     std::vector<uint32_t> h_bt, h_pending, h_order, h_soff, h_srank, h_scost, h_lim, h_rows, h_sdepth, h_spre,
         h_spos, h_src, h_tgt;
@@ -1662,11 +1671,18 @@ struct bfg_ctx {
     uint64_t graph_bytes = 0;
     std::mutex t2_mu;                // two-channel calls share the label buffers below
     std::unique_ptr<TwoChannel> t2;  // set by bfg_ctx_attach_length
+    // the many-to-many join: the context over the transposed graph (same
+    // positions, set by bfg_ctx_pair_reverse; not owned), the context this one
+    // is the reverse of, and an all-zero split table for upward-only batches
+    bfg_ctx *rev = nullptr, *rev_of = nullptr;
+    DevBuf<uint8_t> zero_split;
 
     bfg_ctx() = default;
     bfg_ctx(const bfg_ctx &) = delete;
     bfg_ctx &operator=(const bfg_ctx &) = delete;
     ~bfg_ctx() {
+        if (rev) rev->rev_of = nullptr;  // a pair ends with either of its contexts
+        if (rev_of) rev_of->rev = nullptr;
         (void)hipSetDevice(device);
         scratch_work.reset();
         works.clear();  // each Work drains and releases its stream
@@ -1750,6 +1766,7 @@ int begin_call(bfg_ctx *c, CallScope &cs) {
 // region [0, n_top) followed by the nodes of its target paths (path_pos
 // element i at n_top + i).
 uint32_t label_stride(const bfg_ctx *c, const Work *w) {
+    if (w->up_only) return c->L.n_top;
     return w->paths_on ? c->L.n_top + (uint32_t)w->path_nodes : c->L.n;
 }
 
@@ -1971,6 +1988,8 @@ int launch_phases(bfg_ctx *c, Work *w, const Plan &P, uint32_t nl, PhaseTimes *p
     constexpr uint32_t WPB = TILE_THREADS / 64;  // wave groups per workgroup
     const uint32_t wgx = (wgs + WPB - 1) / WPB;  // workgroups across the wave groups
     const uint32_t tss = c->tri_segs ? c->tri_segs - 1 : tri_scalar_segs();
+    // an upward-only batch never merges DOWN units with atomics
+    const uint8_t *up_split = w->up_only ? c->zero_split.p : P.split.p;
     if (nseg) {
         hipLaunchKernelGGL(fill_reach_kernel<KL>, dim3(wgs, (nseg + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, nseg,
                            c->seg_pos.p, R, w->arena.p, n);
@@ -1987,7 +2006,7 @@ int launch_phases(bfg_ctx *c, Work *w, const Plan &P, uint32_t nl, PhaseTimes *p
         for (uint32_t b = s0; b < s1; b += MAX_GRID_Y, ++nlaunch)
             hipLaunchKernelGGL((tri_kernel<KL, false, false, false>), dim3(wgx, std::min(s1 - b, MAX_GRID_Y)),
                                dim3(TILE_THREADS), 0, s, b, std::min(s1 - b, MAX_GRID_Y), c->seg_pos.p, c->tb_up.p,
-                               c->tb_w.p, P.split.p, R, G0, w->arena.p, n, wgs);
+                               c->tb_w.p, up_split, R, G0, w->arena.p, n, wgs);
         const uint32_t k0 = L.pt_blk[s0], k1 = L.pt_blk[s1];
         for (uint32_t b = k0; b < k1; b += MAX_GRID_Y, ++nlaunch)
             hipLaunchKernelGGL((rows_kernel<KL, true, false>), dim3(wgx, std::min(k1 - b, MAX_GRID_Y)), dim3(TILE_THREADS),
@@ -1997,7 +2016,7 @@ int launch_phases(bfg_ctx *c, Work *w, const Plan &P, uint32_t nl, PhaseTimes *p
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(w->ev[1], s));
-    for (uint32_t l = 0; l < nsl; ++l) {
+    for (uint32_t l = 0; l < nsl && !w->up_only; ++l) {
         const uint32_t u0 = P.lvl[l], u1 = P.lvl[l + 1];
         if (u1 > u0) {
             const bool timed = w->time_dom;
@@ -2054,7 +2073,9 @@ int launch_phases(bfg_ctx *c, Work *w, const Plan &P, uint32_t nl, PhaseTimes *p
                                    c->seg_pos.p, c->tb_dn.p, c->tb_w.p, P.split.p, R, G, w->arena.p, n, wgs);
         }
     }
-    if (w->paths_on) {
+    if (w->up_only) {
+        // the upward labels are the result
+    } else if (w->paths_on) {
         constexpr int NSP = NT / KL;
         if (w->n_paths) {
             const int rc = stage_path_elements(c, w);
@@ -2269,10 +2290,13 @@ int run_all(bfg_ctx *c, Work *w, const Req &req, size_t extra, F &&out) {
             w->h_lim[i] = req.limit[o];
         }
         w->h_soff[nl] = (uint32_t)w->h_srank.size();
+        static const Plan no_plan;  // an upward-only batch needs no DOWN plan
         std::shared_ptr<const Plan> P;
-        rc = get_plan(c, w, (nl + 63) / 64, P);
-        if (rc) return rc;
-        rc = run_batch(c, w, *P, nl, &pt);
+        if (!w->up_only) {
+            rc = get_plan(c, w, (nl + 63) / 64, P);
+            if (rc) return rc;
+        }
+        rc = run_batch(c, w, P ? *P : no_plan, nl, &pt);
         if (rc) return rc;
         if (w->gate && !w->fixed) {  // how much the gate left open (flags of this batch)
             const size_t wgs = (nl + 63) / 64, groups = (nl + c->lanes - 1) / c->lanes;
@@ -3403,6 +3427,133 @@ int matrix_multi(bfg_ctx *c, Work *w, uint32_t nr, const uint32_t *src_off, cons
     });
 }
 
+// The root position of the bottom tree (maximal bottom-region subtree) that
+// holds bottom position p.
+uint32_t bottom_root(const bfg::Layout &L, uint32_t p) {
+    for (;;) {
+        const uint32_t q = L.pos_parent[p];
+        if (q == NONE || q < L.n_top) return p;
+        p = q;
+    }
+}
+
+// Runs one Work's batches through the upward phase only; always switched off
+// again.
+struct UpOnly {
+    Work *w;
+    explicit UpOnly(Work *ww) : w(ww) { w->up_only = true; }
+    ~UpOnly() { w->up_only = false; }
+};
+
+// The bucket many-to-many (table_bucket_parallel_bounded, bucket_ch.rs:
+// 3436-3520) on the elimination tree: the target lanes' upward phase on the
+// transposed context (the backward searches), the source lanes' upward phase
+// on this one (the forward searches), then the join over their common
+// meeting nodes: the tiles of top-region segments both wave groups reach
+// (bfg_meet::launch_join) and the pairs that meet inside one bottom tree.
+// Cells above thr are INF.  Target batches are the outer loop, so a matrix
+// too large for two arenas reruns the sources once per target batch.
+int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *tgt, uint32_t T, uint32_t thr,
+                  uint32_t *d_out, uint32_t *host_out) {
+    bfg_ctx *r = c->rev;
+    if (!r) return fail(BFG_NOT_SUPPORTED, "no reverse context is paired (bfg_ctx_pair_reverse)");
+    std::shared_lock<std::shared_mutex> rcfg(r->cfg);
+    WorkLease rlease;
+    int rc = acquire_work(r, rlease);
+    if (rc) return rc;
+    CallScope cs(c);
+    rc = begin_call(c, cs);
+    if (rc) return rc;
+    Work *wf = cs.lease.w, *wb = rlease.w;
+    UpOnly uf(wf), ub(wb);
+    const bfg::Layout &L = c->L;
+    Req rs, rt;
+    single_source_req(rs, src, S, thr);
+    single_source_req(rt, tgt, T, thr);
+    DevBuf<uint32_t> tmp;
+    if (!d_out) {
+        HIP_TRY(tmp.alloc((size_t)S * T));
+        d_out = tmp.p;
+    }
+    const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1, words = std::max<uint32_t>((nseg + 63) / 64, 1);
+    const auto t_start = std::chrono::steady_clock::now();
+    std::unordered_map<uint32_t, std::vector<uint32_t>> tree_tgts;  // bottom tree root -> target lanes
+    float up_f = 0, join_ms = 0;
+    uint32_t joins = 0;
+    uint64_t pairs = 0;
+    rc = run_all(r, wb, rt, 0, [&](uint32_t, uint32_t tnl, const std::vector<uint32_t> &trows, float *) -> int {
+        const uint32_t wgb = (tnl + 63) / 64;
+        ENSURE(wb, reach, (size_t)wgb * words, "reach bits");
+        HIP_TRY(bfg_meet::launch_reach(wb->stream, wb->seed_off.p, wb->seed_pre.p, tnl, r->pos_pre.p, r->pos_sub.p,
+                                       r->seg_pos.p, nseg, words, wb->reach.p));
+        HIP_TRY(hipStreamSynchronize(wb->stream));
+        tree_tgts.clear();
+        for (uint32_t b = 0; b < tnl; ++b) {
+            const uint32_t p = L.pos_of_rank[tgt[trows[b]]];
+            if (p >= L.n_top) tree_tgts[bottom_root(L, p)].push_back(b);
+        }
+        const int rc2 = run_all(c, wf, rs, 0, [&](uint32_t, uint32_t snl, const std::vector<uint32_t> &srows,
+                                                  float *) -> int {
+            const uint32_t wga = (snl + 63) / 64;
+            ENSURE(wf, reach, (size_t)wga * words, "reach bits");
+            HIP_TRY(hipEventRecord(wf->ev[0], wf->stream));
+            HIP_TRY(bfg_meet::launch_reach(wf->stream, wf->seed_off.p, wf->seed_pre.p, snl, c->pos_pre.p,
+                                           c->pos_sub.p, c->seg_pos.p, nseg, words, wf->reach.p));
+            const bfg_meet::Side A{wf->arena.p, L.n_top, c->lanes, snl, wf->reach.p, wf->row_map.p};
+            const bfg_meet::Side B{wb->arena.p, L.n_top, r->lanes, tnl, wb->reach.p, wb->row_map.p};
+            HIP_TRY(bfg_meet::launch_join(wf->stream, A, B, c->seg_pos.p, words, thr, d_out, T));
+            // sources and targets in one bottom tree also meet below the top region
+            std::vector<uint32_t> &pr = wf->h_pairs;
+            pr.clear();
+            if (!tree_tgts.empty())
+                for (uint32_t a = 0; a < snl; ++a) {
+                    const uint32_t p = L.pos_of_rank[src[srows[a]]];
+                    if (p < L.n_top) continue;
+                    const auto it = tree_tgts.find(bottom_root(L, p));
+                    if (it == tree_tgts.end()) continue;
+                    for (uint32_t b : it->second) {
+                        pr.push_back(a);
+                        pr.push_back(b);
+                    }
+                }
+            const StreamDrain drain{wf->stream};  // the pair list is read by an asynchronous copy
+            if (!pr.empty()) {
+                ENSURE(wf, pairs, pr.size(), "bottom pairs");
+                HIP_TRY(hipMemcpyAsync(wf->pairs.p, pr.data(), pr.size() * 4, hipMemcpyHostToDevice, wf->stream));
+                const bfg_meet::PathSide PA{wf->seed_off.p, wf->seed_pos.p, wf->pathbuf.p, wf->row_map.p};
+                const bfg_meet::PathSide PB{wb->seed_off.p, wb->seed_pos.p, wb->pathbuf.p, wb->row_map.p};
+                HIP_TRY(bfg_meet::launch_bottom_pairs(wf->stream, wf->pairs.p, (uint32_t)(pr.size() / 2), PA, PB,
+                                                      d1_of(c), c->pos_parent.p, c->pos_depth.p, L.n_top, thr,
+                                                      d_out, T));
+                pairs += pr.size() / 2;
+            }
+            HIP_TRY(hipEventRecord(wf->ev[3], wf->stream));
+            HIP_TRY(hipEventSynchronize(wf->ev[3]));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, wf->ev[0], wf->ev[3]));
+            join_ms += ms;
+            ++joins;
+            return BFG_SUCCESS;
+        });
+        if (rc2) return rc2;
+        up_f += wf->last.up_ms;
+        return BFG_SUCCESS;
+    });
+    if (rc) return rc;
+    if (host_out) HIP_TRY(hipMemcpy(host_out, d_out, (size_t)S * T * 4, hipMemcpyDeviceToHost));
+    bfg_timings tm{};
+    tm.up_ms = up_f + wb->last.up_ms;  // forward and backward searches
+    tm.sweep_ms = join_ms;             // the join takes the place of the DOWN sweep
+    tm.batches = joins;
+    tm.sweep_launches = joins;
+    tm.lanes_run = (uint64_t)S + T;
+    tm.nodes_swept = pairs;  // pairs that met inside a bottom tree
+    tm.total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    std::lock_guard<std::mutex> g(c->last_mu);
+    c->last = tm;
+    return BFG_SUCCESS;
+}
+
 // Seeded lanes of a call from the caller's CSR (at most 64 seeds per lane).
 int seeded_req(bfg_ctx *c, Req &req, const uint32_t *seed_offsets, const uint32_t *seed_ranks,
                const uint32_t *seed_costs, uint32_t L, const uint32_t *thresholds) {
@@ -3737,6 +3888,70 @@ int bfg_matrix_device_noalloc(bfg_ctx *c, const uint32_t *d_src, uint32_t S, con
     const hipError_t e = hipStreamSynchronize(w->stream);  // the next call may reuse the array
     if (!rc && e != hipSuccess) return fail(BFG_DEVICE_ERROR, hipGetErrorString(e));
     return rc;
+}
+
+int bfg_ctx_pair_reverse(bfg_ctx *c, bfg_ctx *rev) {
+    if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
+    std::unique_lock<std::shared_mutex> g(c->cfg);
+    if (!rev) {  // unpair
+        if (c->rev) c->rev->rev_of = nullptr;
+        c->rev = nullptr;
+        return BFG_SUCCESS;
+    }
+    if (rev == c) return fail(BFG_INVALID_PARAMETER, "a context cannot be its own reverse");
+    if (rev->device != c->device) return fail(BFG_INVALID_PARAMETER, "the reverse context is on another device");
+    if (rev->rev || c->rev_of) return fail(BFG_INVALID_PARAMETER, "a pair's reverse context cannot itself be paired");
+    if (rev->rev_of && rev->rev_of != c) return fail(BFG_INVALID_PARAMETER, "the reverse context is paired already");
+    const bfg::Layout &A = c->L, &B = rev->L;
+    // the transposed graph has the same finite pattern, so the same
+    // elimination tree and positions; its UP arcs are this graph's DOWN arcs
+    // reversed, position by position
+    if (A.n != B.n || A.n_top != B.n_top || A.pos_of_rank != B.pos_of_rank || A.seg_pos != B.seg_pos ||
+        A.m_up != B.m_down || A.m_down != B.m_up)
+        return fail(BFG_INVALID_PARAMETER, "the reverse context is not over the transposed graph");
+    for (uint32_t p = 0; p < A.n; ++p)
+        if (A.in_off[p + 1] - A.in_off[p] != B.up_off[p + 1] - B.up_off[p] ||
+            B.in_off[p + 1] - B.in_off[p] != A.up_off[p + 1] - A.up_off[p])
+            return fail(BFG_INVALID_PARAMETER, "the reverse context is not over the transposed graph");
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t nseg = std::max<size_t>(A.seg_pos.size() - 1, 1);
+    for (bfg_ctx *x : {c, rev})
+        if (x->zero_split.n < nseg) {
+            HIP_TRY(x->zero_split.alloc(nseg));
+            HIP_TRY(hipMemset(x->zero_split.p, 0, nseg));
+        }
+    if (c->rev) c->rev->rev_of = nullptr;
+    c->rev = rev;
+    rev->rev_of = c;
+    return BFG_SUCCESS;
+}
+
+int bfg_table_bucket(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *tgt, uint32_t T, uint32_t thr,
+                     uint32_t *out) {
+    if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
+    if ((S && !src) || (T && !tgt) || (S && T && !out)) return fail(BFG_INVALID_PARAMETER, "null buffer");
+    int rc = check_ranks(c, src, S, "source");
+    if (!rc) rc = check_ranks(c, tgt, T, "target");
+    if (rc) return rc;
+    if (!c->rev) return fail(BFG_NOT_SUPPORTED, "no reverse context is paired (bfg_ctx_pair_reverse)");
+    if (S == 0 || T == 0) return BFG_SUCCESS;
+    return bucket_matrix(c, src, S, tgt, T, thr, nullptr, out);
+}
+
+int bfg_table_bucket_device(bfg_ctx *c, const uint32_t *d_src, uint32_t S, const uint32_t *d_tgt, uint32_t T,
+                            uint32_t thr, uint32_t *d_out) {
+    if (!c) return fail(BFG_INVALID_PARAMETER, "null context");
+    if ((S && !d_src) || (T && !d_tgt) || (S && T && !d_out)) return fail(BFG_INVALID_PARAMETER, "null buffer");
+    if (!c->rev) return fail(BFG_NOT_SUPPORTED, "no reverse context is paired (bfg_ctx_pair_reverse)");
+    if (S == 0 || T == 0) return BFG_SUCCESS;
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<uint32_t> hs(S), ht(T);
+    HIP_TRY(hipMemcpy(hs.data(), d_src, (size_t)S * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ht.data(), d_tgt, (size_t)T * 4, hipMemcpyDeviceToHost));
+    int rc = check_ranks(c, hs.data(), S, "source");
+    if (!rc) rc = check_ranks(c, ht.data(), T, "target");
+    if (rc) return rc;
+    return bucket_matrix(c, hs.data(), S, ht.data(), T, thr, d_out, nullptr);
 }
 
 int bfg_query_batch(bfg_ctx *c, const uint32_t *src, uint32_t S, uint32_t thr, uint32_t *dist) {

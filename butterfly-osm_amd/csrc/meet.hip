@@ -1,0 +1,200 @@
+// meet.hip — the join of the many-to-many (bucket) matrix on gfx950.
+//
+// Forward labels fw(s, x) (upward phase of the source lanes on the CCH) and
+// backward labels bw(t, x) (upward phase of the target lanes on the
+// transposed CCH) are finite only on the root paths of s and t, so
+// d(s, t) = min over common ancestors x of fw(s, x) + bw(t, x): the join of
+// the reference's bucket many-to-many (backward_join_parallel_prefix,
+// route/src/matrix/bucket_ch.rs:3935-4010) with the buckets replaced by the
+// tree.  A tile is 64 source lanes x 64 target lanes (one wave group of each
+// side); its common meeting nodes are the top-region segments both wave
+// groups reach (an AND of two reach bitsets), and the tile is a min-plus
+// product over their positions.  Labels of lanes whose node is below a
+// reached segment but not under a given position are INF there, so every
+// position of a common segment can enter the product.
+//
+// Arithmetic is u32 with saturating adds (v_add_u32 clamp) and v_min3, so the
+// cells equal the reference's bucket join and its PHAST matrix bit for bit.
+#include "meet.hpp"
+
+namespace bfg_meet {
+namespace {
+
+constexpr uint32_t INF = 0xFFFFFFFFu;
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr uint32_t XCDS = 8;  // MI355X accelerator complex dies, each with its own L2
+
+__device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) { return __builtin_elementwise_add_sat(a, b); }
+
+// ---------------------------------------------------------------------------
+// reach: workgroup (word w, wave group g) tests segments 64w .. 64w+63; lane l
+// checks its seeds against the segment head's pre-order interval and a ballot
+// gives the bit.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void meet_reach_kernel(const uint32_t *__restrict__ seed_off,
+                                                        const uint32_t *__restrict__ seed_pre, uint32_t n_lanes,
+                                                        const uint32_t *__restrict__ pos_pre,
+                                                        const uint32_t *__restrict__ pos_sub,
+                                                        const uint32_t *__restrict__ seg_pos, uint32_t nseg,
+                                                        uint32_t words, unsigned long long *__restrict__ reach) {
+    const uint32_t w = blockIdx.x, g = blockIdx.y, gl = g * 64 + threadIdx.x;
+    uint32_t j0 = 0, j1 = 0;
+    if (gl < n_lanes) {
+        j0 = seed_off[gl];
+        j1 = seed_off[gl + 1];
+    }
+    const uint32_t first = gl < n_lanes && j1 > j0 ? seed_pre[j0] : 0;
+    unsigned long long word = 0;
+    const uint32_t s1 = min(nseg, w * 64 + 64);
+    for (uint32_t s = w * 64; s < s1; ++s) {
+        const uint32_t p = seg_pos[s];
+        const uint32_t pr = pos_pre[p], ps = pos_sub[p];
+        bool r = j1 > j0 && first - pr < ps;
+        for (uint32_t j = j0 + 1; j < j1; ++j) r |= seed_pre[j] - pr < ps;
+        if (__ballot(r)) word |= 1ull << (s - w * 64);
+    }
+    if (threadIdx.x == 0) reach[(size_t)g * words + w] = word;
+}
+
+// ---------------------------------------------------------------------------
+// join: one wavefront per tile (source wave group A, target wave group B).
+// Thread (i, j) = (lane / 8, lane % 8) holds the 8 x 8 cells of source lanes
+// 64A + 8i .. and target lanes 64B + 8j .. in registers; per position x it
+// loads its 8 source labels and 8 target labels (two 16-byte loads each; the
+// 8 threads of a row share an address) and relaxes 64 cells.  Two positions
+// per step: acc = min3(acc, a0 + b0, a1 + b1).
+// Tiles go to the XCDs in contiguous ranges (block b runs on XCD b % 8), so
+// the tiles of one source wave group run side by side on one XCD and share its
+// labels in that XCD's L2.
+// ---------------------------------------------------------------------------
+struct JoinArgs {
+    Side A, B;
+    const uint32_t *seg_pos;
+    uint32_t words, wga, wgb, thr;
+    uint32_t *out;
+    uint64_t ld;
+};
+
+__device__ __forceinline__ void load8(uint32_t (&v)[8], const uint32_t *p) {
+    const uint4 lo = *reinterpret_cast<const uint4 *>(p);
+    const uint4 hi = *reinterpret_cast<const uint4 *>(p + 4);
+    v[0] = lo.x; v[1] = lo.y; v[2] = lo.z; v[3] = lo.w;
+    v[4] = hi.x; v[5] = hi.y; v[6] = hi.z; v[7] = hi.w;
+}
+
+__global__ __launch_bounds__(64) void meet_join_kernel(JoinArgs a) {
+    const uint32_t tiles = a.wga * a.wgb;
+    const uint32_t per_xcd = (tiles + XCDS - 1) / XCDS;
+    const uint32_t t = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
+    if (t >= tiles) return;
+    const uint32_t ta = t / a.wgb, tb = t % a.wgb;
+    const uint32_t i = threadIdx.x >> 3, j = threadIdx.x & 7;
+    const uint32_t la = ta * 64 + 8 * i, lb = tb * 64 + 8 * j;
+    const uint32_t kla = a.A.kl, klb = a.B.kl;
+    const uint32_t *fa = a.A.lab + (size_t)(la / kla) * a.A.stride * kla + la % kla;
+    const uint32_t *fb = a.B.lab + (size_t)(lb / klb) * a.B.stride * klb + lb % klb;
+    const unsigned long long *ra = a.A.reach + (size_t)ta * a.words;
+    const unsigned long long *rb = a.B.reach + (size_t)tb * a.words;
+    uint32_t acc[64];
+#pragma unroll
+    for (int k = 0; k < 64; ++k) acc[k] = INF;
+    for (uint32_t w = 0; w < a.words; ++w) {
+        unsigned long long m = ra[w] & rb[w];
+        while (m) {
+            const uint32_t s = w * 64 + (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+            const uint32_t p0 = a.seg_pos[s], p1 = a.seg_pos[s + 1];
+            for (uint32_t x = p0; x < p1; x += 2) {
+                const uint32_t y = min(x + 1, p1 - 1);  // an odd last position pairs with itself
+                uint32_t a0[8], a1[8], b0[8], b1[8];
+                load8(a0, fa + (size_t)x * kla);
+                load8(b0, fb + (size_t)x * klb);
+                load8(a1, fa + (size_t)y * kla);
+                load8(b1, fb + (size_t)y * klb);
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+#pragma unroll
+                    for (int c = 0; c < 8; ++c)
+                        acc[r * 8 + c] = min(acc[r * 8 + c], min(sat_add(a0[r], b0[c]), sat_add(a1[r], b1[c])));
+            }
+        }
+    }
+    uint32_t col[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) col[c] = lb + c < a.B.n_lanes ? a.B.index[lb + c] : 0u;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        if (la + r >= a.A.n_lanes) break;
+        uint32_t *o = a.out + (uint64_t)a.A.index[la + r] * a.ld;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (lb + c < a.B.n_lanes) o[col[c]] = acc[r * 8 + c] > a.thr ? INF : acc[r * 8 + c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// bottom pairs: one thread per pair of a source and a target in one bottom
+// tree.  Both walk up to their lowest common ancestor; every common ancestor
+// from there to the tree's root (the last bottom node) is a meeting node whose
+// labels are the lanes' path slots.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void meet_bottom_kernel(const uint32_t *__restrict__ pairs, uint32_t n_pairs,
+                                                          PathSide A, PathSide B, uint32_t d1,
+                                                          const uint32_t *__restrict__ pos_parent,
+                                                          const uint32_t *__restrict__ pos_depth, uint32_t n_top,
+                                                          uint32_t thr, uint32_t *__restrict__ out, uint64_t ld) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_pairs) return;
+    const uint32_t la = pairs[2 * k], lb = pairs[2 * k + 1];
+    const uint32_t ja = A.seed_off[la], jb = B.seed_off[lb];
+    const uint32_t pa = A.seed_pos[ja], pb = B.seed_pos[jb];
+    const uint32_t da = pos_depth[pa], db = pos_depth[pb];
+    uint32_t x = pa, y = pb, dx = da, dy = db;
+    while (dx > dy && x != NONE) { x = pos_parent[x]; --dx; }
+    while (dy > dx && y != NONE) { y = pos_parent[y]; --dy; }
+    while (x != y && x != NONE && y != NONE) { x = pos_parent[x]; y = pos_parent[y]; }
+    uint32_t best = INF;
+    const uint32_t *sa = A.path + (size_t)ja * d1, *sb = B.path + (size_t)jb * d1;
+    while (x != NONE && x >= n_top && x == y) {
+        const uint32_t d = pos_depth[x];
+        if (da - d >= d1 || db - d >= d1) break;  // deeper than a path holds: not a bottom ancestor
+        best = min(best, sat_add(sa[da - d], sb[db - d]));
+        x = pos_parent[x];
+        y = x;
+    }
+    if (best <= thr) atomicMin(out + (uint64_t)A.index[la] * ld + B.index[lb], best);
+}
+
+}  // namespace
+
+hipError_t launch_reach(hipStream_t st, const uint32_t *seed_off, const uint32_t *seed_pre, uint32_t n_lanes,
+                        const uint32_t *pos_pre, const uint32_t *pos_sub, const uint32_t *seg_pos, uint32_t nseg,
+                        uint32_t words, unsigned long long *reach) {
+    const uint32_t wgs = (n_lanes + 63) / 64;
+    if (!wgs || !words) return hipSuccess;
+    hipLaunchKernelGGL(meet_reach_kernel, dim3(words, wgs), dim3(64), 0, st, seed_off, seed_pre, n_lanes, pos_pre,
+                       pos_sub, seg_pos, nseg, words, reach);
+    return hipGetLastError();
+}
+
+hipError_t launch_join(hipStream_t st, const Side &A, const Side &B, const uint32_t *seg_pos, uint32_t words,
+                       uint32_t thr, uint32_t *out, uint64_t ld) {
+    JoinArgs a{A, B, seg_pos, words, (A.n_lanes + 63) / 64, (B.n_lanes + 63) / 64, thr, out, ld};
+    const uint64_t tiles = (uint64_t)a.wga * a.wgb;
+    if (!tiles) return hipSuccess;
+    const uint64_t blocks = (tiles + XCDS - 1) / XCDS * XCDS;
+    if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(meet_join_kernel, dim3((uint32_t)blocks), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bottom_pairs(hipStream_t st, const uint32_t *pairs, uint32_t n_pairs, const PathSide &A,
+                               const PathSide &B, uint32_t d1, const uint32_t *pos_parent,
+                               const uint32_t *pos_depth, uint32_t n_top, uint32_t thr, uint32_t *out, uint64_t ld) {
+    if (!n_pairs) return hipSuccess;
+    hipLaunchKernelGGL(meet_bottom_kernel, dim3((n_pairs + 255) / 256), dim3(256), 0, st, pairs, n_pairs, A, B, d1,
+                       pos_parent, pos_depth, n_top, thr, out, ld);
+    return hipGetLastError();
+}
+
+}  // namespace bfg_meet

@@ -1,0 +1,57 @@
+// meet.hpp — launchers of the many-to-many join kernels (meet.hip).
+//
+// The bucket many-to-many of the reference (table_bucket_parallel_bounded,
+// route/src/matrix/bucket_ch.rs:3436-3520: a forward UP search per source
+// filling buckets, a backward search per target over DOWN arcs reversed, and
+// a join min(d_fw + d_bw) at every node both searches settle) has, on the
+// elimination tree, a fixed meeting set: the common ancestors of the source
+// and the target.  Both searches are the upward phase of a PHAST batch, the
+// forward one on the CCH and the backward one on its transpose, so their
+// labels sit in two label arenas with the same positions; the join is then a
+// min-plus product of 64-lane tiles over the top-region segments both tiles
+// reach, plus the few pairs whose lowest common ancestor lies in the bottom
+// region.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bfg_meet {
+
+// One side of a join: the upward labels of one batch of lanes (sorted by the
+// pre-order of their node) in an arena [lane group][position][kl] of
+// `stride` positions per lane, the 64-lane wave groups' reach bits over the
+// top-region segments ([wave group][words]), and the original index of each
+// sorted lane (a matrix row for sources, a column for targets).
+struct Side {
+    const uint32_t *lab = nullptr;
+    uint32_t stride = 0, kl = 0, n_lanes = 0;
+    const unsigned long long *reach = nullptr;
+    const uint32_t *index = nullptr;
+};
+
+// reach[g * words + s / 64] bit s % 64: wave group g (lanes 64g ..) has a
+// seed in the subtree of segment s's head (the segment is on the root path of
+// one of its lanes).
+hipError_t launch_reach(hipStream_t st, const uint32_t *seed_off, const uint32_t *seed_pre, uint32_t n_lanes,
+                        const uint32_t *pos_pre, const uint32_t *pos_sub, const uint32_t *seg_pos, uint32_t nseg,
+                        uint32_t words, unsigned long long *reach);
+
+// out[A.index[a] * ld + B.index[b]] = min over the top positions x of the
+// segments both lanes' wave groups reach of A.lab(a, x) + B.lab(b, x)
+// (saturating; INF where no segment is common), INF where above thr.
+hipError_t launch_join(hipStream_t st, const Side &A, const Side &B, const uint32_t *seg_pos, uint32_t words,
+                       uint32_t thr, uint32_t *out, uint64_t ld);
+
+// The pairs (pairs[2k] = source lane, pairs[2k+1] = target lane) of nodes in
+// one bottom tree: min over their common bottom ancestors y of the path slots
+// (path[j * d1 + depth(seed) - depth(y)] of each lane's seed j) merged into
+// the cell with atomic min when within thr.
+struct PathSide {
+    const uint32_t *seed_off = nullptr, *seed_pos = nullptr, *path = nullptr, *index = nullptr;
+};
+hipError_t launch_bottom_pairs(hipStream_t st, const uint32_t *pairs, uint32_t n_pairs, const PathSide &A,
+                               const PathSide &B, uint32_t d1, const uint32_t *pos_parent,
+                               const uint32_t *pos_depth, uint32_t n_top, uint32_t thr, uint32_t *out, uint64_t ld);
+
+}  // namespace bfg_meet

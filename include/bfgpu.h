@@ -248,6 +248,34 @@ int bfg_table_seeded(bfg_ctx *ctx, const uint32_t *src_offsets, const uint32_t *
                      const uint32_t *tgt_ranks, const uint32_t *tgt_parts, uint32_t n_tgt,
                      uint32_t threshold, uint32_t *out, uint8_t *conflicts);
 
+/* The bucket side of the matrix router (table_seeded_bounded_routed sends
+ * matrices to it unless phast_wins, matrix/bucket_ch.rs:6535-6574).
+ * bfg_ctx_pair_reverse gives ctx the context over the transposed graph
+ * (UP arcs = this graph's DOWN arcs reversed, DOWN arcs = its UP arcs
+ * reversed: the reference's DownReverseAdjFlat, bucket_ch.rs:191-424), whose
+ * upward phase is the bucket engine's backward search.  Both contexts must
+ * stay alive while paired; destroying either ends the pair, and rev == NULL
+ * unpairs.  A context paired as a reverse cannot itself be paired.  Returns
+ * BFG_INVALID_PARAMETER when rev is not over the transposed graph (different
+ * positions or arc counts) or is on another device. */
+int bfg_ctx_pair_reverse(bfg_ctx *ctx, bfg_ctx *rev);
+
+/* table_bucket_parallel_bounded(n_nodes, up_adj_flat, down_rev_flat, sources,
+ * targets, threshold) (matrix/bucket_ch.rs:3436-3520): the forward UP search
+ * of every source and the backward search of every target meet at their
+ * common elimination-tree ancestors; out[i * n_tgt + j] = min over those of
+ * fw + bw (saturating), BFG_INF when none or above threshold.  The cells equal
+ * bfg_matrix_bounded's.  Needs a paired reverse context (BFG_NOT_SUPPORTED
+ * otherwise); sources lease a Work of ctx, targets one of the reverse
+ * context. */
+int bfg_table_bucket(bfg_ctx *ctx, const uint32_t *sources, uint32_t n_src, const uint32_t *targets,
+                     uint32_t n_tgt, uint32_t threshold, uint32_t *out);
+
+/* bfg_table_bucket with sources, targets and the S x T result in HIP device
+ * memory. */
+int bfg_table_bucket_device(bfg_ctx *ctx, const uint32_t *d_sources, uint32_t n_src, const uint32_t *d_targets,
+                            uint32_t n_tgt, uint32_t threshold, uint32_t *d_out);
+
 /* Length-along-time weights for the two-channel calls: the CchWeights the
  * reference loads beside the time weights (cch.lat.<mode>.u32,
  * server/state.rs:2880-2891) and reads through flats that share the time

@@ -54,6 +54,7 @@ def lib():
             "orc_build_down_flat": [cp, fp],
             "orc_build_down_reverse_flat": [cp, fp],
             "orc_dijkstra": [C.c_uint32, u64p, u32p, u32p, C.c_uint32, u32p],
+            "orc_table_bucket": [fp, fp, u32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32, u32p],
         }
         for k, v in sig.items():
             f = getattr(_lib, k)
@@ -178,6 +179,29 @@ class Oracle:
 
     def down_reverse_flat(self):
         return self._flat("orc_build_down_reverse_flat")
+
+    def _flat_struct(self, which):
+        """An INF-filtered flat as the C struct (kept alive on this object)."""
+        if not hasattr(self, "_flats"):
+            self._flats = {}
+        if which not in self._flats:
+            off, tgt, w = self._flat(which)
+            arrs = (np.ascontiguousarray(off, dtype=np.uint64), _u32(tgt), _u32(w))
+            f = _Flat(self.n, arrs[0].ctypes.data_as(u64p), _p(arrs[1]), _p(arrs[2]))
+            self._flats[which] = (f, arrs)
+        return self._flats[which][0]
+
+    def table_bucket(self, sources, targets, threshold=INF):
+        """table_bucket_parallel_bounded (bucket_ch.rs:3436-3520): the bucket
+        many-to-many, row-major S*T; cells above threshold are unspecified
+        (the reference's callers mask them)."""
+        s, t = _u32(sources), _u32(targets)
+        out = np.empty(s.size * t.size, dtype=np.uint32)
+        up = self._flat_struct("orc_build_up_flat")
+        dr = self._flat_struct("orc_build_down_reverse_flat")
+        self.L.orc_table_bucket(C.byref(up), C.byref(dr), _p(s), s.size, _p(t), t.size, int(threshold) & INF,
+                                _p(out))
+        return out
 
     def seeded(self, seeds, threshold):
         """run_phast_bounded_fast_seeded -> list of (rank, dist)."""

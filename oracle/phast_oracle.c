@@ -23,6 +23,9 @@
  *                              (generation stamps, 4096-rank block gating)
  *   orc_phast_seeded_2ch       server/isochrone_handler.rs:500-625 (time and
  *                              length-along-time, lexicographic tie-break)
+ *   orc_table_bucket           matrix/bucket_ch.rs:3436-3520 (bucket many-to-many:
+ *                              forward UP sweeps into buckets, backward sweeps over
+ *                              DownReverseAdjFlat, join at every settled node)
  *   orc_dijkstra               plain Dijkstra, the brute-force pin
  *
  * Parity pins (tests/): the reference's hand vectors (range/phast.rs:1198-1554),
@@ -684,6 +687,79 @@ uint64_t orc_phast_seeded_2ch(orc_seeded_state *s, const orc_flat *up, const orc
             }
     }
     return cnt;
+}
+
+/* matrix/bucket_ch.rs:3436-3520 (table_bucket_parallel_bounded) in one
+ * thread: per source a heap-Dijkstra over the UP flat (forward_fill_buckets_flat,
+ * :3240-3272) pushing (node, source, dist) for every settled node; the items
+ * grouped by node (PrefixSumBuckets); per target a heap-Dijkstra over the
+ * DOWN-reverse flat (backward_join_prefix, :3357-3416) joining
+ * cell = min(cell, bucket dist + d) at every settled node.  A pop above
+ * `threshold` ends a sweep (SearchState::pop, :2435-2450), so cells above it
+ * are whatever the sweeps leave (callers mask).  out: row-major S x T. */
+typedef struct { uint32_t node, src, dist; } bucket_item;
+
+static void bucket_sweep(const orc_flat *f, uint32_t origin, uint32_t threshold, uint32_t *dist, uint32_t *touched,
+                         size_t *nt, bucket_item **items, size_t *ni, size_t *cap, uint32_t tag, int collect,
+                         const uint64_t *boff, const bucket_item *bk, uint32_t *row, uint32_t T, uint32_t col) {
+    heap_t h = {0};
+    *nt = 0;
+    dist[origin] = 0;
+    touched[(*nt)++] = origin;
+    hp_push(&h, 0, origin);
+    uint32_t d, u;
+    while (hp_pop(&h, &d, &u)) {
+        if (d > dist[u]) continue;  /* a stale heap entry: u settled earlier */
+        if (d > threshold) break;
+        if (collect) {
+            if (*ni == *cap) {
+                *cap = *cap ? *cap * 2 : 1024;
+                *items = (bucket_item *)realloc(*items, *cap * sizeof(bucket_item));
+            }
+            (*items)[(*ni)++] = (bucket_item){u, tag, d};
+        } else {
+            for (uint64_t k = boff[u]; k < boff[u + 1]; ++k) {
+                uint32_t *cell = row + (size_t)bk[k].src * T + col;
+                uint32_t tot = sat_add(bk[k].dist, d);
+                if (tot < *cell) *cell = tot;
+            }
+        }
+        for (uint64_t i = f->off[u]; i < f->off[u + 1]; ++i) {
+            uint32_t v = f->tgt[i], nd = sat_add(d, f->w[i]);
+            if (nd < dist[v]) {
+                if (dist[v] == INF) touched[(*nt)++] = v;
+                dist[v] = nd;
+                hp_push(&h, nd, v);
+            }
+        }
+    }
+    for (size_t k = 0; k < *nt; ++k) dist[touched[k]] = INF;
+    hp_free(&h);
+}
+
+int orc_table_bucket(const orc_flat *up, const orc_flat *down_rev, const uint32_t *src, uint32_t S,
+                     const uint32_t *tgt, uint32_t T, uint32_t threshold, uint32_t *out) {
+    const uint32_t n = up->n;
+    for (size_t i = 0; i < (size_t)S * T; ++i) out[i] = INF;
+    uint32_t *dist = (uint32_t *)malloc((size_t)n * sizeof(uint32_t));
+    uint32_t *touched = (uint32_t *)malloc((size_t)n * sizeof(uint32_t));
+    for (uint32_t v = 0; v < n; ++v) dist[v] = INF;
+    bucket_item *items = NULL;
+    size_t ni = 0, cap = 0, nt = 0;
+    for (uint32_t s = 0; s < S; ++s)
+        bucket_sweep(up, src[s], threshold, dist, touched, &nt, &items, &ni, &cap, s, 1, NULL, NULL, NULL, 0, 0);
+    /* PrefixSumBuckets: items grouped by node, in push order within a node */
+    uint64_t *boff = (uint64_t *)calloc((size_t)n + 1, sizeof(uint64_t));
+    for (size_t k = 0; k < ni; ++k) boff[items[k].node + 1]++;
+    for (uint32_t v = 0; v < n; ++v) boff[v + 1] += boff[v];
+    bucket_item *bk = (bucket_item *)malloc((ni ? ni : 1) * sizeof(bucket_item));
+    uint64_t *fill = (uint64_t *)malloc(((size_t)n + 1) * sizeof(uint64_t));
+    memcpy(fill, boff, ((size_t)n + 1) * sizeof(uint64_t));
+    for (size_t k = 0; k < ni; ++k) bk[fill[items[k].node]++] = items[k];
+    for (uint32_t t = 0; t < T; ++t)
+        bucket_sweep(down_rev, tgt[t], threshold, dist, touched, &nt, NULL, NULL, NULL, 0, 0, boff, bk, out, T, t);
+    free(fill); free(bk); free(boff); free(items); free(touched); free(dist);
+    return 0;
 }
 
 /* Plain Dijkstra on a CSR graph: the brute-force pin. */

@@ -1,0 +1,143 @@
+"""GPU parity of the bucket many-to-many (bfg_table_bucket): the forward
+upward phase of the sources on the CCH and the backward upward phase of the
+targets on the transposed CCH, joined at their common elimination-tree
+ancestors (table_bucket_parallel_bounded, route/src/matrix/bucket_ch.rs:
+3436-3520).
+
+Bit-exact against the oracle's bucket restatement and its PHAST matrix
+(u32 integer work); bounded calls against the masked unbounded cells.  Cases:
+the hand graphs, the broom closed form, the 224 x 224 grid with diagonal and
+same-bottom-tree pairs (sources and targets that meet below the top region),
+thresholds, lane widths 16/32/64, several source and target batches, the
+device entry point, and the chord grid."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import broom_expect, broom_graph, grid_graph, hand_graph
+from oracle.oracle import INF, Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def mask(d, thr):
+    d = np.array(d, dtype=np.uint32, copy=True)
+    d[d > thr] = INF
+    return d
+
+
+@pytest.fixture(scope="module")
+def engine_cls():
+    from bfgpu import BatchedPhastEngine
+    return BatchedPhastEngine
+
+
+def near_pairs(g, rng, count):
+    """Sources and targets that share a bottom tree: each source with one
+    of its UP neighbours (an ancestor) and with itself."""
+    s = rng.integers(0, g.n, count).astype(np.uint32)
+    t = []
+    for v in s:
+        a, b = int(g.up_off[v]), int(g.up_off[v + 1])
+        t.append(int(g.up_tgt[a + rng.integers(0, b - a)]) if b > a else int(v))
+    return s, np.concatenate([np.asarray(t, dtype=np.uint32), s])
+
+
+@pytest.mark.parametrize("name", ["five_node", "asymmetric_three_node", "inf_arc_four_node"])
+def test_bucket_hand_vectors(engine_cls, name):
+    say("bucket hand", name)
+    g, _ = hand_graph(name)
+    o = Oracle(g)
+    a = np.arange(g.n, dtype=np.uint32)
+    with engine_cls(g) as e:
+        m, _ = e.table_bucket_parallel(a, a)
+    assert np.array_equal(m, o.table_bucket(a, a))
+    assert np.array_equal(m, o.compute_matrix_flat(a, a))
+
+
+@pytest.mark.parametrize("n_leaf", [12, 200_000])
+def test_bucket_broom_closed_form(engine_cls, n_leaf):
+    say("bucket broom", n_leaf)
+    g, wu, wd = broom_graph(n_leaf)
+    rng = np.random.default_rng(n_leaf + 1)
+    s = rng.choice(n_leaf, min(n_leaf, 300), replace=False)
+    t = rng.choice(n_leaf, min(n_leaf, 500), replace=False)
+    t[0] = s[0]
+    exp = broom_expect(wu, wd, s, t)
+    with engine_cls(g) as e:
+        m, _ = e.table_bucket_parallel(s, t)
+        assert np.array_equal(m.reshape(s.size, t.size), exp)
+        for thr in (0, 30, 75, 130):
+            mb, _ = e.table_bucket_parallel_bounded(s, t, thr)
+            assert np.array_equal(mb.reshape(s.size, t.size), mask(exp, thr))
+
+
+def test_bucket_config1_grid(engine_cls, grid_config1):
+    """224 x 224 grid: random pairs, diagonal cells and pairs that meet inside
+    a bottom tree; unbounded and bounded; equal to the PHAST matrix."""
+    g = grid_config1
+    o = Oracle(g)
+    rng = np.random.default_rng(31)
+    s = rng.integers(0, g.n, 100).astype(np.uint32)
+    t = rng.integers(0, g.n, 100).astype(np.uint32)
+    ns, nt = near_pairs(g, rng, 60)
+    s, t = np.concatenate([s, ns]), np.concatenate([t, nt])
+    say("bucket 224 grid", s.size, "x", t.size)
+    exp = o.compute_matrix_flat_mt(s, t)
+    assert np.array_equal(o.table_bucket(s[:20], t), exp.reshape(s.size, t.size)[:20].reshape(-1))
+    with engine_cls(g) as e:
+        m, st = e.table_bucket_parallel(s, t)
+        assert np.array_equal(m, exp)
+        assert st.nodes_swept > 0  # some pairs met inside a bottom tree
+        ph, _ = e.compute_matrix_flat(s, t)
+        assert np.array_equal(m, ph)
+        for thr in (0, 300, 1800):
+            mb, _ = e.table_bucket_parallel_bounded(s, t, thr)
+            assert np.array_equal(mb, mask(exp, thr))
+
+
+@pytest.mark.parametrize("lanes,cap", [(16, 64), (32, 128), (64, 64)])
+def test_bucket_lane_widths_and_batches(engine_cls, grid_config1, lanes, cap):
+    """Lane groups of 16/32/64 on both sides; capacities that cut the sources
+    and the targets into several batches."""
+    g = grid_config1
+    o = Oracle(g)
+    rng = np.random.default_rng(lanes + cap)
+    s = rng.integers(0, g.n, 150).astype(np.uint32)
+    t = rng.integers(0, g.n, 200).astype(np.uint32)
+    ns, nt = near_pairs(g, rng, 30)
+    s, t = np.concatenate([s, ns]), np.concatenate([t, nt])
+    say("bucket lanes", lanes, "cap", cap, s.size, "x", t.size)
+    exp = o.compute_matrix_flat_mt(s, t)
+    with engine_cls(g) as e:
+        rev = e.pair_reverse()
+        for x in (e, rev):
+            x.configure(lanes=lanes, lane_capacity=cap)
+        m, st = e.table_bucket_parallel(s, t)
+        assert np.array_equal(m, exp)
+        assert st.batches > 1
+
+
+def test_bucket_device_entry_and_chords(engine_cls):
+    """The device entry point on the 64 x 64 grid with long chords."""
+    from bfgpu.hip import DeviceArray
+    g = grid_graph(64, chord_frac=0.02)
+    o = Oracle(g)
+    rng = np.random.default_rng(5)
+    s = rng.integers(0, g.n, 90).astype(np.uint32)
+    t = rng.integers(0, g.n, 70).astype(np.uint32)
+    say("bucket chords device", s.size, "x", t.size)
+    exp = o.table_bucket(s, t)
+    with engine_cls(g) as e:
+        ds, dt = DeviceArray.from_host(s), DeviceArray.from_host(t)
+        dout = DeviceArray(s.size * t.size, np.uint32)
+        e.table_bucket_device(ds.ptr, s.size, dt.ptr, t.size, dout.ptr)
+        assert np.array_equal(dout.to_host(), exp)
+        e.table_bucket_device(ds.ptr, s.size, dt.ptr, t.size, dout.ptr, 900)
+        assert np.array_equal(dout.to_host(), mask(exp, 900))

@@ -232,3 +232,48 @@ def test_transposed_phast_is_reverse_phast(grid_small):
         rev = o.phast_query_reverse(t)
         assert np.array_equal(ro.phast_query(t), rev)
         assert np.array_equal(rev, dijkstra(g.n, off, head, w, t))
+
+
+@pytest.mark.parametrize("n_leaf,thresholds", [(5, (0, 15, 25, 55, 1000)), (12, (0, 30, 75, 130, 100_000))])
+def test_table_bucket_broom_bounded(n_leaf, thresholds):
+    """bucket_ch.rs:5617-5736 (max_minutes_bound_tests): the bucket matrix
+    on the broom is the closed form; bounded, every cell within the
+    threshold is exact and every other cell is above it and never below the
+    unbounded one."""
+    g, wu, wd = broom_graph(n_leaf)
+    o = Oracle(g)
+    s = np.arange(n_leaf)
+    unb = o.table_bucket(s, s).reshape(n_leaf, n_leaf)
+    assert np.array_equal(unb, broom_expect(wu, wd, s, s))
+    for thr in thresholds:
+        b = o.table_bucket(s, s, thr).reshape(n_leaf, n_leaf)
+        inb = unb <= thr
+        assert np.array_equal(b[inb], unb[inb])
+        assert np.all(b[~inb] > thr) and np.all(b[~inb] >= unb[~inb])
+
+
+@pytest.mark.parametrize("name", ["five_node", "asymmetric_three_node", "inf_arc_four_node"])
+def test_table_bucket_equals_phast_on_hand_graphs(name):
+    """The bucket join (forward UP sweeps, backward DOWN-reverse sweeps) and
+    the PHAST matrix give the same cells: both are the least up-down path."""
+    g, _ = hand_graph(name)
+    o = Oracle(g)
+    a = np.arange(g.n)
+    assert np.array_equal(o.table_bucket(a, a), o.compute_matrix_flat(a, a))
+
+
+def test_table_bucket_equals_phast_and_bruteforce_on_a_grid(grid_small):
+    g = grid_small
+    o = Oracle(g)
+    rng = np.random.default_rng(21)
+    s = rng.integers(0, g.n, 40)
+    t = rng.integers(0, g.n, 30)
+    t[:5] = s[:5]  # diagonal cells
+    b = o.table_bucket(s, t)
+    assert np.array_equal(b, o.compute_matrix_flat(s, t))
+    for i in range(3):  # and Dijkstra on the edge-based graph
+        d = dijkstra(g.n, g.ebg_off, g.ebg_head, g.ebg_w, int(s[i]))
+        assert np.array_equal(b.reshape(s.size, t.size)[i], d[t])
+    for thr in (0, 500, 2000):
+        bb = o.table_bucket(s, t, thr)
+        assert np.array_equal(mask(bb, thr), mask(b, thr))
