@@ -67,16 +67,26 @@ def _u32(a):
 class BatchedPhastEngine:
     """BatchedPhastEngine::new(topo, weights) (batched_phast.rs:89) on one GPU."""
 
-    def __init__(self, graph: CchGraph, device: int = 0):
-        self.graph = graph
+    def __init__(self, graph: CchGraph, device: int = 0, transposed: bool = False):
+        """transposed: the context over graph's transposed CCH (the reverse
+        metric), which the library transposes itself (bfg_ctx_create_reverse);
+        `graph` then builds CchGraph.transpose only when asked for."""
+        self._graph, self._graph_of = (None, graph) if transposed else (graph, None)
         self._lib = _native.bfgpu()
         h = C.c_void_p()
         d = graph.desc()
-        _check(self._lib.bfg_ctx_create(C.byref(d), device, C.byref(h)))
+        create = self._lib.bfg_ctx_create_reverse if transposed else self._lib.bfg_ctx_create
+        _check(create(C.byref(d), device, C.byref(h)))
         self._h = h
         self.n_nodes = graph.n
         self._device = device
         self._rev = None
+
+    @property
+    def graph(self):
+        if self._graph is None:
+            self._graph = self._graph_of.transpose()
+        return self._graph
 
     def close(self):
         if getattr(self, "_paired", False) and getattr(self, "_h", None):
@@ -418,11 +428,12 @@ class BatchedPhastEngine:
         return out, conf
 
     def reverse_engine(self):
-        """The engine over the reverse metric's CCH (CchGraph.transpose), built on
-        first use: a second device context whose rows are all-to-one distances
+        """The engine over the reverse metric's CCH (the transposed CCH, built
+        by the library), created on first use: a second device context whose
+        rows are all-to-one distances
         (the reference passes the swapped flats, range/phast.rs:869-885)."""
         if self._rev is None:
-            self._rev = BatchedPhastEngine(self.graph.transpose(), device=self._device)
+            self._rev = BatchedPhastEngine(self.graph, device=self._device, transposed=True)
         return self._rev
 
     def pair_reverse(self):

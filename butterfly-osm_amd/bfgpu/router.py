@@ -13,13 +13,16 @@ does (:6535-6574): the PHAST path is the GPU table, the bucket path the
 caller's bucket engine when one is given (it is not part of this build; the
 GPU table serves both when none is given).
 
-`MatrixRouter` is the GPU's own routing of plain matrices, which the
-reference leaves as a follow-up: the lanes of a sweep are either the sources
-on the forward CCH or the targets on the transposed CCH (all-to-one rows,
-range/phast.rs:869-885), and the two give the same cells.  It keeps an EWMA
-of each side's measured milliseconds per lane and routes each call to the
-side whose lanes cost less; before both sides are measured, the side with
-fewer lanes wins.
+`MatrixRouter` is the GPU's own routing of plain matrices.  The same cells
+come from three engines: PHAST lanes over the sources on the forward CCH,
+PHAST lanes over the targets on the transposed CCH (all-to-one rows,
+range/phast.rs:869-885), or the bucket many-to-many (table_bucket_parallel,
+bucket_ch.rs:3436-3520: upward phases of both sides joined at their common
+ancestors).  It keeps EWMA cost cells measured on the calls it routes: ns per
+lane of each PHAST side, and for the bucket ns per upward lane (S + T) plus ns
+per cell of the join (S * T), and sends each call to the cheapest estimate.
+Before a side is measured: the bucket when both sides have at least
+BUCKET_MIN lanes, otherwise PHAST over the side with fewer lanes.
 """
 import os
 import threading
@@ -89,24 +92,49 @@ def table_seeded_bounded_routed(engine, src_seedsets, tgt_seedsets, threshold=IN
     return res
 
 
-class MatrixRouter:
-    """Routes plain matrices to the forward (source lanes) or the reverse
-    (target lanes, transposed CCH) engine by measured cost per lane."""
+BUCKET_MIN = 128  # lanes on each side from which the bucket is tried before it is measured
 
-    def __init__(self, engine):
+
+class MatrixRouter:
+    """Routes plain matrices to forward PHAST (source lanes), reverse PHAST
+    (target lanes, transposed CCH) or the bucket many-to-many by measured
+    cost."""
+
+    def __init__(self, engine, bucket=True):
         self.engine = engine
-        self.fwd_ns, self.rev_ns = CostCell(), CostCell()  # ns per lane of each side
+        self.bucket = bucket
+        self.fwd_ns, self.rev_ns = CostCell(), CostCell()  # ns per lane of each PHAST side
+        self.up_ns, self.cell_ns = CostCell(), CostCell()  # bucket: ns per upward lane, ns per joined cell
         self.last_side = None
 
-    def side(self, n_sources, n_targets):
+    def estimates(self, n_sources, n_targets):
+        """Estimated ns of each side (None where not measured yet)."""
         f, r = self.fwd_ns.load(), self.rev_ns.load()
-        if f > 0 and r > 0:
-            return "forward" if n_sources * f <= n_targets * r else "reverse"
-        return "forward" if n_sources <= n_targets else "reverse"
+        u, c = self.up_ns.load(), self.cell_ns.load()
+        return {"forward": n_sources * f if f else None,
+                "reverse": n_targets * r if r else None,
+                "bucket": ((n_sources + n_targets) * u + n_sources * n_targets * c) if (u and c) else None}
+
+    def side(self, n_sources, n_targets):
+        S, T = n_sources, n_targets
+        f, r = self.fwd_ns.load(), self.rev_ns.load()
+        f, r = (f or r), (r or f)  # one measured PHAST side stands in for the other
+        u, c = self.up_ns.load(), self.cell_ns.load()
+        have_b = self.bucket and bool(u and c)
+        if f:
+            cost = {"forward": S * f, "reverse": T * r}
+            phast = min(cost, key=cost.get)  # forward on a tie
+        else:
+            phast = "forward" if S <= T else "reverse"
+        if f and have_b:
+            return "bucket" if (S + T) * u + S * T * c < cost[phast] else phast
+        if self.bucket and not have_b and min(S, T) >= BUCKET_MIN:
+            return "bucket"
+        return phast
 
     def compute_matrix_block_gated(self, sources, targets, threshold=INF):
         """The S x T matrix (row-major u32), equal to the engine's own
-        compute_matrix_block_gated, through the cheaper side."""
+        compute_matrix_block_gated, through the cheapest side."""
         src = np.ascontiguousarray(sources, dtype=np.uint32)
         tgt = np.ascontiguousarray(targets, dtype=np.uint32)
         side = self.side(src.size, tgt.size)
@@ -114,11 +142,21 @@ class MatrixRouter:
         if side == "forward":
             m, _ = self.engine.compute_matrix_block_gated(src, tgt, threshold)
             lanes, cell = src.size, self.fwd_ns
-        else:
+        elif side == "reverse":
             rev = self.engine.reverse_engine()
             mt, _ = rev.compute_matrix_block_gated(tgt, src, threshold)
             m = np.ascontiguousarray(mt.reshape(tgt.size, src.size).T).reshape(-1)
             lanes, cell = tgt.size, self.rev_ns
+        else:
+            m, st = self.engine.table_bucket_parallel_bounded(src, tgt, threshold)
+            lanes, cell = 0, None
+            total = time.perf_counter_ns() - t0
+            if src.size and tgt.size:
+                # the join's share of the call by the engine's own phase times
+                up, join = float(st.upward_time_ms), float(st.downward_time_ms)
+                share = join / (up + join) if up + join > 0 else 0.5
+                update_cost_ewma(self.up_ns, int(total * (1 - share)) // (src.size + tgt.size))
+                update_cost_ewma(self.cell_ns, max(int(total * share) // (src.size * tgt.size), 1))
         if lanes:
             update_cost_ewma(cell, (time.perf_counter_ns() - t0) // lanes)
         self.last_side = side

@@ -54,6 +54,7 @@
 #include <numeric>
 #include <shared_mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -1590,6 +1591,7 @@ struct Work {
     // alone (see label_stride), path slots for the bottom; the join's reach
     // bits and bottom pairs
     bool up_only = false;
+    uint32_t cap_hint = 0;  // lanes per batch chosen by the caller (0: run_all chooses)
     DevBuf<unsigned long long> reach;
     DevBuf<uint32_t> pairs;
     std::vector<uint32_t> h_pairs;
@@ -2236,7 +2238,8 @@ int run_all(bfg_ctx *c, Work *w, const Req &req, size_t extra, F &&out) {
     w->gate = false;
     for (uint32_t i = 0; i < L; ++i) w->gate |= req.limit[i] != INF;
     if (w->fixed) w->lane_cap = 64;
-    else w->lane_cap = c->cap_fixed ? c->lane_cap : auto_lane_cap(c, w, L, extra);
+    else if (c->cap_fixed) w->lane_cap = c->lane_cap;
+    else w->lane_cap = w->cap_hint ? w->cap_hint : auto_lane_cap(c, w, L, extra);
     int rc = ensure_arena(c, w, (size_t)w->lane_cap * label_stride(c, w));
     if (rc) return rc;
     if (w->gate) {
@@ -3203,6 +3206,36 @@ bfg::CchInput input_of(const bfg_graph_desc *gd) {
     return in;
 }
 
+// The transposed graph of gd (the reverse metric's CCH): UP arcs at tail y
+// are the DOWN arcs x -> y reversed, DOWN arcs at tail v the UP arcs u -> v
+// reversed, each tail's targets ascending (the old tails are visited in
+// ascending order).  Returns false on a target out of range.
+struct Transposed {
+    std::vector<uint64_t> up_off, dn_off;
+    std::vector<uint32_t> up_tgt, up_w, dn_tgt, dn_w;
+};
+
+bool flip_csr(uint32_t n, const uint64_t *off, const uint32_t *tgt, const uint32_t *w, std::vector<uint64_t> &noff,
+              std::vector<uint32_t> &ntgt, std::vector<uint32_t> &nw) {
+    const uint64_t m = off[n];
+    noff.assign((size_t)n + 1, 0);
+    for (uint64_t i = 0; i < m; ++i) {
+        if (tgt[i] >= n) return false;
+        noff[tgt[i] + 1]++;
+    }
+    for (uint32_t v = 0; v < n; ++v) noff[v + 1] += noff[v];
+    ntgt.resize(m);
+    nw.resize(m);
+    std::vector<uint64_t> fill(noff.begin(), noff.end() - 1);
+    for (uint32_t r = 0; r < n; ++r)
+        for (uint64_t i = off[r]; i < off[r + 1]; ++i) {
+            const uint64_t k = fill[tgt[i]]++;
+            ntgt[k] = r;
+            nw[k] = w[i];
+        }
+    return true;
+}
+
 // Host tables of the block gate (see bfg_ctx): segment parents, the bottom
 // trees in pre-order (ch_pos lists the bottom positions in pre-order, and a
 // tree's nodes are consecutive there), the tree of each bottom position and
@@ -3442,8 +3475,33 @@ uint32_t bottom_root(const bfg::Layout &L, uint32_t p) {
 struct UpOnly {
     Work *w;
     explicit UpOnly(Work *ww) : w(ww) { w->up_only = true; }
-    ~UpOnly() { w->up_only = false; }
+    ~UpOnly() {
+        w->up_only = false;
+        w->cap_hint = 0;
+    }
 };
+
+// Hand-over between the two sides of a join run at once: the target side
+// signals its labels ready and waits until the join is done with them.
+struct MeetSync {
+    std::mutex mu;
+    std::condition_variable cv;
+    bool ready = false, failed = false, done = false;
+    void set(bool MeetSync::*flag) {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            this->*flag = true;
+        }
+        cv.notify_all();
+    }
+};
+
+// Wavefronts per tile of the join: enough to give a small matrix's few tiles
+// about 4096 wavefronts (four per SIMD), one for large matrices.
+uint32_t join_splits(uint32_t S, uint32_t T) {
+    const uint64_t tiles = (uint64_t)((S + 63) / 64) * ((T + 63) / 64);
+    return (uint32_t)std::min<uint64_t>(16, std::max<uint64_t>(1, 4096 / std::max<uint64_t>(tiles, 1)));
+}
 
 // The bucket many-to-many (table_bucket_parallel_bounded, bucket_ch.rs:
 // 3436-3520) on the elimination tree: the target lanes' upward phase on the
@@ -3478,10 +3536,26 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
     const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1, words = std::max<uint32_t>((nseg + 63) / 64, 1);
     const auto t_start = std::chrono::steady_clock::now();
     std::unordered_map<uint32_t, std::vector<uint32_t>> tree_tgts;  // bottom tree root -> target lanes
-    float up_f = 0, join_ms = 0;
+    float join_ms = 0;
     uint32_t joins = 0;
     uint64_t pairs = 0;
-    rc = run_all(r, wb, rt, 0, [&](uint32_t, uint32_t tnl, const std::vector<uint32_t> &trows, float *) -> int {
+    // Lane capacities: the targets' arena first, the sources' from what is
+    // left.  When both sides fit one batch, their upward phases run at once
+    // (the target side on a thread of its own) and the join splits its tiles.
+    const uint32_t cap_t = r->cap_fixed ? r->lane_cap : auto_lane_cap(r, wb, T, 0);
+    bool together = cap_t >= T;
+    if (together && !r->cap_fixed) {
+        rc = ensure_arena(r, wb, (size_t)cap_t * label_stride(r, wb));
+        if (rc) return rc;
+        wb->cap_hint = cap_t;
+    }
+    const uint32_t cap_s = c->cap_fixed ? c->lane_cap : auto_lane_cap(c, wf, S, 0);
+    together &= cap_s >= S;
+    if (together && !c->cap_fixed) wf->cap_hint = cap_s;
+    const uint32_t splits = together ? join_splits(S, T) : 1;
+
+    // target batch ready on wb: its reach bits and its bottom targets by tree
+    auto targets_ready = [&](uint32_t tnl, const std::vector<uint32_t> &trows) -> int {
         const uint32_t wgb = (tnl + 63) / 64;
         ENSURE(wb, reach, (size_t)wgb * words, "reach bits");
         HIP_TRY(bfg_meet::launch_reach(wb->stream, wb->seed_off.p, wb->seed_pre.p, tnl, r->pos_pre.p, r->pos_sub.p,
@@ -3492,57 +3566,105 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
             const uint32_t p = L.pos_of_rank[tgt[trows[b]]];
             if (p >= L.n_top) tree_tgts[bottom_root(L, p)].push_back(b);
         }
-        const int rc2 = run_all(c, wf, rs, 0, [&](uint32_t, uint32_t snl, const std::vector<uint32_t> &srows,
-                                                  float *) -> int {
-            const uint32_t wga = (snl + 63) / 64;
-            ENSURE(wf, reach, (size_t)wga * words, "reach bits");
-            HIP_TRY(hipEventRecord(wf->ev[0], wf->stream));
-            HIP_TRY(bfg_meet::launch_reach(wf->stream, wf->seed_off.p, wf->seed_pre.p, snl, c->pos_pre.p,
-                                           c->pos_sub.p, c->seg_pos.p, nseg, words, wf->reach.p));
-            const bfg_meet::Side A{wf->arena.p, L.n_top, c->lanes, snl, wf->reach.p, wf->row_map.p};
-            const bfg_meet::Side B{wb->arena.p, L.n_top, r->lanes, tnl, wb->reach.p, wb->row_map.p};
-            HIP_TRY(bfg_meet::launch_join(wf->stream, A, B, c->seg_pos.p, words, thr, d_out, T));
-            // sources and targets in one bottom tree also meet below the top region
-            std::vector<uint32_t> &pr = wf->h_pairs;
-            pr.clear();
-            if (!tree_tgts.empty())
-                for (uint32_t a = 0; a < snl; ++a) {
-                    const uint32_t p = L.pos_of_rank[src[srows[a]]];
-                    if (p < L.n_top) continue;
-                    const auto it = tree_tgts.find(bottom_root(L, p));
-                    if (it == tree_tgts.end()) continue;
-                    for (uint32_t b : it->second) {
-                        pr.push_back(a);
-                        pr.push_back(b);
-                    }
+        return BFG_SUCCESS;
+    };
+    // the join of source batch snl (on wf) with the ready target batch tnl
+    auto join = [&](uint32_t snl, const std::vector<uint32_t> &srows, uint32_t tnl) -> int {
+        const uint32_t wga = (snl + 63) / 64;
+        ENSURE(wf, reach, (size_t)wga * words, "reach bits");
+        HIP_TRY(hipEventRecord(wf->ev[0], wf->stream));
+        HIP_TRY(bfg_meet::launch_reach(wf->stream, wf->seed_off.p, wf->seed_pre.p, snl, c->pos_pre.p, c->pos_sub.p,
+                                       c->seg_pos.p, nseg, words, wf->reach.p));
+        if (splits > 1) HIP_TRY(bfg_meet::launch_fill_inf(wf->stream, d_out, (uint64_t)S * T));
+        const bfg_meet::Side A{wf->arena.p, L.n_top, c->lanes, snl, wf->reach.p, wf->row_map.p};
+        const bfg_meet::Side B{wb->arena.p, L.n_top, r->lanes, tnl, wb->reach.p, wb->row_map.p};
+        HIP_TRY(bfg_meet::launch_join(wf->stream, A, B, c->seg_pos.p, words, thr, splits, d_out, T));
+        // sources and targets in one bottom tree also meet below the top region
+        std::vector<uint32_t> &pr = wf->h_pairs;
+        pr.clear();
+        if (!tree_tgts.empty())
+            for (uint32_t a = 0; a < snl; ++a) {
+                const uint32_t p = L.pos_of_rank[src[srows[a]]];
+                if (p < L.n_top) continue;
+                const auto it = tree_tgts.find(bottom_root(L, p));
+                if (it == tree_tgts.end()) continue;
+                for (uint32_t b : it->second) {
+                    pr.push_back(a);
+                    pr.push_back(b);
                 }
-            const StreamDrain drain{wf->stream};  // the pair list is read by an asynchronous copy
-            if (!pr.empty()) {
-                ENSURE(wf, pairs, pr.size(), "bottom pairs");
-                HIP_TRY(hipMemcpyAsync(wf->pairs.p, pr.data(), pr.size() * 4, hipMemcpyHostToDevice, wf->stream));
-                const bfg_meet::PathSide PA{wf->seed_off.p, wf->seed_pos.p, wf->pathbuf.p, wf->row_map.p};
-                const bfg_meet::PathSide PB{wb->seed_off.p, wb->seed_pos.p, wb->pathbuf.p, wb->row_map.p};
-                HIP_TRY(bfg_meet::launch_bottom_pairs(wf->stream, wf->pairs.p, (uint32_t)(pr.size() / 2), PA, PB,
-                                                      d1_of(c), c->pos_parent.p, c->pos_depth.p, L.n_top, thr,
-                                                      d_out, T));
-                pairs += pr.size() / 2;
             }
-            HIP_TRY(hipEventRecord(wf->ev[3], wf->stream));
-            HIP_TRY(hipEventSynchronize(wf->ev[3]));
-            float ms = 0;
-            HIP_TRY(hipEventElapsedTime(&ms, wf->ev[0], wf->ev[3]));
-            join_ms += ms;
-            ++joins;
+        const StreamDrain drain{wf->stream};  // the pair list is read by an asynchronous copy
+        if (!pr.empty()) {
+            ENSURE(wf, pairs, pr.size(), "bottom pairs");
+            HIP_TRY(hipMemcpyAsync(wf->pairs.p, pr.data(), pr.size() * 4, hipMemcpyHostToDevice, wf->stream));
+            const bfg_meet::PathSide PA{wf->seed_off.p, wf->seed_pos.p, wf->pathbuf.p, wf->row_map.p};
+            const bfg_meet::PathSide PB{wb->seed_off.p, wb->seed_pos.p, wb->pathbuf.p, wb->row_map.p};
+            HIP_TRY(bfg_meet::launch_bottom_pairs(wf->stream, wf->pairs.p, (uint32_t)(pr.size() / 2), PA, PB,
+                                                  d1_of(c), c->pos_parent.p, c->pos_depth.p, L.n_top, thr, d_out,
+                                                  T));
+            pairs += pr.size() / 2;
+        }
+        HIP_TRY(hipEventRecord(wf->ev[3], wf->stream));
+        HIP_TRY(hipEventSynchronize(wf->ev[3]));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, wf->ev[0], wf->ev[3]));
+        join_ms += ms;
+        ++joins;
+        return BFG_SUCCESS;
+    };
+
+    float up_f = 0;
+    if (together) {
+        MeetSync ms;
+        uint32_t t_lanes = 0;
+        int rc_t = BFG_SUCCESS;
+        std::string err_t;
+        std::thread th([&] {
+            if (hipSetDevice(c->device) != hipSuccess) rc_t = fail(BFG_DEVICE_ERROR, "hipSetDevice");
+            if (!rc_t)
+                rc_t = run_all(r, wb, rt, 0, [&](uint32_t, uint32_t tnl, const std::vector<uint32_t> &trows,
+                                                 float *) -> int {
+                    const int rc2 = targets_ready(tnl, trows);
+                    if (rc2) return rc2;
+                    t_lanes = tnl;
+                    ms.set(&MeetSync::ready);
+                    std::unique_lock<std::mutex> g(ms.mu);
+                    ms.cv.wait(g, [&] { return ms.done; });  // the join reads this batch's labels
+                    return BFG_SUCCESS;
+                });
+            if (rc_t) err_t = g_last_error;
+            ms.set(&MeetSync::failed);  // wakes a join still waiting (no-op once ready)
+        });
+        rc = run_all(c, wf, rs, 0, [&](uint32_t, uint32_t snl, const std::vector<uint32_t> &srows,
+                                       float *) -> int {
+            {
+                std::unique_lock<std::mutex> g(ms.mu);
+                ms.cv.wait(g, [&] { return ms.ready || ms.failed; });
+                if (!ms.ready) return BFG_INTERNAL_ERROR;  // the target side failed; its error is reported
+            }
+            return join(snl, srows, t_lanes);
+        });
+        const std::string err_s = g_last_error;
+        ms.set(&MeetSync::done);
+        th.join();
+        if (rc_t) return fail(rc_t, err_t);
+        if (rc) return fail(rc, err_s);
+        up_f = wf->last.up_ms;
+    } else {
+        rc = run_all(r, wb, rt, 0, [&](uint32_t, uint32_t tnl, const std::vector<uint32_t> &trows, float *) -> int {
+            int rc2 = targets_ready(tnl, trows);
+            if (rc2) return rc2;
+            rc2 = run_all(c, wf, rs, 0, [&](uint32_t, uint32_t snl, const std::vector<uint32_t> &srows,
+                                            float *) -> int { return join(snl, srows, tnl); });
+            if (rc2) return rc2;
+            up_f += wf->last.up_ms;
             return BFG_SUCCESS;
         });
-        if (rc2) return rc2;
-        up_f += wf->last.up_ms;
-        return BFG_SUCCESS;
-    });
-    if (rc) return rc;
+        if (rc) return rc;
+    }
     if (host_out) HIP_TRY(hipMemcpy(host_out, d_out, (size_t)S * T * 4, hipMemcpyDeviceToHost));
     bfg_timings tm{};
-    tm.up_ms = up_f + wb->last.up_ms;  // forward and backward searches
+    tm.up_ms = up_f + wb->last.up_ms;  // forward and backward searches (overlapping when run at once)
     tm.sweep_ms = join_ms;             // the join takes the place of the DOWN sweep
     tm.batches = joins;
     tm.sweep_launches = joins;
@@ -3697,6 +3819,27 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     c->idle.push_back(c->works.front().get());
     *out = c.release();
     return BFG_SUCCESS;
+}
+
+int bfg_ctx_create_reverse(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
+    if (!gd || !out) return fail(BFG_INVALID_PARAMETER, "null argument");
+    *out = nullptr;
+    const uint32_t n = gd->n_nodes;
+    if (!gd->up_offsets || !gd->down_offsets) return fail(BFG_INVALID_PARAMETER, "null offsets");
+    for (uint32_t r = 0; r < n; ++r)
+        if (gd->up_offsets[r + 1] < gd->up_offsets[r] || gd->down_offsets[r + 1] < gd->down_offsets[r])
+            return fail(BFG_INVALID_PARAMETER, "offsets not monotone");
+    if (gd->up_offsets[0] != 0 || gd->down_offsets[0] != 0) return fail(BFG_INVALID_PARAMETER, "offsets must start at 0");
+    if ((gd->up_offsets[n] && (!gd->up_targets || !gd->up_weights)) ||
+        (gd->down_offsets[n] && (!gd->down_targets || !gd->down_weights)))
+        return fail(BFG_INVALID_PARAMETER, "null arc buffer");
+    Transposed t;
+    if (!flip_csr(n, gd->down_offsets, gd->down_targets, gd->down_weights, t.up_off, t.up_tgt, t.up_w) ||
+        !flip_csr(n, gd->up_offsets, gd->up_targets, gd->up_weights, t.dn_off, t.dn_tgt, t.dn_w))
+        return fail(BFG_INVALID_PARAMETER, "arc target out of range");
+    const bfg_graph_desc td{n, t.up_off.data(), t.up_tgt.data(), t.up_w.data(), t.dn_off.data(), t.dn_tgt.data(),
+                            t.dn_w.data()};
+    return bfg_ctx_create(&td, device, out);
 }
 
 void bfg_ctx_destroy(bfg_ctx *c) { delete c; }

@@ -17,6 +17,8 @@
 // cells equal the reference's bucket join and its PHAST matrix bit for bit.
 #include "meet.hpp"
 
+#include <algorithm>
+
 namespace bfg_meet {
 namespace {
 
@@ -65,12 +67,14 @@ __global__ __launch_bounds__(64) void meet_reach_kernel(const uint32_t *__restri
 // per step: acc = min3(acc, a0 + b0, a1 + b1).
 // Tiles go to the XCDs in contiguous ranges (block b runs on XCD b % 8), so
 // the tiles of one source wave group run side by side on one XCD and share its
-// labels in that XCD's L2.
+// labels in that XCD's L2.  With `splits` > 1 (few tiles), split sp of a tile
+// takes its common segments k = sp, sp + splits, .. and merges its cells with
+// atomic min into an output filled with INF beforehand.
 // ---------------------------------------------------------------------------
 struct JoinArgs {
     Side A, B;
     const uint32_t *seg_pos;
-    uint32_t words, wga, wgb, thr;
+    uint32_t words, wga, wgb, thr, splits;
     uint32_t *out;
     uint64_t ld;
 };
@@ -83,10 +87,11 @@ __device__ __forceinline__ void load8(uint32_t (&v)[8], const uint32_t *p) {
 }
 
 __global__ __launch_bounds__(64) void meet_join_kernel(JoinArgs a) {
-    const uint32_t tiles = a.wga * a.wgb;
+    const uint32_t tiles = a.wga * a.wgb * a.splits;
     const uint32_t per_xcd = (tiles + XCDS - 1) / XCDS;
-    const uint32_t t = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
-    if (t >= tiles) return;
+    const uint32_t u = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
+    if (u >= tiles) return;
+    const uint32_t t = u / a.splits, sp = u % a.splits;
     const uint32_t ta = t / a.wgb, tb = t % a.wgb;
     const uint32_t i = threadIdx.x >> 3, j = threadIdx.x & 7;
     const uint32_t la = ta * 64 + 8 * i, lb = tb * 64 + 8 * j;
@@ -98,11 +103,13 @@ __global__ __launch_bounds__(64) void meet_join_kernel(JoinArgs a) {
     uint32_t acc[64];
 #pragma unroll
     for (int k = 0; k < 64; ++k) acc[k] = INF;
+    uint32_t k = 0;  // common segments seen so far
     for (uint32_t w = 0; w < a.words; ++w) {
         unsigned long long m = ra[w] & rb[w];
         while (m) {
             const uint32_t s = w * 64 + (uint32_t)__builtin_ctzll(m);
             m &= m - 1;
+            if (k++ % a.splits != sp) continue;
             const uint32_t p0 = a.seg_pos[s], p1 = a.seg_pos[s + 1];
             for (uint32_t x = p0; x < p1; x += 2) {
                 const uint32_t y = min(x + 1, p1 - 1);  // an odd last position pairs with itself
@@ -127,8 +134,12 @@ __global__ __launch_bounds__(64) void meet_join_kernel(JoinArgs a) {
         if (la + r >= a.A.n_lanes) break;
         uint32_t *o = a.out + (uint64_t)a.A.index[la + r] * a.ld;
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
-            if (lb + c < a.B.n_lanes) o[col[c]] = acc[r * 8 + c] > a.thr ? INF : acc[r * 8 + c];
+        for (int c = 0; c < 8; ++c) {
+            if (lb + c >= a.B.n_lanes) continue;
+            const uint32_t v = acc[r * 8 + c] > a.thr ? INF : acc[r * 8 + c];
+            if (a.splits == 1) o[col[c]] = v;
+            else if (v != INF) atomicMin(o + col[c], v);
+        }
     }
 }
 
@@ -165,7 +176,18 @@ __global__ __launch_bounds__(256) void meet_bottom_kernel(const uint32_t *__rest
     if (best <= thr) atomicMin(out + (uint64_t)A.index[la] * ld + B.index[lb], best);
 }
 
+__global__ __launch_bounds__(256) void meet_fill_kernel(uint32_t *__restrict__ p, uint64_t count) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256) p[i] = INF;
+}
+
 }  // namespace
+
+hipError_t launch_fill_inf(hipStream_t st, uint32_t *p, uint64_t count) {
+    if (!count) return hipSuccess;
+    const uint64_t blocks = std::min<uint64_t>((count + 255) / 256, 16384);
+    hipLaunchKernelGGL(meet_fill_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, p, count);
+    return hipGetLastError();
+}
 
 hipError_t launch_reach(hipStream_t st, const uint32_t *seed_off, const uint32_t *seed_pre, uint32_t n_lanes,
                         const uint32_t *pos_pre, const uint32_t *pos_sub, const uint32_t *seg_pos, uint32_t nseg,
@@ -178,9 +200,10 @@ hipError_t launch_reach(hipStream_t st, const uint32_t *seed_off, const uint32_t
 }
 
 hipError_t launch_join(hipStream_t st, const Side &A, const Side &B, const uint32_t *seg_pos, uint32_t words,
-                       uint32_t thr, uint32_t *out, uint64_t ld) {
-    JoinArgs a{A, B, seg_pos, words, (A.n_lanes + 63) / 64, (B.n_lanes + 63) / 64, thr, out, ld};
-    const uint64_t tiles = (uint64_t)a.wga * a.wgb;
+                       uint32_t thr, uint32_t splits, uint32_t *out, uint64_t ld) {
+    JoinArgs a{A, B, seg_pos, words, (A.n_lanes + 63) / 64, (B.n_lanes + 63) / 64, thr, splits ? splits : 1, out,
+               ld};
+    const uint64_t tiles = (uint64_t)a.wga * a.wgb * a.splits;
     if (!tiles) return hipSuccess;
     const uint64_t blocks = (tiles + XCDS - 1) / XCDS * XCDS;
     if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;

@@ -260,6 +260,13 @@ int bfg_table_seeded(bfg_ctx *ctx, const uint32_t *src_offsets, const uint32_t *
  * positions or arc counts) or is on another device. */
 int bfg_ctx_pair_reverse(bfg_ctx *ctx, bfg_ctx *rev);
 
+/* bfg_ctx_create over the transposed graph of `graph` (the reverse metric's
+ * CCH, whose forward PHAST is the reference's reverse PHAST,
+ * range/phast.rs:879-1108, and whose upward phase is the bucket engine's
+ * backward search over DownReverseAdjFlat, bucket_ch.rs:191-424): the
+ * library transposes the CSR itself (per-tail targets ascending). */
+int bfg_ctx_create_reverse(const bfg_graph_desc *graph, int device, bfg_ctx **out);
+
 /* table_bucket_parallel_bounded(n_nodes, up_adj_flat, down_rev_flat, sources,
  * targets, threshold) (matrix/bucket_ch.rs:3436-3520): the forward UP search
  * of every source and the backward search of every target meet at their

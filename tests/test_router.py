@@ -105,6 +105,14 @@ class MatStub:
             self._rev = MatStub(self.g.transpose())
         return self._rev
 
+    def table_bucket_parallel_bounded(self, s, t, thr):
+        self.calls += 1
+        m = self.o.table_bucket(s, t, thr)
+
+        class St:  # the engine's phase times of a bucket call
+            upward_time_ms, downward_time_ms = 3.0, 1.0
+        return np.where(m > thr, INF, m).astype(np.uint32), St()
+
 
 def test_matrix_router_sides_agree(grid_small):
     g = grid_small
@@ -121,6 +129,29 @@ def test_matrix_router_sides_agree(grid_small):
     assert r.fwd_ns.load() > 0 and r.rev_ns.load() > 0
 
 
+def test_matrix_router_bucket_side(grid_small):
+    """Square shapes go to the bucket many-to-many (tried first once both
+    sides have BUCKET_MIN lanes, then by measured cost); lopsided ones to
+    PHAST over the smaller side; the cells are the same."""
+    g = grid_small
+    rng = np.random.default_rng(9)
+    r = router.MatrixRouter(MatStub(g))
+    o = Oracle(g)
+    for S, T in ((200, 150), (3, 400), (200, 150)):
+        s = rng.integers(0, g.n, S).astype(np.uint32)
+        t = rng.integers(0, g.n, T).astype(np.uint32)
+        got = r.compute_matrix_block_gated(s, t, 1500)
+        exp = o.compute_matrix_flat(s, t)
+        assert np.array_equal(got, np.where(exp > 1500, INF, exp))
+        if (S, T) == (3, 400):
+            assert r.last_side == "forward"
+    assert r.up_ns.load() > 0 and r.cell_ns.load() > 0
+    # with both measured, the estimate decides: a cheap join keeps squares on the bucket
+    r.fwd_ns.value, r.up_ns.value, r.cell_ns.value = 10**6, 10**3, 1
+    assert r.side(1000, 1000) == "bucket" and r.side(2, 10**6) == "forward"
+    assert router.MatrixRouter(MatStub(g), bucket=False).side(1000, 1000) == "forward"
+
+
 @pytest.mark.gpu
 def test_matrix_router_on_gpu(grid_config1):
     from bfgpu import BatchedPhastEngine
@@ -129,7 +160,7 @@ def test_matrix_router_on_gpu(grid_config1):
     o = Oracle(g)
     with BatchedPhastEngine(g) as e:
         r = router.MatrixRouter(e)
-        for S, T in ((300, 20), (20, 300), (64, 64)):
+        for S, T in ((300, 20), (20, 300), (64, 64), (200, 300)):
             s = rng.integers(0, g.n, S).astype(np.uint32)
             t = rng.integers(0, g.n, T).astype(np.uint32)
             got = r.compute_matrix_block_gated(s, t)

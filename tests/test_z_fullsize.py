@@ -7,6 +7,8 @@ Every check is bit-exact against the oracle (u32 integer work):
 - 1000 x 1000 matrix as one plain call with K = 32 lanes, every cell;
 - 10k x 10k matrix delivered to the host, 256 sampled rows (hot-spot sources,
   the lowest- and highest-rank sources);
+- the bucket many-to-many (bfg_table_bucket) on both matrices: every cell
+  equal to the PHAST matrix (1000 x 1000 also against the oracle);
 - 4096 isochrones with uneven 5-60 minute limits in one call: on every lane
   the settled list equals the reach bits (ascending ranks, times within the
   limit, the origin at 0); 32 lanes against the oracle's PHAST rows masked to
@@ -68,6 +70,12 @@ def test_config2_matrix_1000_k32(country, request):
     exp = o.compute_matrix_flat_mt(s, t)
     assert np.array_equal(got, exp)
     assert (got != INF).mean() > 0.9
+    # the bucket many-to-many (upward phases on the CCH and its transpose,
+    # joined at common ancestors): the same cells
+    e.pair_reverse()
+    mb, sb = e.table_bucket_parallel(s, t)
+    progress(request.config, f"config2 bucket {sb.total_time_ms:.1f} ms")
+    assert np.array_equal(mb, exp)
 
 
 def test_config3_matrix_10k_host_rows(country, request):
@@ -86,6 +94,11 @@ def test_config3_matrix_10k_host_rows(country, request):
     exp = o.compute_matrix_flat_mt(s[rows], t).reshape(rows.size, 10000)
     assert rows.size >= 250
     assert np.array_equal(got[rows], exp)
+    # the bucket many-to-many: every one of the 10^8 cells equals the PHAST matrix
+    e.pair_reverse()
+    mb, sb = e.table_bucket_parallel(s, t)
+    progress(request.config, f"config3 bucket {sb.total_time_ms:.1f} ms, bottom pairs {sb.nodes_swept}")
+    assert np.array_equal(mb.reshape(10000, 10000), got)
 
 
 def test_config4_isochrones_4096(country, request):
