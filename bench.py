@@ -11,6 +11,14 @@ Prints one JSON line with `roofline` (the sweep kernel against the 8 TB/s HBM
 peak) and `cpu_baseline` (the oracle's restatement of the reference's
 compute_matrix_flat, timed on a bounded sample on this host).
 
+The same matrix also runs through the router's bucket side
+(bfg_table_bucket_device: upward phases of the sources on the CCH and of the
+targets on the transposed CCH, joined at their common ancestors); `value` is
+the side the router takes with both measured (the bucket at 10k x 10k, every
+cell checked equal to the PHAST matrix), and `roofline` is that step's join
+kernel against the VALU issue bound, with PHAST's DOWN rows kernel beside it
+(`roofline_phast`).
+
 Other modes print their own JSON line instead: --serving (continuous batching
 against plain per-request calls), --isochrones L (block-gated isochrones with
 uneven limits), --table-2ch S (a two-channel seeded table: times and lengths
@@ -35,6 +43,8 @@ from bfgpu.workload import WORKLOADS, isochrone_lanes, make_queries, serving_req
 
 METRIC = "matrix cells/s 10k×10k u32 bit-exact, % HBM roofline; serving p50/p99 vs plain"
 HBM_PEAK_GBPS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs, one 64-lane wave instruction per 4 cycles at 2.4 GHz
+VALU_PEAK_GINSTR = 256 * 4 * 2.4e9 / 4 / 1e9
 INF_U32 = 0xFFFFFFFF
 
 def log(*a):
@@ -128,6 +138,25 @@ def cpu_baseline(g, src, tgt, budget_s, gpu_rows):
             "bit_exact_vs_gpu_rows": ok}, ok
 
 
+def cpu_baseline_bucket(g, src, tgt, S, T, gpu_block, k=4):
+    """The oracle's bucket many-to-many (table_bucket_parallel semantics,
+    single thread) on k sources x k targets: (k + k) sweeps whose per-sweep
+    time, times the S + T sweeps of the full matrix, gives its cells/s; the
+    k x k cells are checked against the GPU's."""
+    from oracle.oracle import Oracle
+    o = Oracle(g)
+    o.table_bucket(src[:1], tgt[:1])  # builds the flats (untimed)
+    t0 = time.perf_counter()
+    m = o.table_bucket(src[:k], tgt[:k]).reshape(k, k)
+    dt = time.perf_counter() - t0
+    per_sweep = dt / (2 * k)
+    full_s = per_sweep * (S + T)
+    return {"value": S * T / full_s, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": f"orc_table_bucket on {k} sources x {k} targets ({2 * k} sweeps, {dt:.1f} s single-thread), "
+                      f"scaled by sweeps to the {S} x {T} matrix's {S + T} sweeps",
+            "bit_exact_vs_gpu_block": bool(np.array_equal(m, gpu_block[:k, :k]))}
+
+
 def cpu_baseline_mt(g, src, tgt, budget_s):
     from oracle.oracle import Oracle
     o = Oracle(g)
@@ -160,12 +189,15 @@ def bucket_bench(eng, d_src, d_tgt, S, T, d_phast, args, phast_ms):
         st = bstep()
         log(f"bucket warmup {i}: up {st.upward_time_ms:.1f} ms join {st.downward_time_ms:.1f} ms")
     sync()
-    up = join = 0.0
+    up = join = kern = 0.0
+    visits = 0
     t0 = time.perf_counter()
     for i in range(args.steps):
         st = bstep()
         up += st.upward_time_ms
         join += st.downward_time_ms
+        kern += st.join_ms
+        visits += st.join_visits
         log(f"bucket step {i}: up {st.upward_time_ms:.1f} ms join {st.downward_time_ms:.1f} ms")
     sync()
     ms = 1000.0 * (time.perf_counter() - t0) / args.steps
@@ -178,10 +210,24 @@ def bucket_bench(eng, d_src, d_tgt, S, T, d_phast, args, phast_ms):
     update_cost_ewma(r.up_ns, int(ms * 1e6 * (1 - share)) // (S + T))
     update_cost_ewma(r.cell_ns, max(int(ms * 1e6 * share) // (S * T), 1))
     side = r.side(S, T)
+    # the join kernel against the VALU issue bound: a visit is one label
+    # position of one 64 x 64 tile, 4096 cell updates of one v_add_u32 (clamp)
+    # each and one v_min3_u32 per two (1.5 wave instructions per 64 updates);
+    # it reads 64 source and 64 target labels (512 bytes) per visit
+    kms, vis = kern / args.steps, visits / args.steps
+    instr = vis * 4096 * 1.5 / 64
+    achieved = instr / (kms / 1000.0) / 1e9 if kms > 0 else 0.0
+    hbm = (vis * 512 + 4 * S * T) / (kms / 1000.0) / 1e9 if kms > 0 else 0.0
+    roof = {"bound": "valu", "kernel": "meet_join_kernel (the bucket join: 64 x 64 min-plus tiles)",
+            "achieved": achieved, "peak": VALU_PEAK_GINSTR, "unit": "G wave-instructions/s (algorithmic)",
+            "frac": achieved / VALU_PEAK_GINSTR, "traffic": None, "launches": 1, "avg_launch_ms": kms,
+            "visits_per_launch": vis, "cell_updates_per_launch": vis * 4096,
+            "algorithmic_bytes_per_launch": vis * 512 + 4 * S * T, "hbm_gbps": hbm, "hbm_frac": hbm / HBM_PEAK_GBPS}
     line = {"ms_per_step": ms, "cells_per_s": S * T / (ms / 1000.0), "steps": args.steps,
-            "phases_ms_per_step": {"up_both_sides": up / args.steps, "join": join / args.steps},
+            "phases_ms_per_step": {"up_both_sides_overlapped": up / args.steps, "join_with_reach_and_pairs":
+                                   join / args.steps, "join_kernel": kms},
             "entry_point": "bfg_table_bucket_device", "every_cell_equal_to_phast": same,
-            "bottom_pairs": int(st.nodes_swept)}
+            "bottom_pairs": int(st.nodes_swept), "roofline": roof}
     return line, side
 
 
@@ -554,7 +600,7 @@ def main():
                 "rows_equal_device_rows": bool(np.array_equal(pinned.a[:n_check * T], gpu_rows.reshape(-1)))}
         log(f"host-delivered {host_ms:.1f} ms per matrix")
         pinned.free()
-    cpu = mt = None
+    cpu = mt = cpu_b = None
     exact = None
     if not args.no_cpu:
         log("cpu baseline (single thread) ...")
@@ -562,20 +608,26 @@ def main():
         log(f"cpu baseline {cpu['value']:.3g} cells/s, bit-exact rows: {exact}")
         mt = cpu_baseline_mt(g, src, tgt, args.cpu_budget)
         log(f"cpu baseline all cores {mt['value']:.3g} cells/s ({mt['cores']} threads)")
+        if bucket is not None:
+            cpu_b = cpu_baseline_bucket(g, src, tgt, S, T, gpu_rows)
+            log(f"cpu bucket baseline {cpu_b['value']:.3g} cells/s (extrapolated), exact: "
+                f"{cpu_b['bit_exact_vs_gpu_block']}")
 
     phast_line = {"ms_per_step": ms_per_step, "cells_per_s": value}
-    if routed == "bucket" and bucket["every_cell_equal_to_phast"]:
+    use_bucket = routed == "bucket" and bucket["every_cell_equal_to_phast"]
+    if use_bucket:
         # the matrix call a user makes goes where the router sends it
         ms_per_step, value = bucket["ms_per_step"], bucket["cells_per_s"]
     res = {
         "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "routed_to": routed or "phast", "higher_is_better": True, "scaling": "weak",
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "routed_to": None, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": f"{S}x{T} duration matrix, {rows}x{rows} jittered grid"
                                f"{' + 2% motorway chords' if chords else ''}, edge-based CCH n={n}",
                    "sources": S, "targets": T, "graph_seed": args.seed, "lanes": KL,
                    "lane_capacity": cap, "n_nodes": n, "m_up": info["m_up"], "m_down": m_dn,
                    "etree_depth": info["etree_depth"], "parallelism": "single GPU", "autotune": tuned},
+        "roofline_phast": None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
@@ -597,8 +649,15 @@ def main():
         "host_delivered": host,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": mt,
+        "cpu_baseline_bucket": cpu_b,
         "bit_exact_sample": exact,
     }
+    if use_bucket:  # the routed step's dominant kernel; PHAST's DOWN rows beside it
+        res["roofline_phast"] = res["roofline"]
+        res["roofline"] = bucket["roofline"]
+    else:
+        res.pop("roofline_phast")
+    res["routed_to"] = "bucket" if use_bucket else "phast"
     print(json.dumps(res), flush=True)
 
 

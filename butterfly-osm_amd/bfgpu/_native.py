@@ -34,7 +34,8 @@ class Timings(C.Structure):
                 ("lanes_run", C.c_uint64), ("nodes_swept", C.c_uint64), ("arcs_swept", C.c_uint64),
                 ("dom_ms", C.c_float), ("dom_launches", C.c_uint32), ("dom_bytes", C.c_uint64),
                 ("gate_seg_units", C.c_uint64), ("gate_seg_open", C.c_uint64),
-                ("gate_bt_units", C.c_uint64), ("gate_bt_open", C.c_uint64)]
+                ("gate_bt_units", C.c_uint64), ("gate_bt_open", C.c_uint64),
+                ("join_ms", C.c_float), ("join_visits", C.c_uint64)]
 
 
 class BfcParams(C.Structure):

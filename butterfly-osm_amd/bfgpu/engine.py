@@ -50,6 +50,8 @@ class BatchedPhastStats:
     gate_seg_open: int = 0
     gate_bt_units: int = 0   # the same for (bottom tree, wave group)
     gate_bt_open: int = 0
+    join_ms: float = 0.0     # bucket matrices: the join kernel's time and label positions visited per tile
+    join_visits: int = 0
 
 
 @dataclasses.dataclass
@@ -126,7 +128,7 @@ class BatchedPhastEngine:
                                  arcs_swept=t.arcs_swept, dom_ms=t.dom_ms, dom_launches=t.dom_launches,
                                  dom_bytes=t.dom_bytes, gate_seg_units=t.gate_seg_units,
                                  gate_seg_open=t.gate_seg_open, gate_bt_units=t.gate_bt_units,
-                                 gate_bt_open=t.gate_bt_open)
+                                 gate_bt_open=t.gate_bt_open, join_ms=t.join_ms, join_visits=t.join_visits)
 
     def configure(self, lanes=0, lane_capacity=0):
         _check(self._lib.bfg_ctx_configure(self._h, lanes, lane_capacity))

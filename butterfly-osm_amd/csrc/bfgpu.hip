@@ -1592,7 +1592,7 @@ struct Work {
     // bits and bottom pairs
     bool up_only = false;
     uint32_t cap_hint = 0;  // lanes per batch chosen by the caller (0: run_all chooses)
-    DevBuf<unsigned long long> reach;
+    DevBuf<unsigned long long> reach, visits;
     DevBuf<uint32_t> pairs;
     std::vector<uint32_t> h_pairs;
     // host staging of a call (kept, so a fixed Work reuses its capacity)
@@ -3536,9 +3536,9 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
     const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1, words = std::max<uint32_t>((nseg + 63) / 64, 1);
     const auto t_start = std::chrono::steady_clock::now();
     std::unordered_map<uint32_t, std::vector<uint32_t>> tree_tgts;  // bottom tree root -> target lanes
-    float join_ms = 0;
+    float join_ms = 0, kernel_ms = 0;
     uint32_t joins = 0;
-    uint64_t pairs = 0;
+    uint64_t pairs = 0, visits = 0;
     // Lane capacities: the targets' arena first, the sources' from what is
     // left.  When both sides fit one batch, their upward phases run at once
     // (the target side on a thread of its own) and the join splits its tiles.
@@ -3575,10 +3575,15 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
         HIP_TRY(hipEventRecord(wf->ev[0], wf->stream));
         HIP_TRY(bfg_meet::launch_reach(wf->stream, wf->seed_off.p, wf->seed_pre.p, snl, c->pos_pre.p, c->pos_sub.p,
                                        c->seg_pos.p, nseg, words, wf->reach.p));
-        if (splits > 1) HIP_TRY(bfg_meet::launch_fill_inf(wf->stream, d_out, (uint64_t)S * T));
+        if (splits > 1) HIP_TRY(bfg_meet::launch_fill(wf->stream, d_out, (uint64_t)S * T, INF));
+        ENSURE(wf, visits, 1, "join counter");
+        HIP_TRY(bfg_meet::launch_fill(wf->stream, (uint32_t *)wf->visits.p, 2, 0));
         const bfg_meet::Side A{wf->arena.p, L.n_top, c->lanes, snl, wf->reach.p, wf->row_map.p};
         const bfg_meet::Side B{wb->arena.p, L.n_top, r->lanes, tnl, wb->reach.p, wb->row_map.p};
-        HIP_TRY(bfg_meet::launch_join(wf->stream, A, B, c->seg_pos.p, words, thr, splits, d_out, T));
+        HIP_TRY(hipEventRecord(wf->ev[1], wf->stream));
+        HIP_TRY(bfg_meet::launch_join(wf->stream, A, B, c->seg_pos.p, words, thr, splits, d_out, T,
+                                      wf->visits.p));
+        HIP_TRY(hipEventRecord(wf->ev[2], wf->stream));
         // sources and targets in one bottom tree also meet below the top region
         std::vector<uint32_t> &pr = wf->h_pairs;
         pr.clear();
@@ -3606,9 +3611,14 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
         }
         HIP_TRY(hipEventRecord(wf->ev[3], wf->stream));
         HIP_TRY(hipEventSynchronize(wf->ev[3]));
-        float ms = 0;
+        float ms = 0, kms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, wf->ev[0], wf->ev[3]));
+        HIP_TRY(hipEventElapsedTime(&kms, wf->ev[1], wf->ev[2]));
         join_ms += ms;
+        kernel_ms += kms;
+        unsigned long long cnt = 0;
+        HIP_TRY(hipMemcpy(&cnt, wf->visits.p, sizeof(cnt), hipMemcpyDeviceToHost));
+        visits += cnt;
         ++joins;
         return BFG_SUCCESS;
     };
@@ -3670,6 +3680,8 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
     tm.sweep_launches = joins;
     tm.lanes_run = (uint64_t)S + T;
     tm.nodes_swept = pairs;  // pairs that met inside a bottom tree
+    tm.join_ms = kernel_ms;
+    tm.join_visits = visits;
     tm.total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     std::lock_guard<std::mutex> g(c->last_mu);
     c->last = tm;

@@ -77,6 +77,7 @@ struct JoinArgs {
     uint32_t words, wga, wgb, thr, splits;
     uint32_t *out;
     uint64_t ld;
+    unsigned long long *visits;
 };
 
 __device__ __forceinline__ void load8(uint32_t (&v)[8], const uint32_t *p) {
@@ -103,7 +104,7 @@ __global__ __launch_bounds__(64) void meet_join_kernel(JoinArgs a) {
     uint32_t acc[64];
 #pragma unroll
     for (int k = 0; k < 64; ++k) acc[k] = INF;
-    uint32_t k = 0;  // common segments seen so far
+    uint32_t k = 0, seen = 0;  // common segments seen so far, positions visited
     for (uint32_t w = 0; w < a.words; ++w) {
         unsigned long long m = ra[w] & rb[w];
         while (m) {
@@ -111,6 +112,7 @@ __global__ __launch_bounds__(64) void meet_join_kernel(JoinArgs a) {
             m &= m - 1;
             if (k++ % a.splits != sp) continue;
             const uint32_t p0 = a.seg_pos[s], p1 = a.seg_pos[s + 1];
+            seen += p1 - p0;
             for (uint32_t x = p0; x < p1; x += 2) {
                 const uint32_t y = min(x + 1, p1 - 1);  // an odd last position pairs with itself
                 uint32_t a0[8], a1[8], b0[8], b1[8];
@@ -126,6 +128,7 @@ __global__ __launch_bounds__(64) void meet_join_kernel(JoinArgs a) {
             }
         }
     }
+    if (a.visits && threadIdx.x == 0 && seen) atomicAdd(a.visits, (unsigned long long)seen);
     uint32_t col[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) col[c] = lb + c < a.B.n_lanes ? a.B.index[lb + c] : 0u;
@@ -176,16 +179,16 @@ __global__ __launch_bounds__(256) void meet_bottom_kernel(const uint32_t *__rest
     if (best <= thr) atomicMin(out + (uint64_t)A.index[la] * ld + B.index[lb], best);
 }
 
-__global__ __launch_bounds__(256) void meet_fill_kernel(uint32_t *__restrict__ p, uint64_t count) {
-    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256) p[i] = INF;
+__global__ __launch_bounds__(256) void meet_fill_kernel(uint32_t *__restrict__ p, uint64_t count, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256) p[i] = v;
 }
 
 }  // namespace
 
-hipError_t launch_fill_inf(hipStream_t st, uint32_t *p, uint64_t count) {
+hipError_t launch_fill(hipStream_t st, uint32_t *p, uint64_t count, uint32_t value) {
     if (!count) return hipSuccess;
     const uint64_t blocks = std::min<uint64_t>((count + 255) / 256, 16384);
-    hipLaunchKernelGGL(meet_fill_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, p, count);
+    hipLaunchKernelGGL(meet_fill_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, p, count, value);
     return hipGetLastError();
 }
 
@@ -200,9 +203,9 @@ hipError_t launch_reach(hipStream_t st, const uint32_t *seed_off, const uint32_t
 }
 
 hipError_t launch_join(hipStream_t st, const Side &A, const Side &B, const uint32_t *seg_pos, uint32_t words,
-                       uint32_t thr, uint32_t splits, uint32_t *out, uint64_t ld) {
+                       uint32_t thr, uint32_t splits, uint32_t *out, uint64_t ld, unsigned long long *visits) {
     JoinArgs a{A, B, seg_pos, words, (A.n_lanes + 63) / 64, (B.n_lanes + 63) / 64, thr, splits ? splits : 1, out,
-               ld};
+               ld, visits};
     const uint64_t tiles = (uint64_t)a.wga * a.wgb * a.splits;
     if (!tiles) return hipSuccess;
     const uint64_t blocks = (tiles + XCDS - 1) / XCDS * XCDS;

@@ -41,12 +41,13 @@ hipError_t launch_reach(hipStream_t st, const uint32_t *seed_off, const uint32_t
 // segments both lanes' wave groups reach of A.lab(a, x) + B.lab(b, x)
 // (saturating; INF where no segment is common), INF where above thr.
 // splits > 1 cuts each tile's common segments over that many wavefronts that
-// merge with atomic min: the output cells must hold INF before (fill_inf).
+// merge with atomic min: the output cells must hold INF before (launch_fill).
+// visits (when not null) gains the label positions the tiles visited.
 hipError_t launch_join(hipStream_t st, const Side &A, const Side &B, const uint32_t *seg_pos, uint32_t words,
-                       uint32_t thr, uint32_t splits, uint32_t *out, uint64_t ld);
+                       uint32_t thr, uint32_t splits, uint32_t *out, uint64_t ld, unsigned long long *visits);
 
-// p[0 .. count) = INF (a kernel, so it may sit in a captured graph).
-hipError_t launch_fill_inf(hipStream_t st, uint32_t *p, uint64_t count);
+// p[0 .. count) = value (a kernel, so it may sit in a captured graph).
+hipError_t launch_fill(hipStream_t st, uint32_t *p, uint64_t count, uint32_t value);
 
 // The pairs (pairs[2k] = source lane, pairs[2k+1] = target lane) of nodes in
 // one bottom tree: min over their common bottom ancestors y of the path slots

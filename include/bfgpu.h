@@ -95,6 +95,11 @@ typedef struct bfg_timings {
      * (segment, 64-lane wave group) units of the top region and how many the
      * gate left open, and the same for the bottom trees */
     uint64_t gate_seg_units, gate_seg_open, gate_bt_units, gate_bt_open;
+    /* bfg_table_bucket: the join kernel's duration (HIP events around its
+     * launches) and the label positions it visited, summed over its 64 x 64
+     * (source, target) tiles: each visit is 4096 min-plus cell updates */
+    float join_ms;
+    uint64_t join_visits;
 } bfg_timings;
 
 /* BatchedPhastEngine::new (batched_phast.rs:89): copies the CCH to `device`,
