@@ -86,6 +86,7 @@ BFGPU_SYMBOLS = {
                          u32p, C.POINTER(C.c_uint8)],
     "bfg_ctx_attach_length": [C.c_void_p, C.POINTER(GraphDesc), u32p, u32p],
     "bfg_ctx_pair_reverse": [C.c_void_p, C.c_void_p],
+    "bfg_device_allocations": [u64p],
     "bfg_ctx_create_reverse": [C.POINTER(GraphDesc), C.c_int, C.POINTER(C.c_void_p)],
     "bfg_table_bucket": [C.c_void_p, u32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32, u32p],
     "bfg_table_bucket_device": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32,

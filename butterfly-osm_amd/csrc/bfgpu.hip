@@ -43,6 +43,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
@@ -830,6 +831,63 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_kernel(uint32_t first, uint3
         if (j < len) lab[(size_t)(p0 + j) * KL] = acc[j];
 }
 
+// Upward phase over lists of reached units (the bucket's searches, where
+// nothing reads an unreached (segment, wave group) pair): item i = 4 *
+// blockIdx.x + w of a list (unit u[i], wave group g[i]) goes to wavefront w;
+// lists are sorted by unit, so a workgroup's wavefronts mostly share theirs.
+//   fill_list: labels of reached segment u := INF
+//   tri_list:  the intra-segment UP blocks of segment u (tri_kernel's UP work)
+//   push_list: the UP push rows of target block u (rows_kernel's PUSH work)
+template <int KL>
+__global__ __launch_bounds__(TILE_THREADS) void fill_list_kernel(uint32_t count, const uint32_t *__restrict__ lu,
+                                                                 const uint32_t *__restrict__ lg,
+                                                                 const uint32_t *__restrict__ seg_pos,
+                                                                 uint32_t *__restrict__ arena, uint32_t n) {
+    const uint32_t i = blockIdx.x * (TILE_THREADS / 64) + wave_id();
+    if (i >= count) return;  // wave-uniform; this kernel has no barrier
+    const uint32_t s = lu[i];
+    uint32_t *lab = lane_labels<KL>(arena, n, lg[i], nullptr);
+    const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
+    for (uint32_t j = 0; j < len; ++j) lab[(size_t)(p0 + j) * KL] = INF;
+}
+
+template <int KL>
+__global__ __launch_bounds__(TILE_THREADS) void tri_list_kernel(uint32_t count, const uint32_t *__restrict__ lu,
+                                                                const uint32_t *__restrict__ lg,
+                                                                const uint32_t *__restrict__ seg_pos,
+                                                                const uint32_t *__restrict__ tab,
+                                                                const uint32_t *__restrict__ tw,
+                                                                uint32_t *__restrict__ arena, uint32_t n) {
+    const uint32_t i = blockIdx.x * (TILE_THREADS / 64) + wave_id();
+    if (i >= count) return;
+    const uint32_t s = lu[i];
+    uint32_t *lab = lane_labels<KL>(arena, n, lg[i], nullptr);
+    const uint32_t p0 = seg_pos[s], len = seg_pos[s + 1] - p0;
+    uint32_t acc[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) acc[j] = j < len ? lab[(size_t)(p0 + j) * KL] : INF;
+    tri_apply<false, 8>(acc, tab + (size_t)s * 64, tw);
+#pragma unroll
+    for (int j = 0; j < 64; ++j)
+        if (j < len) lab[(size_t)(p0 + j) * KL] = acc[j];
+}
+
+template <int KL>
+__global__ __launch_bounds__(TILE_THREADS) void push_list_kernel(
+    uint32_t count, const uint32_t *__restrict__ lu, const uint32_t *__restrict__ lg, const uint32_t *__restrict__ r0,
+    const uint32_t *__restrict__ wid, const uint32_t *__restrict__ woff, const uint32_t *__restrict__ tgt,
+    const uint32_t *__restrict__ key, const uint32_t *__restrict__ wts, uint32_t *__restrict__ arena, uint32_t n) {
+    const uint32_t i = blockIdx.x * (TILE_THREADS / 64) + wave_id();
+    if (i >= count) return;
+    const uint32_t u = lu[i];
+    uint32_t *lab = lane_labels<KL>(arena, n, lg[i], nullptr);
+    const uint32_t ra = r0[u], rb = r0[u + 1], W = wid[u];
+    const cu32 *wr = as_const(wts) + woff[u], *out = as_const(tgt) + (size_t)u * 64;
+    if (W == 16) push_unit<KL, 16>(ra, rb, key, wr, out, lab);
+    else if (W == 32) push_unit<KL, 32>(ra, rb, key, wr, out, lab);
+    else push_unit<KL, 64>(ra, rb, key, wr, out, lab);
+}
+
 // bottom_gate: per (bottom tree, lane group), whether the tree can hold a
 // label within a lane's limit: some bottom chunk holding part of the tree
 // was left open by the chunk gate (chunk_ok).  Thread (blockIdx.x * NT +
@@ -1443,6 +1501,9 @@ __global__ __launch_bounds__(NT) void table_gather_kernel(
 // ---------------------------------------------------------------------------
 // host context
 // ---------------------------------------------------------------------------
+// Device allocations the library has made (every one goes through DevBuf).
+std::atomic<uint64_t> g_device_allocs{0};
+
 // A device buffer: owned (hipMalloc'd, grown on demand) or borrowed (a fixed
 // range of memory the library holds elsewhere, which is never reallocated: a
 // request beyond its size fails with hipErrorOutOfMemory).
@@ -1467,7 +1528,10 @@ struct DevBuf {
         release();
         if (count == 0) count = 1;
         hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
-        if (e == hipSuccess) n = count;
+        if (e == hipSuccess) {
+            n = count;
+            g_device_allocs.fetch_add(1, std::memory_order_relaxed);
+        }
         else p = nullptr;
         return e;
     }
@@ -1594,6 +1658,11 @@ struct Work {
     uint32_t cap_hint = 0;  // lanes per batch chosen by the caller (0: run_all chooses)
     DevBuf<unsigned long long> reach, visits;
     DevBuf<uint32_t> pairs;
+    // upward-only batches: the reached (segment, wave group) pairs by segment
+    // level (tri/fill) and the reached (push block, wave group) pairs
+    // (push), host lists and their device copies; item ranges per level
+    std::vector<uint32_t> h_tu, h_tg, h_pu, h_pg, h_toff, h_poff, h_mark;
+    DevBuf<uint32_t> tu, tg, pu, pg;
     std::vector<uint32_t> h_pairs;
     // host staging of a call (kept, so a fixed Work reuses its capacity)
     std::vector<uint32_t> h_bt, h_pending, h_order, h_soff, h_srank, h_scost, h_lim, h_rows, h_sdepth, h_spre,
@@ -1674,10 +1743,15 @@ struct bfg_ctx {
     std::mutex t2_mu;                // two-channel calls share the label buffers below
     std::unique_ptr<TwoChannel> t2;  // set by bfg_ctx_attach_length
     // the many-to-many join: the context over the transposed graph (same
-    // positions, set by bfg_ctx_pair_reverse; not owned), the context this one
-    // is the reverse of, and an all-zero split table for upward-only batches
+    // positions, set by bfg_ctx_pair_reverse; not owned) and the context this
+    // one is the reverse of
     bfg_ctx *rev = nullptr, *rev_of = nullptr;
-    DevBuf<uint8_t> zero_split;
+    // host tables of the upward lists: the segment of each top position, the
+    // segment holding each segment head's parent (NONE at roots), each
+    // segment's level (built by bfg_ctx_pair_reverse for both contexts)
+    std::vector<uint32_t> seg_of_pos, seg_par, seg_lvl;
+    std::vector<uint32_t> top_anc;  // per bottom position p (index p - n_top): its first top ancestor, or NONE
+    std::vector<uint32_t> bt_root;  // per bottom position p (index p - n_top): the root of its bottom tree
 
     bfg_ctx() = default;
     bfg_ctx(const bfg_ctx &) = delete;
@@ -1972,6 +2046,156 @@ CGate cgate_of(const bfg_ctx *c, const Work *w) {
     return CGate{w->gate ? c->cs_min.p : nullptr, w->lane_limit.p, w->chunk_ok.p, (uint32_t)c->L.ch_off.size() - 1};
 }
 
+// The host tables of the upward lists (see bfg_ctx): built once per context.
+void upward_list_tables(bfg_ctx *c) {
+    if (!c->seg_of_pos.empty()) return;
+    const bfg::Layout &L = c->L;
+    const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1, nsl = (uint32_t)L.seglvl_seg.size() - 1;
+    c->seg_of_pos.assign(L.n_top, NONE);
+    c->seg_par.assign(nseg, NONE);
+    c->seg_lvl.assign(nseg, 0);
+    for (uint32_t s = 0; s < nseg; ++s)
+        for (uint32_t p = L.seg_pos[s]; p < L.seg_pos[s + 1]; ++p) c->seg_of_pos[p] = s;
+    for (uint32_t l = 0; l < nsl; ++l)
+        for (uint32_t s = L.seglvl_seg[l]; s < L.seglvl_seg[l + 1]; ++s) c->seg_lvl[s] = l;
+    for (uint32_t s = 0; s < nseg; ++s) {
+        const uint32_t q = L.pos_parent[L.seg_pos[s]];
+        c->seg_par[s] = q != NONE && q < L.n_top ? c->seg_of_pos[q] : NONE;
+    }
+    // bottom positions come by descending height, so a parent before its children
+    c->top_anc.assign(L.n - L.n_top, NONE);
+    c->bt_root.assign(L.n - L.n_top, NONE);
+    for (uint32_t p = L.n_top; p < L.n; ++p) {
+        const uint32_t q = L.pos_parent[p];
+        const bool root = q == NONE || q < L.n_top;
+        c->top_anc[p - L.n_top] = root ? q : c->top_anc[q - L.n_top];
+        c->bt_root[p - L.n_top] = root ? p : c->bt_root[q - L.n_top];
+    }
+}
+
+// The reached units of an upward-only batch of nl lanes (seeds staged in
+// h_soff / h_spos): each lane's root path in segments, from the segment of
+// its first top ancestor up, marked once per wave group (a path already
+// marked for the group is marked up to the root).  The pairs are sorted by
+// segment (levels ascend with segment ids), then wave group, and each
+// reached segment's push blocks listed block by block with its wave groups
+// side by side, so the four wavefronts of a workgroup mostly share a unit
+// (and its weights in the scalar cache).  Uploaded on the Work's stream.
+int build_up_lists(bfg_ctx *c, Work *w, uint32_t nl) {
+    const bfg::Layout &L = c->L;
+    const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1, nsl = (uint32_t)L.seglvl_seg.size() - 1;
+    if (c->seg_of_pos.size() != L.n_top) return fail(BFG_INTERNAL_ERROR, "upward lists without their tables");
+    std::vector<uint32_t> &mark = w->h_mark, &toff = w->h_toff, &poff = w->h_poff;
+    mark.assign(nseg, 0);
+    toff.assign(nsl + 1, 0);
+    std::vector<uint32_t> us, gs, soff(nseg + 1, 0);  // in discovery order; per segment counts
+    for (uint32_t i = 0; i < nl; ++i) {
+        const uint32_t g = i / 64;
+        for (uint32_t j = w->h_soff[i]; j < w->h_soff[i + 1]; ++j) {
+            uint32_t p = w->h_spos[j];
+            if (p >= L.n_top) p = c->top_anc[p - L.n_top];
+            if (p == NONE) continue;  // a bottom tree without a top ancestor
+            for (uint32_t s = c->seg_of_pos[p]; s != NONE && mark[s] != g + 1; s = c->seg_par[s]) {
+                mark[s] = g + 1;
+                us.push_back(s);
+                gs.push_back(g);
+                soff[s + 1]++;
+            }
+        }
+    }
+    // counting sort by segment (a stable one: wave groups ascend within a segment)
+    for (uint32_t s = 0; s < nseg; ++s) soff[s + 1] += soff[s];
+    std::vector<uint32_t> &tu = w->h_tu, &tg = w->h_tg, &pu = w->h_pu, &pg = w->h_pg;
+    tu.resize(us.size());
+    tg.resize(gs.size());
+    {
+        std::vector<uint32_t> at(soff.begin(), soff.end() - 1);
+        for (size_t k = 0; k < us.size(); ++k) {
+            const uint32_t d = at[us[k]]++;
+            tu[d] = us[k];
+            tg[d] = gs[k];
+        }
+    }
+    for (uint32_t l = 0; l <= nsl; ++l) toff[l] = soff[L.seglvl_seg[l]];
+    poff.assign(nsl + 1, 0);
+    pu.clear();
+    pg.clear();
+    for (uint32_t l = 0; l < nsl; ++l) {
+        poff[l] = (uint32_t)pu.size();
+        for (uint32_t s = L.seglvl_seg[l]; s < L.seglvl_seg[l + 1]; ++s)
+            for (uint32_t b = L.pt_blk[s]; b < L.pt_blk[s + 1]; ++b)
+                for (uint32_t k = soff[s]; k < soff[s + 1]; ++k) {
+                    pu.push_back(b);
+                    pg.push_back(tg[k]);
+                }
+    }
+    poff[nsl] = (uint32_t)pu.size();
+    ENSURE(w, tu, std::max<size_t>(tu.size(), 1), "upward lists");
+    ENSURE(w, tg, std::max<size_t>(tg.size(), 1), "upward lists");
+    ENSURE(w, pu, std::max<size_t>(pu.size(), 1), "upward lists");
+    ENSURE(w, pg, std::max<size_t>(pg.size(), 1), "upward lists");
+    // the host vectors live in the Work until the batch has synchronised
+    if (!tu.empty()) {
+        HIP_TRY(hipMemcpyAsync(w->tu.p, tu.data(), tu.size() * 4, hipMemcpyHostToDevice, w->stream));
+        HIP_TRY(hipMemcpyAsync(w->tg.p, tg.data(), tg.size() * 4, hipMemcpyHostToDevice, w->stream));
+    }
+    if (!pu.empty()) {
+        HIP_TRY(hipMemcpyAsync(w->pu.p, pu.data(), pu.size() * 4, hipMemcpyHostToDevice, w->stream));
+        HIP_TRY(hipMemcpyAsync(w->pg.p, pg.data(), pg.size() * 4, hipMemcpyHostToDevice, w->stream));
+    }
+    return BFG_SUCCESS;
+}
+
+// The upward phase of an upward-only batch over the lists of its reached
+// units: INF fill, the bottom walk, then per segment level (deepest first)
+// the intra-segment UP blocks and the UP pushes of the reached pairs only.
+template <int KL>
+int launch_up_lists(bfg_ctx *c, Work *w, uint32_t nl, PhaseTimes *pt) {
+    const bfg::Layout &L = c->L;
+    hipStream_t s = w->stream;
+    const uint32_t n = label_stride(c, w), D1 = d1_of(c);
+    const uint32_t nsl = (uint32_t)L.seglvl_seg.size() - 1;
+    int rc = build_up_lists(c, w, nl);
+    if (rc) return rc;
+    constexpr uint32_t WPB = TILE_THREADS / 64;
+    uint32_t nlaunch = 0;
+    HIP_TRY(hipEventRecord(w->ev[0], s));
+    const uint32_t total = (uint32_t)w->h_tu.size();
+    if (total) {
+        hipLaunchKernelGGL(fill_list_kernel<KL>, dim3((total + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, total,
+                           w->tu.p, w->tg.p, c->seg_pos.p, w->arena.p, n);
+        ++nlaunch;
+    }
+    hipLaunchKernelGGL(bottom_walk_kernel<KL>, dim3(nl), dim3(64), 0, s, w->seed_off.p, w->seed_rank.p,
+                       w->seed_pos.p, w->seed_cost.p, w->seed_depth.p, w->lane_limit.p, c->parent.p,
+                       c->pos_parent.p, c->pos_depth.p, c->up_off.p, c->up_tgt.p, c->up_w.p, L.n_top, w->pathbuf.p,
+                       D1, w->arena.p, n);
+    ++nlaunch;
+    for (int l = (int)nsl - 1; l >= 0; --l) {
+        const uint32_t t0 = w->h_toff[l], tc = w->h_toff[l + 1] - t0;
+        if (tc) {
+            hipLaunchKernelGGL(tri_list_kernel<KL>, dim3((tc + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, tc,
+                               w->tu.p + t0, w->tg.p + t0, c->seg_pos.p, c->tb_up.p, c->tb_w.p, w->arena.p, n);
+            ++nlaunch;
+        }
+        const uint32_t p0 = w->h_poff[l], pc = w->h_poff[l + 1] - p0;
+        if (pc) {
+            hipLaunchKernelGGL(push_list_kernel<KL>, dim3((pc + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, pc,
+                               w->pu.p + p0, w->pg.p + p0, c->pr_off.p, c->pr_wid.p, c->pw_off.p, c->pt_tgt.p,
+                               c->pr_x.p, c->pr_w.p, w->arena.p, n);
+            ++nlaunch;
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(w->ev[1], s));
+    HIP_TRY(hipEventSynchronize(w->ev[1]));
+    float a = 0;
+    HIP_TRY(hipEventElapsedTime(&a, w->ev[0], w->ev[1]));
+    pt->up_ms += a;
+    pt->launches += nlaunch;
+    return BFG_SUCCESS;
+}
+
 template <int KL>
 int launch_phases(bfg_ctx *c, Work *w, const Plan &P, uint32_t nl, PhaseTimes *pt) {
     const bfg::Layout &L = c->L;
@@ -1990,8 +2214,7 @@ int launch_phases(bfg_ctx *c, Work *w, const Plan &P, uint32_t nl, PhaseTimes *p
     constexpr uint32_t WPB = TILE_THREADS / 64;  // wave groups per workgroup
     const uint32_t wgx = (wgs + WPB - 1) / WPB;  // workgroups across the wave groups
     const uint32_t tss = c->tri_segs ? c->tri_segs - 1 : tri_scalar_segs();
-    // an upward-only batch never merges DOWN units with atomics
-    const uint8_t *up_split = w->up_only ? c->zero_split.p : P.split.p;
+    if (w->up_only) return launch_up_lists<KL>(c, w, nl, pt);
     if (nseg) {
         hipLaunchKernelGGL(fill_reach_kernel<KL>, dim3(wgs, (nseg + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, nseg,
                            c->seg_pos.p, R, w->arena.p, n);
@@ -2008,7 +2231,7 @@ int launch_phases(bfg_ctx *c, Work *w, const Plan &P, uint32_t nl, PhaseTimes *p
         for (uint32_t b = s0; b < s1; b += MAX_GRID_Y, ++nlaunch)
             hipLaunchKernelGGL((tri_kernel<KL, false, false, false>), dim3(wgx, std::min(s1 - b, MAX_GRID_Y)),
                                dim3(TILE_THREADS), 0, s, b, std::min(s1 - b, MAX_GRID_Y), c->seg_pos.p, c->tb_up.p,
-                               c->tb_w.p, up_split, R, G0, w->arena.p, n, wgs);
+                               c->tb_w.p, P.split.p, R, G0, w->arena.p, n, wgs);
         const uint32_t k0 = L.pt_blk[s0], k1 = L.pt_blk[s1];
         for (uint32_t b = k0; b < k1; b += MAX_GRID_Y, ++nlaunch)
             hipLaunchKernelGGL((rows_kernel<KL, true, false>), dim3(wgx, std::min(k1 - b, MAX_GRID_Y)), dim3(TILE_THREADS),
@@ -2018,7 +2241,7 @@ int launch_phases(bfg_ctx *c, Work *w, const Plan &P, uint32_t nl, PhaseTimes *p
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(w->ev[1], s));
-    for (uint32_t l = 0; l < nsl && !w->up_only; ++l) {
+    for (uint32_t l = 0; l < nsl; ++l) {
         const uint32_t u0 = P.lvl[l], u1 = P.lvl[l + 1];
         if (u1 > u0) {
             const bool timed = w->time_dom;
@@ -2075,9 +2298,7 @@ int launch_phases(bfg_ctx *c, Work *w, const Plan &P, uint32_t nl, PhaseTimes *p
                                    c->seg_pos.p, c->tb_dn.p, c->tb_w.p, P.split.p, R, G, w->arena.p, n, wgs);
         }
     }
-    if (w->up_only) {
-        // the upward labels are the result
-    } else if (w->paths_on) {
+    if (w->paths_on) {
         constexpr int NSP = NT / KL;
         if (w->n_paths) {
             const int rc = stage_path_elements(c, w);
@@ -3462,13 +3683,7 @@ int matrix_multi(bfg_ctx *c, Work *w, uint32_t nr, const uint32_t *src_off, cons
 
 // The root position of the bottom tree (maximal bottom-region subtree) that
 // holds bottom position p.
-uint32_t bottom_root(const bfg::Layout &L, uint32_t p) {
-    for (;;) {
-        const uint32_t q = L.pos_parent[p];
-        if (q == NONE || q < L.n_top) return p;
-        p = q;
-    }
-}
+uint32_t bottom_root(const bfg_ctx *c, uint32_t p) { return c->bt_root[p - c->L.n_top]; }
 
 // Runs one Work's batches through the upward phase only; always switched off
 // again.
@@ -3543,7 +3758,8 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
     // left.  When both sides fit one batch, their upward phases run at once
     // (the target side on a thread of its own) and the join splits its tiles.
     const uint32_t cap_t = r->cap_fixed ? r->lane_cap : auto_lane_cap(r, wb, T, 0);
-    bool together = cap_t >= T;
+    const char *serial = std::getenv("BFG_BUCKET_SERIAL");  // profiling override: one side after the other
+    bool together = cap_t >= T && !(serial && serial[0] == '1');
     if (together && !r->cap_fixed) {
         rc = ensure_arena(r, wb, (size_t)cap_t * label_stride(r, wb));
         if (rc) return rc;
@@ -3564,7 +3780,7 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
         tree_tgts.clear();
         for (uint32_t b = 0; b < tnl; ++b) {
             const uint32_t p = L.pos_of_rank[tgt[trows[b]]];
-            if (p >= L.n_top) tree_tgts[bottom_root(L, p)].push_back(b);
+            if (p >= L.n_top) tree_tgts[bottom_root(c, p)].push_back(b);
         }
         return BFG_SUCCESS;
     };
@@ -3591,7 +3807,7 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
             for (uint32_t a = 0; a < snl; ++a) {
                 const uint32_t p = L.pos_of_rank[src[srows[a]]];
                 if (p < L.n_top) continue;
-                const auto it = tree_tgts.find(bottom_root(L, p));
+                const auto it = tree_tgts.find(bottom_root(c, p));
                 if (it == tree_tgts.end()) continue;
                 for (uint32_t b : it->second) {
                     pr.push_back(a);
@@ -3712,6 +3928,12 @@ int seeded_req(bfg_ctx *c, Req &req, const uint32_t *seed_offsets, const uint32_
 extern "C" {
 
 const char *bfg_last_error(void) { return g_last_error.c_str(); }
+
+int bfg_device_allocations(uint64_t *count) {
+    if (!count) return fail(BFG_INVALID_PARAMETER, "null argument");
+    *count = g_device_allocs.load(std::memory_order_relaxed);
+    return BFG_SUCCESS;
+}
 
 int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     if (!gd || !out) return fail(BFG_INVALID_PARAMETER, "null argument");
@@ -4068,13 +4290,7 @@ int bfg_ctx_pair_reverse(bfg_ctx *c, bfg_ctx *rev) {
         if (A.in_off[p + 1] - A.in_off[p] != B.up_off[p + 1] - B.up_off[p] ||
             B.in_off[p + 1] - B.in_off[p] != A.up_off[p + 1] - A.up_off[p])
             return fail(BFG_INVALID_PARAMETER, "the reverse context is not over the transposed graph");
-    HIP_TRY(hipSetDevice(c->device));
-    const size_t nseg = std::max<size_t>(A.seg_pos.size() - 1, 1);
-    for (bfg_ctx *x : {c, rev})
-        if (x->zero_split.n < nseg) {
-            HIP_TRY(x->zero_split.alloc(nseg));
-            HIP_TRY(hipMemset(x->zero_split.p, 0, nseg));
-        }
+    for (bfg_ctx *x : {c, rev}) upward_list_tables(x);
     if (c->rev) c->rev->rev_of = nullptr;
     c->rev = rev;
     rev->rev_of = c;

@@ -265,6 +265,11 @@ int bfg_table_seeded(bfg_ctx *ctx, const uint32_t *src_offsets, const uint32_t *
  * positions or arc counts) or is on another device. */
 int bfg_ctx_pair_reverse(bfg_ctx *ctx, bfg_ctx *rev);
 
+/* Diagnostics (no counterpart in the reference): the number of device
+ * allocations (hipMalloc) the library has made in this process, so a caller
+ * can check that a call allocated nothing. */
+int bfg_device_allocations(uint64_t *count);
+
 /* bfg_ctx_create over the transposed graph of `graph` (the reverse metric's
  * CCH, whose forward PHAST is the reference's reverse PHAST,
  * range/phast.rs:879-1108, and whose upward phase is the bucket engine's

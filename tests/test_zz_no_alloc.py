@@ -1,8 +1,9 @@
 """bfg_matrix_device_noalloc: the allocation-free matrix call against
 bfg_matrix_device on the same inputs, through the C ABI, bit-exact; two calls
 in a row, the second smaller; bounded and unbounded; requests beyond its
-limits are refused with BFG_NOT_SUPPORTED; the device's free memory does not
-change across the calls."""
+limits are refused with BFG_NOT_SUPPORTED; the library's own count of device
+allocations does not change across the calls (the device's free memory may
+still move with the HIP runtime's internal pools, so it is only bounded)."""
 import ctypes as C
 
 import numpy as np
@@ -28,10 +29,14 @@ def run_both(e, hip, src, tgt, thr):
     _check(e._lib.bfg_matrix_device(e._h, d_src.ptr, S, d_tgt.ptr, T, thr, d_a.ptr))
     hip.sync()
     free0, _ = hip.mem_info()
+    n0, n1 = np.zeros(1, np.uint64), np.zeros(1, np.uint64)
+    _check(e._lib.bfg_device_allocations(n0.ctypes.data_as(C.POINTER(C.c_uint64))))
     rc = e._lib.bfg_matrix_device_noalloc(e._h, d_src.ptr, S, d_tgt.ptr, T, thr, d_b.ptr)
     hip.sync()
+    _check(e._lib.bfg_device_allocations(n1.ctypes.data_as(C.POINTER(C.c_uint64))))
     free1, _ = hip.mem_info()
     _check(rc)
+    assert n1[0] == n0[0], "the allocation-free call made a device allocation"
     return d_a.to_host(), d_b.to_host(), free0, free1
 
 
@@ -54,7 +59,8 @@ def test_noalloc_matrix_equals_device_matrix(rows, chords):
             if thr != INF:
                 exp = np.where(exp > thr, INF, exp)
             assert np.array_equal(got, exp)
-            assert free1 == free0, "the allocation-free call changed the device's free memory"
+            # the runtime's own pools (kernel arguments, staging) may grow by a chunk
+            assert free1 >= free0 - (16 << 20), "the allocation-free call took device memory"
 
 
 def test_noalloc_refuses_requests_beyond_its_limits():
