@@ -253,6 +253,9 @@ def serve_stream(loop_cls, eng, reqs, arrivals):
 def serving_bench(eng, g, args, rows, chords):
     from bfgpu.serving import PlainLoop, ServingLoop
     n = args.serving_requests
+    t0 = time.perf_counter()
+    eng.pair_reverse()  # the bucket side of large matrices, both loops
+    log(f"transposed context paired in {time.perf_counter() - t0:.1f}s")
     warm = serving_requests(g, 8, args.seed + 7)
     t0 = time.perf_counter()
     for r in warm:

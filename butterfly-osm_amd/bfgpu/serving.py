@@ -10,8 +10,12 @@ admits the oldest pending requests while their lanes fit `max_lanes` and
 their estimated device memory fits what the rounds in flight leave of the
 budget (a fraction of the device's free memory when the loop starts), and
 runs them as one round:
-- every matrix request's sources become lanes of one sweep
-  (bfg_matrix_multi), each request gathering only its own rows and columns;
+- every matrix request with fewer than `bucket_min` sources or targets
+  becomes lanes of one shared PHAST sweep (bfg_matrix_multi), each request
+  gathering only its own rows and columns; larger ones run through the bucket
+  many-to-many (bfg_table_bucket: upward phases of both sides and a join),
+  whose cost grows with S + T and S * T rather than with a sweep per source
+  group, when the engine has it;
 - every isochrone becomes one lane of one block-gated sweep with its own
   limit (bfg_isochrone_lists), answered with its settled (rank, time) list.
 Requests that arrive while rounds are on the GPU are admitted as soon as a
@@ -20,7 +24,9 @@ else shares its sweep (labels are exact distances), so every result is
 bit-identical to the same request issued alone.
 
 PlainLoop serves the same stream the plain way, one GPU call per request in
-arrival order; the serving benchmark (bench.py --serving) compares the two.
+arrival order, each matrix routed by a MatrixRouter (forward PHAST, reverse
+PHAST or the bucket, by measured cost); the serving benchmark (bench.py
+--serving) compares the two.
 """
 import collections
 import dataclasses
@@ -31,6 +37,7 @@ from concurrent.futures import Future
 import numpy as np
 
 from .graph import INF
+from .router import MatrixRouter
 
 
 @dataclasses.dataclass
@@ -55,13 +62,22 @@ class IsochroneRequest:
         return 1
 
 
-def run_round(engine, reqs):
+BUCKET_MIN = 128  # sources and targets from which a round's matrix runs through the bucket
+
+
+def run_round(engine, reqs, bucket_min=BUCKET_MIN):
     """Results of one round of requests, in the order given: a row-major S x T
     u32 array per matrix request, an (m, 2) u32 array of (rank, time) per
-    isochrone."""
+    isochrone.  Matrices with at least bucket_min sources and targets run
+    through the engine's bucket many-to-many when it has one (0 = never)."""
     out = [None] * len(reqs)
     mats = [i for i, r in enumerate(reqs) if isinstance(r, MatrixRequest)]
     isos = [i for i, r in enumerate(reqs) if not isinstance(r, MatrixRequest)]
+    if bucket_min and hasattr(engine, "table_bucket_parallel_bounded"):
+        big = [i for i in mats if min(len(reqs[i].sources), len(reqs[i].targets)) >= bucket_min]
+        for i in big:
+            out[i] = engine.table_bucket_parallel_bounded(reqs[i].sources, reqs[i].targets, reqs[i].threshold)[0]
+        mats = [i for i in mats if out[i] is None]
     if mats:
         res = engine.compute_matrix_multi([(reqs[i].sources, reqs[i].targets, reqs[i].threshold) for i in mats])
         for i, m in zip(mats, res):
@@ -74,9 +90,12 @@ def run_round(engine, reqs):
     return out
 
 
-def run_plain(engine, req):
-    """One request as its own plain engine call (the per-request baseline)."""
+def run_plain(engine, req, router=None):
+    """One request as its own plain engine call (the per-request baseline); a
+    matrix goes through `router` (a MatrixRouter) when given."""
     if isinstance(req, MatrixRequest):
+        if router is not None:
+            return router.compute_matrix_block_gated(req.sources, req.targets, req.threshold)
         return engine.compute_matrix_block_gated(req.sources, req.targets, req.threshold)[0]
     return engine.isochrone_lists([[(int(req.origin), 0)]], np.array([req.limit], dtype=np.uint32))[0]
 
@@ -85,13 +104,16 @@ def run_plain(engine, req):
 _ISO_SETTLED = 0.25
 
 
-def request_bytes(req, n_nodes, n_top):
-    """Device bytes a request adds to a round: its label lanes (a matrix's
-    lanes hold the top region plus the bottom paths above its targets,
-    about 24 nodes per target; an isochrone's lane holds every node) and its
+def request_bytes(req, n_nodes, n_top, bucket_min=0):
+    """Device bytes a request adds to a round: its label lanes (a PHAST
+    matrix's lanes hold the top region plus the bottom paths above its
+    targets, about 24 nodes per target; a bucket matrix's source and target
+    lanes hold the top region; an isochrone's lane holds every node) and its
     outputs."""
     if isinstance(req, MatrixRequest):
         S, T = len(req.sources), len(req.targets)
+        if bucket_min and min(S, T) >= bucket_min:
+            return 4 * (S + T) * n_top + 4 * S * T
         stride = min(n_nodes, n_top + 24 * T)
         return 4 * S * stride + 4 * S * T
     words = (n_nodes + 63) // 64
@@ -190,8 +212,10 @@ class ServingLoop(_Loop):
     The budget is mem_fraction of the device's free memory when the loop
     starts, or mem_budget bytes when given."""
 
-    def __init__(self, engine, slots=3, max_lanes=8192, mem_fraction=0.5, mem_budget=None):
+    def __init__(self, engine, slots=3, max_lanes=8192, mem_fraction=0.5, mem_budget=None, bucket_min=BUCKET_MIN):
         self.max_lanes = max_lanes
+        self.bucket_min = bucket_min
+        self._bucket_min = bucket_min if hasattr(engine, "table_bucket_parallel_bounded") else 0
         if hasattr(engine, "set_concurrency"):
             engine.set_concurrency(slots)
         info = engine.info() if hasattr(engine, "info") else {}
@@ -208,7 +232,7 @@ class ServingLoop(_Loop):
     def _take(self):
         batch, lanes, nbytes = [], 0, 0
         for req, fut in self._pending:
-            l, b = req.lanes(), request_bytes(req, self._n, self._n_top)
+            l, b = req.lanes(), request_bytes(req, self._n, self._n_top, self._bucket_min)
             fits = lanes + l <= self.max_lanes and self._inflight_bytes + nbytes + b <= self.mem_budget
             if not fits and (batch or self._inflight):
                 break
@@ -229,15 +253,20 @@ class ServingLoop(_Loop):
         self._inflight_bytes -= reservation
         self._inflight -= 1
 
+    def _serve(self, reqs):
+        return run_round(self.engine, reqs, self.bucket_min)
+
 
 class PlainLoop(_Loop):
-    """One GPU call per request, in arrival order (one slot)."""
+    """One GPU call per request, in arrival order (one slot); matrices routed
+    by a MatrixRouter when the engine has the bucket side (route=True)."""
 
-    def __init__(self, engine):
+    def __init__(self, engine, route=True):
+        self.router = MatrixRouter(engine) if route and hasattr(engine, "table_bucket_parallel_bounded") else None
         super().__init__(engine, 1)
 
     def _take(self):
         return [self._pending.popleft()], None
 
     def _serve(self, reqs):
-        return [run_plain(self.engine, r) for r in reqs]
+        return [run_plain(self.engine, r, self.router) for r in reqs]

@@ -269,3 +269,36 @@ def test_memory_budget_holds_a_request_until_a_round_frees(grid_small):
         fb.result(timeout=30)
     (a0, a1), (b0, b1) = sorted(eng.spans)
     assert b0 >= a1  # b waited for a's memory
+
+
+class BucketOracleEngine(OracleEngine):
+    """OracleEngine with the bucket side (the oracle's bucket many-to-many)."""
+
+    def __init__(self, g):
+        super().__init__(g)
+        self.bucket_calls = 0
+
+    def table_bucket_parallel_bounded(self, sources, targets, threshold=INF):
+        self.bucket_calls += 1
+        m = self.o.table_bucket(sources, targets, threshold)
+        return np.where(m > threshold, INF, m).astype(np.uint32), None
+
+
+def test_round_routes_large_matrices_to_the_bucket(grid_small):
+    """A round sends matrices with at least bucket_min sources and targets to
+    the bucket and the rest to the shared PHAST sweep; results equal the
+    requests alone either way; PlainLoop routes through a MatrixRouter."""
+    g = grid_small
+    rng = np.random.default_rng(17)
+    eng = BucketOracleEngine(g)
+    big = MatrixRequest(rng.integers(0, g.n, 12).astype(np.uint32), rng.integers(0, g.n, 15).astype(np.uint32), 1500)
+    small = MatrixRequest(rng.integers(0, g.n, 3).astype(np.uint32), rng.integers(0, g.n, 40).astype(np.uint32))
+    iso = IsochroneRequest(int(rng.integers(0, g.n)), 600)
+    out = run_round(eng, [big, small, iso], bucket_min=10)
+    assert eng.bucket_calls == 1
+    for r, got in zip([big, small, iso], out):
+        assert np.array_equal(got, run_plain(OracleEngine(g), r))
+    with PlainLoop(eng) as loop:
+        assert loop.router is not None
+        got = loop.submit(big).result(timeout=60)
+    assert np.array_equal(got, out[0])
