@@ -170,7 +170,7 @@ def cpu_baseline_mt(g, src, tgt, budget_s):
             "sample": f"{k} sources x {len(tgt)} targets, one K=8 batch per thread"}
 
 
-def bucket_bench(eng, d_src, d_tgt, S, T, d_phast, args, phast_ms):
+def bucket_bench(eng, d_src, d_tgt, S, T, d_phast, args, phast_ms, src=None, tgt=None):
     """The router's bucket side on the same device-resident matrix
     (bfg_table_bucket_device: upward phases of the sources on the CCH and of
     the targets on the transposed CCH, joined at their common ancestors),
@@ -201,11 +201,11 @@ def bucket_bench(eng, d_src, d_tgt, S, T, d_phast, args, phast_ms):
         log(f"bucket step {i}: up {st.upward_time_ms:.1f} ms join {st.downward_time_ms:.1f} ms")
     sync()
     ms = 1000.0 * (time.perf_counter() - t0) / args.steps
-    same = bool(np.array_equal(d_b.to_host(), d_phast.to_host()))
+    same = bool(np.array_equal(d_b.to_host(), d_phast.to_host())) if d_phast is not None else None
     log(f"bucket {ms:.1f} ms per matrix, every cell equal to the PHAST matrix: {same}")
     d_b.free()
     r = MatrixRouter(eng)  # the router's choice with both sides measured on this shape
-    update_cost_ewma(r.fwd_ns, int(phast_ms * 1e6) // S)
+    update_cost_ewma(r.fwd_ns, max(int(phast_ms * 1e6) // S, 1))
     share = join / (up + join) if up + join > 0 else 0.5
     update_cost_ewma(r.up_ns, int(ms * 1e6 * (1 - share)) // (S + T))
     update_cost_ewma(r.cell_ns, max(int(ms * 1e6 * share) // (S * T), 1))
@@ -223,7 +223,21 @@ def bucket_bench(eng, d_src, d_tgt, S, T, d_phast, args, phast_ms):
             "frac": achieved / VALU_PEAK_GINSTR, "traffic": None, "launches": 1, "avg_launch_ms": kms,
             "visits_per_launch": vis, "cell_updates_per_launch": vis * 4096,
             "algorithmic_bytes_per_launch": vis * 512 + 4 * S * T, "hbm_gbps": hbm, "hbm_frac": hbm / HBM_PEAK_GBPS}
-    line = {"ms_per_step": ms, "cells_per_s": S * T / (ms / 1000.0), "steps": args.steps,
+    host = None
+    if args.host_steps > 0 and src is not None:
+        # the same matrix delivered to pinned host memory (PCIe included)
+        from bfgpu.hip import PinnedArray
+        pinned = PinnedArray(S * T)
+        eng.table_bucket_parallel_bounded(src, tgt, INF_U32, out=pinned.a)  # untimed
+        t1 = time.perf_counter()
+        for _ in range(args.host_steps):
+            eng.table_bucket_parallel_bounded(src, tgt, INF_U32, out=pinned.a)
+        hms = 1000.0 * (time.perf_counter() - t1) / args.host_steps
+        host = {"ms_per_matrix": hms, "cells_per_s": S * T / (hms / 1000.0), "steps": args.host_steps,
+                "entry_point": "bfg_table_bucket", "memory": "pinned host (hipHostMalloc)"}
+        log(f"bucket host-delivered {hms:.1f} ms per matrix")
+        pinned.free()
+    line = {"ms_per_step": ms, "cells_per_s": S * T / (ms / 1000.0), "steps": args.steps, "host_delivered": host,
             "phases_ms_per_step": {"up_both_sides_overlapped": up / args.steps, "join_with_reach_and_pairs":
                                    join / args.steps, "join_kernel": kms},
             "entry_point": "bfg_table_bucket_device", "every_cell_equal_to_phast": same,
@@ -450,6 +464,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bucket", action="store_true",
                     help="skip the bucket many-to-many (the router's other side) on the same matrix")
+    ap.add_argument("--bucket-only", action="store_true",
+                    help="instead of the matrix line: the workload's matrix through the bucket many-to-many alone "
+                         "(no PHAST steps, no CPU baseline; for profiling); prints its own JSON line")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the autotuner (by default it chooses the label width, K, the DOWN unit size and "
                          "the source tile on this workload before the timed steps)")
@@ -522,6 +539,14 @@ def main():
     if args.table_2ch:
         table2ch_bench(eng, g, args, rows, chords, src, tgt)
         return
+    if args.bucket_only:
+        d_src, d_tgt = DeviceArray.from_host(src), DeviceArray.from_host(tgt)
+        line, _ = bucket_bench(eng, d_src, d_tgt, S, T, None, args, 0.0, src, tgt)
+        line.update({"metric": "bucket matrix cells/s (profiling run)", "value": line["cells_per_s"],
+                     "unit": "cells/s", "n_gpus": 1, "dtype": "u32", "data": "synthetic",
+                     "config": {"workload": f"{S}x{T} matrix, {rows}x{rows} jittered grid, n={g.n}"}})
+        print(json.dumps(line), flush=True)
+        return
 
     d_src = DeviceArray.from_host(src)
     d_tgt = DeviceArray.from_host(tgt)
@@ -586,7 +611,7 @@ def main():
     gpu_rows = d_out.to_host(0, n_check * T).reshape(n_check, T)
     bucket = routed = None
     if not args.no_bucket:
-        bucket, routed = bucket_bench(eng, d_src, d_tgt, S, T, d_out, args, ms_per_step)
+        bucket, routed = bucket_bench(eng, d_src, d_tgt, S, T, d_out, args, ms_per_step, src, tgt)
     host = None
     if args.host_steps > 0:
         # the same matrix delivered to host memory (PCIe included), the way a
