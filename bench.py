@@ -22,7 +22,8 @@ kernel against the VALU issue bound, with PHAST's DOWN rows kernel beside it
 Other modes print their own JSON line instead: --serving (continuous batching
 against plain per-request calls), --isochrones L (block-gated isochrones with
 uneven limits), --table-2ch S (a two-channel seeded table: times and lengths
-along them).  The matrix line runs the autotuner first (label width, K,
+along them), --customize (the graph's triangle relaxation on the GPU from its
+original arc weights, every weight checked against the CPU builder's).  The matrix line runs the autotuner first (label width, K,
 the DOWN unit size, the source tile; --no-autotune skips it).
 """
 import argparse
@@ -86,11 +87,12 @@ def counter_means(path, counter, match):
     return len(vals), (sum(vals) / len(vals) if vals else 0.0)
 
 
-def traffic_summary(fetch_csv, write_csv, cfetch_csv, cwrite_csv, lanes, workload, copy_bytes=1 << 30):
-    """FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the DOWN-rows kernel,
-    each scaled by what the counter reads per KiB for a plain copy of
-    copy_bytes (read copy_bytes, write copy_bytes)."""
-    kern = f"rows16_kernel<{lanes}, false>"
+def traffic_summary(fetch_csv, write_csv, cfetch_csv, cwrite_csv, lanes, workload, copy_bytes=1 << 30,
+                    kernel=""):
+    """FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the DOWN-rows kernel
+    (or of `kernel`), each scaled by what the counter reads per KiB for a
+    plain copy of copy_bytes (read copy_bytes, write copy_bytes)."""
+    kern = kernel or f"rows16_kernel<{lanes}, false>"
     n, fetch = counter_means(fetch_csv, "FETCH_SIZE", kern)
     _, write = counter_means(write_csv, "WRITE_SIZE", kern)
     _, cf = counter_means(cfetch_csv, "FETCH_SIZE", "copyBuffer")
@@ -107,6 +109,18 @@ def committed_traffic(workload, lanes):
     summary for this workload (FETCH_SIZE + WRITE_SIZE, scaled by a copy
     kernel of known size), or None."""
     path = os.path.join(REPO, "profiles", f"rows_down_{workload}_k{lanes}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return {"bytes_per_launch": d["hbm_bytes_per_launch"], "source": os.path.relpath(path, REPO)}
+
+
+def committed_join_traffic(S, T):
+    """HBM bytes per launch of the bucket join kernel for an S x T matrix from
+    the committed counter summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy
+    kernel of known size), or None."""
+    path = os.path.join(REPO, "profiles", f"join_bucket_{S}x{T}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -218,11 +232,19 @@ def bucket_bench(eng, d_src, d_tgt, S, T, d_phast, args, phast_ms, src=None, tgt
     instr = vis * 4096 * 1.5 / 64
     achieved = instr / (kms / 1000.0) / 1e9 if kms > 0 else 0.0
     hbm = (vis * 512 + 4 * S * T) / (kms / 1000.0) / 1e9 if kms > 0 else 0.0
-    roof = {"bound": "valu", "kernel": "meet_join_kernel (the bucket join: 64 x 64 min-plus tiles)",
-            "achieved": achieved, "peak": VALU_PEAK_GINSTR, "unit": "G wave-instructions/s (algorithmic)",
-            "frac": achieved / VALU_PEAK_GINSTR, "traffic": None, "launches": 1, "avg_launch_ms": kms,
-            "visits_per_launch": vis, "cell_updates_per_launch": vis * 4096,
-            "algorithmic_bytes_per_launch": vis * 512 + 4 * S * T, "hbm_gbps": hbm, "hbm_frac": hbm / HBM_PEAK_GBPS}
+    # against the HBM roofline the metric names, with the bound that limits the
+    # kernel (VALU issue) beside it
+    traffic = committed_join_traffic(S, T)
+    roof = {"bound": "hbm", "kernel": "meet_join_kernel (the bucket join: 64 x 64 min-plus tiles)",
+            "achieved": hbm, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": hbm / HBM_PEAK_GBPS,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "traffic_source": traffic["source"] if traffic else None,
+            "launches": 1, "avg_launch_ms": kms, "visits_per_launch": vis, "cell_updates_per_launch": vis * 4096,
+            "algorithmic_bytes_per_launch": vis * 512 + 4 * S * T,
+            "limited_by": "valu issue",
+            "valu": {"achieved": achieved, "peak": VALU_PEAK_GINSTR,
+                     "unit": "G wave-instructions/s (algorithmic: 1.5 per 64 cell updates)",
+                     "frac": achieved / VALU_PEAK_GINSTR}}
     host = None
     if args.host_steps > 0 and src is not None:
         # the same matrix delivered to pinned host memory (PCIe included)
@@ -452,6 +474,85 @@ def table2ch_bench(eng, g, args, rows, chords, src, tgt):
     print(json.dumps(res), flush=True)
 
 
+def customize_bench(g, args, rows, chords):
+    """Triangle relaxation (customization) of the whole CCH on the GPU from the
+    original arc weights (shortcuts INF), bfg_triangle_relax: one call uploads
+    the topology, orders the apexes by dependency level, relaxes every lower
+    triangle once and downloads weights and middles.  Every weight is checked
+    against the CPU builder's customized graph.  The unit is triangles (a DOWN
+    arc x -> m and an UP arc m -> y, both finite) per second; the CPU baseline
+    is the oracle's restatement of triangle_relax_parallel on a 160 x 160
+    grid (about 20 s), single thread."""
+    from bfgpu import original_arc_weights, triangle_relax_parallel
+    t0 = time.perf_counter()
+    uw, dw, usc, dsc = original_arc_weights(g)
+    log(f"original weights in {time.perf_counter() - t0:.1f}s: {int(usc.sum())} UP and {int(dsc.sum())} "
+        f"DOWN shortcuts")
+    walls, kernels = [], []
+    r = None
+    for i in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        r = triangle_relax_parallel(g, uw, dw)
+        w = time.perf_counter() - t0
+        log(f"relax {i}: {1e3 * w:.0f} ms wall, kernels {r.stats.kernel_ms:.1f} ms, {r.stats}")
+        if i >= args.warmup:
+            walls.append(w)
+            kernels.append(r.stats.kernel_ms)
+    exact = bool(np.array_equal(r.up_weights, g.up_w) and np.array_equal(r.down_weights, g.dn_w))
+    log(f"every weight equal to the builder's: {exact}")
+    kms = float(np.median(kernels))
+    tri = r.stats.triangles
+    cpu = None
+    if not args.no_cpu:
+        from bfgpu import GridSpec, build_grid
+        from oracle import oracle as O
+        small = build_grid(GridSpec(rows=160, cols=160, seed=args.seed))
+        su, sd, _, _ = original_arc_weights(small)
+        rs = triangle_relax_parallel(small, su, sd)
+        t0 = time.perf_counter()
+        ou, od, om, odm, passes = O.triangle_relax(small, su, sd)
+        cw = time.perf_counter() - t0
+        same = bool(np.array_equal(ou, rs.up_weights) and np.array_equal(od, rs.down_weights)
+                    and np.array_equal(om, rs.up_middles) and np.array_equal(odm, rs.down_middles))
+        # the oracle evaluates every triangle once per pass
+        cpu = {"value": rs.stats.triangles * passes / cw, "unit": "triangles/s", "cores": 1, "kind": "port",
+               "sample": f"orc_triangle_relax (triangle_relax_parallel, customization.rs:1083-1200) on the "
+                         f"160x160 grid (n={small.n}, {rs.stats.triangles} triangles), {passes} passes in "
+                         f"{cw:.1f} s; weights and middles equal to the GPU's: {same}",
+               "bit_exact_vs_gpu": same}
+        log(f"cpu baseline: {cpu}")
+    # algorithmic bytes: per triangle the arc m -> y (target and word, 12 B) and
+    # the arc x -> y located and read (12 B); per lowering candidate an 8-byte
+    # atomic min
+    algo = 24 * tri + 8 * r.stats.improvements
+    gbps = algo / (kms / 1e3) / 1e9
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", f"relax_{args.workload}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["hbm_bytes_per_launch"] * r.stats.launches
+    roof = {"bound": "hbm", "kernel": "relax_level_kernel<false> (one launch per dependency level)",
+            "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": gbps / HBM_PEAK_GBPS,
+            "traffic": traffic, "traffic_note": "HBM bytes per call (all levels), from FETCH_SIZE + WRITE_SIZE"
+            if traffic else None, "algorithmic_bytes_per_call": algo, "launches": r.stats.launches,
+            "avg_launch_ms": kms / max(r.stats.launches, 1),
+            "limited_by": "dependent gathers (arc lookup, word check) per 64-triangle chunk"}
+    res = {
+        "metric": "customization triangles/s (triangle relaxation of the whole CCH, one call)",
+        "value": tri / (kms / 1e3), "unit": "triangles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": float(np.median(walls)) * 1e3, "kernel_ms": kms,
+        "higher_is_better": True, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"customization of the {rows}x{rows} jittered grid + {int(chords * 100)}% chords, "
+                               f"n={g.n}, m_up={g.m_up}, m_down={g.m_down}",
+                   "initial_weights": "original arcs their edge-based weight, shortcuts INF"},
+        "triangles": tri, "improvements": r.stats.improvements, "levels": r.stats.levels,
+        "launches": r.stats.launches, "wall_triangles_per_s": tri / float(np.median(walls)),
+        "bit_exact_vs_builder": exact, "roofline": roof, "cpu_baseline": cpu,
+        "builder_contract_customize_s": g.meta.get("t_contract_s"),
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -492,17 +593,21 @@ def main():
     ap.add_argument("--table-2ch", type=int, default=0, metavar="S",
                     help="instead of the matrix line: an S x S two-channel seeded table (times and lengths "
                          "along them) on the workload's graph; prints its own JSON line")
+    ap.add_argument("--customize", action="store_true",
+                    help="time the GPU triangle relaxation (customization) of the workload's graph")
     ap.add_argument("--traffic-summary", nargs=4, default=None,
                     metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="print the DOWN-rows kernel's HBM bytes per launch as JSON: its FETCH_SIZE and "
                          "WRITE_SIZE runs, scaled by the same counters of a --copy-probe run of 1 GiB")
+    ap.add_argument("--traffic-kernel", default="",
+                    help="the kernel --traffic-summary reports (default: the 16-bit DOWN rows kernel)")
     args = ap.parse_args()
     if args.summarize_counters:
         summarize_counters(args.summarize_counters)
         return
     if args.traffic_summary:
-        print(json.dumps(traffic_summary(*args.traffic_summary, lanes=args.lanes, workload=args.workload),
-                         indent=1))
+        print(json.dumps(traffic_summary(*args.traffic_summary, lanes=args.lanes, workload=args.workload,
+                                         kernel=args.traffic_kernel), indent=1))
         return
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -519,9 +624,13 @@ def main():
     spec = GridSpec(rows=rows, cols=rows, seed=args.seed, chord_frac=chords)
     t0 = time.perf_counter()
     log(f"building graph {rows}x{rows} chords={chords} ...")
-    g = cached_grid(spec, args.graph_cache, verbose=True, keep_ebg=False) if args.graph_cache else \
-        build_grid(spec, verbose=True, keep_ebg=False)
+    keep_ebg = args.customize  # the original arc weights come from the edge-based arcs
+    g = cached_grid(spec, args.graph_cache, verbose=True, keep_ebg=keep_ebg) if args.graph_cache else \
+        build_grid(spec, verbose=True, keep_ebg=keep_ebg)
     log(f"graph n={g.n} m_up={g.m_up} m_down={g.m_down} built in {time.perf_counter() - t0:.1f}s")
+    if args.customize:
+        customize_bench(g, args, rows, chords)
+        return
     src = make_queries(g, S, args.seed + 1)
     tgt = make_queries(g, T, args.seed + 2)
 

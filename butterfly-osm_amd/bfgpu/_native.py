@@ -38,6 +38,12 @@ class Timings(C.Structure):
                 ("join_ms", C.c_float), ("join_visits", C.c_uint64)]
 
 
+class RelaxStats(C.Structure):
+    _fields_ = [("triangles", C.c_uint64), ("improvements", C.c_uint64), ("levels", C.c_uint32),
+                ("launches", C.c_uint32), ("passes", C.c_uint32), ("kernel_ms", C.c_float),
+                ("total_ms", C.c_float)]
+
+
 class BfcParams(C.Structure):
     _fields_ = [("rows", C.c_int32), ("cols", C.c_int32), ("seed", C.c_uint64),
                 ("spacing_m", C.c_double), ("jitter_frac", C.c_double), ("delete_frac", C.c_double),
@@ -94,6 +100,10 @@ BFGPU_SYMBOLS = {
     "bfg_phast_seeded_2ch": [C.c_void_p, u32p, u32p, u32p, u32p, C.c_uint32, u32p, u32p, u32p],
     "bfg_table_seeded_2ch": [C.c_void_p, u32p, u32p, u32p, u32p, C.c_uint32, u32p, u32p, u32p, u32p, C.c_uint32,
                              C.c_uint32, u32p, u32p, C.POINTER(C.c_uint8)],
+    "bfg_triangle_relax": [C.POINTER(GraphDesc), C.c_int, u32p, u32p, u32p, u32p, u32p, u32p,
+                           C.POINTER(RelaxStats)],
+    "bfg_triangle_relax_lex": [C.POINTER(GraphDesc), C.c_int, u32p, u32p, u64p, u64p, u32p, u32p, u32p, u32p,
+                               u32p, u32p, C.POINTER(RelaxStats)],
 }
 
 _bfgpu = None

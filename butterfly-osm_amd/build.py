@@ -1,7 +1,8 @@
 """Builds the native libraries in-tree.
 
 libbfgpu.so  — HIP kernels + C ABI (hipcc, gfx950)        -> butterfly-osm_amd/lib/
-               (bfgpu.hip: PHAST engine and C ABI; meet.hip: the many-to-many join)
+               (bfgpu.hip: PHAST engine and C ABI; meet.hip: the many-to-many join;
+                customize.hip: triangle relaxation of the CCH)
 libbfcch.so  — synthetic road network -> CCH builder (g++)  -> butterfly-osm_amd/lib/
 """
 import os
@@ -34,7 +35,7 @@ def build_bfgpu():
     lay_o = os.path.join(BUILD, "layout.o")
     _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "layout.cpp"), "-o", lay_o])
     objs = []
-    for name in ("bfgpu", "meet"):
+    for name in ("bfgpu", "meet", "customize"):
         o = os.path.join(BUILD, name + ".o")
         _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, name + ".hip"), "-o", o])
         objs.append(o)

@@ -60,6 +60,7 @@
 #include <vector>
 
 #include "../../include/bfgpu.h"
+#include "errors.hpp"
 #include "layout.hpp"
 #include "meet.hpp"
 
@@ -3924,6 +3925,10 @@ int seeded_req(bfg_ctx *c, Req &req, const uint32_t *seed_offsets, const uint32_
 }
 
 }  // namespace
+
+namespace bfg {
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+}  // namespace bfg
 
 extern "C" {
 

@@ -337,6 +337,52 @@ int bfg_table_seeded_2ch(bfg_ctx *ctx, const uint32_t *src_offsets, const uint32
                          const uint32_t *tgt_part_lens, uint32_t n_tgt, uint32_t threshold, uint32_t *times,
                          uint32_t *lens, uint8_t *conflicts);
 
+/* ---- Customization: triangle relaxation of the CCH ----------------------
+ * The topology is the whole CchTopo CSR (route/src/formats/cch_topo.rs:
+ * 250-286), INF arcs included; its weights field holds each arc's weight
+ * before relaxation (the reference's bottom-up weights: the metric weight of
+ * an original arc, the sum over its contraction middle for a shortcut, or
+ * BFG_INF for a shortcut not yet weighted).  No context is needed: a call
+ * uploads the topology to `device`, relaxes it and downloads the result. */
+typedef struct bfg_relax_stats {
+    uint64_t triangles;     /* (x -> m, m -> y) pairs evaluated with both halves finite   */
+    uint64_t improvements;  /* atomic mins that lowered an arc (depends on the schedule,
+                               like the reference's per-pass update counts)                */
+    uint32_t levels;        /* dependency levels of apexes (one launch each when non-empty) */
+    uint32_t launches;      /* relaxation kernel launches                                   */
+    uint32_t passes;        /* 1: one pass in level order reaches the fixpoint             */
+    float kernel_ms;        /* device time of the relaxation launches (HIP events)          */
+    float total_ms;         /* wall time of the call: checks, levels, copies, kernels       */
+} bfg_relax_stats;
+
+/* triangle_relax_parallel (route/src/customization.rs:1083-1200): for every
+ * apex m, DOWN arc x -> m and UP arc m -> y (y != x) with both weights finite,
+ * the packed word (weight, middle) of the arc x -> y takes the least of
+ * itself and (w(x -> m) + w(m -> y) saturating, m), to the fixpoint.
+ * up_middle / down_middle (nullable: all u32::MAX, the reference's mark of
+ * an original arc) are the middles before relaxation.  Outputs: the relaxed
+ * weights, and the middles (nullable).  Equal to the reference's result bit
+ * for bit (weights and middles; ties go to the smallest apex).  UP and DOWN
+ * targets must ascend per tail (find_edge_index, :1507-1520); an arc x -> y
+ * missing from the topology is skipped, as there. */
+int bfg_triangle_relax(const bfg_graph_desc *topo, int device, const uint32_t *up_middle,
+                       const uint32_t *down_middle, uint32_t *up_weights_out, uint32_t *down_weights_out,
+                       uint32_t *up_middle_out, uint32_t *down_middle_out, bfg_relax_stats *stats);
+
+/* triangle_relax_lex_parallel (route/src/customization.rs:1253-1440): the
+ * same relaxation over the packed word (time, length), the topology's weights
+ * being the times and up_length / down_length the lengths along them; then
+ * each shortcut arc (bit i % 64 of word i / 64 of up_is_shortcut /
+ * down_is_shortcut set) gets as middle the least apex whose two halves sum to
+ * exactly its converged word, or its given middle (nullable: u32::MAX) when
+ * none does; other arcs keep their given middle.  Outputs the relaxed times
+ * and the middles. */
+int bfg_triangle_relax_lex(const bfg_graph_desc *topo, int device, const uint32_t *up_length,
+                           const uint32_t *down_length, const uint64_t *up_is_shortcut,
+                           const uint64_t *down_is_shortcut, const uint32_t *up_middle, const uint32_t *down_middle,
+                           uint32_t *up_times_out, uint32_t *down_times_out, uint32_t *up_middle_out,
+                           uint32_t *down_middle_out, bfg_relax_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,11 @@ def lib():
             f = getattr(_lib, k)
             f.argtypes = v
             f.restype = C.c_int
+        _lib.orc_triangle_relax.argtypes = [C.c_uint32, u64p, u32p, u64p, u32p, u64p, u64p, C.c_int, u64p]
+        _lib.orc_triangle_relax.restype = C.c_int
+        _lib.orc_elect_middles.argtypes = [C.c_uint32, u64p, u32p, u64p, u32p, u64p, u64p, u64p, u64p, u32p, u32p,
+                                           u32p, u32p]
+        _lib.orc_elect_middles.restype = C.c_int
         _lib.orc_flat_free.argtypes = [fp]
         _lib.orc_flat_free.restype = None
         _lib.orc_seeded_state_new.argtypes = [C.c_uint32]
@@ -359,6 +364,53 @@ class Oracle:
         if st:
             self.L.orc_seeded_state_free(st)
             self._st = None
+
+
+def _pack(hi, lo):
+    return (np.asarray(hi, dtype=np.uint64) << np.uint64(32)) | np.asarray(lo, dtype=np.uint64)
+
+
+def _sc_words(mask):
+    mask = np.asarray(mask, dtype=bool)
+    words = np.zeros((mask.size + 63) // 64 * 8, dtype=np.uint8)
+    b = np.packbits(mask, bitorder="little")
+    words[:b.size] = b
+    return words.view(np.uint64)
+
+
+def triangle_relax(g, up_w, dn_w, up_mid=None, dn_mid=None):
+    """triangle_relax_parallel (customization.rs:1083-1200) over g's topology:
+    (up weights, down weights, up middles, down middles, passes)."""
+    up_pk = _pack(up_w, INF if up_mid is None else up_mid)
+    dn_pk = _pack(dn_w, INF if dn_mid is None else dn_mid)
+    passes = lib().orc_triangle_relax(g.n, _p(g.up_off, u64p), _p(g.up_tgt), _p(g.dn_off, u64p), _p(g.dn_tgt),
+                                      _p(up_pk, u64p), _p(dn_pk, u64p), 0, None)
+    if passes < 0:
+        raise RuntimeError("CCH customization did not converge after 100 passes")
+    lo = np.uint64(0xFFFFFFFF)
+    return ((up_pk >> np.uint64(32)).astype(np.uint32), (dn_pk >> np.uint64(32)).astype(np.uint32),
+            (up_pk & lo).astype(np.uint32), (dn_pk & lo).astype(np.uint32), passes)
+
+
+def triangle_relax_lex(g, t_up, t_dn, len_up, len_dn, up_sc, dn_sc, up_mid=None, dn_mid=None):
+    """triangle_relax_lex_parallel (customization.rs:1253-1440) over g's
+    topology: (up times, down times, up middles, down middles, passes, up
+    lengths, down lengths), the lengths being the converged low words."""
+    up_pk = _pack(t_up, len_up)
+    dn_pk = _pack(t_dn, len_dn)
+    L = lib()
+    passes = L.orc_triangle_relax(g.n, _p(g.up_off, u64p), _p(g.up_tgt), _p(g.dn_off, u64p), _p(g.dn_tgt),
+                                  _p(up_pk, u64p), _p(dn_pk, u64p), 1, None)
+    um = np.empty(g.m_up, dtype=np.uint32)
+    dm = np.empty(g.m_down, dtype=np.uint32)
+    umi = None if up_mid is None else _u32(up_mid)
+    dmi = None if dn_mid is None else _u32(dn_mid)
+    L.orc_elect_middles(g.n, _p(g.up_off, u64p), _p(g.up_tgt), _p(g.dn_off, u64p), _p(g.dn_tgt), _p(up_pk, u64p),
+                        _p(dn_pk, u64p), _p(_sc_words(up_sc), u64p), _p(_sc_words(dn_sc), u64p),
+                        None if umi is None else _p(umi), None if dmi is None else _p(dmi), _p(um), _p(dm))
+    lo = np.uint64(0xFFFFFFFF)
+    return ((up_pk >> np.uint64(32)).astype(np.uint32), (dn_pk >> np.uint64(32)).astype(np.uint32), um, dm, passes,
+            (up_pk & lo).astype(np.uint32), (dn_pk & lo).astype(np.uint32))
 
 
 def dijkstra(n, off, head, w, src):

@@ -27,10 +27,16 @@
  *                              forward UP sweeps into buckets, backward sweeps over
  *                              DownReverseAdjFlat, join at every settled node)
  *   orc_dijkstra               plain Dijkstra, the brute-force pin
+ *   orc_triangle_relax         customization.rs:1083-1200 (triangle_relax_parallel)
+ *                              and :1253-1356 (the (time, length) relaxation of
+ *                              triangle_relax_lex_parallel)
+ *   orc_elect_middles          customization.rs:1358-1437 (its middle election)
  *
- * Parity pins (tests/): the reference's hand vectors (range/phast.rs:1198-1554),
+ * Parity pins (tests/): the reference's hand vectors (range/phast.rs:1198-1554,
+ * and the customization vectors of customization.rs:2265-2500),
  * the broom closed form (bucket_ch.rs:5638-5670), brute-force Dijkstra on the
- * CCH union graph and on the edge-based graph, and the invariants the
+ * CCH union graph and on the edge-based graph, Floyd-Warshall in rank order for
+ * the customized weights, and the invariants the
  * reference's own bench/validators assert (K-lane == single source, SoA == AoS,
  * bounded == unbounded within T, gated == ungated within T).
  */
@@ -778,5 +784,142 @@ int orc_dijkstra(uint32_t n, const uint64_t *off, const uint32_t *head, const ui
         }
     }
     hp_free(&h);
+    return 0;
+}
+
+/* ---- CCH customization: triangle relaxation ---------------------------- */
+
+/* find_edge_index (customization.rs:1507-1520): binary search of v among the
+ * ascending targets of u; 1 and *idx when found. */
+static int find_edge_index(uint32_t u, uint32_t v, const uint64_t *off, const uint32_t *tgt, uint64_t *idx) {
+    uint64_t lo = off[u], hi = off[u + 1];
+    while (lo < hi) {
+        uint64_t mid = lo + (hi - lo) / 2;
+        if (tgt[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < off[u + 1] && tgt[lo] == v) { *idx = lo; return 1; }
+    return 0;
+}
+
+/* ReverseDownAdj (customization.rs:832-905): per apex m the DOWN arcs x -> m
+ * as (x, arc index), x ascending.  Caller frees *roff, *rsrc, *ridx. */
+static void reverse_down_adj(uint32_t n, const uint64_t *dn_off, const uint32_t *dn_tgt, uint64_t **roff,
+                             uint32_t **rsrc, uint64_t **ridx) {
+    uint64_t md = dn_off[n];
+    uint64_t *off = (uint64_t *)calloc((size_t)n + 1, sizeof(uint64_t));
+    for (uint64_t i = 0; i < md; ++i) off[dn_tgt[i] + 1]++;
+    for (uint32_t m = 0; m < n; ++m) off[m + 1] += off[m];
+    uint64_t *ins = (uint64_t *)calloc((size_t)n + 1, sizeof(uint64_t));
+    uint32_t *src = (uint32_t *)malloc((md ? md : 1) * sizeof(uint32_t));
+    uint64_t *idx = (uint64_t *)malloc((md ? md : 1) * sizeof(uint64_t));
+    for (uint32_t u = 0; u < n; ++u)
+        for (uint64_t i = dn_off[u]; i < dn_off[u + 1]; ++i) {
+            uint32_t m = dn_tgt[i];
+            uint64_t pos = off[m] + ins[m]++;
+            src[pos] = u;
+            idx[pos] = i;
+        }
+    free(ins);
+    *roff = off; *rsrc = src; *ridx = idx;
+}
+
+/* triangle_relax_parallel (customization.rs:1083-1200) with lex = 0: words
+ * are (weight << 32 | middle) and a candidate through apex m is
+ * (w(x->m) + w(m->y) saturating) << 32 | m.  With lex = 1 it is
+ * triangle_relax_lex_parallel's relaxation (:1253-1356): words are
+ * (time << 32 | length) and both channels add separately, saturating.
+ * Passes run over every apex until one changes nothing (the reference runs a
+ * pass's apexes in parallel with relaxed fetch_min; a pass here takes them in
+ * rank order, one of those interleavings, and the fixpoint is the same).
+ * Returns the number of passes, or -1 when the time-only variant has not
+ * converged after 100 passes (the reference panics there, :1180-1186).
+ * *updates (nullable) receives the sum of the passes' update counts. */
+int orc_triangle_relax(uint32_t n, const uint64_t *up_off, const uint32_t *up_tgt, const uint64_t *dn_off,
+                       const uint32_t *dn_tgt, uint64_t *up_pk, uint64_t *dn_pk, int lex, uint64_t *updates) {
+    uint64_t *roff; uint32_t *rsrc; uint64_t *ridx;
+    reverse_down_adj(n, dn_off, dn_tgt, &roff, &rsrc, &ridx);
+    uint64_t total = 0;
+    int pass = 0;
+    for (;;) {
+        ++pass;
+        uint64_t pu = 0;
+        for (uint32_t m = 0; m < n; ++m) {
+            for (uint64_t ir = roff[m]; ir < roff[m + 1]; ++ir) {
+                uint32_t x = rsrc[ir];
+                uint64_t p_xm = dn_pk[ridx[ir]];
+                uint32_t w_xm = (uint32_t)(p_xm >> 32);
+                if (w_xm == INF) continue;
+                for (uint64_t i_my = up_off[m]; i_my < up_off[m + 1]; ++i_my) {
+                    uint32_t y = up_tgt[i_my];
+                    if (y == x) continue;
+                    uint64_t p_my = up_pk[i_my];
+                    uint32_t w_my = (uint32_t)(p_my >> 32);
+                    if (w_my == INF) continue;
+                    uint32_t low = lex ? sat_add((uint32_t)p_xm, (uint32_t)p_my) : m;
+                    uint64_t cand = ((uint64_t)sat_add(w_xm, w_my) << 32) | low;
+                    uint64_t idx;
+                    uint64_t *word = NULL;
+                    if (y > x) {
+                        if (find_edge_index(x, y, up_off, up_tgt, &idx)) word = &up_pk[idx];
+                    } else if (find_edge_index(x, y, dn_off, dn_tgt, &idx)) {
+                        word = &dn_pk[idx];
+                    }
+                    if (word && cand < *word) { *word = cand; ++pu; }
+                }
+            }
+        }
+        total += pu;
+        if (pu == 0) break;
+        if (!lex && pass >= 100) { pass = -1; break; }
+    }
+    free(roff); free(rsrc); free(ridx);
+    if (updates) *updates = total;
+    return pass;
+}
+
+/* The middle election after triangle_relax_lex_parallel's relaxation
+ * (customization.rs:1358-1437): every shortcut arc (bit i % 64 of word i / 64
+ * of *_sc) takes the least apex whose halves' packed sum equals its converged
+ * word; one with no such apex, and every original arc, keeps mid_in (nullable:
+ * u32::MAX). */
+int orc_elect_middles(uint32_t n, const uint64_t *up_off, const uint32_t *up_tgt, const uint64_t *dn_off,
+                      const uint32_t *dn_tgt, const uint64_t *up_pk, const uint64_t *dn_pk, const uint64_t *up_sc,
+                      const uint64_t *dn_sc, const uint32_t *up_mid_in, const uint32_t *dn_mid_in, uint32_t *up_mid,
+                      uint32_t *dn_mid) {
+    uint64_t mu = up_off[n], md = dn_off[n];
+    uint64_t *roff; uint32_t *rsrc; uint64_t *ridx;
+    reverse_down_adj(n, dn_off, dn_tgt, &roff, &rsrc, &ridx);
+    for (uint64_t i = 0; i < mu; ++i) up_mid[i] = INF;
+    for (uint64_t i = 0; i < md; ++i) dn_mid[i] = INF;
+    for (uint32_t m = 0; m < n; ++m)
+        for (uint64_t ir = roff[m]; ir < roff[m + 1]; ++ir) {
+            uint32_t x = rsrc[ir];
+            uint64_t p_xm = dn_pk[ridx[ir]];
+            uint32_t t_xm = (uint32_t)(p_xm >> 32);
+            if (t_xm == INF) continue;
+            for (uint64_t i_my = up_off[m]; i_my < up_off[m + 1]; ++i_my) {
+                uint32_t y = up_tgt[i_my];
+                if (y == x) continue;
+                uint64_t p_my = up_pk[i_my];
+                uint32_t t_my = (uint32_t)(p_my >> 32);
+                if (t_my == INF) continue;
+                uint64_t cand = ((uint64_t)sat_add(t_xm, t_my) << 32) | sat_add((uint32_t)p_xm, (uint32_t)p_my);
+                uint64_t idx;
+                if (y > x) {
+                    if (find_edge_index(x, y, up_off, up_tgt, &idx) && ((up_sc[idx >> 6] >> (idx & 63)) & 1) &&
+                        cand == up_pk[idx] && m < up_mid[idx])
+                        up_mid[idx] = m;
+                } else if (find_edge_index(x, y, dn_off, dn_tgt, &idx) && ((dn_sc[idx >> 6] >> (idx & 63)) & 1) &&
+                           cand == dn_pk[idx] && m < dn_mid[idx]) {
+                    dn_mid[idx] = m;
+                }
+            }
+        }
+    for (uint64_t i = 0; i < mu; ++i)
+        if (up_mid[i] == INF) up_mid[i] = up_mid_in ? up_mid_in[i] : INF;
+    for (uint64_t i = 0; i < md; ++i)
+        if (dn_mid[i] == INF) dn_mid[i] = dn_mid_in ? dn_mid_in[i] : INF;
+    free(roff); free(rsrc); free(ridx);
     return 0;
 }

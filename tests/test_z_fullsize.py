@@ -15,7 +15,9 @@ Every check is bit-exact against the oracle (u32 integer work):
   the limit (and the bits of a second call) and 4 lanes against its seeded
   PHAST;
 - a mixed serving burst: every result equal to the request issued alone, and
-  a sample against the oracle.
+  a sample against the oracle;
+- the customization of the whole CCH (triangle relaxation from the original
+  arc weights, shortcuts INF): every weight equal to the CPU builder's.
 """
 import time
 
@@ -162,3 +164,25 @@ def test_config5_serving_burst(country, request):
     for k in isos:
         exp = o.seeded([(reqs[k].origin, 0)], reqs[k].limit)
         assert got[k].tolist() == [list(x) for x in exp]
+
+
+def test_customize_country_graph(country_graph, request):
+    """GPU triangle relaxation of the whole 6.7M-node CCH from its original arc
+    weights: every UP and DOWN weight equal to the CPU builder's customized
+    weights (the same fixpoint computed by its fused contraction), and a
+    sample of elected middles splitting their arcs into two that add up."""
+    from bfgpu import customize
+    g = country_graph
+    t0 = time.perf_counter()
+    out, r = customize(g)
+    progress(request.config, f"customization in {time.perf_counter() - t0:.1f} s: {r.stats}")
+    assert np.array_equal(out.up_w, g.up_w) and np.array_equal(out.dn_w, g.dn_w)
+    rng = np.random.default_rng(3)
+    relaxed = np.flatnonzero(r.up_middles != INF)
+    for i in rng.choice(relaxed, 500, replace=False):
+        x = int(np.searchsorted(g.up_off, i, side="right") - 1)
+        y, m = int(g.up_tgt[i]), int(r.up_middles[i])
+        a = int(g.dn_off[x]) + int(np.searchsorted(g.dn_tgt[g.dn_off[x]:g.dn_off[x + 1]], m))
+        b = int(g.up_off[m]) + int(np.searchsorted(g.up_tgt[g.up_off[m]:g.up_off[m + 1]], y))
+        assert g.dn_tgt[a] == m and g.up_tgt[b] == y
+        assert min(int(g.dn_w[a]) + int(g.up_w[b]), INF) == g.up_w[i]
