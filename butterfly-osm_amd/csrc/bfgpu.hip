@@ -2118,19 +2118,46 @@ int build_up_lists(bfg_ctx *c, Work *w, uint32_t nl) {
         }
     }
     for (uint32_t l = 0; l <= nsl; ++l) toff[l] = soff[L.seglvl_seg[l]];
+    // the push list: per level, every target block of a reached segment once
+    // per reached wave group; sized first, then filled by threads over ranges
+    // of levels of about equal size
     poff.assign(nsl + 1, 0);
-    pu.clear();
-    pg.clear();
+    uint64_t total_p = 0;
     for (uint32_t l = 0; l < nsl; ++l) {
-        poff[l] = (uint32_t)pu.size();
         for (uint32_t s = L.seglvl_seg[l]; s < L.seglvl_seg[l + 1]; ++s)
-            for (uint32_t b = L.pt_blk[s]; b < L.pt_blk[s + 1]; ++b)
-                for (uint32_t k = soff[s]; k < soff[s + 1]; ++k) {
-                    pu.push_back(b);
-                    pg.push_back(tg[k]);
-                }
+            total_p += (uint64_t)(L.pt_blk[s + 1] - L.pt_blk[s]) * (soff[s + 1] - soff[s]);
+        if (total_p > 0xFFFFFFFFull) return fail(BFG_NOT_SUPPORTED, "more than 2^32 - 1 upward push units");
+        poff[l + 1] = (uint32_t)total_p;
     }
-    poff[nsl] = (uint32_t)pu.size();
+    pu.resize(total_p);
+    pg.resize(total_p);
+    auto fill_levels = [&](uint32_t l0, uint32_t l1) {
+        for (uint32_t l = l0; l < l1; ++l) {
+            uint32_t o = poff[l];
+            for (uint32_t s = L.seglvl_seg[l]; s < L.seglvl_seg[l + 1]; ++s)
+                for (uint32_t b = L.pt_blk[s]; b < L.pt_blk[s + 1]; ++b)
+                    for (uint32_t k = soff[s]; k < soff[s + 1]; ++k, ++o) {
+                        pu[o] = b;
+                        pg[o] = tg[k];
+                    }
+        }
+    };
+    const uint32_t nth = total_p < (1u << 20) ? 1u : std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+    if (nth == 1) {
+        fill_levels(0, nsl);
+    } else {
+        std::vector<std::thread> th;
+        uint32_t l0 = 0;
+        for (uint32_t t = 1; t <= nth && l0 < nsl; ++t) {
+            const uint64_t goal = total_p * t / nth;
+            uint32_t l1 = l0;
+            while (l1 < nsl && (t == nth || poff[l1 + 1] <= goal)) ++l1;
+            if (l1 == l0) l1 = l0 + 1;
+            th.emplace_back(fill_levels, l0, l1);
+            l0 = l1;
+        }
+        for (auto &h : th) h.join();
+    }
     ENSURE(w, tu, std::max<size_t>(tu.size(), 1), "upward lists");
     ENSURE(w, tg, std::max<size_t>(tg.size(), 1), "upward lists");
     ENSURE(w, pu, std::max<size_t>(pu.size(), 1), "upward lists");
@@ -3751,7 +3778,7 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
     }
     const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1, words = std::max<uint32_t>((nseg + 63) / 64, 1);
     const auto t_start = std::chrono::steady_clock::now();
-    std::unordered_map<uint32_t, std::vector<uint32_t>> tree_tgts;  // bottom tree root -> target lanes
+    std::vector<uint64_t> tree_tgts;  // (bottom tree root << 32 | target lane), ascending
     float join_ms = 0, kernel_ms = 0;
     uint32_t joins = 0;
     uint64_t pairs = 0, visits = 0;
@@ -3781,8 +3808,9 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
         tree_tgts.clear();
         for (uint32_t b = 0; b < tnl; ++b) {
             const uint32_t p = L.pos_of_rank[tgt[trows[b]]];
-            if (p >= L.n_top) tree_tgts[bottom_root(c, p)].push_back(b);
+            if (p >= L.n_top) tree_tgts.push_back((uint64_t)bottom_root(c, p) << 32 | b);
         }
+        std::sort(tree_tgts.begin(), tree_tgts.end());
         return BFG_SUCCESS;
     };
     // the join of source batch snl (on wf) with the ready target batch tnl
@@ -3808,11 +3836,11 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
             for (uint32_t a = 0; a < snl; ++a) {
                 const uint32_t p = L.pos_of_rank[src[srows[a]]];
                 if (p < L.n_top) continue;
-                const auto it = tree_tgts.find(bottom_root(c, p));
-                if (it == tree_tgts.end()) continue;
-                for (uint32_t b : it->second) {
+                const uint64_t root = bottom_root(c, p);
+                for (auto it = std::lower_bound(tree_tgts.begin(), tree_tgts.end(), root << 32);
+                     it != tree_tgts.end() && (*it >> 32) == root; ++it) {
                     pr.push_back(a);
-                    pr.push_back(b);
+                    pr.push_back((uint32_t)*it);
                 }
             }
         const StreamDrain drain{wf->stream};  // the pair list is read by an asynchronous copy

@@ -42,6 +42,7 @@
 #include <chrono>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/bfgpu.h"
@@ -399,14 +400,27 @@ Rows build_rows(const bfg_graph_desc *g) {
     R.m.resize(md);
     R.x.resize(md);
     R.e.resize(md);
-    for (uint32_t x = 0; x < n; ++x)
-        for (uint64_t i = g->down_offsets[x]; i < g->down_offsets[x + 1]; ++i) {
-            const uint32_t m = g->down_targets[i];
-            const uint64_t r = start[m]++;
-            R.m[r] = m;
-            R.x[r] = x;
-            R.e[r] = (uint32_t)i;
-        }
+    // thread t writes the rows of the apexes m = t (mod T), scanning every DOWN
+    // arc in tail order, so each apex's rows keep ascending x
+    const uint32_t T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto fill = [&](uint32_t t) {
+        for (uint32_t x = 0; x < n; ++x)
+            for (uint64_t i = g->down_offsets[x]; i < g->down_offsets[x + 1]; ++i) {
+                const uint32_t m = g->down_targets[i];
+                if (m % T != t) continue;
+                const uint64_t r = start[m]++;
+                R.m[r] = m;
+                R.x[r] = x;
+                R.e[r] = (uint32_t)i;
+            }
+    };
+    if (md < (1u << 22)) {
+        for (uint32_t t = 0; t < T; ++t) fill(t);
+    } else {
+        std::vector<std::thread> th;
+        for (uint32_t t = 0; t < T; ++t) th.emplace_back(fill, t);
+        for (auto &h : th) h.join();
+    }
     R.level_off = std::move(lev_rows);
     return R;
 }

@@ -9,7 +9,8 @@ Bit-exact against the oracle's bucket restatement and its PHAST matrix
 the hand graphs, the broom closed form, the 224 x 224 grid with diagonal and
 same-bottom-tree pairs (sources and targets that meet below the top region),
 thresholds, lane widths 16/32/64, several source and target batches, the
-device entry point, and the chord grid."""
+device entry point, the chord grid, and label widths 16 and 32 (the packed
+16-bit upward pushes and their u32 fallback at x40 weights)."""
 import sys
 
 import numpy as np
@@ -141,3 +142,37 @@ def test_bucket_device_entry_and_chords(engine_cls):
         assert np.array_equal(dout.to_host(), exp)
         e.table_bucket_device(ds.ptr, s.size, dt.ptr, t.size, dout.ptr, 900)
         assert np.array_equal(dout.to_host(), mask(exp, 900))
+
+
+def scaled(g, k):
+    """g with every finite weight times k (saturating below INF)."""
+    import dataclasses
+    def sc(w):
+        w64 = w.astype(np.uint64) * np.uint64(k)
+        return np.where(w == INF, np.uint64(INF), np.minimum(w64, np.uint64(INF - 1))).astype(np.uint32)
+    return dataclasses.replace(g, up_w=sc(g.up_w), dn_w=sc(g.dn_w))
+
+
+@pytest.mark.parametrize("scale", [1, 40])
+@pytest.mark.parametrize("bits", [16, 32])
+def test_bucket_label_widths(engine_cls, grid_config1, scale, bits):
+    """The upward pushes packed 16-bit (two wave groups a wavefront) and in
+    u32 give the same cells; at x40 weights labels pass 16 bits and the
+    packed units fall back to the u32 rows."""
+    g = grid_config1 if scale == 1 else scaled(grid_config1, scale)
+    o = Oracle(g)
+    rng = np.random.default_rng(7 * scale + bits)
+    s = rng.integers(0, g.n, 170).astype(np.uint32)
+    t = rng.integers(0, g.n, 150).astype(np.uint32)
+    ns, nt = near_pairs(g, rng, 20)
+    s, t = np.concatenate([s, ns]), np.concatenate([t, nt])
+    say("bucket label bits", bits, "weights x", scale, s.size, "x", t.size)
+    exp = o.compute_matrix_flat_mt(s, t)
+    with engine_cls(g) as e:
+        rev = e.pair_reverse()
+        for x in (e, rev):
+            x.set_label_width(bits)
+        m, _ = e.table_bucket_parallel(s, t)
+        assert np.array_equal(m, exp)
+        mb, _ = e.table_bucket_parallel_bounded(s, t, 2000 * scale)
+        assert np.array_equal(mb, mask(exp, 2000 * scale))
