@@ -858,7 +858,8 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_list_kernel(uint32_t count, 
                                                                 const uint32_t *__restrict__ seg_pos,
                                                                 const uint32_t *__restrict__ tab,
                                                                 const uint32_t *__restrict__ tw,
-                                                                uint32_t *__restrict__ arena, uint32_t n) {
+                                                                uint32_t *__restrict__ arena, uint32_t n,
+                                                                uint32_t *__restrict__ segmin, uint32_t nseg) {
     const uint32_t i = blockIdx.x * (TILE_THREADS / 64) + wave_id();
     if (i >= count) return;
     const uint32_t s = lu[i];
@@ -871,6 +872,14 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_list_kernel(uint32_t count, 
 #pragma unroll
     for (int j = 0; j < 64; ++j)
         if (j < len) lab[(size_t)(p0 + j) * KL] = acc[j];
+    // the segment's labels are final now: their least over the wave group
+    // bounds every meeting through the segment (the join's pruning)
+    uint32_t m = INF;
+#pragma unroll
+    for (int j = 0; j < 64; ++j)
+        if (j < len) m = min(m, acc[j]);
+    m = wave_min(m);
+    if (__lane_id() == 0) segmin[(size_t)lg[i] * nseg + s] = m;
 }
 
 template <int KL>
@@ -1664,6 +1673,7 @@ struct Work {
     // (push), host lists and their device copies; item ranges per level
     std::vector<uint32_t> h_tu, h_tg, h_pu, h_pg, h_toff, h_poff, h_mark;
     DevBuf<uint32_t> tu, tg, pu, pg;
+    DevBuf<uint32_t> segmin;  // upward-only batches: [wave group][segment] least final label
     std::vector<uint32_t> h_pairs;
     // host staging of a call (kept, so a fixed Work reuses its capacity)
     std::vector<uint32_t> h_bt, h_pending, h_order, h_soff, h_srank, h_scost, h_lim, h_rows, h_sdepth, h_spre,
@@ -2187,6 +2197,11 @@ int launch_up_lists(bfg_ctx *c, Work *w, uint32_t nl, PhaseTimes *pt) {
     if (rc) return rc;
     constexpr uint32_t WPB = TILE_THREADS / 64;
     uint32_t nlaunch = 0;
+    // per (wave group, segment): the least label of the segment once final, INF
+    // where the wave group does not reach it
+    const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
+    ENSURE(w, segmin, std::max<size_t>((size_t)(nl + 63) / 64 * nseg, 1), "segment minima");
+    HIP_TRY(bfg_meet::launch_fill(s, w->segmin.p, (uint64_t)(nl + 63) / 64 * nseg, INF));
     HIP_TRY(hipEventRecord(w->ev[0], s));
     const uint32_t total = (uint32_t)w->h_tu.size();
     if (total) {
@@ -2203,7 +2218,8 @@ int launch_up_lists(bfg_ctx *c, Work *w, uint32_t nl, PhaseTimes *pt) {
         const uint32_t t0 = w->h_toff[l], tc = w->h_toff[l + 1] - t0;
         if (tc) {
             hipLaunchKernelGGL(tri_list_kernel<KL>, dim3((tc + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, tc,
-                               w->tu.p + t0, w->tg.p + t0, c->seg_pos.p, c->tb_up.p, c->tb_w.p, w->arena.p, n);
+                               w->tu.p + t0, w->tg.p + t0, c->seg_pos.p, c->tb_up.p, c->tb_w.p, w->arena.p, n,
+                               w->segmin.p, nseg);
             ++nlaunch;
         }
         const uint32_t p0 = w->h_poff[l], pc = w->h_poff[l + 1] - p0;
@@ -3823,10 +3839,10 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
         if (splits > 1) HIP_TRY(bfg_meet::launch_fill(wf->stream, d_out, (uint64_t)S * T, INF));
         ENSURE(wf, visits, 1, "join counter");
         HIP_TRY(bfg_meet::launch_fill(wf->stream, (uint32_t *)wf->visits.p, 2, 0));
-        const bfg_meet::Side A{wf->arena.p, L.n_top, c->lanes, snl, wf->reach.p, wf->row_map.p};
-        const bfg_meet::Side B{wb->arena.p, L.n_top, r->lanes, tnl, wb->reach.p, wb->row_map.p};
+        const bfg_meet::Side A{wf->arena.p, L.n_top, c->lanes, snl, wf->reach.p, wf->row_map.p, wf->segmin.p};
+        const bfg_meet::Side B{wb->arena.p, L.n_top, r->lanes, tnl, wb->reach.p, wb->row_map.p, wb->segmin.p};
         HIP_TRY(hipEventRecord(wf->ev[1], wf->stream));
-        HIP_TRY(bfg_meet::launch_join(wf->stream, A, B, c->seg_pos.p, words, thr, splits, d_out, T,
+        HIP_TRY(bfg_meet::launch_join(wf->stream, A, B, c->seg_pos.p, words, nseg, thr, splits, d_out, T,
                                       wf->visits.p));
         HIP_TRY(hipEventRecord(wf->ev[2], wf->stream));
         // sources and targets in one bottom tree also meet below the top region

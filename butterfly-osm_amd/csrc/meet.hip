@@ -64,7 +64,9 @@ __global__ __launch_bounds__(64) void meet_reach_kernel(const uint32_t *__restri
 // 64A + 8i .. and target lanes 64B + 8j .. in registers; per position x it
 // loads its 8 source labels and 8 target labels (two 16-byte loads each; the
 // 8 threads of a row share an address) and relaxes 64 cells.  Two positions
-// per step: acc = min3(acc, a0 + b0, a1 + b1).
+// per step: acc = min3(acc, a0 + b0, a1 + b1).  With both sides' segment
+// minima a tile takes its common segments deepest first and skips a segment
+// whose two minima add up to no less than its largest cell so far.
 // Tiles go to the XCDs in contiguous ranges (block b runs on XCD b % 8), so
 // the tiles of one source wave group run side by side on one XCD and share its
 // labels in that XCD's L2.  With `splits` > 1 (few tiles), split sp of a tile
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(64) void meet_reach_kernel(const uint32_t *__restri
 struct JoinArgs {
     Side A, B;
     const uint32_t *seg_pos;
-    uint32_t words, wga, wgb, thr, splits;
+    uint32_t words, nseg, wga, wgb, thr, splits;
     uint32_t *out;
     uint64_t ld;
     unsigned long long *visits;
@@ -85,6 +87,12 @@ __device__ __forceinline__ void load8(uint32_t (&v)[8], const uint32_t *p) {
     const uint4 hi = *reinterpret_cast<const uint4 *>(p + 4);
     v[0] = lo.x; v[1] = lo.y; v[2] = lo.z; v[3] = lo.w;
     v[4] = hi.x; v[5] = hi.y; v[6] = hi.z; v[7] = hi.w;
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
 }
 
 __global__ __launch_bounds__(64) void meet_join_kernel(JoinArgs a) {
@@ -101,16 +109,36 @@ __global__ __launch_bounds__(64) void meet_join_kernel(JoinArgs a) {
     const uint32_t *fb = a.B.lab + (size_t)(lb / klb) * a.B.stride * klb + lb % klb;
     const unsigned long long *ra = a.A.reach + (size_t)ta * a.words;
     const unsigned long long *rb = a.B.reach + (size_t)tb * a.words;
+    // cells of lanes past either side's count start at 0: they are never
+    // written out, and they cannot raise the tile's largest cell
     uint32_t acc[64];
 #pragma unroll
-    for (int k = 0; k < 64; ++k) acc[k] = INF;
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[r * 8 + c] = la + r < a.A.n_lanes && lb + c < a.B.n_lanes ? INF : 0u;
+    const bool prune = a.A.segmin && a.B.segmin;
+    const uint32_t *sma = prune ? a.A.segmin + (size_t)ta * a.nseg : nullptr;
+    const uint32_t *smb = prune ? a.B.segmin + (size_t)tb * a.nseg : nullptr;
     uint32_t k = 0, seen = 0;  // common segments seen so far, positions visited
-    for (uint32_t w = 0; w < a.words; ++w) {
+    // deepest segments first (a segment's index grows with its depth), so
+    // the cells are tight by the time the wide segments near the root come
+    for (uint32_t w = a.words; w-- > 0;) {
         unsigned long long m = ra[w] & rb[w];
         while (m) {
-            const uint32_t s = w * 64 + (uint32_t)__builtin_ctzll(m);
-            m &= m - 1;
+            const uint32_t bit = 63u - (uint32_t)__builtin_clzll(m);
+            m &= ~(1ull << bit);
+            const uint32_t s = w * 64 + bit;
             if (k++ % a.splits != sp) continue;
+            if (prune) {
+                // every sum through s is at least its minima's: a segment whose
+                // bound reaches the tile's largest cell (or passes thr) lowers
+                // no cell that is kept
+                const uint32_t bound = sat_add(sma[s], smb[s]);
+                uint32_t top = 0;
+#pragma unroll
+                for (int q = 0; q < 64; ++q) top = max(top, acc[q]);
+                if (bound >= wave_max(top) || bound > a.thr) continue;
+            }
             const uint32_t p0 = a.seg_pos[s], p1 = a.seg_pos[s + 1];
             seen += p1 - p0;
             for (uint32_t x = p0; x < p1; x += 2) {
@@ -203,9 +231,11 @@ hipError_t launch_reach(hipStream_t st, const uint32_t *seed_off, const uint32_t
 }
 
 hipError_t launch_join(hipStream_t st, const Side &A, const Side &B, const uint32_t *seg_pos, uint32_t words,
-                       uint32_t thr, uint32_t splits, uint32_t *out, uint64_t ld, unsigned long long *visits) {
-    JoinArgs a{A, B, seg_pos, words, (A.n_lanes + 63) / 64, (B.n_lanes + 63) / 64, thr, splits ? splits : 1, out,
-               ld, visits};
+                       uint32_t nseg, uint32_t thr, uint32_t splits, uint32_t *out, uint64_t ld,
+                       unsigned long long *visits) {
+    if ((A.segmin || B.segmin) && (uint64_t)words * 64 < nseg) return hipErrorInvalidValue;
+    JoinArgs a{A, B, seg_pos, words, nseg, (A.n_lanes + 63) / 64, (B.n_lanes + 63) / 64, thr, splits ? splits : 1,
+               out, ld, visits};
     const uint64_t tiles = (uint64_t)a.wga * a.wgb * a.splits;
     if (!tiles) return hipSuccess;
     const uint64_t blocks = (tiles + XCDS - 1) / XCDS * XCDS;

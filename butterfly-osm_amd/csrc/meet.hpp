@@ -28,6 +28,9 @@ struct Side {
     uint32_t stride = 0, kl = 0, n_lanes = 0;
     const unsigned long long *reach = nullptr;
     const uint32_t *index = nullptr;
+    // [wave group][segment]: the least label of the segment's positions over
+    // the wave group's lanes (INF where unreached), or null (no pruning)
+    const uint32_t *segmin = nullptr;
 };
 
 // reach[g * words + s / 64] bit s % 64: wave group g (lanes 64g ..) has a
@@ -39,12 +42,16 @@ hipError_t launch_reach(hipStream_t st, const uint32_t *seed_off, const uint32_t
 
 // out[A.index[a] * ld + B.index[b]] = min over the top positions x of the
 // segments both lanes' wave groups reach of A.lab(a, x) + B.lab(b, x)
-// (saturating; INF where no segment is common), INF where above thr.
+// (saturating; INF where no segment is common), INF where above thr.  With
+// both sides' segment minima (nseg segments a wave group), a tile visits the
+// common segments deepest first and skips a segment whose two minima add up
+// to no less than the tile's largest cell so far: no cell can fall there.
 // splits > 1 cuts each tile's common segments over that many wavefronts that
 // merge with atomic min: the output cells must hold INF before (launch_fill).
 // visits (when not null) gains the label positions the tiles visited.
 hipError_t launch_join(hipStream_t st, const Side &A, const Side &B, const uint32_t *seg_pos, uint32_t words,
-                       uint32_t thr, uint32_t splits, uint32_t *out, uint64_t ld, unsigned long long *visits);
+                       uint32_t nseg, uint32_t thr, uint32_t splits, uint32_t *out, uint64_t ld,
+                       unsigned long long *visits);
 
 // p[0 .. count) = value (a kernel, so it may sit in a captured graph).
 hipError_t launch_fill(hipStream_t st, uint32_t *p, uint64_t count, uint32_t value);
