@@ -872,14 +872,17 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_list_kernel(uint32_t count, 
 #pragma unroll
     for (int j = 0; j < 64; ++j)
         if (j < len) lab[(size_t)(p0 + j) * KL] = acc[j];
-    // the segment's labels are final now: their least over the wave group
-    // bounds every meeting through the segment (the join's pruning)
+    // the segment's labels are final now: their least over each 8 lanes of the
+    // wave group bounds every meeting of those lanes through the segment (the
+    // join's pruning)
     uint32_t m = INF;
 #pragma unroll
     for (int j = 0; j < 64; ++j)
         if (j < len) m = min(m, acc[j]);
-    m = wave_min(m);
-    if (__lane_id() == 0) segmin[(size_t)lg[i] * nseg + s] = m;
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o, 64));
+    const uint32_t lane = __lane_id();
+    if (lane % 8 == 0) segmin[((size_t)lg[i] * nseg + s) * 8 + lane / 8] = m;
 }
 
 template <int KL>
@@ -1673,7 +1676,7 @@ struct Work {
     // (push), host lists and their device copies; item ranges per level
     std::vector<uint32_t> h_tu, h_tg, h_pu, h_pg, h_toff, h_poff, h_mark;
     DevBuf<uint32_t> tu, tg, pu, pg;
-    DevBuf<uint32_t> segmin;  // upward-only batches: [wave group][segment] least final label
+    DevBuf<uint32_t> segmin;  // upward-only batches: [wave group][segment][8 lanes] least final label
     std::vector<uint32_t> h_pairs;
     // host staging of a call (kept, so a fixed Work reuses its capacity)
     std::vector<uint32_t> h_bt, h_pending, h_order, h_soff, h_srank, h_scost, h_lim, h_rows, h_sdepth, h_spre,
@@ -2197,11 +2200,12 @@ int launch_up_lists(bfg_ctx *c, Work *w, uint32_t nl, PhaseTimes *pt) {
     if (rc) return rc;
     constexpr uint32_t WPB = TILE_THREADS / 64;
     uint32_t nlaunch = 0;
-    // per (wave group, segment): the least label of the segment once final, INF
-    // where the wave group does not reach it
+    // per (wave group, segment, 8 lanes): the least label of the segment once
+    // final, INF where the wave group does not reach it
     const uint32_t nseg = (uint32_t)L.seg_pos.size() - 1;
-    ENSURE(w, segmin, std::max<size_t>((size_t)(nl + 63) / 64 * nseg, 1), "segment minima");
-    HIP_TRY(bfg_meet::launch_fill(s, w->segmin.p, (uint64_t)(nl + 63) / 64 * nseg, INF));
+    const uint64_t nmin = (uint64_t)(nl + 63) / 64 * nseg * 8;
+    ENSURE(w, segmin, std::max<size_t>(nmin, 1), "segment minima");
+    HIP_TRY(bfg_meet::launch_fill(s, w->segmin.p, nmin, INF));
     HIP_TRY(hipEventRecord(w->ev[0], s));
     const uint32_t total = (uint32_t)w->h_tu.size();
     if (total) {

@@ -65,8 +65,8 @@ __global__ __launch_bounds__(64) void meet_reach_kernel(const uint32_t *__restri
 // loads its 8 source labels and 8 target labels (two 16-byte loads each; the
 // 8 threads of a row share an address) and relaxes 64 cells.  Two positions
 // per step: acc = min3(acc, a0 + b0, a1 + b1).  With both sides' segment
-// minima a tile takes its common segments deepest first and skips a segment
-// whose two minima add up to no less than its largest cell so far.
+// minima (per 8 lanes) a tile takes its common segments deepest first and
+// skips a segment where no thread's 8 x 8 block could fall.
 // Tiles go to the XCDs in contiguous ranges (block b runs on XCD b % 8), so
 // the tiles of one source wave group run side by side on one XCD and share its
 // labels in that XCD's L2.  With `splits` > 1 (few tiles), split sp of a tile
@@ -87,12 +87,6 @@ __device__ __forceinline__ void load8(uint32_t (&v)[8], const uint32_t *p) {
     const uint4 hi = *reinterpret_cast<const uint4 *>(p + 4);
     v[0] = lo.x; v[1] = lo.y; v[2] = lo.z; v[3] = lo.w;
     v[4] = hi.x; v[5] = hi.y; v[6] = hi.z; v[7] = hi.w;
-}
-
-__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
-    return v;
 }
 
 __global__ __launch_bounds__(64) void meet_join_kernel(JoinArgs a) {
@@ -117,8 +111,8 @@ __global__ __launch_bounds__(64) void meet_join_kernel(JoinArgs a) {
 #pragma unroll
         for (int c = 0; c < 8; ++c) acc[r * 8 + c] = la + r < a.A.n_lanes && lb + c < a.B.n_lanes ? INF : 0u;
     const bool prune = a.A.segmin && a.B.segmin;
-    const uint32_t *sma = prune ? a.A.segmin + (size_t)ta * a.nseg : nullptr;
-    const uint32_t *smb = prune ? a.B.segmin + (size_t)tb * a.nseg : nullptr;
+    const uint32_t *sma = prune ? a.A.segmin + (size_t)ta * a.nseg * 8 + i : nullptr;
+    const uint32_t *smb = prune ? a.B.segmin + (size_t)tb * a.nseg * 8 + j : nullptr;
     uint32_t k = 0, seen = 0;  // common segments seen so far, positions visited
     // deepest segments first (a segment's index grows with its depth), so
     // the cells are tight by the time the wide segments near the root come
@@ -130,14 +124,15 @@ __global__ __launch_bounds__(64) void meet_join_kernel(JoinArgs a) {
             const uint32_t s = w * 64 + bit;
             if (k++ % a.splits != sp) continue;
             if (prune) {
-                // every sum through s is at least its minima's: a segment whose
-                // bound reaches the tile's largest cell (or passes thr) lowers
-                // no cell that is kept
-                const uint32_t bound = sat_add(sma[s], smb[s]);
+                // every sum through s for this thread's 8 x 8 cells is at least
+                // its lane groups' minima: where that bound reaches the block's
+                // largest cell (or passes thr) the segment lowers no kept cell;
+                // the wavefront skips it when that holds for every block
+                const uint32_t bound = sat_add(sma[(size_t)s * 8], smb[(size_t)s * 8]);
                 uint32_t top = 0;
 #pragma unroll
                 for (int q = 0; q < 64; ++q) top = max(top, acc[q]);
-                if (bound >= wave_max(top) || bound > a.thr) continue;
+                if (__ballot(bound < top && bound <= a.thr) == 0) continue;
             }
             const uint32_t p0 = a.seg_pos[s], p1 = a.seg_pos[s + 1];
             seen += p1 - p0;

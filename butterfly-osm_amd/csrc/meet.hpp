@@ -28,8 +28,9 @@ struct Side {
     uint32_t stride = 0, kl = 0, n_lanes = 0;
     const unsigned long long *reach = nullptr;
     const uint32_t *index = nullptr;
-    // [wave group][segment]: the least label of the segment's positions over
-    // the wave group's lanes (INF where unreached), or null (no pruning)
+    // [wave group][segment][8]: the least label of the segment's positions over
+    // each 8 consecutive lanes of the wave group (INF where unreached), or null
+    // (no pruning)
     const uint32_t *segmin = nullptr;
 };
 
@@ -44,8 +45,9 @@ hipError_t launch_reach(hipStream_t st, const uint32_t *seed_off, const uint32_t
 // segments both lanes' wave groups reach of A.lab(a, x) + B.lab(b, x)
 // (saturating; INF where no segment is common), INF where above thr.  With
 // both sides' segment minima (nseg segments a wave group), a tile visits the
-// common segments deepest first and skips a segment whose two minima add up
-// to no less than the tile's largest cell so far: no cell can fall there.
+// common segments deepest first and skips a segment where, for each 8 x 8
+// block of its cells, the two lane groups' minima add up to no less than the
+// block's largest cell so far: no cell can fall there.
 // splits > 1 cuts each tile's common segments over that many wavefronts that
 // merge with atomic min: the output cells must hold INF before (launch_fill).
 // visits (when not null) gains the label positions the tiles visited.
