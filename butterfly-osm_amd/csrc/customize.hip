@@ -52,6 +52,7 @@ namespace {
 
 constexpr uint32_t INF = 0xFFFFFFFFu;
 constexpr uint64_t NOT_FOUND = ~0ull;
+constexpr uint32_t NOT_A_NODE = 0xFFFFFFFFu;
 constexpr uint32_t WAVES_PER_WG = 4;
 
 int fail(int code, const std::string &msg) {
@@ -335,6 +336,24 @@ struct Rows {
     std::vector<uint64_t> level_off;  // rows of level L: [level_off[L], level_off[L+1])
 };
 
+// The arcs of node v break a rule find_edge_index relies on: null if they
+// keep them (offsets already checked).
+const char *node_error(const bfg_graph_desc *g, uint32_t v) {
+    const uint32_t n = g->n_nodes;
+    const uint64_t u0 = g->up_offsets[v], u1 = g->up_offsets[v + 1];
+    for (uint64_t i = u0; i < u1; ++i) {
+        const uint32_t t = g->up_targets[i];
+        if (t <= v || t >= n || (i > u0 && t <= g->up_targets[i - 1]))
+            return "must have UP targets ascending, above the node and < n_nodes";
+    }
+    const uint64_t d0 = g->down_offsets[v], d1 = g->down_offsets[v + 1];
+    for (uint64_t i = d0; i < d1; ++i) {
+        const uint32_t t = g->down_targets[i];
+        if (t >= v || (i > d0 && t <= g->down_targets[i - 1])) return "must have DOWN targets ascending and below it";
+    }
+    return nullptr;
+}
+
 int check_topo(const bfg_graph_desc *g) {
     if (!g || !g->up_offsets || !g->down_offsets) return fail(BFG_INVALID_PARAMETER, "null topology");
     const uint32_t n = g->n_nodes;
@@ -342,25 +361,32 @@ int check_topo(const bfg_graph_desc *g) {
     if ((g->up_offsets[n] && (!g->up_targets || !g->up_weights)) ||
         (g->down_offsets[n] && (!g->down_targets || !g->down_weights)))
         return fail(BFG_INVALID_PARAMETER, "null arc arrays");
-    for (uint32_t v = 0; v < n; ++v) {
-        const uint64_t u0 = g->up_offsets[v], u1 = g->up_offsets[v + 1];
-        const uint64_t d0 = g->down_offsets[v], d1 = g->down_offsets[v + 1];
-        if (u1 < u0 || d1 < d0) return fail(BFG_INVALID_PARAMETER, "offsets not ascending at node " + std::to_string(v));
-        for (uint64_t i = u0; i < u1; ++i) {
-            const uint32_t t = g->up_targets[i];
-            if (t <= v || t >= n || (i > u0 && t <= g->up_targets[i - 1]))
-                return fail(BFG_INVALID_PARAMETER, "UP targets of node " + std::to_string(v) +
-                                                       " must be ascending, above the node and < n_nodes");
-        }
-        for (uint64_t i = d0; i < d1; ++i) {
-            const uint32_t t = g->down_targets[i];
-            if (t >= v || (i > d0 && t <= g->down_targets[i - 1]))
-                return fail(BFG_INVALID_PARAMETER, "DOWN targets of node " + std::to_string(v) +
-                                                       " must be ascending and below the node");
-        }
+    // offsets first (ascending offsets keep every node's arcs inside the arrays)
+    for (uint32_t v = 0; v < n; ++v)
+        if (g->up_offsets[v + 1] < g->up_offsets[v] || g->down_offsets[v + 1] < g->down_offsets[v])
+            return fail(BFG_INVALID_PARAMETER, "offsets not ascending at node " + std::to_string(v));
+    if (g->down_offsets[n] >= (1ull << 32)) return fail(BFG_NOT_SUPPORTED, "more than 2^32 - 1 DOWN arcs");
+    // then the arcs, by threads over ranges of nodes; the first failing node is reported
+    const uint64_t m = g->up_offsets[n] + g->down_offsets[n];
+    const uint32_t T = m < (1u << 22) ? 1u : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<uint32_t> first(T, NOT_A_NODE);
+    auto scan = [&](uint32_t t) {
+        const uint32_t v0 = (uint32_t)((uint64_t)n * t / T), v1 = (uint32_t)((uint64_t)n * (t + 1) / T);
+        for (uint32_t v = v0; v < v1; ++v)
+            if (node_error(g, v)) {
+                first[t] = v;
+                return;
+            }
+    };
+    if (T == 1) {
+        scan(0);
+    } else {
+        std::vector<std::thread> th;
+        for (uint32_t t = 0; t < T; ++t) th.emplace_back(scan, t);
+        for (auto &h : th) h.join();
     }
-    if (g->down_offsets[n] >= (1ull << 32))
-        return fail(BFG_NOT_SUPPORTED, "more than 2^32 - 1 DOWN arcs");
+    const uint32_t v = *std::min_element(first.begin(), first.end());
+    if (v != NOT_A_NODE) return fail(BFG_INVALID_PARAMETER, "node " + std::to_string(v) + " " + node_error(g, v));
     return BFG_SUCCESS;
 }
 

@@ -218,6 +218,12 @@ def test_relax_rejects_bad_inputs():
     with pytest.raises(BfgError) as e:  # an UP target below its tail, refused before any device call
         triangle_relax_parallel(bad, [1, 2, 3], [3, 5])
     assert e.value.code == 1
+    # offsets that fall back: refused before any arc is read
+    bad = CchGraph(n=4, up_off=[0, 3, 1, 3, 3], up_tgt=[3, 3, 3], up_w=np.zeros(3),
+                   dn_off=[0, 0, 0, 2, 2], dn_tgt=[0, 1], dn_w=np.zeros(2))
+    with pytest.raises(BfgError) as e:
+        triangle_relax_parallel(bad, [1, 2, 3], [3, 5])
+    assert e.value.code == 1 and "offsets" in str(e.value)
 
 
 # ---- the GPU path --------------------------------------------------------------
