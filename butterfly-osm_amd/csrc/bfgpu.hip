@@ -860,6 +860,9 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_list_kernel(uint32_t count, 
                                                                 const uint32_t *__restrict__ tw,
                                                                 uint32_t *__restrict__ arena, uint32_t n,
                                                                 uint32_t *__restrict__ segmin, uint32_t nseg) {
+    // a chain's latency is the level's critical path while the other side's
+    // pushes fill the SIMDs: its wavefronts issue first
+    __builtin_amdgcn_s_setprio(3);
     const uint32_t i = blockIdx.x * (TILE_THREADS / 64) + wave_id();
     if (i >= count) return;
     const uint32_t s = lu[i];
