@@ -780,7 +780,7 @@ def main():
         "phases_ms_per_step": {"up": up_ms / args.steps, "sweep": sweep_ms / args.steps,
                                "gather": out_ms / args.steps,
                                "batches_wall": engine_ms / args.steps,
-                               "host_staging": ms_per_step - engine_ms / args.steps},
+                               "host_staging": phast_line["ms_per_step"] - engine_ms / args.steps},
         "phast": phast_line,
         "bucket": bucket,
         "host_delivered": host,
