@@ -116,11 +116,12 @@ def committed_traffic(workload, lanes):
     return {"bytes_per_launch": d["hbm_bytes_per_launch"], "source": os.path.relpath(path, REPO)}
 
 
-def committed_join_traffic(S, T):
-    """HBM bytes per launch of the bucket join kernel for an S x T matrix from
-    the committed counter summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy
-    kernel of known size), or None."""
-    path = os.path.join(REPO, "profiles", f"join_bucket_{S}x{T}_traffic.json")
+def committed_join_traffic(S, T, bits=32):
+    """HBM bytes per launch of the bucket join kernel (the u32 one, or the
+    16-bit one for bits=16) for an S x T matrix from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size),
+    or None."""
+    path = os.path.join(REPO, "profiles", f"join{'16' if bits == 16 else ''}_bucket_{S}x{T}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -204,7 +205,8 @@ def bucket_bench(eng, d_src, d_tgt, S, T, d_phast, args, phast_ms, src=None, tgt
         log(f"bucket warmup {i}: up {st.upward_time_ms:.1f} ms join {st.downward_time_ms:.1f} ms")
     sync()
     up = join = kern = 0.0
-    visits = 0
+    visits = redone = 0
+    bits = 32
     t0 = time.perf_counter()
     for i in range(args.steps):
         st = bstep()
@@ -212,6 +214,8 @@ def bucket_bench(eng, d_src, d_tgt, S, T, d_phast, args, phast_ms, src=None, tgt
         join += st.downward_time_ms
         kern += st.join_ms
         visits += st.join_visits
+        redone += st.join_tiles_u32
+        bits = st.join_label_bits
         log(f"bucket step {i}: up {st.upward_time_ms:.1f} ms join {st.downward_time_ms:.1f} ms")
     sync()
     ms = 1000.0 * (time.perf_counter() - t0) / args.steps
@@ -225,25 +229,32 @@ def bucket_bench(eng, d_src, d_tgt, S, T, d_phast, args, phast_ms, src=None, tgt
     update_cost_ewma(r.cell_ns, max(int(ms * 1e6 * share) // (S * T), 1))
     side = r.side(S, T)
     # the join kernel against the VALU issue bound: a visit is one label
-    # position of one 64 x 64 tile, 4096 cell updates of one v_add_u32 (clamp)
-    # each and one v_min3_u32 per two (1.5 wave instructions per 64 updates);
-    # it reads 64 source and 64 target labels (512 bytes) per visit
+    # position of one 64 x 64 tile, 4096 cell updates.  u32: one v_add_u32
+    # (clamp) each and one v_min3_u32 per two (1.5 wave instructions per 64
+    # updates), 64 source and 64 target labels (512 bytes) read per visit.
+    # 16-bit: one v_pk_add_u16 (clamp) and one v_pk_min_u16 per two updates
+    # (1.0 per 64), 256 bytes per visit; the tiles with a cell past the 16-bit
+    # cap are recomputed by the u32 kernel inside the same timed span
     kms, vis = kern / args.steps, visits / args.steps
-    instr = vis * 4096 * 1.5 / 64
+    per64, vbytes = (1.0, 256) if bits == 16 else (1.5, 512)
+    instr = vis * 4096 * per64 / 64
     achieved = instr / (kms / 1000.0) / 1e9 if kms > 0 else 0.0
-    hbm = (vis * 512 + 4 * S * T) / (kms / 1000.0) / 1e9 if kms > 0 else 0.0
+    hbm = (vis * vbytes + 4 * S * T) / (kms / 1000.0) / 1e9 if kms > 0 else 0.0
     # against the HBM roofline the metric names, with the bound that limits the
     # kernel (VALU issue) beside it
-    traffic = committed_join_traffic(S, T)
-    roof = {"bound": "hbm", "kernel": "meet_join_kernel (the bucket join: 64 x 64 min-plus tiles)",
+    traffic = committed_join_traffic(S, T, bits)
+    kname = ("meet_join16_kernel (the bucket join on 16-bit label copies: 64 x 64 min-plus tiles, "
+             "packed u16 cell pairs)" if bits == 16 else "meet_join_kernel (the bucket join: 64 x 64 min-plus tiles)")
+    roof = {"bound": "hbm", "kernel": kname,
             "achieved": hbm, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": hbm / HBM_PEAK_GBPS,
             "traffic": traffic["bytes_per_launch"] if traffic else None,
             "traffic_source": traffic["source"] if traffic else None,
             "launches": 1, "avg_launch_ms": kms, "visits_per_launch": vis, "cell_updates_per_launch": vis * 4096,
-            "algorithmic_bytes_per_launch": vis * 512 + 4 * S * T,
+            "algorithmic_bytes_per_launch": vis * vbytes + 4 * S * T,
+            "label_bits": bits, "tiles_recomputed_u32_per_launch": redone / args.steps,
             "limited_by": "valu issue",
             "valu": {"achieved": achieved, "peak": VALU_PEAK_GINSTR,
-                     "unit": "G wave-instructions/s (algorithmic: 1.5 per 64 cell updates)",
+                     "unit": f"G wave-instructions/s (algorithmic: {per64} per 64 cell updates)",
                      "frac": achieved / VALU_PEAK_GINSTR}}
     host = None
     if args.host_steps > 0 and src is not None:

@@ -35,7 +35,8 @@ class Timings(C.Structure):
                 ("dom_ms", C.c_float), ("dom_launches", C.c_uint32), ("dom_bytes", C.c_uint64),
                 ("gate_seg_units", C.c_uint64), ("gate_seg_open", C.c_uint64),
                 ("gate_bt_units", C.c_uint64), ("gate_bt_open", C.c_uint64),
-                ("join_ms", C.c_float), ("join_visits", C.c_uint64)]
+                ("join_ms", C.c_float), ("join_visits", C.c_uint64),
+                ("join_label_bits", C.c_uint32), ("join_tiles_u32", C.c_uint64)]
 
 
 class RelaxStats(C.Structure):

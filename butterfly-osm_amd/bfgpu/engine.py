@@ -52,6 +52,8 @@ class BatchedPhastStats:
     gate_bt_open: int = 0
     join_ms: float = 0.0     # bucket matrices: the join kernel's time and label positions visited per tile
     join_visits: int = 0
+    join_label_bits: int = 0  # the width the join ran at (16 or 32) and the tiles redone at 32 bits
+    join_tiles_u32: int = 0
 
 
 @dataclasses.dataclass
@@ -128,7 +130,8 @@ class BatchedPhastEngine:
                                  arcs_swept=t.arcs_swept, dom_ms=t.dom_ms, dom_launches=t.dom_launches,
                                  dom_bytes=t.dom_bytes, gate_seg_units=t.gate_seg_units,
                                  gate_seg_open=t.gate_seg_open, gate_bt_units=t.gate_bt_units,
-                                 gate_bt_open=t.gate_bt_open, join_ms=t.join_ms, join_visits=t.join_visits)
+                                 gate_bt_open=t.gate_bt_open, join_ms=t.join_ms, join_visits=t.join_visits,
+                                 join_label_bits=t.join_label_bits, join_tiles_u32=t.join_tiles_u32)
 
     def configure(self, lanes=0, lane_capacity=0):
         _check(self._lib.bfg_ctx_configure(self._h, lanes, lane_capacity))

@@ -859,7 +859,9 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_list_kernel(uint32_t count, 
                                                                 const uint32_t *__restrict__ tab,
                                                                 const uint32_t *__restrict__ tw,
                                                                 uint32_t *__restrict__ arena, uint32_t n,
-                                                                uint32_t *__restrict__ segmin, uint32_t nseg) {
+                                                                uint32_t *__restrict__ segmin, uint32_t nseg,
+                                                                uint16_t *__restrict__ lab16,
+                                                                uint32_t *__restrict__ slot, uint32_t item0) {
     // a chain's latency is the level's critical path while the other side's
     // pushes fill the SIMDs: its wavefronts issue first
     __builtin_amdgcn_s_setprio(3);
@@ -886,6 +888,15 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_list_kernel(uint32_t count, 
     for (int o = 4; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o, 64));
     const uint32_t lane = __lane_id();
     if (lane % 8 == 0) segmin[((size_t)lg[i] * nseg + s) * 8 + lane / 8] = m;
+    if constexpr (KL == 64) {  // the 16-bit copy the join reads (positions past len: INF)
+        if (lab16) {
+            uint16_t *q = lab16 + (size_t)(item0 + i) * 4096 + lane;
+#pragma unroll
+            for (int j = 0; j < 64; ++j)
+                q[j * 64] = (uint16_t)(j < len && acc[j] != INF ? min(acc[j], bfg_meet::CAP16) : 0xFFFFu);
+            if (lane == 0) slot[(size_t)lg[i] * nseg + s] = item0 + i;
+        }
+    }
 }
 
 template <int KL>
@@ -1680,6 +1691,12 @@ struct Work {
     std::vector<uint32_t> h_tu, h_tg, h_pu, h_pg, h_toff, h_poff, h_mark;
     DevBuf<uint32_t> tu, tg, pu, pg;
     DevBuf<uint32_t> segmin;  // upward-only batches: [wave group][segment][8 lanes] least final label
+    // upward-only batches of 64-lane groups: the 16-bit label of each reached
+    // (segment, wave group) pair (0xFFFF for INF, else min(label, CAP16)),
+    // [list item][64 positions][64 lanes], and the item of each pair,
+    // [wave group][segment] (NONE: not reached)
+    DevBuf<uint16_t> lab16;
+    DevBuf<uint32_t> slot, redo;
     std::vector<uint32_t> h_pairs;
     // host staging of a call (kept, so a fixed Work reuses its capacity)
     std::vector<uint32_t> h_bt, h_pending, h_order, h_soff, h_srank, h_scost, h_lim, h_rows, h_sdepth, h_spre,
@@ -2209,8 +2226,17 @@ int launch_up_lists(bfg_ctx *c, Work *w, uint32_t nl, PhaseTimes *pt) {
     const uint64_t nmin = (uint64_t)(nl + 63) / 64 * nseg * 8;
     ENSURE(w, segmin, std::max<size_t>(nmin, 1), "segment minima");
     HIP_TRY(bfg_meet::launch_fill(s, w->segmin.p, nmin, INF));
-    HIP_TRY(hipEventRecord(w->ev[0], s));
     const uint32_t total = (uint32_t)w->h_tu.size();
+    uint16_t *l16 = nullptr;
+    uint32_t *slot = nullptr;
+    if constexpr (KL == 64) {
+        ENSURE(w, lab16, std::max<size_t>((size_t)total * 4096, 1), "16-bit labels");
+        ENSURE(w, slot, std::max<size_t>(nmin / 8, 1), "label slots");
+        HIP_TRY(bfg_meet::launch_fill(s, w->slot.p, nmin / 8, NONE));
+        l16 = w->lab16.p;
+        slot = w->slot.p;
+    }
+    HIP_TRY(hipEventRecord(w->ev[0], s));
     if (total) {
         hipLaunchKernelGGL(fill_list_kernel<KL>, dim3((total + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, total,
                            w->tu.p, w->tg.p, c->seg_pos.p, w->arena.p, n);
@@ -2226,7 +2252,7 @@ int launch_up_lists(bfg_ctx *c, Work *w, uint32_t nl, PhaseTimes *pt) {
         if (tc) {
             hipLaunchKernelGGL(tri_list_kernel<KL>, dim3((tc + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, tc,
                                w->tu.p + t0, w->tg.p + t0, c->seg_pos.p, c->tb_up.p, c->tb_w.p, w->arena.p, n,
-                               w->segmin.p, nseg);
+                               w->segmin.p, nseg, l16, slot, t0);
             ++nlaunch;
         }
         const uint32_t p0 = w->h_poff[l], pc = w->h_poff[l + 1] - p0;
@@ -3804,7 +3830,9 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
     std::vector<uint64_t> tree_tgts;  // (bottom tree root << 32 | target lane), ascending
     float join_ms = 0, kernel_ms = 0;
     uint32_t joins = 0;
-    uint64_t pairs = 0, visits = 0;
+    uint64_t pairs = 0, visits = 0, redone = 0;
+    bool bits16 = true;
+    const char *u32_only = std::getenv("BFG_JOIN_U32");  // override: the u32 join alone
     // Lane capacities: the targets' arena first, the sources' from what is
     // left.  When both sides fit one batch, their upward phases run at once
     // (the target side on a thread of its own) and the join splits its tiles.
@@ -3844,13 +3872,32 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
         HIP_TRY(bfg_meet::launch_reach(wf->stream, wf->seed_off.p, wf->seed_pre.p, snl, c->pos_pre.p, c->pos_sub.p,
                                        c->seg_pos.p, nseg, words, wf->reach.p));
         if (splits > 1) HIP_TRY(bfg_meet::launch_fill(wf->stream, d_out, (uint64_t)S * T, INF));
-        ENSURE(wf, visits, 1, "join counter");
-        HIP_TRY(bfg_meet::launch_fill(wf->stream, (uint32_t *)wf->visits.p, 2, 0));
-        const bfg_meet::Side A{wf->arena.p, L.n_top, c->lanes, snl, wf->reach.p, wf->row_map.p, wf->segmin.p};
-        const bfg_meet::Side B{wb->arena.p, L.n_top, r->lanes, tnl, wb->reach.p, wb->row_map.p, wb->segmin.p};
+        ENSURE(wf, visits, 2, "join counters");
+        HIP_TRY(bfg_meet::launch_fill(wf->stream, (uint32_t *)wf->visits.p, 4, 0));
+        bfg_meet::Side A{wf->arena.p, L.n_top, c->lanes, snl, wf->reach.p, wf->row_map.p, wf->segmin.p};
+        bfg_meet::Side B{wb->arena.p, L.n_top, r->lanes, tnl, wb->reach.p, wb->row_map.p, wb->segmin.p};
+        // one wavefront a tile on both sides' 16-bit labels, tiles with a cell
+        // past 16 bits again on the u32 labels
+        const bool use16 = splits == 1 && c->lanes == 64 && r->lanes == 64 && !(u32_only && u32_only[0] == '1');
+        if (use16) {
+            A.lab16 = wf->lab16.p;
+            A.slot = wf->slot.p;
+            B.lab16 = wb->lab16.p;
+            B.slot = wb->slot.p;
+            const size_t tiles = (size_t)((snl + 63) / 64) * ((tnl + 63) / 64);
+            ENSURE(wf, redo, std::max<size_t>(tiles, 1), "join tiles");
+            HIP_TRY(bfg_meet::launch_fill(wf->stream, wf->redo.p, tiles, 0));
+        }
         HIP_TRY(hipEventRecord(wf->ev[1], wf->stream));
-        HIP_TRY(bfg_meet::launch_join(wf->stream, A, B, c->seg_pos.p, words, nseg, thr, splits, d_out, T,
-                                      wf->visits.p));
+        if (use16) {
+            HIP_TRY(bfg_meet::launch_join16(wf->stream, A, B, c->seg_pos.p, words, nseg, thr, d_out, T, wf->visits.p,
+                                            wf->redo.p));
+            HIP_TRY(bfg_meet::launch_join(wf->stream, A, B, c->seg_pos.p, words, nseg, thr, 1, d_out, T, nullptr,
+                                          wf->redo.p));
+        } else {
+            HIP_TRY(bfg_meet::launch_join(wf->stream, A, B, c->seg_pos.p, words, nseg, thr, splits, d_out, T,
+                                          wf->visits.p));
+        }
         HIP_TRY(hipEventRecord(wf->ev[2], wf->stream));
         // sources and targets in one bottom tree also meet below the top region
         std::vector<uint32_t> &pr = wf->h_pairs;
@@ -3884,9 +3931,11 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
         HIP_TRY(hipEventElapsedTime(&kms, wf->ev[1], wf->ev[2]));
         join_ms += ms;
         kernel_ms += kms;
-        unsigned long long cnt = 0;
-        HIP_TRY(hipMemcpy(&cnt, wf->visits.p, sizeof(cnt), hipMemcpyDeviceToHost));
-        visits += cnt;
+        unsigned long long cnt[2] = {0, 0};
+        HIP_TRY(hipMemcpy(cnt, wf->visits.p, sizeof(cnt), hipMemcpyDeviceToHost));
+        visits += cnt[0];
+        redone += cnt[1];
+        bits16 &= use16;
         ++joins;
         return BFG_SUCCESS;
     };
@@ -3950,6 +3999,8 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
     tm.nodes_swept = pairs;  // pairs that met inside a bottom tree
     tm.join_ms = kernel_ms;
     tm.join_visits = visits;
+    tm.join_label_bits = joins && bits16 ? 16 : 32;
+    tm.join_tiles_u32 = redone;
     tm.total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     std::lock_guard<std::mutex> g(c->last_mu);
     c->last = tm;

@@ -80,6 +80,8 @@ struct JoinArgs {
     uint32_t *out;
     uint64_t ld;
     unsigned long long *visits;
+    const uint32_t *only;  // tiles to compute (null: all)
+    uint32_t *redo;        // 16-bit join: tiles the u32 join recomputes
 };
 
 __device__ __forceinline__ void load8(uint32_t (&v)[8], const uint32_t *p) {
@@ -95,6 +97,7 @@ __global__ __launch_bounds__(64) void meet_join_kernel(JoinArgs a) {
     const uint32_t u = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
     if (u >= tiles) return;
     const uint32_t t = u / a.splits, sp = u % a.splits;
+    if (a.only && a.only[t] == 0) return;  // wave-uniform; this kernel has no barrier
     const uint32_t ta = t / a.wgb, tb = t % a.wgb;
     const uint32_t i = threadIdx.x >> 3, j = threadIdx.x & 7;
     const uint32_t la = ta * 64 + 8 * i, lb = tb * 64 + 8 * j;
@@ -170,6 +173,117 @@ __global__ __launch_bounds__(64) void meet_join_kernel(JoinArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// join16: meet_join_kernel on the 16-bit label copies, one wavefront per tile
+// (splits 1).  Thread (i, j) holds its 8 x 8 cells as 32 packed pairs:
+// acc[4r + h] = cells (source lane 8i + r; target lanes 8j + 2h, 8j + 2h + 1).
+// Per position one 16-byte load brings the thread's 8 source labels and one
+// its 8 target labels (already paired as the cells are); a source label goes
+// to both halves by an operand swizzle, so a pair of cells costs one
+// v_pk_add_u16 (clamp) and one v_pk_min_u16.  A label is 0xFFFF for INF and
+// min(label, CAP16) else, so the sum of two finite labels never saturates
+// (at most 0xFFFE) and a sum through INF is 0xFFFF: a cell of 0xFFFF is INF,
+// a cell below CAP16 is the u32 join's (a sum through a capped label is at
+// least CAP16), and a tile with a kept cell in [CAP16, 0xFFFE] while thr
+// admits such a value is flagged for the u32 join to redo.
+// ---------------------------------------------------------------------------
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
+
+__global__ __launch_bounds__(64) void meet_join16_kernel(JoinArgs a) {
+    const uint32_t tiles = a.wga * a.wgb;
+    const uint32_t per_xcd = (tiles + XCDS - 1) / XCDS;
+    const uint32_t t = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
+    if (t >= tiles) return;  // wave-uniform; this kernel has no barrier
+    const uint32_t ta = t / a.wgb, tb = t % a.wgb;
+    const uint32_t i = threadIdx.x >> 3, j = threadIdx.x & 7;
+    const uint32_t la = ta * 64 + 8 * i, lb = tb * 64 + 8 * j;
+    const uint32_t nA = a.A.n_lanes, nB = a.B.n_lanes;
+    const unsigned long long *ra = a.A.reach + (size_t)ta * a.words;
+    const unsigned long long *rb = a.B.reach + (size_t)tb * a.words;
+    const uint32_t *qa_of = a.A.slot + (size_t)ta * a.nseg, *qb_of = a.B.slot + (size_t)tb * a.nseg;
+    // cells of lanes past either side's count start at 0, as in meet_join_kernel
+    u16x2 acc[32];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const bool ok = la + r < nA;
+            acc[4 * r + h] = (u16x2){(unsigned short)(ok && lb + 2 * h < nB ? 0xFFFF : 0),
+                                     (unsigned short)(ok && lb + 2 * h + 1 < nB ? 0xFFFF : 0)};
+        }
+    const bool prune = a.A.segmin && a.B.segmin;
+    const uint32_t *sma = prune ? a.A.segmin + (size_t)ta * a.nseg * 8 + i : nullptr;
+    const uint32_t *smb = prune ? a.B.segmin + (size_t)tb * a.nseg * 8 + j : nullptr;
+    uint32_t seen = 0;
+    for (uint32_t w = a.words; w-- > 0;) {
+        unsigned long long m = ra[w] & rb[w];
+        while (m) {
+            const uint32_t bit = 63u - (uint32_t)__builtin_clzll(m);
+            m &= ~(1ull << bit);
+            const uint32_t s = w * 64 + bit;
+            if (prune) {  // as meet_join_kernel, a cell of 0xFFFF being INF
+                const uint32_t bound = sat_add(sma[(size_t)s * 8], smb[(size_t)s * 8]);
+                u16x2 hi2 = acc[0];
+#pragma unroll
+                for (int q = 1; q < 32; ++q) hi2 = __builtin_elementwise_max(hi2, acc[q]);
+                const uint32_t hi = max((uint32_t)hi2.x, (uint32_t)hi2.y);
+                const uint32_t top = hi == 0xFFFFu ? INF : hi;
+                if (__ballot(bound < top && bound <= a.thr) == 0) continue;
+            }
+            const uint32_t qa = qa_of[s], qb = qb_of[s];
+            if (qa == NONE || qb == NONE) continue;  // not reached by one of the wave groups: INF
+            const uint32_t len = a.seg_pos[s + 1] - a.seg_pos[s];
+            seen += len;
+            const uint16_t *pa = a.A.lab16 + (size_t)qa * 4096 + 8 * i;
+            const uint16_t *pb = a.B.lab16 + (size_t)qb * 4096 + 8 * j;
+            for (uint32_t x = 0; x < len; x += 2) {
+                const uint32_t y = min(x + 1, len - 1);  // an odd last position pairs with itself
+                const uint4 a0 = *reinterpret_cast<const uint4 *>(pa + (size_t)x * 64);
+                const uint4 b0 = *reinterpret_cast<const uint4 *>(pb + (size_t)x * 64);
+                const uint4 a1 = *reinterpret_cast<const uint4 *>(pa + (size_t)y * 64);
+                const uint4 b1 = *reinterpret_cast<const uint4 *>(pb + (size_t)y * 64);
+                const uint32_t A0[4] = {a0.x, a0.y, a0.z, a0.w}, A1[4] = {a1.x, a1.y, a1.z, a1.w};
+                const uint32_t B0[4] = {b0.x, b0.y, b0.z, b0.w}, B1[4] = {b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const u16x2 p = as_u16x2(A0[r / 2]), q = as_u16x2(A1[r / 2]);
+                    const u16x2 pr = (r & 1) ? p.yy : p.xx, qr = (r & 1) ? q.yy : q.xx;
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        const u16x2 s0 = __builtin_elementwise_add_sat(pr, as_u16x2(B0[h]));
+                        const u16x2 s1 = __builtin_elementwise_add_sat(qr, as_u16x2(B1[h]));
+                        acc[4 * r + h] = __builtin_elementwise_min(acc[4 * r + h], __builtin_elementwise_min(s0, s1));
+                    }
+                }
+            }
+        }
+    }
+    if (a.visits && threadIdx.x == 0 && seen) atomicAdd(a.visits, (unsigned long long)seen);
+    uint32_t col[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) col[c] = lb + c < nB ? a.B.index[lb + c] : 0u;
+    bool redo = false;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        if (la + r >= nA) break;
+        uint32_t *o = a.out + (uint64_t)a.A.index[la + r] * a.ld;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (lb + c >= nB) continue;
+            const uint32_t v16 = (c & 1) ? acc[4 * r + c / 2].y : acc[4 * r + c / 2].x;
+            const bool capped = v16 >= CAP16 && v16 != 0xFFFFu;
+            redo |= capped && a.thr >= CAP16;
+            o[col[c]] = capped || v16 == 0xFFFFu || v16 > a.thr ? INF : v16;
+        }
+    }
+    if (__ballot(redo) != 0 && threadIdx.x == 0) {
+        a.redo[t] = 1;
+        if (a.visits) atomicAdd(a.visits + 1, 1ull);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // bottom pairs: one thread per pair of a source and a target in one bottom
 // tree.  Both walk up to their lowest common ancestor; every common ancestor
 // from there to the tree's root (the last bottom node) is a meeting node whose
@@ -227,15 +341,31 @@ hipError_t launch_reach(hipStream_t st, const uint32_t *seed_off, const uint32_t
 
 hipError_t launch_join(hipStream_t st, const Side &A, const Side &B, const uint32_t *seg_pos, uint32_t words,
                        uint32_t nseg, uint32_t thr, uint32_t splits, uint32_t *out, uint64_t ld,
-                       unsigned long long *visits) {
+                       unsigned long long *visits, const uint32_t *only) {
     if ((A.segmin || B.segmin) && (uint64_t)words * 64 < nseg) return hipErrorInvalidValue;
     JoinArgs a{A, B, seg_pos, words, nseg, (A.n_lanes + 63) / 64, (B.n_lanes + 63) / 64, thr, splits ? splits : 1,
-               out, ld, visits};
+               out, ld, visits, only, nullptr};
     const uint64_t tiles = (uint64_t)a.wga * a.wgb * a.splits;
     if (!tiles) return hipSuccess;
     const uint64_t blocks = (tiles + XCDS - 1) / XCDS * XCDS;
     if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
     hipLaunchKernelGGL(meet_join_kernel, dim3((uint32_t)blocks), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_join16(hipStream_t st, const Side &A, const Side &B, const uint32_t *seg_pos, uint32_t words,
+                         uint32_t nseg, uint32_t thr, uint32_t *out, uint64_t ld, unsigned long long *visits,
+                         uint32_t *redo) {
+    if (!A.lab16 || !B.lab16 || !A.slot || !B.slot || A.kl != 64 || B.kl != 64 || !redo)
+        return hipErrorInvalidValue;
+    if ((A.segmin || B.segmin) && (uint64_t)words * 64 < nseg) return hipErrorInvalidValue;
+    JoinArgs a{A, B, seg_pos, words, nseg, (A.n_lanes + 63) / 64, (B.n_lanes + 63) / 64, thr, 1,
+               out, ld, visits, nullptr, redo};
+    const uint64_t tiles = (uint64_t)a.wga * a.wgb;
+    if (!tiles) return hipSuccess;
+    const uint64_t blocks = (tiles + XCDS - 1) / XCDS * XCDS;
+    if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(meet_join16_kernel, dim3((uint32_t)blocks), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 

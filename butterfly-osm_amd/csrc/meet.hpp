@@ -32,6 +32,13 @@ struct Side {
     // each 8 consecutive lanes of the wave group (INF where unreached), or null
     // (no pruning)
     const uint32_t *segmin = nullptr;
+    // 16-bit copies of the final labels of the reached (segment, wave group)
+    // pairs (0xFFFF for INF, else min(label, CAP16); 64 positions x 64 lanes
+    // a pair at lab16[4096 slot ..]) and slot[wave group * nseg + segment]
+    // (NONE where the pair was not reached), or null (no 16-bit join); 64
+    // lanes a group
+    const uint16_t *lab16 = nullptr;
+    const uint32_t *slot = nullptr;
 };
 
 // reach[g * words + s / 64] bit s % 64: wave group g (lanes 64g ..) has a
@@ -51,9 +58,25 @@ hipError_t launch_reach(hipStream_t st, const uint32_t *seed_off, const uint32_t
 // splits > 1 cuts each tile's common segments over that many wavefronts that
 // merge with atomic min: the output cells must hold INF before (launch_fill).
 // visits (when not null) gains the label positions the tiles visited.
+// only (when not null): tile t = (source wave group) * (target wave groups)
+// + (target wave group) is computed only where only[t] != 0.
 hipError_t launch_join(hipStream_t st, const Side &A, const Side &B, const uint32_t *seg_pos, uint32_t words,
                        uint32_t nseg, uint32_t thr, uint32_t splits, uint32_t *out, uint64_t ld,
-                       unsigned long long *visits);
+                       unsigned long long *visits, const uint32_t *only = nullptr);
+
+// The join on both sides' 16-bit labels (lab16 / slot set, kl 64, one
+// wavefront a tile): packed saturating u16 adds and mins, two cells an
+// instruction.  Two finite labels sum to at most 0xFFFE and a sum through INF
+// is 0xFFFF, so a cell of 0xFFFF is INF and a cell below CAP16 is exact (a
+// sum through a capped label is at least CAP16); a tile with a kept cell in
+// [CAP16, 0xFFFE] while thr admits such a value is flagged in redo[t] (zero
+// beforehand) and counted in visits[1], and launch_join with only = redo
+// recomputes it from the u32 labels.  visits[0] gains the positions visited,
+// as launch_join.
+constexpr uint32_t CAP16 = 0x7FFF;
+hipError_t launch_join16(hipStream_t st, const Side &A, const Side &B, const uint32_t *seg_pos, uint32_t words,
+                         uint32_t nseg, uint32_t thr, uint32_t *out, uint64_t ld, unsigned long long *visits,
+                         uint32_t *redo);
 
 // p[0 .. count) = value (a kernel, so it may sit in a captured graph).
 hipError_t launch_fill(hipStream_t st, uint32_t *p, uint64_t count, uint32_t value);

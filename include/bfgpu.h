@@ -100,6 +100,12 @@ typedef struct bfg_timings {
      * (source, target) tiles: each visit is 4096 min-plus cell updates */
     float join_ms;
     uint64_t join_visits;
+    /* bfg_table_bucket: the label width the join ran at (16: packed u16
+     * cells, labels capped at 0x7FFF, for 64-lane groups joined one wavefront
+     * a tile; 32 otherwise) and the tiles with a cell at or past the cap that
+     * the u32 join recomputed */
+    uint32_t join_label_bits;
+    uint64_t join_tiles_u32;
 } bfg_timings;
 
 /* BatchedPhastEngine::new (batched_phast.rs:89): copies the CCH to `device`,

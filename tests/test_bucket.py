@@ -9,8 +9,10 @@ Bit-exact against the oracle's bucket restatement and its PHAST matrix
 the hand graphs, the broom closed form, the 224 x 224 grid with diagonal and
 same-bottom-tree pairs (sources and targets that meet below the top region),
 thresholds, lane widths 16/32/64, several source and target batches, the
-device entry point, the chord grid, and label widths 16 and 32 (the packed
-16-bit upward pushes and their u32 fallback at x40 weights)."""
+device entry point, the chord grid, label widths 16 and 32 (the packed
+16-bit upward pushes and their u32 fallback at x40 weights), and the 16-bit
+join of matrices of more than 2048 tiles against the u32 join, with its
+tiles past 16 bits recomputed at 32 bits."""
 import sys
 
 import numpy as np
@@ -176,3 +178,43 @@ def test_bucket_label_widths(engine_cls, grid_config1, scale, bits):
         assert np.array_equal(m, exp)
         mb, _ = e.table_bucket_parallel_bounded(s, t, 2000 * scale)
         assert np.array_equal(mb, mask(exp, 2000 * scale))
+
+
+@pytest.mark.parametrize("scale", [1, 40])
+def test_bucket_join16(engine_cls, grid_config1, scale, monkeypatch):
+    """Matrices of more than 2048 tiles join one wavefront a tile on 16-bit
+    label copies (packed u16 cells, labels capped at 0x7FFF): equal to the
+    u32 join (BFG_JOIN_U32=1), to the PHAST matrix and, on a sample of rows,
+    to the oracle.  At x40 weights most cells pass the cap and their tiles
+    are recomputed at 32 bits; bounds just below, at and above the cap and
+    0xFFFF."""
+    g = grid_config1 if scale == 1 else scaled(grid_config1, scale)
+    o = Oracle(g)
+    rng = np.random.default_rng(11 + scale)
+    s = rng.integers(0, g.n, 2900).astype(np.uint32)
+    t = rng.integers(0, g.n, 2900).astype(np.uint32)
+    ns, nt = near_pairs(g, rng, 60)
+    s, t = np.concatenate([s, ns]), np.concatenate([t, nt])
+    say("bucket join16 weights x", scale, s.size, "x", t.size)
+    with engine_cls(g) as e:
+        e.pair_reverse()
+        m16, st = e.table_bucket_parallel(s, t)
+        assert st.join_label_bits == 16
+        wide = int(np.count_nonzero((m16 >= 0x7FFF) & (m16 != INF)))  # past the 16-bit cap
+        assert (wide > 0) == (scale > 1)
+        if wide:
+            assert st.join_tiles_u32 > 0
+        else:  # none but tiles whose join cell a pair meeting in a bottom tree lowers
+            assert st.join_tiles_u32 < 50
+        monkeypatch.setenv("BFG_JOIN_U32", "1")
+        m32, st32 = e.table_bucket_parallel(s, t)
+        assert st32.join_label_bits == 32 and st32.join_tiles_u32 == 0
+        monkeypatch.delenv("BFG_JOIN_U32")
+        assert np.array_equal(m16, m32)
+        ph, _ = e.compute_matrix_flat(s, t)
+        assert np.array_equal(m16, ph)
+        exp = o.compute_matrix_flat_mt(s[:8], t)
+        assert np.array_equal(m16.reshape(s.size, t.size)[:8].reshape(-1), exp)
+        for thr in (1800 * scale, 0x7FFE, 0x7FFF, 0xFFFF, 100_000):
+            mb, _ = e.table_bucket_parallel_bounded(s, t, thr)
+            assert np.array_equal(mb, mask(ph, thr))
