@@ -190,6 +190,11 @@ typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ u16x2 as_u16x2(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
 
+// the 8 16-bit labels of a thread's lanes at position x of a pair's copy
+__device__ __forceinline__ uint4 ld16(const uint16_t *p, uint32_t x) {
+    return *reinterpret_cast<const uint4 *>(p + (size_t)x * 64);
+}
+
 __global__ __launch_bounds__(64) void meet_join16_kernel(JoinArgs a) {
     const uint32_t tiles = a.wga * a.wgb;
     const uint32_t per_xcd = (tiles + XCDS - 1) / XCDS;
@@ -237,12 +242,17 @@ __global__ __launch_bounds__(64) void meet_join16_kernel(JoinArgs a) {
             seen += len;
             const uint16_t *pa = a.A.lab16 + (size_t)qa * 4096 + 8 * i;
             const uint16_t *pb = a.B.lab16 + (size_t)qb * 4096 + 8 * j;
+            // two positions a step (an odd last one pairs with itself), the
+            // next step's labels loaded before this step's updates and waited
+            // for after them (a read past the last position re-reads it)
+            uint4 na0 = ld16(pa, 0), nb0 = ld16(pb, 0);
+            uint4 na1 = ld16(pa, min(1u, len - 1)), nb1 = ld16(pb, min(1u, len - 1));
             for (uint32_t x = 0; x < len; x += 2) {
-                const uint32_t y = min(x + 1, len - 1);  // an odd last position pairs with itself
-                const uint4 a0 = *reinterpret_cast<const uint4 *>(pa + (size_t)x * 64);
-                const uint4 b0 = *reinterpret_cast<const uint4 *>(pb + (size_t)x * 64);
-                const uint4 a1 = *reinterpret_cast<const uint4 *>(pa + (size_t)y * 64);
-                const uint4 b1 = *reinterpret_cast<const uint4 *>(pb + (size_t)y * 64);
+                const uint4 a0 = na0, b0 = nb0, a1 = na1, b1 = nb1;
+                na0 = ld16(pa, min(x + 2, len - 1));
+                nb0 = ld16(pb, min(x + 2, len - 1));
+                na1 = ld16(pa, min(x + 3, len - 1));
+                nb1 = ld16(pb, min(x + 3, len - 1));
                 const uint32_t A0[4] = {a0.x, a0.y, a0.z, a0.w}, A1[4] = {a1.x, a1.y, a1.z, a1.w};
                 const uint32_t B0[4] = {b0.x, b0.y, b0.z, b0.w}, B1[4] = {b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
@@ -256,6 +266,10 @@ __global__ __launch_bounds__(64) void meet_join16_kernel(JoinArgs a) {
                         acc[4 * r + h] = __builtin_elementwise_min(acc[4 * r + h], __builtin_elementwise_min(s0, s1));
                     }
                 }
+                asm volatile("" : "+v"(na0.x), "+v"(na0.y), "+v"(na0.z), "+v"(na0.w), "+v"(nb0.x), "+v"(nb0.y),
+                             "+v"(nb0.z), "+v"(nb0.w));
+                asm volatile("" : "+v"(na1.x), "+v"(na1.y), "+v"(na1.z), "+v"(na1.w), "+v"(nb1.x), "+v"(nb1.y),
+                             "+v"(nb1.z), "+v"(nb1.w));
             }
         }
     }
