@@ -899,17 +899,109 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_list_kernel(uint32_t count, 
     }
 }
 
+// Upward pushes on column pairs.  A label enters as 16 bits, 0xFFFF for INF
+// and min(label, PUSH_CAP) else, and a weight the same way (the packed copy
+// holds columns 2q, 2q + 1 of a row in dword q).  Two finite values sum to at
+// most 0xFFFE and a sum through INF saturates to 0xFFFF, so a column of
+// 0xFFFF is INF and a column below PUSH_CAP is the u32 result (a sum through
+// a capped value is at least PUSH_CAP).
+constexpr uint32_t PUSH_CAP = 0x7FFF;
+
+__device__ __forceinline__ uint32_t lab16_of(uint32_t x) {
+    return x >= PUSH_CAP ? (x == INF ? 0xFFFFu : PUSH_CAP) : x;
+}
+
+// rows_w with acc[q] = columns 2q (low) and 2q + 1 (high): per two rows and
+// column pair, two v_pk_add_u16 (clamp; the row's label broadcast by
+// op_sel) and two v_pk_min_u16, against three u32 instructions per two
+// updates.  Rows, keys and label prefetch as rows_w; wr16 holds 4k dwords a
+// row.
+template <int KL, int N>
+__device__ __forceinline__ void rows_w16c(u16x2 (&acc)[N / 2], uint32_t r0, uint32_t r1, const uint32_t *key,
+                                          const cu32 *wr16, uint32_t k, const uint32_t *lab) {
+    const uint32_t wl = __lane_id(), W2 = 4 * k;
+    for (uint32_t c0 = r0; c0 < r1; c0 += 64) {
+        const uint32_t cl = min(64u, r1 - c0) - 1;
+        const uint32_t kv = key[c0 + min(wl, cl)];
+        uint32_t n0 = lab[(size_t)rl(kv, 0) * KL];
+        uint32_t n1 = lab[(size_t)rl(kv, min(1u, cl)) * KL];
+        const cu32 *W0 = wr16 + (size_t)(c0 - r0) * W2;
+        for (uint32_t j = 0; j <= cl; j += 2) {
+            const u16x2 a0 = as_u16x2(lab16_of(n0)), a1 = as_u16x2(lab16_of(n1));
+            const cu32 *W1 = j < cl ? W0 + W2 : W0;
+            n0 = lab[(size_t)rl(kv, min(j + 2, cl)) * KL];
+            n1 = lab[(size_t)rl(kv, min(j + 3, cl)) * KL];
+#pragma unroll
+            for (int o = 0; o < N / 8; ++o) {
+                if (o > 0 && (uint32_t)o >= k) break;
+#pragma unroll
+                for (int q = 4 * o; q < 4 * o + 4; ++q) {
+                    const u16x2 s0 = __builtin_elementwise_add_sat(a0.xx, as_u16x2(W0[q]));
+                    const u16x2 s1 = __builtin_elementwise_add_sat(a1.xx, as_u16x2(W1[q]));
+                    acc[q] = __builtin_elementwise_min(acc[q], __builtin_elementwise_min(s0, s1));
+                }
+            }
+            asm volatile("" : "+v"(n0), "+v"(n1));
+            W0 += 2 * W2;
+        }
+    }
+}
+
+// The u32 push of a unit whose 16-bit columns passed the cap, compact rather
+// than fast (labels re-read per column): merges as push_unit.
+template <int KL>
+__device__ __noinline__ void push_unit_u32(uint32_t ra, uint32_t rb, const uint32_t *K, const cu32 *wr, uint32_t W,
+                                           const cu32 *out, uint32_t *lab) {
+    for (uint32_t j = 0; j < W; ++j) {
+        const uint32_t t = out[j];
+        if (t == NONE) continue;
+        uint32_t m = INF;
+        for (uint32_t r = ra; r < rb; ++r) m = min(m, sat_add(lab[(size_t)K[r] * KL], wr[(size_t)(r - ra) * W + j]));
+        if (m != INF) atomicMin(lab + (size_t)t * KL, m);
+    }
+}
+
+// push_unit on column pairs; a unit with a column in [PUSH_CAP, 0xFFFE] on
+// any lane is pushed again in u32 (the exact columns already merged are
+// true sums, so merging them first changes nothing).
+template <int KL, int W>
+__device__ __forceinline__ void push_unit16(uint32_t ra, uint32_t rb, const uint32_t *K, const cu32 *wr16,
+                                            const cu32 *wr, const cu32 *out, uint32_t *lab) {
+    u16x2 acc[W / 2];
+#pragma unroll
+    for (int q = 0; q < W / 2; ++q) acc[q] = (u16x2){0xFFFF, 0xFFFF};
+    rows_w16c<KL, W>(acc, ra, rb, K, wr16, W / 8, lab);
+    bool wide = false;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const uint32_t v = (j & 1) ? acc[j / 2].y : acc[j / 2].x;
+        wide |= v >= PUSH_CAP && v != 0xFFFFu;
+        if (out[j] != NONE && v < PUSH_CAP) atomicMin(lab + (size_t)out[j] * KL, v);
+    }
+    if (__ballot(wide) != 0) push_unit_u32<KL>(ra, rb, K, wr, W, out, lab);
+}
+
+// wts16 (when not null): the packed 16-bit copy of wts, a row's columns in
+// pairs from woff[u] / 2 on
 template <int KL>
 __global__ __launch_bounds__(TILE_THREADS) void push_list_kernel(
     uint32_t count, const uint32_t *__restrict__ lu, const uint32_t *__restrict__ lg, const uint32_t *__restrict__ r0,
     const uint32_t *__restrict__ wid, const uint32_t *__restrict__ woff, const uint32_t *__restrict__ tgt,
-    const uint32_t *__restrict__ key, const uint32_t *__restrict__ wts, uint32_t *__restrict__ arena, uint32_t n) {
+    const uint32_t *__restrict__ key, const uint32_t *__restrict__ wts, const uint32_t *__restrict__ wts16,
+    uint32_t *__restrict__ arena, uint32_t n) {
     const uint32_t i = blockIdx.x * (TILE_THREADS / 64) + wave_id();
     if (i >= count) return;
     const uint32_t u = lu[i];
     uint32_t *lab = lane_labels<KL>(arena, n, lg[i], nullptr);
     const uint32_t ra = r0[u], rb = r0[u + 1], W = wid[u];
     const cu32 *wr = as_const(wts) + woff[u], *out = as_const(tgt) + (size_t)u * 64;
+    if (wts16) {
+        const cu32 *wr16 = as_const(wts16) + woff[u] / 2;
+        if (W == 16) push_unit16<KL, 16>(ra, rb, key, wr16, wr, out, lab);
+        else if (W == 32) push_unit16<KL, 32>(ra, rb, key, wr16, wr, out, lab);
+        else push_unit16<KL, 64>(ra, rb, key, wr16, wr, out, lab);
+        return;
+    }
     if (W == 16) push_unit<KL, 16>(ra, rb, key, wr, out, lab);
     else if (W == 32) push_unit<KL, 32>(ra, rb, key, wr, out, lab);
     else push_unit<KL, 64>(ra, rb, key, wr, out, lab);
@@ -1737,6 +1829,9 @@ struct bfg_ctx {
     DevBuf<uint32_t> up_off, up_tgt, up_w;
     DevBuf<uint32_t> xr_off, xr_run, xw_off, xr_src, xr_w, tb_dn, tb_up, tb_w, pt_blk, pt_tgt, pr_off, pr_wid,
         pw_off, pr_x, pr_w;
+    // pr_w as 16 bits in column pairs (0xFFFF for INF, else min(w, 0x7FFF)):
+    // the upward-list pushes
+    DevBuf<uint32_t> pr_w16;
     DevBuf<uint32_t> pk_seg;
     // gate tables: the least weight of each DOWN row (xr order) and of each
     // bottom chunk source's arcs (cs order); the bottom trees (maximal
@@ -2247,6 +2342,8 @@ int launch_up_lists(bfg_ctx *c, Work *w, uint32_t nl, PhaseTimes *pt) {
                        c->pos_parent.p, c->pos_depth.p, c->up_off.p, c->up_tgt.p, c->up_w.p, L.n_top, w->pathbuf.p,
                        D1, w->arena.p, n);
     ++nlaunch;
+    const char *p32 = std::getenv("BFG_PUSH_U32");  // override: the u32 pushes alone
+    const bool push16 = c->pr_w16.n > 0 && !(p32 && p32[0] == '1');
     for (int l = (int)nsl - 1; l >= 0; --l) {
         const uint32_t t0 = w->h_toff[l], tc = w->h_toff[l + 1] - t0;
         if (tc) {
@@ -2259,7 +2356,7 @@ int launch_up_lists(bfg_ctx *c, Work *w, uint32_t nl, PhaseTimes *pt) {
         if (pc) {
             hipLaunchKernelGGL(push_list_kernel<KL>, dim3((pc + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, pc,
                                w->pu.p + p0, w->pg.p + p0, c->pr_off.p, c->pr_wid.p, c->pw_off.p, c->pt_tgt.p,
-                               c->pr_x.p, c->pr_w.p, w->arena.p, n);
+                               c->pr_x.p, c->pr_w.p, push16 ? c->pr_w16.p : nullptr, w->arena.p, n);
             ++nlaunch;
         }
     }
@@ -4121,6 +4218,13 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
     HIP_TRY(c->pr_off.upload(L.pr_off, s));
     HIP_TRY(c->pr_x.upload(L.pr_x, s));
     HIP_TRY(c->pr_w.upload(L.pr_w, s));
+    {
+        std::vector<uint32_t> w16(L.pr_w.size() / 2);
+        auto h16 = [](uint32_t w) { return w == INF ? 0xFFFFu : std::min(w, PUSH_CAP); };
+        for (size_t q = 0; q < w16.size(); ++q) w16[q] = h16(L.pr_w[2 * q]) | h16(L.pr_w[2 * q + 1]) << 16;
+        HIP_TRY(c->pr_w16.upload(w16, s));
+        HIP_TRY(hipStreamSynchronize(s));  // before w16 goes
+    }
     HIP_TRY(c->pk_seg.upload(L.pk_seg, s));
     HIP_TRY(c->xr_min.upload(gt.xr_min, s));
     HIP_TRY(c->xr_w16.upload(gt.xr_w16, s));
@@ -4135,7 +4239,7 @@ int bfg_ctx_create(const bfg_graph_desc *gd, int device, bfg_ctx **out) {
                     &c->in_split, &c->in_src, &c->in_w, &c->ch_off, &c->ch_pos, &c->cs_off, &c->cs_src, &c->cs_aoff,
                     &c->ca_w, &c->up_off, &c->up_tgt, &c->up_w, &c->xr_off, &c->xr_run, &c->xw_off, &c->pr_wid,
                     &c->pw_off, &c->xr_src, &c->xr_w, &c->tb_dn, &c->tb_up, &c->tb_w, &c->pt_blk, &c->pt_tgt,
-                    &c->pr_off, &c->pr_x, &c->pr_w, &c->pk_seg, &c->xr_min, &c->cs_min, &c->pos_bt, &c->bt_ch, &c->xr_w16, &c->xr_cap,
+                    &c->pr_off, &c->pr_x, &c->pr_w, &c->pr_w16, &c->pk_seg, &c->xr_min, &c->cs_min, &c->pos_bt, &c->bt_ch, &c->xr_w16, &c->xr_cap,
                     &c->pos_of_rank_d})
         c->graph_bytes += b->n * sizeof(uint32_t);
     c->graph_bytes += (c->ca_v.n + c->in_dd.n) * sizeof(uint16_t);
