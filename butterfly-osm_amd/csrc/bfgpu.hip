@@ -1789,6 +1789,7 @@ struct Work {
     // [wave group][segment] (NONE: not reached)
     DevBuf<uint16_t> lab16;
     DevBuf<uint32_t> slot, redo;
+    bool lab16_ok = false;  // the last upward-only batch wrote lab16 and slot
     std::vector<uint32_t> h_pairs;
     // host staging of a call (kept, so a fixed Work reuses its capacity)
     std::vector<uint32_t> h_bt, h_pending, h_order, h_soff, h_srank, h_scost, h_lim, h_rows, h_sdepth, h_spre,
@@ -2324,12 +2325,19 @@ int launch_up_lists(bfg_ctx *c, Work *w, uint32_t nl, PhaseTimes *pt) {
     const uint32_t total = (uint32_t)w->h_tu.size();
     uint16_t *l16 = nullptr;
     uint32_t *slot = nullptr;
+    w->lab16_ok = false;
     if constexpr (KL == 64) {
-        ENSURE(w, lab16, std::max<size_t>((size_t)total * 4096, 1), "16-bit labels");
-        ENSURE(w, slot, std::max<size_t>(nmin / 8, 1), "label slots");
-        HIP_TRY(bfg_meet::launch_fill(s, w->slot.p, nmin / 8, NONE));
-        l16 = w->lab16.p;
-        slot = w->slot.p;
+        // the 16-bit copy serves the join alone: without memory for it the
+        // join runs on the u32 labels
+        if (w->lab16.ensure(std::max<size_t>((size_t)total * 4096, 1)) == hipSuccess &&
+            w->slot.ensure(std::max<size_t>(nmin / 8, 1)) == hipSuccess) {
+            HIP_TRY(bfg_meet::launch_fill(s, w->slot.p, nmin / 8, NONE));
+            l16 = w->lab16.p;
+            slot = w->slot.p;
+            w->lab16_ok = true;
+        } else {
+            (void)hipGetLastError();
+        }
     }
     HIP_TRY(hipEventRecord(w->ev[0], s));
     if (total) {
@@ -3975,7 +3983,8 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
         bfg_meet::Side B{wb->arena.p, L.n_top, r->lanes, tnl, wb->reach.p, wb->row_map.p, wb->segmin.p};
         // one wavefront a tile on both sides' 16-bit labels, tiles with a cell
         // past 16 bits again on the u32 labels
-        const bool use16 = splits == 1 && c->lanes == 64 && r->lanes == 64 && !(u32_only && u32_only[0] == '1');
+        const bool use16 = splits == 1 && c->lanes == 64 && r->lanes == 64 && wf->lab16_ok && wb->lab16_ok &&
+                           !(u32_only && u32_only[0] == '1');
         if (use16) {
             A.lab16 = wf->lab16.p;
             A.slot = wf->slot.p;
