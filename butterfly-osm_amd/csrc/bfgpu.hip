@@ -1007,6 +1007,29 @@ __global__ __launch_bounds__(TILE_THREADS) void push_list_kernel(
     else push_unit<KL, 64>(ra, rb, key, wr, out, lab);
 }
 
+// The UP pushes of the PHAST upward phase (rows_kernel<KL, true, false>) on
+// 16-bit column pairs: workgroup (blockIdx.x, blockIdx.y) takes push block
+// u0 + blockIdx.y for the wave groups 4 blockIdx.x + w that reach its
+// segment; wts16 as for push_list_kernel.
+template <int KL>
+__global__ __launch_bounds__(TILE_THREADS) void rows_push16_kernel(
+    uint32_t u0, uint32_t u1, const uint32_t *__restrict__ seg, const uint32_t *__restrict__ r0,
+    const uint32_t *__restrict__ wid, const uint32_t *__restrict__ woff, const uint32_t *__restrict__ seg_pos,
+    const uint32_t *__restrict__ tgt, const uint32_t *__restrict__ key, const uint32_t *__restrict__ wts,
+    const uint32_t *__restrict__ wts16, Reach R, uint32_t *__restrict__ arena, uint32_t n, uint32_t wgs) {
+    const uint32_t u = u0 + blockIdx.y, wg = blockIdx.x * (TILE_THREADS / 64) + wave_id();
+    if (u >= u1 || wg >= wgs) return;  // wave-uniform; this kernel has no barrier
+    const uint32_t s = seg[u] & UNIT_SEG_MASK;
+    if (!wave_reaches(R, wg, seg_pos[s])) return;  // the segment's labels are all INF: nothing to push
+    uint32_t *lab = lane_labels<KL>(arena, n, wg, nullptr);
+    const uint32_t ra = r0[u], rb = r0[u + 1], W = wid[u];
+    const cu32 *wr = as_const(wts) + woff[u], *wr16 = as_const(wts16) + woff[u] / 2;
+    const cu32 *out = as_const(tgt) + (size_t)u * 64;
+    if (W == 16) push_unit16<KL, 16>(ra, rb, key, wr16, wr, out, lab);
+    else if (W == 32) push_unit16<KL, 32>(ra, rb, key, wr16, wr, out, lab);
+    else push_unit16<KL, 64>(ra, rb, key, wr16, wr, out, lab);
+}
+
 // bottom_gate: per (bottom tree, lane group), whether the tree can hold a
 // label within a lane's limit: some bottom chunk holding part of the tree
 // was left open by the chunk gate (chunk_ok).  Thread (blockIdx.x * NT +
@@ -2408,6 +2431,8 @@ int launch_phases(bfg_ctx *c, Work *w, const Plan &P, uint32_t nl, PhaseTimes *p
                        D1, w->arena.p, n);
     ++nlaunch;
     uint32_t ndom = 0;  // DOWN row launches of this batch
+    const char *p32 = std::getenv("BFG_PUSH_U32");  // override: the u32 pushes alone
+    const bool push16 = c->pr_w16.n > 0 && !(p32 && p32[0] == '1');
     for (int l = (int)nsl - 1; l >= 0; --l) {
         const uint32_t s0 = L.seglvl_seg[l], s1 = L.seglvl_seg[l + 1];
         for (uint32_t b = s0; b < s1; b += MAX_GRID_Y, ++nlaunch)
@@ -2416,10 +2441,16 @@ int launch_phases(bfg_ctx *c, Work *w, const Plan &P, uint32_t nl, PhaseTimes *p
                                c->tb_w.p, P.split.p, R, G0, w->arena.p, n, wgs);
         const uint32_t k0 = L.pt_blk[s0], k1 = L.pt_blk[s1];
         for (uint32_t b = k0; b < k1; b += MAX_GRID_Y, ++nlaunch)
-            hipLaunchKernelGGL((rows_kernel<KL, true, false>), dim3(wgx, std::min(k1 - b, MAX_GRID_Y)), dim3(TILE_THREADS),
-                               0, s, b, std::min(k1, b + MAX_GRID_Y), c->pk_seg.p, c->pr_off.p, c->pr_off.p,
-                               c->pr_wid.p, c->pw_off.p, c->seg_pos.p, c->pt_tgt.p, c->pr_x.p, c->pr_w.p,
-                               R, G0, w->arena.p, n, wgs);
+            if (push16)
+                hipLaunchKernelGGL(rows_push16_kernel<KL>, dim3(wgx, std::min(k1 - b, MAX_GRID_Y)),
+                                   dim3(TILE_THREADS), 0, s, b, std::min(k1, b + MAX_GRID_Y), c->pk_seg.p,
+                                   c->pr_off.p, c->pr_wid.p, c->pw_off.p, c->seg_pos.p, c->pt_tgt.p, c->pr_x.p,
+                                   c->pr_w.p, c->pr_w16.p, R, w->arena.p, n, wgs);
+            else
+                hipLaunchKernelGGL((rows_kernel<KL, true, false>), dim3(wgx, std::min(k1 - b, MAX_GRID_Y)),
+                                   dim3(TILE_THREADS), 0, s, b, std::min(k1, b + MAX_GRID_Y), c->pk_seg.p,
+                                   c->pr_off.p, c->pr_off.p, c->pr_wid.p, c->pw_off.p, c->seg_pos.p, c->pt_tgt.p,
+                                   c->pr_x.p, c->pr_w.p, R, G0, w->arena.p, n, wgs);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(w->ev[1], s));
