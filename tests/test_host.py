@@ -109,3 +109,38 @@ def test_transpose_is_an_involution_and_keeps_rank_alignment(grid_small):
         k2 = np.lexsort((w2, tgt2, t))
         assert np.array_equal(tgt[k1], tgt2[k2]) and np.array_equal(w[k1], w2[k2])
 
+
+
+def _enc16(x, cap=0x7FFF):
+    """The 16-bit label and weight encoding of the bucket's pushes and join:
+    0xFFFF for INF, else min(x, cap)."""
+    x = np.asarray(x, dtype=np.uint64)
+    return np.where(x == 0xFFFFFFFF, 0xFFFF, np.minimum(x, cap)).astype(np.uint64)
+
+
+def test_16bit_cap_encoding_min_plus():
+    """The property the 16-bit pushes and join rely on, on random min-plus
+    products (labels with INF, values on both sides of the cap): with
+    saturating 16-bit sums of encoded operands, a result of 0xFFFF is INF
+    exactly when the u32 result is, a result below the cap equals the u32
+    result, and a result in [cap, 0xFFFE] stands for a finite u32 result of
+    at least the cap (the case the kernels recompute in u32).  The u32
+    reference is the oracle's arithmetic: saturating adds, INF = 2^32 - 1."""
+    rng = np.random.default_rng(16)
+    INF32 = np.uint64(0xFFFFFFFF)
+    for hi in (0x3000, 0x7FFF, 0x9000, 0x30000):
+        a = rng.integers(0, hi, (400, 24)).astype(np.uint64)
+        b = rng.integers(0, hi, (400, 24)).astype(np.uint64)
+        a[rng.random(a.shape) < 0.2] = INF32
+        b[rng.random(b.shape) < 0.2] = INF32
+        a[:5] = INF32  # rows with every sum INF
+        s32 = np.where((a == INF32) | (b == INF32), INF32, np.minimum(a + b, INF32))
+        r32 = s32.min(axis=1)
+        r16 = np.minimum(_enc16(a) + _enc16(b), 0xFFFF).min(axis=1)
+        assert np.array_equal(r16 == 0xFFFF, r32 == INF32)
+        low = r16 < 0x7FFF
+        assert np.array_equal(r16[low], r32[low])
+        mid = (r16 >= 0x7FFF) & (r16 < 0xFFFF)
+        assert np.all((r32[mid] >= 0x7FFF) & (r32[mid] != INF32))
+        # a u32 result below the cap is never reported in the recompute range
+        assert np.all(r16[r32 < 0x7FFF] == r32[r32 < 0x7FFF])
