@@ -861,7 +861,8 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_list_kernel(uint32_t count, 
                                                                 uint32_t *__restrict__ arena, uint32_t n,
                                                                 uint32_t *__restrict__ segmin, uint32_t nseg,
                                                                 uint16_t *__restrict__ lab16,
-                                                                uint32_t *__restrict__ slot, uint32_t item0) {
+                                                                uint32_t *__restrict__ slot,
+                                                                uint16_t *__restrict__ lmin16, uint32_t item0) {
     // a chain's latency is the level's critical path while the other side's
     // pushes fill the SIMDs: its wavefronts issue first
     __builtin_amdgcn_s_setprio(3);
@@ -884,6 +885,9 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_list_kernel(uint32_t count, 
 #pragma unroll
     for (int j = 0; j < 64; ++j)
         if (j < len) m = min(m, acc[j]);
+    if constexpr (KL == 64) {  // the lane's least label as the join's 16-bit bound
+        if (lab16) lmin16[(size_t)(item0 + i) * 64 + __lane_id()] = (uint16_t)(m != INF ? min(m, bfg_meet::CAP16) : 0xFFFFu);
+    }
 #pragma unroll
     for (int o = 4; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o, 64));
     const uint32_t lane = __lane_id();
@@ -1810,7 +1814,7 @@ struct Work {
     // (segment, wave group) pair (0xFFFF for INF, else min(label, CAP16)),
     // [list item][64 positions][64 lanes], and the item of each pair,
     // [wave group][segment] (NONE: not reached)
-    DevBuf<uint16_t> lab16;
+    DevBuf<uint16_t> lab16, lmin16;  // lmin16: [list item][64 lanes] the least 16-bit label of the segment
     DevBuf<uint32_t> slot, redo;
     bool lab16_ok = false;  // the last upward-only batch wrote lab16 and slot
     std::vector<uint32_t> h_pairs;
@@ -2353,7 +2357,8 @@ int launch_up_lists(bfg_ctx *c, Work *w, uint32_t nl, PhaseTimes *pt) {
         // the 16-bit copy serves the join alone: without memory for it the
         // join runs on the u32 labels
         if (w->lab16.ensure(std::max<size_t>((size_t)total * 4096, 1)) == hipSuccess &&
-            w->slot.ensure(std::max<size_t>(nmin / 8, 1)) == hipSuccess) {
+            w->slot.ensure(std::max<size_t>(nmin / 8, 1)) == hipSuccess &&
+            w->lmin16.ensure(std::max<size_t>((size_t)total * 64, 1)) == hipSuccess) {
             HIP_TRY(bfg_meet::launch_fill(s, w->slot.p, nmin / 8, NONE));
             l16 = w->lab16.p;
             slot = w->slot.p;
@@ -2380,7 +2385,7 @@ int launch_up_lists(bfg_ctx *c, Work *w, uint32_t nl, PhaseTimes *pt) {
         if (tc) {
             hipLaunchKernelGGL(tri_list_kernel<KL>, dim3((tc + WPB - 1) / WPB), dim3(TILE_THREADS), 0, s, tc,
                                w->tu.p + t0, w->tg.p + t0, c->seg_pos.p, c->tb_up.p, c->tb_w.p, w->arena.p, n,
-                               w->segmin.p, nseg, l16, slot, t0);
+                               w->segmin.p, nseg, l16, slot, w->lmin16.p, t0);
             ++nlaunch;
         }
         const uint32_t p0 = w->h_poff[l], pc = w->h_poff[l + 1] - p0;
@@ -4019,8 +4024,10 @@ int bucket_matrix(bfg_ctx *c, const uint32_t *src, uint32_t S, const uint32_t *t
         if (use16) {
             A.lab16 = wf->lab16.p;
             A.slot = wf->slot.p;
+            A.lmin16 = wf->lmin16.p;
             B.lab16 = wb->lab16.p;
             B.slot = wb->slot.p;
+            B.lmin16 = wb->lmin16.p;
             const size_t tiles = (size_t)((snl + 63) / 64) * ((tnl + 63) / 64);
             ENSURE(wf, redo, std::max<size_t>(tiles, 1), "join tiles");
             HIP_TRY(bfg_meet::launch_fill(wf->stream, wf->redo.p, tiles, 0));

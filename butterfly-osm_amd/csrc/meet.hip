@@ -238,6 +238,27 @@ __global__ __launch_bounds__(64) void meet_join16_kernel(JoinArgs a) {
             }
             const uint32_t qa = qa_of[s], qb = qb_of[s];
             if (qa == NONE || qb == NONE) continue;  // not reached by one of the wave groups: INF
+            if (a.A.lmin16 && a.B.lmin16) {
+                // per cell: the sum of its two lanes' least labels in the
+                // segment (16-bit, saturating) against the cell; the segment
+                // can lower no cell where no sum is below its cell
+                const uint4 ma = ld16(a.A.lmin16 + (size_t)qa * 64 + 8 * i, 0);
+                const uint4 mb = ld16(a.B.lmin16 + (size_t)qb * 64 + 8 * j, 0);
+                const uint32_t MA[4] = {ma.x, ma.y, ma.z, ma.w}, MB[4] = {mb.x, mb.y, mb.z, mb.w};
+                bool need = false;
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const u16x2 p = as_u16x2(MA[r / 2]);
+                    const u16x2 pr = (r & 1) ? p.yy : p.xx;
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        const u16x2 lo = __builtin_elementwise_min(__builtin_elementwise_add_sat(pr, as_u16x2(MB[h])),
+                                                                   acc[4 * r + h]);
+                        need |= __builtin_bit_cast(uint32_t, lo) != __builtin_bit_cast(uint32_t, acc[4 * r + h]);
+                    }
+                }
+                if (__ballot(need) == 0) continue;
+            }
             const uint32_t len = a.seg_pos[s + 1] - a.seg_pos[s];
             seen += len;
             const uint16_t *pa = a.A.lab16 + (size_t)qa * 4096 + 8 * i;

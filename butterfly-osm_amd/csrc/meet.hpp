@@ -39,6 +39,9 @@ struct Side {
     // lanes a group
     const uint16_t *lab16 = nullptr;
     const uint32_t *slot = nullptr;
+    // per pair slot and lane: the least 16-bit label of the segment,
+    // lmin16[64 slot + lane], or null (blocks pruned by 8-lane minima alone)
+    const uint16_t *lmin16 = nullptr;
 };
 
 // reach[g * words + s / 64] bit s % 64: wave group g (lanes 64g ..) has a
@@ -72,7 +75,9 @@ hipError_t launch_join(hipStream_t st, const Side &A, const Side &B, const uint3
 // [CAP16, 0xFFFE] while thr admits such a value is flagged in redo[t] (zero
 // beforehand) and counted in visits[1], and launch_join with only = redo
 // recomputes it from the u32 labels.  visits[0] gains the positions visited,
-// as launch_join.
+// as launch_join.  With per-lane minima on both sides, a segment that passes
+// the segment test is skipped still when no cell exceeds the sum of its two
+// lanes' minima.
 constexpr uint32_t CAP16 = 0x7FFF;
 hipError_t launch_join16(hipStream_t st, const Side &A, const Side &B, const uint32_t *seg_pos, uint32_t words,
                          uint32_t nseg, uint32_t thr, uint32_t *out, uint64_t ld, unsigned long long *visits,
