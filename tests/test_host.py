@@ -144,3 +144,35 @@ def test_16bit_cap_encoding_min_plus():
         assert np.all((r32[mid] >= 0x7FFF) & (r32[mid] != INF32))
         # a u32 result below the cap is never reported in the recompute range
         assert np.all(r16[r32 < 0x7FFF] == r32[r32 < 0x7FFF])
+
+
+def test_16bit_per_cell_skip_rule():
+    """The 16-bit join's per-cell bound: a segment is skipped when, for every
+    cell of a block, the saturating 16-bit sum of its two lanes' encoded
+    least labels in the segment is no less than the cell.  Whenever that
+    holds, relaxing the segment (the 16-bit min-plus over its positions)
+    changes no cell below the cap and leaves an INF cell INF; cells at or
+    past the cap are recomputed in u32 by the kernels whatever happens."""
+    rng = np.random.default_rng(61)
+    INF32 = 0xFFFFFFFF
+    skipped = 0
+    for trial in range(3000):
+        hi = (0x2000, 0x8000, 0x20000)[trial % 3]
+        A = rng.integers(0, hi, (8, 64)).astype(np.uint64)
+        B = rng.integers(0, hi, (8, 64)).astype(np.uint64)
+        A[rng.random(A.shape) < 0.1] = INF32
+        B[rng.random(B.shape) < 0.1] = INF32
+        if trial % 7 == 0:
+            A[rng.integers(0, 8)] = INF32  # a lane that does not reach the segment
+        cand = np.minimum(_enc16(A.min(axis=1))[:, None] + _enc16(B.min(axis=1))[None, :], 0xFFFF)
+        # cells so far at or a little below the bound (and above it now and then)
+        cells = cand.astype(np.int64) - rng.integers(-3, 60, cand.shape)
+        cells = np.clip(cells, 0, 0xFFFF).astype(np.uint64)
+        if not np.all(cand >= cells):
+            continue
+        skipped += 1
+        sums = np.minimum(_enc16(A)[:, None, :] + _enc16(B)[None, :, :], 0xFFFF).min(axis=2)
+        new = np.minimum(cells, sums)
+        kept = (cells < 0x7FFF) | (cells == 0xFFFF)
+        assert np.array_equal(new[kept], cells[kept])
+    assert skipped > 100
