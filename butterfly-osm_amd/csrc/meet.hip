@@ -13,8 +13,11 @@
 // reached segment but not under a given position are INF there, so every
 // position of a common segment can enter the product.
 //
-// Arithmetic is u32 with saturating adds (v_add_u32 clamp) and v_min3, so the
-// cells equal the reference's bucket join and its PHAST matrix bit for bit.
+// meet_join_kernel computes in u32 with saturating adds (v_add_u32 clamp) and
+// v_min3; meet_join16_kernel in packed u16 on capped label copies, with the
+// tiles it cannot settle in 16 bits recomputed by meet_join_kernel.  Either
+// way the cells equal the reference's bucket join and its PHAST matrix bit
+// for bit.
 #include "meet.hpp"
 
 #include <algorithm>
